@@ -1,0 +1,173 @@
+/*
+ * ga_oracle.h -- CPU restatement of infraGA's stratified ray-tracing path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker the HIP engine is
+ * compared against: only tests/, __graft_entry__.smoke() and the cpu_baseline
+ * leg of bench.py may load it.  The product path (libinfraga_gpu.so and the
+ * infraga-* front ends) never links or calls it.
+ *
+ * Every function restates the algorithm of the reference file:line named in
+ * its comment in ga_oracle.c.  The arithmetic is plain IEEE binary64 in the
+ * reference's operation order (compile without -ffast-math and without FMA
+ * contraction so it reproduces the reference's rounding).
+ *
+ * Coverage (geometry x feature):
+ *   - 3D Cartesian stratified (src/geoac/geoac.eqset.3d.strat.cpp,
+ *     src/atmo/atmo_state.3d.strat.cpp, src/atmo/atmo_state.abs.cart.cpp)
+ *   - spherical stratified    (src/geoac/geoac.eqset.sph.strat.cpp,
+ *     src/atmo/atmo_state.sph.strat.cpp, src/atmo/atmo_state.abs.sph.cpp)
+ *   - flat ground at z0 (no terrain file), boundary-layer vertical wind off
+ *     (topo_use_BLw=false, the reference default).
+ *   - 18 equations (calc_amp=true) and 6 equations (calc_amp=false).
+ *   - -prop fans (write_rays=false bookkeeping) and -eig_search.
+ */
+#ifndef GA_ORACLE_H_
+#define GA_ORACLE_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GA_GEOM_3D  0
+#define GA_GEOM_SPH 1
+
+/* Natural cubic spline on a shared abscissa (interpolation.h:36-42). */
+typedef struct {
+    int n;
+    const double *x;
+    double *f;
+    double *slope;
+    int accel;          /* segment cursor (interpolation.h:38) */
+} go_spline;
+
+/* Stratified medium: c, u, v, rho splines in z plus ground/boundary layer. */
+typedef struct {
+    int geom;
+    int n;
+    double *z;          /* km, shared by all four splines */
+    go_spline c, u, v, rho;   /* km/s, km/s, km/s, g/cm^3 */
+    double r0;          /* globe radius (sph; globe.cpp:10 uses 6370.0) */
+    double z0;          /* ground elevation [km] */
+    double z_max;       /* topography maximum (= z0 without terrain) */
+    double z_bndlyr;    /* boundary-layer top = z_max + 2 */
+    double bndlyr_param;/* 0.1 km (atmo_state.*.cpp) */
+    double tweak_abs;   /* Sutherland-Bass scale factor (drivers set 1.0) */
+} go_atmo;
+
+/* Solver limits (geoac.params.{sph,3d}.cpp) and configuration. */
+typedef struct {
+    int calc_amp;       /* 1 -> 18 equations, 0 -> 6 */
+    int eq_cnt;
+    double s_max, ds_min, ds_max;
+    double alt_max, rng_max, time_max;
+    double x_min, x_max, y_min, y_max;
+    double lat_min, lat_max, lon_min, lon_max;
+} go_params;
+
+/* Per-ray launch context (the reference's refs.src_loc/c0/nu0 + theta/phi). */
+typedef struct {
+    double src[3];      /* sph: r (km from centre), lat, lon [rad]; 3D: x, y, z */
+    double c0;
+    double nu0;
+    double theta, phi;  /* radians, phi = pi/2 - azimuth */
+} go_src;
+
+/* One arrival row of <id>.arrivals.dat (main.infraga.sph.cpp:531-544). */
+typedef struct {
+    int launch_idx;
+    int n_bounce;
+    double incl_launch, az_launch;   /* degrees, as launched */
+    double pos0, pos1;               /* sph: lat, lon [deg]; 3D: x, y [km] */
+    double time, cel, turn_ht, incl_arr, back_az, tl, absorption;
+} go_arrival;
+
+typedef struct {
+    int bounces;
+    double freq;
+    double turn_ht_min;
+    double src0, src1, src_alt;      /* sph: lat, lon [deg], alt [km]; 3D: x, y, z [km] */
+} go_prop;
+
+/* ---- atmosphere set-up ------------------------------------------------ */
+int    go_atmo_init(go_atmo *a, int geom, const double *z, const double *c,
+                    const double *u, const double *v, const double *rho, int n);
+void   go_atmo_free(go_atmo *a);
+void   go_params_default(go_params *p, int geom, const go_atmo *a, int calc_amp);
+
+/* ---- splines (interpolation.cpp) -------------------------------------- */
+void   go_spline_set(go_spline *s);
+int    go_find_segment(double x, const double *xv, int n, int *prev);
+double go_spline_f(go_spline *s, double x);
+double go_spline_df(go_spline *s, double x);
+double go_spline_ddf(go_spline *s, double x);
+void   go_spline_all(go_spline *s, double x, double *f, double *df, double *ddf);
+
+/* ---- medium (atmo_state.*.cpp) ---------------------------------------- */
+double go_c(go_atmo *a, double p0, double p1, double p2);
+double go_rho(go_atmo *a, double p0, double p1, double p2);
+double go_u(go_atmo *a, double p0, double p1, double p2);
+double go_v(go_atmo *a, double p0, double p1, double p2);
+double go_dc(go_atmo *a, double p0, double p1, double p2, int n);
+double go_du(go_atmo *a, double p0, double p1, double p2, int n);
+double go_dv(go_atmo *a, double p0, double p1, double p2, int n);
+/* calc_uvw with first (and optionally second) derivatives; dd may be NULL. */
+void   go_calc_uvw(go_atmo *a, double p0, double p1, double p2,
+                   double *u, double *v, double *w, double du[3], double dv[3], double dw[3],
+                   double ddu[6], double ddv[6], double ddw[6]);
+double go_sb_alpha(go_atmo *a, double p0, double p1, double p2, double freq);
+double go_gc_dist(double r0, double lat1, double lon1, double lat2, double lon2);
+
+/* ---- ray equations (geoac.eqset.*.cpp) -------------------------------- */
+void   go_set_initial(go_atmo *a, const go_params *p, go_src *s, double *y0);
+void   go_rhs(go_atmo *a, const go_params *p, const go_src *s, const double *y, double *dyds);
+double go_set_ds(go_atmo *a, const go_params *p, const double *y);
+int    go_break_check(go_atmo *a, const go_params *p, const go_src *s, double *y);
+int    go_ground_check(go_atmo *a, const double *y);
+void   go_set_refl(go_atmo *a, const go_params *p, const go_src *s,
+                   const double *row_km2, const double *row_km1, double *y0);
+double go_travel_time_seg(go_atmo *a, const double *y0, const double *y1);
+double go_atten_seg(go_atmo *a, const double *y0, const double *y1, double freq);
+double go_jacobian(go_atmo *a, const double *y);
+double go_amp(go_atmo *a, const go_src *s, const double *y);
+double go_eikonal_resid(go_atmo *a, const go_src *s, const double *y);
+
+/* One RK4 step y -> y1 (rk4solver.cpp:29-55); returns ds. */
+double go_rk4_step(go_atmo *a, const go_params *p, const go_src *s,
+                   const double *y, double *y1);
+
+/* Integrate one leg (rk4solver.cpp:14-85) into rows[0..]; rows[0] preset.
+ * Returns k (index of the last row written); *check = leave-region flag. */
+int    go_prop_rk4(go_atmo *a, const go_params *p, const go_src *s,
+                   double *rows, int length, int *check);
+
+/* -prop fan with write_rays=false bookkeeping (main.infraga.sph.cpp:418-558,
+ * main.infraga.3d.cpp:407-548).  launches are (incl_deg, az_deg) pairs.
+ * Returns 0, or -1 when out_cap was too small (n_out still counts all). */
+int    go_fan(go_atmo *a, const go_params *p, const go_prop *pr,
+              const double *launches, int n_launch,
+              go_arrival *out, int out_cap, int *n_out, long long *ray_steps);
+
+/* Eigenray search for one receiver, 3D geometry (main.infraga.3d.cpp:944-951,
+ * geoac.eigenray.3d.cpp).  Returns the number of eigenrays written to out. */
+int    go_eig_search_3d(go_atmo *a, const go_params *p, const double src[3],
+                        const double rcvr[2], double th_min_deg, double th_max_deg,
+                        int bnc_min, int bnc_max, int iterations, double damping,
+                        double tolerance, double az_err_lim, double dth_big_deg,
+                        double dth_sml_deg, double freq,
+                        go_arrival *out, int out_cap, long long *ray_steps);
+
+/* ---- opaque-handle helpers for test bindings ------------------------- */
+void  *go_h_atmo(int geom, const double *z, const double *c, const double *u,
+                 const double *v, const double *rho, int n);
+void   go_h_free(void *h);
+void   go_h_set_ground(void *h, double z0);
+void   go_h_set_tweak(void *h, double t);
+void   go_h_set_r0(void *h, double r0);
+void   go_h_params(void *h, int calc_amp, go_params *p);
+int    go_h_fan(void *h, const go_params *p, const go_prop *pr, const double *launches, int n,
+                go_arrival *out, int cap, int *n_out, long long *steps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
