@@ -1,0 +1,159 @@
+/*
+ * infraga_gpu.h -- C ABI of the MI355X ray-tracing engine (libinfraga_gpu.so).
+ *
+ * Plain C types only: no HIP or torch types appear in any signature.  All
+ * entry points return GA_OK (0) or a negative GA_E_* code and never print
+ * or prompt; ga_last_error() gives the message of the last failure on a
+ * context.  One context may be used by one host thread at a time.
+ *
+ * Units at this boundary follow the reference CLI: km, km/s, g/cm^3, and
+ * angles in degrees.  Atmosphere arrays are the values the reference loader
+ * holds after conversion (c, u, v already divided by 1000).
+ *
+ * Reference interfaces replaced (paths relative to the infraGA source tree):
+ *   ga_atmo_load_strat  <- set_region(char*, char*, bool)
+ *                          src/atmo/atmo_io.sph.strat.cpp:57-144,
+ *                          src/atmo/atmo_io.3d.strat.cpp:57-140
+ *   ga_fan_run          <- the -prop azimuth x inclination x bounce loop
+ *                          src/main.infraga.sph.cpp:418-558 and
+ *                          src/main.infraga.3d.cpp:407-548, which drive
+ *                          geoac::set_initial / prop_rk4 / travel_time /
+ *                          atten / amp / set_refl
+ *                          (src/geoac/geoac.eqset.h:5-38, src/util/rk4solver.h:6)
+ *   ga_eig_search       <- run_eig_search's per-bounce loop over
+ *                          geoac::est_eigenray + geoac::find_eigenray
+ *                          src/main.infraga.3d.cpp:944-951,
+ *                          src/geoac/geoac.eigenray.h:17-20
+ */
+#ifndef INFRAGA_GPU_H_
+#define INFRAGA_GPU_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GA_OK          0
+#define GA_E_ARG      -1   /* invalid argument */
+#define GA_E_HIP      -2   /* HIP runtime failure */
+#define GA_E_NOMEM    -3   /* device or host allocation failed */
+#define GA_E_CAP      -4   /* output buffer too small; *n_out holds the size needed */
+#define GA_E_NODEV    -5   /* no usable gfx950 device */
+#define GA_E_ATMO     -6   /* unknown or released atmosphere id */
+#define GA_E_IO       -7   /* file could not be read or parsed */
+
+#define GA_GEOM_3D     0   /* Cartesian x, y, z (infraga-3d) */
+#define GA_GEOM_SPH    1   /* spherical r, lat, lon (infraga-sph) */
+
+typedef struct ga_ctx ga_ctx;
+
+/* Propagation parameters: the -prop / -eig_search key=value set of the
+ * reference (src/main.infraga.sph.cpp:208-309).  Fill with
+ * ga_prop_params_default() and override. */
+typedef struct {
+    int    geometry;       /* GA_GEOM_3D or GA_GEOM_SPH */
+    int    calc_amp;       /* 1: 18 equations incl. auxiliary (TL), 0: 6 */
+    int    bounces;        /* ground reflections to follow */
+    double freq;           /* Hz, Sutherland-Bass absorption frequency */
+    double abs_coeff;      /* absorption scale (tweak_abs), reference driver default 1.0 */
+    double turn_ht_min;    /* km, shadow-zone test height above ground */
+    double src[3];         /* sph: lat [deg], lon [deg], alt [km]; 3D: x, y, z [km] */
+    double z_grnd;         /* km; NaN = bottom of the profile */
+    double s_max, ds_min, ds_max;     /* km */
+    double alt_max;        /* km; NaN = top of the profile */
+    double rng_max;        /* km */
+    double time_max;       /* hours (sph only, as in the reference) */
+    double bounds[4];      /* sph: lat_min, lat_max, lon_min, lon_max [deg];
+                              3D: x_min, x_max, y_min, y_max [km] */
+    double r_earth;        /* km; the reference hard-codes 6370 */
+} ga_prop_params;
+
+/* One row of <id>.arrivals.dat (src/main.infraga.sph.cpp:531-544). */
+typedef struct {
+    int32_t launch_idx;    /* index into the launch list */
+    int32_t n_bounce;
+    double  incl, az;      /* launch angles [deg] as given */
+    double  pos0, pos1;    /* sph: lat, lon [deg]; 3D: x, y [km] */
+    double  time;          /* s */
+    double  cel;           /* km/s */
+    double  turn_ht;       /* km */
+    double  incl_arr;      /* deg */
+    double  back_az;       /* deg */
+    double  tl;            /* dB (0 when calc_amp = 0) */
+    double  absorption;    /* dB */
+} ga_arrival;
+
+typedef struct {
+    int64_t ray_steps;     /* RK4 iterations summed over rays and legs */
+    double  kernel_ms;     /* device time of the trace kernels (HIP events) */
+    double  wall_ms;       /* host wall time of the call */
+    int32_t n_rays;
+    int32_t n_launches;    /* kernel launches issued */
+} ga_run_stats;
+
+/* Eigenray search controls (src/main.infraga.3d.cpp:773-800 defaults). */
+typedef struct {
+    double incl_min, incl_max;     /* deg */
+    int    bnc_min, bnc_max;
+    int    iterations;             /* LM iteration limit (25) */
+    double damping;                /* 1e-3 */
+    double tolerance;              /* km (0.1) */
+    double az_dev_lim;             /* deg (2.0) */
+    double incl_step_min, incl_step_max;   /* deg (0.001, 0.1) */
+} ga_eig_params;
+
+/* One eigenray: the arrival row written to <id>.arrivals.dat by
+ * find_eigenray (src/geoac/geoac.eigenray.3d.cpp:302-315); launch_idx is the
+ * receiver index. */
+typedef ga_arrival ga_eigenray;
+
+void        ga_prop_params_default(ga_prop_params *p, int geometry);
+void        ga_eig_params_default(ga_eig_params *e);
+
+int         ga_ctx_create(ga_ctx **out, int hip_device);
+void        ga_ctx_destroy(ga_ctx *ctx);
+const char *ga_last_error(const ga_ctx *ctx);
+
+/* Upload a stratified profile (n >= 3 strictly increasing z).  The natural
+ * cubic spline slopes are solved on the host exactly as interp::set does
+ * (src/util/interpolation.cpp:103-136). */
+int ga_atmo_load_strat(ga_ctx *ctx, const double *z_km, const double *c_kms,
+                       const double *u_kms, const double *v_kms, const double *rho_gcc,
+                       int n, int *atmo_id);
+
+/* Parse a .met file (formats zTuvdp, zuvwTdp, zcuvd) with the reference
+ * loader's conversions and upload it. */
+int ga_atmo_load_file(ga_ctx *ctx, const char *path, const char *format, int reverse_winds,
+                      int *atmo_id);
+
+int ga_atmo_release(ga_ctx *ctx, int atmo_id);
+
+/* Trace a fan.  launches holds n_launch (incl_deg, az_deg) pairs.  Arrivals
+ * are returned sorted by (launch_idx, n_bounce).  When out_cap is too small
+ * the call returns GA_E_CAP with *n_out set to the count needed. */
+int ga_fan_run(ga_ctx *ctx, int atmo_id, const ga_prop_params *p,
+               const double *launches, int n_launch,
+               ga_arrival *out, int out_cap, int *n_out, ga_run_stats *stats);
+
+/* Device-resident form for callers that keep buffers in HBM: d_launches
+ * (2*n doubles), d_out (n * (bounces + 1) slots, slot r*(bounces+1)+b holds
+ * bounce b of ray r), d_counts (n ints, arrivals per ray) and d_steps (n
+ * ints, RK4 iterations per ray) are device pointers; stream is a hipStream_t
+ * or NULL for the context stream.  Enqueues only: no synchronisation. */
+int ga_fan_run_device(ga_ctx *ctx, int atmo_id, const ga_prop_params *p,
+                      const double *d_launches, int n_launch,
+                      ga_arrival *d_out, int32_t *d_counts, int32_t *d_steps, void *stream);
+
+/* Batched 3D eigenray search for n_rcvr receivers at (x, y) [km] pairs
+ * rcvr_xy, all from p->src.  Eigenrays are returned grouped by receiver
+ * (launch_idx) in the order the reference would find them. */
+int ga_eig_search(ga_ctx *ctx, int atmo_id, const ga_prop_params *p, const ga_eig_params *e,
+                  const double *rcvr_xy, int n_rcvr,
+                  ga_eigenray *out, int out_cap, int *n_out, double *rcvr_latency_ms,
+                  ga_run_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* INFRAGA_GPU_H_ */

@@ -1,0 +1,340 @@
+// ga_trace.hpp -- per-lane ray tracing loop of the fan engine: launch state,
+// region / ground checks, reflection, arrival evaluation and the step loop.
+// Included by ga_engine.hip (device) and usable from host-side builds.
+#pragma once
+
+#include "ga_device.hpp"
+#include "../../include/infraga_gpu.h"
+
+namespace ga {
+
+constexpr int WG = 256;   // threads per workgroup (4 waves, one per SIMD)
+
+// Raw arrival record written by the step loop at each ground return:
+// row K (18 values), tt, att, rmax, theta, phi, c0, nu0.  The arrival row
+// itself is evaluated by arrival_kernel, which keeps the Jacobian/amplitude
+// code out of the step loop's register budget.
+constexpr int RAW = 25;
+
+struct FanArgs {
+    Medium M;
+    Limits L;
+    double src0, src1, src2;   // sph: r, lat, lon [rad]; 3D: x, y, z [km]
+    double c_src;              // c at the source (arrival inclination)
+    double freq, turn_ht_min, rmax_init;
+    int bounces;
+    int n_rays;
+    int max_steps;             // prop_rk4's row budget per leg (s_max / ds_min) - 1
+    int lds;                   // 1: table staged in LDS
+    const double *launches;    // (incl_deg, az_deg) pairs
+    double *raw;               // RAW doubles per arrival slot (row K + leg sums)
+    double *ypbuf;             // per-lane copy of row k-1, [nlanes][18]
+    int nlanes;
+    ga_arrival *out;
+    int32_t *counts;
+    int32_t *steps;
+    unsigned int *queue;
+};
+
+// ---------------------------------------------------------------------------
+// Launch state (set_initial: geoac.eqset.sph.strat.cpp:59-104,
+// geoac.eqset.3d.strat.cpp:58-105).
+// ---------------------------------------------------------------------------
+template <int G, int NEQ>
+GA_HD void init_ray(const FanArgs &A, const Medium &M, double incl, double az, double *y,
+                                         double &c0, double &nu0s, double &th, double &ph, int &cur)
+{
+    ph = PI / 2.0 - az * (PI / 180.0);
+    th = incl * (PI / 180.0);
+    double src[3] = {A.src0, A.src1, A.src2};
+    double c, u, v, rho;
+    point_medium<G>(M, src, cur, c, u, v, rho);
+    c0 = c;
+    double st, ct, sp, cp;
+    sincos(th, &st, &ct);
+    sincos(ph, &sp, &cp);
+    double mach[3], n0[3], mt[3], mp[3];
+    if (G == GEOM_SPH) {
+        mach[0] = 0.0; mach[1] = v / c0; mach[2] = u / c0;
+        n0[0] = st; n0[1] = ct * sp; n0[2] = ct * cp;
+        mt[0] = ct; mt[1] = -st * sp; mt[2] = -st * cp;
+        mp[0] = 0.0; mp[1] = ct * cp; mp[2] = -ct * sp;
+    } else {
+        mach[0] = u / c0; mach[1] = v / c0; mach[2] = 0.0;
+        n0[0] = ct * cp; n0[1] = ct * sp; n0[2] = st;
+        mt[0] = -st * cp; mt[1] = -st * sp; mt[2] = ct;
+        mp[0] = -ct * sp; mp[1] = ct * cp; mp[2] = 0.0;
+    }
+    double msc = 1.0 + (n0[0] * mach[0] + n0[1] * mach[1] + n0[2] * mach[2]);
+    double imsc = 1.0 / msc;
+    nu0s = imsc;
+    y[0] = src[0]; y[1] = src[1]; y[2] = src[2];
+    for (int i = 0; i < 3; ++i) y[3 + i] = n0[i] * imsc;
+    if (NEQ == 18) {
+        double a = mt[0] * mach[0] + mt[1] * mach[1] + mt[2] * mach[2];
+        double b = mp[0] * mach[0] + mp[1] * mach[1] + mp[2] * mach[2];
+        for (int i = 0; i < 3; ++i) {
+            y[6 + i] = 0.0;
+            y[12 + i] = 0.0;
+            y[9 + i] = mt[i] * imsc - n0[i] * (imsc * imsc) * a;
+            y[15 + i] = mp[i] * imsc - n0[i] * (imsc * imsc) * b;
+        }
+    }
+}
+
+// break_check (sph: wraps longitude in place) / ground_check.
+template <int G>
+GA_HD bool leaves_region(const FanArgs &A, const Medium &M, const Limits &L, double *y,
+                                              double sin_half_dummy)
+{
+    (void)sin_half_dummy;
+    if (G == GEOM_SPH) {
+        if (y[2] > PI) y[2] -= 2.0 * PI;
+        else if (y[2] < -PI) y[2] += 2.0 * PI;
+        if (y[1] < -PI / 2.0 || y[1] > PI / 2.0) return true;
+        if (y[0] - M.r0 > L.alt_max) return true;
+        double s1 = sin((A.src1 - y[1]) * 0.5);
+        double s2 = sin((A.src2 - y[2]) * 0.5);
+        double h = s1 * s1 + cos(y[1]) * cos(A.src1) * (s2 * s2);
+        if (h > L.gc_thresh) return true;
+        if (y[1] < L.b0 || y[1] > L.b1) return true;
+        if (y[2] < L.b2 || y[2] > L.b3) return true;
+        return false;
+    } else {
+        double dx = y[0] - A.src0, dy = y[1] - A.src1;
+        if (sqrt(dx * dx + dy * dy) > L.rng_max) return true;
+        if (y[0] > L.b1 || y[0] < L.b0) return true;
+        if (y[1] > L.b3 || y[1] < L.b2) return true;
+        if (y[2] > L.alt_max) return true;
+        return false;
+    }
+}
+
+template <int G>
+GA_HD bool below_ground(const Medium &M, const double *y)
+{
+    double h = hcoord<G>(y);
+    return (zcoord<G>(M, y) <= M.z_max) && (h < M.zg);
+}
+
+// Jacobian determinant (geoac.eqset.sph.strat.cpp:800-823, 3d.strat.cpp:704-723).
+template <int G>
+GA_HD double jacobian(const double *y, double c, double u, double v)
+{
+    double nm = sqrt(y[3] * y[3] + y[4] * y[4] + y[5] * y[5]);
+    double p0, p1, p2;
+    if (G == GEOM_SPH) { p0 = c * y[3] / nm; p1 = c * y[4] / nm + v; p2 = c * y[5] / nm + u; }
+    else               { p0 = c * y[3] / nm + u; p1 = c * y[4] / nm + v; p2 = c * y[5] / nm; }
+    double pm = sqrt(p0 * p0 + p1 * p1 + p2 * p2);
+    if (G == GEOM_SPH) {
+        double r = y[0], ct = cos(y[1]);
+        double drs = p0 / pm, dts = 1.0 / r * p1 / pm, dps = 1.0 / (r * ct) * p2 / pm;
+        return r * r * ct * (drs * (y[7] * y[14] - y[13] * y[8]) - y[6] * (dts * y[14] - dps * y[13])
+                             + y[12] * (dts * y[8] - dps * y[7]));
+    }
+    double dxs = p0 / pm, dys = p1 / pm, dzs = p2 / pm;
+    return dxs * (y[7] * y[14] - y[13] * y[8]) - y[6] * (dys * y[14] - dzs * y[13]) + y[12] * (dys * y[8] - dzs * y[7]);
+}
+
+// Arrival record (main.infraga.sph.cpp:523-544, main.infraga.3d.cpp:511-530).
+template <int G, int NEQ>
+GA_HD void eval_arrival(const FanArgs &A, const Medium &M, const double *y, int ray, int bnc,
+                                             double incl, double az, double tt, double att, double rmax,
+                                             double c0, double nu0s, double th, double ph, int &cur)
+{
+    ga_arrival R;
+    R.launch_idx = ray;
+    R.n_bounce = bnc;
+    R.incl = incl;
+    R.az = az;
+    double c, u, v, rho;
+    point_medium<G>(M, y, cur, c, u, v, rho);
+    double baz;
+    if (G == GEOM_SPH) {
+        R.incl_arr = -asin(c / A.c_src * y[3]) * (180.0 / PI);
+        baz = 90.0 - atan2(-y[4], -y[5]) * (180.0 / PI);
+        R.pos0 = y[1] * (180.0 / PI);
+        R.pos1 = y[2] * (180.0 / PI);
+        double s1 = sin((A.src1 - y[1]) / 2.0), s2 = sin((A.src2 - y[2]) / 2.0);
+        double gc = 2.0 * M.r0 * asin(sqrt(s1 * s1 + cos(y[1]) * cos(A.src1) * (s2 * s2)));
+        R.cel = gc / tt;
+    } else {
+        double cg = herm_f(seg_point(M, clampz(M, M.z0), cur), 1);
+        R.incl_arr = -asin(cg / A.c_src * y[5]) * 180.0 / PI;
+        baz = 90.0 - atan2(-y[4], -y[3]) * 180.0 / PI;
+        R.pos0 = y[0];
+        R.pos1 = y[1];
+        double dx = y[0] - A.src0, dy = y[1] - A.src1;
+        R.cel = sqrt(dx * dx + dy * dy) / tt;
+    }
+    if (baz < -180.0) baz += 360.0;
+    else if (baz > 180.0) baz -= 360.0;
+    R.back_az = baz;
+    R.time = tt;
+    R.turn_ht = rmax;
+    R.absorption = -2.0 * att;
+    R.tl = 0.0;
+    if (NEQ == 18) {
+        double src[3] = {A.src0, A.src1, A.src2};
+        double cs, us, vs, rhos;
+        point_medium<G>(M, src, cur, cs, us, vs, rhos);
+        double jac = jacobian<G>(y, c, u, v);
+        double st, ct, sp, cp;
+        sincos(th, &st, &ct);
+        sincos(ph, &sp, &cp);
+        double num, den;
+        if (G == GEOM_SPH) {
+            double nm = (c0 - y[4] * v - y[5] * u) / c;
+            double q0 = c0 * (nu0s * st) / nu0s, q1 = c0 * (nu0s * ct * sp) / nu0s + vs, q2 = c0 * (nu0s * ct * cp) / nu0s + us;
+            double p0 = c * y[3] / nm, p1 = c * y[4] / nm + v, p2 = c * y[5] / nm + u;
+            double pm = sqrt(p0 * p0 + p1 * p1 + p2 * p2), qm = sqrt(q0 * q0 + q1 * q1 + q2 * q2);
+            num = rho * nm * (c * c * c) * qm * ct;
+            den = rhos * nu0s * (cs * cs * cs) * pm * jac;
+        } else {
+            double nm = sqrt(y[3] * y[3] + y[4] * y[4] + y[5] * y[5]);
+            double nm0 = 1.0 - (y[3] * us + y[4] * vs) / c0;
+            double p0 = c * y[3] / nm + u, p1 = c * y[4] / nm + v, p2 = c * y[5] / nm;
+            double q0 = c0 * ct * cp + us, q1 = c0 * ct * sp + vs, q2 = c0 * st;
+            double pm = sqrt(p0 * p0 + p1 * p1 + p2 * p2), qm = sqrt(q0 * q0 + q1 * q1 + q2 * q2);
+            num = rho * nm * (c * c * c) * qm * ct;
+            den = rhos * nm0 * (c0 * c0 * c0) * pm * jac;
+        }
+        R.tl = 20.0 * log10(sqrt(fabs(num / den)));
+    }
+    A.out[(size_t)ray * (A.bounces + 1) + bnc] = R;
+}
+
+// Specular reflection off flat ground (set_refl with zero terrain slope:
+// geoac.eqset.sph.strat.cpp:120-211, geoac.eqset.3d.strat.cpp:114-205).
+// yk2, yk1 are rows K-2 and K-1 of the leg; result overwrites y.
+template <int G, int NEQ>
+GA_HD void reflect(const Medium &M, const double *yk2, const double *yk1, double *y,
+                                        double c0, int &cur)
+{
+    const int iv = (G == GEOM_SPH) ? 0 : 2;
+    double dr_k = yk1[iv] - yk2[iv];
+    double dr_g = M.zg - yk1[iv];
+    double prev[NEQ];
+#pragma unroll
+    for (int i = 0; i < NEQ; ++i) prev[i] = yk1[i] + (yk1[i] - yk2[i]) / dr_k * dr_g;
+#pragma unroll
+    for (int i = 0; i < NEQ; ++i) y[i] = prev[i];
+    y[iv] = M.zg;
+    y[3 + iv] = -prev[3 + iv];
+    if (NEQ == 18) {
+        y[6 + iv] = -prev[6 + iv];
+        y[12 + iv] = -prev[12 + iv];
+        // medium at the ground: atmo::c, dc and the wind derivatives with the
+        // boundary-layer factor at zero height (X = 1: sc = 0, dsc = 5)
+        SegPt s = seg_point(M, clampz(M, M.z0), cur);
+        double cg, dcg, u0, du0, v0, dv0;
+        herm_fd(s, 1, cg, dcg);
+        herm_fd(s, 3, u0, du0);
+        herm_fd(s, 5, v0, dv0);
+        double X = exp(-(M.zg - M.zg) * 10.0);
+        double sc = (1.0 - X) / (1.0 + X);
+        double dsc = -2.0 * (-X * 10.0) / ((1.0 + X) * (1.0 + X));
+        double duz = u0 * dsc + du0 * sc, dvz = v0 * dsc + dv0 * sc;
+        double dnv;
+        if (G == GEOM_SPH) {
+            double rg = M.zg;
+            dnv = c0 / cg * dcg + prev[4] * dvz + prev[5] * duz;
+            dnv += cg * cg / c0 * 1.0 / rg * (prev[4] * prev[4] + prev[5] * prev[5]);
+            dnv *= -1.0 / cg;
+        } else {
+            dnv = -1.0 / cg * (c0 / cg * dcg + prev[3] * duz + prev[4] * dvz);
+        }
+        double ds0_t = -c0 / cg * prev[6 + iv] / prev[3 + iv];
+        double ds0_p = -c0 / cg * prev[12 + iv] / prev[3 + iv];
+        y[9 + iv] = -prev[9 + iv] + (-2.0 * dnv) * ds0_t;
+        y[15 + iv] = -prev[15 + iv] + (-2.0 * dnv) * ds0_p;
+    }
+}
+
+// Per-lane trace loop: pulls rays from next_ray() until it returns a value
+// >= A.n_rays.  Shared by the fan kernel and host-side builds of this code.
+template <int G, int NEQ, class NextRay>
+GA_HD void lane_loop(const FanArgs &A, const Medium &M, int lane, NextRay next_ray)
+{
+    const Limits L = A.L;
+
+    double y[NEQ];
+    double *ypl = A.ypbuf + (size_t)lane * 18;
+    double s = 0.0, tt = 0.0, att = 0.0, ttl = 0.0, attl = 0.0, rmax = 0.0;
+    double c0 = 0.0, nu0s = 0.0, th = 0.0, ph = 0.0, incl = 0.0, az = 0.0;
+    int k = 0, bnc = 0, nst = 0, ray = 0, cur = 0, narr = 0;
+    bool fetch = true;
+#pragma unroll
+    for (int i = 0; i < NEQ; ++i) y[i] = 0.0;
+
+    for (;;) {
+        if (fetch) {
+            ray = next_ray();
+            if (ray >= A.n_rays) break;
+            incl = A.launches[2 * ray];
+            az = A.launches[2 * ray + 1];
+            init_ray<G, NEQ>(A, M, incl, az, y, c0, nu0s, th, ph, cur);
+            s = 0.0; tt = 0.0; att = 0.0; ttl = 0.0; attl = 0.0; rmax = A.rmax_init;
+            k = 0; bnc = 0; nst = 0; narr = 0;
+            fetch = false;
+        }
+        double y1[NEQ];
+        s += rk4_step<G, NEQ>(M, L, y, y1, cur);
+        ++k;
+        ++nst;
+        bool brk = false, end = false;
+        if (s > L.s_max) { brk = true; end = true; }
+        else if (leaves_region<G>(A, M, L, y1, 0.0)) { brk = true; end = true; }
+        else if (below_ground<G>(M, y1)) { end = true; }
+        else if (k >= A.max_steps) { brk = true; end = true; }
+        else if (!(y1[0] == y1[0])) { brk = true; end = true; }   // non-finite state: stop the ray
+
+        double dt, da;
+        seg_book<G>(M, y, y1, A.freq, cur, dt, da);
+        ttl += dt;
+        attl += da;
+        rmax = fmax(rmax, zcoord<G>(M, y));
+
+        if (!end) {
+#pragma unroll
+            for (int i = 0; i < NEQ; ++i) { ypl[i] = y[i]; y[i] = y1[i]; }
+            continue;
+        }
+        // leg finished at row K = k (y1); rows K-1 = y, K-2 = yp
+        tt += ttl;
+        att += attl;
+        ttl = 0.0;
+        attl = 0.0;
+        bool done = false;
+        if (rmax < M.z0 + A.turn_ht_min) done = true;
+        else if (brk || k < 2 || (G == GEOM_SPH && (tt / 3600.0) > L.time_max)) done = true;
+        else {
+            {
+                double *r = A.raw + ((size_t)ray * (A.bounces + 1) + bnc) * RAW;
+#pragma unroll
+                for (int i = 0; i < NEQ; ++i) r[i] = y1[i];
+                r[18] = tt; r[19] = att; r[20] = rmax; r[21] = th; r[22] = ph; r[23] = c0; r[24] = nu0s;
+            }
+            ++narr;
+            if (bnc == A.bounces) done = true;
+            else {
+                double yr[NEQ], yp[NEQ];
+#pragma unroll
+                for (int i = 0; i < NEQ; ++i) yp[i] = ypl[i];
+                reflect<G, NEQ>(M, yp, y, yr, c0, cur);
+#pragma unroll
+                for (int i = 0; i < NEQ; ++i) y[i] = yr[i];
+                ++bnc;
+                s = 0.0;
+                k = 0;
+            }
+        }
+        if (done) {
+            A.counts[ray] = narr;
+            A.steps[ray] = nst;
+            fetch = true;
+        }
+    }
+}
+
+}  // namespace ga

@@ -1,0 +1,208 @@
+"""ctypes binding of libinfraga_gpu.so (the C ABI declared in include/infraga_gpu.h).
+
+This is host plumbing for tests, the bench and Python callers; the product
+compute path is the HIP library itself.  Loading fails loudly when the
+library is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libinfraga_gpu.so")
+
+GA_GEOM_3D, GA_GEOM_SPH = 0, 1
+GA_OK, GA_E_ARG, GA_E_HIP, GA_E_NOMEM, GA_E_CAP, GA_E_NODEV, GA_E_ATMO, GA_E_IO = 0, -1, -2, -3, -4, -5, -6, -7
+
+EXPORTED = ["ga_prop_params_default", "ga_eig_params_default", "ga_ctx_create", "ga_ctx_destroy",
+            "ga_last_error", "ga_atmo_load_strat", "ga_atmo_load_file", "ga_atmo_release", "ga_fan_run",
+            "ga_fan_run_device", "ga_eig_search"]
+
+_dp = C.POINTER(C.c_double)
+
+
+class PropParams(C.Structure):
+    _fields_ = [("geometry", C.c_int), ("calc_amp", C.c_int), ("bounces", C.c_int),
+                ("freq", C.c_double), ("abs_coeff", C.c_double), ("turn_ht_min", C.c_double),
+                ("src", C.c_double * 3), ("z_grnd", C.c_double), ("s_max", C.c_double),
+                ("ds_min", C.c_double), ("ds_max", C.c_double), ("alt_max", C.c_double),
+                ("rng_max", C.c_double), ("time_max", C.c_double), ("bounds", C.c_double * 4),
+                ("r_earth", C.c_double)]
+
+
+class EigParams(C.Structure):
+    _fields_ = [("incl_min", C.c_double), ("incl_max", C.c_double), ("bnc_min", C.c_int),
+                ("bnc_max", C.c_int), ("iterations", C.c_int), ("damping", C.c_double),
+                ("tolerance", C.c_double), ("az_dev_lim", C.c_double), ("incl_step_min", C.c_double),
+                ("incl_step_max", C.c_double)]
+
+
+class Arrival(C.Structure):
+    _fields_ = [("launch_idx", C.c_int32), ("n_bounce", C.c_int32), ("incl", C.c_double), ("az", C.c_double),
+                ("pos0", C.c_double), ("pos1", C.c_double), ("time", C.c_double), ("cel", C.c_double),
+                ("turn_ht", C.c_double), ("incl_arr", C.c_double), ("back_az", C.c_double),
+                ("tl", C.c_double), ("absorption", C.c_double)]
+
+
+class RunStats(C.Structure):
+    _fields_ = [("ray_steps", C.c_int64), ("kernel_ms", C.c_double), ("wall_ms", C.c_double),
+                ("n_rays", C.c_int32), ("n_launches", C.c_int32)]
+
+
+ARRIVAL_DTYPE = np.dtype([("launch_idx", "i4"), ("n_bounce", "i4"), ("incl", "f8"), ("az", "f8"),
+                          ("pos0", "f8"), ("pos1", "f8"), ("time", "f8"), ("cel", "f8"), ("turn_ht", "f8"),
+                          ("incl_arr", "f8"), ("back_az", "f8"), ("tl", "f8"), ("absorption", "f8")])
+assert ARRIVAL_DTYPE.itemsize == C.sizeof(Arrival)
+
+
+class EngineError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"infraga_gpu error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load_library(path: str | None = None):
+    """Load libinfraga_gpu.so and declare its signatures."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise FileNotFoundError(f"{p} is missing: build it with __graft_entry__.build()")
+    L = C.CDLL(p)
+    L.ga_prop_params_default.argtypes = [C.POINTER(PropParams), C.c_int]
+    L.ga_eig_params_default.argtypes = [C.POINTER(EigParams)]
+    L.ga_ctx_create.argtypes = [C.POINTER(C.c_void_p), C.c_int]
+    L.ga_ctx_destroy.argtypes = [C.c_void_p]
+    L.ga_last_error.argtypes = [C.c_void_p]
+    L.ga_last_error.restype = C.c_char_p
+    L.ga_atmo_load_strat.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, C.c_int, C.POINTER(C.c_int)]
+    L.ga_atmo_load_file.argtypes = [C.c_void_p, C.c_char_p, C.c_char_p, C.c_int, C.POINTER(C.c_int)]
+    L.ga_atmo_release.argtypes = [C.c_void_p, C.c_int]
+    L.ga_fan_run.argtypes = [C.c_void_p, C.c_int, C.POINTER(PropParams), _dp, C.c_int, C.POINTER(Arrival),
+                             C.c_int, C.POINTER(C.c_int), C.POINTER(RunStats)]
+    L.ga_fan_run_device.argtypes = [C.c_void_p, C.c_int, C.POINTER(PropParams), C.c_void_p, C.c_int,
+                                    C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.ga_eig_search.argtypes = [C.c_void_p, C.c_int, C.POINTER(PropParams), C.POINTER(EigParams), _dp, C.c_int,
+                                C.POINTER(Arrival), C.c_int, C.POINTER(C.c_int), _dp, C.POINTER(RunStats)]
+    for f in EXPORTED:
+        getattr(L, f)
+    if path is None:
+        _lib = L
+    return L
+
+
+def prop_params(geometry, **kw) -> PropParams:
+    """Reference defaults (main.infraga.sph.cpp:208-221) with overrides."""
+    L = load_library()
+    p = PropParams()
+    L.ga_prop_params_default(C.byref(p), geometry)
+    for k, v in kw.items():
+        if k in ("src", "bounds"):
+            arr = getattr(p, k)
+            for i, x in enumerate(v):
+                arr[i] = x
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def eig_params(**kw) -> EigParams:
+    L = load_library()
+    e = EigParams()
+    L.ga_eig_params_default(C.byref(e))
+    for k, v in kw.items():
+        setattr(e, k, v)
+    return e
+
+
+class Engine:
+    """One device context (ga_ctx) with resident atmospheres."""
+
+    def __init__(self, device: int = 0):
+        self.L = load_library()
+        h = C.c_void_p()
+        rc = self.L.ga_ctx_create(C.byref(h), device)
+        if rc != GA_OK:
+            raise EngineError(rc, "ga_ctx_create failed (no usable GPU?)")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.ga_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != GA_OK:
+            raise EngineError(rc, self.L.ga_last_error(self.h).decode())
+
+    def load_atmo(self, z, c, u, v, rho) -> int:
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (z, c, u, v, rho)]
+        aid = C.c_int(0)
+        self._check(self.L.ga_atmo_load_strat(self.h, *[a.ctypes.data_as(_dp) for a in arrs], len(arrs[0]),
+                                              C.byref(aid)))
+        return aid.value
+
+    def load_atmo_file(self, path, fmt="zTuvdp", reverse_winds=False) -> int:
+        aid = C.c_int(0)
+        self._check(self.L.ga_atmo_load_file(self.h, path.encode(), fmt.encode(), int(reverse_winds), C.byref(aid)))
+        return aid.value
+
+    def release_atmo(self, aid):
+        self._check(self.L.ga_atmo_release(self.h, aid))
+
+    def fan(self, aid, params: PropParams, launches):
+        """Trace a fan; returns (arrivals structured array, RunStats)."""
+        Lc = np.ascontiguousarray(launches, dtype=np.float64).reshape(-1, 2)
+        n = Lc.shape[0]
+        cap = max(1, n * (params.bounces + 1))
+        out = np.zeros(cap, dtype=ARRIVAL_DTYPE)
+        n_out = C.c_int(0)
+        st = RunStats()
+        self._check(self.L.ga_fan_run(self.h, aid, C.byref(params), Lc.ctypes.data_as(_dp), n,
+                                      out.ctypes.data_as(C.POINTER(Arrival)), cap, C.byref(n_out), C.byref(st)))
+        return out[: n_out.value].copy(), st
+
+    def fan_device(self, aid, params: PropParams, d_launches: int, n: int, d_out: int, d_counts: int,
+                   d_steps: int, stream: int = 0):
+        """Enqueue a fan on device-resident buffers (raw device pointers)."""
+        self._check(self.L.ga_fan_run_device(self.h, aid, C.byref(params), C.c_void_p(d_launches), n,
+                                             C.c_void_p(d_out), C.c_void_p(d_counts), C.c_void_p(d_steps),
+                                             C.c_void_p(stream) if stream else None))
+
+    def eig_search(self, aid, params: PropParams, eig: EigParams, rcvr_xy):
+        R = np.ascontiguousarray(rcvr_xy, dtype=np.float64).reshape(-1, 2)
+        n = R.shape[0]
+        cap = max(16, 8 * n * (eig.bnc_max - eig.bnc_min + 1))
+        out = np.zeros(cap, dtype=ARRIVAL_DTYPE)
+        lat = np.zeros(n)
+        n_out = C.c_int(0)
+        st = RunStats()
+        rc = self.L.ga_eig_search(self.h, aid, C.byref(params), C.byref(eig), R.ctypes.data_as(_dp), n,
+                                  out.ctypes.data_as(C.POINTER(Arrival)), cap, C.byref(n_out),
+                                  lat.ctypes.data_as(_dp), C.byref(st))
+        if rc == GA_E_CAP:
+            cap = n_out.value
+            out = np.zeros(cap, dtype=ARRIVAL_DTYPE)
+            rc = self.L.ga_eig_search(self.h, aid, C.byref(params), C.byref(eig), R.ctypes.data_as(_dp), n,
+                                      out.ctypes.data_as(C.POINTER(Arrival)), cap, C.byref(n_out),
+                                      lat.ctypes.data_as(_dp), C.byref(st))
+        self._check(rc)
+        return out[: n_out.value].copy(), lat, st
+
+
+def nan():
+    return math.nan

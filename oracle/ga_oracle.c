@@ -1141,6 +1141,7 @@ int go_fan(go_atmo *a, const go_params *p, const go_prop *pr,
     return cnt > out_cap ? -1 : 0;
 }
 
+#ifndef GO_OPCOUNT
 /* ======================================================================
  * Eigenray search, 3D: geoac.eigenray.3d.cpp:34-366 and the driver loop of
  * main.infraga.3d.cpp:944-951.
@@ -1358,6 +1359,8 @@ int go_eig_search_3d(go_atmo *a, const go_params *p, const double src_in[3],
     return cnt;
 }
 
+#endif /* GO_OPCOUNT */
+
 /* ======================================================================
  * Handle helpers for the test bindings (ctypes cannot size go_atmo).
  * ==================================================================== */
@@ -1385,4 +1388,11 @@ int go_h_fan(void *h, const go_params *p, const go_prop *pr, const double *launc
              go_arrival *out, int cap, int *n_out, long long *steps)
 {
     return go_fan((go_atmo *)h, p, pr, launches, n, out, cap, n_out, steps);
+}
+
+void go_h_tables(void *h, int which, double *x, double *f, double *slope)
+{
+    go_atmo *a = (go_atmo *)h;
+    go_spline *sp[4] = {&a->c, &a->u, &a->v, &a->rho};
+    for (int i = 0; i < a->n; i++) { x[i] = a->z[i]; f[i] = sp[which]->f[i]; slope[i] = sp[which]->slope[i]; }
 }

@@ -163,6 +163,7 @@ void   go_h_free(void *h);
 void   go_h_set_ground(void *h, double z0);
 void   go_h_set_tweak(void *h, double t);
 void   go_h_set_r0(void *h, double r0);
+void   go_h_tables(void *h, int which, double *x, double *f, double *slope);
 void   go_h_params(void *h, int calc_amp, go_params *p);
 int    go_h_fan(void *h, const go_params *p, const go_prop *pr, const double *launches, int n,
                 go_arrival *out, int cap, int *n_out, long long *steps);

@@ -70,6 +70,7 @@ class Oracle:
         L.go_h_set_tweak.argtypes = [C.c_void_p, C.c_double]
         L.go_h_set_r0.argtypes = [C.c_void_p, C.c_double]
         L.go_h_params.argtypes = [C.c_void_p, C.c_int, C.POINTER(Params)]
+        L.go_h_tables.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp]
         L.go_h_fan.argtypes = [C.c_void_p, C.POINTER(Params), C.POINTER(Prop), _dp, C.c_int,
                                C.POINTER(Arrival), C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_longlong)]
         L.go_h_fan.restype = C.c_int

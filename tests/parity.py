@@ -1,0 +1,66 @@
+"""Arrival-set comparison at the stated parity bar.
+
+Arrivals are keyed by (launch index, bounce).  The key sets must be equal;
+range, turning height and travel time must agree within 1e-6 relative; TL
+and absorption within 0.01 dB.  Positions are compared through the range
+from the source (great-circle for the spherical geometry) and directly to
+1e-6 relative of that range.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+R0 = 6370.0
+
+
+def gc_range(lat_deg, lon_deg, lat0_deg, lon0_deg, r0=R0):
+    la, lo, la0, lo0 = map(np.radians, (lat_deg, lon_deg, lat0_deg, lon0_deg))
+    h = np.sin((la - la0) / 2) ** 2 + np.cos(la) * np.cos(la0) * np.sin((lo - lo0) / 2) ** 2
+    return 2 * r0 * np.arcsin(np.sqrt(h))
+
+
+def keyed(arr):
+    return {(int(a["launch_idx"]), int(a["n_bounce"])): a for a in arr}
+
+
+def compare(gpu, ref, geom_sph, src=(0.0, 0.0), rel=1e-6, db=0.01, near=None):
+    """Return a dict of failures; empty means parity holds.
+
+    near: optional set of launch indices reported separately (threshold cases)."""
+    g, r = keyed(gpu), keyed(ref)
+    fails = {}
+    missing = sorted(set(r) - set(g))
+    extra = sorted(set(g) - set(r))
+    if missing:
+        fails["missing"] = missing[:20]
+    if extra:
+        fails["extra"] = extra[:20]
+    worst = {"range": 0.0, "pos": 0.0, "time": 0.0, "turn_ht": 0.0, "tl": 0.0, "absorption": 0.0}
+    bad = []
+    for k in sorted(set(g) & set(r)):
+        a, b = g[k], r[k]
+        if geom_sph:
+            ra = gc_range(a["pos0"], a["pos1"], *src)
+            rb = gc_range(b["pos0"], b["pos1"], *src)
+            dpos = R0 * np.radians(np.hypot(a["pos0"] - b["pos0"], (a["pos1"] - b["pos1"]) * np.cos(np.radians(b["pos0"]))))
+        else:
+            ra = np.hypot(a["pos0"] - src[0], a["pos1"] - src[1])
+            rb = np.hypot(b["pos0"] - src[0], b["pos1"] - src[1])
+            dpos = np.hypot(a["pos0"] - b["pos0"], a["pos1"] - b["pos1"])
+        e = {
+            "range": abs(ra - rb) / max(abs(rb), 1e-12),
+            "pos": dpos / max(abs(rb), 1e-12),
+            "time": abs(a["time"] - b["time"]) / max(abs(b["time"]), 1e-12),
+            "turn_ht": abs(a["turn_ht"] - b["turn_ht"]) / max(abs(b["turn_ht"]), 1e-12),
+            "tl": abs(a["tl"] - b["tl"]),
+            "absorption": abs(a["absorption"] - b["absorption"]),
+        }
+        for kk, v in e.items():
+            worst[kk] = max(worst[kk], v)
+        if (e["range"] > rel or e["pos"] > rel or e["time"] > rel or e["turn_ht"] > rel
+                or e["tl"] > db or e["absorption"] > db):
+            bad.append((k, e))
+    if bad:
+        fails["values"] = bad[:10]
+    fails_summary = {"worst": worst, "n_common": len(set(g) & set(r))}
+    return fails, fails_summary

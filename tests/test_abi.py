@@ -1,0 +1,55 @@
+"""The C-ABI library loads (no GPU needed) and exports every function that
+include/infraga_gpu.h declares; struct layouts seen from Python match."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+from infraga_amd import engine
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    txt = open(os.path.join(ROOT, "include", "infraga_gpu.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(ga_[a-z_]+)\s*\(", txt)))
+
+
+def test_header_declares_expected_api():
+    names = declared()
+    for n in engine.EXPORTED:
+        assert n in names
+    assert len(names) == len(engine.EXPORTED)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = engine.load_library()
+    out = subprocess.run(["nm", "-D", "--defined-only", engine.LIB_PATH], capture_output=True, text=True, check=True).stdout
+    exported = set(line.split()[-1] for line in out.splitlines() if line.strip())
+    for n in declared():
+        assert n in exported, n
+        getattr(lib, n)
+
+
+def test_struct_sizes():
+    # int + 2 ints + doubles, natural alignment (matches the C header)
+    assert C.sizeof(engine.Arrival) == 8 + 11 * 8
+    assert C.sizeof(engine.RunStats) == 8 + 8 + 8 + 4 + 4
+    assert C.sizeof(engine.PropParams) == 3 * 4 + 4 + 3 * 8 + 3 * 8 + 8 + 3 * 8 + 3 * 8 + 4 * 8 + 8
+
+
+def test_defaults_follow_reference_driver():
+    p = engine.prop_params(engine.GA_GEOM_SPH)
+    assert (p.s_max, p.ds_min, p.ds_max, p.rng_max, p.time_max) == (1000.0, 0.001, 0.05, 1000.0, 10.0)
+    assert (p.freq, p.abs_coeff, p.turn_ht_min, p.bounces, p.r_earth) == (0.1, 1.0, 0.2, 10, 6370.0)
+    q = engine.prop_params(engine.GA_GEOM_3D)
+    assert q.rng_max == 2000.0 and list(q.bounds) == [-2000.0, 2000.0, -2000.0, 2000.0]
+
+
+def test_no_device_reports_error():
+    """Without a GPU (this container) context creation fails cleanly."""
+    lib = engine.load_library()
+    h = C.c_void_p()
+    rc = lib.ga_ctx_create(C.byref(h), 99)
+    assert rc == engine.GA_E_NODEV and not h.value
