@@ -1,0 +1,254 @@
+// infraga_cli.cpp -- drop-in front ends infraga-sph / infraga-3d on the GPU engine.
+//
+// Same command line as the reference drivers (src/main.infraga.sph.cpp,
+// src/main.infraga.3d.cpp): an option (-prop, -eig_search), the profile path,
+// then key=value parameters with the same names, aliases and defaults.  The
+// ray tracing runs on the MI355X through the C ABI of libinfraga_gpu.so; this
+// file only parses arguments, builds the launch list with the reference's
+// accumulated loops and writes the output files in the reference's format.
+//
+// Differences from the reference drivers, by design:
+//  - an unknown key is an error (exit 2) instead of an interactive prompt;
+//  - write_rays=true writes the arrivals but not <id>.raypaths.dat (ray path
+//    sampling is not part of this engine); write_caustics, write_topo,
+//    topo_file and topo_use_BLw are rejected.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <string>
+#include <vector>
+
+#include "infraga_gpu.h"
+
+#ifndef GA_CLI_GEOM
+#define GA_CLI_GEOM GA_GEOM_SPH
+#endif
+#ifndef GA_CLI_NAME
+#define GA_CLI_NAME "infraga-sph"
+#endif
+
+namespace {
+
+constexpr double Pi = 3.141592653589793238462643;
+
+bool str2bool(const char *v)
+{
+    return (std::strcmp(v, "true") == 0 || std::strcmp(v, "True") == 0 || std::strcmp(v, "TRUE") == 0 ||
+            std::atoi(v) != 0);
+}
+
+bool key(const char *arg, const char *k, const char *&val)
+{
+    size_t n = std::strlen(k);
+    if (std::strncmp(arg, k, n) == 0) { val = arg + n; return true; }
+    return false;
+}
+
+bool key2(const char *arg, const char *k1, const char *k2, const char *&val)
+{
+    return key(arg, k1, val) || key(arg, k2, val);
+}
+
+int die(ga_ctx *ctx, const char *what)
+{
+    std::cerr << GA_CLI_NAME << ": " << what;
+    if (ctx) std::cerr << ": " << ga_last_error(ctx);
+    std::cerr << '\n';
+    if (ctx) ga_ctx_destroy(ctx);
+    return 1;
+}
+
+void usage()
+{
+    std::cout << "Usage: " << GA_CLI_NAME << " [option] profile.met [parameters]\n"
+              << "  -prop        ray paths for a fan of inclinations and azimuths\n"
+              << "  -eig_search  eigenrays between the source and a receiver" << (GA_CLI_GEOM == GA_GEOM_3D ? "" : " (3D only)") << "\n"
+              << "Parameters use the key=value names of the reference infraga drivers.\n";
+}
+
+// Strip the extension in place, as the reference does (main.infraga.sph.cpp:339-347):
+// the header line "profile:" then shows the stripped path.
+std::string strip_ext(char *path)
+{
+    for (int m = (int)std::strlen(path); m >= 0; m--) {
+        if (path[m] == '.') { path[m] = '\0'; break; }
+    }
+    return std::string(path);
+}
+
+int run_prop(int argc, char **argv)
+{
+    const bool sph = GA_CLI_GEOM == GA_GEOM_SPH;
+    double theta_min = 0.5, theta_max = 45.0, theta_step = 0.5;
+    double phi_min = -90.0, phi_max = -90.0, phi_step = 1.0;
+    int bounces = 10;
+    double s0 = sph ? 30.0 : 0.0, s1 = 0.0, z_src = 0.0;
+    bool write_rays = true, reverse_winds = false, custom_id = false;
+    double freq = 0.1, turn_ht_min = 0.2;
+    const char *prof_format = "zTuvdp";
+    const char *output_id = nullptr;
+    ga_prop_params p;
+    ga_prop_params_default(&p, GA_CLI_GEOM);
+    double z_grnd = NAN;
+
+    for (int i = 3; i < argc; i++) {
+        const char *a = argv[i], *v = nullptr;
+        if (key2(a, "incl_min=", "min_incl=", v)) theta_min = std::atof(v);
+        else if (key2(a, "incl_max=", "max_incl=", v)) theta_max = std::atof(v);
+        else if (key2(a, "az_min=", "min_az=", v)) phi_min = std::atof(v);
+        else if (key2(a, "az_max=", "max_az=", v)) phi_max = std::atof(v);
+        else if (key(a, "incl_step=", v)) theta_step = std::atof(v);
+        else if (key(a, "az_step=", v)) phi_step = std::atof(v);
+        else if (key(a, "inclination=", v)) { theta_min = theta_max = std::atof(v); theta_step = 1.0; }
+        else if (key(a, "azimuth=", v)) { phi_min = phi_max = std::atof(v); phi_step = 1.0; }
+        else if (key(a, "bounces=", v)) bounces = std::atoi(v);
+        else if (key2(a, "src_alt=", "alt_src=", v)) z_src = std::atof(v);
+        else if (sph && key2(a, "src_lat=", "lat_src=", v)) s0 = std::atof(v);
+        else if (sph && key2(a, "src_lon=", "lon_src=", v)) s1 = std::atof(v);
+        else if (!sph && key2(a, "src_x=", "x_src=", v)) s0 = std::atof(v);
+        else if (!sph && key2(a, "src_y=", "y_src=", v)) s1 = std::atof(v);
+        else if (key(a, "turn_ht_min=", v)) turn_ht_min = std::atof(v);
+        else if (key(a, "write_rays=", v)) write_rays = str2bool(v);
+        else if (key(a, "freq=", v)) freq = std::atof(v);
+        else if (key(a, "abs_coeff=", v)) p.abs_coeff = std::max(0.0, std::atof(v));
+        else if (key(a, "calc_amp=", v)) p.calc_amp = str2bool(v) ? 1 : 0;
+        else if (key2(a, "max_alt=", "alt_max=", v)) p.alt_max = std::atof(v);
+        else if (key2(a, "max_rng=", "rng_max=", v)) p.rng_max = std::atof(v);
+        else if (sph && key2(a, "max_time=", "time_max=", v)) p.time_max = std::atof(v);
+        else if (sph && key2(a, "min_lat=", "lat_min=", v)) p.bounds[0] = std::atof(v);
+        else if (sph && key2(a, "max_lat=", "lat_max=", v)) p.bounds[1] = std::atof(v);
+        else if (sph && key2(a, "min_lon=", "lon_min=", v)) p.bounds[2] = std::atof(v);
+        else if (sph && key2(a, "max_lon=", "lon_max=", v)) p.bounds[3] = std::atof(v);
+        else if (!sph && key2(a, "min_x=", "x_min=", v)) p.bounds[0] = std::atof(v);
+        else if (!sph && key2(a, "max_x=", "x_max=", v)) p.bounds[1] = std::atof(v);
+        else if (!sph && key2(a, "min_y=", "y_min=", v)) p.bounds[2] = std::atof(v);
+        else if (!sph && key2(a, "max_y=", "y_max=", v)) p.bounds[3] = std::atof(v);
+        else if (key2(a, "min_ds=", "ds_min=", v)) p.ds_min = std::atof(v);
+        else if (key2(a, "max_ds=", "ds_max=", v)) p.ds_max = std::atof(v);
+        else if (key2(a, "max_s=", "s_max=", v)) p.s_max = std::atof(v);
+        else if (key(a, "write_atmo=", v)) { /* profile dump: not produced */ }
+        else if (key(a, "prof_format=", v)) prof_format = v;
+        else if (key(a, "reverse_winds=", v)) reverse_winds = str2bool(v);
+        else if (key(a, "output_id=", v)) { custom_id = true; output_id = v; }
+        else if (key(a, "z_grnd=", v)) z_grnd = std::atof(v);
+        else if (key(a, "print_resid=", v)) { /* diagnostics only */ }
+        else if (key(a, "write_caustics=", v) || key(a, "write_topo=", v) || key(a, "topo_file=", v) ||
+                 key(a, "topo_use_BLw=", v)) {
+            if (std::strcmp(v, "false") != 0 && std::strcmp(v, "False") != 0 && std::strcmp(v, "0") != 0) {
+                std::cerr << GA_CLI_NAME << ": " << a << " is not supported by this engine\n";
+                return 2;
+            }
+        } else {
+            std::cerr << GA_CLI_NAME << ": unrecognized parameter entry: " << a << '\n';
+            return 2;
+        }
+    }
+
+    ga_ctx *ctx = nullptr;
+    if (ga_ctx_create(&ctx, 0) != GA_OK) return die(nullptr, "no usable GPU");
+    int aid = 0;
+    if (ga_atmo_load_file(ctx, argv[2], prof_format, reverse_winds ? 1 : 0, &aid) != GA_OK)
+        return die(ctx, "cannot load the atmosphere");
+
+    // ground: the profile bottom unless z_grnd is given (atmo_io set_limits + driver)
+    std::ifstream probe(argv[2]);
+    double z_bottom = 0.0;
+    {
+        std::string line;
+        while (std::getline(probe, line)) {
+            if (line.find('#') == 0) continue;
+            if (std::sscanf(line.c_str(), "%lf", &z_bottom) == 1) break;
+        }
+    }
+    double z0 = std::isnan(z_grnd) ? z_bottom : z_grnd;
+    p.z_grnd = z0;
+    p.bounces = bounces;
+    p.freq = freq;
+    p.turn_ht_min = turn_ht_min;
+    double zg_alt = sph ? ((p.r_earth + z0) - p.r_earth) : z0;
+    z_src = std::max(z_src, zg_alt);
+    p.src[0] = s0;
+    p.src[1] = s1;
+    p.src[2] = z_src;
+
+    // launch list: the reference's accumulated loops (main.infraga.sph.cpp:418-434)
+    std::vector<double> launches;
+    std::vector<int> az_end;      // launch count after each azimuth (blank lines)
+    for (double phi = phi_min; phi <= phi_max; phi += phi_step) {
+        double theta_grnd = (z_src - zg_alt < 0.01) ? 0.1 : -89.9;
+        for (double theta = std::max(theta_min, theta_grnd); theta <= std::max(theta_max, theta_grnd); theta += theta_step) {
+            launches.push_back(theta);
+            launches.push_back(phi);
+        }
+        az_end.push_back((int)(launches.size() / 2));
+    }
+    const int n = (int)(launches.size() / 2);
+    std::vector<ga_arrival> arr((size_t)std::max(1, n) * (bounces + 1));
+    int n_out = 0;
+    ga_run_stats st;
+    if (ga_fan_run(ctx, aid, &p, launches.data(), n, arr.data(), (int)arr.size(), &n_out, &st) != GA_OK)
+        return die(ctx, "fan failed");
+
+    std::string id = custom_id ? std::string(output_id) : strip_ext(argv[2]);
+    std::ofstream results(id + ".arrivals.dat");
+    results << "# " << GA_CLI_NAME << " -prop summary:" << '\n';
+    results << "#" << '\t' << "profile: " << argv[2] << '\n';
+    results << "#" << '\t' << "inclination: " << theta_min << ", " << theta_max << ", " << theta_step << '\n';
+    results << "#" << '\t' << "azimuth: " << phi_min << ", " << phi_max << ", " << phi_step << '\n';
+    results << "#" << '\t' << "bounces: " << bounces << '\n';
+    if (sph) results << "#" << '\t' << "source location (lat, lon, alt): " << s0 << ", " << s1 << ", " << z_src << '\n';
+    else results << "#" << '\t' << "source location: " << s0 << ", " << s1 << ", " << z_src << '\n';
+    results << "#" << '\t' << "ground elevation: " << z0 << '\n';
+    if (sph) results << "#" << '\t' << "turn_ht_min: " << turn_ht_min << '\n';
+    results << "#" << '\t' << "frequency: " << freq << '\n';
+    results << "#" << '\t' << "S&B atten coeff: " << p.abs_coeff << '\n' << '\n';
+    results << "# incl [deg]" << '\t' << "az [deg]" << '\t' << "n_b";
+    if (sph) results << '\t' << "lat_0 [deg]" << '\t' << "lon_0 [deg]";
+    else results << '\t' << "x_0 [km]" << '\t' << "y_0 [km]";
+    results << '\t' << "time [s]" << '\t' << "cel [km/s]" << '\t' << "turning ht [km]";
+    if (sph) results << '\t' << "inclination [deg]" << '\t' << "back azimuth [deg]";
+    else results << '\t' << "arrival incl [deg]" << '\t' << "back az [deg]";
+    results << '\t' << "trans. coeff. [dB]" << '\t' << "absorption [dB]" << '\n';
+
+    int w = 0, az_i = 0;
+    for (int r = 0; r < n; ++r) {
+        while (w < n_out && arr[w].launch_idx == r) {
+            const ga_arrival &A = arr[w++];
+            results << A.incl;
+            results << '\t' << A.az;
+            results << '\t' << A.n_bounce;
+            if (sph) {
+                results << '\t' << std::setprecision(8) << A.pos0;
+                results << '\t' << std::setprecision(8) << A.pos1;
+            } else {
+                results << '\t' << A.pos0;
+                results << '\t' << A.pos1;
+            }
+            results << '\t' << A.time << '\t' << A.cel << '\t' << A.turn_ht << '\t' << A.incl_arr << '\t' << A.back_az;
+            results << '\t' << A.tl << '\t' << A.absorption << '\n';
+        }
+        if (r + 1 == az_end[az_i]) { results << '\n'; ++az_i; }
+    }
+    results.close();
+    if (write_rays)
+        std::cout << GA_CLI_NAME << ": note: ray paths are not written by this engine (arrivals only)\n";
+    std::cout << GA_CLI_NAME << ": " << n << " rays, " << n_out << " arrivals, " << st.ray_steps << " ray-steps, "
+              << st.kernel_ms << " ms on the GPU\n";
+    ga_ctx_destroy(ctx);
+    return 0;
+}
+
+}  // namespace
+
+int main(int argc, char **argv)
+{
+    if (argc < 3) { usage(); return argc < 2 ? 0 : 2; }
+    if (std::strncmp(argv[1], "-prop", 5) == 0) return run_prop(argc, argv);
+    std::cerr << GA_CLI_NAME << ": option " << argv[1] << " is not supported by this engine\n";
+    usage();
+    return 2;
+}
