@@ -145,9 +145,11 @@ int ga_fan_run_device(ga_ctx *ctx, int atmo_id, const ga_prop_params *p,
                       const double *d_launches, int n_launch,
                       ga_arrival *d_out, int32_t *d_counts, int32_t *d_steps, void *stream);
 
-/* Batched 3D eigenray search for n_rcvr receivers at (x, y) [km] pairs
- * rcvr_xy, all from p->src.  Eigenrays are returned grouped by receiver
- * (launch_idx) in the order the reference would find them. */
+/* Batched eigenray search for n_rcvr receivers, all from p->src: 3D
+ * receivers are (x, y) [km] pairs, spherical ones (lat, lon) [deg] pairs.
+ * Eigenrays are returned grouped by receiver (launch_idx), then bounce
+ * count, in the order the reference finds them; rcvr_latency_ms (may be
+ * NULL) receives each receiver's completion time within the call. */
 int ga_eig_search(ga_ctx *ctx, int atmo_id, const ga_prop_params *p, const ga_eig_params *e,
                   const double *rcvr_xy, int n_rcvr,
                   ga_eigenray *out, int out_cap, int *n_out, double *rcvr_latency_ms,

@@ -19,7 +19,7 @@ BIN = os.path.join(HERE, "bin")
 # -disable-machine-licm: keeps the FP64 polynomial constants of exp / sin /
 # Sutherland-Bass from being hoisted into long-lived VGPRs of the step loop.
 # -packed-fp32-ops off: required on these boxes (no v_pk_*_f32 generation).
-HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-mllvm", "-disable-machine-licm",
+HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++20", "-fPIC", "-mllvm", "-disable-machine-licm",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 

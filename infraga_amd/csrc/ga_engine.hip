@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "../../include/infraga_gpu.h"
+#include "ga_eig_search.hpp"
 #include "ga_trace.hpp"
 
 using namespace ga;
@@ -62,6 +63,34 @@ __global__ void __launch_bounds__(256) arrival_kernel(FanArgs A)
 }
 
 __global__ void reset_counter(unsigned int *q) { *q = 0u; }
+
+// Eigenray-search traces: same persistent layout as fan_kernel, jobs instead
+// of launch angles.  WGS lanes per workgroup share one LDS copy of the table.
+template <int G, int NEQ, bool LDS, int WGS>
+__global__ void __launch_bounds__(WGS) eig_kernel(EigArgs E)
+{
+    extern __shared__ double lds_tab[];
+    Medium M = E.F.M;
+    if constexpr (LDS) {
+        const int nd = M.n * NODE;
+        for (int i = threadIdx.x; i < nd; i += blockDim.x) lds_tab[i] = E.F.M.tab[i];
+        __syncthreads();
+        M.tab = lds_tab;
+    }
+    const int lane = blockIdx.x * blockDim.x + threadIdx.x;
+    eig_lane_loop<G, NEQ>(E, M, lane, [&]() { return (int)atomicAdd(E.F.queue, 1u); });
+}
+
+// Arrival rows of the EIG_FINAL jobs [first, first + n).
+template <int G>
+__global__ void __launch_bounds__(256) eig_arrival_kernel(EigArgs E, int first, int n, ga_arrival *out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int cur = 0;
+    const EigJob J = E.jobs[first + i];
+    out[i] = eig_arrival_row<G>(E, E.F.M, J, E.res + (size_t)(first + i) * EIG_FULL, cur);
+}
 
 // ---------------------------------------------------------------------------
 // Host side
@@ -139,6 +168,20 @@ void spline_slopes(const std::vector<double> &x, const std::vector<double> &f, s
     nd[n - 1] = (di - nd[n - 2] * ai) / (bi - nc[n - 2] * ai);
     s[n - 1] = nd[n - 1];
     for (int i = n - 2; i > -1; i--) s[i] = nd[i] - nc[i] * s[i + 1];
+}
+
+// c(z) by the spline formula of the device code, on the host (arrival
+// inclinations and the eigenray update need c at the source and ground).
+double host_c_at(const DevAtmo &a, double z)
+{
+    double zs = std::min(std::max(z, a.z.front()), a.z.back());
+    int k = 0;
+    while (k < a.n - 2 && zs > a.z[k + 1]) ++k;
+    std::vector<double> sl;
+    spline_slopes(a.z, a.c, sl);
+    double dx = a.z[k + 1] - a.z[k], X = (zs - a.z[k]) / dx, df = a.c[k + 1] - a.c[k];
+    double Ac = sl[k] * dx - df, Bc = -sl[k + 1] * dx + df;
+    return a.c[k] + X * (df + (1.0 - X) * (Ac * (1.0 - X) + Bc * X));
 }
 
 bool build_medium(ga_ctx *ctx, const DevAtmo &a, const ga_prop_params *p, Medium &M, Limits &L, int &rc)
@@ -246,18 +289,7 @@ int fan_enqueue(ga_ctx *ctx, int atmo_id, const ga_prop_params *p, const double 
         A.src1 = p->src[1];
         A.src2 = zsrc;
     }
-    // c at the source, evaluated on the host with the same spline formula
-    {
-        double zs = sph ? A.src0 - A.M.r0 : A.src2;
-        zs = std::min(std::max(zs, a.z.front()), a.z.back());
-        int k = 0;
-        while (k < a.n - 2 && zs > a.z[k + 1]) ++k;
-        std::vector<double> sl;
-        spline_slopes(a.z, a.c, sl);
-        double dx = a.z[k + 1] - a.z[k], X = (zs - a.z[k]) / dx, df = a.c[k + 1] - a.c[k];
-        double Ac = sl[k] * dx - df, Bc = -sl[k + 1] * dx + df;
-        A.c_src = a.c[k] + X * (df + (1.0 - X) * (Ac * (1.0 - X) + Bc * X));
-    }
+    A.c_src = host_c_at(a, sph ? A.src0 - A.M.r0 : A.src2);
     A.freq = p->freq;
     A.turn_ht_min = p->turn_ht_min;
     A.rmax_init = sph ? 0.0 : A.M.z0;
@@ -294,7 +326,22 @@ int fan_enqueue(ga_ctx *ctx, int atmo_id, const ga_prop_params *p, const double 
 // RAII device buffer and event pair for the host entry points.
 template <class T> struct DevBuf {
     T *p = nullptr;
-    bool alloc(size_t n) { return hipMalloc(&p, n * sizeof(T)) == hipSuccess; }
+    size_t cap = 0;
+    bool alloc(size_t n)
+    {
+        if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) { p = nullptr; return false; }
+        cap = n;
+        return true;
+    }
+    // grow-only; the caller has synchronised the streams that use the buffer
+    bool ensure(size_t n)
+    {
+        if (n <= cap) return true;
+        if (p && hipFree(p) != hipSuccess) return false;
+        p = nullptr;
+        cap = 0;
+        return alloc(std::max(n, (size_t)1));
+    }
     ~DevBuf() { if (p) (void)hipFree(p); }
 };
 
@@ -306,6 +353,140 @@ struct Events {
     {
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
+    }
+};
+
+// GPU side of the eigenray search: one round of traces per call.  EIG_SWEEP
+// jobs run on the context stream in the 6-equation kernel, EIG_LM/EIG_FINAL
+// jobs on a second stream in the 18-equation kernel, followed there by the
+// arrival rows of the EIG_FINAL jobs.
+struct EigGpu {
+    ga_ctx *ctx = nullptr;
+    int geom = GA_GEOM_3D;
+    FanArgs F{};                 // medium, limits, source, c_src, freq, max_steps, lds
+    hipStream_t s2 = nullptr;
+    DevBuf<EigJob> jobs6, jobs18;
+    DevBuf<double> res6, res18, yp6, yp18;
+    DevBuf<int32_t> brk6, brk18, st6, st18;
+    DevBuf<ga_arrival> arr;
+    DevBuf<unsigned int> q;
+    double kernel_ms = 0.0;
+    int launches = 0;
+
+    ~EigGpu()
+    {
+        if (s2) {
+            (void)hipStreamSynchronize(s2);
+            (void)hipStreamDestroy(s2);
+        }
+    }
+
+    int init()
+    {
+        HIPCHK(ctx, hipStreamCreate(&s2));
+        if (!q.alloc(2)) return fail(ctx, GA_E_NOMEM, "eigenray queue counters");
+        return GA_OK;
+    }
+
+    template <int G, int NEQ, int WGS>
+    int launch(const EigArgs &E0, DevBuf<double> &yp, unsigned int *queue, hipStream_t st)
+    {
+        size_t lds = F.lds ? (size_t)F.M.n * NODE * sizeof(double) : 0;
+        auto kern = F.lds ? eig_kernel<G, NEQ, true, WGS> : eig_kernel<G, NEQ, false, WGS>;
+        if (lds > 65536) HIPCHK(ctx, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        int per_cu = 0;
+        HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, WGS, lds));
+        per_cu = std::max(per_cu, 1);
+        int grid = std::max(1, std::min(per_cu * ctx->n_cu, (E0.n_jobs + WGS - 1) / WGS));
+        if (!yp.ensure((size_t)grid * WGS * 18)) return fail(ctx, GA_E_NOMEM, "eigenray lane buffer");
+        EigArgs E = E0;
+        E.F.ypbuf = yp.p;
+        E.F.nlanes = grid * WGS;
+        E.F.queue = queue;
+        hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, st, queue);
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(WGS), lds, st, E);
+        HIPCHK(ctx, hipGetLastError());
+        launches += 2;
+        return GA_OK;
+    }
+
+    int operator()(const EigBatch &B, EigBatchOut &O)
+    {
+        const size_t n6 = B.sweep.size(), n18 = B.full.size(), nf = n18 - (size_t)B.n_lm;
+        const hipStream_t st = ctx->stream;
+        HIPCHK(ctx, hipStreamSynchronize(st));
+        HIPCHK(ctx, hipStreamSynchronize(s2));
+        if (!jobs6.ensure(n6) || !res6.ensure(n6 * B.sweep_stride) || !brk6.ensure(n6) || !st6.ensure(n6) ||
+            !jobs18.ensure(n18) || !res18.ensure(n18 * EIG_FULL) || !brk18.ensure(n18) || !st18.ensure(n18) ||
+            !arr.ensure(nf))
+            return fail(ctx, GA_E_NOMEM, "eigenray batch buffers (%zu + %zu traces)", n6, n18);
+        Events ev;
+        hipEvent_t mid = nullptr;
+        if (!ev.ok || hipEventCreate(&mid) != hipSuccess) return fail(ctx, GA_E_HIP, "hipEventCreate failed");
+        struct EvGuard { hipEvent_t e; ~EvGuard() { (void)hipEventDestroy(e); } } guard{mid};
+        if (n6) HIPCHK(ctx, hipMemcpyAsync(jobs6.p, B.sweep.data(), n6 * sizeof(EigJob), hipMemcpyHostToDevice, st));
+        if (n18) HIPCHK(ctx, hipMemcpyAsync(jobs18.p, B.full.data(), n18 * sizeof(EigJob), hipMemcpyHostToDevice, st));
+        HIPCHK(ctx, hipEventRecord(ev.e0, st));
+        HIPCHK(ctx, hipStreamWaitEvent(s2, ev.e0, 0));
+        if (n6) {
+            EigArgs E{};
+            E.F = F;
+            E.jobs = jobs6.p;
+            E.n_jobs = (int)n6;
+            E.stride = B.sweep_stride;
+            E.res = res6.p;
+            E.brk_leg = brk6.p;
+            E.steps = st6.p;
+            int rc = geom == GA_GEOM_SPH ? launch<GEOM_SPH, 6, 512>(E, yp6, q.p, st)
+                                         : launch<GEOM_3D, 6, 512>(E, yp6, q.p, st);
+            if (rc != GA_OK) return rc;
+        }
+        if (n18) {
+            EigArgs E{};
+            E.F = F;
+            E.jobs = jobs18.p;
+            E.n_jobs = (int)n18;
+            E.stride = EIG_FULL;
+            E.res = res18.p;
+            E.brk_leg = brk18.p;
+            E.steps = st18.p;
+            int rc = geom == GA_GEOM_SPH ? launch<GEOM_SPH, 18, WG>(E, yp18, q.p + 1, s2)
+                                         : launch<GEOM_3D, 18, WG>(E, yp18, q.p + 1, s2);
+            if (rc != GA_OK) return rc;
+            if (nf) {
+                auto ak = geom == GA_GEOM_SPH ? eig_arrival_kernel<GEOM_SPH> : eig_arrival_kernel<GEOM_3D>;
+                hipLaunchKernelGGL(ak, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, s2, E, B.n_lm,
+                                   (int)nf, arr.p);
+                HIPCHK(ctx, hipGetLastError());
+                launches += 1;
+            }
+        }
+        HIPCHK(ctx, hipEventRecord(mid, s2));
+        HIPCHK(ctx, hipStreamWaitEvent(st, mid, 0));
+        HIPCHK(ctx, hipEventRecord(ev.e1, st));
+        O.sweep_res.resize(n6 * B.sweep_stride);
+        O.sweep_brk.resize(n6);
+        O.sweep_steps.resize(n6);
+        O.full_res.resize(n18 * EIG_FULL);
+        O.full_brk.resize(n18);
+        O.full_steps.resize(n18);
+        O.final_rows.resize(nf);
+        if (n6) {
+            HIPCHK(ctx, hipMemcpyAsync(O.sweep_res.data(), res6.p, O.sweep_res.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+            HIPCHK(ctx, hipMemcpyAsync(O.sweep_brk.data(), brk6.p, n6 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+            HIPCHK(ctx, hipMemcpyAsync(O.sweep_steps.data(), st6.p, n6 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        }
+        if (n18) {
+            HIPCHK(ctx, hipMemcpyAsync(O.full_res.data(), res18.p, O.full_res.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+            HIPCHK(ctx, hipMemcpyAsync(O.full_brk.data(), brk18.p, n18 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+            HIPCHK(ctx, hipMemcpyAsync(O.full_steps.data(), st18.p, n18 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        }
+        if (nf) HIPCHK(ctx, hipMemcpyAsync(O.final_rows.data(), arr.p, nf * sizeof(ga_arrival), hipMemcpyDeviceToHost, st));
+        HIPCHK(ctx, hipStreamSynchronize(st));
+        float ms = 0.0f;
+        HIPCHK(ctx, hipEventElapsedTime(&ms, ev.e0, ev.e1));
+        kernel_ms += ms;
+        return GA_OK;
     }
 };
 
@@ -539,10 +720,110 @@ int ga_fan_run(ga_ctx *ctx, int atmo_id, const ga_prop_params *p, const double *
     return GA_OK;
 }
 
-int ga_eig_search(ga_ctx *ctx, int, const ga_prop_params *, const ga_eig_params *, const double *, int,
-                  ga_eigenray *, int, int *, double *, ga_run_stats *)
+int ga_eig_search(ga_ctx *ctx, int atmo_id, const ga_prop_params *p, const ga_eig_params *e,
+                  const double *rcvr_xy, int n_rcvr, ga_eigenray *out, int out_cap, int *n_out,
+                  double *rcvr_latency_ms, ga_run_stats *stats)
 {
-    return fail(ctx, GA_E_ARG, "ga_eig_search: not built yet");
+    auto t0 = std::chrono::steady_clock::now();
+    if (!ctx || !p || !e || !n_out || (n_rcvr > 0 && !rcvr_xy)) return fail(ctx, GA_E_ARG, "null argument");
+    *n_out = 0;
+    if (n_rcvr < 0) return fail(ctx, GA_E_ARG, "negative receiver count");
+    if (p->geometry != GA_GEOM_3D && p->geometry != GA_GEOM_SPH) return fail(ctx, GA_E_ARG, "bad geometry %d", p->geometry);
+    if (e->bnc_min < 0 || e->bnc_max < e->bnc_min) return fail(ctx, GA_E_ARG, "bad bounce range %d..%d", e->bnc_min, e->bnc_max);
+    if (e->iterations < 0) return fail(ctx, GA_E_ARG, "negative iteration limit");
+    if (!(e->incl_step_max > 0.0) || !(e->incl_step_min > 0.0))
+        return fail(ctx, GA_E_ARG, "inclination steps must be positive");
+    auto it = ctx->atmos.find(atmo_id);
+    if (it == ctx->atmos.end()) return fail(ctx, GA_E_ATMO, "unknown atmosphere id %d", atmo_id);
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const DevAtmo &a = it->second;
+
+    const bool sph = p->geometry == GA_GEOM_SPH;
+    EigGpu gpu;
+    gpu.ctx = ctx;
+    gpu.geom = p->geometry;
+    FanArgs &F = gpu.F;
+    int rc = GA_OK;
+    if (!build_medium(ctx, a, p, F.M, F.L, rc)) return rc;
+    EigSetup S;
+    S.geom = p->geometry;
+    S.r0 = F.M.r0;
+    S.src_ground = F.M.z0;
+    std::vector<double> rcvr(rcvr_xy, rcvr_xy + 2 * (size_t)std::max(n_rcvr, 0));
+    double c_arr;   // divisor of the arrival inclination
+    if (sph) {
+        // main.infraga.sph.cpp:906-909: degrees to radians, source raised to the ground
+        const double zsrc = std::max(p->src[2], F.M.z0);
+        S.src[0] = zsrc;
+        S.src[1] = p->src[0] * (PI / 180.0);
+        S.src[2] = p->src[1] * (PI / 180.0);
+        for (double &x : rcvr) x *= (PI / 180.0);
+        F.src0 = F.M.r0 + zsrc;
+        F.src1 = S.src[1];
+        F.src2 = S.src[2];
+        S.c_src = host_c_at(a, zsrc);
+        // atmo::c(src[0], ...) reads the source altitude as a radius, which
+        // clamps to the profile's bottom value (geoac.eigenray.sph.cpp:273)
+        c_arr = host_c_at(a, a.z.front());
+        // theta_grnd = 0.1 deg for a source on the ground (sph.cpp:966-979)
+        const double th_grnd = (zsrc - F.M.z0 < 0.01) ? 0.1 * (PI / 180.0) : -89.9 * (PI / 180.0);
+        S.th_start = std::max(e->incl_min * (PI / 180.0), th_grnd);
+    } else {
+        // src[2] = max(topo::z(src), src[2]) (main.infraga.3d.cpp:857)
+        F.src0 = p->src[0];
+        F.src1 = p->src[1];
+        F.src2 = std::max(p->src[2], F.M.zg);
+        S.src[0] = F.src0;
+        S.src[1] = F.src1;
+        S.src[2] = F.src2;
+        S.c_src = host_c_at(a, F.src2);
+        c_arr = S.c_src;
+        S.th_start = e->incl_min * (PI / 180.0);
+    }
+    F.c_src = c_arr;
+    F.freq = p->freq;
+    long long rows = (long long)(p->s_max / p->ds_min);
+    F.max_steps = (int)std::min<long long>(rows - 1, 2000000000LL);
+    F.lds = (size_t)a.n * NODE * sizeof(double) <= ctx->max_lds ? 1 : 0;
+
+    S.th_min = e->incl_min * (PI / 180.0);
+    S.th_max = e->incl_max * (PI / 180.0);
+    S.bnc_min = e->bnc_min;
+    S.bnc_max = e->bnc_max;
+    S.iterations = e->iterations;
+    S.damping = e->damping;
+    S.tolerance = e->tolerance;
+    S.az_err_lim = e->az_dev_lim;
+    S.dth_big = e->incl_step_max * (PI / 180.0);
+    S.dth_sml = e->incl_step_min * (PI / 180.0);
+    S.rng_max = p->rng_max;
+    S.c_grnd = host_c_at(a, F.M.z0);
+    S.spec_chunk = 64;
+
+    rc = gpu.init();
+    if (rc != GA_OK) return rc;
+    std::vector<ga_arrival> found;
+    std::vector<double> lat;
+    EigSearchStats es;
+    {
+        EigSearch search(S, rcvr.data(), n_rcvr);
+        rc = search.run([&](const EigBatch &B, EigBatchOut &O) { return gpu(B, O); }, found, lat, es);
+    }
+    if (rc != GA_OK) return rc;
+    *n_out = (int)found.size();
+    if (rcvr_latency_ms)
+        for (int i = 0; i < n_rcvr; ++i) rcvr_latency_ms[i] = lat[i];
+    if (stats) {
+        stats->ray_steps = es.ray_steps;
+        stats->kernel_ms = gpu.kernel_ms;
+        stats->n_rays = (int32_t)std::min<long long>(es.traces, 2147483647LL);
+        stats->n_launches = gpu.launches;
+        stats->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    if ((long long)found.size() > out_cap || (!out && !found.empty()))
+        return fail(ctx, GA_E_CAP, "need %zu eigenray slots, have %d", found.size(), out_cap);
+    for (size_t i = 0; i < found.size(); ++i) out[i] = found[i];
+    return GA_OK;
 }
 
 }  // extern "C"

@@ -40,12 +40,11 @@ struct FanArgs {
 // Launch state (set_initial: geoac.eqset.sph.strat.cpp:59-104,
 // geoac.eqset.3d.strat.cpp:58-105).
 // ---------------------------------------------------------------------------
+// th, ph: launch inclination and azimuth in radians (theta, phi of geoac).
 template <int G, int NEQ>
-GA_HD void init_ray(const FanArgs &A, const Medium &M, double incl, double az, double *y,
-                                         double &c0, double &nu0s, double &th, double &ph, int &cur)
+GA_HD void init_ray_rad(const FanArgs &A, const Medium &M, double th, double ph, double *y,
+                        double &c0, double &nu0s, int &cur)
 {
-    ph = PI / 2.0 - az * (PI / 180.0);
-    th = incl * (PI / 180.0);
     double src[3] = {A.src0, A.src1, A.src2};
     double c, u, v, rho;
     point_medium<G>(M, src, cur, c, u, v, rho);
@@ -80,6 +79,16 @@ GA_HD void init_ray(const FanArgs &A, const Medium &M, double incl, double az, d
             y[15 + i] = mp[i] * imsc - n0[i] * (imsc * imsc) * b;
         }
     }
+}
+
+// incl, az: degrees (inclination above horizontal, azimuth clockwise from north).
+template <int G, int NEQ>
+GA_HD void init_ray(const FanArgs &A, const Medium &M, double incl, double az, double *y,
+                    double &c0, double &nu0s, double &th, double &ph, int &cur)
+{
+    ph = PI / 2.0 - az * (PI / 180.0);
+    th = incl * (PI / 180.0);
+    init_ray_rad<G, NEQ>(A, M, th, ph, y, c0, nu0s, cur);
 }
 
 // break_check (sph: wraps longitude in place) / ground_check.
@@ -138,9 +147,9 @@ GA_HD double jacobian(const double *y, double c, double u, double v)
 
 // Arrival record (main.infraga.sph.cpp:523-544, main.infraga.3d.cpp:511-530).
 template <int G, int NEQ>
-GA_HD void eval_arrival(const FanArgs &A, const Medium &M, const double *y, int ray, int bnc,
-                                             double incl, double az, double tt, double att, double rmax,
-                                             double c0, double nu0s, double th, double ph, int &cur)
+GA_HD ga_arrival arrival_row(const FanArgs &A, const Medium &M, const double *y, int ray, int bnc,
+                             double incl, double az, double tt, double att, double rmax,
+                             double c0, double nu0s, double th, double ph, int &cur)
 {
     ga_arrival R;
     R.launch_idx = ray;
@@ -201,7 +210,16 @@ GA_HD void eval_arrival(const FanArgs &A, const Medium &M, const double *y, int 
         }
         R.tl = 20.0 * log10(sqrt(fabs(num / den)));
     }
-    A.out[(size_t)ray * (A.bounces + 1) + bnc] = R;
+    return R;
+}
+
+template <int G, int NEQ>
+GA_HD void eval_arrival(const FanArgs &A, const Medium &M, const double *y, int ray, int bnc,
+                        double incl, double az, double tt, double att, double rmax,
+                        double c0, double nu0s, double th, double ph, int &cur)
+{
+    A.out[(size_t)ray * (A.bounces + 1) + bnc] =
+        arrival_row<G, NEQ>(A, M, y, ray, bnc, incl, az, tt, att, rmax, c0, nu0s, th, ph, cur);
 }
 
 // Specular reflection off flat ground (set_refl with zero terrain slope:

@@ -11,7 +11,9 @@
 //  - an unknown key is an error (exit 2) instead of an interactive prompt;
 //  - write_rays=true writes the arrivals but not <id>.raypaths.dat (ray path
 //    sampling is not part of this engine); write_caustics, write_topo,
-//    topo_file and topo_use_BLw are rejected.
+//    topo_file and topo_use_BLw are rejected;
+//  - -eig_search writes <id>.arrivals.dat but not the per-eigenray path
+//    files <id>.eigenray-N.dat; verbose=true prints no trace-by-trace log.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -66,7 +68,7 @@ void usage()
 {
     std::cout << "Usage: " << GA_CLI_NAME << " [option] profile.met [parameters]\n"
               << "  -prop        ray paths for a fan of inclinations and azimuths\n"
-              << "  -eig_search  eigenrays between the source and a receiver" << (GA_CLI_GEOM == GA_GEOM_3D ? "" : " (3D only)") << "\n"
+              << "  -eig_search  eigenrays between the source and a receiver\n"
               << "Parameters use the key=value names of the reference infraga drivers.\n";
 }
 
@@ -79,6 +81,8 @@ std::string strip_ext(char *path)
     }
     return std::string(path);
 }
+
+double profile_bottom(const char *path);
 
 int run_prop(int argc, char **argv)
 {
@@ -155,16 +159,7 @@ int run_prop(int argc, char **argv)
         return die(ctx, "cannot load the atmosphere");
 
     // ground: the profile bottom unless z_grnd is given (atmo_io set_limits + driver)
-    std::ifstream probe(argv[2]);
-    double z_bottom = 0.0;
-    {
-        std::string line;
-        while (std::getline(probe, line)) {
-            if (line.find('#') == 0) continue;
-            if (std::sscanf(line.c_str(), "%lf", &z_bottom) == 1) break;
-        }
-    }
-    double z0 = std::isnan(z_grnd) ? z_bottom : z_grnd;
+    double z0 = std::isnan(z_grnd) ? profile_bottom(argv[2]) : z_grnd;
     p.z_grnd = z0;
     p.bounces = bounces;
     p.freq = freq;
@@ -242,12 +237,169 @@ int run_prop(int argc, char **argv)
     return 0;
 }
 
+// The profile's lowest altitude (the default ground height).
+double profile_bottom(const char *path)
+{
+    std::ifstream probe(path);
+    double z_bottom = 0.0;
+    std::string line;
+    while (std::getline(probe, line)) {
+        if (line.find('#') == 0) continue;
+        if (std::sscanf(line.c_str(), "%lf", &z_bottom) == 1) break;
+    }
+    return z_bottom;
+}
+
+// -eig_search (main.infraga.3d.cpp:773-953, main.infraga.sph.cpp:786-996).
+int run_eig_search(int argc, char **argv)
+{
+    const bool sph = GA_CLI_GEOM == GA_GEOM_SPH;
+    // defaults: 3D src (0, 0, 0), rcvr (-250, 0); sph src 30 N 0 E, rcvr 30 N 2.5 W
+    double s0 = sph ? 30.0 : 0.0, s1 = 0.0, z_src = 0.0;
+    double r0 = sph ? 30.0 : -250.0, r1 = sph ? -2.5 : 0.0;
+    bool reverse_winds = false, custom_id = false;
+    double freq = 0.1;
+    const char *prof_format = "zTuvdp";
+    const char *output_id = nullptr;
+    ga_prop_params p;
+    ga_prop_params_default(&p, GA_CLI_GEOM);
+    ga_eig_params e;
+    ga_eig_params_default(&e);
+    double z_grnd = NAN;
+
+    for (int i = 3; i < argc; i++) {
+        const char *a = argv[i], *v = nullptr;
+        if (key2(a, "incl_min=", "min_incl=", v)) e.incl_min = std::atof(v);
+        else if (key2(a, "incl_max=", "max_incl=", v)) e.incl_max = std::atof(v);
+        else if (key2(a, "bnc_min=", "min_bnc=", v)) e.bnc_min = std::atoi(v);
+        else if (key2(a, "bnc_max=", "max_bnc=", v)) e.bnc_max = std::atoi(v);
+        else if (key(a, "bounces=", v)) e.bnc_min = e.bnc_max = std::atoi(v);
+        else if (key2(a, "src_alt=", "alt_src=", v)) z_src = std::atof(v);
+        else if (sph && key2(a, "src_lat=", "lat_src=", v)) s0 = std::atof(v);
+        else if (sph && key2(a, "src_lon=", "lon_src=", v)) s1 = std::atof(v);
+        else if (sph && key2(a, "rcvr_lat=", "lat_rcvr=", v)) r0 = std::atof(v);
+        else if (sph && key2(a, "rcvr_lon=", "lon_rcvr=", v)) r1 = std::atof(v);
+        else if (!sph && key2(a, "src_x=", "x_src=", v)) s0 = std::atof(v);
+        else if (!sph && key2(a, "src_y=", "y_src=", v)) s1 = std::atof(v);
+        else if (!sph && key2(a, "rcvr_x=", "x_rcvr=", v)) r0 = std::atof(v);
+        else if (!sph && key2(a, "rcvr_y=", "y_rcvr=", v)) r1 = std::atof(v);
+        else if (key(a, "verbose=", v)) { /* trace-by-trace log not produced */ }
+        else if (key(a, "iterations=", v)) e.iterations = (int)std::atof(v);
+        else if (key(a, "damping=", v)) e.damping = std::atof(v);
+        else if (key(a, "tolerance=", v)) e.tolerance = std::atof(v);
+        else if (key(a, "az_dev_lim=", v)) e.az_dev_lim = std::atof(v);
+        else if (key2(a, "incl_step_min=", "min_incl_step=", v)) e.incl_step_min = std::atof(v);
+        else if (key2(a, "incl_step_max=", "max_incl_step=", v)) e.incl_step_max = std::atof(v);
+        else if (key(a, "freq=", v)) freq = std::atof(v);
+        else if (key(a, "abs_coeff=", v)) p.abs_coeff = std::max(0.0, std::atof(v));
+        else if (key2(a, "max_alt=", "alt_max=", v)) p.alt_max = std::atof(v);
+        else if (key2(a, "max_rng=", "rng_max=", v)) p.rng_max = std::atof(v);
+        else if (sph && key2(a, "max_time=", "time_max=", v)) p.time_max = std::atof(v);
+        else if (sph && key2(a, "min_lat=", "lat_min=", v)) p.bounds[0] = std::atof(v);
+        else if (sph && key2(a, "max_lat=", "lat_max=", v)) p.bounds[1] = std::atof(v);
+        else if (sph && key2(a, "min_lon=", "lon_min=", v)) p.bounds[2] = std::atof(v);
+        else if (sph && key2(a, "max_lon=", "lon_max=", v)) p.bounds[3] = std::atof(v);
+        else if (!sph && key2(a, "min_x=", "x_min=", v)) p.bounds[0] = std::atof(v);
+        else if (!sph && key2(a, "max_x=", "x_max=", v)) p.bounds[1] = std::atof(v);
+        else if (!sph && key2(a, "min_y=", "y_min=", v)) p.bounds[2] = std::atof(v);
+        else if (!sph && key2(a, "max_y=", "y_max=", v)) p.bounds[3] = std::atof(v);
+        else if (key2(a, "min_ds=", "ds_min=", v)) p.ds_min = std::atof(v);
+        else if (key2(a, "max_ds=", "ds_max=", v)) p.ds_max = std::atof(v);
+        else if (key2(a, "max_s=", "s_max=", v)) p.s_max = std::atof(v);
+        else if (key(a, "write_atmo=", v)) { /* profile dump: not produced */ }
+        else if (key(a, "prof_format=", v)) prof_format = v;
+        else if (key(a, "reverse_winds=", v)) reverse_winds = str2bool(v);
+        else if (key(a, "output_id=", v)) { custom_id = true; output_id = v; }
+        else if (key(a, "z_grnd=", v)) z_grnd = std::atof(v);
+        else if (key(a, "topo_file=", v) || key(a, "topo_use_BLw=", v)) {
+            if (std::strcmp(v, "false") != 0 && std::strcmp(v, "False") != 0 && std::strcmp(v, "0") != 0) {
+                std::cerr << GA_CLI_NAME << ": " << a << " is not supported by this engine\n";
+                return 2;
+            }
+        } else {
+            std::cerr << GA_CLI_NAME << ": unrecognized parameter entry: " << a << '\n';
+            return 2;
+        }
+    }
+
+    ga_ctx *ctx = nullptr;
+    if (ga_ctx_create(&ctx, 0) != GA_OK) return die(nullptr, "no usable GPU");
+    int aid = 0;
+    if (ga_atmo_load_file(ctx, argv[2], prof_format, reverse_winds ? 1 : 0, &aid) != GA_OK)
+        return die(ctx, "cannot load the atmosphere");
+    const double z0 = std::isnan(z_grnd) ? profile_bottom(argv[2]) : z_grnd;
+    p.z_grnd = z0;
+    p.freq = freq;
+    z_src = std::max(z_src, z0);
+    p.src[0] = s0;
+    p.src[1] = s1;
+    p.src[2] = z_src;
+    const double rcvr[2] = {r0, r1};
+    std::vector<ga_eigenray> eig(64);
+    int n_out = 0;
+    double lat_ms = 0.0;
+    ga_run_stats st;
+    int rc = ga_eig_search(ctx, aid, &p, &e, rcvr, 1, eig.data(), (int)eig.size(), &n_out, &lat_ms, &st);
+    if (rc == GA_E_CAP) {
+        eig.resize(n_out);
+        rc = ga_eig_search(ctx, aid, &p, &e, rcvr, 1, eig.data(), (int)eig.size(), &n_out, &lat_ms, &st);
+    }
+    if (rc != GA_OK) return die(ctx, "eigenray search failed");
+
+    std::string id = custom_id ? std::string(output_id) : strip_ext(argv[2]);
+    std::ofstream out(id + ".arrivals.dat");
+    // header (main.infraga.3d.cpp:897-930, main.infraga.sph.cpp:931-960)
+    out << "# " << GA_CLI_NAME << " -eig_search summary:" << '\n';
+    out << "# " << '\t' << "profile: " << argv[2] << '\n';
+    if (sph) {
+        out << "# " << '\t' << "source location (lat, lon, alt): " << (s0 * (Pi / 180.0)) * (180.0 / Pi) << ", "
+            << (s1 * (Pi / 180.0)) * (180.0 / Pi) << ", " << z_src << '\n';
+        out << "# " << '\t' << "receiver location (lat, lon, alt): " << (r0 * (Pi / 180.0)) * (180.0 / Pi) << ", "
+            << (r1 * (Pi / 180.0)) * (180.0 / Pi) << ", " << (p.r_earth + z0) - p.r_earth << '\n';
+    } else {
+        out << "# " << '\t' << "source location: " << s0 << ", " << s1 << ", " << z_src << '\n';
+        out << "# " << '\t' << "receiver location: " << r0 << ", " << r1 << ", " << z0 << '\n';
+    }
+    out << "# " << '\t' << "inclination range: " << (e.incl_min * (Pi / 180.0)) * (180.0 / Pi) << ", "
+        << (e.incl_max * (Pi / 180.0)) * (180.0 / Pi) << '\n';
+    out << "# " << '\t' << "bounces: " << e.bnc_min << ", " << e.bnc_max << '\n';
+    out << "#" << '\t' << "ground elevation: " << z0 << '\n';
+    out << "# " << '\t' << "damping: " << e.damping << '\n';
+    out << "# " << '\t' << "frequency: " << freq << '\n';
+    out << "# " << '\t' << "S&B atten coeff: " << p.abs_coeff << '\n';
+    if (!sph) out << '\n';
+    out << "# incl [deg]" << '\t' << "az [deg]" << '\t' << "n_b";
+    if (sph) out << '\t' << "lat_0 [deg]" << '\t' << "lon_0 [deg]";
+    else out << '\t' << "x_0 [km]" << '\t' << "y_0 [km]";
+    out << '\t' << "time [s]" << '\t' << "cel [km/s]" << '\t' << "turning ht [km]";
+    if (sph) out << '\t' << "inclination [deg]" << '\t' << "back azimuth [deg]";
+    else out << '\t' << "arrival incl [deg]" << '\t' << "back az [deg]";
+    out << '\t' << "trans. coeff. [dB]" << '\t' << "absorption [dB]" << '\n';
+    // rows (geoac.eigenray.3d.cpp:302-315): setprecision(8) stays in effect
+    for (int i = 0; i < n_out; ++i) {
+        const ga_eigenray &A = eig[i];
+        out << std::setprecision(8) << A.incl;
+        out << '\t' << std::setprecision(8) << A.az;
+        out << '\t' << A.n_bounce;
+        out << '\t' << A.pos0 << '\t' << A.pos1 << '\t' << A.time << '\t' << A.cel << '\t' << A.turn_ht;
+        out << '\t' << A.incl_arr << '\t' << A.back_az << '\t' << A.tl << '\t' << A.absorption << '\n';
+        std::cout << '\t' << "Eigenray identified:" << '\t' << "theta, phi = " << std::setprecision(8) << A.incl << ", "
+                  << A.az << " degrees." << '\n';
+    }
+    out.close();
+    std::cout << GA_CLI_NAME << ": " << n_out << " eigenray(s), " << st.n_rays << " ray traces, " << st.ray_steps
+              << " ray-steps, " << st.kernel_ms << " ms on the GPU\n";
+    ga_ctx_destroy(ctx);
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char **argv)
 {
     if (argc < 3) { usage(); return argc < 2 ? 0 : 2; }
     if (std::strncmp(argv[1], "-prop", 5) == 0) return run_prop(argc, argv);
+    if (std::strncmp(argv[1], "-eig_search", 11) == 0) return run_eig_search(argc, argv);
     std::cerr << GA_CLI_NAME << ": option " << argv[1] << " is not supported by this engine\n";
     usage();
     return 2;

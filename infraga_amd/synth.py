@@ -71,6 +71,16 @@ def to_engine_units(z, T, u, v, rho):
             v_txt / 1000.0, np.ascontiguousarray(rho, dtype=np.float64))
 
 
+def from_zTuvdp(cols):
+    """Engine units from zTuvdp columns (z km, T K, u/v m/s, rho g/cm^3,
+    p mbar) by the reference loader's conversion (atmo_io.*.strat.cpp):
+    c = sqrt(0.1 * 1.4 * p / rho) / 1000 km/s, winds / 1000."""
+    d = np.asarray(cols, dtype=np.float64)
+    z, u, v, rho, p = d[:, 0], d[:, 2], d[:, 3], d[:, 4], d[:, 5]
+    c = np.sqrt(0.1 * 1.4 * p / rho) / 1000.0
+    return (np.ascontiguousarray(z), c, u / 1000.0, v / 1000.0, np.ascontiguousarray(rho))
+
+
 def write_zcuvd(path, z, T, u, v, rho):
     c = sound_speed(T)
     with open(path, "w") as f:

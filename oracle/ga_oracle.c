@@ -1153,6 +1153,9 @@ typedef struct {
     long long steps;
 } eig_ctx;
 
+/* Work counters of the last go_eig_search_3d call (see go_eig_counts). */
+static long long eig_cnt[GO_EIG_NCOUNT];
+
 static double mod_dth(const eig_ctx *E, double dr, double dr_dtheta)   /* eigenray.3d.cpp:32 */
 {
     return E->dth_big - (E->dth_big - E->dth_sml) * exp(-1.0 / 2.0 * sq(dr / dr_dtheta));
@@ -1167,6 +1170,8 @@ static int trace_legs(eig_ctx *E, const go_params *p, go_src *s, double *rows, i
     E->steps += k;
     if (!*brk) {
         for (int b = 1; b <= nb; b++) {
+            /* the reference reads row k-2 < 0 here (undefined); defined as a break */
+            if (k < 2) { *brk = 1; break; }
             go_set_refl(E->a, p, s, rows + 18 * (k - 2), rows + 18 * (k - 1), rows);
             k = go_prop_rk4(E->a, p, s, rows, length, brk);
             E->steps += k;
@@ -1190,6 +1195,7 @@ static int est_eigenray(eig_ctx *E, const double src[3], const double rcvr[2], d
     go_src s;
     s.src[0] = src[0]; s.src[1] = src[1]; s.src[2] = src[2];
     s.phi = rcvr_az;
+    eig_cnt[GO_EIG_EST]++;
     while (fabs(dph * (180.0 / GO_PI)) > az_err_lim && iterations < 5) {
         r = rcvr_rng;
         r_prev = rcvr_rng;
@@ -1197,6 +1203,9 @@ static int est_eigenray(eig_ctx *E, const double src[3], const double rcvr[2], d
             if (th + dth >= th_max) th_max_reached = 1;
             s.theta = th;
             k = trace_legs(E, p, &s, rows, length, bncs, &brk, 0);
+            eig_cnt[GO_EIG_SWEEP_TRACES]++;
+            if (iterations >= 1) eig_cnt[GO_EIG_SWEEP_CORR]++;
+            if (iterations >= 2) eig_cnt[GO_EIG_SWEEP_ADAPT]++;
             double *yk = rows + 18 * k;
             r = sqrt(sq(yk[0] - src[0]) + sq(yk[1] - src[1]));
             if (brk) r = p->rng_max;
@@ -1243,6 +1252,8 @@ static int find_eigenray(eig_ctx *E, const double src[3], const double rcvr[2], 
         go_set_initial(a, p, &s, rows);
         k = go_prop_rk4(a, p, &s, rows, length, &brk);
         E->steps += k;
+        eig_cnt[GO_EIG_LM_TRACES]++;
+        if (!brk && bnc_cnt > 0 && k < 2) brk = 1;   /* row k-2 < 0: defined as a break */
         if (brk) {
             s.theta -= dth * step_sc; s.phi -= dph * step_sc; step_sc /= 2.0;
         }
@@ -1271,7 +1282,9 @@ static int find_eigenray(eig_ctx *E, const double src[3], const double rcvr[2], 
                     att += go_atten_seg(a, rows + 18 * (m - 1), rows + 18 * m, freq);
                     zmax = zmax > rows[18 * m + 2] ? zmax : rows[18 * m + 2];
                 }
+                eig_cnt[GO_EIG_FINAL_TRACES]++;
                 for (int b = 1; b <= bnc_cnt; b++) {
+                    if (k < 2) break;   /* row k-2 < 0: no further reflection */
                     go_set_refl(a, p, &s, rows + 18 * (k - 2), rows + 18 * (k - 1), rows);
                     k = go_prop_rk4(a, p, &s, rows, length, &brk);
                     E->steps += k;
@@ -1338,6 +1351,7 @@ int go_eig_search_3d(go_atmo *a, const go_params *p, const double src_in[3],
     E.damping = damping;
     E.tolerance = tolerance;
     E.steps = 0;
+    memset(eig_cnt, 0, sizeof(eig_cnt));
     double src[3] = {src_in[0], src_in[1], src_in[2] > ground(a) ? src_in[2] : ground(a)};
     double th_min = th_min_deg * (GO_PI / 180.0), th_max = th_max_deg * (GO_PI / 180.0);
     int cnt = 0;
@@ -1347,6 +1361,7 @@ int go_eig_search_3d(go_atmo *a, const go_params *p, const double src_in[3],
             int ok = est_eigenray(&E, src, rcvr, th_start, th_max, &th_est, &ph_est, &th_next, nb, az_err_lim);
             if (ok) {
                 go_arrival A;
+                eig_cnt[GO_EIG_FINDS]++;
                 if (find_eigenray(&E, src, rcvr, th_est, ph_est, freq, nb, iterations, &A)) {
                     if (cnt < out_cap) out[cnt] = A;
                     cnt++;
@@ -1356,7 +1371,225 @@ int go_eig_search_3d(go_atmo *a, const go_params *p, const double src_in[3],
         }
     }
     if (ray_steps) *ray_steps = E.steps;
+    eig_cnt[GO_EIG_FOUND] = cnt;
     return cnt;
+}
+
+/* ======================================================================
+ * Eigenray search, spherical: geoac.eigenray.sph.cpp:36-380 and the driver
+ * loop of main.infraga.sph.cpp:962-989.  src = (z [km], lat, lon [rad]) and
+ * rcvr = (lat, lon [rad]) as run_eig_search holds them.
+ * ==================================================================== */
+static double go_bearing(double lat1, double lon1, double lat2, double lon2)   /* globe.cpp:16-21 */
+{
+    double term1 = sin(lon2 - lon1);
+    double term2 = cos(lat1) * tan(lat2) - sin(lat1) * cos(lon2 - lon1);
+    return atan2(term1, term2);
+}
+
+static int est_eigenray_sph(eig_ctx *E, const double src[3], const double rcvr[2], double th_min, double th_max,
+                            double *th_est, double *ph_est, double *th_next, int bncs, double az_err_lim)
+{
+    const go_params *p = &E->p6;
+    const double r0 = E->a->r0;
+    double rcvr_rng = go_gc_dist(r0, src[1], src[2], rcvr[0], rcvr[1]);
+    double rcvr_az = go_bearing(src[1], src[2], rcvr[0], rcvr[1]);
+    int iterations = 0, k, length = (int)(p->s_max / p->ds_min);
+    double r, r_prev, dth = E->dth_big, dph = 100.0;
+    int brk, th_max_reached = 0;
+    double *rows = get_rows(length);
+    go_src s;
+    s.src[0] = r0 + src[0]; s.src[1] = src[1]; s.src[2] = src[2];   /* set_initial, eqset.sph :59-62 */
+    s.phi = GO_PI / 2.0 - rcvr_az;
+    eig_cnt[GO_EIG_EST]++;
+    while (fabs(dph * (180.0 / GO_PI)) > az_err_lim && iterations < 5) {
+        r = rcvr_rng;
+        r_prev = rcvr_rng;
+        for (double th = th_min; th < th_max; th += dth) {          /* sph: strict bound */
+            if (th + dth >= th_max) th_max_reached = 1;
+            s.theta = th;
+            k = trace_legs(E, p, &s, rows, length, bncs, &brk, 0);
+            eig_cnt[GO_EIG_SWEEP_TRACES]++;
+            if (iterations >= 1) eig_cnt[GO_EIG_SWEEP_CORR]++;
+            if (iterations >= 3) eig_cnt[GO_EIG_SWEEP_ADAPT]++;
+            double *yk = rows + 18 * k;
+            r = go_gc_dist(r0, src[1], src[2], yk[1], yk[2]);
+            if (brk) r = p->rng_max;
+            if (((r - rcvr_rng) * (r_prev - rcvr_rng) <= 0.0) && (r < p->rng_max) && (r_prev < p->rng_max) && (th > th_min)) {
+                if (iterations == 0) *th_next = s.theta;
+                dph = go_bearing(src[1], src[2], rcvr[0], rcvr[1]) * (180.0 / GO_PI);
+                dph -= go_bearing(src[1], src[2], yk[1], yk[2]) * (180.0 / GO_PI);
+                while (dph > 180.0) dph -= 360.0;
+                while (dph < -180.0) dph += 360.0;
+                if (fabs(dph) < az_err_lim) {
+                    *th_est = s.theta - dth;
+                    *ph_est = s.phi;
+                    return 1;
+                }
+                s.phi -= dph * (GO_PI / 180.0) * 0.9;
+                th_min = (s.theta - 5.0 * (GO_PI / 180.0)) > th_min ? (s.theta - 5.0 * (GO_PI / 180.0)) : th_min;
+                iterations++;
+                break;
+            }
+            if (iterations >= 3) dth = mod_dth(E, r - rcvr_rng, (r - r_prev) / (2.0 * dth));   /* sph: >= 3 */
+            r_prev = r;
+        }
+        if (th_max_reached) { *th_next = th_max; break; }
+    }
+    return 0;
+}
+
+static int find_eigenray_sph(eig_ctx *E, const double src[3], const double rcvr[2], double th_est, double ph_est,
+                             double freq, int bnc_cnt, int iterate_limit, go_arrival *A)
+{
+    const go_params *p = &E->p18;
+    go_atmo *a = E->a;
+    const double r0 = a->r0;
+    int brk, k, length = (int)(p->s_max / p->ds_min);
+    double dr, dr_prev = 10000.0, step_sc = 1.0;
+    long double r_grnd, c_src, c_grnd, dzg_dlat, dzg_dlon, ds_norm, ds_dth, ds_dph, dlat, dlon;
+    long double dlat_dth, dlon_dth, dlat_dph, dlon_dph;
+    long double det, dth = 0.0L, dph = 0.0L;     /* uninitialised in the reference */
+    double *rows = get_rows(length);
+    go_src s;
+    s.src[0] = r0 + src[0]; s.src[1] = src[1]; s.src[2] = src[2];
+    s.theta = th_est;
+    s.phi = ph_est;
+    for (int n = 0; n <= iterate_limit; n++) {
+        if (n == iterate_limit) break;
+        k = trace_legs(E, p, &s, rows, length, bnc_cnt, &brk, 0);    /* eigenray.sph.cpp:178-201 */
+        eig_cnt[GO_EIG_LM_TRACES]++;
+        if (brk) { s.theta -= dth * step_sc; s.phi -= dph * step_sc; step_sc /= 2.0; }
+        if (!brk) {
+            double *yk = rows + 18 * k;
+            dr = go_gc_dist(r0, yk[1], yk[2], rcvr[0], rcvr[1]);
+            if (dr < E->tolerance) {
+                /* accept: re-trace with per-step bookkeeping (sph.cpp:219-271) */
+                double att = 0.0, tt = 0.0, rmax = 0.0;
+                go_set_initial(a, p, &s, rows);
+                k = go_prop_rk4(a, p, &s, rows, length, &brk);
+                E->steps += k;
+                eig_cnt[GO_EIG_FINAL_TRACES]++;
+                for (int m = 1; m < k; m++) {
+                    tt += go_travel_time_seg(a, rows + 18 * (m - 1), rows + 18 * m);
+                    att += go_atten_seg(a, rows + 18 * (m - 1), rows + 18 * m, freq);
+                    rmax = rmax > rows[18 * m] - r0 ? rmax : rows[18 * m] - r0;
+                }
+                for (int b = 1; b <= bnc_cnt; b++) {
+                    if (k < 2) break;   /* row k-2 < 0: no further reflection */
+                    go_set_refl(a, p, &s, rows + 18 * (k - 2), rows + 18 * (k - 1), rows);
+                    k = go_prop_rk4(a, p, &s, rows, length, &brk);
+                    E->steps += k;
+                    for (int m = 1; m < k; m++) {
+                        tt += go_travel_time_seg(a, rows + 18 * (m - 1), rows + 18 * m);
+                        att += go_atten_seg(a, rows + 18 * (m - 1), rows + 18 * m, freq);
+                        rmax = rmax > rows[18 * m] - r0 ? rmax : rows[18 * m] - r0;
+                    }
+                }
+                yk = rows + 18 * k;
+                memset(A, 0, sizeof(*A));
+                A->n_bounce = bnc_cnt;
+                A->incl_launch = s.theta * (180.0 / GO_PI);
+                A->az_launch = 90.0 - s.phi * (180.0 / GO_PI);
+                A->pos0 = yk[1] * 180.0 / GO_PI;
+                A->pos1 = yk[2] * 180.0 / GO_PI;
+                A->time = tt;
+                A->cel = go_gc_dist(r0, yk[1], yk[2], src[1], src[2]) / tt;
+                A->turn_ht = rmax;
+                /* atmo::c(src[0], ...) takes the source altitude as a radius:
+                 * the profile's bottom value (sph.cpp:273) */
+                A->incl_arr = -asin(go_c(a, yk[0], yk[1], yk[2]) / go_c(a, src[0], src[1], src[2]) * yk[3]) * 180.0 / GO_PI;
+                double baz = 90.0 - atan2(-yk[4], -yk[5]) * (180.0 / GO_PI);
+                while (baz < -180.0) baz += 360.0;
+                while (baz > 180.0) baz -= 360.0;
+                A->back_az = baz;
+                A->tl = 20.0 * log10(go_amp(a, &s, yk));
+                A->absorption = -2.0 * att;
+                return 1;
+            } else if (n > 0 && dr > dr_prev) {
+                s.theta -= dth * step_sc; s.phi -= dph * step_sc; step_sc /= 2.0;
+            } else {
+                step_sc = (1.0 < step_sc * 1.25) ? 1.0 : step_sc * 1.25;
+                dlat = rcvr[0] - yk[1];
+                dlon = rcvr[1] - yk[2];
+                r_grnd = ground(a);                                 /* topo::z: r0 + z0 */
+                c_grnd = go_c(a, r_grnd, yk[1], yk[2]);
+                c_src = go_c(a, src[0] + r0, src[1], src[2]);
+                dzg_dlat = 0.0 / r_grnd;                            /* flat ground: topo::dz = 0 */
+                dzg_dlon = 0.0 / (r_grnd * cos(yk[1]));
+                ds_norm = yk[3] - dzg_dlat * yk[4] - dzg_dlon * yk[5];
+                ds_dth = -c_src / c_grnd * (yk[6] - dzg_dlat * r_grnd * yk[7] - dzg_dlon * (r_grnd * cos(yk[1])) * yk[8]) / ds_norm;
+                ds_dph = -c_src / c_grnd * (yk[12] - dzg_dlat * r_grnd * yk[13] - dzg_dlon * (r_grnd * cos(yk[1])) * yk[14]) / ds_norm;
+                dlat_dth = yk[7] + yk[4] / r_grnd * ds_dth;
+                dlon_dth = yk[8] + yk[5] / (r_grnd * cos(yk[1])) * ds_dth;
+                dlat_dph = yk[13] + yk[4] / r_grnd * ds_dph;
+                dlon_dph = yk[14] + yk[5] / (r_grnd * cos(yk[1])) * ds_dph;
+                /* sph.cpp:351: (1 + damping)^2 multiplies the first product only */
+                det = pow(1.0 + E->damping, 2) * dlat_dth * dlon_dph - dlat_dph * dlon_dth;
+                dth = ((1.0 + E->damping) * dlon_dph * dlat - dlat_dph * dlon) / det;
+                dph = ((1.0 + E->damping) * dlat_dth * dlon - dlon_dth * dlat) / det;
+                s.theta += dth * step_sc;
+                s.phi += dph * step_sc;
+                dr_prev = dr;
+            }
+        }
+        if (sqrtl(dth * dth + dph * dph) * step_sc < 1.0e-12) break;
+    }
+    return 0;
+}
+
+int go_eig_search_sph(go_atmo *a, const go_params *p, const double src_in[3],
+                      const double rcvr_in[2], double th_min_deg, double th_max_deg,
+                      int bnc_min, int bnc_max, int iterations, double damping,
+                      double tolerance, double az_err_lim, double dth_big_deg,
+                      double dth_sml_deg, double freq,
+                      go_arrival *out, int out_cap, long long *ray_steps)
+{
+    eig_ctx E;
+    E.a = a;
+    E.p6 = *p; E.p6.calc_amp = 0; E.p6.eq_cnt = 6;
+    E.p18 = *p; E.p18.calc_amp = 1; E.p18.eq_cnt = 18;
+    E.dth_big = dth_big_deg * (GO_PI / 180.0);
+    E.dth_sml = dth_sml_deg * (GO_PI / 180.0);
+    E.damping = damping;
+    E.tolerance = tolerance;
+    E.steps = 0;
+    memset(eig_cnt, 0, sizeof(eig_cnt));
+    /* main.infraga.sph.cpp:906-909 */
+    double src[3], rcvr[2];
+    src[1] = src_in[0] * (GO_PI / 180.0);
+    src[2] = src_in[1] * (GO_PI / 180.0);
+    rcvr[0] = rcvr_in[0] * (GO_PI / 180.0);
+    rcvr[1] = rcvr_in[1] * (GO_PI / 180.0);
+    src[0] = src_in[2] > ground(a) - a->r0 ? src_in[2] : ground(a) - a->r0;
+    double th_min = th_min_deg * (GO_PI / 180.0), th_max = th_max_deg * (GO_PI / 180.0);
+    /* ground source: theta_grnd = 0.1 deg (sph.cpp:966-979) */
+    double th_grnd = (src[0] - (ground(a) - a->r0) < 0.01) ? 0.1 * (GO_PI / 180.0) : -89.9 * (GO_PI / 180.0);
+    int cnt = 0;
+    for (int nb = bnc_min; nb <= bnc_max; nb++) {
+        double th_start = th_min > th_grnd ? th_min : th_grnd;
+        double th_next = th_start, th_est = 0, ph_est = 0;
+        while (th_start < th_max) {
+            int ok = est_eigenray_sph(&E, src, rcvr, th_start, th_max, &th_est, &ph_est, &th_next, nb, az_err_lim);
+            if (ok) {
+                go_arrival A;
+                eig_cnt[GO_EIG_FINDS]++;
+                if (find_eigenray_sph(&E, src, rcvr, th_est, ph_est, freq, nb, iterations, &A)) {
+                    if (cnt < out_cap) out[cnt] = A;
+                    cnt++;
+                }
+            }
+            th_start = th_next;
+        }
+    }
+    if (ray_steps) *ray_steps = E.steps;
+    eig_cnt[GO_EIG_FOUND] = cnt;
+    return cnt;
+}
+
+void go_eig_counts(long long out[GO_EIG_NCOUNT])
+{
+    for (int i = 0; i < GO_EIG_NCOUNT; i++) out[i] = eig_cnt[i];
 }
 
 #endif /* GO_OPCOUNT */

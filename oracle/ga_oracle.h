@@ -156,6 +156,21 @@ int    go_eig_search_3d(go_atmo *a, const go_params *p, const double src[3],
                         double dth_sml_deg, double freq,
                         go_arrival *out, int out_cap, long long *ray_steps);
 
+/* Eigenray search for one receiver, spherical geometry
+ * (main.infraga.sph.cpp:962-989, geoac.eigenray.sph.cpp).  src = (lat [deg],
+ * lon [deg], alt [km]), rcvr = (lat, lon) [deg]. */
+int    go_eig_search_sph(go_atmo *a, const go_params *p, const double src[3],
+                         const double rcvr[2], double th_min_deg, double th_max_deg,
+                         int bnc_min, int bnc_max, int iterations, double damping,
+                         double tolerance, double az_err_lim, double dth_big_deg,
+                         double dth_sml_deg, double freq,
+                         go_arrival *out, int out_cap, long long *ray_steps);
+
+/* Work counters of the last go_eig_search_3d / go_eig_search_sph call. */
+enum { GO_EIG_EST, GO_EIG_SWEEP_TRACES, GO_EIG_SWEEP_CORR, GO_EIG_SWEEP_ADAPT, GO_EIG_LM_TRACES,
+       GO_EIG_FINAL_TRACES, GO_EIG_FINDS, GO_EIG_FOUND, GO_EIG_NCOUNT };
+void   go_eig_counts(long long out[GO_EIG_NCOUNT]);
+
 /* ---- opaque-handle helpers for test bindings ------------------------- */
 void  *go_h_atmo(int geom, const double *z, const double *c, const double *u,
                  const double *v, const double *rho, int n);

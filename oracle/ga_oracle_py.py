@@ -105,6 +105,18 @@ class Oracle:
                                        C.c_double, C.c_double, C.c_double, C.POINTER(Arrival), C.c_int,
                                        C.POINTER(C.c_longlong)]
         L.go_eig_search_3d.restype = C.c_int
+        L.go_eig_search_sph.argtypes = L.go_eig_search_3d.argtypes
+        L.go_eig_search_sph.restype = C.c_int
+        L.go_eig_counts.argtypes = [C.POINTER(C.c_longlong)]
+
+    EIG_COUNT_NAMES = ("est_calls", "sweep_traces", "sweep_traces_corrected", "sweep_traces_adaptive",
+                       "lm_traces", "final_traces", "finds", "found")
+
+    def eig_counts(self):
+        """Work counters of the last eigenray search."""
+        buf = (C.c_longlong * len(self.EIG_COUNT_NAMES))()
+        self.lib.go_eig_counts(buf)
+        return dict(zip(self.EIG_COUNT_NAMES, list(buf)))
 
 
 class Src(C.Structure):
@@ -157,9 +169,10 @@ class Medium:
         cap = 64
         out = (Arrival * cap)()
         steps = C.c_longlong(0)
-        n = self.o.lib.go_eig_search_3d(self.h, C.byref(params), _ptr(s), _ptr(r), incl_min, incl_max,
-                                        bnc_min, bnc_max, iterations, damping, tolerance, az_dev_lim,
-                                        incl_step_max, incl_step_min, freq, out, cap, C.byref(steps))
+        fn = self.o.lib.go_eig_search_sph if self.geom == GEOM_SPH else self.o.lib.go_eig_search_3d
+        n = fn(self.h, C.byref(params), _ptr(s), _ptr(r), incl_min, incl_max,
+               bnc_min, bnc_max, iterations, damping, tolerance, az_dev_lim,
+               incl_step_max, incl_step_min, freq, out, cap, C.byref(steps))
         return arrivals_to_array(out, min(n, cap)), steps.value
 
 

@@ -31,6 +31,19 @@ def pert_units():
 
 
 @pytest.fixture(scope="session")
+def toyatmo_cols():
+    """The reference's examples/ToyAtmo.met (zTuvdp), from the committed fixture."""
+    import numpy as np
+    return np.load(os.path.join(ROOT, "tests", "golden", "toyatmo.npz"))["zTuvdp"]
+
+
+@pytest.fixture(scope="session")
+def toyatmo_units(toyatmo_cols):
+    from infraga_amd import synth
+    return synth.from_zTuvdp(toyatmo_cols)
+
+
+@pytest.fixture(scope="session")
 def engine():
     from infraga_amd.engine import Engine
     e = Engine(0)

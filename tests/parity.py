@@ -23,11 +23,48 @@ def keyed(arr):
     return {(int(a["launch_idx"]), int(a["n_bounce"])): a for a in arr}
 
 
-def compare(gpu, ref, geom_sph, src=(0.0, 0.0), rel=1e-6, db=0.01, near=None):
+def keyed_eig(arr):
+    """Eigenrays keyed by (receiver, bounces, order found within those)."""
+    out, seen = {}, {}
+    for a in arr:
+        k = (int(a["launch_idx"]), int(a["n_bounce"]))
+        j = seen.get(k, 0)
+        seen[k] = j + 1
+        out[k + (j,)] = a
+    return out
+
+
+def _incl(a):
+    return a["incl"] if "incl" in a.dtype.names else a["incl_launch"]
+
+
+def _az(a):
+    return a["az"] if "az" in a.dtype.names else a["az_launch"]
+
+
+def compare_eig(gpu, ref, geom_sph, src=(0.0, 0.0), rel=1e-6, db=0.01):
+    """compare() for eigenray sets; launch angles must also agree to rel."""
+    fails, summ = compare(gpu, ref, geom_sph, src, rel, db, key=keyed_eig)
+    g, r = keyed_eig(gpu), keyed_eig(ref)
+    worst = 0.0
+    bad = []
+    for k in sorted(set(g) & set(r)):
+        e = max(abs(_incl(g[k]) - _incl(r[k])) / max(abs(_incl(r[k])), 1e-12),
+                abs(_az(g[k]) - _az(r[k])) / 360.0)
+        worst = max(worst, e)
+        if e > rel:
+            bad.append((k, e))
+    summ["worst"]["launch"] = worst
+    if bad:
+        fails["launch"] = bad[:10]
+    return fails, summ
+
+
+def compare(gpu, ref, geom_sph, src=(0.0, 0.0), rel=1e-6, db=0.01, near=None, key=keyed):
     """Return a dict of failures; empty means parity holds.
 
     near: optional set of launch indices reported separately (threshold cases)."""
-    g, r = keyed(gpu), keyed(ref)
+    g, r = key(gpu), key(ref)
     fails = {}
     missing = sorted(set(r) - set(g))
     extra = sorted(set(g) - set(r))
