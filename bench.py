@@ -9,6 +9,13 @@ Workload (one "step" = one full pass of the fan through the engine):
   timed region; the pass enqueues reset, trace and arrival kernels on torch's
   current stream, which the HIP events bracket.
 
+A second leg times the batched eigenray search: 512 seeded source-receiver
+pairs (range U(100, 800) km, azimuth U(0, 360) deg) on the base stratified
+profile in the 3D geometry, source at the origin on the ground, bounces 0..3,
+0.1 km tolerance, one call; it reports per-receiver completion latency
+(p50/p99) beside the batch time, and the oracle's serial search on one pair
+as its CPU baseline.
+
 Prints one JSON line.  --gpus N with N > 1 exits: this build runs one GPU.
 """
 from __future__ import annotations
@@ -19,10 +26,16 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 F_STEP = 5690.9          # algorithmic FP64 flops per ray-step (oracle/opcount.cpp, 400-ray sample)
+# HBM bytes per fan_kernel dispatch on this workload, from rocprofv3 --pmc
+# FETCH_SIZE / WRITE_SIZE passes scaled by a 256 MiB copy (factors 2.00 and
+# 1.00): 65.7 MB read + 657.5 MB written (profiles/fan_kernel_sph162k_pmc.md)
+TRAFFIC_BYTES = 65.7e6 + 657.5e6
 TRANS_STEP = 167.8       # transcendental calls per ray-step (same count, not included in F_STEP)
 PEAK_FP64 = 78.6e12      # MI355X FP64 vector / matrix peak
 METRIC = "ray-steps/s on 162k-ray sph fan (FP64)"
@@ -57,12 +70,54 @@ def cpu_baseline(units, L, target_s=12.0):
                       f"{total_steps} ray-steps in {total_t:.1f} s, oracle restatement, 1 thread"}
 
 
+def eig_cpu_baseline(pair=6):
+    """The oracle's serial search (one host core) for one pair of the batch."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from ga_oracle_py import GEOM_3D, Medium, Oracle
+    from infraga_amd import synth
+    m = Medium(Oracle(), GEOM_3D, *synth.to_engine_units(*synth.base_profile()))
+    R = synth.eig_pairs()
+    t0 = time.perf_counter()
+    arr, steps = m.eig_search(m.params(True), (0.0, 0.0, 0.0), R[pair], bnc_min=0, bnc_max=3)
+    dt = time.perf_counter() - t0
+    return {"value": dt, "unit": "s per receiver", "cores": 1, "kind": "port",
+            "sample": f"pair {pair} ({float(np.hypot(*R[pair])):.0f} km), bounces 0..3: {len(arr)} eigenrays, "
+                      f"{steps} ray-steps in {dt:.1f} s, oracle restatement, 1 thread"}
+
+
+def eig_leg(eng, with_cpu):
+    from infraga_amd import synth
+    from infraga_amd.engine import GA_GEOM_3D, eig_params, prop_params
+    aid = eng.load_atmo(*synth.to_engine_units(*synth.base_profile()))
+    p = prop_params(GA_GEOM_3D, src=(0.0, 0.0, 0.0))
+    e = eig_params(bnc_min=0, bnc_max=3, tolerance=0.1)
+    R = synth.eig_pairs()
+    eng.eig_search(aid, p, e, R[:4])                 # warm-up (kernel set-up, buffers)
+    t0 = time.perf_counter()
+    arr, lat, st = eng.eig_search(aid, p, e, R)
+    wall = time.perf_counter() - t0
+    eng.release_atmo(aid)
+    out = {"workload": "512 pairs, 100-800 km, az 0-360, base profile 3D, bounces 0..3, tolerance 0.1 km",
+           "wall_ms": wall * 1e3, "kernel_ms": st.kernel_ms, "p50_ms": float(np.percentile(lat, 50)),
+           "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat.max()), "eigenrays": int(len(arr)),
+           "ray_steps": int(st.ray_steps), "traces": int(st.n_rays), "kernel_launches": int(st.n_launches),
+           "ray_steps_per_s": st.ray_steps / wall}
+    if with_cpu:
+        out["cpu_baseline"] = eig_cpu_baseline()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-eig", action="store_true")
+    ap.add_argument("--one-pass", action="store_true",
+                    help="one untimed fan pass (for hardware-counter runs); no JSON line")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="with --one-pass: copy a 256 MiB buffer first (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1",
@@ -70,7 +125,6 @@ def main():
                                     "(see DESIGN.md), which this build does not launch"}))
         sys.exit(2)
 
-    import numpy as np
     import torch
     from infraga_amd.engine import ARRIVAL_DTYPE, GA_GEOM_SPH, Engine, prop_params
 
@@ -89,6 +143,19 @@ def main():
 
     def one_pass():
         eng.fan_device(aid, p, d_l.data_ptr(), n, d_out.data_ptr(), d_cnt.data_ptr(), d_st.data_ptr(), stream)
+
+    if args.one_pass:
+        if args.calibrate:
+            a = torch.ones(256 * 1024 * 1024 // 8, dtype=torch.float64, device=dev)
+            b = torch.empty_like(a)
+            torch.cuda.synchronize()
+            b.copy_(a)
+        torch.cuda.synchronize()
+        one_pass()
+        torch.cuda.synchronize()
+        print(f"one pass: {int(d_st.to(torch.int64).sum().item())} ray-steps", flush=True)
+        eng.close()
+        return
 
     for _ in range(args.warmup):
         one_pass()
@@ -126,12 +193,14 @@ def main():
                    "rays": n, "ray_steps_per_pass": steps_per_pass, "arrivals_per_pass": arrivals},
         "roofline": {"bound": "mfma", "pipe": "fp64 (VALU; MI355X FP64 vector = matrix peak)",
                      "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64, "traffic": None,
+                     "frac": achieved / PEAK_FP64, "traffic": TRAFFIC_BYTES,
                      "kernel_ms_events": kern_ms, "flops_per_step": F_STEP,
                      "transcendentals_per_step": TRANS_STEP},
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(units, L)
+    if not args.no_eig:
+        line["eigenray"] = eig_leg(eng, not args.no_cpu)
     eng.close()
     print(json.dumps(line))
 

@@ -61,6 +61,10 @@ struct Medium {
 struct Limits {
     double s_max, ds_min, ds_max;
     double alt_max, rng_max, time_max;
+    // rows above this altitude [km] cannot be row K-2 of a ground return
+    // (a step moves the position by at most ds <= ds_max), so the trace
+    // loops keep their copy of the previous row only below it
+    double z_keep;
     double b0, b1, b2, b3;       // sph: lat/lon bounds [rad]; 3D: x/y bounds [km]
     double gc_thresh;            // sph: sin^2(rng_max / (2 r0)) for the range test
 };

@@ -90,8 +90,12 @@ GA_HD void eig_lane_loop(const EigArgs &E, const Medium &M, int lane, NextJob ne
                 const double zz = zcoord<G>(M, y1);
                 zmax = (zmax < zz) ? zz : zmax;
             }
+            if (zcoord<G>(M, y) < L.z_keep) {
 #pragma unroll
-            for (int i = 0; i < NEQ; ++i) { ypl[i] = y[i]; y[i] = y1[i]; }
+                for (int i = 0; i < NEQ; ++i) ypl[i] = y[i];
+            }
+#pragma unroll
+            for (int i = 0; i < NEQ; ++i) y[i] = y1[i];
             continue;
         }
         // leg finished at row K = k (y1); rows K-1 = y, K-2 = ypl

@@ -95,6 +95,23 @@ public:
     int run(const EigTracer &trace, std::vector<ga_arrival> &out, std::vector<double> &latency_ms,
             EigSearchStats &st)
     {
+        start(latency_ms);
+        EigBatch B;
+        while (next_batch(B)) {
+            EigBatchOut O;
+            int rc = trace(B, O);
+            if (rc != 0) { drop_all(); return rc; }
+            feed(B, O, st);
+        }
+        results(out);
+        return 0;
+    }
+
+    // Stepwise use (several searches interleaved by a scheduler): start(),
+    // then next_batch() / feed() until next_batch() returns false, then
+    // results().
+    void start(std::vector<double> &latency_ms)
+    {
         t0_ = std::chrono::steady_clock::now();
         latency_ms.assign(n_rcvr_, 0.0);
         live_.assign(n_rcvr_, 0);
@@ -102,48 +119,59 @@ public:
         for (int r = 0; r < n_rcvr_; ++r)
             for (int nb = S_.bnc_min; nb <= S_.bnc_max; ++nb)
                 spawn(S_.geom == GA_GEOM_SPH ? chain_sph(r, nb) : chain(r, nb), r);
-        for (;;) {
-            while (!ready_.empty()) {
-                Handle h = ready_.back();
-                ready_.pop_back();
-                h.resume();
-                if (h.done()) finish(h);
-            }
-            if (waiting_.empty()) break;
-            EigBatch B;
-            B.sweep_stride = stride_;
-            for (const Key &k : req_sweep_) B.sweep.push_back(job_of(k));
-            for (const Key &k : req_full_)
-                if (k.mode == EIG_LM) B.full.push_back(job_of(k));
-            B.n_lm = (int)B.full.size();
-            for (const Key &k : req_full_)
-                if (k.mode == EIG_FINAL) B.full.push_back(job_of(k));
-            EigBatchOut O;
-            int rc = trace(B, O);
-            if (rc != 0) { drop_all(); return rc; }
-            st.rounds++;
-            st.traces += (long long)B.sweep.size() + (long long)B.full.size();
-            for (size_t i = 0; i < B.sweep.size(); ++i) {
-                st.ray_steps += O.sweep_steps[i];
-                uint32_t id = (uint32_t)sweep_brk_.size();
-                sweep_brk_.push_back(O.sweep_brk[i]);
-                sweep_store_.insert(sweep_store_.end(), O.sweep_res.begin() + i * stride_,
-                                    O.sweep_res.begin() + (i + 1) * stride_);
-                sweep_idx_[key_of(B.sweep[i])] = id;
-            }
-            for (size_t i = 0; i < B.full.size(); ++i) {
-                st.ray_steps += O.full_steps[i];
-                uint32_t id = (uint32_t)full_brk_.size();
-                full_brk_.push_back(O.full_brk[i]);
-                full_store_.insert(full_store_.end(), O.full_res.begin() + i * EIG_FULL,
-                                   O.full_res.begin() + (i + 1) * EIG_FULL);
-                final_rows_.push_back((int)i >= B.n_lm ? O.final_rows[i - B.n_lm] : ga_arrival{});
-                full_idx_[key_of(B.full[i])] = id;
-            }
-            req_sweep_.clear();
-            req_full_.clear();
-            ready_.swap(waiting_);
+    }
+
+    // Advances every search as far as the traces at hand allow; fills B with
+    // the traces they wait for.  False: all searches are finished.
+    bool next_batch(EigBatch &B)
+    {
+        while (!ready_.empty()) {
+            Handle h = ready_.back();
+            ready_.pop_back();
+            h.resume();
+            if (h.done()) finish(h);
         }
+        B.sweep.clear();
+        B.full.clear();
+        if (waiting_.empty()) return false;
+        B.sweep_stride = stride_;
+        for (const Key &k : req_sweep_) B.sweep.push_back(job_of(k));
+        for (const Key &k : req_full_)
+            if (k.mode == EIG_LM) B.full.push_back(job_of(k));
+        B.n_lm = (int)B.full.size();
+        for (const Key &k : req_full_)
+            if (k.mode == EIG_FINAL) B.full.push_back(job_of(k));
+        return true;
+    }
+
+    void feed(const EigBatch &B, const EigBatchOut &O, EigSearchStats &st)
+    {
+        st.rounds++;
+        st.traces += (long long)B.sweep.size() + (long long)B.full.size();
+        for (size_t i = 0; i < B.sweep.size(); ++i) {
+            st.ray_steps += O.sweep_steps[i];
+            uint32_t id = (uint32_t)sweep_brk_.size();
+            sweep_brk_.push_back(O.sweep_brk[i]);
+            sweep_store_.insert(sweep_store_.end(), O.sweep_res.begin() + i * stride_,
+                                O.sweep_res.begin() + (i + 1) * stride_);
+            sweep_idx_[key_of(B.sweep[i])] = id;
+        }
+        for (size_t i = 0; i < B.full.size(); ++i) {
+            st.ray_steps += O.full_steps[i];
+            uint32_t id = (uint32_t)full_brk_.size();
+            full_brk_.push_back(O.full_brk[i]);
+            full_store_.insert(full_store_.end(), O.full_res.begin() + i * EIG_FULL,
+                               O.full_res.begin() + (i + 1) * EIG_FULL);
+            final_rows_.push_back((int)i >= B.n_lm ? O.final_rows[i - B.n_lm] : ga_arrival{});
+            full_idx_[key_of(B.full[i])] = id;
+        }
+        req_sweep_.clear();
+        req_full_.clear();
+        ready_.swap(waiting_);
+    }
+
+    void results(std::vector<ga_arrival> &out)
+    {
         std::sort(found_.begin(), found_.end(), [](const Found &a, const Found &b) {
             if (a.rcvr != b.rcvr) return a.rcvr < b.rcvr;
             if (a.nb != b.nb) return a.nb < b.nb;
@@ -151,7 +179,6 @@ public:
         });
         out.clear();
         for (const Found &f : found_) out.push_back(f.row);
-        return 0;
     }
 
 private:

@@ -31,7 +31,7 @@ using namespace ga;
 namespace {
 
 template <int G, int NEQ, bool LDS>
-__global__ void __launch_bounds__(WG) fan_kernel(FanArgs A)
+__global__ void __launch_bounds__((trace_wg<G, NEQ>())) fan_kernel(FanArgs A)
 {
     extern __shared__ double lds_tab[];
     Medium M = A.M;
@@ -65,7 +65,12 @@ __global__ void __launch_bounds__(256) arrival_kernel(FanArgs A)
 __global__ void reset_counter(unsigned int *q) { *q = 0u; }
 
 // Eigenray-search traces: same persistent layout as fan_kernel, jobs instead
-// of launch angles.  WGS lanes per workgroup share one LDS copy of the table.
+// of launch angles.  WGS lanes per workgroup share one LDS copy of the table;
+// the 6-equation sweep kernel needs <= 110 registers and runs four waves per
+// SIMD.
+#ifndef GA_EIG_WG6
+#define GA_EIG_WG6 1024
+#endif
 template <int G, int NEQ, bool LDS, int WGS>
 __global__ void __launch_bounds__(WGS) eig_kernel(EigArgs E)
 {
@@ -206,6 +211,7 @@ bool build_medium(ga_ctx *ctx, const DevAtmo &a, const ga_prop_params *p, Medium
     L.alt_max = std::isnan(p->alt_max) ? a.z.back() : p->alt_max;
     L.rng_max = p->rng_max;
     L.time_max = p->time_max;
+    L.z_keep = M.z_max + 4.0 * p->ds_max;
     if (sph) {
         L.b0 = p->bounds[0] * (PI / 180.0);
         L.b1 = p->bounds[1] * (PI / 180.0);
@@ -235,13 +241,14 @@ int launch_fan(ga_ctx *ctx, const FanArgs &A, hipStream_t st)
     auto kern = A.lds ? fan_kernel<G, NEQ, true> : fan_kernel<G, NEQ, false>;
     if (lds > 65536) HIPCHK(ctx, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     int per_cu = 0;
-    HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, WG, lds));
+    constexpr int W = trace_wg<G, NEQ>();
+    HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, W, lds));
     if (per_cu < 1) per_cu = 1;
     int grid = per_cu * ctx->n_cu;
-    int need = (A.n_rays + WG - 1) / WG;
+    int need = (A.n_rays + W - 1) / W;
     grid = std::max(1, std::min(grid, need));
     FanArgs B = A;
-    B.nlanes = grid * WG;
+    B.nlanes = grid * W;
     size_t ypn = (size_t)B.nlanes * 18;
     if (ypn > ctx->yp_cap) {
         HIPCHK(ctx, hipStreamSynchronize(st));
@@ -253,7 +260,7 @@ int launch_fan(ga_ctx *ctx, const FanArgs &A, hipStream_t st)
     }
     B.ypbuf = ctx->d_yp;
     hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, st, A.queue);
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(WG), lds, st, B);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(W), lds, st, B);
     HIPCHK(ctx, hipGetLastError());
     long long slots = (long long)A.n_rays * (A.bounces + 1);
     int agrid = (int)((slots + 255) / 256);
@@ -437,8 +444,8 @@ struct EigGpu {
             E.res = res6.p;
             E.brk_leg = brk6.p;
             E.steps = st6.p;
-            int rc = geom == GA_GEOM_SPH ? launch<GEOM_SPH, 6, 512>(E, yp6, q.p, st)
-                                         : launch<GEOM_3D, 6, 512>(E, yp6, q.p, st);
+            int rc = geom == GA_GEOM_SPH ? launch<GEOM_SPH, 6, GA_EIG_WG6>(E, yp6, q.p, st)
+                                         : launch<GEOM_3D, 6, GA_EIG_WG6>(E, yp6, q.p, st);
             if (rc != GA_OK) return rc;
         }
         if (n18) {
@@ -450,8 +457,8 @@ struct EigGpu {
             E.res = res18.p;
             E.brk_leg = brk18.p;
             E.steps = st18.p;
-            int rc = geom == GA_GEOM_SPH ? launch<GEOM_SPH, 18, WG>(E, yp18, q.p + 1, s2)
-                                         : launch<GEOM_3D, 18, WG>(E, yp18, q.p + 1, s2);
+            int rc = geom == GA_GEOM_SPH ? launch<GEOM_SPH, 18, trace_wg<GEOM_SPH, 18>()>(E, yp18, q.p + 1, s2)
+                                         : launch<GEOM_3D, 18, trace_wg<GEOM_3D, 18>()>(E, yp18, q.p + 1, s2);
             if (rc != GA_OK) return rc;
             if (nf) {
                 auto ak = geom == GA_GEOM_SPH ? eig_arrival_kernel<GEOM_SPH> : eig_arrival_kernel<GEOM_3D>;

@@ -8,7 +8,26 @@
 
 namespace ga {
 
-constexpr int WG = 256;   // threads per workgroup (4 waves, one per SIMD)
+// Threads per workgroup of the trace kernels.  The spline table takes one
+// LDS copy per workgroup (120 KB for 1501 levels), so one workgroup runs per
+// CU and its size sets the waves per SIMD; the register budget of the
+// equation set bounds it: 18 spherical equations want ~280 registers per
+// lane and run two waves per SIMD with a few spilled to scratch (measured
+// faster than one wave without spills), 18 Cartesian ones ~215 (two), 6
+// equations ~165 (three).
+#ifndef GA_WG_SPH18
+#define GA_WG_SPH18 512
+#endif
+#ifndef GA_WG_3D18
+#define GA_WG_3D18 512
+#endif
+#ifndef GA_WG6
+#define GA_WG6 768
+#endif
+template <int G, int NEQ> constexpr int trace_wg()
+{
+    return NEQ == 18 ? (G == GEOM_SPH ? GA_WG_SPH18 : GA_WG_3D18) : GA_WG6;
+}
 
 // Raw arrival record written by the step loop at each ground return:
 // row K (18 values), tt, att, rmax, theta, phi, c0, nu0.  The arrival row
@@ -314,8 +333,12 @@ GA_HD void lane_loop(const FanArgs &A, const Medium &M, int lane, NextRay next_r
         rmax = fmax(rmax, zcoord<G>(M, y));
 
         if (!end) {
+            if (zcoord<G>(M, y) < L.z_keep) {
 #pragma unroll
-            for (int i = 0; i < NEQ; ++i) { ypl[i] = y[i]; y[i] = y1[i]; }
+                for (int i = 0; i < NEQ; ++i) ypl[i] = y[i];
+            }
+#pragma unroll
+            for (int i = 0; i < NEQ; ++i) y[i] = y1[i];
             continue;
         }
         // leg finished at row K = k (y1); rows K-1 = y, K-2 = yp
