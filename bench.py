@@ -16,6 +16,14 @@ profile in the 3D geometry, source at the origin on the ground, bounces 0..3,
 (p50/p99) beside the batch time, and the oracle's serial search on one pair
 as its CPU baseline.
 
+A third leg replays a served request stream: 16 resident atmospheres (8 in
+the 3D geometry, 8 spherical, each a seeded perturbation of the base
+profile), Poisson arrivals at a fixed rate, a seeded mix of fans (1k-200k
+rays, log-uniform) and eigenray queries (1-64 receivers, bounces 0..3).
+The same trace runs through the request server twice: continuous batching
+(4 launch slots, eigenray queries merged into running searches) and one
+request at a time; both report p50/p99 request latency and ray-steps/s.
+
 Prints one JSON line.  --gpus N with N > 1 exits: this build runs one GPU.
 """
 from __future__ import annotations
@@ -34,8 +42,8 @@ sys.path.insert(0, ROOT)
 F_STEP = 5690.9          # algorithmic FP64 flops per ray-step (oracle/opcount.cpp, 400-ray sample)
 # HBM bytes per fan_kernel dispatch on this workload, from rocprofv3 --pmc
 # FETCH_SIZE / WRITE_SIZE passes scaled by a 256 MiB copy (factors 2.00 and
-# 1.00): 65.7 MB read + 657.5 MB written (profiles/fan_kernel_sph162k_pmc.md)
-TRAFFIC_BYTES = 65.7e6 + 657.5e6
+# 1.00): 65.4 MB read + 655.0 MB written (profiles/fan_kernel_sph162k_pmc.md)
+TRAFFIC_BYTES = 65.4e6 + 655.0e6
 TRANS_STEP = 167.8       # transcendental calls per ray-step (same count, not included in F_STEP)
 PEAK_FP64 = 78.6e12      # MI355X FP64 vector / matrix peak
 METRIC = "ray-steps/s on 162k-ray sph fan (FP64)"
@@ -107,6 +115,84 @@ def eig_leg(eng, with_cpu):
     return out
 
 
+def serve_trace(n_req=24, qps=1.5, seed=11):
+    """Seeded request trace: (arrival_s, kind, atmosphere index, size)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    t = np.cumsum(rng.exponential(1.0 / qps, n_req))
+    out = []
+    for i in range(n_req):
+        atm = int(rng.integers(0, 16))
+        if rng.random() < 0.5:
+            out.append((float(t[i]), "fan", atm, int(np.exp(rng.uniform(np.log(1e3), np.log(2e5))))))
+        else:
+            out.append((float(t[i]), "eig", atm, int(np.clip(np.exp(rng.uniform(0.0, np.log(64.0))), 1, 64))))
+    return out
+
+
+def serve_leg(eng, trace):
+    from infraga_amd import synth
+    from infraga_amd.engine import GA_GEOM_3D, GA_GEOM_SPH, Server, eig_params, prop_params
+    atm = [eng.load_atmo(*synth.to_engine_units(*synth.perturbed_profile(seed=1000 + k))) for k in range(16)]
+    geom = [GA_GEOM_3D] * 8 + [GA_GEOM_SPH] * 8
+    rng = np.random.Generator(np.random.PCG64(5))
+    reqs = []
+    for (t, kind, a, size) in trace:
+        g = geom[a]
+        if kind == "fan":
+            n_az = max(1, int(round(np.sqrt(size / 2.0))))
+            n_in = max(1, size // n_az)
+            L = synth.fan_launches(0.5, 0.5 + 44.5 * (n_in - 1) / max(n_in - 1, 1), 44.5 / max(n_in - 1, 1),
+                                   0.0, 360.0 * (n_az - 1) / n_az, 360.0 / n_az)
+            p = (prop_params(GA_GEOM_3D, bounces=2, src=(0.0, 0.0, 0.0), rng_max=800.0) if g == GA_GEOM_3D
+                 else prop_params(GA_GEOM_SPH, bounces=5, src=(30.0, 0.0, 0.0), rng_max=1000.0))
+            reqs.append((t, "fan", atm[a], p, None, L))
+        else:
+            r = rng.uniform(100.0, 800.0, size)
+            az = rng.uniform(0.0, 360.0, size)
+            if g == GA_GEOM_3D:
+                R = np.stack([r * np.sin(np.radians(az)), r * np.cos(np.radians(az))], axis=1)
+                p = prop_params(GA_GEOM_3D, src=(0.0, 0.0, 0.0))
+            else:
+                d = r / 6370.0
+                la0, lo0 = np.radians(30.0), 0.0
+                la = np.arcsin(np.sin(la0) * np.cos(d) + np.cos(la0) * np.sin(d) * np.cos(np.radians(az)))
+                lo = lo0 + np.arctan2(np.sin(np.radians(az)) * np.sin(d) * np.cos(la0),
+                                      np.cos(d) - np.sin(la0) * np.sin(la))
+                R = np.degrees(np.stack([la, lo], axis=1))
+                p = prop_params(GA_GEOM_SPH, src=(30.0, 0.0, 0.0))
+            reqs.append((t, "eig", atm[a], p, eig_params(bnc_min=0, bnc_max=3), R))
+    out = {"workload": f"{len(trace)} requests, Poisson {len(trace) / trace[-1][0]:.2f}/s over 16 resident "
+                       "atmospheres (8 3D, 8 sph); fans 1k-200k rays, eigenray queries 1-64 receivers"}
+    for mode, kw in (("batched", dict(slots=4, merge=True, serial=False)),
+                     ("one_request_at_a_time", dict(slots=1, merge=False, serial=True))):
+        srv = Server(eng, **kw)
+        t0 = time.perf_counter()
+        tickets = []
+        for (t, kind, a, p, e, data) in reqs:
+            dt = t - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            tickets.append((kind, srv.submit_fan(a, p, data) if kind == "fan" else srv.submit_eig(a, p, e, data)))
+        lat = {"fan": [], "eig": []}
+        for kind, tk in tickets:
+            rows, res = srv.poll(tk, timeout_ms=-1)
+            lat[kind].append(res.total_ms)
+        wall = time.perf_counter() - t0
+        steps, launches, kms = srv.stats()
+        srv.close()
+        allv = lat["fan"] + lat["eig"]
+        out[mode] = {"wall_s": wall, "ray_steps": int(steps), "ray_steps_per_s": steps / wall,
+                     "p50_ms": float(np.percentile(allv, 50)), "p99_ms": float(np.percentile(allv, 99)),
+                     "eig_p50_ms": float(np.percentile(lat["eig"], 50)) if lat["eig"] else None,
+                     "eig_p99_ms": float(np.percentile(lat["eig"], 99)) if lat["eig"] else None,
+                     "fan_p50_ms": float(np.percentile(lat["fan"], 50)) if lat["fan"] else None,
+                     "fan_p99_ms": float(np.percentile(lat["fan"], 99)) if lat["fan"] else None,
+                     "kernel_launches": int(launches)}
+    for a in atm:
+        eng.release_atmo(a)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,6 +200,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-eig", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--one-pass", action="store_true",
                     help="one untimed fan pass (for hardware-counter runs); no JSON line")
     ap.add_argument("--calibrate", action="store_true",
@@ -201,6 +288,8 @@ def main():
         line["cpu_baseline"] = cpu_baseline(units, L)
     if not args.no_eig:
         line["eigenray"] = eig_leg(eng, not args.no_cpu)
+    if not args.no_serve:
+        line["served"] = serve_leg(eng, serve_trace())
     eng.close()
     print(json.dumps(line))
 

@@ -155,6 +155,56 @@ int ga_eig_search(ga_ctx *ctx, int atmo_id, const ga_prop_params *p, const ga_ei
                   ga_eigenray *out, int out_cap, int *n_out, double *rcvr_latency_ms,
                   ga_run_stats *stats);
 
+/* ---- Request server: continuous batching of fans and eigenray queries ----
+ *
+ * No counterpart in the reference, whose unit of work is one process per
+ * run (run_prop / run_eig_search, src/main.infraga.sph.cpp:200-564,
+ * 786-996); it serves callers that submit many requests against resident
+ * atmospheres.  A scheduler thread owns the context's GPU work while the
+ * server exists (make no other calls on that context meanwhile).  Requests
+ * are admitted in arrival order while their device footprint fits the
+ * budget; up to n_slots launches run at once, each on its own streams, and
+ * their workgroups share the GPU.  Eigenray queries with the same
+ * atmosphere, source and parameters join one running search: receivers
+ * that arrive while it runs are traced in its next round. */
+typedef struct ga_server ga_server;
+
+enum { GA_REQ_FAN = 0, GA_REQ_EIG = 1 };
+enum { GA_SERVE_MERGE = 1,        /* merge eigenray queries into running searches */
+       GA_SERVE_SERIAL = 2 };     /* one request at a time (one launch per request step) */
+#define GA_PENDING 1              /* ga_server_poll: not finished within the timeout */
+
+typedef struct {
+    int32_t kind;                 /* GA_REQ_FAN or GA_REQ_EIG */
+    int32_t atmo_id;
+    ga_prop_params prop;
+    ga_eig_params eig;            /* GA_REQ_EIG */
+    const double *data;           /* fan: (incl, az) pairs; eigenray: receiver pairs (copied) */
+    int32_t n;                    /* launches or receivers */
+} ga_request;
+
+typedef struct {
+    int32_t status;               /* GA_OK or a GA_E_* code */
+    int32_t n_rows;               /* arrivals (fan) or eigenrays */
+    double  queue_ms;             /* submit -> admission */
+    double  total_ms;             /* submit -> completion */
+    int64_t ray_steps;            /* fan requests: RK4 iterations */
+} ga_result;
+
+int  ga_server_create(ga_ctx *ctx, int n_slots, double hbm_budget_bytes, int flags, ga_server **out);
+/* Thread-safe.  Copies the request's data; *ticket identifies it. */
+int  ga_server_submit(ga_server *srv, const ga_request *req, uint64_t *ticket);
+/* Thread-safe.  Waits up to timeout_ms (< 0: forever).  Returns GA_PENDING
+ * if unfinished; GA_E_CAP if rows_cap is too small (res->n_rows says how
+ * many; poll again with a larger buffer); otherwise GA_OK or the request's
+ * error, and the request is released. */
+int  ga_server_poll(ga_server *srv, uint64_t ticket, int timeout_ms, ga_result *res,
+                    ga_arrival *rows, int rows_cap);
+/* Device totals since creation. */
+void ga_server_stats(ga_server *srv, int64_t *ray_steps, int32_t *launches, double *kernel_ms);
+/* Finishes admitted and queued requests, then stops the scheduler. */
+void ga_server_destroy(ga_server *srv);
+
 #ifdef __cplusplus
 }
 #endif

@@ -81,11 +81,12 @@ struct EigSearchStats {
 
 class EigSearch {
 public:
-    EigSearch(const EigSetup &S, const double *rcvr_xy, int n_rcvr) : S_(S), n_rcvr_(n_rcvr)
+    EigSearch(const EigSetup &S, const double *rcvr_xy, int n_rcvr) : S_(S), n_rcvr_(0)
     {
-        rcvr_.assign(rcvr_xy, rcvr_xy + 2 * (size_t)n_rcvr);
         stride_ = 2 * (S.bnc_max + 1);
+        init_xy_.assign(rcvr_xy, rcvr_xy + 2 * (size_t)std::max(n_rcvr, 0));
     }
+    explicit EigSearch(const EigSetup &S) : S_(S), n_rcvr_(0) { stride_ = 2 * (S.bnc_max + 1); }
     ~EigSearch() { drop_all(); }
 
     // Runs every search to completion.  out: eigenrays grouped by receiver,
@@ -113,12 +114,52 @@ public:
     void start(std::vector<double> &latency_ms)
     {
         t0_ = std::chrono::steady_clock::now();
-        latency_ms.assign(n_rcvr_, 0.0);
-        live_.assign(n_rcvr_, 0);
         lat_ = &latency_ms;
-        for (int r = 0; r < n_rcvr_; ++r)
+        add_receivers(init_xy_.data(), (int)(init_xy_.size() / 2));
+        latency_ms.assign(n_rcvr_, 0.0);
+    }
+
+    // Adds receivers to a running search (their searches join the next
+    // batch); returns the index of the first one.  Before the first call,
+    // set_clock() fixes the time origin of the completion times.
+    int add_receivers(const double *xy, int n)
+    {
+        const int base = n_rcvr_;
+        rcvr_.insert(rcvr_.end(), xy, xy + 2 * (size_t)n);
+        n_rcvr_ += n;
+        live_.resize(n_rcvr_, 0);
+        done_ms_.resize(n_rcvr_, 0.0);
+        for (int r = base; r < n_rcvr_; ++r) {
             for (int nb = S_.bnc_min; nb <= S_.bnc_max; ++nb)
                 spawn(S_.geom == GA_GEOM_SPH ? chain_sph(r, nb) : chain(r, nb), r);
+            if (S_.bnc_max < S_.bnc_min) receiver_done(r);
+        }
+        return base;
+    }
+    void set_clock(std::chrono::steady_clock::time_point t0) { t0_ = t0; }
+
+    // Receivers whose searches all finished since the last call.
+    std::vector<int> take_done()
+    {
+        std::vector<int> d;
+        d.swap(newly_done_);
+        return d;
+    }
+    bool idle() const { return ready_.empty() && waiting_.empty(); }
+    int n_receivers() const { return n_rcvr_; }
+
+    // Eigenrays of receivers [r0, r0 + n), launch_idx renumbered from 0.
+    void results_for(int r0, int n, std::vector<ga_arrival> &out)
+    {
+        std::vector<Found> sel;
+        for (const Found &f : found_)
+            if (f.rcvr >= r0 && f.rcvr < r0 + n) sel.push_back(f);
+        std::sort(sel.begin(), sel.end(), found_order);
+        out.clear();
+        for (Found &f : sel) {
+            f.row.launch_idx = f.rcvr - r0;
+            out.push_back(f.row);
+        }
     }
 
     // Advances every search as far as the traces at hand allow; fills B with
@@ -172,13 +213,10 @@ public:
 
     void results(std::vector<ga_arrival> &out)
     {
-        std::sort(found_.begin(), found_.end(), [](const Found &a, const Found &b) {
-            if (a.rcvr != b.rcvr) return a.rcvr < b.rcvr;
-            if (a.nb != b.nb) return a.nb < b.nb;
-            return a.seq < b.seq;
-        });
+        std::sort(found_.begin(), found_.end(), found_order);
         out.clear();
         for (const Found &f : found_) out.push_back(f.row);
+        if (lat_) *lat_ = done_ms_;
     }
 
 private:
@@ -209,8 +247,12 @@ private:
     {
         int r = h.promise().rcvr;
         h.destroy();
-        if (--live_[r] == 0)
-            (*lat_)[r] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0_).count();
+        if (--live_[r] == 0) receiver_done(r);
+    }
+    void receiver_done(int r)
+    {
+        done_ms_[r] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0_).count();
+        newly_done_.push_back(r);
     }
 
     void drop_all()
@@ -607,6 +649,12 @@ private:
         int rcvr, nb, seq;
         ga_arrival row;
     };
+    static bool found_order(const Found &a, const Found &b)
+    {
+        if (a.rcvr != b.rcvr) return a.rcvr < b.rcvr;
+        if (a.nb != b.nb) return a.nb < b.nb;
+        return a.seq < b.seq;
+    }
 
     EigSetup S_;
     int n_rcvr_;
@@ -619,6 +667,9 @@ private:
     std::vector<Key> req_sweep_, req_full_;
     std::vector<Handle> ready_, waiting_;
     std::vector<int> live_;
+    std::vector<double> done_ms_;
+    std::vector<int> newly_done_;
+    std::vector<double> init_xy_;
     std::vector<double> *lat_ = nullptr;
     std::vector<Found> found_;
     std::chrono::steady_clock::time_point t0_;

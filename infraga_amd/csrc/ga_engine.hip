@@ -13,13 +13,19 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <fstream>
 #include <map>
+#include <memory>
+#include <mutex>
 #include <sstream>
 #include <string>
+#include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/infraga_gpu.h"
@@ -30,7 +36,9 @@ using namespace ga;
 
 namespace {
 
-template <int G, int NEQ, bool LDS>
+// ONE: one ray per lane (grid covers all rays, for launches that share the
+// device); otherwise persistent lanes pull rays from the queue counter.
+template <int G, int NEQ, bool LDS, bool ONE>
 __global__ void __launch_bounds__((trace_wg<G, NEQ>())) fan_kernel(FanArgs A)
 {
     extern __shared__ double lds_tab[];
@@ -42,7 +50,16 @@ __global__ void __launch_bounds__((trace_wg<G, NEQ>())) fan_kernel(FanArgs A)
         M.tab = lds_tab;
     }
     const int lane = blockIdx.x * blockDim.x + threadIdx.x;
-    lane_loop<G, NEQ>(A, M, lane, [&]() { return (int)atomicAdd(A.queue, 1u); });
+    if constexpr (ONE) {
+        bool first = true;
+        lane_loop<G, NEQ>(A, M, lane, [&]() {
+            const int r = first ? lane : A.n_rays;
+            first = false;
+            return r;
+        });
+    } else {
+        lane_loop<G, NEQ>(A, M, lane, [&]() { return (int)atomicAdd(A.queue, 1u); });
+    }
 }
 
 template <int G, int NEQ>
@@ -71,7 +88,7 @@ __global__ void reset_counter(unsigned int *q) { *q = 0u; }
 #ifndef GA_EIG_WG6
 #define GA_EIG_WG6 1024
 #endif
-template <int G, int NEQ, bool LDS, int WGS>
+template <int G, int NEQ, bool LDS, int WGS, bool ONE>
 __global__ void __launch_bounds__(WGS) eig_kernel(EigArgs E)
 {
     extern __shared__ double lds_tab[];
@@ -83,7 +100,16 @@ __global__ void __launch_bounds__(WGS) eig_kernel(EigArgs E)
         M.tab = lds_tab;
     }
     const int lane = blockIdx.x * blockDim.x + threadIdx.x;
-    eig_lane_loop<G, NEQ>(E, M, lane, [&]() { return (int)atomicAdd(E.F.queue, 1u); });
+    if constexpr (ONE) {
+        bool first = true;
+        eig_lane_loop<G, NEQ>(E, M, lane, [&]() {
+            const int r = first ? lane : E.n_jobs;
+            first = false;
+            return r;
+        });
+    } else {
+        eig_lane_loop<G, NEQ>(E, M, lane, [&]() { return (int)atomicAdd(E.F.queue, 1u); });
+    }
 }
 
 // Arrival rows of the EIG_FINAL jobs [first, first + n).
@@ -108,14 +134,28 @@ struct DevAtmo {
 
 }  // namespace
 
+// Device scratch of one fan launch: raw arrival records, the per-lane copy
+// of row k-1 and the work-queue counter.  Grown on demand; a launch in
+// flight owns its set, so launches on different streams use different sets.
+struct FanBufs {
+    unsigned int *queue = nullptr;
+    double *raw = nullptr;
+    size_t raw_cap = 0;            // doubles
+    double *yp = nullptr;
+    size_t yp_cap = 0;             // doubles
+    void release()
+    {
+        if (queue) (void)hipFree(queue);
+        if (raw) (void)hipFree(raw);
+        if (yp) (void)hipFree(yp);
+        queue = nullptr; raw = nullptr; yp = nullptr; raw_cap = 0; yp_cap = 0;
+    }
+};
+
 struct ga_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    unsigned int *d_queue = nullptr;
-    double *d_raw = nullptr;       // raw arrival records (grown on demand)
-    double *d_yp = nullptr;        // per-lane row k-1 buffer (grown on demand)
-    size_t yp_cap = 0;
-    size_t raw_cap = 0;            // in doubles
+    FanBufs fb;                    // scratch of ga_fan_run / ga_fan_run_device
     int n_cu = 0;
     size_t max_lds = 0;
     std::map<int, DevAtmo> atmos;
@@ -235,10 +275,11 @@ bool build_medium(ga_ctx *ctx, const DevAtmo &a, const ga_prop_params *p, Medium
 }
 
 template <int G, int NEQ>
-int launch_fan(ga_ctx *ctx, const FanArgs &A, hipStream_t st)
+int launch_fan(ga_ctx *ctx, FanBufs &fb, const FanArgs &A, hipStream_t st)
 {
     size_t lds = A.lds ? (size_t)A.M.n * NODE * sizeof(double) : 0;
-    auto kern = A.lds ? fan_kernel<G, NEQ, true> : fan_kernel<G, NEQ, false>;
+    auto kern = A.one_shot ? (A.lds ? fan_kernel<G, NEQ, true, true> : fan_kernel<G, NEQ, false, true>)
+                           : (A.lds ? fan_kernel<G, NEQ, true, false> : fan_kernel<G, NEQ, false, false>);
     if (lds > 65536) HIPCHK(ctx, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     int per_cu = 0;
     constexpr int W = trace_wg<G, NEQ>();
@@ -246,19 +287,19 @@ int launch_fan(ga_ctx *ctx, const FanArgs &A, hipStream_t st)
     if (per_cu < 1) per_cu = 1;
     int grid = per_cu * ctx->n_cu;
     int need = (A.n_rays + W - 1) / W;
-    grid = std::max(1, std::min(grid, need));
+    grid = A.one_shot ? std::max(1, need) : std::max(1, std::min(grid, need));
     FanArgs B = A;
     B.nlanes = grid * W;
     size_t ypn = (size_t)B.nlanes * 18;
-    if (ypn > ctx->yp_cap) {
+    if (ypn > fb.yp_cap) {
         HIPCHK(ctx, hipStreamSynchronize(st));
-        HIPCHK(ctx, hipFree(ctx->d_yp));
-        ctx->d_yp = nullptr;
-        ctx->yp_cap = 0;
-        if (hipMalloc(&ctx->d_yp, ypn * sizeof(double)) != hipSuccess) return fail(ctx, GA_E_NOMEM, "lane buffer");
-        ctx->yp_cap = ypn;
+        HIPCHK(ctx, hipFree(fb.yp));
+        fb.yp = nullptr;
+        fb.yp_cap = 0;
+        if (hipMalloc(&fb.yp, ypn * sizeof(double)) != hipSuccess) return fail(ctx, GA_E_NOMEM, "lane buffer");
+        fb.yp_cap = ypn;
     }
-    B.ypbuf = ctx->d_yp;
+    B.ypbuf = fb.yp;
     hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, st, A.queue);
     hipLaunchKernelGGL(kern, dim3(grid), dim3(W), lds, st, B);
     HIPCHK(ctx, hipGetLastError());
@@ -269,7 +310,10 @@ int launch_fan(ga_ctx *ctx, const FanArgs &A, hipStream_t st)
     return GA_OK;
 }
 
-int fan_enqueue(ga_ctx *ctx, int atmo_id, const ga_prop_params *p, const double *d_launches, int n,
+// Set by the request server around its fan launches (one ray per lane).
+thread_local int serve_one_shot = 0;
+
+int fan_enqueue(ga_ctx *ctx, FanBufs &fb, int atmo_id, const ga_prop_params *p, const double *d_launches, int n,
                 ga_arrival *d_out, int32_t *d_counts, int32_t *d_steps, hipStream_t st)
 {
     auto it = ctx->atmos.find(atmo_id);
@@ -307,26 +351,29 @@ int fan_enqueue(ga_ctx *ctx, int atmo_id, const ga_prop_params *p, const double 
     A.launches = d_launches;
     {
         size_t need = (size_t)std::max(n, 1) * (size_t)(p->bounces + 1) * RAW;
-        if (need > ctx->raw_cap) {
+        if (need > fb.raw_cap) {
             HIPCHK(ctx, hipStreamSynchronize(st));
-            HIPCHK(ctx, hipFree(ctx->d_raw));
-            ctx->d_raw = nullptr;
-            ctx->raw_cap = 0;
-            if (hipMalloc(&ctx->d_raw, need * sizeof(double)) != hipSuccess)
+            HIPCHK(ctx, hipFree(fb.raw));
+            fb.raw = nullptr;
+            fb.raw_cap = 0;
+            if (hipMalloc(&fb.raw, need * sizeof(double)) != hipSuccess)
                 return fail(ctx, GA_E_NOMEM, "raw arrival buffer (%zu doubles)", need);
-            ctx->raw_cap = need;
+            fb.raw_cap = need;
         }
-        A.raw = ctx->d_raw;
+        A.raw = fb.raw;
     }
     A.out = d_out;
     A.counts = d_counts;
     A.steps = d_steps;
-    A.queue = ctx->d_queue;
+    if (!fb.queue && hipMalloc(&fb.queue, sizeof(unsigned int)) != hipSuccess)
+        return fail(ctx, GA_E_NOMEM, "queue counter");
+    A.queue = fb.queue;
+    A.one_shot = serve_one_shot;
     size_t lds = (size_t)a.n * NODE * sizeof(double);
     A.lds = lds <= ctx->max_lds ? 1 : 0;
     if (n == 0) return GA_OK;
-    if (sph) return p->calc_amp ? launch_fan<GEOM_SPH, 18>(ctx, A, st) : launch_fan<GEOM_SPH, 6>(ctx, A, st);
-    return p->calc_amp ? launch_fan<GEOM_3D, 18>(ctx, A, st) : launch_fan<GEOM_3D, 6>(ctx, A, st);
+    if (sph) return p->calc_amp ? launch_fan<GEOM_SPH, 18>(ctx, fb, A, st) : launch_fan<GEOM_SPH, 6>(ctx, fb, A, st);
+    return p->calc_amp ? launch_fan<GEOM_3D, 18>(ctx, fb, A, st) : launch_fan<GEOM_3D, 6>(ctx, fb, A, st);
 }
 
 
@@ -363,90 +410,134 @@ struct Events {
     }
 };
 
-// GPU side of the eigenray search: one round of traces per call.  EIG_SWEEP
-// jobs run on the context stream in the 6-equation kernel, EIG_LM/EIG_FINAL
-// jobs on a second stream in the 18-equation kernel, followed there by the
-// arrival rows of the EIG_FINAL jobs.
+// Pinned host staging buffer (grow-only), so that copies queued on a
+// stream run asynchronously to the host.
+template <class T> struct PinBuf {
+    T *p = nullptr;
+    size_t cap = 0;
+    bool ensure(size_t n)
+    {
+        if (n <= cap) return true;
+        if (p && hipHostFree(p) != hipSuccess) return false;
+        p = nullptr;
+        cap = 0;
+        if (hipHostMalloc((void **)&p, std::max(n, (size_t)1) * sizeof(T), hipHostMallocDefault) != hipSuccess) {
+            p = nullptr;
+            return false;
+        }
+        cap = std::max(n, (size_t)1);
+        return true;
+    }
+    ~PinBuf() { if (p) (void)hipHostFree(p); }
+};
+
+// GPU side of the eigenray search: one round of traces per enqueue().
+// EIG_SWEEP jobs run on stream st in the 6-equation kernel, EIG_LM/EIG_FINAL
+// jobs on stream s2 in the 18-equation kernel, followed there by the arrival
+// rows of the EIG_FINAL jobs; results come back through pinned buffers and
+// the event `done` marks their arrival.  One round may be in flight at a
+// time per EigGpu.
 struct EigGpu {
     ga_ctx *ctx = nullptr;
     int geom = GA_GEOM_3D;
     FanArgs F{};                 // medium, limits, source, c_src, freq, max_steps, lds
-    hipStream_t s2 = nullptr;
+    hipStream_t st = nullptr, s2 = nullptr;
+    bool own_st = false;
     DevBuf<EigJob> jobs6, jobs18;
     DevBuf<double> res6, res18, yp6, yp18;
     DevBuf<int32_t> brk6, brk18, st6, st18;
     DevBuf<ga_arrival> arr;
     DevBuf<unsigned int> q;
+    PinBuf<EigJob> h_jobs6, h_jobs18;
+    PinBuf<double> h_res6, h_res18;
+    PinBuf<int32_t> h_brk6, h_brk18, h_st6, h_st18;
+    PinBuf<ga_arrival> h_arr;
+    hipEvent_t e0 = nullptr, e1 = nullptr, mid = nullptr;
     double kernel_ms = 0.0;
     int launches = 0;
+    size_t n6 = 0, n18 = 0, nf = 0;
+    int stride = 0;
 
     ~EigGpu()
     {
-        if (s2) {
-            (void)hipStreamSynchronize(s2);
-            (void)hipStreamDestroy(s2);
-        }
+        if (s2) { (void)hipStreamSynchronize(s2); (void)hipStreamDestroy(s2); }
+        if (own_st && st) { (void)hipStreamSynchronize(st); (void)hipStreamDestroy(st); }
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (mid) (void)hipEventDestroy(mid);
     }
 
-    int init()
+    // stream: the stream of the 6-equation kernels (nullptr: a new one)
+    int init(hipStream_t stream)
     {
+        if (stream) st = stream;
+        else { HIPCHK(ctx, hipStreamCreate(&st)); own_st = true; }
         HIPCHK(ctx, hipStreamCreate(&s2));
+        HIPCHK(ctx, hipEventCreate(&e0));
+        HIPCHK(ctx, hipEventCreate(&e1));
+        HIPCHK(ctx, hipEventCreate(&mid));
         if (!q.alloc(2)) return fail(ctx, GA_E_NOMEM, "eigenray queue counters");
         return GA_OK;
     }
 
     template <int G, int NEQ, int WGS>
-    int launch(const EigArgs &E0, DevBuf<double> &yp, unsigned int *queue, hipStream_t st)
+    int launch(const EigArgs &E0, DevBuf<double> &yp, unsigned int *queue, hipStream_t s)
     {
         size_t lds = F.lds ? (size_t)F.M.n * NODE * sizeof(double) : 0;
-        auto kern = F.lds ? eig_kernel<G, NEQ, true, WGS> : eig_kernel<G, NEQ, false, WGS>;
+        auto kern = E0.F.one_shot ? (F.lds ? eig_kernel<G, NEQ, true, WGS, true> : eig_kernel<G, NEQ, false, WGS, true>)
+                                  : (F.lds ? eig_kernel<G, NEQ, true, WGS, false> : eig_kernel<G, NEQ, false, WGS, false>);
         if (lds > 65536) HIPCHK(ctx, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         int per_cu = 0;
         HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, WGS, lds));
         per_cu = std::max(per_cu, 1);
-        int grid = std::max(1, std::min(per_cu * ctx->n_cu, (E0.n_jobs + WGS - 1) / WGS));
+        const int need = (E0.n_jobs + WGS - 1) / WGS;
+        int grid = E0.F.one_shot ? std::max(1, need) : std::max(1, std::min(per_cu * ctx->n_cu, need));
         if (!yp.ensure((size_t)grid * WGS * 18)) return fail(ctx, GA_E_NOMEM, "eigenray lane buffer");
         EigArgs E = E0;
         E.F.ypbuf = yp.p;
         E.F.nlanes = grid * WGS;
         E.F.queue = queue;
-        hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, st, queue);
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(WGS), lds, st, E);
+        hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, s, queue);
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(WGS), lds, s, E);
         HIPCHK(ctx, hipGetLastError());
         launches += 2;
         return GA_OK;
     }
 
-    int operator()(const EigBatch &B, EigBatchOut &O)
+    int enqueue(const EigBatch &B)
     {
-        const size_t n6 = B.sweep.size(), n18 = B.full.size(), nf = n18 - (size_t)B.n_lm;
-        const hipStream_t st = ctx->stream;
-        HIPCHK(ctx, hipStreamSynchronize(st));
-        HIPCHK(ctx, hipStreamSynchronize(s2));
-        if (!jobs6.ensure(n6) || !res6.ensure(n6 * B.sweep_stride) || !brk6.ensure(n6) || !st6.ensure(n6) ||
+        n6 = B.sweep.size();
+        n18 = B.full.size();
+        nf = n18 - (size_t)B.n_lm;
+        stride = B.sweep_stride;
+        // buffers are reused: the previous round has been collected
+        if (!jobs6.ensure(n6) || !res6.ensure(n6 * stride) || !brk6.ensure(n6) || !st6.ensure(n6) ||
             !jobs18.ensure(n18) || !res18.ensure(n18 * EIG_FULL) || !brk18.ensure(n18) || !st18.ensure(n18) ||
-            !arr.ensure(nf))
+            !arr.ensure(nf) || !h_jobs6.ensure(n6) || !h_jobs18.ensure(n18) || !h_res6.ensure(n6 * stride) ||
+            !h_brk6.ensure(n6) || !h_st6.ensure(n6) || !h_res18.ensure(n18 * EIG_FULL) || !h_brk18.ensure(n18) ||
+            !h_st18.ensure(n18) || !h_arr.ensure(nf))
             return fail(ctx, GA_E_NOMEM, "eigenray batch buffers (%zu + %zu traces)", n6, n18);
-        Events ev;
-        hipEvent_t mid = nullptr;
-        if (!ev.ok || hipEventCreate(&mid) != hipSuccess) return fail(ctx, GA_E_HIP, "hipEventCreate failed");
-        struct EvGuard { hipEvent_t e; ~EvGuard() { (void)hipEventDestroy(e); } } guard{mid};
-        if (n6) HIPCHK(ctx, hipMemcpyAsync(jobs6.p, B.sweep.data(), n6 * sizeof(EigJob), hipMemcpyHostToDevice, st));
-        if (n18) HIPCHK(ctx, hipMemcpyAsync(jobs18.p, B.full.data(), n18 * sizeof(EigJob), hipMemcpyHostToDevice, st));
-        HIPCHK(ctx, hipEventRecord(ev.e0, st));
-        HIPCHK(ctx, hipStreamWaitEvent(s2, ev.e0, 0));
+        if (n6) std::memcpy(h_jobs6.p, B.sweep.data(), n6 * sizeof(EigJob));
+        if (n18) std::memcpy(h_jobs18.p, B.full.data(), n18 * sizeof(EigJob));
+        if (n6) HIPCHK(ctx, hipMemcpyAsync(jobs6.p, h_jobs6.p, n6 * sizeof(EigJob), hipMemcpyHostToDevice, st));
+        if (n18) HIPCHK(ctx, hipMemcpyAsync(jobs18.p, h_jobs18.p, n18 * sizeof(EigJob), hipMemcpyHostToDevice, st));
+        HIPCHK(ctx, hipEventRecord(e0, st));
+        HIPCHK(ctx, hipStreamWaitEvent(s2, e0, 0));
         if (n6) {
             EigArgs E{};
             E.F = F;
             E.jobs = jobs6.p;
             E.n_jobs = (int)n6;
-            E.stride = B.sweep_stride;
+            E.stride = stride;
             E.res = res6.p;
             E.brk_leg = brk6.p;
             E.steps = st6.p;
             int rc = geom == GA_GEOM_SPH ? launch<GEOM_SPH, 6, GA_EIG_WG6>(E, yp6, q.p, st)
                                          : launch<GEOM_3D, 6, GA_EIG_WG6>(E, yp6, q.p, st);
             if (rc != GA_OK) return rc;
+            HIPCHK(ctx, hipMemcpyAsync(h_res6.p, res6.p, n6 * stride * sizeof(double), hipMemcpyDeviceToHost, st));
+            HIPCHK(ctx, hipMemcpyAsync(h_brk6.p, brk6.p, n6 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+            HIPCHK(ctx, hipMemcpyAsync(h_st6.p, st6.p, n6 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
         }
         if (n18) {
             EigArgs E{};
@@ -466,38 +557,461 @@ struct EigGpu {
                                    (int)nf, arr.p);
                 HIPCHK(ctx, hipGetLastError());
                 launches += 1;
+                HIPCHK(ctx, hipMemcpyAsync(h_arr.p, arr.p, nf * sizeof(ga_arrival), hipMemcpyDeviceToHost, s2));
             }
+            HIPCHK(ctx, hipMemcpyAsync(h_res18.p, res18.p, n18 * EIG_FULL * sizeof(double), hipMemcpyDeviceToHost, s2));
+            HIPCHK(ctx, hipMemcpyAsync(h_brk18.p, brk18.p, n18 * sizeof(int32_t), hipMemcpyDeviceToHost, s2));
+            HIPCHK(ctx, hipMemcpyAsync(h_st18.p, st18.p, n18 * sizeof(int32_t), hipMemcpyDeviceToHost, s2));
         }
         HIPCHK(ctx, hipEventRecord(mid, s2));
         HIPCHK(ctx, hipStreamWaitEvent(st, mid, 0));
-        HIPCHK(ctx, hipEventRecord(ev.e1, st));
-        O.sweep_res.resize(n6 * B.sweep_stride);
-        O.sweep_brk.resize(n6);
-        O.sweep_steps.resize(n6);
-        O.full_res.resize(n18 * EIG_FULL);
-        O.full_brk.resize(n18);
-        O.full_steps.resize(n18);
-        O.final_rows.resize(nf);
-        if (n6) {
-            HIPCHK(ctx, hipMemcpyAsync(O.sweep_res.data(), res6.p, O.sweep_res.size() * sizeof(double), hipMemcpyDeviceToHost, st));
-            HIPCHK(ctx, hipMemcpyAsync(O.sweep_brk.data(), brk6.p, n6 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
-            HIPCHK(ctx, hipMemcpyAsync(O.sweep_steps.data(), st6.p, n6 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
-        }
-        if (n18) {
-            HIPCHK(ctx, hipMemcpyAsync(O.full_res.data(), res18.p, O.full_res.size() * sizeof(double), hipMemcpyDeviceToHost, st));
-            HIPCHK(ctx, hipMemcpyAsync(O.full_brk.data(), brk18.p, n18 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
-            HIPCHK(ctx, hipMemcpyAsync(O.full_steps.data(), st18.p, n18 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
-        }
-        if (nf) HIPCHK(ctx, hipMemcpyAsync(O.final_rows.data(), arr.p, nf * sizeof(ga_arrival), hipMemcpyDeviceToHost, st));
-        HIPCHK(ctx, hipStreamSynchronize(st));
-        float ms = 0.0f;
-        HIPCHK(ctx, hipEventElapsedTime(&ms, ev.e0, ev.e1));
-        kernel_ms += ms;
+        HIPCHK(ctx, hipEventRecord(e1, st));
         return GA_OK;
+    }
+
+    bool ready() const { return hipEventQuery(e1) != hipErrorNotReady; }
+
+    int collect(EigBatchOut &O)
+    {
+        HIPCHK(ctx, hipEventSynchronize(e1));
+        float ms = 0.0f;
+        HIPCHK(ctx, hipEventElapsedTime(&ms, e0, e1));
+        kernel_ms += ms;
+        O.sweep_res.assign(h_res6.p, h_res6.p + n6 * stride);
+        O.sweep_brk.assign(h_brk6.p, h_brk6.p + n6);
+        O.sweep_steps.assign(h_st6.p, h_st6.p + n6);
+        O.full_res.assign(h_res18.p, h_res18.p + n18 * EIG_FULL);
+        O.full_brk.assign(h_brk18.p, h_brk18.p + n18);
+        O.full_steps.assign(h_st18.p, h_st18.p + n18);
+        O.final_rows.assign(h_arr.p, h_arr.p + nf);
+        return GA_OK;
+    }
+
+    int operator()(const EigBatch &B, EigBatchOut &O)
+    {
+        int rc = enqueue(B);
+        if (rc != GA_OK) return rc;
+        return collect(O);
     }
 };
 
+// Medium, limits and search set-up of an eigenray search (shared by
+// ga_eig_search and the request server).  rcvr: receiver pairs in the
+// caller's units, converted in place to the search's (sph: radians).
+int eig_prepare(ga_ctx *ctx, const DevAtmo &a, const ga_prop_params *p, const ga_eig_params *e, FanArgs &F,
+                EigSetup &S, std::vector<double> &rcvr)
+{
+    const bool sph = p->geometry == GA_GEOM_SPH;
+    int rc = GA_OK;
+    if (!build_medium(ctx, a, p, F.M, F.L, rc)) return rc;
+    S.geom = p->geometry;
+    S.r0 = F.M.r0;
+    S.src_ground = F.M.z0;
+    double c_arr;   // divisor of the arrival inclination
+    if (sph) {
+        // main.infraga.sph.cpp:906-909: degrees to radians, source raised to the ground
+        const double zsrc = std::max(p->src[2], F.M.z0);
+        S.src[0] = zsrc;
+        S.src[1] = p->src[0] * (PI / 180.0);
+        S.src[2] = p->src[1] * (PI / 180.0);
+        for (double &x : rcvr) x *= (PI / 180.0);
+        F.src0 = F.M.r0 + zsrc;
+        F.src1 = S.src[1];
+        F.src2 = S.src[2];
+        S.c_src = host_c_at(a, zsrc);
+        // atmo::c(src[0], ...) reads the source altitude as a radius, which
+        // clamps to the profile's bottom value (geoac.eigenray.sph.cpp:273)
+        c_arr = host_c_at(a, a.z.front());
+        // theta_grnd = 0.1 deg for a source on the ground (sph.cpp:966-979)
+        const double th_grnd = (zsrc - F.M.z0 < 0.01) ? 0.1 * (PI / 180.0) : -89.9 * (PI / 180.0);
+        S.th_start = std::max(e->incl_min * (PI / 180.0), th_grnd);
+    } else {
+        // src[2] = max(topo::z(src), src[2]) (main.infraga.3d.cpp:857)
+        F.src0 = p->src[0];
+        F.src1 = p->src[1];
+        F.src2 = std::max(p->src[2], F.M.zg);
+        S.src[0] = F.src0;
+        S.src[1] = F.src1;
+        S.src[2] = F.src2;
+        S.c_src = host_c_at(a, F.src2);
+        c_arr = S.c_src;
+        S.th_start = e->incl_min * (PI / 180.0);
+    }
+    F.c_src = c_arr;
+    F.freq = p->freq;
+    long long rows = (long long)(p->s_max / p->ds_min);
+    F.max_steps = (int)std::min<long long>(rows - 1, 2000000000LL);
+    F.lds = (size_t)a.n * NODE * sizeof(double) <= ctx->max_lds ? 1 : 0;
+    S.th_min = e->incl_min * (PI / 180.0);
+    S.th_max = e->incl_max * (PI / 180.0);
+    S.bnc_min = e->bnc_min;
+    S.bnc_max = e->bnc_max;
+    S.iterations = e->iterations;
+    S.damping = e->damping;
+    S.tolerance = e->tolerance;
+    S.az_err_lim = e->az_dev_lim;
+    S.dth_big = e->incl_step_max * (PI / 180.0);
+    S.dth_sml = e->incl_step_min * (PI / 180.0);
+    S.rng_max = p->rng_max;
+    S.c_grnd = host_c_at(a, F.M.z0);
+    S.spec_chunk = 64;
+    return GA_OK;
+}
+
+int eig_check(ga_ctx *ctx, const ga_prop_params *p, const ga_eig_params *e)
+{
+    if (p->geometry != GA_GEOM_3D && p->geometry != GA_GEOM_SPH) return fail(ctx, GA_E_ARG, "bad geometry %d", p->geometry);
+    if (e->bnc_min < 0 || e->bnc_max < e->bnc_min) return fail(ctx, GA_E_ARG, "bad bounce range %d..%d", e->bnc_min, e->bnc_max);
+    if (e->iterations < 0) return fail(ctx, GA_E_ARG, "negative iteration limit");
+    if (!(e->incl_step_max > 0.0) || !(e->incl_step_min > 0.0))
+        return fail(ctx, GA_E_ARG, "inclination steps must be positive");
+    return GA_OK;
+}
+
 }  // namespace
+
+
+// ===========================================================================
+// Request server (continuous batching)
+// ===========================================================================
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+double ms_since(Clock::time_point a, Clock::time_point b)
+{
+    return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+struct EigGroup;
+
+struct SrvReq {
+    uint64_t ticket = 0;
+    ga_request req{};
+    std::vector<double> data;
+    bool done = false;
+    int status = GA_OK;
+    std::vector<ga_arrival> rows;
+    long long ray_steps = 0;
+    Clock::time_point t_submit, t_admit, t_done;
+    double footprint = 0.0;
+    EigGroup *group = nullptr;   // eigenray requests: the search they joined
+    int r0 = 0;                  // first receiver index inside that search
+};
+
+// One running eigenray search shared by the requests that joined it.
+struct EigGroup {
+    int atmo = 0;
+    ga_prop_params p{};
+    ga_eig_params e{};
+    FanArgs F{};
+    EigSetup S{};
+    std::unique_ptr<EigSearch> search;
+    std::vector<SrvReq *> owner;        // receiver index -> request
+    std::vector<int> left;              // per request in `members`: receivers not finished
+    std::vector<SrvReq *> members;
+    EigBatch batch;
+    EigSearchStats st;
+    bool inflight = false;
+    bool closed = false;                // takes no new receivers
+};
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr, start = nullptr;
+    FanBufs fb;
+    DevBuf<double> d_l;
+    DevBuf<ga_arrival> d_o;
+    DevBuf<int32_t> d_c, d_s;
+    PinBuf<ga_arrival> h_o;
+    PinBuf<int32_t> h_c, h_s;
+    std::unique_ptr<EigGpu> eg;
+    int kind = -1;                      // -1 idle, GA_REQ_FAN, GA_REQ_EIG
+    SrvReq *fan = nullptr;
+    EigGroup *group = nullptr;
+};
+
+bool same_bytes(const void *a, const void *b, size_t n) { return std::memcmp(a, b, n) == 0; }
+
+}  // namespace
+
+struct ga_server {
+    ga_ctx *ctx = nullptr;
+    double budget = 0.0;
+    int flags = 0;
+    std::vector<std::unique_ptr<Slot>> slots;
+    std::thread th;
+    std::mutex mu;
+    std::condition_variable cv_done, cv_work;
+    std::deque<SrvReq *> submitted;                      // guarded by mu
+    std::unordered_map<uint64_t, SrvReq *> reqs;         // guarded by mu
+    uint64_t next_ticket = 1;
+    bool stop = false;
+    // scheduler thread only
+    std::deque<SrvReq *> pending, fans;
+    std::vector<std::unique_ptr<EigGroup>> groups;
+    int active = 0;
+    double inflight = 0.0;
+    size_t rr = 0;
+    // totals (guarded by mu)
+    long long steps = 0;
+    int launches = 0;
+    double kernel_ms = 0.0;
+
+    double footprint(const SrvReq &r) const
+    {
+        if (r.req.kind == GA_REQ_FAN) {
+            const double slots_ = (double)r.req.n * (r.req.prop.bounces + 1);
+            return slots_ * (sizeof(ga_arrival) + RAW * sizeof(double)) + r.req.n * 24.0 + 64.0e6;
+        }
+        return r.req.n * 1.0e6 + 64.0e6;
+    }
+
+    void finish(SrvReq *r, int status)
+    {
+        r->t_done = Clock::now();
+        inflight -= r->footprint;
+        active--;
+        std::lock_guard<std::mutex> lk(mu);
+        r->status = status;
+        r->done = true;
+        cv_done.notify_all();
+    }
+
+    void admit(SrvReq *r)
+    {
+        r->t_admit = Clock::now();
+        r->footprint = footprint(*r);
+        inflight += r->footprint;
+        active++;
+        if (r->req.kind == GA_REQ_FAN) { fans.push_back(r); return; }
+        auto it = ctx->atmos.find(r->req.atmo_id);
+        if (it == ctx->atmos.end()) { finish(r, fail(ctx, GA_E_ATMO, "unknown atmosphere id %d", r->req.atmo_id)); return; }
+        int rc = eig_check(ctx, &r->req.prop, &r->req.eig);
+        if (rc != GA_OK) { finish(r, rc); return; }
+        EigGroup *g = nullptr;
+        if (flags & GA_SERVE_MERGE)
+            for (auto &q : groups)
+                if (!q->closed && q->atmo == r->req.atmo_id && same_bytes(&q->p, &r->req.prop, sizeof(ga_prop_params)) &&
+                    same_bytes(&q->e, &r->req.eig, sizeof(ga_eig_params))) { g = q.get(); break; }
+        std::vector<double> rc_xy = r->data;
+        if (!g) {
+            auto ng = std::make_unique<EigGroup>();
+            ng->atmo = r->req.atmo_id;
+            ng->p = r->req.prop;
+            ng->e = r->req.eig;
+            std::vector<double> none;
+            rc = eig_prepare(ctx, it->second, &ng->p, &ng->e, ng->F, ng->S, none);
+            if (rc != GA_OK) { finish(r, rc); return; }
+            ng->F.one_shot = 1;
+            ng->search = std::make_unique<EigSearch>(ng->S);
+            ng->search->set_clock(Clock::now());
+            g = ng.get();
+            groups.push_back(std::move(ng));
+        }
+        if (r->req.prop.geometry == GA_GEOM_SPH)
+            for (double &x : rc_xy) x *= (PI / 180.0);
+        r->group = g;
+        g->members.push_back(r);
+        g->left.push_back(r->req.n);
+        r->r0 = g->search->add_receivers(rc_xy.data(), r->req.n);
+        g->owner.resize(g->search->n_receivers(), r);
+        if (!(flags & GA_SERVE_MERGE)) g->closed = true;
+        if (r->req.n == 0) close_member(g, r);
+    }
+
+    void close_member(EigGroup *g, SrvReq *r)
+    {
+        g->search->results_for(r->r0, r->req.n, r->rows);
+        for (size_t i = 0; i < g->members.size(); ++i)
+            if (g->members[i] == r) {
+                g->members.erase(g->members.begin() + i);
+                g->left.erase(g->left.begin() + i);
+                break;
+            }
+        finish(r, GA_OK);
+    }
+
+    void receivers_done(EigGroup *g)
+    {
+        for (int rcv : g->search->take_done()) {
+            SrvReq *r = g->owner[rcv];
+            for (size_t i = 0; i < g->members.size(); ++i)
+                if (g->members[i] == r && --g->left[i] == 0) { close_member(g, r); break; }
+        }
+    }
+
+    Slot *free_slot()
+    {
+        for (auto &sl : slots)
+            if (sl->kind < 0) return sl.get();
+        return nullptr;
+    }
+
+    // Launches one unit of work on a free slot; false when nothing is ready.
+    bool dispatch()
+    {
+        Slot *sl = free_slot();
+        if (!sl) return false;
+        // eigenray searches first (their rounds are short and latency-bound)
+        for (size_t k = 0; k < groups.size(); ++k) {
+            EigGroup *g = groups[(rr + k) % groups.size()].get();
+            if (g->inflight) continue;
+            bool more = g->search->next_batch(g->batch);
+            receivers_done(g);
+            if (!more) continue;
+            sl->eg->F = g->F;
+            sl->eg->geom = g->p.geometry;
+            int rc = sl->eg->enqueue(g->batch);
+            if (rc != GA_OK) { fail_group(g, rc); return true; }
+            g->inflight = true;
+            sl->kind = GA_REQ_EIG;
+            sl->group = g;
+            rr = (rr + k + 1) % std::max<size_t>(groups.size(), 1);
+            return true;
+        }
+        if (fans.empty()) return false;
+        SrvReq *r = fans.front();
+        fans.pop_front();
+        int rc = launch_fan(*sl, r);
+        if (rc != GA_OK) { finish(r, rc); return true; }
+        sl->kind = GA_REQ_FAN;
+        sl->fan = r;
+        return true;
+    }
+
+    void fail_group(EigGroup *g, int rc)
+    {
+        std::vector<SrvReq *> m = g->members;
+        for (SrvReq *r : m) finish(r, rc);
+        g->members.clear();
+        g->left.clear();
+        g->closed = true;
+    }
+
+    int launch_fan(Slot &sl, SrvReq *r)
+    {
+        const int n = r->req.n, nb = r->req.prop.bounces + 1;
+        const size_t nl = (size_t)std::max(n, 1);
+        if (!sl.d_l.ensure(nl * 2) || !sl.d_o.ensure(nl * nb) || !sl.d_c.ensure(nl) || !sl.d_s.ensure(nl) ||
+            !sl.h_o.ensure(nl * nb) || !sl.h_c.ensure(nl) || !sl.h_s.ensure(nl))
+            return fail(ctx, GA_E_NOMEM, "fan buffers for %d rays", n);
+        HIPCHK(ctx, hipEventRecord(sl.start, sl.stream));
+        if (n > 0) {
+            HIPCHK(ctx, hipMemcpyAsync(sl.d_l.p, r->data.data(), (size_t)n * 2 * sizeof(double), hipMemcpyHostToDevice, sl.stream));
+            ga_prop_params p = r->req.prop;
+            // one ray per lane: the launch's workgroups interleave with other requests'
+            serve_one_shot = 1;
+            int rc = fan_enqueue(ctx, sl.fb, r->req.atmo_id, &p, sl.d_l.p, n, sl.d_o.p, sl.d_c.p, sl.d_s.p, sl.stream);
+            serve_one_shot = 0;
+            if (rc != GA_OK) return rc;
+            HIPCHK(ctx, hipMemcpyAsync(sl.h_o.p, sl.d_o.p, (size_t)n * nb * sizeof(ga_arrival), hipMemcpyDeviceToHost, sl.stream));
+            HIPCHK(ctx, hipMemcpyAsync(sl.h_c.p, sl.d_c.p, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost, sl.stream));
+            HIPCHK(ctx, hipMemcpyAsync(sl.h_s.p, sl.d_s.p, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost, sl.stream));
+        }
+        HIPCHK(ctx, hipEventRecord(sl.done, sl.stream));
+        std::lock_guard<std::mutex> lk(mu);
+        launches += 3;
+        return GA_OK;
+    }
+
+    // Retires a finished launch; false when the slot is still busy.
+    bool retire(Slot &sl)
+    {
+        if (sl.kind == GA_REQ_FAN) {
+            if (hipEventQuery(sl.done) == hipErrorNotReady) return false;
+            SrvReq *r = sl.fan;
+            int rc = hipEventSynchronize(sl.done) == hipSuccess ? GA_OK : fail(ctx, GA_E_HIP, "fan launch failed");
+            float ms = 0.0f;
+            if (rc == GA_OK) (void)hipEventElapsedTime(&ms, sl.start, sl.done);
+            const int n = r->req.n, nb = r->req.prop.bounces + 1;
+            long long st = 0;
+            if (rc == GA_OK)
+                for (int i = 0; i < n; ++i) {
+                    st += sl.h_s.p[i];
+                    for (int b = 0; b < sl.h_c.p[i]; ++b) r->rows.push_back(sl.h_o.p[(size_t)i * nb + b]);
+                }
+            r->ray_steps = st;
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                steps += st;
+                kernel_ms += ms;
+            }
+            sl.kind = -1;
+            sl.fan = nullptr;
+            finish(r, rc);
+            return true;
+        }
+        EigGpu &eg = *sl.eg;
+        if (!eg.ready()) return false;
+        EigGroup *g = sl.group;
+        EigBatchOut O;
+        const double k0 = eg.kernel_ms;
+        const int l0 = eg.launches;
+        int rc = eg.collect(O);
+        sl.kind = -1;
+        sl.group = nullptr;
+        g->inflight = false;
+        if (rc != GA_OK) { fail_group(g, rc); return true; }
+        const long long s0 = g->st.ray_steps;
+        g->search->feed(g->batch, O, g->st);
+        std::lock_guard<std::mutex> lk(mu);
+        steps += g->st.ray_steps - s0;
+        kernel_ms += eg.kernel_ms - k0;
+        launches += eg.launches - l0;
+        return true;
+    }
+
+    void loop()
+    {
+        (void)hipSetDevice(ctx->device);
+        Clock::time_point last = Clock::now();
+        for (;;) {
+            bool progress = false;
+            {
+                // after destroy() was called, give up on work that makes no
+                // progress for a minute (no launch in flight can be waited for)
+                std::lock_guard<std::mutex> lk(mu);
+                bool inflight_launch = false;
+                for (auto &sl : slots) inflight_launch = inflight_launch || sl->kind >= 0;
+                if (stop && !inflight_launch && ms_since(last, Clock::now()) > 60000.0) break;
+            }
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                while (!submitted.empty()) {
+                    pending.push_back(submitted.front());
+                    submitted.pop_front();
+                }
+                bool busy = active > 0 || !pending.empty();
+                if (!busy) {
+                    if (stop) break;
+                    cv_work.wait_for(lk, std::chrono::milliseconds(50));
+                    continue;
+                }
+            }
+            // admission in arrival order while the footprint fits
+            while (!pending.empty()) {
+                if ((flags & GA_SERVE_SERIAL) && active > 0) break;
+                SrvReq *r = pending.front();
+                if (active > 0 && inflight + footprint(*r) > budget) break;
+                pending.pop_front();
+                admit(r);
+                progress = true;
+            }
+            for (auto &sl : slots)
+                if (sl->kind >= 0 && retire(*sl)) progress = true;
+            while (dispatch()) progress = true;
+            // searches with no receivers left
+            for (size_t i = 0; i < groups.size();) {
+                EigGroup *g = groups[i].get();
+                if (!g->inflight && g->members.empty() && (g->closed || g->search->idle())) groups.erase(groups.begin() + i);
+                else ++i;
+            }
+            if (progress) last = Clock::now();
+            else std::this_thread::sleep_for(std::chrono::microseconds(200));
+        }
+    }
+};
 
 // ===========================================================================
 // C ABI
@@ -559,7 +1073,7 @@ int ga_ctx_create(ga_ctx **out, int dev)
     ctx->n_cu = prop.multiProcessorCount;
     ctx->max_lds = prop.sharedMemPerBlock;
     if (hipStreamCreate(&ctx->stream) != hipSuccess) { delete ctx; return GA_E_HIP; }
-    if (hipMalloc(&ctx->d_queue, sizeof(unsigned int)) != hipSuccess) {
+    if (hipMalloc(&ctx->fb.queue, sizeof(unsigned int)) != hipSuccess) {
         (void)hipStreamDestroy(ctx->stream);
         delete ctx;
         return GA_E_NOMEM;
@@ -574,9 +1088,7 @@ void ga_ctx_destroy(ga_ctx *ctx)
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     for (auto &kv : ctx->atmos) (void)hipFree(kv.second.d_tab);
-    (void)hipFree(ctx->d_queue);
-    (void)hipFree(ctx->d_raw);
-    (void)hipFree(ctx->d_yp);
+    ctx->fb.release();
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -674,7 +1186,7 @@ int ga_fan_run_device(ga_ctx *ctx, int atmo_id, const ga_prop_params *p, const d
     if (n < 0) return fail(ctx, GA_E_ARG, "negative ray count");
     HIPCHK(ctx, hipSetDevice(ctx->device));
     hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
-    return fan_enqueue(ctx, atmo_id, p, d_launches, n, d_out, d_counts, d_steps, st);
+    return fan_enqueue(ctx, ctx->fb, atmo_id, p, d_launches, n, d_out, d_counts, d_steps, st);
 }
 
 int ga_fan_run(ga_ctx *ctx, int atmo_id, const ga_prop_params *p, const double *launches, int n,
@@ -696,7 +1208,7 @@ int ga_fan_run(ga_ctx *ctx, int atmo_id, const ga_prop_params *p, const double *
     if (!ev.ok) return fail(ctx, GA_E_HIP, "hipEventCreate failed");
     HIPCHK(ctx, hipMemcpyAsync(d_l.p, launches, (size_t)n * 2 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(ctx, hipEventRecord(ev.e0, ctx->stream));
-    int rc = fan_enqueue(ctx, atmo_id, p, d_l.p, n, d_o.p, d_c.p, d_s.p, ctx->stream);
+    int rc = fan_enqueue(ctx, ctx->fb, atmo_id, p, d_l.p, n, d_o.p, d_c.p, d_s.p, ctx->stream);
     if (rc != GA_OK) return rc;
     HIPCHK(ctx, hipEventRecord(ev.e1, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -735,79 +1247,19 @@ int ga_eig_search(ga_ctx *ctx, int atmo_id, const ga_prop_params *p, const ga_ei
     if (!ctx || !p || !e || !n_out || (n_rcvr > 0 && !rcvr_xy)) return fail(ctx, GA_E_ARG, "null argument");
     *n_out = 0;
     if (n_rcvr < 0) return fail(ctx, GA_E_ARG, "negative receiver count");
-    if (p->geometry != GA_GEOM_3D && p->geometry != GA_GEOM_SPH) return fail(ctx, GA_E_ARG, "bad geometry %d", p->geometry);
-    if (e->bnc_min < 0 || e->bnc_max < e->bnc_min) return fail(ctx, GA_E_ARG, "bad bounce range %d..%d", e->bnc_min, e->bnc_max);
-    if (e->iterations < 0) return fail(ctx, GA_E_ARG, "negative iteration limit");
-    if (!(e->incl_step_max > 0.0) || !(e->incl_step_min > 0.0))
-        return fail(ctx, GA_E_ARG, "inclination steps must be positive");
+    int rc = eig_check(ctx, p, e);
+    if (rc != GA_OK) return rc;
     auto it = ctx->atmos.find(atmo_id);
     if (it == ctx->atmos.end()) return fail(ctx, GA_E_ATMO, "unknown atmosphere id %d", atmo_id);
     HIPCHK(ctx, hipSetDevice(ctx->device));
-    const DevAtmo &a = it->second;
-
-    const bool sph = p->geometry == GA_GEOM_SPH;
     EigGpu gpu;
     gpu.ctx = ctx;
     gpu.geom = p->geometry;
-    FanArgs &F = gpu.F;
-    int rc = GA_OK;
-    if (!build_medium(ctx, a, p, F.M, F.L, rc)) return rc;
     EigSetup S;
-    S.geom = p->geometry;
-    S.r0 = F.M.r0;
-    S.src_ground = F.M.z0;
-    std::vector<double> rcvr(rcvr_xy, rcvr_xy + 2 * (size_t)std::max(n_rcvr, 0));
-    double c_arr;   // divisor of the arrival inclination
-    if (sph) {
-        // main.infraga.sph.cpp:906-909: degrees to radians, source raised to the ground
-        const double zsrc = std::max(p->src[2], F.M.z0);
-        S.src[0] = zsrc;
-        S.src[1] = p->src[0] * (PI / 180.0);
-        S.src[2] = p->src[1] * (PI / 180.0);
-        for (double &x : rcvr) x *= (PI / 180.0);
-        F.src0 = F.M.r0 + zsrc;
-        F.src1 = S.src[1];
-        F.src2 = S.src[2];
-        S.c_src = host_c_at(a, zsrc);
-        // atmo::c(src[0], ...) reads the source altitude as a radius, which
-        // clamps to the profile's bottom value (geoac.eigenray.sph.cpp:273)
-        c_arr = host_c_at(a, a.z.front());
-        // theta_grnd = 0.1 deg for a source on the ground (sph.cpp:966-979)
-        const double th_grnd = (zsrc - F.M.z0 < 0.01) ? 0.1 * (PI / 180.0) : -89.9 * (PI / 180.0);
-        S.th_start = std::max(e->incl_min * (PI / 180.0), th_grnd);
-    } else {
-        // src[2] = max(topo::z(src), src[2]) (main.infraga.3d.cpp:857)
-        F.src0 = p->src[0];
-        F.src1 = p->src[1];
-        F.src2 = std::max(p->src[2], F.M.zg);
-        S.src[0] = F.src0;
-        S.src[1] = F.src1;
-        S.src[2] = F.src2;
-        S.c_src = host_c_at(a, F.src2);
-        c_arr = S.c_src;
-        S.th_start = e->incl_min * (PI / 180.0);
-    }
-    F.c_src = c_arr;
-    F.freq = p->freq;
-    long long rows = (long long)(p->s_max / p->ds_min);
-    F.max_steps = (int)std::min<long long>(rows - 1, 2000000000LL);
-    F.lds = (size_t)a.n * NODE * sizeof(double) <= ctx->max_lds ? 1 : 0;
-
-    S.th_min = e->incl_min * (PI / 180.0);
-    S.th_max = e->incl_max * (PI / 180.0);
-    S.bnc_min = e->bnc_min;
-    S.bnc_max = e->bnc_max;
-    S.iterations = e->iterations;
-    S.damping = e->damping;
-    S.tolerance = e->tolerance;
-    S.az_err_lim = e->az_dev_lim;
-    S.dth_big = e->incl_step_max * (PI / 180.0);
-    S.dth_sml = e->incl_step_min * (PI / 180.0);
-    S.rng_max = p->rng_max;
-    S.c_grnd = host_c_at(a, F.M.z0);
-    S.spec_chunk = 64;
-
-    rc = gpu.init();
+    std::vector<double> rcvr(rcvr_xy, rcvr_xy + 2 * (size_t)n_rcvr);
+    rc = eig_prepare(ctx, it->second, p, e, gpu.F, S, rcvr);
+    if (rc != GA_OK) return rc;
+    rc = gpu.init(ctx->stream);
     if (rc != GA_OK) return rc;
     std::vector<ga_arrival> found;
     std::vector<double> lat;
@@ -831,6 +1283,109 @@ int ga_eig_search(ga_ctx *ctx, int atmo_id, const ga_prop_params *p, const ga_ei
         return fail(ctx, GA_E_CAP, "need %zu eigenray slots, have %d", found.size(), out_cap);
     for (size_t i = 0; i < found.size(); ++i) out[i] = found[i];
     return GA_OK;
+}
+
+
+int ga_server_create(ga_ctx *ctx, int n_slots, double hbm_budget_bytes, int flags, ga_server **out)
+{
+    if (!ctx || !out) return fail(ctx, GA_E_ARG, "null argument");
+    *out = nullptr;
+    if (n_slots < 1 || n_slots > 16) return fail(ctx, GA_E_ARG, "n_slots must be 1..16 (got %d)", n_slots);
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    auto srv = std::make_unique<ga_server>();
+    srv->ctx = ctx;
+    srv->budget = hbm_budget_bytes > 0.0 ? hbm_budget_bytes : 16.0e9;
+    srv->flags = flags;
+    for (int i = 0; i < n_slots; ++i) {
+        auto sl = std::make_unique<Slot>();
+        HIPCHK(ctx, hipStreamCreate(&sl->stream));
+        HIPCHK(ctx, hipEventCreate(&sl->done));
+        HIPCHK(ctx, hipEventCreate(&sl->start));
+        sl->eg = std::make_unique<EigGpu>();
+        sl->eg->ctx = ctx;
+        int rc = sl->eg->init(sl->stream);
+        if (rc != GA_OK) return rc;
+        srv->slots.push_back(std::move(sl));
+    }
+    ga_server *s = srv.release();
+    s->th = std::thread([s]() { s->loop(); });
+    *out = s;
+    return GA_OK;
+}
+
+int ga_server_submit(ga_server *srv, const ga_request *req, uint64_t *ticket)
+{
+    if (!srv || !req || !ticket) return GA_E_ARG;
+    if (req->n < 0 || (req->n > 0 && !req->data)) return GA_E_ARG;
+    if (req->kind != GA_REQ_FAN && req->kind != GA_REQ_EIG) return GA_E_ARG;
+    auto *r = new SrvReq();
+    r->req = *req;
+    r->data.assign(req->data, req->data + 2 * (size_t)req->n);
+    r->req.data = nullptr;
+    r->t_submit = Clock::now();
+    std::lock_guard<std::mutex> lk(srv->mu);
+    if (srv->stop) { delete r; return GA_E_ARG; }
+    r->ticket = srv->next_ticket++;
+    srv->reqs[r->ticket] = r;
+    srv->submitted.push_back(r);
+    *ticket = r->ticket;
+    srv->cv_work.notify_all();
+    return GA_OK;
+}
+
+int ga_server_poll(ga_server *srv, uint64_t ticket, int timeout_ms, ga_result *res, ga_arrival *rows, int rows_cap)
+{
+    if (!srv) return GA_E_ARG;
+    std::unique_lock<std::mutex> lk(srv->mu);
+    auto it = srv->reqs.find(ticket);
+    if (it == srv->reqs.end()) return GA_E_ARG;
+    SrvReq *r = it->second;
+    if (timeout_ms < 0) srv->cv_done.wait(lk, [&]() { return r->done; });
+    else srv->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), [&]() { return r->done; });
+    if (!r->done) return GA_PENDING;
+    if (res) {
+        res->status = r->status;
+        res->n_rows = (int32_t)r->rows.size();
+        res->queue_ms = ms_since(r->t_submit, r->t_admit);
+        res->total_ms = ms_since(r->t_submit, r->t_done);
+        res->ray_steps = r->ray_steps;
+    }
+    if ((long long)r->rows.size() > rows_cap || (!rows && !r->rows.empty())) return GA_E_CAP;
+    for (size_t i = 0; i < r->rows.size(); ++i) rows[i] = r->rows[i];
+    const int st = r->status;
+    srv->reqs.erase(it);
+    delete r;
+    return st;
+}
+
+void ga_server_stats(ga_server *srv, int64_t *ray_steps, int32_t *launches, double *kernel_ms)
+{
+    if (!srv) return;
+    std::lock_guard<std::mutex> lk(srv->mu);
+    if (ray_steps) *ray_steps = srv->steps;
+    if (launches) *launches = srv->launches;
+    if (kernel_ms) *kernel_ms = srv->kernel_ms;
+}
+
+void ga_server_destroy(ga_server *srv)
+{
+    if (!srv) return;
+    {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        srv->stop = true;
+        srv->cv_work.notify_all();
+    }
+    if (srv->th.joinable()) srv->th.join();
+    for (auto &sl : srv->slots) {
+        (void)hipStreamSynchronize(sl->stream);
+        sl->eg.reset();
+        sl->fb.release();
+        (void)hipEventDestroy(sl->done);
+        (void)hipEventDestroy(sl->start);
+        (void)hipStreamDestroy(sl->stream);
+    }
+    for (auto &kv : srv->reqs) delete kv.second;
+    delete srv;
 }
 
 }  // extern "C"

@@ -49,6 +49,8 @@ struct FanArgs {
     double *raw;               // RAW doubles per arrival slot (row K + leg sums)
     double *ypbuf;             // per-lane copy of row k-1, [nlanes][18]
     int nlanes;
+    int one_shot;              // 1: lane i traces item i only (grid covers all items);
+                               // 0: lanes pull items from the queue counter
     ga_arrival *out;
     int32_t *counts;
     int32_t *steps;

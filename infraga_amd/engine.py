@@ -18,9 +18,14 @@ LIB_PATH = os.path.join(HERE, "libinfraga_gpu.so")
 GA_GEOM_3D, GA_GEOM_SPH = 0, 1
 GA_OK, GA_E_ARG, GA_E_HIP, GA_E_NOMEM, GA_E_CAP, GA_E_NODEV, GA_E_ATMO, GA_E_IO = 0, -1, -2, -3, -4, -5, -6, -7
 
+GA_REQ_FAN, GA_REQ_EIG = 0, 1
+GA_SERVE_MERGE, GA_SERVE_SERIAL = 1, 2
+GA_PENDING = 1
+
 EXPORTED = ["ga_prop_params_default", "ga_eig_params_default", "ga_ctx_create", "ga_ctx_destroy",
             "ga_last_error", "ga_atmo_load_strat", "ga_atmo_load_file", "ga_atmo_release", "ga_fan_run",
-            "ga_fan_run_device", "ga_eig_search"]
+            "ga_fan_run_device", "ga_eig_search", "ga_server_create", "ga_server_submit", "ga_server_poll",
+            "ga_server_stats", "ga_server_destroy"]
 
 _dp = C.POINTER(C.c_double)
 
@@ -46,6 +51,16 @@ class Arrival(C.Structure):
                 ("pos0", C.c_double), ("pos1", C.c_double), ("time", C.c_double), ("cel", C.c_double),
                 ("turn_ht", C.c_double), ("incl_arr", C.c_double), ("back_az", C.c_double),
                 ("tl", C.c_double), ("absorption", C.c_double)]
+
+
+class Request(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("atmo_id", C.c_int32), ("prop", PropParams), ("eig", EigParams),
+                ("data", _dp), ("n", C.c_int32)]
+
+
+class Result(C.Structure):
+    _fields_ = [("status", C.c_int32), ("n_rows", C.c_int32), ("queue_ms", C.c_double),
+                ("total_ms", C.c_double), ("ray_steps", C.c_int64)]
 
 
 class RunStats(C.Structure):
@@ -92,6 +107,11 @@ def load_library(path: str | None = None):
                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     L.ga_eig_search.argtypes = [C.c_void_p, C.c_int, C.POINTER(PropParams), C.POINTER(EigParams), _dp, C.c_int,
                                 C.POINTER(Arrival), C.c_int, C.POINTER(C.c_int), _dp, C.POINTER(RunStats)]
+    L.ga_server_create.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_void_p)]
+    L.ga_server_submit.argtypes = [C.c_void_p, C.POINTER(Request), C.POINTER(C.c_uint64)]
+    L.ga_server_poll.argtypes = [C.c_void_p, C.c_uint64, C.c_int, C.POINTER(Result), C.POINTER(Arrival), C.c_int]
+    L.ga_server_stats.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.POINTER(C.c_double)]
+    L.ga_server_destroy.argtypes = [C.c_void_p]
     for f in EXPORTED:
         getattr(L, f)
     if path is None:
@@ -202,6 +222,71 @@ class Engine:
                                       lat.ctypes.data_as(_dp), C.byref(st))
         self._check(rc)
         return out[: n_out.value].copy(), lat, st
+
+
+class Server:
+    """Request server on an Engine's context (ga_server_*): submit fans and
+    eigenray queries, poll their results.  While it exists, use only the
+    server on that engine."""
+
+    def __init__(self, engine: Engine, slots: int = 4, budget_bytes: float = 16e9, merge: bool = True,
+                 serial: bool = False):
+        self.eng = engine
+        self.L = engine.L
+        h = C.c_void_p()
+        flags = (GA_SERVE_MERGE if merge else 0) | (GA_SERVE_SERIAL if serial else 0)
+        engine._check(self.L.ga_server_create(engine.h, slots, budget_bytes, flags, C.byref(h)))
+        self.h = h
+
+    def submit_fan(self, aid, params: PropParams, launches) -> int:
+        Lc = np.ascontiguousarray(launches, dtype=np.float64).reshape(-1, 2)
+        r = Request(GA_REQ_FAN, aid, params, eig_params(), Lc.ctypes.data_as(_dp), Lc.shape[0])
+        return self._submit(r)
+
+    def submit_eig(self, aid, params: PropParams, eig: EigParams, receivers) -> int:
+        R = np.ascontiguousarray(receivers, dtype=np.float64).reshape(-1, 2)
+        r = Request(GA_REQ_EIG, aid, params, eig, R.ctypes.data_as(_dp), R.shape[0])
+        return self._submit(r)
+
+    def _submit(self, r: Request) -> int:
+        t = C.c_uint64(0)
+        rc = self.L.ga_server_submit(self.h, C.byref(r), C.byref(t))
+        if rc != GA_OK:
+            raise EngineError(rc, "ga_server_submit rejected the request")
+        return t.value
+
+    def poll(self, ticket: int, timeout_ms: int = -1):
+        """(rows, Result) once finished, None while pending."""
+        res = Result()
+        cap = 1024
+        while True:
+            out = np.zeros(cap, dtype=ARRIVAL_DTYPE)
+            rc = self.L.ga_server_poll(self.h, ticket, timeout_ms, C.byref(res),
+                                       out.ctypes.data_as(C.POINTER(Arrival)), cap)
+            if rc == GA_PENDING:
+                return None
+            if rc == GA_E_CAP:
+                cap = res.n_rows
+                continue
+            if rc != GA_OK:
+                raise EngineError(rc, self.L.ga_last_error(self.eng.h).decode())
+            return out[: res.n_rows].copy(), res
+
+    def stats(self):
+        st, la, ms = C.c_int64(0), C.c_int32(0), C.c_double(0.0)
+        self.L.ga_server_stats(self.h, C.byref(st), C.byref(la), C.byref(ms))
+        return st.value, la.value, ms.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.ga_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def nan():

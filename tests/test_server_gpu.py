@@ -1,0 +1,89 @@
+"""The request server (ga_server_*): continuous batching of fans and
+eigenray queries over resident atmospheres.
+
+Each request's result in a mixed stream must equal the same request run
+alone through the direct entry points (ga_fan_run, ga_eig_search), to the
+parity bar: identical arrival / eigenray keys, range, time and turning
+height within 1e-6 relative, TL and absorption within 0.01 dB.
+"""
+import numpy as np
+import pytest
+
+from infraga_amd import synth
+from infraga_amd.engine import GA_GEOM_3D, GA_GEOM_SPH, Server, eig_params, prop_params
+from parity import compare, compare_eig
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def atmos(engine, base_units, pert_units):
+    ids = {"3d": engine.load_atmo(*base_units), "sph": engine.load_atmo(*pert_units)}
+    yield ids
+    for a in ids.values():
+        engine.release_atmo(a)
+
+
+def _requests(atmos):
+    R = synth.eig_pairs()
+    fan3 = synth.fan_launches(0.5, 40.0, 1.5, 0.0, 300.0, 60.0)
+    fans = synth.fan_launches(0.5, 40.0, 2.5, -90.0, 45.0, 135.0)
+    p3 = prop_params(GA_GEOM_3D, bounces=2, src=(0.0, 0.0, 0.0), rng_max=800.0)
+    ps = prop_params(GA_GEOM_SPH, bounces=2, src=(30.0, -100.0, 0.0), rng_max=800.0)
+    pe = prop_params(GA_GEOM_3D, src=(0.0, 0.0, 0.0))
+    e = eig_params(bnc_min=0, bnc_max=1)
+    return [
+        ("fan", atmos["3d"], p3, None, fan3),
+        ("eig", atmos["3d"], pe, e, R[[1, 5]]),
+        ("fan", atmos["sph"], ps, None, fans),
+        ("eig", atmos["3d"], pe, e, R[[0, 7]]),      # joins the running search of the same source
+        ("fan", atmos["3d"], p3, None, fan3[::2]),
+    ]
+
+
+def _alone(engine, kind, aid, p, e, data):
+    if kind == "fan":
+        arr, _ = engine.fan(aid, p, data)
+        return arr
+    arr, _, _ = engine.eig_search(aid, p, e, data)
+    return arr
+
+
+@pytest.mark.parametrize("merge,serial,slots", [(True, False, 4), (False, True, 1)], ids=["batched", "serial"])
+def test_mixed_stream_equals_alone(engine, atmos, merge, serial, slots):
+    reqs = _requests(atmos)
+    alone = [_alone(engine, *r) for r in reqs]
+    srv = Server(engine, slots=slots, merge=merge, serial=serial)
+    try:
+        tickets = []
+        for kind, aid, p, e, data in reqs:
+            tickets.append(srv.submit_fan(aid, p, data) if kind == "fan" else srv.submit_eig(aid, p, e, data))
+        results = [srv.poll(t, timeout_ms=120000) for t in tickets]
+        steps, launches, kms = srv.stats()
+    finally:
+        srv.close()
+    assert steps > 0 and launches > 0
+    for (kind, aid, p, e, data), ref, got in zip(reqs, alone, results):
+        assert got is not None, "request did not finish"
+        rows, res = got
+        assert res.status == 0 and res.total_ms >= res.queue_ms >= 0.0
+        sph = p.geometry == GA_GEOM_SPH
+        if kind == "fan":
+            fails, summ = compare(rows, ref, sph, src=(p.src[0], p.src[1]))
+        else:
+            fails, summ = compare_eig(rows, ref, sph, src=(p.src[0], p.src[1]))
+        assert not fails, (kind, fails, summ)
+        assert len(rows) == len(ref) and len(rows) > 0
+
+
+def test_server_reports_request_errors(engine, atmos):
+    srv = Server(engine, slots=2)
+    try:
+        t = srv.submit_fan(atmos["3d"] + 1000, prop_params(GA_GEOM_3D), [(10.0, 0.0)])
+        with pytest.raises(Exception):
+            srv.poll(t, timeout_ms=60000)
+        t2 = srv.submit_eig(atmos["3d"], prop_params(GA_GEOM_3D), eig_params(), np.zeros((0, 2)))
+        rows, res = srv.poll(t2, timeout_ms=60000)
+        assert res.status == 0 and len(rows) == 0
+    finally:
+        srv.close()
