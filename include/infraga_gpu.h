@@ -155,6 +155,31 @@ int ga_eig_search(ga_ctx *ctx, int atmo_id, const ga_prop_params *p, const ga_ei
                   ga_eigenray *out, int out_cap, int *n_out, double *rcvr_latency_ms,
                   ga_run_stats *stats);
 
+/* ---- Range-dependent atmosphere and topography --------------------------
+ *
+ * ga_rngdep_load replaces the loader of the range-dependent spherical
+ * geometry (set_region, src/atmo/atmo_io.sph.rngdep.cpp:61-200: a
+ * lat x lon grid of profiles into interp::hybrid_spline_3D c/u/v/rho
+ * splines, src/util/interpolation.cpp:886-1045); node coordinates in degrees
+ * (as the node files give them), fields [nlat][nlon][nz] in engine units
+ * (km/s, g/cm^3).  ga_topo_load replaces the topography loader (a
+ * natural_cubic_spline_2D, interpolation.cpp:266-317) with heights
+ * [nlat][nlon] in km.  ga_rngdep_eval evaluates at n points (lat, lon [rad],
+ * z [km]) what atmo::c/u/v/rho and their derivatives
+ * (atmo_state.sph.rngdep.cpp, interp::eval_all of the hybrid spline) and
+ * topo::z/dz/ddz/dddz (atmo_state.sph.rngdep.cpp:59-97) compute:
+ * atmo_out[n][4][10] = per field (c, u, v, rho) {f, f_lat, f_lon, f_z,
+ * f_latlat, f_lonlon, f_zz, f_latlon, f_latz, f_lonz}; topo_out[n][10] =
+ * {z, z_lat, z_lon, z_latlat, z_latlon, z_lonlon, z_latlatlat, z_latlatlon,
+ * z_latlonlon, z_lonlonlon} (zeros without topography). */
+int ga_rngdep_load(ga_ctx *ctx, const double *lat_deg, int nlat, const double *lon_deg, int nlon,
+                   const double *z_km, int nz, const double *c, const double *u, const double *v,
+                   const double *rho, int *rd_id);
+int ga_topo_load(ga_ctx *ctx, int rd_id, const double *lat_deg, int nlat, const double *lon_deg, int nlon,
+                 const double *zg_km);
+int ga_rngdep_eval(ga_ctx *ctx, int rd_id, const double *pts, int n, double *atmo_out, double *topo_out);
+int ga_rngdep_release(ga_ctx *ctx, int rd_id);
+
 /* ---- Request server: continuous batching of fans and eigenray queries ----
  *
  * No counterpart in the reference, whose unit of work is one process per

@@ -30,6 +30,7 @@
 
 #include "../../include/infraga_gpu.h"
 #include "ga_eig_search.hpp"
+#include "ga_rngdep.hpp"
 #include "ga_trace.hpp"
 
 using namespace ga;
@@ -80,6 +81,29 @@ __global__ void __launch_bounds__(256) arrival_kernel(FanArgs A)
 }
 
 __global__ void reset_counter(unsigned int *q) { *q = 0u; }
+
+// Range-dependent atmosphere and topography at n points (lat, lon [rad], z
+// [km]): per point 4 fields x {f, fx, fy, fz, fxx, fyy, fzz, fxy, fxz, fyz}
+// and the topography {z, zx, zy, zxx, zxy, zyy, zxxx, zxxy, zxyy, zyyy}.
+__global__ void __launch_bounds__(256) rd_eval_kernel(RdGrid g, TopoGrid t, int has_topo, const double *pts, int n,
+                                                      double *aout, double *tout)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double x = pts[3 * i], y = pts[3 * i + 1], z = pts[3 * i + 2];
+    for (int fld = 0; fld < 4; ++fld) {
+        double f, df[3], ddf[6];
+        rd_field_all(g, fld, x, y, z, f, df, ddf);
+        double *o = aout + ((size_t)i * 4 + fld) * 10;
+        o[0] = f;
+        for (int k = 0; k < 3; ++k) o[1 + k] = df[k];
+        for (int k = 0; k < 6; ++k) o[4 + k] = ddf[k];
+    }
+    double tz[10];
+    if (has_topo) topo_all(t, x, y, tz);
+    else for (int k = 0; k < 10; ++k) tz[k] = 0.0;
+    for (int k = 0; k < 10; ++k) tout[(size_t)i * 10 + k] = tz[k];
+}
 
 // Eigenray-search traces: same persistent layout as fan_kernel, jobs instead
 // of launch angles.  WGS lanes per workgroup share one LDS copy of the table;
@@ -132,6 +156,22 @@ struct DevAtmo {
     std::vector<double> z, c, u, v, rho;
 };
 
+// Range-dependent atmosphere (hybrid-spline tables) and optional topography.
+struct DevRd {
+    double *d_axes = nullptr;      // lat [nx], lon [ny], z [nz]
+    double *d_tab = nullptr;       // [4][nx][ny][nz][RD_Q]
+    int nx = 0, ny = 0, nz = 0;
+    double *d_taxes = nullptr;     // topography lat [tnx], lon [tny]
+    double *d_ttab = nullptr;      // [tnx][tny][TP_Q]
+    int tnx = 0, tny = 0;
+    void release()
+    {
+        for (double *p : {d_axes, d_tab, d_taxes, d_ttab})
+            if (p) (void)hipFree(p);
+        d_axes = d_tab = d_taxes = d_ttab = nullptr;
+    }
+};
+
 }  // namespace
 
 // Device scratch of one fan launch: raw arrival records, the per-lane copy
@@ -159,6 +199,7 @@ struct ga_ctx {
     int n_cu = 0;
     size_t max_lds = 0;
     std::map<int, DevAtmo> atmos;
+    std::map<int, DevRd> rds;
     int next_id = 1;
     std::string err;
 };
@@ -1088,6 +1129,7 @@ void ga_ctx_destroy(ga_ctx *ctx)
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     for (auto &kv : ctx->atmos) (void)hipFree(kv.second.d_tab);
+    for (auto &kv : ctx->rds) kv.second.release();
     ctx->fb.release();
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -1285,6 +1327,166 @@ int ga_eig_search(ga_ctx *ctx, int atmo_id, const ga_prop_params *p, const ga_ei
     return GA_OK;
 }
 
+
+
+// ---- range-dependent atmosphere and topography ---------------------------
+
+static bool increasing(const double *v, int n)
+{
+    for (int i = 0; i + 1 < n; ++i)
+        if (!(v[i + 1] > v[i])) return false;
+    return true;
+}
+
+int ga_rngdep_load(ga_ctx *ctx, const double *lat_deg, int nlat, const double *lon_deg, int nlon, const double *z_km,
+                   int nz, const double *c, const double *u, const double *v, const double *rho, int *rd_id)
+{
+    if (!ctx || !lat_deg || !lon_deg || !z_km || !c || !u || !v || !rho || !rd_id) return fail(ctx, GA_E_ARG, "null argument");
+    if (nlat < 2 || nlon < 2 || nz < 3) return fail(ctx, GA_E_ARG, "grid needs >= 2 x 2 nodes and >= 3 levels");
+    if (!increasing(lat_deg, nlat) || !increasing(lon_deg, nlon) || !increasing(z_km, nz))
+        return fail(ctx, GA_E_ARG, "node coordinates must increase strictly");
+    const size_t N = (size_t)nlat * nlon * nz;
+    std::vector<double> x(nlat), y(nlon), z(z_km, z_km + nz);
+    // node files hold degrees; the splines run in radians (atmo_io.sph.rngdep.cpp)
+    for (int i = 0; i < nlat; ++i) x[i] = lat_deg[i] * (PI / 180.0);
+    for (int j = 0; j < nlon; ++j) y[j] = lon_deg[j] * (PI / 180.0);
+    std::vector<double> tab(4 * N * RD_Q);
+    const double *fields[4] = {c, u, v, rho};
+    std::vector<double> col(nz), sl, dcol(nz), dsl;
+    for (int fl = 0; fl < 4; ++fl) {
+        const double *F = fields[fl];
+        for (size_t q = 0; q < N; ++q)
+            if (!std::isfinite(F[q])) return fail(ctx, GA_E_ARG, "non-finite value in field %d", fl);
+        auto at = [&](int i, int j, int k) { return F[((size_t)i * nlon + j) * nz + k]; };
+        for (int i = 0; i < nlat; ++i)
+            for (int j = 0; j < nlon; ++j) {
+                const int iu = std::min(i + 1, nlat - 1), id = std::max(i - 1, 0);
+                const int ju = std::min(j + 1, nlon - 1), jd = std::max(j - 1, 0);
+                for (int k = 0; k < nz; ++k) {
+                    col[k] = at(i, j, k);
+                    dcol[k] = (at(iu, j, k) - at(id, j, k)) / (x[iu] - x[id]);
+                }
+                spline_slopes(z, col, sl);
+                spline_slopes(z, dcol, dsl);
+                for (int k = 0; k < nz; ++k) {
+                    double *t = &tab[((((size_t)fl * nlat + i) * nlon + j) * nz + k) * RD_Q];
+                    t[0] = col[k];
+                    t[1] = sl[k];
+                    t[2] = dcol[k];
+                    t[3] = dsl[k];
+                    t[4] = (at(i, ju, k) - at(i, jd, k)) / (y[ju] - y[jd]);
+                }
+            }
+    }
+    std::vector<double> axes;
+    axes.insert(axes.end(), x.begin(), x.end());
+    axes.insert(axes.end(), y.begin(), y.end());
+    axes.insert(axes.end(), z.begin(), z.end());
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    DevRd r;
+    r.nx = nlat; r.ny = nlon; r.nz = nz;
+    if (hipMalloc(&r.d_axes, axes.size() * sizeof(double)) != hipSuccess ||
+        hipMalloc(&r.d_tab, tab.size() * sizeof(double)) != hipSuccess) {
+        r.release();
+        return fail(ctx, GA_E_NOMEM, "range-dependent tables (%zu MB)", tab.size() * 8 / 1000000);
+    }
+    if (hipMemcpy(r.d_axes, axes.data(), axes.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(r.d_tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+        r.release();
+        return fail(ctx, GA_E_HIP, "range-dependent table upload failed");
+    }
+    int id = ctx->next_id++;
+    ctx->rds[id] = r;
+    *rd_id = id;
+    return GA_OK;
+}
+
+int ga_topo_load(ga_ctx *ctx, int rd_id, const double *lat_deg, int nlat, const double *lon_deg, int nlon,
+                 const double *zg_km)
+{
+    if (!ctx || !lat_deg || !lon_deg || !zg_km) return fail(ctx, GA_E_ARG, "null argument");
+    auto it = ctx->rds.find(rd_id);
+    if (it == ctx->rds.end()) return fail(ctx, GA_E_ATMO, "unknown range-dependent atmosphere id %d", rd_id);
+    if (nlat < 2 || nlon < 2) return fail(ctx, GA_E_ARG, "topography needs >= 2 x 2 nodes");
+    if (!increasing(lat_deg, nlat) || !increasing(lon_deg, nlon)) return fail(ctx, GA_E_ARG, "topography nodes must increase");
+    std::vector<double> x(nlat), y(nlon);
+    for (int i = 0; i < nlat; ++i) x[i] = lat_deg[i] * (PI / 180.0);
+    for (int j = 0; j < nlon; ++j) y[j] = lon_deg[j] * (PI / 180.0);
+    // per node: z, y slope of its row, x slope of its column of z, x slope of
+    // its column of y slopes
+    std::vector<double> fy((size_t)nlat * nlon), tab((size_t)nlat * nlon * TP_Q), row(nlon), sl, colv(nlat), csl;
+    for (int i = 0; i < nlat; ++i) {
+        for (int j = 0; j < nlon; ++j) {
+            row[j] = zg_km[(size_t)i * nlon + j];
+            if (!std::isfinite(row[j])) return fail(ctx, GA_E_ARG, "non-finite topography at (%d, %d)", i, j);
+        }
+        spline_slopes(y, row, sl);
+        for (int j = 0; j < nlon; ++j) fy[(size_t)i * nlon + j] = sl[j];
+    }
+    for (int j = 0; j < nlon; ++j) {
+        for (int i = 0; i < nlat; ++i) colv[i] = zg_km[(size_t)i * nlon + j];
+        spline_slopes(x, colv, csl);
+        for (int i = 0; i < nlat; ++i) {
+            double *t = &tab[((size_t)i * nlon + j) * TP_Q];
+            t[0] = colv[i];
+            t[1] = fy[(size_t)i * nlon + j];
+            t[2] = csl[i];
+        }
+        for (int i = 0; i < nlat; ++i) colv[i] = fy[(size_t)i * nlon + j];
+        spline_slopes(x, colv, csl);
+        for (int i = 0; i < nlat; ++i) tab[((size_t)i * nlon + j) * TP_Q + 3] = csl[i];
+    }
+    std::vector<double> axes(x);
+    axes.insert(axes.end(), y.begin(), y.end());
+    DevRd &r = it->second;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    if (r.d_taxes) { HIPCHK(ctx, hipFree(r.d_taxes)); r.d_taxes = nullptr; }
+    if (r.d_ttab) { HIPCHK(ctx, hipFree(r.d_ttab)); r.d_ttab = nullptr; }
+    if (hipMalloc(&r.d_taxes, axes.size() * sizeof(double)) != hipSuccess ||
+        hipMalloc(&r.d_ttab, tab.size() * sizeof(double)) != hipSuccess)
+        return fail(ctx, GA_E_NOMEM, "topography tables");
+    HIPCHK(ctx, hipMemcpy(r.d_taxes, axes.data(), axes.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(r.d_ttab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+    r.tnx = nlat;
+    r.tny = nlon;
+    return GA_OK;
+}
+
+int ga_rngdep_eval(ga_ctx *ctx, int rd_id, const double *pts, int n, double *atmo_out, double *topo_out)
+{
+    if (!ctx || (n > 0 && (!pts || !atmo_out || !topo_out))) return fail(ctx, GA_E_ARG, "null argument");
+    if (n < 0) return fail(ctx, GA_E_ARG, "negative point count");
+    auto it = ctx->rds.find(rd_id);
+    if (it == ctx->rds.end()) return fail(ctx, GA_E_ATMO, "unknown range-dependent atmosphere id %d", rd_id);
+    if (n == 0) return GA_OK;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const DevRd &r = it->second;
+    RdGrid g{r.d_axes, r.d_axes + r.nx, r.d_axes + r.nx + r.ny, r.nx, r.ny, r.nz, r.d_tab};
+    TopoGrid t{r.d_taxes, r.d_taxes ? r.d_taxes + r.tnx : nullptr, r.tnx, r.tny, r.d_ttab};
+    DevBuf<double> d_p, d_a, d_t;
+    if (!d_p.alloc((size_t)n * 3) || !d_a.alloc((size_t)n * 40) || !d_t.alloc((size_t)n * 10))
+        return fail(ctx, GA_E_NOMEM, "evaluation buffers");
+    HIPCHK(ctx, hipMemcpyAsync(d_p.p, pts, (size_t)n * 3 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(rd_eval_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, g, t, r.d_ttab ? 1 : 0, d_p.p,
+                       n, d_a.p, d_t.p);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipMemcpyAsync(atmo_out, d_a.p, (size_t)n * 40 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(topo_out, d_t.p, (size_t)n * 10 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return GA_OK;
+}
+
+int ga_rngdep_release(ga_ctx *ctx, int rd_id)
+{
+    if (!ctx) return GA_E_ARG;
+    auto it = ctx->rds.find(rd_id);
+    if (it == ctx->rds.end()) return fail(ctx, GA_E_ATMO, "unknown range-dependent atmosphere id %d", rd_id);
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    it->second.release();
+    ctx->rds.erase(it);
+    return GA_OK;
+}
 
 int ga_server_create(ga_ctx *ctx, int n_slots, double hbm_budget_bytes, int flags, ga_server **out)
 {

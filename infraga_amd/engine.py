@@ -25,7 +25,8 @@ GA_PENDING = 1
 EXPORTED = ["ga_prop_params_default", "ga_eig_params_default", "ga_ctx_create", "ga_ctx_destroy",
             "ga_last_error", "ga_atmo_load_strat", "ga_atmo_load_file", "ga_atmo_release", "ga_fan_run",
             "ga_fan_run_device", "ga_eig_search", "ga_server_create", "ga_server_submit", "ga_server_poll",
-            "ga_server_stats", "ga_server_destroy"]
+            "ga_server_stats", "ga_server_destroy", "ga_rngdep_load", "ga_topo_load", "ga_rngdep_eval",
+            "ga_rngdep_release"]
 
 _dp = C.POINTER(C.c_double)
 
@@ -107,6 +108,11 @@ def load_library(path: str | None = None):
                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     L.ga_eig_search.argtypes = [C.c_void_p, C.c_int, C.POINTER(PropParams), C.POINTER(EigParams), _dp, C.c_int,
                                 C.POINTER(Arrival), C.c_int, C.POINTER(C.c_int), _dp, C.POINTER(RunStats)]
+    L.ga_rngdep_load.argtypes = [C.c_void_p, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, _dp, _dp, _dp,
+                                 C.POINTER(C.c_int)]
+    L.ga_topo_load.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp]
+    L.ga_rngdep_eval.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int, _dp, _dp]
+    L.ga_rngdep_release.argtypes = [C.c_void_p, C.c_int]
     L.ga_server_create.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_void_p)]
     L.ga_server_submit.argtypes = [C.c_void_p, C.POINTER(Request), C.POINTER(C.c_uint64)]
     L.ga_server_poll.argtypes = [C.c_void_p, C.c_uint64, C.c_int, C.POINTER(Result), C.POINTER(Arrival), C.c_int]
@@ -202,6 +208,34 @@ class Engine:
         self._check(self.L.ga_fan_run_device(self.h, aid, C.byref(params), C.c_void_p(d_launches), n,
                                              C.c_void_p(d_out), C.c_void_p(d_counts), C.c_void_p(d_steps),
                                              C.c_void_p(stream) if stream else None))
+
+    def rngdep_load(self, lat_deg, lon_deg, z_km, c, u, v, rho) -> int:
+        """Range-dependent atmosphere: node lat/lon [deg], z [km], fields [nlat][nlon][nz]."""
+        a = [np.ascontiguousarray(q, dtype=np.float64) for q in (lat_deg, lon_deg, z_km, c, u, v, rho)]
+        rid = C.c_int(0)
+        self._check(self.L.ga_rngdep_load(self.h, a[0].ctypes.data_as(_dp), len(a[0]), a[1].ctypes.data_as(_dp),
+                                          len(a[1]), a[2].ctypes.data_as(_dp), len(a[2]),
+                                          *[q.ctypes.data_as(_dp) for q in a[3:]], C.byref(rid)))
+        self._keep_rd = a
+        return rid.value
+
+    def topo_load(self, rid, lat_deg, lon_deg, zg_km):
+        a = [np.ascontiguousarray(q, dtype=np.float64) for q in (lat_deg, lon_deg, zg_km)]
+        self._check(self.L.ga_topo_load(self.h, rid, a[0].ctypes.data_as(_dp), len(a[0]), a[1].ctypes.data_as(_dp),
+                                        len(a[1]), a[2].ctypes.data_as(_dp)))
+
+    def rngdep_eval(self, rid, pts):
+        """pts: (lat, lon [rad], z [km]) rows -> (atmo [n, 4, 10], topo [n, 10])."""
+        P = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
+        n = P.shape[0]
+        A = np.zeros((n, 4, 10))
+        T = np.zeros((n, 10))
+        self._check(self.L.ga_rngdep_eval(self.h, rid, P.ctypes.data_as(_dp), n, A.ctypes.data_as(_dp),
+                                          T.ctypes.data_as(_dp)))
+        return A, T
+
+    def rngdep_release(self, rid):
+        self._check(self.L.ga_rngdep_release(self.h, rid))
 
     def eig_search(self, aid, params: PropParams, eig: EigParams, rcvr_xy):
         R = np.ascontiguousarray(rcvr_xy, dtype=np.float64).reshape(-1, 2)
