@@ -126,3 +126,76 @@ def eig_pairs(n_pairs: int = 512, seed: int = 7, rmin: float = 100.0, rmax: floa
     x = r * np.sin(np.radians(az))
     y = r * np.cos(np.radians(az))
     return np.stack([x, y], axis=1)
+
+
+# ---------------------------------------------------------------------------
+# Range-dependent spherical workload: a lat/lon grid of profiles (the base
+# profile plus a seeded random field, 500 km horizontal and 5 km vertical
+# correlation, peak amplitude 15 m/s on u and v and 5 K on T) and seeded
+# fractal terrain (power-law spectrum, scaled to 0..2 km).
+# ---------------------------------------------------------------------------
+R_EARTH = 6370.0
+
+
+def _grid_field(rng, lat, lon, z, amp, corr_km, corr_z):
+    """Random field on the (lat, lon, z) nodes with Gaussian horizontal
+    (great-circle distance, corr_km) and vertical (corr_z) correlation,
+    scaled to peak |value| = amp."""
+    la, lo = np.meshgrid(np.radians(lat), np.radians(lon), indexing="ij")
+    la, lo = la.ravel(), lo.ravel()
+    s = np.sin((la[:, None] - la[None, :]) / 2.0) ** 2 + \
+        np.cos(la[:, None]) * np.cos(la[None, :]) * np.sin((lo[:, None] - lo[None, :]) / 2.0) ** 2
+    d = 2.0 * R_EARTH * np.arcsin(np.sqrt(np.clip(s, 0.0, 1.0)))
+    Wh = np.exp(-(d / corr_km) ** 2)
+    w = rng.standard_normal((la.size, z.size))
+    f = Wh @ w
+    dz = z[1] - z[0]
+    half = int(np.ceil(3.0 * corr_z / dz))
+    k = np.exp(-((np.arange(-half, half + 1) * dz) / corr_z) ** 2)
+    f = np.stack([np.convolve(row, k, mode="same") for row in f])
+    f = f - f.mean()
+    f *= amp / np.max(np.abs(f))
+    return f.reshape(lat.size, lon.size, z.size)
+
+
+def rngdep_grid(seed: int = 3, lat=None, lon=None, dz: float = 0.1, z_top: float = 150.0,
+                amp_uv: float = 15.0, amp_T: float = 5.0, corr_km: float = 500.0, corr_z: float = 5.0):
+    """Profiles on a lat/lon node grid (default 11 x 11 nodes, 2 deg apart,
+    30..50 N, 110..90 W).  Returns (lat, lon, z, T, u, v, rho) with the
+    fields shaped [nlat, nlon, nz]; u, v in m/s."""
+    lat = np.linspace(30.0, 50.0, 11) if lat is None else np.asarray(lat, dtype=np.float64)
+    lon = np.linspace(-110.0, -90.0, 11) if lon is None else np.asarray(lon, dtype=np.float64)
+    z, T, u, v, rho = base_profile(dz, z_top)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    shape = (lat.size, lon.size, z.size)
+    T3 = T + _grid_field(rng, lat, lon, z, amp_T, corr_km, corr_z)
+    u3 = u + _grid_field(rng, lat, lon, z, amp_uv, corr_km, corr_z)
+    v3 = v + _grid_field(rng, lat, lon, z, amp_uv, corr_km, corr_z)
+    rho3 = np.ascontiguousarray(np.broadcast_to(rho, shape))
+    return lat, lon, z, T3, u3, v3, rho3
+
+
+def rngdep_engine_units(lat, lon, z, T3, u3, v3, rho3):
+    """Fields as the reference loader holds them after reading zcuvd files
+    written with 17 digits: c, u, v in km/s (text value / 1000.0)."""
+    c = np.vectorize(lambda x: float(f"{x:.17g}"))(sound_speed(T3))
+    return (np.asarray(lat, dtype=np.float64), np.asarray(lon, dtype=np.float64),
+            np.ascontiguousarray(z, dtype=np.float64), c / 1000.0, u3 / 1000.0, v3 / 1000.0,
+            np.ascontiguousarray(rho3, dtype=np.float64))
+
+
+def fractal_topo(seed: int = 5, lat=None, lon=None, amp_km: float = 2.0, beta: float = 2.0):
+    """Seeded fractal terrain (spectral synthesis, power ~ |k|^-2 beta) on a
+    lat/lon grid (default 401 x 401 nodes every 0.05 deg, 30..50 N,
+    110..90 W), scaled to 0..amp_km.  Returns (lat, lon, zg[nlat, nlon])."""
+    lat = np.linspace(30.0, 50.0, 401) if lat is None else np.asarray(lat, dtype=np.float64)
+    lon = np.linspace(-110.0, -90.0, 401) if lon is None else np.asarray(lon, dtype=np.float64)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    w = np.fft.fft2(rng.standard_normal((lat.size, lon.size)))
+    kx = np.fft.fftfreq(lat.size)[:, None]
+    ky = np.fft.fftfreq(lon.size)[None, :]
+    k = np.hypot(kx, ky)
+    k[0, 0] = 1.0
+    f = np.real(np.fft.ifft2(w * k ** (-beta)))
+    f = (f - f.min()) / (f.max() - f.min()) * amp_km
+    return lat, lon, np.ascontiguousarray(f)

@@ -359,9 +359,8 @@ double go_gc_dist(double r0, double lat1, double lon1, double lat2, double lon2)
 /* Sutherland & Bass (2004) absorption in dB/km.  Spherical variant:
  * atmo_state.abs.sph.cpp:14-151; Cartesian variant (different rotational
  * and classical terms, pow(z,2) in the N profile): atmo_state.abs.cart.cpp:12-141. */
-double go_sb_alpha(go_atmo *a, double p0, double p1, double p2, double freq)
+double go_sb_alpha_at(double c, double rho, double z, int sph, double tweak, double freq)
 {
-    const int sph = (a->geom == GA_GEOM_SPH);
     const double gam = 1.4, R = 287.058;         /* atmo_state.*.strat.cpp:23-27 */
     double mu_o = 18.192e-6, T_o = 293.15, P_o = 101325, S = 117.0;
     double Cv_R[4] = {5.0 / 2.0, 5.0 / 2.0, 3.0, 3.0};
@@ -371,11 +370,10 @@ double go_sb_alpha(go_atmo *a, double p0, double p1, double p2, double freq)
     double beta_0, beta_1, beta_2, alpha_1, alpha_2, a_cl, a_rot, a_diff = 0.0, a_vib;
     double T_z, P_z, c_snd_z, A1, A2, B, C, D, E, F, G, H, I, J, K, L, ZZ, hu;
     double f_vib[4], C_R, A_max, Tr;
-    double z = sph ? (p0 - a->r0) : p2;
 
-    T_z = sq(go_c(a, p0, p1, p2) * 1.0e3) / (R * gam);
-    P_z = go_rho(a, p0, p1, p2) * 1.0e3 * sq(go_c(a, p0, p1, p2) * 1.0e3) / gam;
-    c_snd_z = go_c(a, p0, p1, p2) * 1.0e3;
+    T_z = sq(c * 1.0e3) / (R * gam);
+    P_z = rho * 1.0e3 * sq(c * 1.0e3) / gam;
+    c_snd_z = c * 1.0e3;
 
     mu = mu_o * sqrt(T_z / T_o) * ((1.0 + S / T_o) / (1.0 + S / T_z));
     nu = (8.0 * GO_PI * freq * mu) / (3.0 * P_z);
@@ -461,8 +459,17 @@ double go_sb_alpha(go_atmo *a, double p0, double p1, double p2, double freq)
         A_max = (X[m] * (GO_PI / 2) * C_R) / (Cp_R[m] * (Cv_R[m] + C_R));
         a_vib += (A_max / c_snd_z) * ((2 * (sq(freq)) / f_vib[m]) / (1.0 + sq(freq / f_vib[m])));
     }
-    if (sph) return (a_cl + a_rot + a_diff + a_vib) * a->tweak_abs * 8.685889e3;
-    return (a_cl * 1.003 + a_rot + a_vib) * a->tweak_abs * 8.685889e3;
+    if (sph) return (a_cl + a_rot + a_diff + a_vib) * tweak * 8.685889e3;
+    return (a_cl * 1.003 + a_rot + a_vib) * tweak * 8.685889e3;
+}
+
+/* SB_alpha at a point of the stratified medium: c and rho from the splines,
+ * z = r - r0 (sph) or z (3D). */
+double go_sb_alpha(go_atmo *a, double p0, double p1, double p2, double freq)
+{
+    const int sph = (a->geom == GA_GEOM_SPH);
+    double z = sph ? (p0 - a->r0) : p2;
+    return go_sb_alpha_at(go_c(a, p0, p1, p2), go_rho(a, p0, p1, p2), z, sph, a->tweak_abs, freq);
 }
 
 /* ======================================================================
@@ -1069,8 +1076,8 @@ int go_fan(go_atmo *a, const go_params *p, const go_prop *pr,
     double lat_src = 0, lon_src = 0, z_src;
 
     if (sph) {
-        lat_src = pr->src0 * GO_PI / 180.0;
-        lon_src = pr->src1 * GO_PI / 180.0;
+        lat_src = pr->src0 * (GO_PI / 180.0);                /* sph :311-312 */
+        lon_src = pr->src1 * (GO_PI / 180.0);
         z_src = pr->src_alt;
         double zg = ground(a) - a->r0;
         z_src = z_src > zg ? z_src : zg;                      /* sph :314 */

@@ -115,6 +115,9 @@ void   go_calc_uvw(go_atmo *a, double p0, double p1, double p2,
                    double *u, double *v, double *w, double du[3], double dv[3], double dw[3],
                    double ddu[6], double ddv[6], double ddw[6]);
 double go_sb_alpha(go_atmo *a, double p0, double p1, double p2, double freq);
+/* Sutherland-Bass [dB/km] from point values: c [km/s], rho [g/cm^3], height
+ * z [km]; sph selects the spherical source file's variant. */
+double go_sb_alpha_at(double c, double rho, double z, int sph, double tweak, double freq);
 double go_gc_dist(double r0, double lat1, double lon1, double lat2, double lon2);
 
 /* ---- ray equations (geoac.eqset.*.cpp) -------------------------------- */

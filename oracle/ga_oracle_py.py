@@ -220,3 +220,317 @@ class Ref:
         x, f, s = np.zeros(n), np.zeros(n), np.zeros(n)
         self.lib.ref_spline_tables(which, _ptr(x), _ptr(f), _ptr(s))
         return x, f, s
+
+
+# ---------------------------------------------------------------------------
+# Range-dependent spherical medium (ga_oracle_rd.c) and the reference's own
+# compiled range-dependent atmosphere (_ref/libref_sphrd.so).
+# ---------------------------------------------------------------------------
+def _rd_bind(L):
+    if getattr(L, "_rd_bound", False):
+        return
+    L.go_rd_new.restype = C.c_void_p
+    L.go_rd_new.argtypes = [_dp, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, _dp, _dp, _dp]
+    L.go_rd_delete.argtypes = [C.c_void_p]
+    L.go_rd_set_topo.argtypes = [C.c_void_p, _dp, C.c_int, _dp, C.c_int, _dp]
+    L.go_rd_set_topo.restype = C.c_int
+    L.go_rd_set_ground.argtypes = [C.c_void_p, C.c_double]
+    L.go_rd_h_config.argtypes = [C.c_void_p, C.c_int, C.c_double]
+    L.go_rd_h_limits.argtypes = [C.c_void_p, _dp]
+    L.go_rd_params.argtypes = [C.c_void_p, C.c_int, C.POINTER(Params)]
+    L.go_rd_centre.argtypes = [C.c_void_p, _dp, _dp]
+    for name in ("go_rd_c", "go_rd_rho", "go_rd_u", "go_rd_v", "go_rd_w", "go_rd_topo_z", "go_rd_bl_sc"):
+        f = getattr(L, name)
+        f.restype = C.c_double
+        f.argtypes = [C.c_void_p] + [C.c_double] * (2 if name == "go_rd_topo_z" else 3)
+    for name in ("go_rd_dc", "go_rd_du", "go_rd_dv", "go_rd_dw", "go_rd_bl_dsc"):
+        f = getattr(L, name)
+        f.restype = C.c_double
+        f.argtypes = [C.c_void_p] + [C.c_double] * 3 + [C.c_int]
+    L.go_rd_bl_ddsc.restype = C.c_double
+    L.go_rd_bl_ddsc.argtypes = [C.c_void_p] + [C.c_double] * 3 + [C.c_int, C.c_int]
+    L.go_rd_topo_dz.restype = C.c_double
+    L.go_rd_topo_dz.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int]
+    L.go_rd_topo_ddz.restype = C.c_double
+    L.go_rd_topo_ddz.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int, C.c_int]
+    L.go_rd_topo_dddz.restype = C.c_double
+    L.go_rd_topo_dddz.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int]
+    L.go_rd_calc_uvw.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, _dp, _dp, _dp,
+                                 _dp, _dp, _dp, _dp, _dp, _dp]
+    L.go_rd_sb_alpha.restype = C.c_double
+    L.go_rd_sb_alpha.argtypes = [C.c_void_p] + [C.c_double] * 4
+    L.go_rd_set_initial.argtypes = [C.c_void_p, C.POINTER(Params), C.c_void_p, _dp]
+    L.go_rd_rhs.argtypes = [C.c_void_p, C.POINTER(Params), _dp, _dp]
+    L.go_rd_rk4_step.restype = C.c_double
+    L.go_rd_rk4_step.argtypes = [C.c_void_p, C.POINTER(Params), _dp, _dp]
+    L.go_rd_set_refl.argtypes = [C.c_void_p, C.POINTER(Params), C.c_void_p, _dp, _dp, _dp]
+    for name in ("go_rd_jacobian",):
+        getattr(L, name).restype = C.c_double
+        getattr(L, name).argtypes = [C.c_void_p, _dp]
+    for name in ("go_rd_amp", "go_rd_eikonal_resid"):
+        getattr(L, name).restype = C.c_double
+        getattr(L, name).argtypes = [C.c_void_p, C.c_void_p, _dp]
+    L.go_rd_eikonal_deriv_resid.restype = C.c_double
+    L.go_rd_eikonal_deriv_resid.argtypes = [C.c_void_p, _dp]
+    L.go_rd_travel_time_seg.restype = C.c_double
+    L.go_rd_travel_time_seg.argtypes = [C.c_void_p, _dp, _dp]
+    L.go_rd_atten_seg.restype = C.c_double
+    L.go_rd_atten_seg.argtypes = [C.c_void_p, _dp, _dp, C.c_double]
+    L.go_rd_theta_grnd.restype = C.c_double
+    L.go_rd_theta_grnd.argtypes = [C.c_void_p] + [C.c_double] * 4
+    L.go_rd_fan.restype = C.c_int
+    L.go_rd_fan.argtypes = [C.c_void_p, C.POINTER(Params), C.POINTER(Prop), _dp, C.c_int,
+                            C.POINTER(Arrival), C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_longlong)]
+    L._rd_bound = True
+
+
+class RdMedium:
+    """One range-dependent spherical atmosphere of the oracle.
+
+    Fields are [nlat, nlon, nz] arrays in km/s (c, u, v) and g/cm^3 (rho);
+    node coordinates in degrees, z in km."""
+
+    def __init__(self, oracle: Oracle, lat_deg, lon_deg, z, c, u, v, rho):
+        self.o = oracle
+        L = oracle.lib
+        _rd_bind(L)
+        self._keep = [np.ascontiguousarray(a, dtype=np.float64) for a in (lat_deg, lon_deg, z, c, u, v, rho)]
+        la, lo, zz, cc, uu, vv, rr = self._keep
+        nlat, nlon, nz = la.size, lo.size, zz.size
+        for f in (cc, uu, vv, rr):
+            if f.size != nlat * nlon * nz:
+                raise ValueError("field shape must be [nlat, nlon, nz]")
+        self.h = L.go_rd_new(_ptr(la), nlat, _ptr(lo), nlon, _ptr(zz), nz, _ptr(cc), _ptr(uu), _ptr(vv), _ptr(rr))
+        if not self.h:
+            raise ValueError("oracle rejected the range-dependent atmosphere")
+
+    def __del__(self):
+        try:
+            self.o.lib.go_rd_delete(self.h)
+        except Exception:
+            pass
+
+    def set_topo(self, lat_deg, lon_deg, zg):
+        la = np.ascontiguousarray(lat_deg, dtype=np.float64)
+        lo = np.ascontiguousarray(lon_deg, dtype=np.float64)
+        zz = np.ascontiguousarray(zg, dtype=np.float64)
+        self._topo_keep = (la, lo, zz)
+        if self.o.lib.go_rd_set_topo(self.h, _ptr(la), la.size, _ptr(lo), lo.size, _ptr(zz)) != 0:
+            raise ValueError("oracle rejected the topography")
+
+    def set_ground(self, z0):
+        self.o.lib.go_rd_set_ground(self.h, z0)
+
+    def config(self, use_blw=False, tweak_abs=1.0):
+        self.o.lib.go_rd_h_config(self.h, int(use_blw), tweak_abs)
+
+    def limits(self):
+        out = np.zeros(8)
+        self.o.lib.go_rd_h_limits(self.h, _ptr(out))
+        return out
+
+    def centre(self):
+        a, b = C.c_double(), C.c_double()
+        self.o.lib.go_rd_centre(self.h, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def params(self, calc_amp=True, **kw):
+        p = Params()
+        self.o.lib.go_rd_params(self.h, int(calc_amp), C.byref(p))
+        for k, v in kw.items():
+            setattr(p, k, v)
+        return p
+
+    def point(self, r, t, p):
+        L, h = self.o.lib, self.h
+        out = [L.go_rd_c(h, r, t, p), L.go_rd_rho(h, r, t, p), L.go_rd_u(h, r, t, p),
+               L.go_rd_v(h, r, t, p), L.go_rd_w(h, r, t, p)]
+        for fn in (L.go_rd_dc, L.go_rd_du, L.go_rd_dv, L.go_rd_dw):
+            out += [fn(h, r, t, p, n) for n in range(3)]
+        return np.array(out)
+
+    def calc_uvw(self, r, t, p, second):
+        u, v, w = C.c_double(), C.c_double(), C.c_double()
+        d = [np.zeros(3) for _ in range(3)]
+        dd = [np.zeros(6) for _ in range(3)]
+        ddp = [_ptr(x) for x in dd] if second else [None, None, None]
+        self.o.lib.go_rd_calc_uvw(self.h, r, t, p, C.byref(u), C.byref(v), C.byref(w),
+                                  _ptr(d[0]), _ptr(d[1]), _ptr(d[2]), *ddp)
+        out = [u.value, v.value, w.value] + list(d[0]) + list(d[1]) + list(d[2])
+        if second:
+            out += list(dd[0]) + list(dd[1]) + list(dd[2])
+        return np.array(out)
+
+    def topo(self, lat, lon):
+        L, h = self.o.lib, self.h
+        return np.array([L.go_rd_topo_z(h, lat, lon), L.go_rd_topo_dz(h, lat, lon, 0), L.go_rd_topo_dz(h, lat, lon, 1),
+                         L.go_rd_topo_ddz(h, lat, lon, 0, 0), L.go_rd_topo_ddz(h, lat, lon, 0, 1),
+                         L.go_rd_topo_ddz(h, lat, lon, 1, 1), L.go_rd_topo_dddz(h, lat, lon, 0, 0, 0),
+                         L.go_rd_topo_dddz(h, lat, lon, 0, 0, 1), L.go_rd_topo_dddz(h, lat, lon, 0, 1, 1),
+                         L.go_rd_topo_dddz(h, lat, lon, 1, 1, 1)])
+
+    def bl(self, r, t, p):
+        L, h = self.o.lib, self.h
+        out = [L.go_rd_bl_sc(h, r, t, p)] + [L.go_rd_bl_dsc(h, r, t, p, n) for n in range(3)]
+        out += [L.go_rd_bl_ddsc(h, r, t, p, n, m) for n in range(3) for m in range(n, 3)]
+        return np.array(out)
+
+    def sb_alpha(self, r, t, p, freq):
+        return self.o.lib.go_rd_sb_alpha(self.h, r, t, p, freq)
+
+    def rhs(self, y, params):
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        f = np.zeros(18)
+        self.o.lib.go_rd_rhs(self.h, C.byref(params), _ptr(y), _ptr(f))
+        return f[:params.eq_cnt]
+
+    def rk4_step(self, y, params):
+        y = np.zeros(18) + np.pad(np.asarray(y, dtype=np.float64), (0, 18 - len(y)))
+        y1 = np.zeros(18)
+        ds = self.o.lib.go_rd_rk4_step(self.h, C.byref(params), _ptr(y), _ptr(y1))
+        return y1[:params.eq_cnt], ds
+
+    def set_initial(self, params, theta, phi, src_r, src_lat, src_lon):
+        s = Src((C.c_double * 3)(src_r, src_lat, src_lon), 0.0, 0.0, theta, phi)
+        y0 = np.zeros(18)
+        self.o.lib.go_rd_set_initial(self.h, C.byref(params), C.byref(s), _ptr(y0))
+        return y0[:params.eq_cnt], s
+
+    def set_refl(self, params, src, rk2, rk1):
+        a = np.ascontiguousarray(np.pad(rk2, (0, 18 - len(rk2))), dtype=np.float64)
+        b = np.ascontiguousarray(np.pad(rk1, (0, 18 - len(rk1))), dtype=np.float64)
+        y0 = np.zeros(18)
+        self.o.lib.go_rd_set_refl(self.h, C.byref(params), C.byref(src), _ptr(a), _ptr(b), _ptr(y0))
+        return y0[:params.eq_cnt]
+
+    def eikonal_resid(self, src, y):
+        y = np.ascontiguousarray(np.pad(y, (0, 18 - len(y))), dtype=np.float64)
+        return self.o.lib.go_rd_eikonal_resid(self.h, C.byref(src), _ptr(y))
+
+    def amp(self, src, y):
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        return self.o.lib.go_rd_amp(self.h, C.byref(src), _ptr(y))
+
+    def aux_resid(self, y):
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        return self.o.lib.go_rd_eikonal_deriv_resid(self.h, _ptr(y))
+
+    def trace(self, params, src, nsteps, y0):
+        """nsteps RK4 steps from y0 (no checks); returns rows [nsteps+1, neq]."""
+        rows = [np.asarray(y0, dtype=np.float64)]
+        for _ in range(nsteps):
+            y1, _ = self.rk4_step(rows[-1], params)
+            rows.append(y1)
+        return np.array(rows)
+
+    def theta_grnd(self, lat_src, lon_src, z_src, az_deg):
+        return self.o.lib.go_rd_theta_grnd(self.h, lat_src, lon_src, z_src, az_deg)
+
+    def fan(self, launches, params, bounces=2, freq=0.1, turn_ht_min=0.2, src=None):
+        """src = (lat_deg, lon_deg, alt_km); None = the region centre at the ground."""
+        if src is None:
+            la, lo = self.centre()
+            src = (la, lo, 0.0)
+        Lr = np.ascontiguousarray(launches, dtype=np.float64).reshape(-1, 2)
+        pr = Prop(bounces, freq, turn_ht_min, src[0], src[1], src[2])
+        cap = max(1, Lr.shape[0] * (bounces + 1))
+        out = (Arrival * cap)()
+        n_out = C.c_int(0)
+        steps = C.c_longlong(0)
+        rc = self.o.lib.go_rd_fan(self.h, C.byref(params), C.byref(pr), _ptr(Lr), Lr.shape[0], out, cap,
+                                  C.byref(n_out), C.byref(steps))
+        if rc != 0:
+            raise RuntimeError("oracle fan overflow")
+        return arrivals_to_array(out, n_out.value), steps.value
+
+
+def write_rd_files(d, lat_deg, lon_deg, z, c_ms, u_ms, v_ms, rho, topo=None, prefix="prof"):
+    """Write a range-dependent grid in the reference's input layout: node
+    files (one degree value per line), prefix{i*nlon+j}.met in format zcuvd
+    (z [km], c, u, v [m/s], rho [g/cm^3]) and optionally a topography file of
+    "lat lon z" rows, longitude fastest.  Values are written with 17
+    significant digits so the text parses back to the same doubles.
+    Returns (prefix_path, lat_path, lon_path, topo_path or None)."""
+    os.makedirs(d, exist_ok=True)
+    lat_p, lon_p = os.path.join(d, "lats.dat"), os.path.join(d, "lons.dat")
+    with open(lat_p, "w") as f:
+        f.write("".join("%.17g\n" % x for x in lat_deg))
+    with open(lon_p, "w") as f:
+        f.write("".join("%.17g\n" % x for x in lon_deg))
+    pre = os.path.join(d, prefix)
+    nlat, nlon = len(lat_deg), len(lon_deg)
+    for i in range(nlat):
+        for j in range(nlon):
+            rows = np.column_stack([z, c_ms[i, j], u_ms[i, j], v_ms[i, j], rho[i, j]])
+            with open("%s%d.met" % (pre, i * nlon + j), "w") as f:
+                f.write("".join(" ".join("%.17g" % x for x in r) + "\n" for r in rows))
+    topo_p = None
+    if topo is not None:
+        tlat, tlon, tz = topo
+        topo_p = os.path.join(d, "topo.dat")
+        with open(topo_p, "w") as f:
+            for i, la in enumerate(tlat):
+                f.write("".join("%.17g %.17g %.17g\n" % (la, lo, tz[i, j]) for j, lo in enumerate(tlon)))
+    return pre, lat_p, lon_p, topo_p
+
+
+class RefRd:
+    """The reference's own compiled range-dependent spherical atmosphere."""
+
+    def __init__(self):
+        path = os.path.join(HERE, "_ref", "libref_sphrd.so")
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        L = self.lib = C.CDLL(path)
+        L.ref_rd_load.argtypes = [C.c_char_p] * 5
+        L.ref_rd_load.restype = C.c_int
+        L.ref_rd_use_blw.argtypes = [C.c_int]
+        L.ref_rd_set_tweak.argtypes = [C.c_double]
+        L.ref_rd_limits.argtypes = [_dp]
+        L.ref_rd_point.argtypes = [C.c_double] * 3 + [_dp]
+        L.ref_rd_calc_uvw.argtypes = [C.c_double] * 3 + [C.c_int, _dp]
+        L.ref_rd_topo.argtypes = [C.c_double] * 2 + [_dp]
+        L.ref_rd_bl.argtypes = [C.c_double] * 3 + [_dp]
+        L.ref_rd_sb_alpha.argtypes = [C.c_double] * 4
+        L.ref_rd_sb_alpha.restype = C.c_double
+        self.loaded = False
+
+    def load(self, prefix, lats, lons, topo=None, fmt="zcuvd"):
+        if self.loaded:
+            self.lib.ref_rd_clear()
+        rc = self.lib.ref_rd_load(prefix.encode(), lats.encode(), lons.encode(),
+                                  (topo or "").encode(), fmt.encode())
+        if rc != 1:
+            raise RuntimeError("reference set_region failed")
+        self.loaded = True
+
+    def use_blw(self, on):
+        self.lib.ref_rd_use_blw(int(on))
+
+    def limits(self):
+        out = np.zeros(8)
+        self.lib.ref_rd_limits(_ptr(out))
+        return out
+
+    def point(self, r, t, p):
+        out = np.zeros(17)
+        self.lib.ref_rd_point(r, t, p, _ptr(out))
+        return out
+
+    def calc_uvw(self, r, t, p, second):
+        out = np.zeros(30)
+        self.lib.ref_rd_calc_uvw(r, t, p, int(second), _ptr(out))
+        return out[:30 if second else 12]
+
+    def topo(self, lat, lon):
+        out = np.zeros(10)
+        self.lib.ref_rd_topo(lat, lon, _ptr(out))
+        return out
+
+    def bl(self, r, t, p):
+        out = np.zeros(10)
+        self.lib.ref_rd_bl(r, t, p, _ptr(out))
+        return out
+
+    def sb_alpha(self, r, t, p, freq):
+        return self.lib.ref_rd_sb_alpha(r, t, p, freq)

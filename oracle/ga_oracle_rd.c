@@ -1,0 +1,1073 @@
+/* ga_oracle_rd.c -- CPU restatement of infraGA's spherical range-dependent
+ * propagation (TEST INFRASTRUCTURE ONLY: loaded by tests/ as the checker of
+ * the GPU range-dependent trace; never part of the product path).
+ *
+ * Operation order follows the reference source expression by expression so
+ * that, compiled with gcc -O2 and no FMA contraction, the medium functions
+ * are bit-identical to the reference's own compiled atmo_state.sph.rngdep.cpp
+ * (tests/test_oracle_rd.py checks that against oracle/_ref).  File:line
+ * citations are relative to the infraGA source tree.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "ga_oracle_rd.h"
+
+#define RD_PI 3.141592653589793238462643   /* geoac.params.sph.cpp:9 */
+
+static double sq(double x) { return pow(x, 2); }
+static double in_iv(double x, double lo, double hi)   /* interpolation.cpp:52-54 */
+{
+    double m = x > lo ? x : lo;
+    return m < hi ? m : hi;
+}
+static double dmax(double a, double b) { return a < b ? b : a; }   /* std::max */
+static double dmin(double a, double b) { return b < a ? b : a; }   /* std::min */
+
+/* ======================================================================
+ * Set-up: atmo_io.sph.rngdep.cpp:27-43 (set_limits), 61-200 / 203-366
+ * (set_region), atmo_state.sph.rngdep.cpp:42-55 (set_bndlyr)
+ * ==================================================================== */
+static void set_limits(go_rd *a)
+{
+    double buffer = 1.0e-3 * (RD_PI / 180.0);
+    const go_h3d *c = &a->c;
+    a->lat_min = c->x[0] + buffer;  a->lat_max = c->x[c->nx - 1] - buffer;
+    a->lon_min = c->y[0] + buffer;  a->lon_max = c->y[c->ny - 1] - buffer;
+    if (a->is_topo) {
+        const go_s2d *t = &a->topo;
+        a->lat_min = dmax(a->lat_min, t->x[0] + buffer); a->lat_max = dmin(a->lat_max, t->x[t->nx - 1] - buffer);
+        a->lon_min = dmax(a->lon_min, t->y[0] + buffer); a->lon_max = dmin(a->lon_max, t->y[t->ny - 1] - buffer);
+    }
+    a->z0 = c->z[0];
+    a->alt_max = c->z[c->nz - 1];
+}
+
+static void set_bndlyr(go_rd *a)
+{
+    if (a->is_topo) {
+        a->z_max = 0.0;
+        for (int i = 0; i < a->topo.nx; i++)
+            for (int j = 0; j < a->topo.ny; j++) a->z_max = dmax(a->z_max, a->topo.f[i * a->topo.ny + j]);
+        a->z_bndlyr = a->z_max + 2.0;
+        a->z_max += 0.01;
+    } else {
+        a->z_max = a->z0;
+        a->z_bndlyr = a->z_max + 2.0;
+    }
+}
+
+int go_rd_init(go_rd *a, const double *lat_deg, int nlat, const double *lon_deg, int nlon,
+               const double *z, int nz, const double *c, const double *u, const double *v,
+               const double *rho)
+{
+    memset(a, 0, sizeof(*a));
+    if (nlat < 2 || nlon < 2 || nz < 3) return -1;
+    double *x = (double *)malloc(sizeof(double) * (size_t)nlat);
+    double *y = (double *)malloc(sizeof(double) * (size_t)nlon);
+    for (int i = 0; i < nlat; i++) { x[i] = lat_deg[i]; x[i] *= RD_PI / 180.0; }   /* :98 */
+    for (int j = 0; j < nlon; j++) { y[j] = lon_deg[j]; y[j] *= RD_PI / 180.0; }   /* :109 */
+    int rc = go_h3d_init(&a->c, x, nlat, y, nlon, z, nz, c);
+    rc |= go_h3d_init(&a->u, x, nlat, y, nlon, z, nz, u);
+    rc |= go_h3d_init(&a->v, x, nlat, y, nlon, z, nz, v);
+    rc |= go_h3d_init(&a->rho, x, nlat, y, nlon, z, nz, rho);
+    free(x);
+    free(y);
+    if (rc) return -1;
+    a->r0 = 6370.0;                /* globe.cpp:10 */
+    a->bl_param = 0.1;             /* atmo_state.sph.rngdep.cpp:95 */
+    a->vert_wind_grad = 0.5;       /* :96 */
+    a->tweak_abs = 1.0;            /* main.infraga.sph.rngdep.cpp:224 */
+    a->is_topo = 0;
+    a->use_blw = 0;
+    set_limits(a);
+    set_bndlyr(a);
+    return 0;
+}
+
+int go_rd_set_topo(go_rd *a, const double *lat_deg, int nlat, const double *lon_deg, int nlon,
+                   const double *zg)
+{
+    double *x = (double *)malloc(sizeof(double) * (size_t)nlat);
+    double *y = (double *)malloc(sizeof(double) * (size_t)nlon);
+    for (int i = 0; i < nlat; i++) { x[i] = lat_deg[i]; x[i] *= RD_PI / 180.0; }   /* :231 */
+    for (int j = 0; j < nlon; j++) { y[j] = lon_deg[j]; y[j] *= RD_PI / 180.0; }   /* :232 */
+    if (a->is_topo) go_s2d_free(&a->topo);
+    int rc = go_s2d_init(&a->topo, x, nlat, y, nlon, zg);
+    free(x);
+    free(y);
+    if (rc) return -1;
+    a->is_topo = 1;
+    set_limits(a);
+    set_bndlyr(a);
+    return 0;
+}
+
+void go_rd_set_ground(go_rd *a, double z0)
+{
+    if (a->is_topo) return;        /* "cannot adjust ground elevation with topography" */
+    a->z0 = z0;
+    a->z_max = z0;
+    a->z_bndlyr = z0 + 2.0;
+}
+
+void go_rd_free(go_rd *a)
+{
+    go_h3d_free(&a->c); go_h3d_free(&a->u); go_h3d_free(&a->v); go_h3d_free(&a->rho);
+    if (a->is_topo) go_s2d_free(&a->topo);
+    memset(a, 0, sizeof(*a));
+}
+
+void go_rd_params(const go_rd *a, int calc_amp, go_params *p)
+{
+    memset(p, 0, sizeof(*p));
+    p->calc_amp = calc_amp;
+    p->eq_cnt = calc_amp ? 18 : 6;
+    p->s_max = 1000.0; p->ds_min = 0.001; p->ds_max = 0.05;
+    p->alt_max = a->alt_max;
+    p->rng_max = 1000.0;
+    p->time_max = 10.0;
+    p->lat_min = a->lat_min; p->lat_max = a->lat_max;
+    p->lon_min = a->lon_min; p->lon_max = a->lon_max;
+}
+
+void go_rd_centre(const go_rd *a, double *lat_deg, double *lon_deg)
+{
+    *lat_deg = (a->lat_max + a->lat_min) / 2.0 * (180.0 / RD_PI);
+    *lon_deg = (a->lon_max + a->lon_min) / 2.0 * (180.0 / RD_PI);
+}
+
+/* ======================================================================
+ * Topography and boundary layer: atmo_state.sph.rngdep.cpp:57-133
+ * ==================================================================== */
+static double tx(go_rd *a, double lat) { return in_iv(lat, a->topo.x[0], a->topo.x[a->topo.nx - 1]); }
+static double ty(go_rd *a, double lon) { return in_iv(lon, a->topo.y[0], a->topo.y[a->topo.ny - 1]); }
+
+double go_rd_topo_z(go_rd *a, double lat, double lon)
+{
+    double result = a->r0;
+    if (a->is_topo) result += go_s2d_f(&a->topo, tx(a, lat), ty(a, lon));
+    else result += a->z0;
+    return result;
+}
+double go_rd_topo_dz(go_rd *a, double lat, double lon, int n)
+{
+    double result = 0.0;
+    if (a->is_topo) result += go_s2d_df(&a->topo, tx(a, lat), ty(a, lon), n);
+    return result;
+}
+double go_rd_topo_ddz(go_rd *a, double lat, double lon, int n1, int n2)
+{
+    double result = 0.0;
+    if (a->is_topo) result += go_s2d_ddf(&a->topo, tx(a, lat), ty(a, lon), n1, n2);
+    return result;
+}
+double go_rd_topo_dddz(go_rd *a, double lat, double lon, int n1, int n2, int n3)
+{
+    if (a->is_topo) return go_s2d_dddf(&a->topo, tx(a, lat), ty(a, lon), n1, n2, n3);
+    return 0.0;
+}
+
+double go_rd_bl_sc(go_rd *a, double r, double lat, double lon)
+{
+    double X, result = 1.0;
+    if (r - a->r0 < a->z_bndlyr) {
+        X = exp(-(r - go_rd_topo_z(a, lat, lon)) / a->bl_param);
+        result = (1.0 - X) / (1.0 + X);
+    }
+    return result;
+}
+double go_rd_bl_dsc(go_rd *a, double r, double lat, double lon, int n)
+{
+    double X, dX, result = 0.0;
+    if (r - a->r0 < a->z_bndlyr) {
+        X = exp(-(r - go_rd_topo_z(a, lat, lon)) / a->bl_param);
+        if (n > 0) dX = X / a->bl_param * go_rd_topo_dz(a, lat, lon, n - 1);
+        else dX = -X / a->bl_param;
+        result = -2.0 * dX / sq(1.0 + X);
+    }
+    return result;
+}
+double go_rd_bl_ddsc(go_rd *a, double r, double lat, double lon, int n1, int n2)
+{
+    double X, dX1, dX2, ddX, result = 0.0, bp = a->bl_param;
+    if (r - a->r0 < a->z_bndlyr) {
+        X = exp(-(r - go_rd_topo_z(a, lat, lon)) / bp);
+        if (n1 > 0) dX1 = X / bp * go_rd_topo_dz(a, lat, lon, n1 - 1); else dX1 = -X / bp;
+        if (n2 > 0) dX2 = X / bp * go_rd_topo_dz(a, lat, lon, n2 - 1); else dX2 = -X / bp;
+        if (n1 > 0 && n2 > 0)
+            ddX = 1.0 / bp * (go_rd_topo_ddz(a, lat, lon, n1 - 1, n2 - 1)
+                              + 1.0 / bp * go_rd_topo_dz(a, lat, lon, n1 - 1) * go_rd_topo_dz(a, lat, lon, n2 - 1)) * X;
+        else if (n1 > 0 && n2 == 0) ddX = -X / sq(bp) * go_rd_topo_dz(a, lat, lon, n1 - 1);
+        else if (n1 == 0 && n2 > 0) ddX = -X / sq(bp) * go_rd_topo_dz(a, lat, lon, n2 - 1);
+        else ddX = X / sq(bp);
+        result = -2.0 * (ddX / sq(1.0 + X) - 2.0 * dX1 * dX2 / pow(1.0 + X, 3));
+    }
+    return result;
+}
+
+/* ======================================================================
+ * Medium point functions: atmo_state.sph.rngdep.cpp:149-403.  The splines
+ * run over (lat, lon, z); derivative index n of (r, lat, lon) maps to spline
+ * index 2 for n = 0 and n - 1 otherwise.
+ * ==================================================================== */
+static void ev(go_rd *a, const go_h3d *s, double r, double lat, double lon, double *xe, double *ye, double *ze)
+{
+    *xe = in_iv(lat, s->x[0], s->x[s->nx - 1]);
+    *ye = in_iv(lon, s->y[0], s->y[s->ny - 1]);
+    *ze = in_iv(r - a->r0, s->z[0], s->z[s->nz - 1]);
+}
+static int nmap(int n) { return n == 0 ? 2 : n - 1; }
+
+double go_rd_rho(go_rd *a, double r, double lat, double lon)
+{
+    double x, y, z;
+    ev(a, &a->rho, r, lat, lon, &x, &y, &z);
+    return go_h3d_f(&a->rho, x, y, z);
+}
+double go_rd_c(go_rd *a, double r, double lat, double lon)
+{
+    double x, y, z;
+    ev(a, &a->c, r, lat, lon, &x, &y, &z);
+    return go_h3d_f(&a->c, x, y, z);
+}
+double go_rd_dc(go_rd *a, double r, double lat, double lon, int n)
+{
+    double x, y, z;
+    ev(a, &a->c, r, lat, lon, &x, &y, &z);
+    return go_h3d_df(&a->c, x, y, z, nmap(n));
+}
+static double wind(go_rd *a, go_h3d *s, double r, double lat, double lon)
+{
+    double x, y, z;
+    ev(a, s, r, lat, lon, &x, &y, &z);
+    return go_h3d_f(s, x, y, z) * go_rd_bl_sc(a, r, lat, lon);
+}
+static double dwind(go_rd *a, go_h3d *s, double r, double lat, double lon, int n)
+{
+    double x, y, z, result;
+    ev(a, s, r, lat, lon, &x, &y, &z);
+    result = go_h3d_df(s, x, y, z, nmap(n)) * go_rd_bl_sc(a, r, lat, lon);
+    result += go_h3d_f(s, x, y, z) * go_rd_bl_dsc(a, r, lat, lon, n);
+    return result;
+}
+double go_rd_u(go_rd *a, double r, double lat, double lon) { return wind(a, &a->u, r, lat, lon); }
+double go_rd_v(go_rd *a, double r, double lat, double lon) { return wind(a, &a->v, r, lat, lon); }
+double go_rd_du(go_rd *a, double r, double lat, double lon, int n) { return dwind(a, &a->u, r, lat, lon, n); }
+double go_rd_dv(go_rd *a, double r, double lat, double lon, int n) { return dwind(a, &a->v, r, lat, lon, n); }
+
+static int blw_on(go_rd *a, double r) { return a->is_topo && (r - a->r0 < a->z_bndlyr) && a->use_blw; }
+
+/* atmo::w, :308-338 */
+double go_rd_w(go_rd *a, double r, double lat, double lon)
+{
+    double x, y, z, zg, dzg[2], X, S, dS[2], w0, sg_ratio, result = 0.0, bp = a->bl_param;
+    if (blw_on(a, r)) {
+        ev(a, &a->v, r, lat, lon, &x, &y, &z);
+        zg = go_rd_topo_z(a, lat, lon);
+        dzg[0] = go_rd_topo_dz(a, lat, lon, 0);
+        dzg[1] = go_rd_topo_dz(a, lat, lon, 1);
+        sg_ratio = bp / a->vert_wind_grad;
+        X = exp(-(r - zg) / bp);
+        S = (1.0 - X) / (1.0 + X);
+        dS[0] = -2.0 / bp * (X / sq(1.0 + X)) * dzg[0];
+        dS[1] = -2.0 / bp * (X / sq(1.0 + X)) * dzg[1];
+        w0 = go_h3d_f(&a->v, x, y, z) * dS[0];
+        w0 += go_h3d_f(&a->u, x, y, z) * dS[1] / cos(lat);
+        w0 *= -2.0 * bp / (a->r0 + zg);
+        result = S * X * w0;
+        result /= (X + (sg_ratio / 2.0) * (1.0 - sq(X)));
+    }
+    return result;
+}
+
+/* atmo::dw, :340-403 */
+double go_rd_dw(go_rd *a, double r, double lat, double lon, int n)
+{
+    double x, y, z, X, dX[3], ddX[2], S, dS[3], ddS[2], N, D, sg_ratio, result = 0.0, bp = a->bl_param;
+    double zg, dzg[2], u0, v0, w0, dw0, dN, dD;
+    if (blw_on(a, r)) {
+        ev(a, &a->v, r, lat, lon, &x, &y, &z);
+        sg_ratio = bp / a->vert_wind_grad;
+        zg = go_rd_topo_z(a, lat, lon);
+        dzg[0] = go_rd_topo_dz(a, lat, lon, 0);
+        dzg[1] = go_rd_topo_dz(a, lat, lon, 1);
+        u0 = go_h3d_f(&a->u, x, y, z);
+        v0 = go_h3d_f(&a->v, x, y, z);
+        X = exp(-(r - zg) / bp);
+        for (int j = 0; j < 3; j++) {
+            if (j > 0) dX[j] = X / bp * dzg[j - 1];
+            else dX[j] = -X / bp;
+        }
+        S = (1.0 - X) / (1.0 + X);
+        for (int j = 0; j < 3; j++) dS[j] = -2.0 * dX[j] / sq(1.0 + X);
+        if (n > 0) {
+            ddX[0] = X / bp * (go_rd_topo_ddz(a, lat, lon, 0, n - 1) + 1.0 / bp * dzg[0] * dzg[n - 1]);
+            ddX[1] = X / bp * (go_rd_topo_ddz(a, lat, lon, 1, n - 1) + 1.0 / bp * dzg[1] * dzg[n - 1]);
+        } else {
+            ddX[0] = -X / sq(bp) * dzg[0];
+            ddX[1] = -X / sq(bp) * dzg[1];
+        }
+        ddS[0] = -2.0 * (ddX[0] / sq(1.0 + X) - 2.0 * dX[1] * dX[n] / pow(1.0 + X, 3));
+        ddS[1] = -2.0 * (ddX[1] / sq(1.0 + X) - 2.0 * dX[2] * dX[n] / pow(1.0 + X, 3));
+        w0 = v0 * dS[1] + u0 * dS[2] / cos(lat);
+        w0 *= -2.0 * bp / (a->r0 + zg);
+        N = S * X * w0;
+        D = X + (sg_ratio / 2.0) * (1.0 - sq(X));
+        dw0 = v0 * ddS[0] + u0 * ddS[1] / cos(lat);
+        if (n == 0) {
+            dw0 += go_h3d_df(&a->v, x, y, z, 2) * dS[1];
+            dw0 += go_h3d_df(&a->u, x, y, z, 2) * dS[2] / cos(lat);
+        } else if (n == 1) {
+            dw0 += u0 * dS[2] * sin(lat) / sq(cos(lat));
+        }
+        dw0 *= -2.0 * bp / (a->r0 + zg);
+        dN = (dS[n] * X + S * dX[n]) * w0 + S * X * dw0;
+        dD = (1.0 - sg_ratio * X) * dX[n];
+        result = (dN * D - N * dD) / sq(D);
+    }
+    return result;
+}
+
+/* atmo::calc_uvw, both overloads: :584-691 (first derivatives) and :693-882
+ * (second derivatives).  The terrain is evaluated at the unclamped (lat, lon)
+ * and its first/second derivatives at the clamped ones in the first overload,
+ * as the reference does; inside the region both agree. */
+void go_rd_calc_uvw(go_rd *a, double r, double lat, double lon, double *u, double *v, double *w,
+                    double du[3], double dv[3], double dw[3], double ddu[6], double ddv[6], double ddw[6])
+{
+    const double bp = a->bl_param;
+    double lat_e, lon_e, z_e, zg, dzg[2], ddzg[2][2], dddzg[2][2][2];
+    double u0, v0, du0[3], dv0[3], ddu0[6], ddv0[6], t3[3], t6[6];
+    double X, dX[3], ddX[3][3], dddX[2][3][3], sc0, dsc0[3], ddsc0[3][3], dddsc0[2][3][3];
+    double w0, sg_ratio, N, D;
+    const int second = (ddu != NULL);
+
+    ev(a, &a->v, r, lat, lon, &lat_e, &lon_e, &z_e);
+    memset(dddzg, 0, sizeof(dddzg));
+    if (a->is_topo && (r - a->r0 < a->z_bndlyr)) {
+        if (!second) {
+            zg = go_s2d_f(&a->topo, lat, lon);
+            zg += a->r0;
+            for (int n = 0; n < 2; n++) {
+                dzg[n] = go_rd_topo_dz(a, lat, lon, n);
+                for (int m = n; m < 2; m++) {
+                    ddzg[n][m] = go_rd_topo_ddz(a, lat, lon, n, m);
+                    ddzg[m][n] = ddzg[n][m];
+                }
+            }
+        } else {
+            zg = go_s2d_f(&a->topo, lat, lon);
+            zg += a->r0;
+            dzg[0] = go_s2d_df(&a->topo, lat, lon, 0);
+            dzg[1] = go_s2d_df(&a->topo, lat, lon, 1);
+            for (int n = 0; n < 2; n++)
+                for (int m = 0; m < 2; m++) {
+                    ddzg[n][m] = go_s2d_ddf(&a->topo, lat, lon, n, m);
+                    for (int k = 0; k < 2; k++) dddzg[n][m][k] = go_s2d_dddf(&a->topo, lat, lon, n, m, k);
+                }
+        }
+    } else {
+        zg = go_rd_topo_z(a, lat, lon);
+        for (int n = 0; n < 2; n++) {
+            dzg[n] = 0.0;
+            for (int m = 0; m < 2; m++) ddzg[n][m] = 0.0;
+        }
+    }
+
+    go_h3d_all(&a->u, lat_e, lon_e, z_e, &u0, t3, t6);
+    du0[1] = t3[0]; du0[2] = t3[1]; du0[0] = t3[2];
+    ddu0[1] = t6[0]; ddu0[2] = t6[1]; ddu0[0] = t6[2]; ddu0[5] = t6[3]; ddu0[3] = t6[4]; ddu0[4] = t6[5];
+    go_h3d_all(&a->v, lat_e, lon_e, z_e, &v0, t3, t6);
+    dv0[1] = t3[0]; dv0[2] = t3[1]; dv0[0] = t3[2];
+    ddv0[1] = t6[0]; ddv0[2] = t6[1]; ddv0[0] = t6[2]; ddv0[5] = t6[3]; ddv0[3] = t6[4]; ddv0[4] = t6[5];
+
+    X = exp(-(r - zg) / bp);
+    if (!second) {
+        for (int n = 0; n < 3; n++) {
+            if (n > 0) dX[n] = X / bp * dzg[n - 1];
+            else dX[n] = -X / bp;
+            for (int m = 0; m < 3; m++) {
+                if ((n > 0) && (m > 0)) ddX[n][m] = X / bp * (ddzg[n - 1][m - 1] + 1.0 / bp * dzg[n - 1] * dzg[m - 1]);
+                else if ((n == 0) && (m > 0)) ddX[n][m] = -X / sq(bp) * dzg[m - 1];
+                else if ((n > 0) && (m == 0)) ddX[n][m] = -X / sq(bp) * dzg[n - 1];
+                else ddX[n][m] = X / sq(bp);
+            }
+        }
+    } else {
+        for (int n = 1; n < 3; n++) {
+            dX[n] = X / bp * dzg[n - 1];
+            for (int m = 0; m < 3; m++) {
+                if (m == 0) {
+                    ddX[n][m] = -X / sq(bp) * dzg[n - 1];
+                    ddX[m][n] = ddX[n][m];
+                    for (int k = 0; k < 2; k++) {
+                        dddX[k][n][m] = ddzg[n - 1][k] + dzg[n - 1] * dzg[k] / bp;
+                        dddX[k][n][m] *= -X / sq(bp);
+                        dddX[k][m][n] = dddX[k][n][m];
+                    }
+                } else {
+                    ddX[n][m] = ddzg[n - 1][m - 1] + dzg[n - 1] * dzg[m - 1] / bp;
+                    ddX[n][m] *= X / bp;
+                    ddX[m][n] = ddX[n][m];
+                    for (int k = 0; k < 2; k++) {
+                        dddX[k][n][m] = dddzg[k][n - 1][m - 1];
+                        dddX[k][n][m] += (ddzg[n - 1][m - 1] * dzg[k] + ddzg[k][m - 1] * dzg[n - 1] + ddzg[k][n - 1] * dzg[m - 1]) / bp;
+                        dddX[k][n][m] += dzg[k] * dzg[n - 1] * dzg[m - 1] / sq(bp);
+                        dddX[k][n][m] *= X / bp;
+                        dddX[k][m][n] = dddX[k][n][m];
+                    }
+                }
+            }
+        }
+        dX[0] = -X / bp;
+        ddX[0][0] = X / sq(bp);
+        dddX[0][0][0] = X / pow(bp, 3) * dzg[0];
+        dddX[1][0][0] = X / pow(bp, 3) * dzg[1];
+    }
+
+    sc0 = (1.0 - X) / (1.0 + X);
+    for (int n = 0; n < 3; n++) {
+        dsc0[n] = -2.0 * dX[n] / sq(1.0 + X);
+        for (int m = 0; m < 3; m++) {
+            ddsc0[n][m] = -2.0 * (ddX[n][m] / sq(1.0 + X) - 2.0 * dX[n] * dX[m] / pow(1.0 + X, 3));
+            if (second)
+                for (int k = 0; k < 2; k++) {
+                    dddsc0[k][n][m] = dddX[k][n][m] / sq(1.0 + X);
+                    dddsc0[k][n][m] -= 2.0 * (ddX[n][m] * dX[k + 1] + ddX[n][k + 1] * dX[m] + ddX[k + 1][m] * dX[n]) / pow(1.0 + X, 3);
+                    dddsc0[k][n][m] += 6.0 * dX[n] * dX[m] * dX[k + 1] / pow(1.0 + X, 4);
+                    dddsc0[k][n][m] *= -2.0;
+                }
+        }
+    }
+
+    *u = u0 * sc0;
+    *v = v0 * sc0;
+    for (int n = 0; n < 3; n++) {
+        du[n] = du0[n] * sc0 + u0 * dsc0[n];
+        dv[n] = dv0[n] * sc0 + v0 * dsc0[n];
+    }
+    if (second) {
+        for (int n = 0; n < 3; n++) {
+            ddu[n] = ddu0[n] * sc0 + 2.0 * du0[n] * dsc0[n] + u0 * ddsc0[n][n];
+            ddv[n] = ddv0[n] * sc0 + 2.0 * dv0[n] * dsc0[n] + v0 * ddsc0[n][n];
+        }
+        ddu[3] = ddu0[3] * sc0 + du0[0] * dsc0[1] + du0[1] * dsc0[0] + u0 * ddsc0[0][1];
+        ddu[4] = ddu0[4] * sc0 + du0[0] * dsc0[2] + du0[2] * dsc0[0] + u0 * ddsc0[0][2];
+        ddu[5] = ddu0[5] * sc0 + du0[1] * dsc0[2] + du0[2] * dsc0[1] + u0 * ddsc0[1][2];
+        ddv[3] = ddv0[3] * sc0 + dv0[0] * dsc0[1] + dv0[1] * dsc0[0] + v0 * ddsc0[0][1];
+        ddv[4] = ddv0[4] * sc0 + dv0[0] * dsc0[2] + dv0[2] * dsc0[0] + v0 * ddsc0[0][2];
+        ddv[5] = ddv0[5] * sc0 + dv0[1] * dsc0[2] + dv0[2] * dsc0[1] + v0 * ddsc0[1][2];
+    }
+
+    if (a->is_topo && (r - a->r0 < a->z_bndlyr) && a->use_blw) {
+        w0 = v0 * dsc0[1] + u0 * dsc0[2] / cos(lat);
+        w0 *= -2.0 * bp / (a->r0 + zg);
+        sg_ratio = bp / a->vert_wind_grad;
+        N = sc0 * X * w0;
+        D = X + (sg_ratio / 2.0) * (1.0 - sq(X));
+        *w = N / D;
+        if (!second) {
+            for (int n = 0; n < 3; n++) {
+                double dw0, dN, dD;
+                if (n == 0) {
+                    dw0 = dv0[0] * dsc0[1] + v0 * ddsc0[1][0];
+                    dw0 += (du0[0] * dsc0[2] + u0 * ddsc0[2][0]) / cos(lat);
+                } else if (n == 1) {
+                    dw0 = v0 * ddsc0[1][1];
+                    dw0 += u0 / cos(lat) * (ddsc0[1][2] + dsc0[2] * tan(lat));
+                } else {
+                    dw0 = v0 * ddsc0[1][2];
+                    dw0 += u0 * ddsc0[2][2] / cos(lat);
+                }
+                dw0 *= -2.0 * bp / (a->r0 + zg);
+                dN = (dsc0[n] * X + sc0 * dX[n]) * w0 + sc0 * X * dw0;
+                dD = (1.0 - sg_ratio * X) * dX[n];
+                dw[n] = (dN * D - N * dD) / sq(D);
+            }
+        } else {
+            double dw0[3], dN[3], dD[3], ddw0, ddN, ddD;
+            for (int n = 0; n < 3; n++) {
+                dw0[n] = v0 * ddsc0[1][n] + u0 * ddsc0[2][n] / cos(lat);
+                if (n == 0) dw0[n] += dv0[0] * dsc0[1] + du0[0] * dsc0[2] / cos(lat);
+                else if (n == 1) dw0[n] += u0 * dsc0[2] * sin(lat) / sq(cos(lat));
+                dw0[n] *= -2.0 * bp / (a->r0 + zg);
+                dN[n] = (dsc0[n] * X + sc0 * dX[n]) * w0 + sc0 * X * dw0[n];
+                dD[n] = (1.0 - sg_ratio * X) * dX[n];
+                dw[n] = (dN[n] * D - N * dD[n]) / sq(D);
+
+                ddw0 = dddsc0[0][n][n] * v0 + dddsc0[1][n][n] * u0 / cos(lat);
+                if (n == 0) {
+                    ddw0 += 2.0 * ddsc0[1][0] * dv0[0] + dsc0[1] * ddv0[0];
+                    ddw0 += (2.0 * ddsc0[2][0] * du0[0] + dsc0[2] * ddu0[0]) / cos(lat);
+                } else if (n == 1) {
+                    ddw0 += 2.0 * sin(lat) / sq(cos(lat)) * ddsc0[1][2] * u0;
+                    ddw0 += (sq(sin(lat)) + 1.0) / pow(cos(lat), 3) * dsc0[2] * u0;
+                }
+                ddw0 *= -2.0 * bp / (a->r0 + zg);
+                ddN = ddsc0[n][n] * X * w0 + sc0 * ddX[n][n] * w0 + sc0 * X * ddw0;
+                ddN += 2.0 * (dsc0[n] * dX[n] * w0 + dsc0[n] * X * dw0[n] + sc0 * dX[n] * dw0[n]);
+                ddD = -sg_ratio * sq(dX[n]) + (1.0 - sg_ratio * X) * ddX[n][n];
+                ddw[n] = ((ddN * D - 2.0 * dN[n] * dD[n] - N * ddD) * sq(D) + 2.0 * N * D * sq(dD[n])) / pow(D, 4);
+            }
+            /* ddw[3..5]: r-lat, r-lon, lat-lon (:848-876) */
+            ddw0 = dddsc0[0][0][1] * v0 + ddsc0[1][1] * dv0[0] + (dddsc0[1][0][1] * u0 + ddsc0[1][2] * du0[0]) / cos(lat);
+            ddw0 += sin(lat) / sq(cos(lat)) * (ddsc0[0][2] * u0 + dsc0[2] * du0[0]);
+            ddw0 *= -2.0 * bp / (a->r0 + zg);
+            ddN = ddsc0[0][1] * X * w0 + sc0 * ddX[0][1] * w0 + sc0 * X * ddw0;
+            ddN += dsc0[0] * dX[1] * w0 + dsc0[0] * X * dw0[1] + sc0 * dX[0] * dw0[1];
+            ddN += dsc0[1] * dX[0] * w0 + dsc0[1] * X * dw0[0] + sc0 * dX[1] * dw0[0];
+            ddD = -sg_ratio * dX[0] * dX[1] + (1.0 - sg_ratio * X) * ddX[0][1];
+            ddw[3] = ((ddN * D - dN[0] * dD[1] - dN[1] * dD[0] - N * ddD) * sq(D) + 2.0 * N * D * dD[0] * dD[1]) / pow(D, 4);
+
+            ddw0 = dddsc0[0][0][2] * v0 + ddsc0[1][2] * dv0[0] + (dddsc0[1][0][2] * u0 + ddsc0[2][2] * du0[0]) / cos(lat);
+            ddw0 *= -2.0 * bp / (a->r0 + zg);
+            ddN = ddsc0[0][2] * X * w0 + sc0 * ddX[0][2] * w0 + sc0 * X * ddw0;
+            ddN += dsc0[0] * dX[2] * w0 + dsc0[0] * X * dw0[2] + sc0 * dX[0] * dw0[2];
+            ddN += dsc0[2] * dX[0] * w0 + dsc0[2] * X * dw0[0] + sc0 * dX[2] * dw0[0];
+            ddD = -sg_ratio * dX[0] * dX[2] + (1.0 - sg_ratio * X) * ddX[0][2];
+            ddw[4] = ((ddN * D - dN[0] * dD[2] - dN[2] * dD[0] - N * ddD) * sq(D) + 2.0 * N * D * dD[0] * dD[2]) / pow(D, 4);
+
+            ddw0 = dddsc0[0][1][2] * v0 + dddsc0[1][1][2] * u0 / cos(lat);
+            ddw0 += sin(lat) / sq(cos(lat)) * ddsc0[2][2] * u0;
+            ddw0 *= -2.0 * bp / (a->r0 + zg);
+            ddN = ddsc0[1][2] * X * w0 + sc0 * ddX[1][2] * w0 + sc0 * X * ddw0;
+            ddN += dsc0[1] * dX[2] * w0 + dsc0[1] * X * dw0[2] + sc0 * dX[1] * dw0[2];
+            ddN += dsc0[2] * dX[1] * w0 + dsc0[2] * X * dw0[1] + sc0 * dX[2] * dw0[1];
+            ddD = -sg_ratio * dX[1] * dX[2] + (1.0 - sg_ratio * X) * ddX[1][2];
+            ddw[5] = ((ddN * D - dN[1] * dD[2] - dN[2] * dD[1] - N * ddD) * sq(D) + 2.0 * N * D * dD[1] * dD[2]) / pow(D, 4);
+        }
+    } else {
+        *w = 0.0;
+        for (int n = 0; n < 3; n++) dw[n] = 0.0;
+        if (second) for (int n = 0; n < 6; n++) ddw[n] = 0.0;
+    }
+}
+
+/* atmo::SB_alpha over the range-dependent c and rho (atmo_state.abs.sph.cpp:14-151). */
+double go_rd_sb_alpha(go_rd *a, double r, double lat, double lon, double freq)
+{
+    return go_sb_alpha_at(go_rd_c(a, r, lat, lon), go_rd_rho(a, r, lat, lon), r - a->r0, 1, a->tweak_abs, freq);
+}
+
+/* ======================================================================
+ * Ray equations: geoac.eqset.sph.rngdep.cpp
+ * ==================================================================== */
+
+/* set_initial, :68-112 */
+void go_rd_set_initial(go_rd *a, const go_params *p, go_src *s, double *y0)
+{
+    double th = s->theta, ph = s->phi, *L = s->src;
+    s->c0 = go_rd_c(a, L[0], L[1], L[2]);
+    double mach[3] = {go_rd_w(a, L[0], L[1], L[2]) / s->c0, go_rd_v(a, L[0], L[1], L[2]) / s->c0,
+                      go_rd_u(a, L[0], L[1], L[2]) / s->c0};
+    double nu0[3] = {sin(th), cos(th) * sin(ph), cos(th) * cos(ph)};
+    double mt[3] = {cos(th), -sin(th) * sin(ph), -sin(th) * cos(ph)};
+    double mp[3] = {0.0, cos(th) * cos(ph), -cos(th) * sin(ph)};
+    double msc = 1.0 + (nu0[0] * mach[0] + nu0[1] * mach[1] + nu0[2] * mach[2]);
+    s->nu0 = 1.0 / msc;
+    for (int i = 0; i < 3; i++) y0[i] = L[i];
+    for (int i = 0; i < 3; i++) y0[3 + i] = nu0[i] / msc;
+    if (p->eq_cnt == 18) {
+        for (int i = 0; i < 3; i++) { y0[6 + i] = 0.0; y0[12 + i] = 0.0; }
+        for (int i = 0; i < 3; i++) {
+            y0[9 + i] = mt[i] / msc - nu0[i] / sq(msc) * (mt[0] * mach[0] + mt[1] * mach[1] + mt[2] * mach[2]);
+            y0[15 + i] = mp[i] / msc - nu0[i] / sq(msc) * (mp[0] * mach[0] + mp[1] * mach[1] + mp[2] * mach[2]);
+        }
+    }
+}
+
+typedef struct {
+    double c, dc[5], ddc[3][2];
+    double u, du[5], ddu[3][2];
+    double v, dv[5], ddv[3][2];
+    double w, dw[5], ddw[3][2];
+    double nu_mag, dnu_mag[2];
+    double cg[3], cg_mag, dcg[3][2], dcg_mag[2];
+    double G[3], dG[3][2], T[3], dT[3][2];
+} rd_refs;
+
+/* update_refs, :238-373 */
+static void refs_rd(go_rd *a, const go_params *p, const double *y, rd_refs *R)
+{
+    double r = y[0], t = y[1], pp = y[2];
+    const double *nu = y + 3;
+    double dct[3], ddct[6], dd_u_t[6], dd_v_t[6], dd_w_t[6];
+    double dd_c[3][3], dd_u[3][3], dd_v[3][3], dd_w[3][3];
+    memset(R, 0, sizeof(*R));
+    R->nu_mag = sqrt(sq(nu[0]) + sq(nu[1]) + sq(nu[2]));
+    go_h3d_all(&a->c, t, pp, r - a->r0, &R->c, dct, ddct);
+    R->dc[1] = dct[0]; R->dc[2] = dct[1]; R->dc[0] = dct[2];
+    if (!p->calc_amp) {
+        go_rd_calc_uvw(a, r, t, pp, &R->u, &R->v, &R->w, R->du, R->dv, R->dw, NULL, NULL, NULL);
+    } else {
+        dd_c[1][1] = ddct[0]; dd_c[2][2] = ddct[1]; dd_c[0][0] = ddct[2];
+        dd_c[1][2] = ddct[3]; dd_c[2][1] = dd_c[1][2];
+        dd_c[0][1] = ddct[4]; dd_c[1][0] = dd_c[0][1];
+        dd_c[0][2] = ddct[5]; dd_c[2][0] = dd_c[0][2];
+        go_rd_calc_uvw(a, r, t, pp, &R->u, &R->v, &R->w, R->du, R->dv, R->dw, dd_u_t, dd_v_t, dd_w_t);
+    }
+    R->cg[0] = R->c * nu[0] / R->nu_mag + R->w;
+    R->cg[1] = R->c * nu[1] / R->nu_mag + R->v;
+    R->cg[2] = R->c * nu[2] / R->nu_mag + R->u;
+    R->cg_mag = sqrt(sq(R->cg[0]) + sq(R->cg[1]) + sq(R->cg[2]));
+
+    R->G[0] = 1.0;                R->T[0] = -1.0 / r * (nu[1] * R->cg[1] + nu[2] * R->cg[2]);
+    R->G[1] = 1.0 / r;            R->T[1] = (nu[0] * R->v - nu[1] * R->w) + nu[0] * R->cg[1] + nu[2] * R->cg[2] * tan(t);
+    R->G[2] = 1.0 / (r * cos(t)); R->T[2] = (nu[0] * R->u - nu[2] * R->w) * cos(t) + (nu[1] * R->u - nu[2] * R->v) * sin(t)
+                                            + R->cg[2] * (nu[0] * cos(t) - nu[1] * sin(t));
+    if (!p->calc_amp) return;
+
+    for (int n = 0; n < 3; n++) { dd_u[n][n] = dd_u_t[n]; dd_v[n][n] = dd_v_t[n]; dd_w[n][n] = dd_w_t[n]; }
+    dd_w[0][1] = dd_w[1][0] = dd_w_t[3]; dd_w[0][2] = dd_w[2][0] = dd_w_t[4]; dd_w[1][2] = dd_w[2][1] = dd_w_t[5];
+    dd_v[0][1] = dd_v[1][0] = dd_v_t[3]; dd_v[0][2] = dd_v[2][0] = dd_v_t[4]; dd_v[1][2] = dd_v[2][1] = dd_v_t[5];
+    dd_u[0][1] = dd_u[1][0] = dd_u_t[3]; dd_u[0][2] = dd_u[2][0] = dd_u_t[4]; dd_u[1][2] = dd_u[2][1] = dd_u_t[5];
+
+    const double *Rt = y + 6, *mt = y + 9, *Rp = y + 12, *mp = y + 15;
+    for (int n = 0; n < 3; n++) {
+        R->dc[3] += Rt[n] * R->dc[n];  R->dc[4] += Rp[n] * R->dc[n];
+        R->dw[3] += Rt[n] * R->dw[n];  R->dw[4] += Rp[n] * R->dw[n];
+        R->dv[3] += Rt[n] * R->dv[n];  R->dv[4] += Rp[n] * R->dv[n];
+        R->du[3] += Rt[n] * R->du[n];  R->du[4] += Rp[n] * R->du[n];
+        for (int m = 0; m < 3; m++) {
+            R->ddc[m][0] += Rt[n] * dd_c[m][n]; R->ddc[m][1] += Rp[n] * dd_c[m][n];
+            R->ddw[m][0] += Rt[n] * dd_w[m][n]; R->ddw[m][1] += Rp[n] * dd_w[m][n];
+            R->ddv[m][0] += Rt[n] * dd_v[m][n]; R->ddv[m][1] += Rp[n] * dd_v[m][n];
+            R->ddu[m][0] += Rt[n] * dd_u[m][n]; R->ddu[m][1] += Rp[n] * dd_u[m][n];
+        }
+    }
+    R->dnu_mag[0] = (nu[0] * mt[0] + nu[1] * mt[1] + nu[2] * mt[2]) / R->nu_mag;
+    R->dnu_mag[1] = (nu[0] * mp[0] + nu[1] * mp[1] + nu[2] * mp[2]) / R->nu_mag;
+
+    const double *wd[3] = {R->dw, R->dv, R->du};
+    for (int i = 0; i < 3; i++)
+        R->dcg[i][0] = nu[i] / R->nu_mag * R->dc[3] + R->c * mt[i] / R->nu_mag - R->c * nu[i] / sq(R->nu_mag) * R->dnu_mag[0] + wd[i][3];
+    for (int i = 0; i < 3; i++)
+        R->dcg[i][1] = nu[i] / R->nu_mag * R->dc[4] + R->c * mp[i] / R->nu_mag - R->c * nu[i] / sq(R->nu_mag) * R->dnu_mag[1] + wd[i][4];
+    R->dcg_mag[0] = (R->cg[0] * R->dcg[0][0] + R->cg[1] * R->dcg[1][0] + R->cg[2] * R->dcg[2][0]) / R->cg_mag;
+    R->dcg_mag[1] = (R->cg[0] * R->dcg[0][1] + R->cg[1] * R->dcg[1][1] + R->cg[2] * R->dcg[2][1]) / R->cg_mag;
+
+    R->dG[0][0] = 0.0; R->dG[0][1] = 0.0;
+    R->dG[1][0] = -Rt[0] / (sq(r));  R->dG[1][1] = -Rp[0] / (sq(r));
+    R->dG[2][0] = -Rt[0] / (sq(r) * cos(t)) + sin(t) / (r * sq(cos(t))) * Rt[1];
+    R->dG[2][1] = -Rp[0] / (sq(r) * cos(t)) + sin(t) / (r * sq(cos(t))) * Rp[1];
+
+    for (int j = 0; j < 2; j++) {
+        const double *Rj = j ? Rp : Rt, *mj = j ? mp : mt;
+        int d = 3 + j;
+        R->dT[0][j] = Rj[0] / sq(r) * (nu[1] * R->cg[1] + nu[2] * R->cg[2])
+                      - (1.0 / r) * (mj[1] * R->cg[1] + nu[1] * R->dcg[1][j] + mj[2] * R->cg[2] + nu[2] * R->dcg[2][j]);
+        R->dT[1][j] = (mj[0] * R->v + nu[0] * R->dv[d] - mj[1] * R->w - nu[1] * R->dw[d]);
+        R->dT[1][j] += mj[0] * R->cg[1] + nu[0] * R->dcg[1][j] + (mj[2] * R->cg[2] + nu[2] * R->dcg[2][j]) * tan(t)
+                       + nu[2] * R->cg[2] * Rj[1] / sq(cos(t));
+        R->dT[2][j] = (mj[0] * R->u + nu[0] * R->du[d] - mj[2] * R->w - nu[2] * R->dw[d]) * cos(t)
+                      - (nu[0] * R->u - nu[2] * R->w) * Rj[1] * sin(t);
+        R->dT[2][j] += (mj[1] * R->u + nu[1] * R->du[d] - mj[2] * R->v - nu[2] * R->dv[d]) * sin(t)
+                       + (nu[1] * R->u - nu[2] * R->v) * Rj[1] * cos(t);
+        R->dT[2][j] += R->dcg[2][j] * (nu[0] * cos(t) - nu[1] * sin(t))
+                       + R->cg[2] * ((mj[0] - nu[1] * Rj[1]) * cos(t) - (mj[1] + nu[0] * Rj[1]) * sin(t));
+    }
+}
+
+/* eval_src_eq, :378-436 */
+static void eqs_rd(const go_params *p, const double *y, const rd_refs *R, double *f)
+{
+    const double *nu = y + 3;
+    for (int i = 0; i < 3; i++) f[i] = R->G[i] * R->cg[i] / R->cg_mag;
+    for (int i = 0; i < 3; i++)
+        f[3 + i] = -R->G[i] / R->cg_mag * (R->nu_mag * R->dc[i] + nu[0] * R->dw[i] + nu[1] * R->dv[i] + nu[2] * R->du[i] + R->T[i]);
+    if (p->eq_cnt < 18) return;
+    for (int j = 0; j < 2; j++) {
+        const double *mj = y + (j ? 15 : 9);
+        int oR = j ? 12 : 6, om = j ? 15 : 9;
+        for (int i = 0; i < 3; i++)
+            f[oR + i] = R->dG[i][j] * R->cg[i] / R->cg_mag + R->G[i] * R->dcg[i][j] / R->cg_mag
+                        - R->G[i] * R->cg[i] / sq(R->cg_mag) * R->dcg_mag[j];
+        for (int i = 0; i < 3; i++)
+            f[om + i] = -R->dG[i][j] / R->cg_mag * (R->nu_mag * R->dc[i] + nu[0] * R->dw[i] + nu[1] * R->dv[i] + nu[2] * R->du[i] + R->T[i])
+                        + R->G[i] / sq(R->cg_mag) * R->dcg_mag[j] * (R->nu_mag * R->dc[i] + nu[0] * R->dw[i] + nu[1] * R->dv[i] + nu[2] * R->du[i])
+                        - R->G[i] / R->cg_mag * (R->dnu_mag[j] * R->dc[i] + R->nu_mag * R->ddc[i][j] + mj[0] * R->dw[i] + mj[1] * R->dv[i] + mj[2] * R->du[i]
+                                                 + nu[0] * R->ddw[i][j] + nu[1] * R->ddv[i][j] + nu[2] * R->ddu[i][j] + R->dT[i][j]);
+    }
+}
+
+void go_rd_rhs(go_rd *a, const go_params *p, const double *y, double *f)
+{
+    rd_refs R;
+    refs_rd(a, p, y, &R);
+    eqs_rd(p, y, &R, f);
+}
+
+/* set_ds, :227-233 */
+double go_rd_set_ds(go_rd *a, const go_params *p, const double *y)
+{
+    double result = p->ds_max;
+    if (y[0] - a->r0 < a->z_bndlyr)
+        result -= (p->ds_max - p->ds_min) * exp(-(y[0] - go_rd_topo_z(a, y[1], y[2])) / 0.2);
+    return result;
+}
+
+/* break_check, :483-494 (wraps longitude in place) */
+int go_rd_break_check(go_rd *a, const go_params *p, const go_src *s, double *y)
+{
+    if (y[2] > RD_PI) y[2] -= 2.0 * RD_PI;
+    else if (y[2] < -RD_PI) y[2] += 2.0 * RD_PI;
+    if (y[1] < -RD_PI / 2.0 || y[1] > RD_PI / 2.0) return 1;
+    if (y[0] - a->r0 > p->alt_max) return 1;
+    if (go_gc_dist(a->r0, y[1], y[2], s->src[1], s->src[2]) > p->rng_max) return 1;
+    if (y[1] < p->lat_min || y[1] > p->lat_max) return 1;
+    if (y[2] < p->lon_min || y[2] > p->lon_max) return 1;
+    return 0;
+}
+
+/* ground_check, :496-503 */
+int go_rd_ground_check(go_rd *a, const double *y)
+{
+    if (y[0] - a->r0 <= a->z_max)
+        if (y[0] < go_rd_topo_z(a, y[1], y[2])) return 1;
+    return 0;
+}
+
+/* rk4solver.cpp:29-55 */
+double go_rd_rk4_step(go_rd *a, const go_params *p, const double *y, double *y1)
+{
+    int n = p->eq_cnt;
+    double t1[18], t2[18], t3[18], t4[18], pa[18], f[18];
+    double ds = go_rd_set_ds(a, p, y);
+    go_rd_rhs(a, p, y, f);
+    for (int i = 0; i < n; i++) { t1[i] = ds * f[i]; pa[i] = y[i] + t1[i] / 2.0; }
+    go_rd_rhs(a, p, pa, f);
+    for (int i = 0; i < n; i++) t2[i] = ds * f[i];
+    for (int i = 0; i < n; i++) pa[i] = y[i] + t2[i] / 2.0;
+    go_rd_rhs(a, p, pa, f);
+    for (int i = 0; i < n; i++) t3[i] = ds * f[i];
+    for (int i = 0; i < n; i++) pa[i] = y[i] + t3[i];
+    go_rd_rhs(a, p, pa, f);
+    for (int i = 0; i < n; i++) {
+        t4[i] = ds * f[i];
+        y1[i] = y[i] + t1[i] / 6.0 + t2[i] / 3.0 + t3[i] / 3.0 + t4[i] / 6.0;
+    }
+    return ds;
+}
+
+/* rk4solver.cpp:14-85 */
+int go_rd_prop_rk4(go_rd *a, const go_params *p, const go_src *s, double *rows, int length, int *check)
+{
+    int k;
+    double sum_s = 0.0;
+    *check = 0;
+    for (k = 0; k < length - 1; k++) {
+        double *y = rows + (size_t)k * 18, *y1 = rows + (size_t)(k + 1) * 18;
+        double ds = go_rd_rk4_step(a, p, y, y1);
+        sum_s += ds;
+        if (sum_s > p->s_max) { *check = 1; break; }
+        if (go_rd_break_check(a, p, s, y1)) { *check = 1; break; }
+        if (go_rd_ground_check(a, y1)) { *check = 0; break; }
+    }
+    return k + 1;
+}
+
+/* approx_intercept + set_refl, :117-221.  Reference expressions kept as they
+ * are, including tan(prev[4]) in db_dlth / db_dlph (:161, :164) and db_dlth
+ * twice in dC2ph_dlth (:171). */
+void go_rd_set_refl(go_rd *a, const go_params *p, const go_src *s, const double *rk2, const double *rk1, double *y0)
+{
+    int neq = p->eq_cnt;
+    double prev[18] = {0};
+    double dr_grnd = go_rd_topo_z(a, rk1[1], rk1[2]) - rk1[0];
+    double dr_k = (rk1[0] - rk2[0])
+                  - (rk1[1] - rk2[1]) * 1.0 / rk1[0] * go_rd_topo_dz(a, rk1[1], rk1[2], 0)
+                  - (rk1[2] - rk2[2]) * 1.0 / (rk1[0] * cos(rk1[1])) * go_rd_topo_dz(a, rk1[1], rk1[2], 1);
+    for (int i = 0; i < neq; i++) prev[i] = rk1[i] + (rk1[i] - rk2[i]) / dr_k * dr_grnd;
+
+    double rg, c_grnd, aa, b, norm1, norm2, C1r, C1th, C1ph, C2th, C2ph;
+    double da_dlth = 0, db_dlth = 0, da_dlph = 0, db_dlph = 0;
+    double dC1r_dlth = 0, dC1th_dlth = 0, dC1ph_dlth = 0, dC2th_dlth = 0, dC2ph_dlth = 0, ds0_dlth = 0;
+    double dC1r_dlph = 0, dC1th_dlph = 0, dC1ph_dlph = 0, dC2th_dlph = 0, dC2ph_dlph = 0, ds0_dlph = 0;
+    double dnu_r_ds = 0, dnu_th_ds = 0, dnu_ph_ds = 0;
+
+    rg = go_rd_topo_z(a, prev[1], prev[2]);
+    c_grnd = go_rd_c(a, rg, prev[1], prev[2]);
+    aa = 1.0 / rg * go_rd_topo_dz(a, prev[1], prev[2], 0);
+    b = 1.0 / (rg * cos(prev[1])) * go_rd_topo_dz(a, prev[1], prev[2], 1);
+    norm1 = 1.0 + sq(aa) + sq(b);
+    C1r = (1.0 - sq(aa) - sq(b)) / norm1;
+    C1th = (1.0 - sq(aa) + sq(b)) / norm1;  C2th = 2.0 * aa / norm1;
+    C1ph = (1.0 + sq(aa) - sq(b)) / norm1;  C2ph = 2.0 * b / norm1;
+
+    if (p->calc_amp) {
+        double z00 = go_rd_topo_ddz(a, prev[1], prev[2], 0, 0), z01 = go_rd_topo_ddz(a, prev[1], prev[2], 0, 1);
+        double z10 = go_rd_topo_ddz(a, prev[1], prev[2], 1, 0), z11 = go_rd_topo_ddz(a, prev[1], prev[2], 1, 1);
+        norm2 = prev[3] - aa * prev[4] - b * prev[5];
+        da_dlth = -prev[6] / rg * aa + 1.0 / rg * ((prev[7] * (prev[3] - b * prev[5]) - prev[4] * (prev[6] / rg - cos(prev[1]) * b * prev[8])) / norm2 * z00
+                                                   + (prev[8] * (prev[3] - aa * prev[4]) - prev[5] * (prev[6] / (rg * cos(prev[1])) - aa / cos(prev[1]) * prev[7])) / norm2 * z01);
+        da_dlph = -prev[12] / rg * aa + 1.0 / rg * ((prev[13] * (prev[3] - b * prev[5]) - prev[4] * (prev[12] / rg - cos(prev[1]) * b * prev[14])) / norm2 * z00
+                                                    + (prev[14] * (prev[3] - aa * prev[4]) - prev[5] * (prev[12] / (rg * cos(prev[1])) - aa / cos(prev[1]) * prev[13])) / norm2 * z01);
+        db_dlth = -(prev[6] / rg - tan(prev[4]) * prev[7]) * b + 1.0 / (rg * cos(prev[1])) * ((prev[7] * (prev[3] - b * prev[5]) - prev[4] * (prev[6] / rg - cos(prev[1]) * b * prev[8])) / norm2 * z10
+                                                                                              + (prev[8] * (prev[3] - aa * prev[4]) - prev[5] * (prev[6] / (rg * cos(prev[1])) - aa / cos(prev[1]) * prev[7])) / norm2 * z11);
+        db_dlph = -(prev[12] / rg - tan(prev[4]) * prev[13]) * b + 1.0 / (rg * cos(prev[1])) * ((prev[13] * (prev[3] - b * prev[5]) - prev[4] * (prev[12] / rg - cos(prev[1]) * b * prev[14])) / norm2 * z10
+                                                                                                + (prev[14] * (prev[3] - aa * prev[4]) - prev[5] * (prev[12] / (rg * cos(prev[1])) - aa / cos(prev[1]) * prev[13])) / norm2 * z11);
+
+        dC1r_dlth = -4.0 / sq(norm1) * (aa * da_dlth + b * db_dlth);
+        dC1r_dlph = -4.0 / sq(norm1) * (aa * da_dlph + b * db_dlph);
+        dC1th_dlth = -4.0 * aa / sq(norm1) * ((1.0 + sq(b)) * da_dlth - aa * b * db_dlth);
+        dC1th_dlph = -4.0 * aa / sq(norm1) * ((1.0 + sq(b)) * da_dlph - aa * b * db_dlph);
+        dC1ph_dlth = -4.0 * b / sq(norm1) * ((1.0 + sq(aa)) * db_dlth - aa * b * da_dlth);
+        dC1ph_dlph = -4.0 * b / sq(norm1) * ((1.0 + sq(aa)) * db_dlph - aa * b * da_dlph);
+        dC2th_dlth = 2.0 / sq(norm1) * ((1.0 - sq(aa) + sq(b)) * da_dlth - 2.0 * aa * b * db_dlth);
+        dC2th_dlph = 2.0 / sq(norm1) * ((1.0 - sq(aa) + sq(b)) * da_dlph - 2.0 * aa * b * db_dlph);
+        dC2ph_dlth = 2.0 / sq(norm1) * ((1.0 + sq(aa) - sq(b)) * db_dlth - 2.0 * aa * b * db_dlth);
+        dC2ph_dlph = 2.0 / sq(norm1) * ((1.0 + sq(aa) - sq(b)) * db_dlph - 2.0 * aa * b * da_dlph);
+
+        ds0_dlth = -s->c0 / c_grnd * (prev[6] - rg * aa * prev[7] - rg * cos(prev[1]) * b * prev[8]) / norm2;
+        ds0_dlph = -s->c0 / c_grnd * (prev[12] - rg * aa * prev[13] - rg * cos(prev[1]) * b * prev[14]) / norm2;
+
+        dnu_r_ds = s->c0 / c_grnd * go_rd_dc(a, rg, prev[1], prev[2], 0) + prev[3] * go_rd_dw(a, rg, prev[1], prev[2], 0)
+                   + prev[4] * go_rd_dv(a, rg, prev[1], prev[2], 0) + prev[5] * go_rd_du(a, rg, prev[1], prev[2], 0);
+        dnu_th_ds = s->c0 / c_grnd * go_rd_dc(a, rg, prev[1], prev[2], 1) + prev[3] * go_rd_dw(a, rg, prev[1], prev[2], 1)
+                    + prev[4] * go_rd_dv(a, rg, prev[1], prev[2], 1) + prev[5] * go_rd_du(a, rg, prev[1], prev[2], 1);
+        dnu_ph_ds = s->c0 / c_grnd * go_rd_dc(a, rg, prev[1], prev[2], 2) + prev[3] * go_rd_dw(a, rg, prev[1], prev[2], 2)
+                    + prev[4] * go_rd_dv(a, rg, prev[1], prev[2], 2) + prev[5] * go_rd_du(a, rg, prev[1], prev[2], 2);
+        dnu_r_ds += sq(c_grnd) / s->c0 * 1.0 / rg * (sq(prev[4]) + sq(prev[5]));
+        dnu_th_ds -= sq(c_grnd) / s->c0 * (prev[3] * prev[4] + tan(prev[1]) * sq(prev[5]));
+        dnu_ph_ds -= sq(c_grnd) / s->c0 * prev[5] * (prev[3] * cos(prev[1]) + prev[4] * sin(prev[1]));
+        dnu_r_ds *= -1.0 / c_grnd;
+        dnu_th_ds *= -1.0 / (c_grnd * rg);
+        dnu_ph_ds *= -1.0 / (c_grnd * rg * cos(prev[1]));
+    }
+
+    y0[0] = rg;
+    y0[1] = prev[1];
+    y0[2] = prev[2];
+    y0[3] = -C1r * prev[3] + C2th * prev[4] + C2ph * prev[5];
+    y0[4] = C1th * prev[4] + C2th * (prev[3] - b * prev[5]);
+    y0[5] = C1ph * prev[5] + C2ph * (prev[3] - aa * prev[4]);
+    if (neq < 18) return;
+    for (int j = 0; j < 2; j++) {
+        int o = j ? 12 : 6;
+        y0[o] = -C1r * prev[o] + rg * C2th * prev[o + 1] + rg * cos(prev[1]) * C2ph * prev[o + 2];
+        y0[o + 1] = C1th * prev[o + 1] + C2th * (prev[o] / rg - b * cos(prev[1]) * prev[o + 2]);
+        y0[o + 2] = C1ph * prev[o + 2] + C2ph * (prev[o] / (rg * cos(prev[1])) - aa / cos(prev[1]) * prev[o + 1]);
+    }
+    y0[9] = -C1r * prev[9] + C2th * prev[10] + C2ph * prev[11] - dC1r_dlth * prev[3] + dC2th_dlth * prev[4] + dC2ph_dlth * prev[5]
+            + (-(C1r + 1.0) * dnu_r_ds + C2th * dnu_th_ds + C2ph * dnu_ph_ds) * ds0_dlth;
+    y0[10] = C1th * prev[10] + C2th * (prev[9] - b * prev[11] - db_dlth * prev[5]) + dC1th_dlth * prev[4] + dC2th_dlth * (prev[3] - b * prev[5])
+             + ((C1th - 1.0) * dnu_th_ds + C2th * (dnu_r_ds - b * dnu_ph_ds)) * ds0_dlth;
+    y0[11] = C1ph * prev[11] + C2ph * (prev[9] - aa * prev[10] - da_dlth * prev[4]) + dC1ph_dlth * prev[5] + dC2ph_dlth * (prev[3] - aa * prev[4])
+             + ((C1ph - 1.0) * dnu_ph_ds + C2ph * (dnu_r_ds - aa * dnu_th_ds)) * ds0_dlth;
+    y0[15] = -C1r * prev[15] + C2th * prev[16] + C2ph * prev[17] - dC1r_dlph * prev[3] + dC2th_dlph * prev[4] + dC2ph_dlph * prev[5]
+             + (-(C1r + 1.0) * dnu_r_ds + C2th * dnu_th_ds + C2ph * dnu_ph_ds) * ds0_dlph;
+    y0[16] = C1th * prev[16] + C2th * (prev[15] - b * prev[17] - db_dlph * prev[5]) + dC1th_dlph * prev[4] + dC2th_dlph * (prev[3] - b * prev[5])
+             + ((C1th - 1.0) * dnu_th_ds + C2th * (dnu_r_ds - b * dnu_ph_ds)) * ds0_dlph;
+    y0[17] = C1ph * prev[17] + C2ph * (prev[15] - aa * prev[16] - da_dlph * prev[4]) + dC1ph_dlph * prev[5] + dC2ph_dlph * (prev[3] - aa * prev[4])
+             + ((C1ph - 1.0) * dnu_ph_ds + C2ph * (dnu_r_ds - aa * dnu_th_ds)) * ds0_dlph;
+}
+
+/* travel_time (segment form of :547-575) */
+double go_rd_travel_time_seg(go_rd *a, const double *y0, const double *y1)
+{
+    double dr = y1[0] - y0[0], r = y0[0] + dr / 2.0;
+    double dt = y1[1] - y0[1], t = y0[1] + dt / 2.0;
+    double dp = y1[2] - y0[2], pp = y0[2] + dp / 2.0;
+    double nu[3], nu_mag, cp[3], cp_mag, snd, ds;
+    if (dp > RD_PI) dp -= 2.0 * RD_PI;
+    else if (dp < -RD_PI) dp += 2.0 * RD_PI;
+    ds = sqrt(sq(dr) + sq(r * dt) + sq(r * cos(t) * dp));
+    for (int i = 0; i < 3; i++) nu[i] = y0[3 + i] + (y1[3 + i] - y0[3 + i]) / 2.0;
+    nu_mag = sqrt(sq(nu[0]) + sq(nu[1]) + sq(nu[2]));
+    snd = go_rd_c(a, r, t, pp);
+    cp[0] = snd * nu[0] / nu_mag + go_rd_w(a, r, t, pp);
+    cp[1] = snd * nu[1] / nu_mag + go_rd_v(a, r, t, pp);
+    cp[2] = snd * nu[2] / nu_mag + go_rd_u(a, r, t, pp);
+    cp_mag = sqrt(sq(cp[0]) + sq(cp[1]) + sq(cp[2]));
+    return ds / cp_mag;
+}
+
+/* atten (segment form of :775-792) */
+double go_rd_atten_seg(go_rd *a, const double *y0, const double *y1, double freq)
+{
+    double dr = y1[0] - y0[0], r = y0[0] + dr / 2.0;
+    double dt = y1[1] - y0[1], t = y0[1] + dt / 2.0;
+    double dp = y1[2] - y0[2], pp = y0[2] + dp / 2.0;
+    if (dp > RD_PI) dp -= 2.0 * RD_PI;
+    else if (dp < -RD_PI) dp += 2.0 * RD_PI;
+    double ds = sqrt(sq(dr) + sq(r * dt) + sq(r * cos(t) * dp));
+    return go_rd_sb_alpha(a, r, t, pp, freq) * ds;
+}
+
+/* jacobian, :798-821 */
+double go_rd_jacobian(go_rd *a, const double *y)
+{
+    double r = y[0], t = y[1], p = y[2];
+    double nu_mag = sqrt(sq(y[3]) + sq(y[4]) + sq(y[5]));
+    double cp[3], cp_mag;
+    cp[0] = go_rd_c(a, r, t, p) * y[3] / nu_mag + go_rd_w(a, r, t, p);
+    cp[1] = go_rd_c(a, r, t, p) * y[4] / nu_mag + go_rd_v(a, r, t, p);
+    cp[2] = go_rd_c(a, r, t, p) * y[5] / nu_mag + go_rd_u(a, r, t, p);
+    cp_mag = sqrt(sq(cp[0]) + sq(cp[1]) + sq(cp[2]));
+    double dr_ds = cp[0] / cp_mag, dt_ds = 1.0 / r * cp[1] / cp_mag, dp_ds = 1.0 / (r * cos(t)) * cp[2] / cp_mag;
+    double dr_t = y[6], dt_t = y[7], dp_t = y[8], dr_p = y[12], dt_p = y[13], dp_p = y[14];
+    return sq(r) * cos(t) * (dr_ds * (dt_t * dp_p - dt_p * dp_t) - dr_t * (dt_ds * dp_p - dp_ds * dt_p)
+                             + dr_p * (dt_ds * dp_t - dp_ds * dt_t));
+}
+
+/* amp, :823-847 */
+double go_rd_amp(go_rd *a, const go_src *s, const double *y)
+{
+    double r = y[0], t = y[1], p = y[2];
+    double r0 = s->src[0], t0 = s->src[1], p0 = s->src[2];
+    double nu0[3] = {s->nu0 * sin(s->theta), s->nu0 * cos(s->theta) * sin(s->phi), s->nu0 * cos(s->theta) * cos(s->phi)};
+    double nu_mag = (s->c0 - y[3] * go_rd_w(a, r, t, p) - y[4] * go_rd_v(a, r, t, p) - y[5] * go_rd_u(a, r, t, p)) / go_rd_c(a, r, t, p);
+    double nu_mag0 = s->nu0;
+    double c0p[3], cp[3];
+    c0p[0] = s->c0 * nu0[0] / nu_mag0 + go_rd_w(a, r0, t0, p0);  cp[0] = go_rd_c(a, r, t, p) * y[3] / nu_mag + go_rd_w(a, r, t, p);
+    c0p[1] = s->c0 * nu0[1] / nu_mag0 + go_rd_v(a, r0, t0, p0);  cp[1] = go_rd_c(a, r, t, p) * y[4] / nu_mag + go_rd_v(a, r, t, p);
+    c0p[2] = s->c0 * nu0[2] / nu_mag0 + go_rd_u(a, r0, t0, p0);  cp[2] = go_rd_c(a, r, t, p) * y[5] / nu_mag + go_rd_u(a, r, t, p);
+    double cp_mag = sqrt(sq(cp[0]) + sq(cp[1]) + sq(cp[2]));
+    double cp_mag0 = sqrt(sq(c0p[0]) + sq(c0p[1]) + sq(c0p[2]));
+    double num = go_rd_rho(a, r, t, p) * nu_mag * pow(go_rd_c(a, r, t, p), 3) * cp_mag0 * cos(s->theta);
+    double den = go_rd_rho(a, r0, t0, p0) * nu_mag0 * pow(go_rd_c(a, r0, t0, p0), 3) * cp_mag * go_rd_jacobian(a, y);
+    return sqrt(fabs(num / den));
+}
+
+/* eval_eikonal, :441-447 */
+double go_rd_eikonal_resid(go_rd *a, const go_src *s, const double *y)
+{
+    double r = y[0], t = y[1], p = y[2];
+    return sqrt(y[3] * y[3] + y[4] * y[4] + y[5] * y[5]) - s->c0 / go_rd_c(a, r, t, p)
+           + (go_rd_w(a, r, t, p) * y[3] + go_rd_v(a, r, t, p) * y[4] + go_rd_u(a, r, t, p) * y[5]) / go_rd_c(a, r, t, p);
+}
+
+/* eval_eikonal_deriv, :457-478: launch-angle derivatives of the eikonal
+ * residual carried by the auxiliary variables. */
+double go_rd_eikonal_deriv_resid(go_rd *a, const double *y)
+{
+    double r = y[0], t = y[1], p = y[2];
+    const double *nu = y + 3, *Rt = y + 6, *mt = y + 9, *Rp = y + 12, *mp = y + 15;
+    double mag_nu = sqrt(nu[0] * nu[0] + nu[1] * nu[1] + nu[2] * nu[2]);
+    double dc_t = 0, dc_p = 0, du_t = 0, du_p = 0, dv_t = 0, dv_p = 0, dw_t = 0, dw_p = 0;
+    for (int n = 0; n < 3; n++) {
+        double dc = go_rd_dc(a, r, t, p, n), du = go_rd_du(a, r, t, p, n);
+        double dv = go_rd_dv(a, r, t, p, n), dw = go_rd_dw(a, r, t, p, n);
+        dc_t += Rt[n] * dc; dc_p += Rp[n] * dc;
+        du_t += Rt[n] * du; du_p += Rp[n] * du;
+        dv_t += Rt[n] * dv; dv_p += Rp[n] * dv;
+        dw_t += Rt[n] * dw; dw_p += Rp[n] * dw;
+    }
+    double c = go_rd_c(a, r, t, p), u = go_rd_u(a, r, t, p), v = go_rd_v(a, r, t, p), w = go_rd_w(a, r, t, p);
+    double res_t = (nu[0] * mt[0] + nu[1] * mt[1] + nu[2] * mt[2]) / mag_nu + mag_nu / c * dc_t
+                   + 1.0 / c * (mt[0] * w + mt[1] * v + mt[2] * u + nu[0] * dw_t + nu[1] * dv_t + nu[2] * du_t);
+    double res_p = (nu[0] * mp[0] + nu[1] * mp[1] + nu[2] * mp[2]) / mag_nu + mag_nu / c * dc_p
+                   + 1.0 / c * (mp[0] * w + mp[1] * v + mp[2] * u + nu[0] * dw_p + nu[1] * dv_p + nu[2] * du_p);
+    return sqrt(pow(res_t, 2) + pow(res_p, 2));
+}
+
+/* ======================================================================
+ * run_prop: main.infraga.sph.rngdep.cpp:311-554
+ * ==================================================================== */
+double go_rd_theta_grnd(go_rd *a, double lat_src, double lon_src, double z_src, double az_deg)
+{
+    double phi = az_deg, dzgdx, dzgdy;
+    if (z_src - (go_rd_topo_z(a, lat_src, lon_src) - a->r0) < 0.01) {
+        if (a->is_topo) {
+            dzgdx = 1.0 / (a->r0 * cos(lat_src)) * go_rd_topo_dz(a, lat_src, lon_src, 1);
+            dzgdy = 1.0 / a->r0 * go_rd_topo_dz(a, lat_src, lon_src, 0);
+            return atan(dzgdx * sin(phi * RD_PI / 180.0) + dzgdy * cos(phi * RD_PI / 180.0)) * (180.0 / RD_PI) + 0.1;
+        }
+        return 0.1;
+    }
+    return -89.9;
+}
+
+static double *rd_rows = NULL;
+static int rd_rows_len = 0;
+
+int go_rd_fan(go_rd *a, const go_params *p, const go_prop *pr, const double *launches, int n_launch,
+              go_arrival *out, int out_cap, int *n_out, long long *ray_steps)
+{
+    int length = (int)(p->s_max / p->ds_min);
+    if (rd_rows_len < length) {
+        free(rd_rows);
+        rd_rows = (double *)malloc(sizeof(double) * 18 * (size_t)length);
+        rd_rows_len = length;
+    }
+    double *rows = rd_rows;
+    go_src s;
+    int cnt = 0;
+    long long steps = 0;
+    double lat_src = pr->src0, lon_src = pr->src1, z_src = pr->src_alt;
+    lat_src *= RD_PI / 180.0;                                          /* :311-312 */
+    lon_src *= RD_PI / 180.0;
+    z_src = dmax(z_src, go_rd_topo_z(a, lat_src, lon_src) - a->r0);     /* :315 */
+    s.src[0] = a->r0 + z_src; s.src[1] = lat_src; s.src[2] = lon_src;
+
+    for (int li = 0; li < n_launch; li++) {
+        double incl = launches[2 * li], az = launches[2 * li + 1];
+        s.phi = RD_PI / 2.0 - az * (RD_PI / 180.0);
+        s.theta = incl * (RD_PI / 180.0);
+        go_rd_set_initial(a, p, &s, rows);
+        double tt = 0.0, att = 0.0, rmax = 0.0;
+        int brk = 0, k;
+        for (int bnc = 0; bnc <= pr->bounces; bnc++) {
+            k = go_rd_prop_rk4(a, p, &s, rows, length, &brk);
+            steps += k;
+            double tleg = 0.0, aleg = 0.0;
+            for (int n = 0; n < k; n++) tleg += go_rd_travel_time_seg(a, rows + 18 * n, rows + 18 * (n + 1));
+            tt += tleg;
+            for (int n = 0; n < k; n++) aleg += go_rd_atten_seg(a, rows + 18 * n, rows + 18 * (n + 1), pr->freq);
+            att += aleg;
+            for (int m = 0; m < k; m++) rmax = dmax(rmax, rows[18 * m] - a->r0);
+            if (!a->is_topo && rmax < (a->z0 + pr->turn_ht_min)) break;     /* :507 */
+            if (brk || k < 2 || (tt / 3600.0) > p->time_max) break;         /* :511 */
+            double *yk = rows + 18 * k;
+            go_arrival A;
+            memset(&A, 0, sizeof(A));
+            A.launch_idx = li;
+            A.n_bounce = bnc;
+            A.incl_launch = incl;
+            A.az_launch = az;
+            /* :515, no leading minus sign (the stratified driver has one) */
+            A.incl_arr = asin(go_rd_c(a, yk[0], yk[1], yk[2]) / go_rd_c(a, a->r0 + z_src, lat_src, lon_src) * yk[3]) * 180.0 / RD_PI;
+            double baz = 90.0 - atan2(-yk[4], -yk[5]) * 180.0 / RD_PI;
+            if (baz < -180.0) baz += 360.0;
+            else if (baz > 180.0) baz -= 360.0;
+            A.back_az = baz;
+            A.pos0 = yk[1] * (180.0 / RD_PI);
+            A.pos1 = yk[2] * (180.0 / RD_PI);
+            A.time = tt;
+            A.cel = go_gc_dist(a->r0, yk[1], yk[2], lat_src, lon_src) / tt;
+            A.turn_ht = rmax;
+            A.tl = p->calc_amp ? 20.0 * log10(go_rd_amp(a, &s, yk)) : 0.0;
+            A.absorption = -2.0 * att;
+            if (cnt < out_cap) out[cnt] = A;
+            cnt++;
+            go_rd_set_refl(a, p, &s, rows + 18 * (k - 2), rows + 18 * (k - 1), rows);
+        }
+    }
+    *n_out = cnt;
+    if (ray_steps) *ray_steps = steps;
+    return cnt > out_cap ? -1 : 0;
+}
+
+/* ---- handles ------------------------------------------------------------ */
+void *go_rd_new(const double *lat_deg, int nlat, const double *lon_deg, int nlon, const double *z, int nz,
+                const double *c, const double *u, const double *v, const double *rho)
+{
+    go_rd *a = (go_rd *)malloc(sizeof(go_rd));
+    if (go_rd_init(a, lat_deg, nlat, lon_deg, nlon, z, nz, c, u, v, rho)) { free(a); return NULL; }
+    return a;
+}
+void go_rd_delete(void *h) { if (h) { go_rd_free((go_rd *)h); free(h); } }
+void go_rd_h_config(void *h, int use_blw, double tweak_abs)
+{
+    go_rd *a = (go_rd *)h;
+    a->use_blw = use_blw;
+    a->tweak_abs = tweak_abs;
+}
+void go_rd_h_limits(void *h, double out[8])
+{
+    go_rd *a = (go_rd *)h;
+    out[0] = a->lat_min; out[1] = a->lat_max; out[2] = a->lon_min; out[3] = a->lon_max;
+    out[4] = a->alt_max; out[5] = a->z0; out[6] = a->z_max; out[7] = a->z_bndlyr;
+}
