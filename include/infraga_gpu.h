@@ -45,6 +45,9 @@ extern "C" {
 
 #define GA_GEOM_3D     0   /* Cartesian x, y, z (infraga-3d) */
 #define GA_GEOM_SPH    1   /* spherical r, lat, lon (infraga-sph) */
+#define GA_GEOM_SPH_RNGDEP 2   /* spherical, range-dependent atmosphere and
+                                  terrain (infraga-sph-rngdep); atmo_id is a
+                                  ga_rngdep_load id */
 
 typedef struct ga_ctx ga_ctx;
 
@@ -65,8 +68,12 @@ typedef struct {
     double rng_max;        /* km */
     double time_max;       /* hours (sph only, as in the reference) */
     double bounds[4];      /* sph: lat_min, lat_max, lon_min, lon_max [deg];
-                              3D: x_min, x_max, y_min, y_max [km] */
+                              3D: x_min, x_max, y_min, y_max [km];
+                              range-dependent: NaN = the grid's region
+                              (set_limits, atmo_io.sph.rngdep.cpp:27-43) */
     double r_earth;        /* km; the reference hard-codes 6370 */
+    int32_t topo_use_blw;  /* range-dependent: boundary-layer vertical wind
+                              over terrain (topo_use_BLw, default 0) */
 } ga_prop_params;
 
 /* One row of <id>.arrivals.dat (src/main.infraga.sph.cpp:531-544). */
@@ -179,6 +186,22 @@ int ga_topo_load(ga_ctx *ctx, int rd_id, const double *lat_deg, int nlat, const 
                  const double *zg_km);
 int ga_rngdep_eval(ga_ctx *ctx, int rd_id, const double *pts, int n, double *atmo_out, double *topo_out);
 int ga_rngdep_release(ga_ctx *ctx, int rd_id);
+
+/* The reference's file layout: node files of lat / lon degrees (one per
+ * line), profiles <prefix><i * nlon + j>.met in format zTuvdp, zuvwTdp or
+ * zcuvd, and optionally a terrain file of "lat lon z" rows (lon fastest)
+ * (set_region, atmo_io.sph.rngdep.cpp:61-200 / 203-366; file_length and
+ * file_2d_dims, src/util/fileIO.cpp:18-85).  topo_path may be NULL. */
+int ga_rngdep_load_files(ga_ctx *ctx, const char *prefix, const char *lat_path, const char *lon_path,
+                         const char *topo_path, const char *format, int reverse_winds, int *rd_id);
+
+/* Range-dependent fans run through ga_fan_run / ga_fan_run_device with
+ * p->geometry = GA_GEOM_SPH_RNGDEP and atmo_id a ga_rngdep_load id
+ * (run_prop, src/main.infraga.sph.rngdep.cpp:412-554).  p->src[0..1] NaN =
+ * the region centre; the source sits at max(src[2], terrain).  The launch
+ * list is explicit: the reference starts each azimuth's inclinations at the
+ * terrain slope along it plus 0.1 degree (main.infraga.sph.rngdep.cpp:
+ * 415-428), which a caller reads from ga_rngdep_eval's terrain derivatives. */
 
 /* ---- Request server: continuous batching of fans and eigenray queries ----
  *

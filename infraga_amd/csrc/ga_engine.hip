@@ -30,6 +30,7 @@
 
 #include "../../include/infraga_gpu.h"
 #include "ga_eig_search.hpp"
+#include "ga_rdtrace.hpp"
 #include "ga_rngdep.hpp"
 #include "ga_trace.hpp"
 
@@ -105,6 +106,50 @@ __global__ void __launch_bounds__(256) rd_eval_kernel(RdGrid g, TopoGrid t, int 
     for (int k = 0; k < 10; ++k) tout[(size_t)i * 10 + k] = tz[k];
 }
 
+// Range-dependent fans: the hybrid-spline tables stay in global memory
+// (29 MB for an 11 x 11 x 1501 grid) and are read through L1/L2; lanes pull
+// rays from the queue counter (ONE: one ray per lane).
+#ifndef GA_WG_RD
+#define GA_WG_RD 256
+#endif
+template <int NEQ, bool ONE>
+__global__ void __launch_bounds__(GA_WG_RD) rd_fan_kernel(RdFanArgs A)
+{
+    const int lane = blockIdx.x * blockDim.x + threadIdx.x;
+    if constexpr (ONE) {
+        bool first = true;
+        rd_lane_loop<NEQ>(A, lane, [&]() {
+            const int r = first ? lane : A.n_rays;
+            first = false;
+            return r;
+        });
+    } else {
+        rd_lane_loop<NEQ>(A, lane, [&]() { return (int)atomicAdd(A.queue, 1u); });
+    }
+}
+
+template <int NEQ>
+__global__ void __launch_bounds__(256) rd_arrival_kernel(RdFanArgs A)
+{
+    const long long slot = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int nb = A.bounces + 1;
+    if (slot >= (long long)A.n_rays * nb) return;
+    const int ray = (int)(slot / nb), bnc = (int)(slot % nb);
+    if (bnc >= A.counts[ray]) return;
+    const double *r = A.raw + slot * RAW;
+    double y[18];
+#pragma unroll
+    for (int i = 0; i < 18; ++i) y[i] = (i < NEQ) ? r[i] : 0.0;
+    A.out[slot] = rd_arrival_row<NEQ>(A, y, ray, bnc, A.launches[2 * ray], A.launches[2 * ray + 1], r[18], r[19],
+                                      r[20], r[23], r[24], r[21], r[22]);
+}
+
+// Terrain height above sea level at one point (the source of a fan).
+__global__ void rd_ground_kernel(RdMed M, double lat, double lon, double *out)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) out[0] = rd_ground(M, lat, lon) - M.r0;
+}
+
 // Eigenray-search traces: same persistent layout as fan_kernel, jobs instead
 // of launch angles.  WGS lanes per workgroup share one LDS copy of the table;
 // the 6-equation sweep kernel needs <= 110 registers and runs four waves per
@@ -164,6 +209,9 @@ struct DevRd {
     double *d_taxes = nullptr;     // topography lat [tnx], lon [tny]
     double *d_ttab = nullptr;      // [tnx][tny][TP_Q]
     int tnx = 0, tny = 0;
+    std::vector<double> hx, hy, hz;    // host copies of the node axes [rad, rad, km]
+    std::vector<double> thx, thy;      // terrain axes [rad]
+    double t_max = 0.0;                // terrain maximum, floored at 0 (set_bndlyr)
     void release()
     {
         for (double *p : {d_axes, d_tab, d_taxes, d_ttab})
@@ -179,6 +227,7 @@ struct DevRd {
 // flight owns its set, so launches on different streams use different sets.
 struct FanBufs {
     unsigned int *queue = nullptr;
+    double *scalar = nullptr;      // one-value readback (terrain height at a source)
     double *raw = nullptr;
     size_t raw_cap = 0;            // doubles
     double *yp = nullptr;
@@ -186,9 +235,10 @@ struct FanBufs {
     void release()
     {
         if (queue) (void)hipFree(queue);
+        if (scalar) (void)hipFree(scalar);
         if (raw) (void)hipFree(raw);
         if (yp) (void)hipFree(yp);
-        queue = nullptr; raw = nullptr; yp = nullptr; raw_cap = 0; yp_cap = 0;
+        queue = nullptr; scalar = nullptr; raw = nullptr; yp = nullptr; raw_cap = 0; yp_cap = 0;
     }
 };
 
@@ -354,9 +404,163 @@ int launch_fan(ga_ctx *ctx, FanBufs &fb, const FanArgs &A, hipStream_t st)
 // Set by the request server around its fan launches (one ray per lane).
 thread_local int serve_one_shot = 0;
 
+// Range-dependent fan: medium and region as set_limits / set_bndlyr derive
+// them (atmo_io.sph.rngdep.cpp:27-43, atmo_state.sph.rngdep.cpp:42-55), the
+// source and the reference's z_grnd handling from run_prop
+// (main.infraga.sph.rngdep.cpp:229-315).
+int rd_fan_enqueue(ga_ctx *ctx, FanBufs &fb, int rd_id, const ga_prop_params *p, const double *d_launches, int n,
+                   ga_arrival *d_out, int32_t *d_counts, int32_t *d_steps, hipStream_t st)
+{
+    auto it = ctx->rds.find(rd_id);
+    if (it == ctx->rds.end()) return fail(ctx, GA_E_ATMO, "unknown range-dependent atmosphere id %d", rd_id);
+    const DevRd &R = it->second;
+    if (p->bounces < 0) return fail(ctx, GA_E_ARG, "bounces must be >= 0");
+    if (!(p->ds_min > 0.0) || !(p->ds_max >= p->ds_min) || !(p->s_max > 0.0))
+        return fail(ctx, GA_E_ARG, "invalid step parameters ds_min=%g ds_max=%g s_max=%g", p->ds_min, p->ds_max, p->s_max);
+    RdFanArgs A;
+    std::memset(&A, 0, sizeof(A));
+    RdMed &M = A.M;
+    M.g.x = R.d_axes;
+    M.g.y = R.d_axes + R.nx;
+    M.g.z = R.d_axes + R.nx + R.ny;
+    M.g.nx = R.nx;
+    M.g.ny = R.ny;
+    M.g.nz = R.nz;
+    M.g.tab = R.d_tab;
+    M.is_topo = R.d_ttab ? 1 : 0;
+    if (M.is_topo) {
+        M.t.x = R.d_taxes;
+        M.t.y = R.d_taxes + R.tnx;
+        M.t.nx = R.tnx;
+        M.t.ny = R.tny;
+        M.t.tab = R.d_ttab;
+    }
+    M.use_blw = p->topo_use_blw ? 1 : 0;
+    M.r0 = p->r_earth;
+    M.tweak = std::max(0.0, p->abs_coeff);
+    M.sbc[0] = std::pow(10.0, -0.67887);
+    M.sbc[1] = std::pow(10.0, -0.10744);
+    M.sbc[2] = std::pow(10.0, -3.3979);
+    const double buffer = 1.0e-3 * (PI / 180.0);
+    double lat_min = R.hx.front() + buffer, lat_max = R.hx.back() - buffer;
+    double lon_min = R.hy.front() + buffer, lon_max = R.hy.back() - buffer;
+    if (M.is_topo) {
+        lat_min = std::max(lat_min, R.thx.front() + buffer);
+        lat_max = std::min(lat_max, R.thx.back() - buffer);
+        lon_min = std::max(lon_min, R.thy.front() + buffer);
+        lon_max = std::min(lon_max, R.thy.back() - buffer);
+        if (lat_max < lat_min || lon_max < lon_min)
+            return fail(ctx, GA_E_ARG, "atmosphere grid and topography grid have no overlap");
+        M.z0 = R.hz.front();
+        M.z_max = R.t_max;
+        M.z_bndlyr = M.z_max + 2.0;
+        M.z_max += 0.01;
+    } else {
+        M.z0 = std::isnan(p->z_grnd) ? R.hz.front() : p->z_grnd;
+        M.z_max = M.z0;
+        M.z_bndlyr = M.z0 + 2.0;
+    }
+    Limits &L = A.L;
+    L.s_max = p->s_max;
+    L.ds_min = p->ds_min;
+    L.ds_max = p->ds_max;
+    L.alt_max = std::isnan(p->alt_max) ? R.hz.back() : p->alt_max;
+    L.rng_max = p->rng_max;
+    L.time_max = p->time_max;
+    L.z_keep = M.z_max + 4.0 * p->ds_max;
+    L.b0 = std::isnan(p->bounds[0]) ? lat_min : p->bounds[0] * (PI / 180.0);
+    L.b1 = std::isnan(p->bounds[1]) ? lat_max : p->bounds[1] * (PI / 180.0);
+    L.b2 = std::isnan(p->bounds[2]) ? lon_min : p->bounds[2] * (PI / 180.0);
+    L.b3 = std::isnan(p->bounds[3]) ? lon_max : p->bounds[3] * (PI / 180.0);
+    {
+        double q = p->rng_max / (2.0 * M.r0);
+        double sq = q >= PI / 2.0 ? 2.0 : std::sin(q);
+        L.gc_thresh = sq >= 1.0 ? 2.0 : sq * sq;
+    }
+    // source: region centre by default, at max(z_src, terrain) (:242-243, :311-315)
+    double lat_deg = std::isnan(p->src[0]) ? (lat_max + lat_min) / 2.0 * (180.0 / PI) : p->src[0];
+    double lon_deg = std::isnan(p->src[1]) ? (lon_max + lon_min) / 2.0 * (180.0 / PI) : p->src[1];
+    A.src1 = lat_deg * (PI / 180.0);
+    A.src2 = lon_deg * (PI / 180.0);
+    double zsrc = p->src[2];
+    if (M.is_topo) {
+        if (!fb.scalar && hipMalloc(&fb.scalar, sizeof(double)) != hipSuccess) return fail(ctx, GA_E_NOMEM, "scalar");
+        hipLaunchKernelGGL(rd_ground_kernel, dim3(1), dim3(64), 0, st, M, A.src1, A.src2, fb.scalar);
+        HIPCHK(ctx, hipGetLastError());
+        double zg = 0.0;
+        HIPCHK(ctx, hipMemcpyAsync(&zg, fb.scalar, sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHK(ctx, hipStreamSynchronize(st));
+        zsrc = std::max(zsrc, zg);
+    } else {
+        zsrc = std::max(zsrc, M.z0);
+    }
+    A.src0 = M.r0 + zsrc;
+    A.freq = p->freq;
+    A.turn_ht_min = p->turn_ht_min;
+    A.bounces = p->bounces;
+    A.n_rays = n;
+    long long rows = (long long)(p->s_max / p->ds_min);
+    A.max_steps = (int)std::min<long long>(rows - 1, 2000000000LL);
+    A.launches = d_launches;
+    {
+        size_t need = (size_t)std::max(n, 1) * (size_t)(p->bounces + 1) * RAW;
+        if (need > fb.raw_cap) {
+            HIPCHK(ctx, hipStreamSynchronize(st));
+            HIPCHK(ctx, hipFree(fb.raw));
+            fb.raw = nullptr;
+            fb.raw_cap = 0;
+            if (hipMalloc(&fb.raw, need * sizeof(double)) != hipSuccess)
+                return fail(ctx, GA_E_NOMEM, "raw arrival buffer (%zu doubles)", need);
+            fb.raw_cap = need;
+        }
+        A.raw = fb.raw;
+    }
+    A.out = d_out;
+    A.counts = d_counts;
+    A.steps = d_steps;
+    if (!fb.queue && hipMalloc(&fb.queue, sizeof(unsigned int)) != hipSuccess)
+        return fail(ctx, GA_E_NOMEM, "queue counter");
+    A.queue = fb.queue;
+    A.one_shot = serve_one_shot;
+    if (n == 0) return GA_OK;
+
+    const bool amp = p->calc_amp != 0;
+    auto kern = amp ? (A.one_shot ? rd_fan_kernel<18, true> : rd_fan_kernel<18, false>)
+                    : (A.one_shot ? rd_fan_kernel<6, true> : rd_fan_kernel<6, false>);
+    int per_cu = 0;
+    constexpr int W = GA_WG_RD;
+    HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, W, 0));
+    if (per_cu < 1) per_cu = 1;
+    int grid = per_cu * ctx->n_cu;
+    int need = (n + W - 1) / W;
+    grid = A.one_shot ? std::max(1, need) : std::max(1, std::min(grid, need));
+    A.nlanes = grid * W;
+    size_t ypn = (size_t)A.nlanes * 18;
+    if (ypn > fb.yp_cap) {
+        HIPCHK(ctx, hipStreamSynchronize(st));
+        HIPCHK(ctx, hipFree(fb.yp));
+        fb.yp = nullptr;
+        fb.yp_cap = 0;
+        if (hipMalloc(&fb.yp, ypn * sizeof(double)) != hipSuccess) return fail(ctx, GA_E_NOMEM, "lane buffer");
+        fb.yp_cap = ypn;
+    }
+    A.ypbuf = fb.yp;
+    hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, st, A.queue);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(W), 0, st, A);
+    HIPCHK(ctx, hipGetLastError());
+    long long slots = (long long)n * (p->bounces + 1);
+    int agrid = (int)((slots + 255) / 256);
+    if (amp) hipLaunchKernelGGL(rd_arrival_kernel<18>, dim3(agrid), dim3(256), 0, st, A);
+    else hipLaunchKernelGGL(rd_arrival_kernel<6>, dim3(agrid), dim3(256), 0, st, A);
+    HIPCHK(ctx, hipGetLastError());
+    return GA_OK;
+}
+
 int fan_enqueue(ga_ctx *ctx, FanBufs &fb, int atmo_id, const ga_prop_params *p, const double *d_launches, int n,
                 ga_arrival *d_out, int32_t *d_counts, int32_t *d_steps, hipStream_t st)
 {
+    if (p->geometry == GA_GEOM_SPH_RNGDEP)
+        return rd_fan_enqueue(ctx, fb, atmo_id, p, d_launches, n, d_out, d_counts, d_steps, st);
     auto it = ctx->atmos.find(atmo_id);
     if (it == ctx->atmos.end()) return fail(ctx, GA_E_ATMO, "unknown atmosphere id %d", atmo_id);
     if (p->geometry != GA_GEOM_SPH && p->geometry != GA_GEOM_3D) return fail(ctx, GA_E_ARG, "bad geometry %d", p->geometry);
@@ -1079,6 +1283,13 @@ void ga_prop_params_default(ga_prop_params *p, int geometry)
         p->src[0] = 30.0; p->src[1] = 0.0; p->src[2] = 0.0;
         p->rng_max = 1000.0;
         p->bounds[0] = -90.0; p->bounds[1] = 90.0; p->bounds[2] = -180.0; p->bounds[3] = 180.0;
+    } else if (geometry == GA_GEOM_SPH_RNGDEP) {
+        // run_prop (main.infraga.sph.rngdep.cpp:212-227): source at the region
+        // centre, two bounces, region from the grid
+        p->src[0] = NAN; p->src[1] = NAN; p->src[2] = 0.0;
+        p->bounces = 2;
+        p->rng_max = 1000.0;
+        for (double &b : p->bounds) b = NAN;
     } else {
         p->src[0] = 0.0; p->src[1] = 0.0; p->src[2] = 0.0;
         p->rng_max = 2000.0;   // atmo_io.3d.strat.cpp:30
@@ -1385,6 +1596,7 @@ int ga_rngdep_load(ga_ctx *ctx, const double *lat_deg, int nlat, const double *l
     HIPCHK(ctx, hipSetDevice(ctx->device));
     DevRd r;
     r.nx = nlat; r.ny = nlon; r.nz = nz;
+    r.hx = x; r.hy = y; r.hz = z;
     if (hipMalloc(&r.d_axes, axes.size() * sizeof(double)) != hipSuccess ||
         hipMalloc(&r.d_tab, tab.size() * sizeof(double)) != hipSuccess) {
         r.release();
@@ -1449,7 +1661,117 @@ int ga_topo_load(ga_ctx *ctx, int rd_id, const double *lat_deg, int nlat, const 
     HIPCHK(ctx, hipMemcpy(r.d_ttab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
     r.tnx = nlat;
     r.tny = nlon;
+    r.thx = x;
+    r.thy = y;
+    r.t_max = 0.0;                                   // set_bndlyr starts from 0 (atmo_state.sph.rngdep.cpp:44-48)
+    for (size_t q = 0; q < (size_t)nlat * nlon; ++q) r.t_max = std::max(r.t_max, zg_km[q]);
     return GA_OK;
+}
+
+// fileIO.cpp:18-35: lines not starting with '#', minus one (a file written
+// with a trailing newline reads one empty line at the end).
+static int file_length(const std::string &path)
+{
+    std::ifstream in(path);
+    if (!in.is_open()) return 0;
+    int count = 0;
+    std::string line;
+    while (!in.eof()) {
+        std::getline(in, line);
+        if (line.find('#') != 0) ++count;
+    }
+    return count - 1;
+}
+
+// fileIO.cpp:64-85: rows "x y f" with y fastest; n2 = length of the first y run.
+static bool file_2d_dims(const std::string &path, int &n1, int &n2)
+{
+    const int N = file_length(path);
+    std::ifstream in(path);
+    if (!in.is_open() || N < 2) return false;
+    double xj, yj, fj, yprev;
+    in >> xj >> yprev >> fj;
+    n2 = 0;
+    for (int n = 1; n < N; ++n) {
+        in >> xj >> yj >> fj;
+        if (yj < yprev) { n2 = n; break; }
+        yprev = yj;
+    }
+    if (n2 < 1) return false;
+    n1 = N / n2;
+    return true;
+}
+
+int ga_rngdep_load_files(ga_ctx *ctx, const char *prefix, const char *lat_path, const char *lon_path,
+                         const char *topo_path, const char *format, int reverse_winds, int *rd_id)
+{
+    if (!ctx || !prefix || !lat_path || !lon_path || !rd_id) return fail(ctx, GA_E_ARG, "null argument");
+    const std::string fmt = format ? format : "zTuvdp";
+    const int need = (fmt.compare(0, 6, "zTuvdp") == 0) ? 6 : (fmt.compare(0, 7, "zuvwTdp") == 0) ? 7
+                     : (fmt.compare(0, 5, "zcuvd") == 0) ? 5 : -1;
+    if (need < 0) return fail(ctx, GA_E_ARG, "unrecognized profile format %s (zTuvdp, zuvwTdp, zcuvd)", fmt.c_str());
+    const int nlat = file_length(lat_path), nlon = file_length(lon_path);
+    if (nlat < 2 || nlon < 2) return fail(ctx, GA_E_IO, "invalid grid specification (check the node files %s, %s)", lat_path, lon_path);
+    std::vector<double> lat(nlat), lon(nlon);
+    {
+        std::ifstream a(lat_path), b(lon_path);
+        for (double &x : lat) if (!(a >> x)) return fail(ctx, GA_E_IO, "cannot read %s", lat_path);
+        for (double &x : lon) if (!(b >> x)) return fail(ctx, GA_E_IO, "cannot read %s", lon_path);
+    }
+    const std::string p0 = std::string(prefix) + "0.met";
+    const int nz = file_length(p0);
+    if (nz < 3) return fail(ctx, GA_E_IO, "invalid atmospheric specification (%s)", p0.c_str());
+    const size_t N = (size_t)nlat * nlon * nz;
+    std::vector<double> z(nz), c(N), u(N), v(N), rho(N);
+    for (int i = 0; i < nlat; ++i)
+        for (int j = 0; j < nlon; ++j) {
+            const std::string path = std::string(prefix) + std::to_string(i * nlon + j) + ".met";
+            std::ifstream in(path);
+            if (!in.is_open()) return fail(ctx, GA_E_IO, "invalid atmospheric specification (%s)", path.c_str());
+            std::string line;
+            int k = 0;
+            while (!in.eof() && k < nz) {
+                std::getline(in, line);
+                if (line.find('#') == 0) continue;
+                std::stringstream ss(line);
+                double vals[7];
+                int got = 0;
+                while (got < need && (ss >> vals[got])) ++got;
+                if (got < need) return fail(ctx, GA_E_IO, "short row %d in %s", k, path.c_str());
+                const size_t q = ((size_t)i * nlon + j) * nz + k;
+                double rr, cc, uu, vv;
+                if (need == 6) { uu = vals[2]; vv = vals[3]; rr = vals[4]; cc = std::sqrt(0.1 * 1.4 * vals[5] / rr) / 1000.0; }
+                else if (need == 7) { uu = vals[1]; vv = vals[2]; rr = vals[5]; cc = std::sqrt(0.1 * 1.4 * vals[6] / rr) / 1000.0; }
+                else { cc = vals[1] / 1000.0; uu = vals[2]; vv = vals[3]; rr = vals[4]; }
+                if (reverse_winds) { uu /= -1000.0; vv /= -1000.0; }
+                else { uu /= 1000.0; vv /= 1000.0; }
+                z[k] = vals[0];   // every profile writes the shared heights, as the reference's loader does
+                c[q] = cc; u[q] = uu; v[q] = vv; rho[q] = rr;
+                ++k;
+            }
+            if (k < nz) return fail(ctx, GA_E_IO, "%s has %d rows, expected %d", path.c_str(), k, nz);
+        }
+    int rc = ga_rngdep_load(ctx, lat.data(), nlat, lon.data(), nlon, z.data(), nz, c.data(), u.data(), v.data(),
+                            rho.data(), rd_id);
+    if (rc != GA_OK || !topo_path || !topo_path[0]) return rc;
+    int n1 = 0, n2 = 0;
+    if (!file_2d_dims(topo_path, n1, n2) || n1 < 2 || n2 < 2) {
+        ga_rngdep_release(ctx, *rd_id);
+        return fail(ctx, GA_E_IO, "invalid terrain file (%s)", topo_path);
+    }
+    std::vector<double> tl(n1), to(n2), tz((size_t)n1 * n2);
+    {
+        std::ifstream in(topo_path);
+        for (int i = 0; i < n1; ++i)
+            for (int j = 0; j < n2; ++j)
+                if (!(in >> tl[i] >> to[j] >> tz[(size_t)i * n2 + j])) {
+                    ga_rngdep_release(ctx, *rd_id);
+                    return fail(ctx, GA_E_IO, "cannot read terrain row %d in %s", i * n2 + j, topo_path);
+                }
+    }
+    rc = ga_topo_load(ctx, *rd_id, tl.data(), n1, to.data(), n2, tz.data());
+    if (rc != GA_OK) ga_rngdep_release(ctx, *rd_id);
+    return rc;
 }
 
 int ga_rngdep_eval(ga_ctx *ctx, int rd_id, const double *pts, int n, double *atmo_out, double *topo_out)

@@ -132,50 +132,74 @@ GA_HD double bicubic_poly(const double A[16], double xs, double ys, double dx, d
     return r;
 }
 
-// One field of the hybrid spline: f, df = {fx, fy, fz}, ddf = {fxx, fyy, fzz,
-// fxy, fxz, fyz} at (x, y, z) (interp::eval_all, interpolation.cpp:1618-1788).
-GA_HD void rd_field_all(const RdGrid &g, int field, double x, double y, double z, double &f, double df[3],
-                        double ddf[6])
+// Cell of a query point, shared by the fields of one grid: segments, the
+// clamped and scaled coordinates and the 4 x 4 node columns around the cell
+// (indices clamped at the grid edge, as the reference's finite differences
+// clamp their neighbours).
+struct RdCell {
+    int kz;
+    int ci[4], cj[4];
+    double dx, dy, dz, xs, ys, Xz, Xz_u;
+};
+
+GA_HD RdCell rd_cell(const RdGrid &g, double x, double y, double z)
 {
+    RdCell c;
     const double xe = fmin(fmax(x, g.x[0]), g.x[g.nx - 1]);
     const double ye = fmin(fmax(y, g.y[0]), g.y[g.ny - 1]);
     const double ze = fmin(fmax(z, g.z[0]), g.z[g.nz - 1]);
-    const int kx = seg_of(g.x, g.nx, xe), ky = seg_of(g.y, g.ny, ye), kz = seg_of(g.z, g.nz, ze);
-    const double dx = g.x[kx + 1] - g.x[kx], xs = (xe - g.x[kx]) / dx;
-    const double dy = g.y[ky + 1] - g.y[ky], ys = (ye - g.y[ky]) / dy;
-    const double dz = g.z[kz + 1] - g.z[kz];
-    const double Xz = (ze - g.z[kz]) / dz, Xz_u = (z - g.z[kz]) / dz;   // f_zz patch: unclamped z
-    int ci[4], cj[4];
+    const int kx = seg_of(g.x, g.nx, xe), ky = seg_of(g.y, g.ny, ye);
+    c.kz = seg_of(g.z, g.nz, ze);
+    c.dx = g.x[kx + 1] - g.x[kx];
+    c.xs = (xe - g.x[kx]) / c.dx;
+    c.dy = g.y[ky + 1] - g.y[ky];
+    c.ys = (ye - g.y[ky]) / c.dy;
+    c.dz = g.z[c.kz + 1] - g.z[c.kz];
+    c.Xz = (ze - g.z[c.kz]) / c.dz;
+    c.Xz_u = (z - g.z[c.kz]) / c.dz;   // f_zz patch: unclamped z (interpolation.cpp:1763-1771)
     for (int a = 0; a < 4; ++a) {
-        ci[a] = clampi(kx - 1 + a, 0, g.nx - 1);
-        cj[a] = clampi(ky - 1 + a, 0, g.ny - 1);
+        c.ci[a] = clampi(kx - 1 + a, 0, g.nx - 1);
+        c.cj[a] = clampi(ky - 1 + a, 0, g.ny - 1);
     }
+    return c;
+}
+
+// One field of the hybrid spline at a cell (interp::eval_f / eval_all,
+// interpolation.cpp:1334-1369, 1544-1788).  MODE 0: f; 1: also
+// df = {fx, fy, fz}; 2: also ddf = {fxx, fyy, fzz, fxy, fxz, fyz}.
+template <int MODE>
+GA_HD void rd_field_eval(const RdGrid &g, const RdCell &cl, int field, double &f, double *df, double *ddf)
+{
+    constexpr int NQ = MODE == 0 ? 1 : (MODE == 1 ? 2 : 5);
     // node quantities on the 4 x 4 columns: 0 f, 1 f_z, 2 f_zz, 3 x-diff, 4 y-diff
-    double Q[5][4][4];
+    double Q[NQ][4][4];
     const double *base = g.tab + (size_t)field * g.nx * g.ny * g.nz * RD_Q;
     for (int a = 0; a < 4; ++a)
         for (int b = 0; b < 4; ++b) {
-            const double *p = base + (((size_t)ci[a] * g.ny + cj[b]) * g.nz + kz) * RD_Q;
+            const double *p = base + (((size_t)cl.ci[a] * g.ny + cl.cj[b]) * g.nz + cl.kz) * RD_Q;
             const double *q = p + RD_Q;
-            Q[0][a][b] = hv(p[0], q[0], p[1], q[1], dz, Xz);
-            Q[1][a][b] = hd1(p[0], q[0], p[1], q[1], dz, Xz);
-            Q[2][a][b] = hd2(p[0], q[0], p[1], q[1], dz, Xz_u);
-            Q[3][a][b] = hv(p[2], q[2], p[3], q[3], dz, Xz);
-            Q[4][a][b] = hv(p[4], q[4], p[3], q[3], dz, Xz);
+            Q[0][a][b] = hv(p[0], q[0], p[1], q[1], cl.dz, cl.Xz);
+            if constexpr (NQ > 1) Q[1][a][b] = hd1(p[0], q[0], p[1], q[1], cl.dz, cl.Xz);
+            if constexpr (NQ > 2) {
+                Q[2][a][b] = hd2(p[0], q[0], p[1], q[1], cl.dz, cl.Xz_u);
+                Q[3][a][b] = hv(p[2], q[2], p[3], q[3], cl.dz, cl.Xz);
+                Q[4][a][b] = hv(p[4], q[4], p[3], q[3], cl.dz, cl.Xz);
+            }
         }
     // corner (a, b) in {1, 2}^2; differences over the clamped neighbours
     auto ddx = [&](int q, int a, int b) {
-        return (Q[q][a + 1][b] - Q[q][a - 1][b]) / (g.x[ci[a + 1]] - g.x[ci[a - 1]]);
+        return (Q[q][a + 1][b] - Q[q][a - 1][b]) / (g.x[cl.ci[a + 1]] - g.x[cl.ci[a - 1]]);
     };
     auto ddy = [&](int q, int a, int b) {
-        return (Q[q][a][b + 1] - Q[q][a][b - 1]) / (g.y[cj[b + 1]] - g.y[cj[b - 1]]);
+        return (Q[q][a][b + 1] - Q[q][a][b - 1]) / (g.y[cl.cj[b + 1]] - g.y[cl.cj[b - 1]]);
     };
     auto ddxy = [&](int q, int a, int b) {
         return (Q[q][a + 1][b + 1] - Q[q][a + 1][b - 1] - Q[q][a - 1][b + 1] + Q[q][a - 1][b - 1]) /
-               ((g.x[ci[a + 1]] - g.x[ci[a - 1]]) * (g.y[cj[b + 1]] - g.y[cj[b - 1]]));
+               ((g.x[cl.ci[a + 1]] - g.x[cl.ci[a - 1]]) * (g.y[cl.cj[b + 1]] - g.y[cl.cj[b - 1]]));
     };
     // corner order of the reference's X_vec: (kx,ky), (kx+1,ky), (kx,ky+1), (kx+1,ky+1)
     const int A_[4] = {1, 2, 1, 2}, B_[4] = {1, 1, 2, 2};
+    const double dx = cl.dx, dy = cl.dy, xs = cl.xs, ys = cl.ys;
     double X[16], C[16];
     // patch of quantity qv with x slopes from (qx, kind) etc.; kinds: 0 ddx, 1 ddy, 2 ddxy
     auto build = [&](int qv, int qxs, int kxs, int qys, int kys, int qxys) {
@@ -190,19 +214,34 @@ GA_HD void rd_field_all(const RdGrid &g, int field, double x, double y, double z
     };
     build(0, 0, 0, 0, 1, 0);                         // f
     f = bicubic_poly(C, xs, ys, dx, dy, 0);
-    df[0] = bicubic_poly(C, xs, ys, dx, dy, 1);
-    df[1] = bicubic_poly(C, xs, ys, dx, dy, 2);
-    ddf[3] = bicubic_poly(C, xs, ys, dx, dy, 3);
-    build(1, 1, 0, 1, 1, 1);                         // f_z
-    df[2] = bicubic_poly(C, xs, ys, dx, dy, 0);
-    ddf[4] = bicubic_poly(C, xs, ys, dx, dy, 1);
-    ddf[5] = bicubic_poly(C, xs, ys, dx, dy, 2);
-    build(3, 3, 0, 0, 2, 3);                         // x-difference: slopes ddx(Qx), ddxy(f), ddxy(Qx)
-    ddf[0] = bicubic_poly(C, xs, ys, dx, dy, 1);
-    build(4, 0, 2, 4, 1, 4);                         // y-difference: slopes ddxy(f), ddy(Qy), ddxy(Qy)
-    ddf[1] = bicubic_poly(C, xs, ys, dx, dy, 2);
-    build(2, 2, 0, 2, 1, 2);                         // f_zz
-    ddf[2] = bicubic_poly(C, xs, ys, dx, dy, 0);
+    if constexpr (MODE >= 1) {
+        df[0] = bicubic_poly(C, xs, ys, dx, dy, 1);
+        df[1] = bicubic_poly(C, xs, ys, dx, dy, 2);
+    }
+    if constexpr (MODE == 2) ddf[3] = bicubic_poly(C, xs, ys, dx, dy, 3);
+    if constexpr (MODE >= 1) {
+        build(1, 1, 0, 1, 1, 1);                     // f_z
+        df[2] = bicubic_poly(C, xs, ys, dx, dy, 0);
+    }
+    if constexpr (MODE == 2) {
+        ddf[4] = bicubic_poly(C, xs, ys, dx, dy, 1);
+        ddf[5] = bicubic_poly(C, xs, ys, dx, dy, 2);
+        build(3, 3, 0, 0, 2, 3);                     // x-difference: slopes ddx(Qx), ddxy(f), ddxy(Qx)
+        ddf[0] = bicubic_poly(C, xs, ys, dx, dy, 1);
+        build(4, 0, 2, 4, 1, 4);                     // y-difference: slopes ddxy(f), ddy(Qy), ddxy(Qy)
+        ddf[1] = bicubic_poly(C, xs, ys, dx, dy, 2);
+        build(2, 2, 0, 2, 1, 2);                     // f_zz
+        ddf[2] = bicubic_poly(C, xs, ys, dx, dy, 0);
+    }
+}
+
+// One field of the hybrid spline: f, df = {fx, fy, fz}, ddf = {fxx, fyy, fzz,
+// fxy, fxz, fyz} at (x, y, z) (interp::eval_all, interpolation.cpp:1618-1788).
+GA_HD void rd_field_all(const RdGrid &g, int field, double x, double y, double z, double &f, double df[3],
+                        double ddf[6])
+{
+    const RdCell c = rd_cell(g, x, y, z);
+    rd_field_eval<2>(g, c, field, f, df, ddf);
 }
 
 // Topography z_g(x, y) and its derivatives: out = {z, zx, zy, zxx, zxy, zyy,

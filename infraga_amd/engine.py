@@ -15,7 +15,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libinfraga_gpu.so")
 
-GA_GEOM_3D, GA_GEOM_SPH = 0, 1
+GA_GEOM_3D, GA_GEOM_SPH, GA_GEOM_SPH_RNGDEP = 0, 1, 2
 GA_OK, GA_E_ARG, GA_E_HIP, GA_E_NOMEM, GA_E_CAP, GA_E_NODEV, GA_E_ATMO, GA_E_IO = 0, -1, -2, -3, -4, -5, -6, -7
 
 GA_REQ_FAN, GA_REQ_EIG = 0, 1
@@ -26,7 +26,7 @@ EXPORTED = ["ga_prop_params_default", "ga_eig_params_default", "ga_ctx_create", 
             "ga_last_error", "ga_atmo_load_strat", "ga_atmo_load_file", "ga_atmo_release", "ga_fan_run",
             "ga_fan_run_device", "ga_eig_search", "ga_server_create", "ga_server_submit", "ga_server_poll",
             "ga_server_stats", "ga_server_destroy", "ga_rngdep_load", "ga_topo_load", "ga_rngdep_eval",
-            "ga_rngdep_release"]
+            "ga_rngdep_release", "ga_rngdep_load_files"]
 
 _dp = C.POINTER(C.c_double)
 
@@ -37,7 +37,7 @@ class PropParams(C.Structure):
                 ("src", C.c_double * 3), ("z_grnd", C.c_double), ("s_max", C.c_double),
                 ("ds_min", C.c_double), ("ds_max", C.c_double), ("alt_max", C.c_double),
                 ("rng_max", C.c_double), ("time_max", C.c_double), ("bounds", C.c_double * 4),
-                ("r_earth", C.c_double)]
+                ("r_earth", C.c_double), ("topo_use_blw", C.c_int32)]
 
 
 class EigParams(C.Structure):
@@ -112,6 +112,8 @@ def load_library(path: str | None = None):
                                  C.POINTER(C.c_int)]
     L.ga_topo_load.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp]
     L.ga_rngdep_eval.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int, _dp, _dp]
+    L.ga_rngdep_load_files.argtypes = [C.c_void_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p,
+                                       C.c_int, C.POINTER(C.c_int)]
     L.ga_rngdep_release.argtypes = [C.c_void_p, C.c_int]
     L.ga_server_create.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_void_p)]
     L.ga_server_submit.argtypes = [C.c_void_p, C.POINTER(Request), C.POINTER(C.c_uint64)]
@@ -236,6 +238,14 @@ class Engine:
 
     def rngdep_release(self, rid):
         self._check(self.L.ga_rngdep_release(self.h, rid))
+
+    def rngdep_load_files(self, prefix, lat_path, lon_path, topo_path=None, fmt="zTuvdp", reverse_winds=False) -> int:
+        """The reference's range-dependent file layout (node files, prefix{n}.met, terrain rows)."""
+        rid = C.c_int(0)
+        self._check(self.L.ga_rngdep_load_files(self.h, prefix.encode(), lat_path.encode(), lon_path.encode(),
+                                                topo_path.encode() if topo_path else None, fmt.encode(),
+                                                int(reverse_winds), C.byref(rid)))
+        return rid.value
 
     def eig_search(self, aid, params: PropParams, eig: EigParams, rcvr_xy):
         R = np.ascontiguousarray(rcvr_xy, dtype=np.float64).reshape(-1, 2)

@@ -32,11 +32,28 @@ def test_library_exports_every_declared_symbol():
         getattr(lib, n)
 
 
-def test_struct_sizes():
-    # int + 2 ints + doubles, natural alignment (matches the C header)
-    assert C.sizeof(engine.Arrival) == 8 + 11 * 8
-    assert C.sizeof(engine.RunStats) == 8 + 8 + 8 + 4 + 4
-    assert C.sizeof(engine.PropParams) == 3 * 4 + 4 + 3 * 8 + 3 * 8 + 8 + 3 * 8 + 3 * 8 + 4 * 8 + 8
+def test_struct_layouts_match_the_c_header(tmp_path):
+    """ctypes mirrors of the ABI structs have the C compiler's size and
+    field offsets for include/infraga_gpu.h."""
+    hdr = os.path.join(ROOT, "include", "infraga_gpu.h")
+    structs = {"ga_prop_params": engine.PropParams, "ga_arrival": engine.Arrival, "ga_run_stats": engine.RunStats,
+               "ga_eig_params": engine.EigParams, "ga_request": engine.Request, "ga_result": engine.Result}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "%s"' % hdr, "int main(void) {"]
+    for cname, cls in structs.items():
+        lines.append('printf("%s size %%zu\\n", sizeof(%s));' % (cname, cname))
+        for f, _ in cls._fields_:
+            lines.append('printf("%s %s %%zu\\n", offsetof(%s, %s));' % (cname, f, cname, f))
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.check_call(["gcc", "-o", str(exe), str(src)])
+    out = subprocess.check_output([str(exe)], text=True).split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l}
+    for cname, cls in structs.items():
+        assert got[(cname, "size")] == C.sizeof(cls), cname
+        for f, _ in cls._fields_:
+            assert got[(cname, f)] == getattr(cls, f).offset, (cname, f)
 
 
 def test_defaults_follow_reference_driver():
