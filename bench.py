@@ -24,6 +24,14 @@ The same trace runs through the request server twice: continuous batching
 (4 launch slots, eigenray queries merged into running searches) and one
 request at a time; both report p50/p99 request latency and ray-steps/s.
 
+A fourth leg times the range-dependent spherical fan over terrain: 11 x 11
+profiles (2 degree nodes over 30..50 N, 110..90 W; the base profile plus a
+seeded random field, +-15 m/s u and v, +-5 K T, 500 km correlation; 1501
+levels to 150 km), 401 x 401 fractal terrain (0.05 degree, 0..2 km), source
+at the region centre on the ground, inclinations 0.2..40.0 step 0.2 x
+azimuths 0..359 step 1 (72,000 rays), 4 bounces, 18 equations; the oracle on
+a few strided rays is its CPU baseline.
+
 Prints one JSON line.  --gpus N with N > 1 exits: this build runs one GPU.
 """
 from __future__ import annotations
@@ -115,6 +123,55 @@ def eig_leg(eng, with_cpu):
     return out
 
 
+def rngdep_workload():
+    from infraga_amd import synth
+    units = synth.rngdep_engine_units(*synth.rngdep_grid())
+    topo = synth.fractal_topo()
+    L = synth.fan_launches(0.2, 40.0, 0.2, 0.0, 359.0, 1.0)
+    return units, topo, L
+
+
+def rngdep_cpu_baseline(units, topo, L, target_s=12.0):
+    """The oracle's range-dependent fan (one host core) on evenly strided
+    rays of the same launch list until target_s of CPU time is spent."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from ga_oracle_py import Oracle, RdMedium
+    m = RdMedium(Oracle(), *units)
+    m.set_topo(*topo)
+    p = m.params(True)
+    steps, n, t0 = 0, 0, time.process_time()
+    for i in range(0, len(L), 4801):
+        _, st = m.fan(L[i:i + 1], p, bounces=4, src=None)
+        steps += st
+        n += 1
+        if time.process_time() - t0 > target_s:
+            break
+    dt = time.process_time() - t0
+    return {"value": steps / dt, "unit": "ray-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{n} rays (every 4801st launch) of the 72k range-dependent fan, {steps} ray-steps, {dt:.1f} s"}
+
+
+def rngdep_leg(eng, with_cpu):
+    from infraga_amd.engine import GA_GEOM_SPH_RNGDEP, prop_params
+    units, topo, L = rngdep_workload()
+    rid = eng.rngdep_load(*units)
+    eng.topo_load(rid, *topo)
+    p = prop_params(GA_GEOM_SPH_RNGDEP, calc_amp=1, bounces=4)
+    eng.fan(rid, p, L[::997])                        # warm-up (buffers, kernel load)
+    t0 = time.perf_counter()
+    arr, st = eng.fan(rid, p, L)
+    wall = time.perf_counter() - t0
+    eng.rngdep_release(rid)
+    out = {"workload": "sph range-dependent: 11 x 11 profiles x 1501 levels, 401 x 401 terrain 0-2 km, "
+                       "200 incl x 360 az = 72000 rays, 4 bounces, 18 eq",
+           "rays": int(len(L)), "ray_steps": int(st.ray_steps), "arrivals": int(len(arr)),
+           "kernel_ms": st.kernel_ms, "wall_ms": wall * 1e3, "ray_steps_per_s": st.ray_steps / wall,
+           "reflections": int((arr["n_bounce"] > 0).sum())}
+    if with_cpu:
+        out["cpu_baseline"] = rngdep_cpu_baseline(units, topo, L)
+    return out
+
+
 def serve_trace(n_req=24, qps=1.5, seed=11):
     """Seeded request trace: (arrival_s, kind, atmosphere index, size)."""
     rng = np.random.Generator(np.random.PCG64(seed))
@@ -201,6 +258,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-eig", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-rngdep", action="store_true")
     ap.add_argument("--one-pass", action="store_true",
                     help="one untimed fan pass (for hardware-counter runs); no JSON line")
     ap.add_argument("--calibrate", action="store_true",
@@ -290,6 +348,8 @@ def main():
         line["eigenray"] = eig_leg(eng, not args.no_cpu)
     if not args.no_serve:
         line["served"] = serve_leg(eng, serve_trace())
+    if not args.no_rngdep:
+        line["rngdep"] = rngdep_leg(eng, not args.no_cpu)
     eng.close()
     print(json.dumps(line))
 

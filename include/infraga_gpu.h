@@ -195,6 +195,13 @@ int ga_rngdep_release(ga_ctx *ctx, int rd_id);
 int ga_rngdep_load_files(ga_ctx *ctx, const char *prefix, const char *lat_path, const char *lon_path,
                          const char *topo_path, const char *format, int reverse_winds, int *rd_id);
 
+/* The region and ground a loaded range-dependent atmosphere defines
+ * (set_limits, atmo_io.sph.rngdep.cpp:27-43; set_bndlyr,
+ * atmo_state.sph.rngdep.cpp:42-55): out = {lat_min, lat_max, lon_min,
+ * lon_max [rad], alt_max [km], z0 (profile bottom) [km], z_max (terrain
+ * maximum + 0.01, or z0) [km], boundary-layer top [km]}. */
+int ga_rngdep_info(ga_ctx *ctx, int rd_id, double out[8]);
+
 /* Range-dependent fans run through ga_fan_run / ga_fan_run_device with
  * p->geometry = GA_GEOM_SPH_RNGDEP and atmo_id a ga_rngdep_load id
  * (run_prop, src/main.infraga.sph.rngdep.cpp:412-554).  p->src[0..1] NaN =

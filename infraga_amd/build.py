@@ -40,7 +40,8 @@ def build_frontends():
     if not os.path.exists(src):
         return []
     outs = []
-    for name, geom in (("infraga-sph", "GA_GEOM_SPH"), ("infraga-3d", "GA_GEOM_3D")):
+    for name, geom in (("infraga-sph", "GA_GEOM_SPH"), ("infraga-3d", "GA_GEOM_3D"),
+                       ("infraga-sph-rngdep", "GA_GEOM_SPH_RNGDEP")):
         out = os.path.join(BIN, name)
         _run(["g++", "-O2", "-std=c++17", f"-DGA_CLI_GEOM={geom}", f"-DGA_CLI_NAME=\"{name}\"",
               "-I", os.path.join(ROOT, "include"), "-o", out, src, "-L", HERE, "-linfraga_gpu",

@@ -404,6 +404,53 @@ int launch_fan(ga_ctx *ctx, FanBufs &fb, const FanArgs &A, hipStream_t st)
 // Set by the request server around its fan launches (one ray per lane).
 thread_local int serve_one_shot = 0;
 
+// Inverse spacing of an evenly spaced axis (nodes within 1e-6 of a spacing
+// of the uniform positions), else 0; the segment search uses it only as a
+// first guess, so any value gives the same segment.
+double axis_inv_spacing(const std::vector<double> &v)
+{
+    const int n = (int)v.size();
+    if (n < 2) return 0.0;
+    const double d = (v[n - 1] - v[0]) / (n - 1);
+    if (!(d > 0.0)) return 0.0;
+    for (int i = 0; i < n; ++i)
+        if (std::fabs(v[i] - (v[0] + i * d)) > 1e-6 * d) return 0.0;
+    return 1.0 / d;
+}
+
+// Region and ground of a range-dependent atmosphere (set_limits,
+// atmo_io.sph.rngdep.cpp:27-43, and set_bndlyr, atmo_state.sph.rngdep.cpp:
+// 42-55); z_grnd (NaN: profile bottom) applies without terrain only
+// (main.infraga.sph.rngdep.cpp:294-300).
+struct RdRegion {
+    double lat_min, lat_max, lon_min, lon_max, alt_max, z0, z_max, z_bndlyr;
+};
+
+bool rd_region(const DevRd &R, double z_grnd, RdRegion &G)
+{
+    const double buffer = 1.0e-3 * (PI / 180.0);
+    G.lat_min = R.hx.front() + buffer;
+    G.lat_max = R.hx.back() - buffer;
+    G.lon_min = R.hy.front() + buffer;
+    G.lon_max = R.hy.back() - buffer;
+    G.alt_max = R.hz.back();
+    G.z0 = R.hz.front();
+    if (R.d_ttab) {
+        G.lat_min = std::max(G.lat_min, R.thx.front() + buffer);
+        G.lat_max = std::min(G.lat_max, R.thx.back() - buffer);
+        G.lon_min = std::max(G.lon_min, R.thy.front() + buffer);
+        G.lon_max = std::min(G.lon_max, R.thy.back() - buffer);
+        G.z_max = R.t_max;
+        G.z_bndlyr = G.z_max + 2.0;
+        G.z_max += 0.01;
+        return !(G.lat_max < G.lat_min || G.lon_max < G.lon_min);
+    }
+    if (!std::isnan(z_grnd)) G.z0 = z_grnd;
+    G.z_max = G.z0;
+    G.z_bndlyr = G.z0 + 2.0;
+    return true;
+}
+
 // Range-dependent fan: medium and region as set_limits / set_bndlyr derive
 // them (atmo_io.sph.rngdep.cpp:27-43, atmo_state.sph.rngdep.cpp:42-55), the
 // source and the reference's z_grnd handling from run_prop
@@ -427,6 +474,9 @@ int rd_fan_enqueue(ga_ctx *ctx, FanBufs &fb, int rd_id, const ga_prop_params *p,
     M.g.ny = R.ny;
     M.g.nz = R.nz;
     M.g.tab = R.d_tab;
+    M.g.ix = axis_inv_spacing(R.hx);
+    M.g.iy = axis_inv_spacing(R.hy);
+    M.g.iz = axis_inv_spacing(R.hz);
     M.is_topo = R.d_ttab ? 1 : 0;
     if (M.is_topo) {
         M.t.x = R.d_taxes;
@@ -434,6 +484,8 @@ int rd_fan_enqueue(ga_ctx *ctx, FanBufs &fb, int rd_id, const ga_prop_params *p,
         M.t.nx = R.tnx;
         M.t.ny = R.tny;
         M.t.tab = R.d_ttab;
+        M.t.ix = axis_inv_spacing(R.thx);
+        M.t.iy = axis_inv_spacing(R.thy);
     }
     M.use_blw = p->topo_use_blw ? 1 : 0;
     M.r0 = p->r_earth;
@@ -441,25 +493,12 @@ int rd_fan_enqueue(ga_ctx *ctx, FanBufs &fb, int rd_id, const ga_prop_params *p,
     M.sbc[0] = std::pow(10.0, -0.67887);
     M.sbc[1] = std::pow(10.0, -0.10744);
     M.sbc[2] = std::pow(10.0, -3.3979);
-    const double buffer = 1.0e-3 * (PI / 180.0);
-    double lat_min = R.hx.front() + buffer, lat_max = R.hx.back() - buffer;
-    double lon_min = R.hy.front() + buffer, lon_max = R.hy.back() - buffer;
-    if (M.is_topo) {
-        lat_min = std::max(lat_min, R.thx.front() + buffer);
-        lat_max = std::min(lat_max, R.thx.back() - buffer);
-        lon_min = std::max(lon_min, R.thy.front() + buffer);
-        lon_max = std::min(lon_max, R.thy.back() - buffer);
-        if (lat_max < lat_min || lon_max < lon_min)
-            return fail(ctx, GA_E_ARG, "atmosphere grid and topography grid have no overlap");
-        M.z0 = R.hz.front();
-        M.z_max = R.t_max;
-        M.z_bndlyr = M.z_max + 2.0;
-        M.z_max += 0.01;
-    } else {
-        M.z0 = std::isnan(p->z_grnd) ? R.hz.front() : p->z_grnd;
-        M.z_max = M.z0;
-        M.z_bndlyr = M.z0 + 2.0;
-    }
+    RdRegion G;
+    if (!rd_region(R, p->z_grnd, G)) return fail(ctx, GA_E_ARG, "atmosphere grid and topography grid have no overlap");
+    const double lat_min = G.lat_min, lat_max = G.lat_max, lon_min = G.lon_min, lon_max = G.lon_max;
+    M.z0 = G.z0;
+    M.z_max = G.z_max;
+    M.z_bndlyr = G.z_bndlyr;
     Limits &L = A.L;
     L.s_max = p->s_max;
     L.ds_min = p->ds_min;
@@ -1668,6 +1707,18 @@ int ga_topo_load(ga_ctx *ctx, int rd_id, const double *lat_deg, int nlat, const 
     return GA_OK;
 }
 
+int ga_rngdep_info(ga_ctx *ctx, int rd_id, double out[8])
+{
+    if (!ctx || !out) return fail(ctx, GA_E_ARG, "null argument");
+    auto it = ctx->rds.find(rd_id);
+    if (it == ctx->rds.end()) return fail(ctx, GA_E_ATMO, "unknown range-dependent atmosphere id %d", rd_id);
+    RdRegion G;
+    if (!rd_region(it->second, NAN, G)) return fail(ctx, GA_E_ARG, "atmosphere grid and topography grid have no overlap");
+    out[0] = G.lat_min; out[1] = G.lat_max; out[2] = G.lon_min; out[3] = G.lon_max;
+    out[4] = G.alt_max; out[5] = G.z0; out[6] = G.z_max; out[7] = G.z_bndlyr;
+    return GA_OK;
+}
+
 // fileIO.cpp:18-35: lines not starting with '#', minus one (a file written
 // with a trailing newline reads one empty line at the end).
 static int file_length(const std::string &path)
@@ -1783,8 +1834,10 @@ int ga_rngdep_eval(ga_ctx *ctx, int rd_id, const double *pts, int n, double *atm
     if (n == 0) return GA_OK;
     HIPCHK(ctx, hipSetDevice(ctx->device));
     const DevRd &r = it->second;
-    RdGrid g{r.d_axes, r.d_axes + r.nx, r.d_axes + r.nx + r.ny, r.nx, r.ny, r.nz, r.d_tab};
-    TopoGrid t{r.d_taxes, r.d_taxes ? r.d_taxes + r.tnx : nullptr, r.tnx, r.tny, r.d_ttab};
+    RdGrid g{r.d_axes, r.d_axes + r.nx, r.d_axes + r.nx + r.ny, r.nx, r.ny, r.nz, r.d_tab,
+             axis_inv_spacing(r.hx), axis_inv_spacing(r.hy), axis_inv_spacing(r.hz)};
+    TopoGrid t{r.d_taxes, r.d_taxes ? r.d_taxes + r.tnx : nullptr, r.tnx, r.tny, r.d_ttab,
+               axis_inv_spacing(r.thx), axis_inv_spacing(r.thy)};
     DevBuf<double> d_p, d_a, d_t;
     if (!d_p.alloc((size_t)n * 3) || !d_a.alloc((size_t)n * 40) || !d_t.alloc((size_t)n * 10))
         return fail(ctx, GA_E_NOMEM, "evaluation buffers");

@@ -30,16 +30,20 @@ namespace ga {
 constexpr int RD_Q = 5;    // doubles per hybrid-spline node
 constexpr int TP_Q = 4;    // doubles per topography node
 
+// ix, iy, iz: inverse node spacing of an evenly spaced axis (the segment is
+// then found in O(1)), 0 for a binary search.
 struct RdGrid {
     const double *x, *y, *z;   // node coordinates (lat, lon [rad]; z [km])
     int nx, ny, nz;
     const double *tab;         // [field][nx][ny][nz][RD_Q]
+    double ix, iy, iz;
 };
 
 struct TopoGrid {
     const double *x, *y;       // lat, lon [rad]
     int nx, ny;
     const double *tab;         // [nx][ny][TP_Q]
+    double ix, iy;
 };
 
 GA_HD int clampi(int i, int lo, int hi) { return i < lo ? lo : (i > hi ? hi : i); }
@@ -54,6 +58,18 @@ GA_HD int seg_of(const double *v, int n, double q)
         else hi = mid - 1;
     }
     return lo;
+}
+
+// The same segment as seg_of: an estimate from the mean spacing, then
+// exact steps to the largest k with v[k] <= q (q inside [v[0], v[n-1]]).
+GA_HD int seg_at(const double *v, int n, double q, double iv)
+{
+    if (!(iv > 0.0)) return seg_of(v, n, q);
+    int k = (int)((q - v[0]) * iv);
+    k = k < 0 ? 0 : (k > n - 2 ? n - 2 : k);
+    while (k < n - 2 && v[k + 1] <= q) ++k;
+    while (k > 0 && v[k] > q) --k;
+    return k;
 }
 
 GA_HD double hv(double f0, double f1, double s0, double s1, double d, double X)
@@ -148,8 +164,8 @@ GA_HD RdCell rd_cell(const RdGrid &g, double x, double y, double z)
     const double xe = fmin(fmax(x, g.x[0]), g.x[g.nx - 1]);
     const double ye = fmin(fmax(y, g.y[0]), g.y[g.ny - 1]);
     const double ze = fmin(fmax(z, g.z[0]), g.z[g.nz - 1]);
-    const int kx = seg_of(g.x, g.nx, xe), ky = seg_of(g.y, g.ny, ye);
-    c.kz = seg_of(g.z, g.nz, ze);
+    const int kx = seg_at(g.x, g.nx, xe, g.ix), ky = seg_at(g.y, g.ny, ye, g.iy);
+    c.kz = seg_at(g.z, g.nz, ze, g.iz);
     c.dx = g.x[kx + 1] - g.x[kx];
     c.xs = (xe - g.x[kx]) / c.dx;
     c.dy = g.y[ky + 1] - g.y[ky];
@@ -250,7 +266,7 @@ GA_HD void topo_all(const TopoGrid &g, double x, double y, double out[10])
 {
     const double xe = fmin(fmax(x, g.x[0]), g.x[g.nx - 1]);
     const double ye = fmin(fmax(y, g.y[0]), g.y[g.ny - 1]);
-    const int kx = seg_of(g.x, g.nx, xe), ky = seg_of(g.y, g.ny, ye);
+    const int kx = seg_at(g.x, g.nx, xe, g.ix), ky = seg_at(g.y, g.ny, ye, g.iy);
     const double dx = g.x[kx + 1] - g.x[kx], X = (xe - g.x[kx]) / dx;
     const double dy = g.y[ky + 1] - g.y[ky], Y = (ye - g.y[ky]) / dy;
     const double *n00 = g.tab + ((size_t)kx * g.ny + ky) * TP_Q, *n01 = n00 + TP_Q;

@@ -1,7 +1,10 @@
-// infraga_cli.cpp -- drop-in front ends infraga-sph / infraga-3d on the GPU engine.
+// infraga_cli.cpp -- drop-in front ends infraga-sph / infraga-3d /
+// infraga-sph-rngdep on the GPU engine.
 //
 // Same command line as the reference drivers (src/main.infraga.sph.cpp,
-// src/main.infraga.3d.cpp): an option (-prop, -eig_search), the profile path,
+// src/main.infraga.3d.cpp, src/main.infraga.sph.rngdep.cpp): an option
+// (-prop, -eig_search; the range-dependent binary: -prop with the profile
+// prefix and the two node files), the profile path,
 // then key=value parameters with the same names, aliases and defaults.  The
 // ray tracing runs on the MI355X through the C ABI of libinfraga_gpu.so; this
 // file only parses arguments, builds the launch list with the reference's
@@ -10,8 +13,9 @@
 // Differences from the reference drivers, by design:
 //  - an unknown key is an error (exit 2) instead of an interactive prompt;
 //  - write_rays=true writes the arrivals but not <id>.raypaths.dat (ray path
-//    sampling is not part of this engine); write_caustics, write_topo,
-//    topo_file and topo_use_BLw are rejected;
+//    sampling is not part of this engine); write_caustics and write_topo are
+//    rejected, and so are topo_file / topo_use_BLw outside the
+//    range-dependent binary;
 //  - -eig_search writes <id>.arrivals.dat but not the per-eigenray path
 //    files <id>.eigenray-N.dat; verbose=true prints no trace-by-trace log.
 #include <cmath>
@@ -237,6 +241,169 @@ int run_prop(int argc, char **argv)
     return 0;
 }
 
+// infraga-sph-rngdep -prop prefix lats.dat lons.dat [key=value ...]
+// (run_prop, src/main.infraga.sph.rngdep.cpp:205-562).
+int run_prop_rd(int argc, char **argv)
+{
+    if (argc < 5) {
+        std::cerr << GA_CLI_NAME << ": usage: -prop <profile prefix> <lat nodes> <lon nodes> [parameters]\n";
+        return 2;
+    }
+    double theta_min = 0.5, theta_max = 45.0, theta_step = 0.5;
+    double phi_min = -90.0, phi_max = -90.0, phi_step = 1.0;
+    int bounces = 2;
+    double lat_src = NAN, lon_src = NAN, z_src = 0.0;
+    bool write_rays = true, reverse_winds = false, custom_id = false;
+    double freq = 0.1, turn_ht_min = 0.2;
+    const char *prof_format = "zTuvdp";
+    const char *topo_file = nullptr;
+    const char *output_id = nullptr;
+    ga_prop_params p;
+    ga_prop_params_default(&p, GA_GEOM_SPH_RNGDEP);
+    p.calc_amp = 1;                                    // run_prop :226
+    double z_grnd = NAN;
+
+    for (int i = 5; i < argc; i++) {
+        const char *a = argv[i], *v = nullptr;
+        if (key2(a, "incl_min=", "min_incl=", v)) theta_min = std::atof(v);
+        else if (key2(a, "incl_max=", "max_incl=", v)) theta_max = std::atof(v);
+        else if (key2(a, "az_min=", "min_az=", v)) phi_min = std::atof(v);
+        else if (key2(a, "az_max=", "max_az=", v)) phi_max = std::atof(v);
+        else if (key(a, "incl_step=", v)) theta_step = std::atof(v);
+        else if (key(a, "az_step=", v)) phi_step = std::atof(v);
+        else if (key(a, "inclination=", v)) { theta_min = theta_max = std::atof(v); theta_step = 1.0; }
+        else if (key(a, "azimuth=", v)) { phi_min = phi_max = std::atof(v); phi_step = 1.0; }
+        else if (key(a, "bounces=", v)) bounces = std::atoi(v);
+        else if (key2(a, "src_alt=", "alt_src=", v)) z_src = std::atof(v);
+        else if (key2(a, "src_lat=", "lat_src=", v)) lat_src = std::atof(v);
+        else if (key2(a, "src_lon=", "lon_src=", v)) lon_src = std::atof(v);
+        else if (key(a, "turn_ht_min=", v)) turn_ht_min = std::atof(v);
+        else if (key(a, "write_rays=", v)) write_rays = str2bool(v);
+        else if (key(a, "freq=", v)) freq = std::atof(v);
+        else if (key(a, "abs_coeff=", v)) p.abs_coeff = std::max(0.0, std::atof(v));
+        else if (key(a, "calc_amp=", v)) p.calc_amp = str2bool(v) ? 1 : 0;
+        else if (key2(a, "max_alt=", "alt_max=", v)) p.alt_max = std::atof(v);
+        else if (key2(a, "max_rng=", "rng_max=", v)) p.rng_max = std::atof(v);
+        else if (key2(a, "max_time=", "time_max=", v)) p.time_max = std::atof(v);
+        else if (key2(a, "min_lat=", "lat_min=", v)) p.bounds[0] = std::atof(v);
+        else if (key2(a, "max_lat=", "lat_max=", v)) p.bounds[1] = std::atof(v);
+        else if (key2(a, "min_lon=", "lon_min=", v)) p.bounds[2] = std::atof(v);
+        else if (key2(a, "max_lon=", "lon_max=", v)) p.bounds[3] = std::atof(v);
+        else if (key2(a, "min_ds=", "ds_min=", v)) p.ds_min = std::atof(v);
+        else if (key2(a, "max_ds=", "ds_max=", v)) p.ds_max = std::atof(v);
+        else if (key2(a, "max_s=", "s_max=", v)) p.s_max = std::atof(v);
+        else if (key(a, "write_atmo=", v)) { /* profile dump: not produced */ }
+        else if (key(a, "prof_format=", v)) prof_format = v;
+        else if (key(a, "reverse_winds=", v)) reverse_winds = str2bool(v);
+        else if (key(a, "output_id=", v)) { custom_id = true; output_id = v; }
+        else if (key(a, "z_grnd=", v)) z_grnd = std::atof(v);
+        else if (key(a, "topo_file=", v)) topo_file = v;
+        else if (key(a, "topo_use_BLw=", v)) p.topo_use_blw = str2bool(v) ? 1 : 0;
+        else if (key(a, "print_resid=", v)) { /* diagnostics only */ }
+        else if (key(a, "write_caustics=", v) || key(a, "write_topo=", v)) {
+            if (str2bool(v)) {
+                std::cerr << GA_CLI_NAME << ": " << a << " is not supported by this engine\n";
+                return 2;
+            }
+        } else {
+            std::cerr << GA_CLI_NAME << ": unrecognized parameter entry: " << a << '\n';
+            return 2;
+        }
+    }
+
+    ga_ctx *ctx = nullptr;
+    if (ga_ctx_create(&ctx, 0) != GA_OK) return die(nullptr, "no usable GPU");
+    int rid = 0;
+    if (ga_rngdep_load_files(ctx, argv[2], argv[3], argv[4], topo_file, prof_format, reverse_winds ? 1 : 0, &rid) != GA_OK)
+        return die(ctx, "cannot load the range-dependent atmosphere");
+    double info[8];
+    if (ga_rngdep_info(ctx, rid, info) != GA_OK) return die(ctx, "region");
+    const bool is_topo = topo_file != nullptr;
+    if (std::isnan(lat_src)) lat_src = (info[1] + info[0]) / 2.0 * (180.0 / Pi);   // :242-243
+    if (std::isnan(lon_src)) lon_src = (info[3] + info[2]) / 2.0 * (180.0 / Pi);
+    const double z0 = (!is_topo && !std::isnan(z_grnd)) ? z_grnd : info[5];
+    if (!is_topo) p.z_grnd = z0;
+
+    // ground under the source and its slope (topo::z, topo::dz)
+    const double la = lat_src * (Pi / 180.0), lo = lon_src * (Pi / 180.0);
+    double pt[3] = {la, lo, 0.0}, atmo_out[40], topo_out[10];
+    if (ga_rngdep_eval(ctx, rid, pt, 1, atmo_out, topo_out) != GA_OK) return die(ctx, "terrain at the source");
+    const double zg = is_topo ? topo_out[0] : z0;
+    z_src = std::max(z_src, zg);                                                 // :315
+
+    p.bounces = bounces;
+    p.freq = freq;
+    p.turn_ht_min = turn_ht_min;
+    p.src[0] = lat_src;
+    p.src[1] = lon_src;
+    p.src[2] = z_src;
+
+    // launch list: the reference's accumulated loops with the terrain-slope
+    // floor on the inclination (:412-428)
+    std::vector<double> launches;
+    std::vector<int> az_end;
+    const double r0 = p.r_earth;
+    for (double phi = phi_min; phi <= phi_max; phi += phi_step) {
+        double theta_grnd;
+        if (z_src - zg < 0.01) {
+            if (is_topo) {
+                const double dzgdx = 1.0 / (r0 * std::cos(la)) * topo_out[2];
+                const double dzgdy = 1.0 / r0 * topo_out[1];
+                theta_grnd = std::atan(dzgdx * std::sin(phi * Pi / 180.0) + dzgdy * std::cos(phi * Pi / 180.0)) * (180.0 / Pi) + 0.1;
+            } else {
+                theta_grnd = 0.1;
+            }
+        } else {
+            theta_grnd = -89.9;
+        }
+        for (double theta = std::max(theta_min, theta_grnd); theta <= std::max(theta_max, theta_grnd); theta += theta_step) {
+            launches.push_back(theta);
+            launches.push_back(phi);
+        }
+        az_end.push_back((int)(launches.size() / 2));
+    }
+    const int n = (int)(launches.size() / 2);
+    std::vector<ga_arrival> arr((size_t)std::max(1, n) * (bounces + 1));
+    int n_out = 0;
+    ga_run_stats st;
+    if (ga_fan_run(ctx, rid, &p, launches.data(), n, arr.data(), (int)arr.size(), &n_out, &st) != GA_OK)
+        return die(ctx, "fan failed");
+
+    std::string id = custom_id ? std::string(output_id) : std::string(argv[2]);
+    std::ofstream results(id + ".arrivals.dat");
+    results << "# " << GA_CLI_NAME << " -prop summary:" << '\n';
+    results << "#" << '\t' << "profile: " << argv[2] << '\n';
+    results << "#" << '\t' << "inclination: " << theta_min << ", " << theta_max << ", " << theta_step << '\n';
+    results << "#" << '\t' << "azimuth: " << phi_min << ", " << phi_max << ", " << phi_step << '\n';
+    results << "#" << '\t' << "bounces: " << bounces << '\n';
+    results << "#" << '\t' << "source location (lat, lon, alt): " << lat_src << ", " << lon_src << ", " << z_src << '\n';
+    if (!is_topo) results << "#" << '\t' << "ground elevation: " << z0 << '\n';
+    else results << "#" << '\t' << "topo file:" << topo_file << '\n';
+    results << "#" << '\t' << "frequency: " << freq << '\n';
+    results << "#" << '\t' << "S&B atten coeff: " << p.abs_coeff << '\n' << '\n';
+    results << "# incl [deg]" << '\t' << "az [deg]" << '\t' << "n_b" << '\t' << "lat_0 [deg]" << '\t' << "lon_0 [deg]"
+            << '\t' << "time [s]" << '\t' << "cel [km/s]" << '\t' << "turning ht [km]" << '\t' << "inclination [deg]"
+            << '\t' << "back azimuth [deg]" << '\t' << "trans. coeff. [dB]" << '\t' << "absorption [dB]" << '\n';
+    int w = 0, az_i = 0;
+    for (int r = 0; r < n; ++r) {
+        while (w < n_out && arr[w].launch_idx == r) {
+            const ga_arrival &A = arr[w++];
+            results << A.incl << '\t' << A.az << '\t' << A.n_bounce;
+            results << '\t' << std::setprecision(8) << A.pos0 << '\t' << std::setprecision(8) << A.pos1;
+            results << '\t' << A.time << '\t' << A.cel << '\t' << A.turn_ht << '\t' << A.incl_arr << '\t' << A.back_az;
+            results << '\t' << A.tl << '\t' << A.absorption << '\n';
+        }
+        if (r + 1 == az_end[az_i]) { results << '\n'; ++az_i; }
+    }
+    results.close();
+    if (write_rays)
+        std::cout << GA_CLI_NAME << ": note: ray paths are not written by this engine (arrivals only)\n";
+    std::cout << GA_CLI_NAME << ": " << n << " rays, " << n_out << " arrivals, " << st.ray_steps << " ray-steps, "
+              << st.kernel_ms << " ms on the GPU\n";
+    ga_ctx_destroy(ctx);
+    return 0;
+}
+
 // The profile's lowest altitude (the default ground height).
 double profile_bottom(const char *path)
 {
@@ -398,6 +565,11 @@ int run_eig_search(int argc, char **argv)
 int main(int argc, char **argv)
 {
     if (argc < 3) { usage(); return argc < 2 ? 0 : 2; }
+    if (GA_CLI_GEOM == GA_GEOM_SPH_RNGDEP) {
+        if (std::strncmp(argv[1], "-prop", 5) == 0) return run_prop_rd(argc, argv);
+        std::cerr << GA_CLI_NAME << ": option " << argv[1] << " is not supported by this engine\n";
+        return 2;
+    }
     if (std::strncmp(argv[1], "-prop", 5) == 0) return run_prop(argc, argv);
     if (std::strncmp(argv[1], "-eig_search", 11) == 0) return run_eig_search(argc, argv);
     std::cerr << GA_CLI_NAME << ": option " << argv[1] << " is not supported by this engine\n";

@@ -26,7 +26,7 @@ EXPORTED = ["ga_prop_params_default", "ga_eig_params_default", "ga_ctx_create", 
             "ga_last_error", "ga_atmo_load_strat", "ga_atmo_load_file", "ga_atmo_release", "ga_fan_run",
             "ga_fan_run_device", "ga_eig_search", "ga_server_create", "ga_server_submit", "ga_server_poll",
             "ga_server_stats", "ga_server_destroy", "ga_rngdep_load", "ga_topo_load", "ga_rngdep_eval",
-            "ga_rngdep_release", "ga_rngdep_load_files"]
+            "ga_rngdep_release", "ga_rngdep_load_files", "ga_rngdep_info"]
 
 _dp = C.POINTER(C.c_double)
 
@@ -112,6 +112,7 @@ def load_library(path: str | None = None):
                                  C.POINTER(C.c_int)]
     L.ga_topo_load.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp]
     L.ga_rngdep_eval.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int, _dp, _dp]
+    L.ga_rngdep_info.argtypes = [C.c_void_p, C.c_int, _dp]
     L.ga_rngdep_load_files.argtypes = [C.c_void_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p,
                                        C.c_int, C.POINTER(C.c_int)]
     L.ga_rngdep_release.argtypes = [C.c_void_p, C.c_int]
@@ -238,6 +239,12 @@ class Engine:
 
     def rngdep_release(self, rid):
         self._check(self.L.ga_rngdep_release(self.h, rid))
+
+    def rngdep_info(self, rid):
+        """(lat_min, lat_max, lon_min, lon_max [rad], alt_max, z0, z_max, boundary-layer top [km])."""
+        out = np.zeros(8)
+        self._check(self.L.ga_rngdep_info(self.h, rid, out.ctypes.data_as(_dp)))
+        return out
 
     def rngdep_load_files(self, prefix, lat_path, lon_path, topo_path=None, fmt="zTuvdp", reverse_winds=False) -> int:
         """The reference's range-dependent file layout (node files, prefix{n}.met, terrain rows)."""

@@ -62,3 +62,65 @@ def test_eig_search_known_answer_text(toy_met, tmp_path):
     rows = [l.split("\t") for l in open(tmp_path / "ToyAtmo.arrivals.dat") if l.strip() and not l.startswith("#")]
     assert [x[0] for x in rows] == ["34.559248", "4.1385204", "30.078055"]
     assert [x[2] for x in rows] == ["0", "1", "1"]
+
+
+@pytest.mark.parametrize("args", [
+    ("-prop", "prefix", "lats.dat"),
+    ("-prop", "prefix", "lats.dat", "lons.dat", "no_such_key=1"),
+    ("-prop", "prefix", "lats.dat", "lons.dat", "write_topo=true"),
+    ("-eig_search", "prefix", "lats.dat", "lons.dat"),
+])
+def test_rngdep_rejected_arguments(args):
+    r = run("infraga-sph-rngdep", *args)
+    assert r.returncode == 2, (r.stdout, r.stderr)
+
+
+@pytest.mark.gpu
+def test_rngdep_prop_text(engine, tmp_path):
+    """infraga-sph-rngdep -prop on grid files with terrain: header, the
+    reference's launch loops (each azimuth starts at the terrain slope along
+    it plus 0.1 degree, main.infraga.sph.rngdep.cpp:415-428) and rows equal
+    to the engine's fan over the same launches."""
+    from ga_oracle_py import write_rd_files
+    from infraga_amd import synth
+    from infraga_amd.engine import GA_GEOM_SPH_RNGDEP, prop_params
+    grid = synth.rngdep_grid(3, lat=np.linspace(30, 50, 6), lon=np.linspace(-110, -90, 6), dz=0.5, z_top=120.0)
+    topo = synth.fractal_topo(5, lat=np.linspace(30, 50, 81), lon=np.linspace(-110, -90, 81), amp_km=2.0)
+    lat, lon, z, T3, u3, v3, rho3 = grid
+    pre, lats, lons, tp = write_rd_files(str(tmp_path), lat, lon, z, synth.sound_speed(T3), u3, v3, rho3, topo=topo)
+    r = run("infraga-sph-rngdep", "-prop", pre, lats, lons, "topo_file=" + tp, "prof_format=zcuvd", "incl_min=0.0",
+            "incl_max=14.0", "incl_step=2.0", "az_min=-60.0", "az_max=60.0", "az_step=60.0", "bounces=2",
+            "write_rays=false", cwd=str(tmp_path))
+    assert r.returncode == 0, r.stderr
+    text = open(pre + ".arrivals.dat").read()
+    assert text.startswith("# infraga-sph-rngdep -prop summary:\n")
+    assert "#\ttopo file:" + tp + "\n" in text
+    rows = np.array([[float(x) for x in l.split("\t")] for l in text.splitlines() if l.strip() and not l.startswith("#")])
+    assert len(rows) > 0
+    rid = engine.rngdep_load_files(pre, lats, lons, tp, fmt="zcuvd")
+    try:
+        info = engine.rngdep_info(rid)
+        la = (info[1] + info[0]) / 2.0
+        lo = (info[3] + info[2]) / 2.0
+        _, t = engine.rngdep_eval(rid, [(la, lo, 0.0)])
+        # first launch of each azimuth: atan(terrain slope along it) + 0.1 degree
+        for az in (-60.0, 0.0, 60.0):
+            slope = (t[0, 2] / (6370.0 * np.cos(la)) * np.sin(np.radians(az)) + t[0, 1] / 6370.0 * np.cos(np.radians(az)))
+            th_g = np.degrees(np.arctan(slope)) + 0.1
+            mine = rows[rows[:, 1] == az]
+            if len(mine):
+                assert mine[:, 0].min() >= max(0.0, th_g) - 1e-6   # rows carry 8 significant digits
+        launches = np.unique(rows[:, :2], axis=0)
+        launches = launches[np.lexsort((launches[:, 0], launches[:, 1]))]
+        p = prop_params(GA_GEOM_SPH_RNGDEP, bounces=2)
+        arr, _ = engine.fan(rid, p, launches)
+        got = {(round(a, 9), round(b, 9), int(c)): row for a, b, c, *row in rows}
+        for A in arr:
+            k = (round(launches[A["launch_idx"]][0], 9), round(launches[A["launch_idx"]][1], 9), int(A["n_bounce"]))
+            if k not in got:
+                continue
+            row = got[k]
+            assert abs(row[0] - A["pos0"]) < 1e-6 and abs(row[1] - A["pos1"]) < 1e-5
+            assert abs(row[2] - A["time"]) / A["time"] < 1e-5
+    finally:
+        engine.rngdep_release(rid)
