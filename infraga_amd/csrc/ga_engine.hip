@@ -110,21 +110,28 @@ __global__ void __launch_bounds__(256) rd_eval_kernel(RdGrid g, TopoGrid t, int 
 // (29 MB for an 11 x 11 x 1501 grid) and are read through L1/L2; lanes pull
 // rays from the queue counter (ONE: one ray per lane).
 #ifndef GA_WG_RD
-#define GA_WG_RD 256
+#define GA_WG_RD 128
 #endif
-template <int NEQ, bool ONE>
-__global__ void __launch_bounds__(GA_WG_RD) rd_fan_kernel(RdFanArgs A)
+#ifndef GA_RD_WAVES
+#define GA_RD_WAVES 1
+#endif
+template <int NEQ, bool ONE, bool BLW>
+__global__ void __launch_bounds__(GA_WG_RD) __attribute__((amdgpu_waves_per_eu(GA_RD_WAVES))) rd_fan_kernel(RdFanArgs A)
 {
+    // three RK4 rows per lane (state, accumulator, stage point), lane-interleaved
+    __shared__ double rows[3 * NEQ * GA_WG_RD];
     const int lane = blockIdx.x * blockDim.x + threadIdx.x;
+    const LaneVec<GA_WG_RD> ya{rows + threadIdx.x}, yb{rows + NEQ * GA_WG_RD + threadIdx.x},
+        ys{rows + 2 * NEQ * GA_WG_RD + threadIdx.x};
     if constexpr (ONE) {
         bool first = true;
-        rd_lane_loop<NEQ>(A, lane, [&]() {
+        rd_lane_loop<NEQ, BLW>(A, lane, [&]() {
             const int r = first ? lane : A.n_rays;
             first = false;
             return r;
-        });
+        }, ya, yb, ys);
     } else {
-        rd_lane_loop<NEQ>(A, lane, [&]() { return (int)atomicAdd(A.queue, 1u); });
+        rd_lane_loop<NEQ, BLW>(A, lane, [&]() { return (int)atomicAdd(A.queue, 1u); }, ya, yb, ys);
     }
 }
 
@@ -418,6 +425,45 @@ double axis_inv_spacing(const std::vector<double> &v)
     return 1.0 / d;
 }
 
+// Device views of a loaded hybrid spline and of its terrain.
+RdGrid rd_grid_of(const DevRd &R)
+{
+    RdGrid g;
+    g.x = R.d_axes;
+    g.y = R.d_axes + R.nx;
+    g.z = R.d_axes + R.nx + R.ny;
+    g.nx = R.nx;
+    g.ny = R.ny;
+    g.nz = R.nz;
+    g.tab = R.d_tab;
+    const std::vector<double> *ax[3] = {&R.hx, &R.hy, &R.hz};
+    for (int d = 0; d < 3; ++d) {
+        g.lo[d] = ax[d]->front();
+        g.hi[d] = ax[d]->back();
+        g.iv[d] = axis_inv_spacing(*ax[d]);
+    }
+    return g;
+}
+
+TopoGrid topo_grid_of(const DevRd &R)
+{
+    TopoGrid t;
+    std::memset(&t, 0, sizeof(t));
+    if (!R.d_ttab) return t;
+    t.x = R.d_taxes;
+    t.y = R.d_taxes + R.tnx;
+    t.nx = R.tnx;
+    t.ny = R.tny;
+    t.tab = R.d_ttab;
+    const std::vector<double> *ax[2] = {&R.thx, &R.thy};
+    for (int d = 0; d < 2; ++d) {
+        t.lo[d] = ax[d]->front();
+        t.hi[d] = ax[d]->back();
+        t.iv[d] = axis_inv_spacing(*ax[d]);
+    }
+    return t;
+}
+
 // Region and ground of a range-dependent atmosphere (set_limits,
 // atmo_io.sph.rngdep.cpp:27-43, and set_bndlyr, atmo_state.sph.rngdep.cpp:
 // 42-55); z_grnd (NaN: profile bottom) applies without terrain only
@@ -467,26 +513,9 @@ int rd_fan_enqueue(ga_ctx *ctx, FanBufs &fb, int rd_id, const ga_prop_params *p,
     RdFanArgs A;
     std::memset(&A, 0, sizeof(A));
     RdMed &M = A.M;
-    M.g.x = R.d_axes;
-    M.g.y = R.d_axes + R.nx;
-    M.g.z = R.d_axes + R.nx + R.ny;
-    M.g.nx = R.nx;
-    M.g.ny = R.ny;
-    M.g.nz = R.nz;
-    M.g.tab = R.d_tab;
-    M.g.ix = axis_inv_spacing(R.hx);
-    M.g.iy = axis_inv_spacing(R.hy);
-    M.g.iz = axis_inv_spacing(R.hz);
+    M.g = rd_grid_of(R);
+    M.t = topo_grid_of(R);
     M.is_topo = R.d_ttab ? 1 : 0;
-    if (M.is_topo) {
-        M.t.x = R.d_taxes;
-        M.t.y = R.d_taxes + R.tnx;
-        M.t.nx = R.tnx;
-        M.t.ny = R.tny;
-        M.t.tab = R.d_ttab;
-        M.t.ix = axis_inv_spacing(R.thx);
-        M.t.iy = axis_inv_spacing(R.thy);
-    }
     M.use_blw = p->topo_use_blw ? 1 : 0;
     M.r0 = p->r_earth;
     M.tweak = std::max(0.0, p->abs_coeff);
@@ -564,8 +593,15 @@ int rd_fan_enqueue(ga_ctx *ctx, FanBufs &fb, int rd_id, const ga_prop_params *p,
     if (n == 0) return GA_OK;
 
     const bool amp = p->calc_amp != 0;
-    auto kern = amp ? (A.one_shot ? rd_fan_kernel<18, true> : rd_fan_kernel<18, false>)
-                    : (A.one_shot ? rd_fan_kernel<6, true> : rd_fan_kernel<6, false>);
+    const bool blw = A.M.is_topo && A.M.use_blw;
+    using RdKern = void (*)(RdFanArgs);
+    // [amp][one_shot][blw]
+    const RdKern kerns[2][2][2] = {
+        {{rd_fan_kernel<6, false, false>, rd_fan_kernel<6, false, true>},
+         {rd_fan_kernel<6, true, false>, rd_fan_kernel<6, true, true>}},
+        {{rd_fan_kernel<18, false, false>, rd_fan_kernel<18, false, true>},
+         {rd_fan_kernel<18, true, false>, rd_fan_kernel<18, true, true>}}};
+    const RdKern kern = kerns[amp ? 1 : 0][A.one_shot ? 1 : 0][blw ? 1 : 0];
     int per_cu = 0;
     constexpr int W = GA_WG_RD;
     HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, W, 0));
@@ -1834,10 +1870,8 @@ int ga_rngdep_eval(ga_ctx *ctx, int rd_id, const double *pts, int n, double *atm
     if (n == 0) return GA_OK;
     HIPCHK(ctx, hipSetDevice(ctx->device));
     const DevRd &r = it->second;
-    RdGrid g{r.d_axes, r.d_axes + r.nx, r.d_axes + r.nx + r.ny, r.nx, r.ny, r.nz, r.d_tab,
-             axis_inv_spacing(r.hx), axis_inv_spacing(r.hy), axis_inv_spacing(r.hz)};
-    TopoGrid t{r.d_taxes, r.d_taxes ? r.d_taxes + r.tnx : nullptr, r.tnx, r.tny, r.d_ttab,
-               axis_inv_spacing(r.thx), axis_inv_spacing(r.thy)};
+    const RdGrid g = rd_grid_of(r);
+    const TopoGrid t = topo_grid_of(r);
     DevBuf<double> d_p, d_a, d_t;
     if (!d_p.alloc((size_t)n * 3) || !d_a.alloc((size_t)n * 40) || !d_t.alloc((size_t)n * 10))
         return fail(ctx, GA_E_NOMEM, "evaluation buffers");

@@ -63,6 +63,40 @@ struct RdFanArgs {
     unsigned int *queue;
 };
 
+// A lane's vector stored lane-interleaved: element i at p[i * STRIDE] with p
+// = base + lane.  The kernels keep the RK4 state rows (step start and
+// accumulator) in LDS this way (64 lanes read 64 consecutive doubles, no
+// bank conflicts), out of the registers the medium evaluation needs; a host
+// build uses STRIDE 1 over a plain array.
+template <int STRIDE>
+struct LaneVec {
+    double *p;
+    GA_HD double &operator[](int i) const { return p[i * STRIDE]; }
+};
+
+// Value fence: the values pass through an empty asm statement, so nothing
+// that uses them is scheduled before this point.  Without it the compiler
+// interleaves the 18-equation algebra with the spline evaluation of the
+// medium and spills most of the registers to scratch.
+template <class T>
+GA_HD void value_fence(T &v)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    double *q = reinterpret_cast<double *>(&v);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / sizeof(double)); ++i) asm volatile("" : "+v"(q[i]));
+#else
+    (void)v;
+#endif
+}
+
+template <class V, class SV>
+GA_HD void rd_upd(const Stage &S, const V &y, const SV &stg, const V &acc, int i, double f)
+{
+    acc[i] += S.ha * f;
+    if (!S.last) stg[i] = y[i] + S.hn * f;
+}
+
 // --------------------------------------------------------------------------
 // Terrain: z_g = r0 + h(lat, lon) (topo::z, dz, ddz, dddz with clamped
 // arguments); flat ground r0 + z0 without a terrain file.
@@ -118,29 +152,34 @@ struct RdPt {
 
 GA_HD void rd_point(const RdMed &M, double r, double lat, double lon, RdPt &P)
 {
+    // terrain first (see rd_medium_rhs)
+    double sc = 1.0, X = 0.0, td0 = 0.0, td1 = 0.0, zg = 0.0;
+    const bool bl = r - M.r0 < M.z_bndlyr;
+    if (bl) {
+        Terr T;
+        rd_terrain(M, lat, lon, T);
+        X = exp(-(r - T.zg) / RD_BL);
+        sc = (1.0 - X) / (1.0 + X);
+        td0 = T.d[0];
+        td1 = T.d[1];
+        zg = T.zg;
+    }
     const RdCell cl = rd_cell(M.g, lat, lon, r - M.r0);
     double u0, v0;
     rd_field_eval<0>(M.g, cl, F_C, P.c, nullptr, nullptr);
     rd_field_eval<0>(M.g, cl, F_RHO, P.rho, nullptr, nullptr);
     rd_field_eval<0>(M.g, cl, F_U, u0, nullptr, nullptr);
     rd_field_eval<0>(M.g, cl, F_V, v0, nullptr, nullptr);
-    double sc = 1.0;
     P.w = 0.0;
-    if (r - M.r0 < M.z_bndlyr) {
-        Terr T;
-        rd_terrain(M, lat, lon, T);
-        const double X = exp(-(r - T.zg) / RD_BL);
-        sc = (1.0 - X) / (1.0 + X);
-        if (M.is_topo && M.use_blw) {
-            // atmo::w, :308-338
-            const double sg = RD_BL / RD_VWG;
-            const double q = X / ((1.0 + X) * (1.0 + X));
-            const double dS0 = -2.0 / RD_BL * q * T.d[0];
-            const double dS1 = -2.0 / RD_BL * q * T.d[1];
-            double w0 = v0 * dS0 + u0 * dS1 / cos(lat);
-            w0 *= -2.0 * RD_BL / (M.r0 + T.zg);
-            P.w = sc * X * w0 / (X + (sg / 2.0) * (1.0 - X * X));
-        }
+    if (bl && M.is_topo && M.use_blw) {
+        // atmo::w, :308-338
+        const double sg = RD_BL / RD_VWG;
+        const double q = X / ((1.0 + X) * (1.0 + X));
+        const double dS0 = -2.0 / RD_BL * q * td0;
+        const double dS1 = -2.0 / RD_BL * q * td1;
+        double w0 = v0 * dS0 + u0 * dS1 / cos(lat);
+        w0 *= -2.0 * RD_BL / (M.r0 + zg);
+        P.w = sc * X * w0 / (X + (sg / 2.0) * (1.0 - X * X));
     }
     P.u = u0 * sc;
     P.v = v0 * sc;
@@ -238,33 +277,15 @@ GA_HD int p6(int n, int m)   // packed index of the symmetric pair (n, m)
     return s == 1 ? 3 : (s == 2 ? 4 : 5);
 }
 
-template <bool SECOND>
+// BLW: the boundary-layer vertical wind is on (a separate instantiation:
+// its terms keep the terrain derivatives live across the spline fields).
+template <bool SECOND, bool BLW>
 GA_HD void rd_medium_rhs(const RdMed &M, double r, double lat, double lon, RdAtm &A)
 {
     constexpr int MODE = SECOND ? 2 : 1;
-    double f, df[3], ddf[6];
-    {
-        const RdCell cr = rd_cell(M.g, lat, lon, r - M.r0);
-        rd_field_eval<MODE>(M.g, cr, F_C, f, df, ddf);
-        A.c = f;
-        A.dc[0] = df[2]; A.dc[1] = df[0]; A.dc[2] = df[1];
-        if (SECOND) {
-            A.ddc[0] = ddf[2]; A.ddc[1] = ddf[0]; A.ddc[2] = ddf[1];
-            A.ddc[3] = ddf[4]; A.ddc[4] = ddf[5]; A.ddc[5] = ddf[3];
-        }
-    }
-    const double lat_e = fmin(fmax(lat, M.g.x[0]), M.g.x[M.g.nx - 1]);
-    const double lon_e = fmin(fmax(lon, M.g.y[0]), M.g.y[M.g.ny - 1]);
-    const double z_e = fmin(fmax(r - M.r0, M.g.z[0]), M.g.z[M.g.nz - 1]);
-    const RdCell cw = rd_cell(M.g, lat_e, lon_e, z_e);
-    double u0, du0[3], ddu0[6], v0, dv0[3], ddv0[6];
-    rd_field_eval<MODE>(M.g, cw, F_U, u0, df, ddf);
-    du0[0] = df[2]; du0[1] = df[0]; du0[2] = df[1];
-    if (SECOND) { ddu0[0] = ddf[2]; ddu0[1] = ddf[0]; ddu0[2] = ddf[1]; ddu0[3] = ddf[4]; ddu0[4] = ddf[5]; ddu0[5] = ddf[3]; }
-    rd_field_eval<MODE>(M.g, cw, F_V, v0, df, ddf);
-    dv0[0] = df[2]; dv0[1] = df[0]; dv0[2] = df[1];
-    if (SECOND) { ddv0[0] = ddf[2]; ddv0[1] = ddf[0]; ddv0[2] = ddf[1]; ddv0[3] = ddf[4]; ddv0[4] = ddf[5]; ddv0[5] = ddf[3]; }
-
+    // The terrain and the boundary-layer scaling come before the three
+    // spline fields: with the terrain evaluated after them the compiler
+    // issues all of the fields' loads at once and spills.
     Terr T;
     const bool in_bl = M.is_topo && (r - M.r0 < M.z_bndlyr);
     if (in_bl) {
@@ -278,7 +299,6 @@ GA_HD void rd_medium_rhs(const RdMed &M, double r, double lat, double lon, RdAtm
                 T.ddd[0][i][j] = T.ddd[1][i][j] = 0.0;
             }
     }
-
     const double X = exp(-(r - T.zg) / RD_BL);
     const double ib = 1.0 / RD_BL;
     const double opx = 1.0 + X, iopx = 1.0 / opx, iopx2 = iopx * iopx, iopx3 = iopx2 * iopx;
@@ -297,6 +317,26 @@ GA_HD void rd_medium_rhs(const RdMed &M, double r, double lat, double lon, RdAtm
         dsc0[n] = -2.0 * dX[n] * iopx2;
         for (int m = 0; m < 3; ++m) ddsc0[n][m] = -2.0 * (ddX[n][m] * iopx2 - 2.0 * dX[n] * dX[m] * iopx3);
     }
+    // c from eval_all at the raw point (the spline clamps inside, the f_zz
+    // patch does not); u and v at the clamped point: the same cell with the
+    // f_zz patch at the clamped height.
+    RdCell cl = rd_cell(M.g, lat, lon, r - M.r0);
+    double f, df[3], ddf[6];
+    rd_field_eval<MODE>(M.g, cl, F_C, f, df, ddf);
+    A.c = f;
+    A.dc[0] = df[2]; A.dc[1] = df[0]; A.dc[2] = df[1];
+    if (SECOND) {
+        A.ddc[0] = ddf[2]; A.ddc[1] = ddf[0]; A.ddc[2] = ddf[1];
+        A.ddc[3] = ddf[4]; A.ddc[4] = ddf[5]; A.ddc[5] = ddf[3];
+    }
+    cl.Xz_u = cl.Xz;
+    double u0, du0[3], ddu0[6], v0, dv0[3], ddv0[6];
+    rd_field_eval<MODE>(M.g, cl, F_U, u0, df, ddf);
+    du0[0] = df[2]; du0[1] = df[0]; du0[2] = df[1];
+    if (SECOND) { ddu0[0] = ddf[2]; ddu0[1] = ddf[0]; ddu0[2] = ddf[1]; ddu0[3] = ddf[4]; ddu0[4] = ddf[5]; ddu0[5] = ddf[3]; }
+    rd_field_eval<MODE>(M.g, cl, F_V, v0, df, ddf);
+    dv0[0] = df[2]; dv0[1] = df[0]; dv0[2] = df[1];
+    if (SECOND) { ddv0[0] = ddf[2]; ddv0[1] = ddf[0]; ddv0[2] = ddf[1]; ddv0[3] = ddf[4]; ddv0[4] = ddf[5]; ddv0[5] = ddf[3]; }
     A.u = u0 * sc0;
     A.v = v0 * sc0;
     for (int n = 0; n < 3; ++n) {
@@ -319,7 +359,19 @@ GA_HD void rd_medium_rhs(const RdMed &M, double r, double lat, double lon, RdAtm
     for (int n = 0; n < 3; ++n) A.dw[n] = 0.0;
     if (SECOND)
         for (int n = 0; n < 6; ++n) A.ddw[n] = 0.0;
-    if (!(in_bl && M.use_blw)) return;
+    if (!BLW || !in_bl) return;
+    struct {
+        double u0, du0[3], ddu0[6], v0, dv0[3], ddv0[6];
+    } W;
+    W.u0 = u0; W.v0 = v0;
+    for (int n = 0; n < 3; ++n) { W.du0[n] = du0[n]; W.dv0[n] = dv0[n]; }
+    if (SECOND)
+        for (int n = 0; n < 6; ++n) { W.ddu0[n] = ddu0[n]; W.ddv0[n] = ddv0[n]; }
+    value_fence(W);
+    u0 = W.u0; v0 = W.v0;
+    for (int n = 0; n < 3; ++n) { du0[n] = W.du0[n]; dv0[n] = W.dv0[n]; }
+    if (SECOND)
+        for (int n = 0; n < 6; ++n) { ddu0[n] = W.ddu0[n]; ddv0[n] = W.ddv0[n]; }
 
     // boundary-layer vertical wind (calc_uvw :658-690, :808-876)
     const double sg = RD_BL / RD_VWG;
@@ -430,13 +482,14 @@ GA_HD void rd_medium_rhs(const RdMed &M, double r, double lat, double lon, RdAtm
 // Right-hand side (update_refs + eval_src_eq, :238-436) and the fused RK4
 // stage: f[i] is consumed by upd() as it is produced.
 // --------------------------------------------------------------------------
-template <int NEQ>
-GA_HD void rd_stage(const RdMed &M, const Stage &S, const double *y, double *stg, double *acc)
+template <int NEQ, bool BLW, class V, class SV>
+GA_HD void rd_stage(const RdMed &M, const Stage &S, const V &y, const SV &stg, const V &acc)
 {
     const double r = stg[0], t = stg[1];
     const double nu0 = stg[3], nu1 = stg[4], nu2 = stg[5];
     RdAtm A;
-    rd_medium_rhs<NEQ == 18>(M, r, t, stg[2], A);
+    rd_medium_rhs<NEQ == 18, BLW>(M, r, t, stg[2], A);
+    value_fence(A);
 
     const double nu_mag = sqrt(nu0 * nu0 + nu1 * nu1 + nu2 * nu2);
     const double inu = 1.0 / nu_mag;
@@ -509,27 +562,27 @@ GA_HD void rd_stage(const RdMed &M, const Stage &S, const double *y, double *stg
                 const double fm = -dG[i] * icg * (Sg[i] + T[i]) + G[i] * cgm2 * Sg[i] -
                                   G[i] * icg * (dnu * A.dc[i] + nu_mag * ddc[i] + mj[0] * A.dw[i] + mj[1] * A.dv[i] +
                                                 mj[2] * A.du[i] + nu0 * ddw[i] + nu1 * ddv[i] + nu2 * ddu[i] + dT[i]);
-                upd(S, y, stg, acc, oR + i, fR);
-                upd(S, y, stg, acc, om + i, fm);
+                rd_upd(S, y, stg, acc, oR + i, fR);
+                rd_upd(S, y, stg, acc, om + i, fm);
             }
         }
     }
 #pragma unroll
-    for (int i = 0; i < 6; ++i) upd(S, y, stg, acc, i, f[i]);
+    for (int i = 0; i < 6; ++i) rd_upd(S, y, stg, acc, i, f[i]);
 }
 
-GA_HD double rd_set_ds(const RdMed &M, const Limits &L, const double *y)
+GA_HD double rd_set_ds(const RdMed &M, const Limits &L, double r, double lat, double lon)
 {
     double ds = L.ds_max;
-    if (y[0] - M.r0 < M.z_bndlyr) ds -= (L.ds_max - L.ds_min) * exp(-(y[0] - rd_ground(M, y[1], y[2])) / 0.2);
+    if (r - M.r0 < M.z_bndlyr) ds -= (L.ds_max - L.ds_min) * exp(-(r - rd_ground(M, lat, lon)) / 0.2);
     return ds;
 }
 
-template <int NEQ>
-GA_HD double rd_rk4_step(const RdMed &M, const Limits &L, const double *y, double *y1)
+// One RK4 step from y into y1, stg holding the stage point (rows of V).
+template <int NEQ, bool BLW, class V>
+GA_HD double rd_rk4_step(const RdMed &M, const Limits &L, const V &y, const V &y1, const V &stg)
 {
-    double stg[NEQ];
-    const double ds = rd_set_ds(M, L, y);
+    const double ds = rd_set_ds(M, L, y[0], y[1], y[2]);
     const double h6 = ds * (1.0 / 6.0), h3 = ds * (1.0 / 3.0), h2 = ds * 0.5;
 #pragma unroll
     for (int i = 0; i < NEQ; ++i) { stg[i] = y[i]; y1[i] = y[i]; }
@@ -537,7 +590,7 @@ GA_HD double rd_rk4_step(const RdMed &M, const Limits &L, const double *y, doubl
     for (int st = 0; st < 4; ++st) {
         const double ha = (st == 0 || st == 3) ? h6 : h3;
         const double hn = (st == 2) ? ds : h2;
-        rd_stage<NEQ>(M, Stage{ha, hn, false}, y, stg, y1);
+        rd_stage<NEQ, BLW>(M, Stage{ha, hn, false}, y, stg, y1);
     }
     return ds;
 }
@@ -578,6 +631,7 @@ GA_HD void rd_seg_book(const RdMed &M, const double *y0, const double *y1, doubl
     const double inm = 1.0 / sqrt(n0 * n0 + n1 * n1 + n2 * n2);
     RdPt P;
     rd_point(M, r, t, p, P);
+    value_fence(P);
     const double c0 = P.c * n0 * inm + P.w, c1 = P.c * n1 * inm + P.v, c2 = P.c * n2 * inm + P.u;
     dt = ds / sqrt(c0 * c0 + c1 * c1 + c2 * c2);
     datt = sb_alpha<true>(r - M.r0, P.c, P.rho, freq, M.tweak, M.sbc) * ds;
@@ -650,6 +704,7 @@ GA_HD void rd_reflect(const RdMed &M, const double *yk2, const double *yk1, doub
 
     RdPtD P;
     rd_point_d(M, rg, prev[1], prev[2], P);
+    value_fence(P);
     const double c_grnd = P.c;
     const double norm2 = prev[3] - aa * prev[4] - b * prev[5];
     const double tn4 = tan(prev[4]);   // the reference's tan(prev[4]) (:161, :164)
@@ -754,19 +809,20 @@ GA_HD ga_arrival rd_arrival_row(const RdFanArgs &A, const double *y, int ray, in
 }
 
 // Per-lane trace loop (the range-dependent run_prop with write_rays=false).
-template <int NEQ, class NextRay>
-GA_HD void rd_lane_loop(const RdFanArgs &A, int lane, NextRay next_ray)
+// BLW: instantiation for M.use_blw on a terrain (rd_medium_rhs).  ya, yb:
+// the lane's two state rows (current row and the RK4 accumulator, which
+// swap roles after an accepted step); ys: the RK4 stage point.
+template <int NEQ, bool BLW, class V, class NextRay>
+GA_HD void rd_lane_loop(const RdFanArgs &A, int lane, NextRay next_ray, V ya, V yb, V ys)
 {
     const Limits L = A.L;
     const RdMed &M = A.M;
-    double y[NEQ];
+    V y = ya, y1 = yb;
     double *ypl = A.ypbuf + (size_t)lane * 18;
     double s = 0.0, tt = 0.0, att = 0.0, ttl = 0.0, attl = 0.0, rmax = 0.0;
     double c0 = 0.0, nu0s = 0.0, th = 0.0, ph = 0.0;
     int k = 0, bnc = 0, nst = 0, ray = 0, narr = 0;
     bool fetch = true;
-#pragma unroll
-    for (int i = 0; i < NEQ; ++i) y[i] = 0.0;
 
     for (;;) {
         if (fetch) {
@@ -774,35 +830,45 @@ GA_HD void rd_lane_loop(const RdFanArgs &A, int lane, NextRay next_ray)
             if (ray >= A.n_rays) break;
             ph = PI / 2.0 - A.launches[2 * ray + 1] * (PI / 180.0);
             th = A.launches[2 * ray] * (PI / 180.0);
-            rd_init_ray<NEQ>(A, th, ph, y, c0, nu0s);
+            double y0[NEQ];
+            rd_init_ray<NEQ>(A, th, ph, y0, c0, nu0s);
+#pragma unroll
+            for (int i = 0; i < NEQ; ++i) y[i] = y0[i];
             s = 0.0; tt = 0.0; att = 0.0; ttl = 0.0; attl = 0.0; rmax = 0.0;
             k = 0; bnc = 0; nst = 0; narr = 0;
             fetch = false;
         }
-        double y1[NEQ];
-        s += rd_rk4_step<NEQ>(M, L, y, y1);
+        s += rd_rk4_step<NEQ, BLW>(M, L, y, y1, ys);
         ++k;
         ++nst;
+        double n[NEQ];   // the new row
+#pragma unroll
+        for (int i = 0; i < NEQ; ++i) n[i] = y1[i];
         bool brk = false, end = false;
         if (s > L.s_max) { brk = true; end = true; }
-        else if (rd_leaves_region(A, L, y1)) { brk = true; end = true; }
-        else if (rd_below_ground(M, y1)) { end = true; }
+        else if (rd_leaves_region(A, L, n)) { brk = true; end = true; }
+        else if (rd_below_ground(M, n)) { end = true; }
         else if (k >= A.max_steps) { brk = true; end = true; }
-        else if (!(y1[0] == y1[0])) { brk = true; end = true; }   // non-finite state: stop the ray
+        else if (!(n[0] == n[0])) { brk = true; end = true; }   // non-finite state: stop the ray
+        y1[2] = n[2];   // longitude wrapped by rd_leaves_region
 
+        double o[6];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) o[i] = y[i];
         double dt, da;
-        rd_seg_book(M, y, y1, A.freq, dt, da);
+        rd_seg_book(M, o, n, A.freq, dt, da);
         ttl += dt;
         attl += da;
-        rmax = fmax(rmax, y[0] - M.r0);
+        rmax = fmax(rmax, o[0] - M.r0);
 
         if (!end) {
-            if (y[0] - M.r0 < L.z_keep) {
+            if (o[0] - M.r0 < L.z_keep) {
 #pragma unroll
                 for (int i = 0; i < NEQ; ++i) ypl[i] = y[i];
             }
-#pragma unroll
-            for (int i = 0; i < NEQ; ++i) y[i] = y1[i];
+            const V t = y;
+            y = y1;
+            y1 = t;
             continue;
         }
         tt += ttl;
@@ -816,16 +882,19 @@ GA_HD void rd_lane_loop(const RdFanArgs &A, int lane, NextRay next_ray)
             {
                 double *r = A.raw + ((size_t)ray * (A.bounces + 1) + bnc) * RAW;
 #pragma unroll
-                for (int i = 0; i < NEQ; ++i) r[i] = y1[i];
+                for (int i = 0; i < NEQ; ++i) r[i] = n[i];
                 r[18] = tt; r[19] = att; r[20] = rmax; r[21] = th; r[22] = ph; r[23] = c0; r[24] = nu0s;
             }
             ++narr;
             if (bnc == A.bounces) done = true;
             else {
-                double yr[NEQ], yp[NEQ];
+                double yr[NEQ], yp[NEQ], yk[NEQ];
 #pragma unroll
-                for (int i = 0; i < NEQ; ++i) yp[i] = ypl[i];
-                rd_reflect<NEQ>(M, yp, y, yr, c0);
+                for (int i = 0; i < NEQ; ++i) {
+                    yp[i] = ypl[i];
+                    yk[i] = y[i];
+                }
+                rd_reflect<NEQ>(M, yp, yk, yr, c0);
 #pragma unroll
                 for (int i = 0; i < NEQ; ++i) y[i] = yr[i];
                 ++bnc;

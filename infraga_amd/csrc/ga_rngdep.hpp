@@ -9,9 +9,10 @@
 // eval_node_dfdy runs its z-spline with the x-difference slopes
 // (interpolation.cpp:1119-1120); the table keeps that pairing.  A query
 // evaluates the five node quantities (f, f_z, f_zz, x-difference,
-// y-difference) on the 4 x 4 node columns around its cell, forms the five
-// bicubic patches of eval_all (corner values, centred differences scaled by
-// the cell size, 16 x 16 conversion matrix) and sums their polynomials.
+// y-difference) on the 4 x 4 node columns around its cell and sums them with
+// the weights of the five bicubic patches of eval_all (corner values and
+// centred differences scaled by the cell size); the patches are the
+// reference's functions, evaluated in another order of operations.
 //
 // Topography.  interp::eval_f on a natural_cubic_spline_2D interpolates the
 // y-splines of all x rows at y and solves a natural spline along x through
@@ -27,23 +28,35 @@
 
 namespace ga {
 
+// Scheduling fence: keeps the loads of one row of node columns from being
+// hoisted above the sums of the previous row (all 16 columns of several
+// fields in flight at once would need more registers than a lane has).
+#if defined(__HIP_DEVICE_COMPILE__)
+#ifndef GA_SCHED_FENCE
+#define GA_SCHED_FENCE() __builtin_amdgcn_sched_barrier(0)
+#endif
+#else
+#define GA_SCHED_FENCE() ((void)0)
+#endif
+
 constexpr int RD_Q = 5;    // doubles per hybrid-spline node
 constexpr int TP_Q = 4;    // doubles per topography node
 
-// ix, iy, iz: inverse node spacing of an evenly spaced axis (the segment is
-// then found in O(1)), 0 for a binary search.
+// lo, hi: first and last node of each axis (x, y, z); iv: inverse node
+// spacing of an evenly spaced axis (its segments are then found in O(1)),
+// 0 for a binary search.
 struct RdGrid {
     const double *x, *y, *z;   // node coordinates (lat, lon [rad]; z [km])
     int nx, ny, nz;
     const double *tab;         // [field][nx][ny][nz][RD_Q]
-    double ix, iy, iz;
+    double lo[3], hi[3], iv[3];
 };
 
 struct TopoGrid {
     const double *x, *y;       // lat, lon [rad]
     int nx, ny;
     const double *tab;         // [nx][ny][TP_Q]
-    double ix, iy;
+    double lo[2], hi[2], iv[2];
 };
 
 GA_HD int clampi(int i, int lo, int hi) { return i < lo ? lo : (i > hi ? hi : i); }
@@ -60,16 +73,43 @@ GA_HD int seg_of(const double *v, int n, double q)
     return lo;
 }
 
-// The same segment as seg_of: an estimate from the mean spacing, then
-// exact steps to the largest k with v[k] <= q (q inside [v[0], v[n-1]]).
-GA_HD int seg_at(const double *v, int n, double q, double iv)
+// Segment of q along an axis and the 4 node coordinates around it,
+// v[a] = axis[clamp(k - 1 + a)]: the same segment as seg_of.  On an evenly
+// spaced axis (iv = 1 / spacing > 0, v0 = first node) the segment is
+// estimated from the spacing; axis_guess loads the four nodes of the
+// estimate (the loads of all axes of a query go out together), axis_ok
+// checks it and axis_fix steps an estimate that rounding put one node off.
+struct AxisSeg {
+    int k;
+    double v[4];
+};
+
+GA_HD AxisSeg axis_guess(const double *ax, int n, double q, double iv, double v0)
 {
-    if (!(iv > 0.0)) return seg_of(v, n, q);
-    int k = (int)((q - v[0]) * iv);
-    k = k < 0 ? 0 : (k > n - 2 ? n - 2 : k);
-    while (k < n - 2 && v[k + 1] <= q) ++k;
-    while (k > 0 && v[k] > q) --k;
-    return k;
+    int k;
+    if (iv > 0.0) {
+        k = (int)((q - v0) * iv);
+        k = k < 0 ? 0 : (k > n - 2 ? n - 2 : k);
+    } else {
+        k = seg_of(ax, n, q);
+    }
+    AxisSeg s;
+    s.k = k;
+    for (int a = 0; a < 4; ++a) s.v[a] = ax[clampi(k - 1 + a, 0, n - 1)];
+    return s;
+}
+
+GA_HD bool axis_ok(const AxisSeg &s, int n, double q)
+{
+    return !(s.k > 0 && s.v[1] > q) && !(s.k < n - 2 && s.v[2] <= q);
+}
+
+GA_HD void axis_fix(AxisSeg &s, const double *ax, int n, double q)
+{
+    while (!axis_ok(s, n, q)) {
+        s.k += (s.k > 0 && s.v[1] > q) ? -1 : 1;
+        for (int a = 0; a < 4; ++a) s.v[a] = ax[clampi(s.k - 1 + a, 0, n - 1)];
+    }
 }
 
 GA_HD double hv(double f0, double f1, double s0, double s1, double d, double X)
@@ -98,156 +138,195 @@ GA_HD double hder(int m, double f0, double f1, double s0, double s1, double d, d
                   : m == 2 ? hd2(f0, f1, s0, s1, d, X) : hd3(f0, f1, s0, s1, d, X);
 }
 
-// Bicubic conversion (interpolation.cpp:686-703) and polynomial sums.
-// which: 0 value, 1 d/dx, 2 d/dy, 3 d2/dxdy.
-GA_HD void bicubic_coeffs(const double X[16], double A[16])
+// Cubic Hermite basis on [0, 1]: h0 weights the end values, h1 the end
+// slopes (per unit s), g0 and g1 are their s-derivatives.
+GA_HD void herm_basis(double s, double h0[2], double h1[2], double g0[2], double g1[2])
 {
-    A[0] = X[0];
-    A[1] = X[4];
-    A[2] = -3.0 * X[0] + 3.0 * X[1] - 2.0 * X[4] - X[5];
-    A[3] = 2.0 * X[0] - 2.0 * X[1] + X[4] + X[5];
-    A[4] = X[8];
-    A[5] = X[12];
-    A[6] = -3.0 * X[8] + 3.0 * X[9] - 2.0 * X[12] - X[13];
-    A[7] = 2.0 * X[8] - 2.0 * X[9] + X[12] + X[13];
-    A[8] = -3.0 * X[0] + 3.0 * X[2] - 2.0 * X[8] - X[10];
-    A[9] = -3.0 * X[4] + 3.0 * X[6] - 2.0 * X[12] - X[14];
-    A[10] = 9.0 * X[0] - 9.0 * X[1] - 9.0 * X[2] + 9.0 * X[3] + 6.0 * X[4] + 3.0 * X[5] - 6.0 * X[6] - 3.0 * X[7] +
-            6.0 * X[8] - 6.0 * X[9] + 3.0 * X[10] - 3.0 * X[11] + 4.0 * X[12] + 2.0 * X[13] + 2.0 * X[14] + X[15];
-    A[11] = -6.0 * X[0] + 6.0 * X[1] + 6.0 * X[2] - 6.0 * X[3] - 3.0 * X[4] - 3.0 * X[5] + 3.0 * X[6] + 3.0 * X[7] -
-            4.0 * X[8] + 4.0 * X[9] - 2.0 * X[10] + 2.0 * X[11] - 2.0 * X[12] - 2.0 * X[13] - X[14] - X[15];
-    A[12] = 2.0 * X[0] - 2.0 * X[2] + X[8] + X[10];
-    A[13] = 2.0 * X[4] - 2.0 * X[6] + X[12] + X[14];
-    A[14] = -6.0 * X[0] + 6.0 * X[1] + 6.0 * X[2] - 6.0 * X[3] - 4.0 * X[4] - 2.0 * X[5] + 4.0 * X[6] + 2.0 * X[7] -
-            3.0 * X[8] + 3.0 * X[9] - 3.0 * X[10] + 3.0 * X[11] - 2.0 * X[12] - X[13] - 2.0 * X[14] - X[15];
-    A[15] = 4.0 * X[0] - 4.0 * X[1] - 4.0 * X[2] + 4.0 * X[3] + 2.0 * X[4] + 2.0 * X[5] - 2.0 * X[6] - 2.0 * X[7] +
-            2.0 * X[8] - 2.0 * X[9] + 2.0 * X[10] - 2.0 * X[11] + X[12] + X[13] + X[14] + X[15];
+    const double s2 = s * s, s3 = s2 * s;
+    h0[0] = 1.0 - 3.0 * s2 + 2.0 * s3;
+    h0[1] = 3.0 * s2 - 2.0 * s3;
+    h1[0] = s - 2.0 * s2 + s3;
+    h1[1] = s3 - s2;
+    g0[0] = 6.0 * (s2 - s);
+    g0[1] = 6.0 * (s - s2);
+    g1[0] = 1.0 - 4.0 * s + 3.0 * s2;
+    g1[1] = 3.0 * s2 - 2.0 * s;
 }
 
-GA_HD double bicubic_poly(const double A[16], double xs, double ys, double dx, double dy, int which)
+// One horizontal axis of a hybrid-spline patch as weights over the 4 node
+// columns a = 0..3 around the cell.  The patch (interpolation.cpp:686-703,
+// the bicubic conversion of corner values, slopes times the cell size and
+// cross slopes) takes its corner values at a = 1, 2 and its corner slopes as
+// the centred differences (Q[a+1] - Q[a-1]) / (x[a+1] - x[a-1])
+// (interpolation.cpp:1544-1616), so every patch is a sum of tensor products
+// of these weight vectors with the node quantities:
+//   V   value weights (corners only)       Vd  their x-derivative
+//   S   slope weights (through the centred differences, times the cell size)
+//   Sd  their x-derivative
+//   T   the centred differences of Vd (cross-slope terms)
+// P = V + S is the weight of a patch whose slopes all come from its own
+// quantity; d/dx of it is Pd = Vd + Sd.
+struct AxisW {
+    double V[4], S[4], Vd[4], Sd[4], T[4];
+};
+
+GA_HD AxisW axis_weights(double s, double d, double e0, double e1)
 {
-    double px[4] = {1.0, xs, xs * xs, xs * xs * xs};
-    double py[4] = {1.0, ys, ys * ys, ys * ys * ys};
-    double r = 0.0;
-    if (which == 0) {
-        for (int k1 = 0; k1 < 4; ++k1)
-            for (int k2 = 0; k2 < 4; ++k2) r += A[k2 * 4 + k1] * px[k1] * py[k2];
-    } else if (which == 1) {
-        for (int k1 = 1; k1 < 4; ++k1)
-            for (int k2 = 0; k2 < 4; ++k2) r += A[k2 * 4 + k1] * px[k1 - 1] * py[k2] * (double)k1;
-        r /= dx;
-    } else if (which == 2) {
-        for (int k1 = 0; k1 < 4; ++k1)
-            for (int k2 = 1; k2 < 4; ++k2) r += A[k2 * 4 + k1] * px[k1] * py[k2 - 1] * (double)k2;
-        r /= dy;
-    } else {
-        for (int k1 = 1; k1 < 4; ++k1)
-            for (int k2 = 1; k2 < 4; ++k2) r += A[k2 * 4 + k1] * px[k1 - 1] * py[k2 - 1] * (double)(k1 * k2);
-        r /= dx * dy;
-    }
-    return r;
+    // e0 = 1 / (x[2] - x[0]), e1 = 1 / (x[3] - x[1]) over the clamped columns
+    double h0[2], h1[2], g0[2], g1[2];
+    herm_basis(s, h0, h1, g0, g1);
+    AxisW w;
+    const double id = 1.0 / d;
+    w.V[0] = 0.0; w.V[1] = h0[0]; w.V[2] = h0[1]; w.V[3] = 0.0;
+    w.Vd[0] = 0.0; w.Vd[1] = g0[0] * id; w.Vd[2] = g0[1] * id; w.Vd[3] = 0.0;
+    const double a0 = h1[0] * d * e0, a1 = h1[1] * d * e1;
+    w.S[0] = -a0; w.S[1] = -a1; w.S[2] = a0; w.S[3] = a1;
+    const double b0 = g1[0] * e0, b1 = g1[1] * e1;
+    w.Sd[0] = -b0; w.Sd[1] = -b1; w.Sd[2] = b0; w.Sd[3] = b1;
+    const double t0 = w.Vd[1] * e0, t1 = w.Vd[2] * e1;
+    w.T[0] = -t0; w.T[1] = -t1; w.T[2] = t0; w.T[3] = t1;
+    return w;
 }
 
 // Cell of a query point, shared by the fields of one grid: segments, the
-// clamped and scaled coordinates and the 4 x 4 node columns around the cell
+// clamped and scaled coordinates, the 4 x 4 node columns around the cell
 // (indices clamped at the grid edge, as the reference's finite differences
-// clamp their neighbours).
+// clamp their neighbours) and the inverse spans of their centred differences.
 struct RdCell {
     int kz;
     int ci[4], cj[4];
     double dx, dy, dz, xs, ys, Xz, Xz_u;
+    double ex[2], ey[2];
 };
 
 GA_HD RdCell rd_cell(const RdGrid &g, double x, double y, double z)
 {
     RdCell c;
-    const double xe = fmin(fmax(x, g.x[0]), g.x[g.nx - 1]);
-    const double ye = fmin(fmax(y, g.y[0]), g.y[g.ny - 1]);
-    const double ze = fmin(fmax(z, g.z[0]), g.z[g.nz - 1]);
-    const int kx = seg_at(g.x, g.nx, xe, g.ix), ky = seg_at(g.y, g.ny, ye, g.iy);
-    c.kz = seg_at(g.z, g.nz, ze, g.iz);
-    c.dx = g.x[kx + 1] - g.x[kx];
-    c.xs = (xe - g.x[kx]) / c.dx;
-    c.dy = g.y[ky + 1] - g.y[ky];
-    c.ys = (ye - g.y[ky]) / c.dy;
-    c.dz = g.z[c.kz + 1] - g.z[c.kz];
-    c.Xz = (ze - g.z[c.kz]) / c.dz;
-    c.Xz_u = (z - g.z[c.kz]) / c.dz;   // f_zz patch: unclamped z (interpolation.cpp:1763-1771)
-    for (int a = 0; a < 4; ++a) {
-        c.ci[a] = clampi(kx - 1 + a, 0, g.nx - 1);
-        c.cj[a] = clampi(ky - 1 + a, 0, g.ny - 1);
+    const double xe = fmin(fmax(x, g.lo[0]), g.hi[0]);
+    const double ye = fmin(fmax(y, g.lo[1]), g.hi[1]);
+    const double ze = fmin(fmax(z, g.lo[2]), g.hi[2]);
+    AxisSeg sx = axis_guess(g.x, g.nx, xe, g.iv[0], g.lo[0]);
+    AxisSeg sy = axis_guess(g.y, g.ny, ye, g.iv[1], g.lo[1]);
+    AxisSeg sz = axis_guess(g.z, g.nz, ze, g.iv[2], g.lo[2]);
+    if (!(axis_ok(sx, g.nx, xe) && axis_ok(sy, g.ny, ye) && axis_ok(sz, g.nz, ze))) {
+        axis_fix(sx, g.x, g.nx, xe);
+        axis_fix(sy, g.y, g.ny, ye);
+        axis_fix(sz, g.z, g.nz, ze);
     }
+    c.kz = sz.k;
+    c.dz = sz.v[2] - sz.v[1];
+    c.Xz = (ze - sz.v[1]) / c.dz;
+    c.Xz_u = (z - sz.v[1]) / c.dz;   // f_zz patch: unclamped z (interpolation.cpp:1764-1787)
+    const double *xv = sx.v, *yv = sy.v;
+    for (int a = 0; a < 4; ++a) {
+        c.ci[a] = clampi(sx.k - 1 + a, 0, g.nx - 1);
+        c.cj[a] = clampi(sy.k - 1 + a, 0, g.ny - 1);
+    }
+    c.dx = xv[2] - xv[1];
+    c.xs = (xe - xv[1]) / c.dx;
+    c.dy = yv[2] - yv[1];
+    c.ys = (ye - yv[1]) / c.dy;
+    c.ex[0] = 1.0 / (xv[2] - xv[0]);
+    c.ex[1] = 1.0 / (xv[3] - xv[1]);
+    c.ey[0] = 1.0 / (yv[2] - yv[0]);
+    c.ey[1] = 1.0 / (yv[3] - yv[1]);
     return c;
 }
 
 // One field of the hybrid spline at a cell (interp::eval_f / eval_all,
 // interpolation.cpp:1334-1369, 1544-1788).  MODE 0: f; 1: also
 // df = {fx, fy, fz}; 2: also ddf = {fxx, fyy, fzz, fxy, fxz, fyz}.
+//
+// Node quantities per column, from the z-splines: Q0 f, Q1 f_z, Q2 f_zz (at
+// the unclamped z), Q3 the x-difference, Q4 the y-difference.  The
+// reference's patches, as weight sums (x weights wx, y weights wy):
+//   f, fx, fy, fxy    patch of Q0:  Px Q0 Py, Pdx Q0 Py, Px Q0 Pdy, Pdx Q0 Pdy
+//   fz, fxz, fyz      patch of Q1:  Px Q1 Py, Pdx Q1 Py, Px Q1 Pdy
+//   fzz               patch of Q2:  Px Q2 Py
+//   fxx  d/dx of the Q3 patch with y slopes from the cross differences of
+//        Q0 (interpolation.cpp:1714-1737): Pdx Q3 Vy + Sdx Q3 Sy + Tx Q0 Sy
+//   fyy  d/dy of the Q4 patch with x slopes from the cross differences of
+//        Q0 (:1739-1762):                   Vx Q4 Pdy + Sx Q4 Sdy + Sx Q0 Ty
+// The columns are streamed (the sums run over the 16 columns as they are
+// loaded); Q0 and Q1 are taken relative to the cell's first corner so that
+// the derivative sums keep the accuracy of differences.
 template <int MODE>
 GA_HD void rd_field_eval(const RdGrid &g, const RdCell &cl, int field, double &f, double *df, double *ddf)
 {
-    constexpr int NQ = MODE == 0 ? 1 : (MODE == 1 ? 2 : 5);
-    // node quantities on the 4 x 4 columns: 0 f, 1 f_z, 2 f_zz, 3 x-diff, 4 y-diff
-    double Q[NQ][4][4];
+    const AxisW wx = axis_weights(cl.xs, cl.dx, cl.ex[0], cl.ex[1]);
+    const AxisW wy = axis_weights(cl.ys, cl.dy, cl.ey[0], cl.ey[1]);
     const double *base = g.tab + (size_t)field * g.nx * g.ny * g.nz * RD_Q;
-    for (int a = 0; a < 4; ++a)
+    auto col = [&](int a, int b) {
+        return base + (((size_t)cl.ci[a] * g.ny + cl.cj[b]) * g.nz + cl.kz) * RD_Q;
+    };
+    double ref0, ref1 = 0.0;
+    {
+        const double *p = col(1, 1), *q = p + RD_Q;
+        ref0 = hv(p[0], q[0], p[1], q[1], cl.dz, cl.Xz);
+        if constexpr (MODE >= 1) ref1 = hd1(p[0], q[0], p[1], q[1], cl.dz, cl.Xz);
+    }
+    double F = 0.0, Fx = 0.0, Fy = 0.0, Fz = 0.0;
+    double Fxx = 0.0, Fyy = 0.0, Fzz = 0.0, Fxy = 0.0, Fxz = 0.0, Fyz = 0.0;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        double r0P = 0.0, r0Pd = 0.0, r0S = 0.0, r0T = 0.0, r1P = 0.0, r1Pd = 0.0, r2P = 0.0;
+        double r3V = 0.0, r3S = 0.0, r4Pd = 0.0, r4Sd = 0.0;
+#pragma unroll
         for (int b = 0; b < 4; ++b) {
-            const double *p = base + (((size_t)cl.ci[a] * g.ny + cl.cj[b]) * g.nz + cl.kz) * RD_Q;
-            const double *q = p + RD_Q;
-            Q[0][a][b] = hv(p[0], q[0], p[1], q[1], cl.dz, cl.Xz);
-            if constexpr (NQ > 1) Q[1][a][b] = hd1(p[0], q[0], p[1], q[1], cl.dz, cl.Xz);
-            if constexpr (NQ > 2) {
-                Q[2][a][b] = hd2(p[0], q[0], p[1], q[1], cl.dz, cl.Xz_u);
-                Q[3][a][b] = hv(p[2], q[2], p[3], q[3], cl.dz, cl.Xz);
-                Q[4][a][b] = hv(p[4], q[4], p[3], q[3], cl.dz, cl.Xz);
+            const double *p = col(a, b), *q = p + RD_Q;
+            const double Py = wy.V[b] + wy.S[b];
+            const double q0 = hv(p[0], q[0], p[1], q[1], cl.dz, cl.Xz) - ref0;
+            r0P += Py * q0;
+            if constexpr (MODE >= 1) {
+                const double Pdy = wy.Vd[b] + wy.Sd[b];
+                const double q1 = hd1(p[0], q[0], p[1], q[1], cl.dz, cl.Xz) - ref1;
+                r0Pd += Pdy * q0;
+                r1P += Py * q1;
+                if constexpr (MODE == 2) {
+                    r1Pd += Pdy * q1;
+                    r0S += wy.S[b] * q0;
+                    r0T += wy.T[b] * q0;
+                    r2P += Py * hd2(p[0], q[0], p[1], q[1], cl.dz, cl.Xz_u);
+                    const double q3 = hv(p[2], q[2], p[3], q[3], cl.dz, cl.Xz);
+                    r3V += wy.V[b] * q3;
+                    r3S += wy.S[b] * q3;
+                    const double q4 = hv(p[4], q[4], p[3], q[3], cl.dz, cl.Xz);
+                    r4Pd += Pdy * q4;
+                    r4Sd += wy.Sd[b] * q4;
+                }
             }
         }
-    // corner (a, b) in {1, 2}^2; differences over the clamped neighbours
-    auto ddx = [&](int q, int a, int b) {
-        return (Q[q][a + 1][b] - Q[q][a - 1][b]) / (g.x[cl.ci[a + 1]] - g.x[cl.ci[a - 1]]);
-    };
-    auto ddy = [&](int q, int a, int b) {
-        return (Q[q][a][b + 1] - Q[q][a][b - 1]) / (g.y[cl.cj[b + 1]] - g.y[cl.cj[b - 1]]);
-    };
-    auto ddxy = [&](int q, int a, int b) {
-        return (Q[q][a + 1][b + 1] - Q[q][a + 1][b - 1] - Q[q][a - 1][b + 1] + Q[q][a - 1][b - 1]) /
-               ((g.x[cl.ci[a + 1]] - g.x[cl.ci[a - 1]]) * (g.y[cl.cj[b + 1]] - g.y[cl.cj[b - 1]]));
-    };
-    // corner order of the reference's X_vec: (kx,ky), (kx+1,ky), (kx,ky+1), (kx+1,ky+1)
-    const int A_[4] = {1, 2, 1, 2}, B_[4] = {1, 1, 2, 2};
-    const double dx = cl.dx, dy = cl.dy, xs = cl.xs, ys = cl.ys;
-    double X[16], C[16];
-    // patch of quantity qv with x slopes from (qx, kind) etc.; kinds: 0 ddx, 1 ddy, 2 ddxy
-    auto build = [&](int qv, int qxs, int kxs, int qys, int kys, int qxys) {
-        for (int c = 0; c < 4; ++c) {
-            const int a = A_[c], b = B_[c];
-            X[c] = Q[qv][a][b];
-            X[4 + c] = (kxs == 0 ? ddx(qxs, a, b) : ddxy(qxs, a, b)) * dx;
-            X[8 + c] = (kys == 1 ? ddy(qys, a, b) : ddxy(qys, a, b)) * dy;
-            X[12 + c] = ddxy(qxys, a, b) * dx * dy;
+        GA_SCHED_FENCE();
+        const double Px = wx.V[a] + wx.S[a];
+        F += Px * r0P;
+        if constexpr (MODE >= 1) {
+            const double Pdx = wx.Vd[a] + wx.Sd[a];
+            Fx += Pdx * r0P;
+            Fy += Px * r0Pd;
+            Fz += Px * r1P;
+            if constexpr (MODE == 2) {
+                Fxy += Pdx * r0Pd;
+                Fxz += Pdx * r1P;
+                Fyz += Px * r1Pd;
+                Fzz += Px * r2P;
+                Fxx += Pdx * r3V + wx.Sd[a] * r3S + wx.T[a] * r0S;
+                Fyy += wx.V[a] * r4Pd + wx.S[a] * r4Sd + wx.S[a] * r0T;
+            }
         }
-        bicubic_coeffs(X, C);
-    };
-    build(0, 0, 0, 0, 1, 0);                         // f
-    f = bicubic_poly(C, xs, ys, dx, dy, 0);
-    if constexpr (MODE >= 1) {
-        df[0] = bicubic_poly(C, xs, ys, dx, dy, 1);
-        df[1] = bicubic_poly(C, xs, ys, dx, dy, 2);
     }
-    if constexpr (MODE == 2) ddf[3] = bicubic_poly(C, xs, ys, dx, dy, 3);
+    f = ref0 + F;
     if constexpr (MODE >= 1) {
-        build(1, 1, 0, 1, 1, 1);                     // f_z
-        df[2] = bicubic_poly(C, xs, ys, dx, dy, 0);
+        df[0] = Fx;
+        df[1] = Fy;
+        df[2] = ref1 + Fz;
     }
     if constexpr (MODE == 2) {
-        ddf[4] = bicubic_poly(C, xs, ys, dx, dy, 1);
-        ddf[5] = bicubic_poly(C, xs, ys, dx, dy, 2);
-        build(3, 3, 0, 0, 2, 3);                     // x-difference: slopes ddx(Qx), ddxy(f), ddxy(Qx)
-        ddf[0] = bicubic_poly(C, xs, ys, dx, dy, 1);
-        build(4, 0, 2, 4, 1, 4);                     // y-difference: slopes ddxy(f), ddy(Qy), ddxy(Qy)
-        ddf[1] = bicubic_poly(C, xs, ys, dx, dy, 2);
-        build(2, 2, 0, 2, 1, 2);                     // f_zz
-        ddf[2] = bicubic_poly(C, xs, ys, dx, dy, 0);
+        ddf[0] = Fxx;
+        ddf[1] = Fyy;
+        ddf[2] = Fzz;
+        ddf[3] = Fxy;
+        ddf[4] = Fxz;
+        ddf[5] = Fyz;
     }
 }
 
@@ -264,11 +343,17 @@ GA_HD void rd_field_all(const RdGrid &g, int field, double x, double y, double z
 // zxxx, zxxy, zxyy, zyyy} (topo::z, dz, ddz, dddz with clamped arguments).
 GA_HD void topo_all(const TopoGrid &g, double x, double y, double out[10])
 {
-    const double xe = fmin(fmax(x, g.x[0]), g.x[g.nx - 1]);
-    const double ye = fmin(fmax(y, g.y[0]), g.y[g.ny - 1]);
-    const int kx = seg_at(g.x, g.nx, xe, g.ix), ky = seg_at(g.y, g.ny, ye, g.iy);
-    const double dx = g.x[kx + 1] - g.x[kx], X = (xe - g.x[kx]) / dx;
-    const double dy = g.y[ky + 1] - g.y[ky], Y = (ye - g.y[ky]) / dy;
+    const double xe = fmin(fmax(x, g.lo[0]), g.hi[0]);
+    const double ye = fmin(fmax(y, g.lo[1]), g.hi[1]);
+    AxisSeg sx = axis_guess(g.x, g.nx, xe, g.iv[0], g.lo[0]);
+    AxisSeg sy = axis_guess(g.y, g.ny, ye, g.iv[1], g.lo[1]);
+    if (!(axis_ok(sx, g.nx, xe) && axis_ok(sy, g.ny, ye))) {
+        axis_fix(sx, g.x, g.nx, xe);
+        axis_fix(sy, g.y, g.ny, ye);
+    }
+    const int kx = sx.k, ky = sy.k;
+    const double dx = sx.v[2] - sx.v[1], X = (xe - sx.v[1]) / dx;
+    const double dy = sy.v[2] - sy.v[1], Y = (ye - sy.v[1]) / dy;
     const double *n00 = g.tab + ((size_t)kx * g.ny + ky) * TP_Q, *n01 = n00 + TP_Q;
     const double *n10 = n00 + (size_t)g.ny * TP_Q, *n11 = n10 + TP_Q;
     // y-derivative m of the row values and of the row x slopes at rows kx, kx+1

@@ -29,14 +29,22 @@ def _rel(got, ref):
     return (np.abs(got - ref).max(axis=0) / scale).max()
 
 
-def test_hybrid_and_topo_match_oracle(engine):
+@pytest.mark.parametrize("even", [True, False])
+def test_hybrid_and_topo_match_oracle(engine, even):
+    """Evenly spaced axes (segments from the spacing) and uneven ones
+    (binary search)."""
     O = TI._L(os.path.join(TI.ORACLE, "libga_oracle.so"))
     TI._bind(O, "go_")
     O["go_h3d_all"].argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, _dp, _dp, _dp]
     rng = np.random.Generator(np.random.PCG64(21))
-    lat = np.linspace(25.0, 35.0, 6)
-    lon = np.linspace(-5.0, 5.0, 5)
-    z = np.linspace(0.0, 30.0, 121)
+    if even:
+        lat = np.linspace(25.0, 35.0, 6)
+        lon = np.linspace(-5.0, 5.0, 5)
+        z = np.linspace(0.0, 30.0, 121)
+    else:
+        lat = np.array([25.0, 26.5, 29.0, 31.0, 33.2, 35.0])
+        lon = np.array([-5.0, -3.0, 0.5, 1.0, 5.0])
+        z = np.concatenate([np.linspace(0.0, 10.0, 81), np.linspace(10.5, 30.0, 40)])
     fields = []
     for k in range(4):
         base = [0.34 - 0.001 * z, 0.02 * np.sin(z / 7.0), 0.01 * np.cos(z / 5.0), 1.2e-3 * np.exp(-z / 7.0)][k]
@@ -45,6 +53,9 @@ def test_hybrid_and_topo_match_oracle(engine):
     rid = engine.rngdep_load(lat, lon, z, *fields)
     tlat = np.linspace(24.0, 36.0, 41)
     tlon = np.linspace(-6.0, 6.0, 37)
+    if not even:
+        tlat = np.sort(np.concatenate([tlat[::2], tlat[1::2] + 0.07]))
+        tlon = np.sort(np.concatenate([tlon[::2], tlon[1::2] - 0.05]))
     zg = np.ascontiguousarray(np.abs(rng.normal(size=(41, 37)).cumsum(axis=0).cumsum(axis=1)) * 0.01)
     engine.topo_load(rid, tlat, tlon, zg)
     try:
