@@ -261,6 +261,8 @@ def main():
     ap.add_argument("--no-rngdep", action="store_true")
     ap.add_argument("--one-pass", action="store_true",
                     help="one untimed fan pass (for hardware-counter runs); no JSON line")
+    ap.add_argument("--leg", choices=("fan", "rngdep"), default="fan",
+                    help="with --one-pass: the spherical fan or the range-dependent fan")
     ap.add_argument("--calibrate", action="store_true",
                     help="with --one-pass: copy a 256 MiB buffer first (HBM counter calibration)")
     args = ap.parse_args()
@@ -288,6 +290,23 @@ def main():
 
     def one_pass():
         eng.fan_device(aid, p, d_l.data_ptr(), n, d_out.data_ptr(), d_cnt.data_ptr(), d_st.data_ptr(), stream)
+
+    if args.one_pass and args.leg == "rngdep":
+        from infraga_amd.engine import GA_GEOM_SPH_RNGDEP
+        units_r, topo_r, L_r = rngdep_workload()
+        rid = eng.rngdep_load(*units_r)
+        eng.topo_load(rid, *topo_r)
+        p_r = prop_params(GA_GEOM_SPH_RNGDEP, calc_amp=1, bounces=4)
+        if args.calibrate:
+            a = torch.ones(256 * 1024 * 1024 // 8, dtype=torch.float64, device="cuda:0")
+            b = torch.empty_like(a)
+            torch.cuda.synchronize()
+            b.copy_(a)
+            torch.cuda.synchronize()
+        _, st = eng.fan(rid, p_r, L_r)
+        print(f"one range-dependent pass: {st.ray_steps} ray-steps, {st.kernel_ms:.0f} ms", flush=True)
+        eng.close()
+        return
 
     if args.one_pass:
         if args.calibrate:

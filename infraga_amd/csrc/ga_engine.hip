@@ -119,10 +119,15 @@ template <int NEQ, bool ONE, bool BLW>
 __global__ void __launch_bounds__(GA_WG_RD) __attribute__((amdgpu_waves_per_eu(GA_RD_WAVES))) rd_fan_kernel(RdFanArgs A)
 {
     // three RK4 rows per lane (state, accumulator, stage point), lane-interleaved
-    __shared__ double rows[3 * NEQ * GA_WG_RD];
     const int lane = blockIdx.x * blockDim.x + threadIdx.x;
+#ifdef GA_RD_STG_REG
+    __shared__ double rows[2 * NEQ * GA_WG_RD + 1];
+    const LaneVec<GA_WG_RD> ya{rows + threadIdx.x}, yb{rows + NEQ * GA_WG_RD + threadIdx.x}, ys{rows + 2 * NEQ * GA_WG_RD};
+#else
+    __shared__ double rows[3 * NEQ * GA_WG_RD];
     const LaneVec<GA_WG_RD> ya{rows + threadIdx.x}, yb{rows + NEQ * GA_WG_RD + threadIdx.x},
         ys{rows + 2 * NEQ * GA_WG_RD + threadIdx.x};
+#endif
     if constexpr (ONE) {
         bool first = true;
         rd_lane_loop<NEQ, BLW>(A, lane, [&]() {

@@ -578,9 +578,9 @@ GA_HD double rd_set_ds(const RdMed &M, const Limits &L, double r, double lat, do
     return ds;
 }
 
-// One RK4 step from y into y1, stg holding the stage point (rows of V).
-template <int NEQ, bool BLW, class V>
-GA_HD double rd_rk4_step(const RdMed &M, const Limits &L, const V &y, const V &y1, const V &stg)
+// One RK4 step from y into y1 (rows of V), stg holding the stage point.
+template <int NEQ, bool BLW, class V, class SV>
+GA_HD double rd_rk4_step(const RdMed &M, const Limits &L, const V &y, const V &y1, SV &stg)
 {
     const double ds = rd_set_ds(M, L, y[0], y[1], y[2]);
     const double h6 = ds * (1.0 / 6.0), h3 = ds * (1.0 / 3.0), h2 = ds * 0.5;
@@ -838,7 +838,12 @@ GA_HD void rd_lane_loop(const RdFanArgs &A, int lane, NextRay next_ray, V ya, V 
             k = 0; bnc = 0; nst = 0; narr = 0;
             fetch = false;
         }
+#ifdef GA_RD_STG_REG
+        double sreg[NEQ], *sp = sreg;
+        s += rd_rk4_step<NEQ, BLW>(M, L, y, y1, sp);
+#else
         s += rd_rk4_step<NEQ, BLW>(M, L, y, y1, ys);
+#endif
         ++k;
         ++nst;
         double n[NEQ];   // the new row

@@ -1240,6 +1240,71 @@ static int est_eigenray(eig_ctx *E, const double src[3], const double rcvr[2], d
     return 0;
 }
 
+/* The accepted eigenray's row (find_eigenray, geoac.eigenray.3d.cpp:223-276):
+ * a re-trace of launch (s->theta, s->phi) with per-step bookkeeping of the
+ * segments 1..k-1 of each leg over bnc_cnt reflections, the row taken at
+ * row k of the last leg.  Returns the RK4 steps taken. */
+static long long eig_accept_3d(go_atmo *a, const go_params *p, go_src *s, const double src[3], int bnc_cnt,
+                               double freq, double *rows, int length, go_arrival *A)
+{
+    double att = 0.0, tt = 0.0, zmax = 0.0;
+    int brk, k;
+    long long steps = 0;
+    go_set_initial(a, p, s, rows);
+    k = go_prop_rk4(a, p, s, rows, length, &brk);
+    steps += k;
+    for (int m = 1; m < k; m++) {
+        tt += go_travel_time_seg(a, rows + 18 * (m - 1), rows + 18 * m);
+        att += go_atten_seg(a, rows + 18 * (m - 1), rows + 18 * m, freq);
+        zmax = zmax > rows[18 * m + 2] ? zmax : rows[18 * m + 2];
+    }
+    for (int b = 1; b <= bnc_cnt; b++) {
+        if (k < 2) break;   /* row k-2 < 0: no further reflection */
+        go_set_refl(a, p, s, rows + 18 * (k - 2), rows + 18 * (k - 1), rows);
+        k = go_prop_rk4(a, p, s, rows, length, &brk);
+        steps += k;
+        for (int m = 1; m < k; m++) {
+            tt += go_travel_time_seg(a, rows + 18 * (m - 1), rows + 18 * m);
+            att += go_atten_seg(a, rows + 18 * (m - 1), rows + 18 * m, freq);
+            zmax = zmax > rows[18 * m + 2] ? zmax : rows[18 * m + 2];
+        }
+    }
+    const double *yk = rows + 18 * k;
+    memset(A, 0, sizeof(*A));
+    A->n_bounce = bnc_cnt;
+    A->incl_launch = s->theta * (180.0 / GO_PI);
+    A->az_launch = 90.0 - s->phi * (180.0 / GO_PI);
+    A->pos0 = yk[0];
+    A->pos1 = yk[1];
+    A->time = tt;
+    A->cel = sqrt(sq(yk[0] - src[0]) + sq(yk[1] - src[1])) / tt;
+    A->turn_ht = zmax;
+    A->incl_arr = -asin(go_c(a, yk[0], yk[1], yk[2]) / go_c(a, src[0], src[1], src[2]) * yk[5]) * (180.0 / GO_PI);
+    double baz = 90.0 - atan2(-yk[4], -yk[3]) * (180.0 / GO_PI);
+    while (baz > 180.0) baz -= 360.0;
+    while (baz < -180.0) baz += 360.0;
+    A->back_az = baz;
+    A->tl = 20.0 * log10(go_amp(a, s, yk));
+    A->absorption = -2.0 * att;
+    return steps;
+}
+
+/* The accepted-eigenray row for given launch angles (degrees; azimuth
+ * clockwise from north as in the output rows): a pure function of the
+ * launch, so that a search's rows can be checked one by one. */
+int go_eig_row_3d(go_atmo *a, const go_params *p, const double src[3], double incl_deg, double az_deg,
+                  int bnc_cnt, double freq, go_arrival *A)
+{
+    int length = (int)(p->s_max / p->ds_min);
+    double *rows = get_rows(length);
+    go_src s;
+    s.src[0] = src[0]; s.src[1] = src[1]; s.src[2] = src[2];
+    s.theta = incl_deg * (GO_PI / 180.0);
+    s.phi = (90.0 - az_deg) * (GO_PI / 180.0);
+    eig_accept_3d(a, p, &s, src, bnc_cnt, freq, rows, length, A);
+    return 0;
+}
+
 static int find_eigenray(eig_ctx *E, const double src[3], const double rcvr[2], double th_est, double ph_est,
                          double freq, int bnc_cnt, int iterate_limit, go_arrival *A)
 {
@@ -1279,45 +1344,8 @@ static int find_eigenray(eig_ctx *E, const double src[3], const double rcvr[2], 
             y = yk[1]; dy = rcvr[1] - y;
             dr = (double)sqrtl(powl(dx, 2) + powl(dy, 2));
             if (dr < E->tolerance) {
-                /* accept: re-trace with per-step bookkeeping (3d.cpp:223-276) */
-                double att = 0.0, tt = 0.0, zmax = 0.0;
-                go_set_initial(a, p, &s, rows);
-                k = go_prop_rk4(a, p, &s, rows, length, &brk);
-                E->steps += k;
-                for (int m = 1; m < k; m++) {
-                    tt += go_travel_time_seg(a, rows + 18 * (m - 1), rows + 18 * m);
-                    att += go_atten_seg(a, rows + 18 * (m - 1), rows + 18 * m, freq);
-                    zmax = zmax > rows[18 * m + 2] ? zmax : rows[18 * m + 2];
-                }
+                E->steps += eig_accept_3d(a, p, &s, src, bnc_cnt, freq, rows, length, A);
                 eig_cnt[GO_EIG_FINAL_TRACES]++;
-                for (int b = 1; b <= bnc_cnt; b++) {
-                    if (k < 2) break;   /* row k-2 < 0: no further reflection */
-                    go_set_refl(a, p, &s, rows + 18 * (k - 2), rows + 18 * (k - 1), rows);
-                    k = go_prop_rk4(a, p, &s, rows, length, &brk);
-                    E->steps += k;
-                    for (int m = 1; m < k; m++) {
-                        tt += go_travel_time_seg(a, rows + 18 * (m - 1), rows + 18 * m);
-                        att += go_atten_seg(a, rows + 18 * (m - 1), rows + 18 * m, freq);
-                        zmax = zmax > rows[18 * m + 2] ? zmax : rows[18 * m + 2];
-                    }
-                }
-                yk = rows + 18 * k;
-                memset(A, 0, sizeof(*A));
-                A->n_bounce = bnc_cnt;
-                A->incl_launch = s.theta * (180.0 / GO_PI);
-                A->az_launch = 90.0 - s.phi * (180.0 / GO_PI);
-                A->pos0 = yk[0];
-                A->pos1 = yk[1];
-                A->time = tt;
-                A->cel = sqrt(sq(yk[0] - src[0]) + sq(yk[1] - src[1])) / tt;
-                A->turn_ht = zmax;
-                A->incl_arr = -asin(go_c(a, yk[0], yk[1], yk[2]) / go_c(a, src[0], src[1], src[2]) * yk[5]) * (180.0 / GO_PI);
-                double baz = 90.0 - atan2(-yk[4], -yk[3]) * (180.0 / GO_PI);
-                while (baz > 180.0) baz -= 360.0;
-                while (baz < -180.0) baz += 360.0;
-                A->back_az = baz;
-                A->tl = 20.0 * log10(go_amp(a, &s, yk));
-                A->absorption = -2.0 * att;
                 return 1;
             } else if (n > 0 && dr > dr_prev) {
                 s.theta -= dth * step_sc; s.phi -= dph * step_sc; step_sc /= 2.0;

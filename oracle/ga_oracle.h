@@ -152,6 +152,10 @@ int    go_fan(go_atmo *a, const go_params *p, const go_prop *pr,
 
 /* Eigenray search for one receiver, 3D geometry (main.infraga.3d.cpp:944-951,
  * geoac.eigenray.3d.cpp).  Returns the number of eigenrays written to out. */
+/* The row find_eigenray writes for an accepted 3D eigenray with these launch
+ * angles (degrees) and bounces. */
+int    go_eig_row_3d(go_atmo *a, const go_params *p, const double src[3], double incl_deg, double az_deg,
+                     int bnc_cnt, double freq, go_arrival *A);
 int    go_eig_search_3d(go_atmo *a, const go_params *p, const double src[3],
                         const double rcvr[2], double th_min_deg, double th_max_deg,
                         int bnc_min, int bnc_max, int iterations, double damping,

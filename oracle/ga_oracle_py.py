@@ -107,6 +107,9 @@ class Oracle:
         L.go_eig_search_3d.restype = C.c_int
         L.go_eig_search_sph.argtypes = L.go_eig_search_3d.argtypes
         L.go_eig_search_sph.restype = C.c_int
+        L.go_eig_row_3d.argtypes = [C.c_void_p, C.POINTER(Params), _dp, C.c_double, C.c_double, C.c_int,
+                                    C.c_double, C.POINTER(Arrival)]
+        L.go_eig_row_3d.restype = C.c_int
         L.go_eig_counts.argtypes = [C.POINTER(C.c_longlong)]
 
     EIG_COUNT_NAMES = ("est_calls", "sweep_traces", "sweep_traces_corrected", "sweep_traces_adaptive",
@@ -174,6 +177,18 @@ class Medium:
                bnc_min, bnc_max, iterations, damping, tolerance, az_dev_lim,
                incl_step_max, incl_step_min, freq, out, cap, C.byref(steps))
         return arrivals_to_array(out, min(n, cap)), steps.value
+
+
+    def eig_row(self, params, src, incl_deg, az_deg, bounces, freq=0.1):
+        """3D: the row find_eigenray writes for an accepted eigenray with these
+        launch angles (go_eig_row_3d)."""
+        if self.geom == GEOM_SPH:
+            raise NotImplementedError("eig_row: 3D only")
+        s = np.ascontiguousarray(src, dtype=np.float64)
+        A = Arrival()
+        self.o.lib.go_eig_row_3d(self.h, C.byref(params), _ptr(s), float(incl_deg), float(az_deg), int(bounces),
+                                 freq, C.byref(A))
+        return arrivals_to_array((Arrival * 1)(A), 1)[0]
 
 
 class Ref:

@@ -54,3 +54,22 @@ def test_eigenray_search_known_answers(toy):
         d = Oracle().lib.go_gc_dist(6370.0, math.radians(r["pos0"]), math.radians(r["pos1"]),
                                     math.radians(30.25), math.radians(-104.25))
         assert d < 0.1
+
+
+def test_eigenray_row_function_reproduces_search_rows(oracle, base_units):
+    """go_eig_row_3d (the accepted-eigenray trace as a function of the launch
+    angles) gives back the rows the oracle's 3D search wrote into the
+    committed 512-pair fixture, from their printed launch angles: every
+    third row, to 1e-9 relative (TL, absorption to 1e-6 dB)."""
+    import os
+    import numpy as np
+    from ga_oracle_py import GEOM_3D
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "eig3d_base.npz"))["eigenrays"]
+    m = Medium(oracle, GEOM_3D, *base_units)
+    p = m.params(True)
+    for r in g[::3]:
+        row = m.eig_row(p, (0.0, 0.0, 0.0), r["incl_launch"], r["az_launch"], int(r["n_bounce"]))
+        for f in ("pos0", "pos1", "time", "turn_ht", "incl_arr", "back_az"):
+            assert abs(row[f] - r[f]) <= 1e-9 * max(abs(r[f]), 1.0), (int(r["launch_idx"]), f, row[f], r[f])
+        for f in ("tl", "absorption"):
+            assert abs(row[f] - r[f]) <= 1e-6, (int(r["launch_idx"]), f, row[f], r[f])
