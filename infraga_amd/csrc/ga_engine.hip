@@ -110,24 +110,16 @@ __global__ void __launch_bounds__(256) rd_eval_kernel(RdGrid g, TopoGrid t, int 
 // (29 MB for an 11 x 11 x 1501 grid) and are read through L1/L2; lanes pull
 // rays from the queue counter (ONE: one ray per lane).
 #ifndef GA_WG_RD
-#define GA_WG_RD 128
-#endif
-#ifndef GA_RD_WAVES
-#define GA_RD_WAVES 1
+#define GA_WG_RD 64
 #endif
 template <int NEQ, bool ONE, bool BLW>
-__global__ void __launch_bounds__(GA_WG_RD) __attribute__((amdgpu_waves_per_eu(GA_RD_WAVES))) rd_fan_kernel(RdFanArgs A)
+__global__ void __launch_bounds__(GA_WG_RD) rd_fan_kernel(RdFanArgs A)
 {
     // three RK4 rows per lane (state, accumulator, stage point), lane-interleaved
     const int lane = blockIdx.x * blockDim.x + threadIdx.x;
-#ifdef GA_RD_STG_REG
-    __shared__ double rows[2 * NEQ * GA_WG_RD + 1];
-    const LaneVec<GA_WG_RD> ya{rows + threadIdx.x}, yb{rows + NEQ * GA_WG_RD + threadIdx.x}, ys{rows + 2 * NEQ * GA_WG_RD};
-#else
     __shared__ double rows[3 * NEQ * GA_WG_RD];
     const LaneVec<GA_WG_RD> ya{rows + threadIdx.x}, yb{rows + NEQ * GA_WG_RD + threadIdx.x},
         ys{rows + 2 * NEQ * GA_WG_RD + threadIdx.x};
-#endif
     if constexpr (ONE) {
         bool first = true;
         rd_lane_loop<NEQ, BLW>(A, lane, [&]() {

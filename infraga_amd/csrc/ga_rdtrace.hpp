@@ -838,12 +838,7 @@ GA_HD void rd_lane_loop(const RdFanArgs &A, int lane, NextRay next_ray, V ya, V 
             k = 0; bnc = 0; nst = 0; narr = 0;
             fetch = false;
         }
-#ifdef GA_RD_STG_REG
-        double sreg[NEQ], *sp = sreg;
-        s += rd_rk4_step<NEQ, BLW>(M, L, y, y1, sp);
-#else
         s += rd_rk4_step<NEQ, BLW>(M, L, y, y1, ys);
-#endif
         ++k;
         ++nst;
         double n[NEQ];   // the new row

@@ -28,17 +28,6 @@
 
 namespace ga {
 
-// Scheduling fence: keeps the loads of one row of node columns from being
-// hoisted above the sums of the previous row (all 16 columns of several
-// fields in flight at once would need more registers than a lane has).
-#if defined(__HIP_DEVICE_COMPILE__)
-#ifndef GA_SCHED_FENCE
-#define GA_SCHED_FENCE() __builtin_amdgcn_sched_barrier(0)
-#endif
-#else
-#define GA_SCHED_FENCE() ((void)0)
-#endif
-
 constexpr int RD_Q = 5;    // doubles per hybrid-spline node
 constexpr int TP_Q = 4;    // doubles per topography node
 
@@ -296,7 +285,6 @@ GA_HD void rd_field_eval(const RdGrid &g, const RdCell &cl, int field, double &f
                 }
             }
         }
-        GA_SCHED_FENCE();
         const double Px = wx.V[a] + wx.S[a];
         F += Px * r0P;
         if constexpr (MODE >= 1) {

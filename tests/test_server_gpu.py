@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 from infraga_amd import synth
-from infraga_amd.engine import GA_GEOM_3D, GA_GEOM_SPH, Server, eig_params, prop_params
+from infraga_amd.engine import GA_GEOM_3D, GA_GEOM_SPH, GA_GEOM_SPH_RNGDEP, Server, eig_params, prop_params
 from parity import compare, compare_eig
 
 pytestmark = pytest.mark.gpu
@@ -19,9 +19,13 @@ pytestmark = pytest.mark.gpu
 @pytest.fixture(scope="module")
 def atmos(engine, base_units, pert_units):
     ids = {"3d": engine.load_atmo(*base_units), "sph": engine.load_atmo(*pert_units)}
-    yield ids
+    grid = synth.rngdep_grid(3, lat=np.linspace(30, 50, 6), lon=np.linspace(-110, -90, 6), dz=0.5, z_top=120.0)
+    rid = engine.rngdep_load(*synth.rngdep_engine_units(*grid))
+    engine.topo_load(rid, *synth.fractal_topo(5, lat=np.linspace(30, 50, 81), lon=np.linspace(-110, -90, 81)))
+    yield dict(ids, rd=rid)
     for a in ids.values():
         engine.release_atmo(a)
+    engine.rngdep_release(rid)
 
 
 def _requests(atmos):
@@ -31,11 +35,13 @@ def _requests(atmos):
     p3 = prop_params(GA_GEOM_3D, bounces=2, src=(0.0, 0.0, 0.0), rng_max=800.0)
     ps = prop_params(GA_GEOM_SPH, bounces=2, src=(30.0, -100.0, 0.0), rng_max=800.0)
     pe = prop_params(GA_GEOM_3D, src=(0.0, 0.0, 0.0))
+    pr = prop_params(GA_GEOM_SPH_RNGDEP, bounces=2, src=(40.0, -100.0, 0.0))
     e = eig_params(bnc_min=0, bnc_max=1)
     return [
         ("fan", atmos["3d"], p3, None, fan3),
         ("eig", atmos["3d"], pe, e, R[[1, 5]]),
         ("fan", atmos["sph"], ps, None, fans),
+        ("fan", atmos["rd"], pr, None, fans[::3]),    # range-dependent atmosphere over terrain
         ("eig", atmos["3d"], pe, e, R[[0, 7]]),      # joins the running search of the same source
         ("fan", atmos["3d"], p3, None, fan3[::2]),
     ]
@@ -67,7 +73,7 @@ def test_mixed_stream_equals_alone(engine, atmos, merge, serial, slots):
         assert got is not None, "request did not finish"
         rows, res = got
         assert res.status == 0 and res.total_ms >= res.queue_ms >= 0.0
-        sph = p.geometry == GA_GEOM_SPH
+        sph = p.geometry in (GA_GEOM_SPH, GA_GEOM_SPH_RNGDEP)
         if kind == "fan":
             fails, summ = compare(rows, ref, sph, src=(p.src[0], p.src[1]))
         else:
