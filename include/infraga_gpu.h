@@ -24,6 +24,14 @@
  *                          geoac::est_eigenray + geoac::find_eigenray
  *                          src/main.infraga.3d.cpp:944-951,
  *                          src/geoac/geoac.eigenray.h:17-20
+ *   ga_fan_run_paths    <- the write_rays / write_caustics branch of the
+ *                          -prop loop (<id>.raypaths.dat, <id>.caustics-N.dat)
+ *                          src/main.infraga.sph.cpp:388-413, 456-496, 549;
+ *                          src/main.infraga.3d.cpp:378-400, 442-487
+ *   ga_eig_search_paths <- the same search, plus find_eigenray's
+ *                          <id>.eigenray-N.dat rows
+ *                          src/geoac/geoac.eigenray.3d.cpp:223-276,
+ *                          src/geoac/geoac.eigenray.sph.cpp:228-271
  */
 #ifndef INFRAGA_GPU_H_
 #define INFRAGA_GPU_H_
@@ -91,6 +99,27 @@ typedef struct {
     double  absorption;    /* dB */
 } ga_arrival;
 
+/* One row of <id>.raypaths.dat / <id>.eigenray-N.dat (kind GA_PT_RAY:
+ * rows m == 1 or m % 15 == 0 of every leg, main.infraga.sph.cpp:471-482) or
+ * of <id>.caustics-<leg>.dat (kind GA_PT_CAUSTIC: the Jacobian determinant
+ * changed sign between rows m-1 and m, main.infraga.sph.cpp:485-490). */
+typedef struct {
+    int32_t launch_idx;    /* ray (index into the launch list) or eigenray (index into the output) */
+    int32_t leg;           /* reflections before the row */
+    int32_t step;          /* row index m within the leg */
+    int32_t kind;          /* GA_PT_RAY or GA_PT_CAUSTIC */
+    double  pos0, pos1;    /* sph: lat, lon [deg]; 3D: x, y [km] */
+    double  z;             /* sph: altitude r - r0; 3D: max(z, ground) [km] */
+    double  tl;            /* 20 log10(amplitude) [dB]; 0 without amplitudes and for caustics */
+    double  absorption;    /* -2 x the absorption integral so far [dB] */
+    double  time;          /* travel time so far [s] */
+} ga_path_pt;
+
+#define GA_PATH_RAYS      1   /* write_rays */
+#define GA_PATH_CAUSTICS  2   /* write_caustics (needs calc_amp) */
+#define GA_PT_RAY         0
+#define GA_PT_CAUSTIC     1
+
 typedef struct {
     int64_t ray_steps;     /* RK4 iterations summed over rays and legs */
     double  kernel_ms;     /* device time of the trace kernels (HIP events) */
@@ -152,6 +181,22 @@ int ga_fan_run_device(ga_ctx *ctx, int atmo_id, const ga_prop_params *p,
                       const double *d_launches, int n_launch,
                       ga_arrival *d_out, int32_t *d_counts, int32_t *d_steps, void *stream);
 
+/* ga_fan_run with the reference's write_rays / write_caustics bookkeeping
+ * (path_flags = GA_PATH_RAYS | GA_PATH_CAUSTICS): travel time and
+ * absorption are summed segment by segment over rows 1..k-1 of each leg, so
+ * the arrivals are those the reference writes with write_rays=true (they
+ * differ from ga_fan_run's write_rays=false sums by the segment into the
+ * sub-ground row of each leg).  Path rows come back in pts ordered by ray,
+ * then leg and step.  pts_cap bounds both the records returned and the
+ * device space the call takes (64 B per record plus 2 KB per ray); when the
+ * rays produce more records the call returns GA_E_CAP with *n_pts set to the
+ * count needed (arrivals are still returned when out_cap allows).  RNGDEP
+ * atmospheres are accepted. */
+int ga_fan_run_paths(ga_ctx *ctx, int atmo_id, const ga_prop_params *p,
+                     const double *launches, int n_launch, int path_flags,
+                     ga_arrival *out, int out_cap, int *n_out,
+                     ga_path_pt *pts, int64_t pts_cap, int64_t *n_pts, ga_run_stats *stats);
+
 /* Batched eigenray search for n_rcvr receivers, all from p->src: 3D
  * receivers are (x, y) [km] pairs, spherical ones (lat, lon) [deg] pairs.
  * Eigenrays are returned grouped by receiver (launch_idx), then bounce
@@ -161,6 +206,16 @@ int ga_eig_search(ga_ctx *ctx, int atmo_id, const ga_prop_params *p, const ga_ei
                   const double *rcvr_xy, int n_rcvr,
                   ga_eigenray *out, int out_cap, int *n_out, double *rcvr_latency_ms,
                   ga_run_stats *stats);
+
+/* ga_eig_search plus the path rows of every eigenray found (the rows
+ * find_eigenray writes to <id>.eigenray-N.dat, N counting the eigenrays in
+ * output order): pts[].launch_idx is the eigenray's index in out.  When
+ * pts_cap is too small the call returns GA_E_CAP with *n_pts set to the
+ * count needed; the eigenrays are returned either way. */
+int ga_eig_search_paths(ga_ctx *ctx, int atmo_id, const ga_prop_params *p, const ga_eig_params *e,
+                        const double *rcvr_xy, int n_rcvr,
+                        ga_eigenray *out, int out_cap, int *n_out, double *rcvr_latency_ms,
+                        ga_path_pt *pts, int64_t pts_cap, int64_t *n_pts, ga_run_stats *stats);
 
 /* ---- Range-dependent atmosphere and topography --------------------------
  *

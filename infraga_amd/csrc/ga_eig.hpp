@@ -43,7 +43,11 @@ struct EigArgs {
     int32_t *steps;          // RK4 steps of the job
 };
 
-template <int G, int NEQ, class NextJob>
+// PATHS (EIG_FINAL jobs, 18 equations): rows m == 1 or m % 15 == 0 of every
+// leg go to E.F.path with the job index as item, as find_eigenray writes
+// <id>.eigenray-N.dat (geoac.eigenray.3d.cpp:240-266,
+// geoac.eigenray.sph.cpp:246-273).
+template <int G, int NEQ, bool PATHS = false, class NextJob>
 GA_HD void eig_lane_loop(const EigArgs &E, const Medium &M, int lane, NextJob next_job)
 {
     const FanArgs &A = E.F;
@@ -51,9 +55,11 @@ GA_HD void eig_lane_loop(const EigArgs &E, const Medium &M, int lane, NextJob ne
 
     double y[NEQ];
     double *ypl = A.ypbuf + (size_t)lane * 18;
-    double s = 0.0, tt = 0.0, att = 0.0, zmax = 0.0, c0 = 0.0, nu0s = 0.0;
+    double s = 0.0, tt = 0.0, att = 0.0, zmax = 0.0, c0 = 0.0, nu0s = 0.0, th = 0.0, ph = 0.0;
     int k = 0, leg = 0, nst = 0, job = 0, cur = 0, legs = 0, mode = 0, brk_leg = 0;
     bool fetch = true;
+    PathLane pl;
+    if (PATHS) path_reset(pl);
 #pragma unroll
     for (int i = 0; i < NEQ; ++i) y[i] = 0.0;
 
@@ -62,6 +68,8 @@ GA_HD void eig_lane_loop(const EigArgs &E, const Medium &M, int lane, NextJob ne
             job = next_job();
             if (job >= E.n_jobs) break;
             const EigJob J = E.jobs[job];
+            th = J.th;
+            ph = J.ph;
             init_ray_rad<G, NEQ>(A, M, J.th, J.ph, y, c0, nu0s, cur);
             legs = J.legs;
             mode = J.mode;
@@ -89,6 +97,12 @@ GA_HD void eig_lane_loop(const EigArgs &E, const Medium &M, int lane, NextJob ne
                 att += da;
                 const double zz = zcoord<G>(M, y1);
                 zmax = (zmax < zz) ? zz : zmax;
+                if (PATHS && (k == 1 || k % 15 == 0)) {
+                    double c, u, v, rho;
+                    point_medium<G>(M, y1, cur, c, u, v, rho);
+                    const double tl = amp_db<G>(A, M, y1, c, u, v, rho, c0, nu0s, th, ph, cur);
+                    path_put(A.path, pl, job, path_row<G>(M, y1, job, leg, k, GA_PT_RAY, tl, att, tt));
+                }
             }
             if (zcoord<G>(M, y) < L.z_keep) {
 #pragma unroll
@@ -121,6 +135,7 @@ GA_HD void eig_lane_loop(const EigArgs &E, const Medium &M, int lane, NextJob ne
             }
             E.brk_leg[job] = brk_leg;
             E.steps[job] = nst;
+            if (PATHS) path_close(A.path, pl, job);
             fetch = true;
             continue;
         }

@@ -219,6 +219,14 @@ public:
         if (lat_) *lat_ = done_ms_;
     }
 
+    // The accepted traces of the eigenrays results() returns, in the same
+    // order (call after results()).
+    void result_jobs(std::vector<EigJob> &jobs) const
+    {
+        jobs.clear();
+        for (const Found &f : found_) jobs.push_back(f.job);
+    }
+
 private:
     static constexpr uint32_t PENDING = 0xffffffffu;
 
@@ -495,7 +503,7 @@ private:
                     ga_arrival row = f.row;
                     row.launch_idx = ri;
                     row.n_bounce = nb;
-                    found_.push_back(Found{ri, nb, seq, row});
+                    found_.push_back(Found{ri, nb, seq, row, EigJob{theta, phi, nb, EIG_FINAL}});
                     break;
                 } else if (n > 0 && dr > dr_prev) {
                     theta -= dth * step_sc;
@@ -610,7 +618,7 @@ private:
                     ga_arrival row = f.row;
                     row.launch_idx = ri;
                     row.n_bounce = nb;
-                    found_.push_back(Found{ri, nb, seq, row});
+                    found_.push_back(Found{ri, nb, seq, row, EigJob{theta, phi, nb, EIG_FINAL}});
                     break;
                 } else if (n > 0 && dr > dr_prev) {
                     theta -= dth * step_sc;
@@ -648,6 +656,7 @@ private:
     struct Found {
         int rcvr, nb, seq;
         ga_arrival row;
+        EigJob job;                // the accepted trace (exact launch angles)
     };
     static bool found_order(const Found &a, const Found &b)
     {

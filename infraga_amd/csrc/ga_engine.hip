@@ -40,7 +40,8 @@ namespace {
 
 // ONE: one ray per lane (grid covers all rays, for launches that share the
 // device); otherwise persistent lanes pull rays from the queue counter.
-template <int G, int NEQ, bool LDS, bool ONE>
+// PATHS: write_rays / write_caustics bookkeeping with path records to A.path.
+template <int G, int NEQ, bool LDS, bool ONE, bool PATHS = false>
 __global__ void __launch_bounds__((trace_wg<G, NEQ>())) fan_kernel(FanArgs A)
 {
     extern __shared__ double lds_tab[];
@@ -60,7 +61,7 @@ __global__ void __launch_bounds__((trace_wg<G, NEQ>())) fan_kernel(FanArgs A)
             return r;
         });
     } else {
-        lane_loop<G, NEQ>(A, M, lane, [&]() { return (int)atomicAdd(A.queue, 1u); });
+        lane_loop<G, NEQ, PATHS>(A, M, lane, [&]() { return (int)atomicAdd(A.queue, 1u); });
     }
 }
 
@@ -112,7 +113,7 @@ __global__ void __launch_bounds__(256) rd_eval_kernel(RdGrid g, TopoGrid t, int 
 #ifndef GA_WG_RD
 #define GA_WG_RD 64
 #endif
-template <int NEQ, bool ONE, bool BLW>
+template <int NEQ, bool ONE, bool BLW, bool PATHS = false>
 __global__ void __launch_bounds__(GA_WG_RD) rd_fan_kernel(RdFanArgs A)
 {
     // three RK4 rows per lane (state, accumulator, stage point), lane-interleaved
@@ -128,7 +129,7 @@ __global__ void __launch_bounds__(GA_WG_RD) rd_fan_kernel(RdFanArgs A)
             return r;
         }, ya, yb, ys);
     } else {
-        rd_lane_loop<NEQ, BLW>(A, lane, [&]() { return (int)atomicAdd(A.queue, 1u); }, ya, yb, ys);
+        rd_lane_loop<NEQ, BLW, PATHS>(A, lane, [&]() { return (int)atomicAdd(A.queue, 1u); }, ya, yb, ys);
     }
 }
 
@@ -161,7 +162,7 @@ __global__ void rd_ground_kernel(RdMed M, double lat, double lon, double *out)
 #ifndef GA_EIG_WG6
 #define GA_EIG_WG6 1024
 #endif
-template <int G, int NEQ, bool LDS, int WGS, bool ONE>
+template <int G, int NEQ, bool LDS, int WGS, bool ONE, bool PATHS = false>
 __global__ void __launch_bounds__(WGS) eig_kernel(EigArgs E)
 {
     extern __shared__ double lds_tab[];
@@ -181,7 +182,7 @@ __global__ void __launch_bounds__(WGS) eig_kernel(EigArgs E)
             return r;
         });
     } else {
-        eig_lane_loop<G, NEQ>(E, M, lane, [&]() { return (int)atomicAdd(E.F.queue, 1u); });
+        eig_lane_loop<G, NEQ, PATHS>(E, M, lane, [&]() { return (int)atomicAdd(E.F.queue, 1u); });
     }
 }
 
@@ -370,10 +371,11 @@ bool build_medium(ga_ctx *ctx, const DevAtmo &a, const ga_prop_params *p, Medium
 }
 
 template <int G, int NEQ>
-int launch_fan(ga_ctx *ctx, FanBufs &fb, const FanArgs &A, hipStream_t st)
+int launch_fan(ga_ctx *ctx, FanBufs &fb, const FanArgs &A, hipStream_t st, bool paths)
 {
     size_t lds = A.lds ? (size_t)A.M.n * NODE * sizeof(double) : 0;
-    auto kern = A.one_shot ? (A.lds ? fan_kernel<G, NEQ, true, true> : fan_kernel<G, NEQ, false, true>)
+    auto kern = paths ? (A.lds ? fan_kernel<G, NEQ, true, false, true> : fan_kernel<G, NEQ, false, false, true>)
+              : A.one_shot ? (A.lds ? fan_kernel<G, NEQ, true, true> : fan_kernel<G, NEQ, false, true>)
                            : (A.lds ? fan_kernel<G, NEQ, true, false> : fan_kernel<G, NEQ, false, false>);
     if (lds > 65536) HIPCHK(ctx, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     int per_cu = 0;
@@ -382,7 +384,7 @@ int launch_fan(ga_ctx *ctx, FanBufs &fb, const FanArgs &A, hipStream_t st)
     if (per_cu < 1) per_cu = 1;
     int grid = per_cu * ctx->n_cu;
     int need = (A.n_rays + W - 1) / W;
-    grid = A.one_shot ? std::max(1, need) : std::max(1, std::min(grid, need));
+    grid = (A.one_shot && !paths) ? std::max(1, need) : std::max(1, std::min(grid, need));
     FanArgs B = A;
     B.nlanes = grid * W;
     size_t ypn = (size_t)B.nlanes * 18;
@@ -499,7 +501,8 @@ bool rd_region(const DevRd &R, double z_grnd, RdRegion &G)
 // source and the reference's z_grnd handling from run_prop
 // (main.infraga.sph.rngdep.cpp:229-315).
 int rd_fan_enqueue(ga_ctx *ctx, FanBufs &fb, int rd_id, const ga_prop_params *p, const double *d_launches, int n,
-                   ga_arrival *d_out, int32_t *d_counts, int32_t *d_steps, hipStream_t st)
+                   ga_arrival *d_out, int32_t *d_counts, int32_t *d_steps, hipStream_t st,
+                   const PathOut *path = nullptr)
 {
     auto it = ctx->rds.find(rd_id);
     if (it == ctx->rds.end()) return fail(ctx, GA_E_ATMO, "unknown range-dependent atmosphere id %d", rd_id);
@@ -586,7 +589,8 @@ int rd_fan_enqueue(ga_ctx *ctx, FanBufs &fb, int rd_id, const ga_prop_params *p,
     if (!fb.queue && hipMalloc(&fb.queue, sizeof(unsigned int)) != hipSuccess)
         return fail(ctx, GA_E_NOMEM, "queue counter");
     A.queue = fb.queue;
-    A.one_shot = serve_one_shot;
+    A.one_shot = path ? 0 : serve_one_shot;
+    if (path) A.path = *path;
     if (n == 0) return GA_OK;
 
     const bool amp = p->calc_amp != 0;
@@ -598,7 +602,10 @@ int rd_fan_enqueue(ga_ctx *ctx, FanBufs &fb, int rd_id, const ga_prop_params *p,
          {rd_fan_kernel<6, true, false>, rd_fan_kernel<6, true, true>}},
         {{rd_fan_kernel<18, false, false>, rd_fan_kernel<18, false, true>},
          {rd_fan_kernel<18, true, false>, rd_fan_kernel<18, true, true>}}};
-    const RdKern kern = kerns[amp ? 1 : 0][A.one_shot ? 1 : 0][blw ? 1 : 0];
+    // [amp][blw], write mode
+    const RdKern pkerns[2][2] = {{rd_fan_kernel<6, false, false, true>, rd_fan_kernel<6, false, true, true>},
+                                 {rd_fan_kernel<18, false, false, true>, rd_fan_kernel<18, false, true, true>}};
+    const RdKern kern = path ? pkerns[amp ? 1 : 0][blw ? 1 : 0] : kerns[amp ? 1 : 0][A.one_shot ? 1 : 0][blw ? 1 : 0];
     int per_cu = 0;
     constexpr int W = GA_WG_RD;
     HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, W, 0));
@@ -628,11 +635,13 @@ int rd_fan_enqueue(ga_ctx *ctx, FanBufs &fb, int rd_id, const ga_prop_params *p,
     return GA_OK;
 }
 
+// path: write-mode bookkeeping with path records (nullptr: write_rays=false sums).
 int fan_enqueue(ga_ctx *ctx, FanBufs &fb, int atmo_id, const ga_prop_params *p, const double *d_launches, int n,
-                ga_arrival *d_out, int32_t *d_counts, int32_t *d_steps, hipStream_t st)
+                ga_arrival *d_out, int32_t *d_counts, int32_t *d_steps, hipStream_t st,
+                const PathOut *path = nullptr)
 {
     if (p->geometry == GA_GEOM_SPH_RNGDEP)
-        return rd_fan_enqueue(ctx, fb, atmo_id, p, d_launches, n, d_out, d_counts, d_steps, st);
+        return rd_fan_enqueue(ctx, fb, atmo_id, p, d_launches, n, d_out, d_counts, d_steps, st, path);
     auto it = ctx->atmos.find(atmo_id);
     if (it == ctx->atmos.end()) return fail(ctx, GA_E_ATMO, "unknown atmosphere id %d", atmo_id);
     if (p->geometry != GA_GEOM_SPH && p->geometry != GA_GEOM_3D) return fail(ctx, GA_E_ARG, "bad geometry %d", p->geometry);
@@ -686,11 +695,13 @@ int fan_enqueue(ga_ctx *ctx, FanBufs &fb, int atmo_id, const ga_prop_params *p, 
         return fail(ctx, GA_E_NOMEM, "queue counter");
     A.queue = fb.queue;
     A.one_shot = serve_one_shot;
+    if (path) A.path = *path;
+    const bool pw = path != nullptr;
     size_t lds = (size_t)a.n * NODE * sizeof(double);
     A.lds = lds <= ctx->max_lds ? 1 : 0;
     if (n == 0) return GA_OK;
-    if (sph) return p->calc_amp ? launch_fan<GEOM_SPH, 18>(ctx, fb, A, st) : launch_fan<GEOM_SPH, 6>(ctx, fb, A, st);
-    return p->calc_amp ? launch_fan<GEOM_3D, 18>(ctx, fb, A, st) : launch_fan<GEOM_3D, 6>(ctx, fb, A, st);
+    if (sph) return p->calc_amp ? launch_fan<GEOM_SPH, 18>(ctx, fb, A, st, pw) : launch_fan<GEOM_SPH, 6>(ctx, fb, A, st, pw);
+    return p->calc_amp ? launch_fan<GEOM_3D, 18>(ctx, fb, A, st, pw) : launch_fan<GEOM_3D, 6>(ctx, fb, A, st, pw);
 }
 
 
@@ -748,6 +759,73 @@ template <class T> struct PinBuf {
     ~PinBuf() { if (p) (void)hipHostFree(p); }
 };
 
+// Device side of a path-record output (PathOut): chunk storage for `cap`
+// records plus one partly filled chunk per item, the chunk tags and the
+// per-item counts.  gather() puts the records of the used chunks in item
+// order on the host.
+struct PathDev {
+    DevBuf<ga_path_pt> pts;
+    DevBuf<int32_t> item, seq, fill, item_pts;
+    DevBuf<unsigned int> ctr;
+    unsigned int max_chunks = 0;
+    int n_items = 0;
+
+    int setup(ga_ctx *ctx, int64_t cap, int items, int flags, hipStream_t st, PathOut &P)
+    {
+        const long long chunks = (std::max<int64_t>(cap, 0) + PCHUNK - 1) / PCHUNK + (long long)items;
+        if (chunks >= 0x7fffffffLL) return fail(ctx, GA_E_ARG, "path capacity too large (%lld chunks)", chunks);
+        max_chunks = (unsigned int)std::max(chunks, 1LL);
+        n_items = items;
+        if (!pts.ensure((size_t)max_chunks * PCHUNK) || !item.ensure(max_chunks) || !seq.ensure(max_chunks) ||
+            !fill.ensure(max_chunks) || !item_pts.ensure((size_t)std::max(items, 1)) || !ctr.ensure(1))
+            return fail(ctx, GA_E_NOMEM, "path buffers (%u chunks of %d records)", max_chunks, PCHUNK);
+        hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, st, ctr.p);
+        HIPCHK(ctx, hipGetLastError());
+        P.pts = pts.p;
+        P.chunk_item = item.p;
+        P.chunk_seq = seq.p;
+        P.chunk_fill = fill.p;
+        P.item_pts = item_pts.p;
+        P.ctr = ctr.p;
+        P.max_chunks = max_chunks;
+        P.flags = flags;
+        return GA_OK;
+    }
+
+    // After the stream has finished: *total = records produced; when they
+    // fit in cap (and dst is given) they are written to dst in item order.
+    int gather(ga_ctx *ctx, ga_path_pt *dst, int64_t cap, int64_t *total)
+    {
+        unsigned int used = 0;
+        HIPCHK(ctx, hipMemcpy(&used, ctr.p, sizeof(unsigned int), hipMemcpyDeviceToHost));
+        std::vector<int32_t> cnt((size_t)std::max(n_items, 1), 0);
+        if (n_items > 0)
+            HIPCHK(ctx, hipMemcpy(cnt.data(), item_pts.p, (size_t)n_items * sizeof(int32_t), hipMemcpyDeviceToHost));
+        std::vector<int64_t> off((size_t)n_items + 1, 0);
+        for (int i = 0; i < n_items; ++i) off[i + 1] = off[i] + cnt[i];
+        *total = off[n_items];
+        if (*total > cap || used > max_chunks)
+            return fail(ctx, GA_E_CAP, "need %lld path records, have %lld", (long long)*total, (long long)cap);
+        if (*total == 0) return GA_OK;
+        if (!dst) return fail(ctx, GA_E_ARG, "null path buffer");
+        std::vector<int32_t> ci(used), cs(used), cf(used);
+        std::vector<ga_path_pt> buf((size_t)used * PCHUNK);
+        HIPCHK(ctx, hipMemcpy(ci.data(), item.p, used * sizeof(int32_t), hipMemcpyDeviceToHost));
+        HIPCHK(ctx, hipMemcpy(cs.data(), seq.p, used * sizeof(int32_t), hipMemcpyDeviceToHost));
+        HIPCHK(ctx, hipMemcpy(cf.data(), fill.p, used * sizeof(int32_t), hipMemcpyDeviceToHost));
+        HIPCHK(ctx, hipMemcpy(buf.data(), pts.p, buf.size() * sizeof(ga_path_pt), hipMemcpyDeviceToHost));
+        for (unsigned int c = 0; c < used; ++c) {
+            const int it = ci[c];
+            if (it < 0 || it >= n_items || cf[c] < 0 || cf[c] > PCHUNK)
+                return fail(ctx, GA_E_HIP, "corrupt path chunk %u (item %d, fill %d)", c, it, cf[c]);
+            const int64_t at = off[it] + (int64_t)cs[c] * PCHUNK;
+            if (at + cf[c] > off[it + 1]) return fail(ctx, GA_E_HIP, "path chunk %u overruns item %d", c, it);
+            std::memcpy(dst + at, buf.data() + (size_t)c * PCHUNK, (size_t)cf[c] * sizeof(ga_path_pt));
+        }
+        return GA_OK;
+    }
+};
+
 // GPU side of the eigenray search: one round of traces per enqueue().
 // EIG_SWEEP jobs run on stream st in the 6-equation kernel, EIG_LM/EIG_FINAL
 // jobs on stream s2 in the 18-equation kernel, followed there by the arrival
@@ -797,18 +875,19 @@ struct EigGpu {
         return GA_OK;
     }
 
-    template <int G, int NEQ, int WGS>
+    template <int G, int NEQ, int WGS, bool PATHS = false>
     int launch(const EigArgs &E0, DevBuf<double> &yp, unsigned int *queue, hipStream_t s)
     {
         size_t lds = F.lds ? (size_t)F.M.n * NODE * sizeof(double) : 0;
-        auto kern = E0.F.one_shot ? (F.lds ? eig_kernel<G, NEQ, true, WGS, true> : eig_kernel<G, NEQ, false, WGS, true>)
+        auto kern = PATHS ? (F.lds ? eig_kernel<G, NEQ, true, WGS, false, PATHS> : eig_kernel<G, NEQ, false, WGS, false, PATHS>)
+                  : E0.F.one_shot ? (F.lds ? eig_kernel<G, NEQ, true, WGS, true> : eig_kernel<G, NEQ, false, WGS, true>)
                                   : (F.lds ? eig_kernel<G, NEQ, true, WGS, false> : eig_kernel<G, NEQ, false, WGS, false>);
         if (lds > 65536) HIPCHK(ctx, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         int per_cu = 0;
         HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, WGS, lds));
         per_cu = std::max(per_cu, 1);
         const int need = (E0.n_jobs + WGS - 1) / WGS;
-        int grid = E0.F.one_shot ? std::max(1, need) : std::max(1, std::min(per_cu * ctx->n_cu, need));
+        int grid = (E0.F.one_shot && !PATHS) ? std::max(1, need) : std::max(1, std::min(per_cu * ctx->n_cu, need));
         if (!yp.ensure((size_t)grid * WGS * 18)) return fail(ctx, GA_E_NOMEM, "eigenray lane buffer");
         EigArgs E = E0;
         E.F.ypbuf = yp.p;
@@ -887,6 +966,37 @@ struct EigGpu {
     }
 
     bool ready() const { return hipEventQuery(e1) != hipErrorNotReady; }
+
+    // Re-traces accepted eigenrays (EIG_FINAL jobs, item = index in jobs)
+    // with their path rows into pd, on stream s2; synchronises.
+    int trace_paths(const std::vector<EigJob> &jobs, PathDev &pd, int64_t cap)
+    {
+        const size_t n = jobs.size();
+        if (!jobs18.ensure(n) || !res18.ensure(n * EIG_FULL) || !brk18.ensure(n) || !st18.ensure(n) ||
+            !h_jobs18.ensure(n))
+            return fail(ctx, GA_E_NOMEM, "eigenray path buffers (%zu traces)", n);
+        PathOut P{};
+        int rc = pd.setup(ctx, cap, (int)n, GA_PATH_RAYS, s2, P);
+        if (rc != GA_OK) return rc;
+        if (n) {
+            std::memcpy(h_jobs18.p, jobs.data(), n * sizeof(EigJob));
+            HIPCHK(ctx, hipMemcpyAsync(jobs18.p, h_jobs18.p, n * sizeof(EigJob), hipMemcpyHostToDevice, s2));
+            EigArgs E{};
+            E.F = F;
+            E.F.path = P;
+            E.jobs = jobs18.p;
+            E.n_jobs = (int)n;
+            E.stride = EIG_FULL;
+            E.res = res18.p;
+            E.brk_leg = brk18.p;
+            E.steps = st18.p;
+            rc = geom == GA_GEOM_SPH ? launch<GEOM_SPH, 18, trace_wg<GEOM_SPH, 18>(), true>(E, yp18, q.p + 1, s2)
+                                     : launch<GEOM_3D, 18, trace_wg<GEOM_3D, 18>(), true>(E, yp18, q.p + 1, s2);
+            if (rc != GA_OK) return rc;
+        }
+        HIPCHK(ctx, hipStreamSynchronize(s2));
+        return GA_OK;
+    }
 
     int collect(EigBatchOut &O)
     {
@@ -1564,9 +1674,72 @@ int ga_fan_run(ga_ctx *ctx, int atmo_id, const ga_prop_params *p, const double *
     return GA_OK;
 }
 
-int ga_eig_search(ga_ctx *ctx, int atmo_id, const ga_prop_params *p, const ga_eig_params *e,
-                  const double *rcvr_xy, int n_rcvr, ga_eigenray *out, int out_cap, int *n_out,
-                  double *rcvr_latency_ms, ga_run_stats *stats)
+int ga_fan_run_paths(ga_ctx *ctx, int atmo_id, const ga_prop_params *p, const double *launches, int n,
+                     int path_flags, ga_arrival *out, int out_cap, int *n_out, ga_path_pt *pts, int64_t pts_cap,
+                     int64_t *n_pts, ga_run_stats *stats)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    if (!ctx || !p || !n_out || !n_pts || (n > 0 && !launches)) return fail(ctx, GA_E_ARG, "null argument");
+    *n_out = 0;
+    *n_pts = 0;
+    if (n < 0) return fail(ctx, GA_E_ARG, "negative ray count");
+    if (pts_cap < 0) return fail(ctx, GA_E_ARG, "negative path capacity");
+    if (p->bounces < 0) return fail(ctx, GA_E_ARG, "bounces must be >= 0");
+    if (path_flags & ~(GA_PATH_RAYS | GA_PATH_CAUSTICS)) return fail(ctx, GA_E_ARG, "unknown path flags %d", path_flags);
+    if ((path_flags & GA_PATH_CAUSTICS) && !p->calc_amp)
+        return fail(ctx, GA_E_ARG, "caustics need the auxiliary equations (calc_amp)");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const int slots = p->bounces + 1;
+    const size_t nl = (size_t)std::max(n, 1);
+    DevBuf<double> d_l;
+    DevBuf<ga_arrival> d_o;
+    DevBuf<int32_t> d_c, d_s;
+    if (!d_l.alloc(nl * 2) || !d_o.alloc(nl * slots) || !d_c.alloc(nl) || !d_s.alloc(nl))
+        return fail(ctx, GA_E_NOMEM, "device buffers for %d rays", n);
+    PathDev pd;
+    PathOut P{};
+    int rc = pd.setup(ctx, pts_cap, n, path_flags, ctx->stream, P);
+    if (rc != GA_OK) return rc;
+    Events ev;
+    if (!ev.ok) return fail(ctx, GA_E_HIP, "hipEventCreate failed");
+    HIPCHK(ctx, hipMemcpyAsync(d_l.p, launches, (size_t)n * 2 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipEventRecord(ev.e0, ctx->stream));
+    rc = fan_enqueue(ctx, ctx->fb, atmo_id, p, d_l.p, n, d_o.p, d_c.p, d_s.p, ctx->stream, &P);
+    if (rc != GA_OK) return rc;
+    HIPCHK(ctx, hipEventRecord(ev.e1, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    float ms = 0.0f;
+    HIPCHK(ctx, hipEventElapsedTime(&ms, ev.e0, ev.e1));
+    std::vector<int32_t> cnt(nl, 0), stp(nl, 0);
+    std::vector<ga_arrival> all;
+    if (n > 0) {
+        HIPCHK(ctx, hipMemcpy(cnt.data(), d_c.p, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost));
+        HIPCHK(ctx, hipMemcpy(stp.data(), d_s.p, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost));
+        all.resize((size_t)n * slots);
+        HIPCHK(ctx, hipMemcpy(all.data(), d_o.p, all.size() * sizeof(ga_arrival), hipMemcpyDeviceToHost));
+    }
+    long long total = 0, steps = 0;
+    for (int r = 0; r < n; ++r) { total += cnt[r]; steps += stp[r]; }
+    *n_out = (int)total;
+    const int prc = pd.gather(ctx, pts, pts_cap, n_pts);
+    if (stats) {
+        stats->ray_steps = steps;
+        stats->kernel_ms = ms;
+        stats->n_rays = n;
+        stats->n_launches = 4;
+        stats->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    if (total > out_cap || (!out && total > 0)) return fail(ctx, GA_E_CAP, "need %lld arrival slots, have %d", total, out_cap);
+    size_t w = 0;
+    for (int r = 0; r < n; ++r)
+        for (int b = 0; b < cnt[r]; ++b) out[w++] = all[(size_t)r * slots + b];
+    return prc;
+}
+
+static int eig_search_impl(ga_ctx *ctx, int atmo_id, const ga_prop_params *p, const ga_eig_params *e,
+                           const double *rcvr_xy, int n_rcvr, ga_eigenray *out, int out_cap, int *n_out,
+                           double *rcvr_latency_ms, ga_run_stats *stats, bool paths, ga_path_pt *pts,
+                           int64_t pts_cap, int64_t *n_pts)
 {
     auto t0 = std::chrono::steady_clock::now();
     if (!ctx || !p || !e || !n_out || (n_rcvr > 0 && !rcvr_xy)) return fail(ctx, GA_E_ARG, "null argument");
@@ -1587,16 +1760,26 @@ int ga_eig_search(ga_ctx *ctx, int atmo_id, const ga_prop_params *p, const ga_ei
     rc = gpu.init(ctx->stream);
     if (rc != GA_OK) return rc;
     std::vector<ga_arrival> found;
+    std::vector<EigJob> jobs;
     std::vector<double> lat;
     EigSearchStats es;
     {
         EigSearch search(S, rcvr.data(), n_rcvr);
         rc = search.run([&](const EigBatch &B, EigBatchOut &O) { return gpu(B, O); }, found, lat, es);
+        search.result_jobs(jobs);
     }
     if (rc != GA_OK) return rc;
     *n_out = (int)found.size();
     if (rcvr_latency_ms)
         for (int i = 0; i < n_rcvr; ++i) rcvr_latency_ms[i] = lat[i];
+    int prc = GA_OK;
+    if (paths) {
+        // the accepted traces once more, writing their path rows
+        PathDev pd;
+        prc = gpu.trace_paths(jobs, pd, pts_cap);
+        if (prc == GA_OK) prc = pd.gather(ctx, pts, pts_cap, n_pts);
+        else *n_pts = 0;
+    }
     if (stats) {
         stats->ray_steps = es.ray_steps;
         stats->kernel_ms = gpu.kernel_ms;
@@ -1607,7 +1790,27 @@ int ga_eig_search(ga_ctx *ctx, int atmo_id, const ga_prop_params *p, const ga_ei
     if ((long long)found.size() > out_cap || (!out && !found.empty()))
         return fail(ctx, GA_E_CAP, "need %zu eigenray slots, have %d", found.size(), out_cap);
     for (size_t i = 0; i < found.size(); ++i) out[i] = found[i];
-    return GA_OK;
+    return prc;
+}
+
+int ga_eig_search(ga_ctx *ctx, int atmo_id, const ga_prop_params *p, const ga_eig_params *e,
+                  const double *rcvr_xy, int n_rcvr, ga_eigenray *out, int out_cap, int *n_out,
+                  double *rcvr_latency_ms, ga_run_stats *stats)
+{
+    return eig_search_impl(ctx, atmo_id, p, e, rcvr_xy, n_rcvr, out, out_cap, n_out, rcvr_latency_ms, stats, false,
+                           nullptr, 0, nullptr);
+}
+
+int ga_eig_search_paths(ga_ctx *ctx, int atmo_id, const ga_prop_params *p, const ga_eig_params *e,
+                        const double *rcvr_xy, int n_rcvr, ga_eigenray *out, int out_cap, int *n_out,
+                        double *rcvr_latency_ms, ga_path_pt *pts, int64_t pts_cap, int64_t *n_pts,
+                        ga_run_stats *stats)
+{
+    if (!n_pts) return fail(ctx, GA_E_ARG, "null argument");
+    *n_pts = 0;
+    if (pts_cap < 0) return fail(ctx, GA_E_ARG, "negative path capacity");
+    return eig_search_impl(ctx, atmo_id, p, e, rcvr_xy, n_rcvr, out, out_cap, n_out, rcvr_latency_ms, stats, true,
+                           pts, pts_cap, n_pts);
 }
 
 

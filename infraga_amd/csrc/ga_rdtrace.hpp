@@ -61,6 +61,7 @@ struct RdFanArgs {
     int32_t *counts;
     int32_t *steps;
     unsigned int *queue;
+    PathOut path;              // path output of the write-mode kernels
 };
 
 // A lane's vector stored lane-interleaved: element i at p[i * STRIDE] with p
@@ -753,6 +754,37 @@ GA_HD void rd_reflect(const RdMed &M, const double *yk2, const double *yk1, doub
     }
 }
 
+// Jacobian determinant at row y with medium values P (geoac.eqset.sph.rngdep.cpp:798-821).
+GA_HD double rd_jacobian(const double *y, const RdPt &P)
+{
+    const double r = y[0], ct_ = cos(y[1]);
+    const double nm = sqrt(y[3] * y[3] + y[4] * y[4] + y[5] * y[5]);
+    const double p0 = P.c * y[3] / nm + P.w, p1 = P.c * y[4] / nm + P.v, p2 = P.c * y[5] / nm + P.u;
+    const double pm = sqrt(p0 * p0 + p1 * p1 + p2 * p2);
+    const double drs = p0 / pm, dts = 1.0 / r * p1 / pm, dps = 1.0 / (r * ct_) * p2 / pm;
+    return r * r * ct_ * (drs * (y[7] * y[14] - y[13] * y[8]) - y[6] * (dts * y[14] - dps * y[13]) +
+                          y[12] * (dts * y[8] - dps * y[7]));
+}
+
+// 20 log10 of the amplitude at row y (geoac.eqset.sph.rngdep.cpp:823-847);
+// P: medium at y, S: medium at the source.
+GA_HD double rd_amp_db(const double *y, const RdPt &P, const RdPt &S, double c0, double nu0s, double th, double ph)
+{
+    const double jac = rd_jacobian(y, P);
+    double st, ct, sp, cp;
+    sincos(th, &st, &ct);
+    sincos(ph, &sp, &cp);
+    const double nm = (c0 - y[3] * P.w - y[4] * P.v - y[5] * P.u) / P.c;
+    const double q0 = c0 * (nu0s * st) / nu0s + S.w, q1 = c0 * (nu0s * ct * sp) / nu0s + S.v,
+                 q2 = c0 * (nu0s * ct * cp) / nu0s + S.u;
+    const double p0 = P.c * y[3] / nm + P.w, p1 = P.c * y[4] / nm + P.v, p2 = P.c * y[5] / nm + P.u;
+    const double pm = sqrt(p0 * p0 + p1 * p1 + p2 * p2);
+    const double qm = sqrt(q0 * q0 + q1 * q1 + q2 * q2);
+    const double num = P.rho * nm * (P.c * P.c * P.c) * qm * ct;
+    const double den = S.rho * nu0s * (S.c * S.c * S.c) * pm * jac;
+    return 20.0 * log10(sqrt(fabs(num / den)));
+}
+
 // Arrival row (run_prop :515-539): inclination without the stratified
 // driver's minus sign; TL from the Jacobian and amplitude (:798-847).
 template <int NEQ>
@@ -779,40 +811,19 @@ GA_HD ga_arrival rd_arrival_row(const RdFanArgs &A, const double *y, int ray, in
     R.time = tt;
     R.turn_ht = rmax;
     R.absorption = -2.0 * att;
-    R.tl = 0.0;
-    if (NEQ == 18) {
-        const double r = y[0], ct_ = cos(y[1]);
-        // jacobian (:798-821)
-        double nm = sqrt(y[3] * y[3] + y[4] * y[4] + y[5] * y[5]);
-        double p0 = P.c * y[3] / nm + P.w, p1 = P.c * y[4] / nm + P.v, p2 = P.c * y[5] / nm + P.u;
-        double pm = sqrt(p0 * p0 + p1 * p1 + p2 * p2);
-        const double drs = p0 / pm, dts = 1.0 / r * p1 / pm, dps = 1.0 / (r * ct_) * p2 / pm;
-        const double jac = r * r * ct_ * (drs * (y[7] * y[14] - y[13] * y[8]) - y[6] * (dts * y[14] - dps * y[13]) +
-                                          y[12] * (dts * y[8] - dps * y[7]));
-        // amp (:823-847)
-        double st, ct, sp, cp;
-        sincos(th, &st, &ct);
-        sincos(ph, &sp, &cp);
-        nm = (c0 - y[3] * P.w - y[4] * P.v - y[5] * P.u) / P.c;
-        const double q0 = c0 * (nu0s * st) / nu0s + S.w, q1 = c0 * (nu0s * ct * sp) / nu0s + S.v,
-                     q2 = c0 * (nu0s * ct * cp) / nu0s + S.u;
-        p0 = P.c * y[3] / nm + P.w;
-        p1 = P.c * y[4] / nm + P.v;
-        p2 = P.c * y[5] / nm + P.u;
-        pm = sqrt(p0 * p0 + p1 * p1 + p2 * p2);
-        const double qm = sqrt(q0 * q0 + q1 * q1 + q2 * q2);
-        const double num = P.rho * nm * (P.c * P.c * P.c) * qm * ct;
-        const double den = S.rho * nu0s * (S.c * S.c * S.c) * pm * jac;
-        R.tl = 20.0 * log10(sqrt(fabs(num / den)));
-    }
+    R.tl = (NEQ == 18) ? rd_amp_db(y, P, S, c0, nu0s, th, ph) : 0.0;
     return R;
 }
 
 // Per-lane trace loop (the range-dependent run_prop with write_rays=false).
 // BLW: instantiation for M.use_blw on a terrain (rd_medium_rhs).  ya, yb:
 // the lane's two state rows (current row and the RK4 accumulator, which
-// swap roles after an accepted step); ys: the RK4 stage point.
-template <int NEQ, bool BLW, class V, class NextRay>
+// swap roles after an accepted step); ys: the RK4 stage point.  PATHS: the
+// write_rays / write_caustics bookkeeping of run_prop
+// (main.infraga.sph.rngdep.cpp:451-490): segments of rows 1..k-1 added one
+// by one, rows m == 1 or m % 15 == 0 and Jacobian sign changes written to
+// A.path.
+template <int NEQ, bool BLW, bool PATHS = false, class V, class NextRay>
 GA_HD void rd_lane_loop(const RdFanArgs &A, int lane, NextRay next_ray, V ya, V yb, V ys)
 {
     const Limits L = A.L;
@@ -823,6 +834,9 @@ GA_HD void rd_lane_loop(const RdFanArgs &A, int lane, NextRay next_ray, V ya, V 
     double c0 = 0.0, nu0s = 0.0, th = 0.0, ph = 0.0;
     int k = 0, bnc = 0, nst = 0, ray = 0, narr = 0;
     bool fetch = true;
+    PathLane pl;
+    double d_prev = 0.0;
+    if (PATHS) path_reset(pl);
 
     for (;;) {
         if (fetch) {
@@ -855,10 +869,50 @@ GA_HD void rd_lane_loop(const RdFanArgs &A, int lane, NextRay next_ray, V ya, V 
         double o[6];
 #pragma unroll
         for (int i = 0; i < 6; ++i) o[i] = y[i];
-        double dt, da;
-        rd_seg_book(M, o, n, A.freq, dt, da);
-        ttl += dt;
-        attl += da;
+        if (PATHS) {
+            if (!end) {
+                double dt, da;
+                rd_seg_book(M, o, n, A.freq, dt, da);
+                tt += dt;
+                att += da;
+                const bool caus = NEQ == 18 && (A.path.flags & GA_PATH_CAUSTICS);
+                const bool samp = (A.path.flags & GA_PATH_RAYS) && (k == 1 || k % 15 == 0);
+                if (caus || samp) {
+                    RdPt P;
+                    rd_point(M, n[0], n[1], n[2], P);
+                    if (samp) {
+                        double tl = 0.0;
+                        if (NEQ == 18) {
+                            RdPt S;
+                            rd_point(M, A.src0, A.src1, A.src2, S);
+                            tl = rd_amp_db(n, P, S, c0, nu0s, th, ph);
+                        }
+                        ga_path_pt R;
+                        R.launch_idx = ray; R.leg = bnc; R.step = k; R.kind = GA_PT_RAY;
+                        R.pos0 = n[1] * (180.0 / PI); R.pos1 = n[2] * (180.0 / PI); R.z = n[0] - M.r0;
+                        R.tl = tl; R.absorption = -2.0 * att; R.time = tt;
+                        path_put(A.path, pl, ray, R);
+                    }
+                    if (caus) {
+                        const double d = rd_jacobian(n, P);
+                        if (k == 1) d_prev = d;
+                        if (d * d_prev < 0.0) {
+                            ga_path_pt R;
+                            R.launch_idx = ray; R.leg = bnc; R.step = k; R.kind = GA_PT_CAUSTIC;
+                            R.pos0 = n[1] * (180.0 / PI); R.pos1 = n[2] * (180.0 / PI); R.z = n[0] - M.r0;
+                            R.tl = 0.0; R.absorption = -2.0 * att; R.time = tt;
+                            path_put(A.path, pl, ray, R);
+                        }
+                        d_prev = d;
+                    }
+                }
+            }
+        } else {
+            double dt, da;
+            rd_seg_book(M, o, n, A.freq, dt, da);
+            ttl += dt;
+            attl += da;
+        }
         rmax = fmax(rmax, o[0] - M.r0);
 
         if (!end) {
@@ -905,6 +959,7 @@ GA_HD void rd_lane_loop(const RdFanArgs &A, int lane, NextRay next_ray, V ya, V 
         if (done) {
             A.counts[ray] = narr;
             A.steps[ray] = nst;
+            if (PATHS) path_close(A.path, pl, ray);
             fetch = true;
         }
     }

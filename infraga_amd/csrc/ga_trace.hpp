@@ -35,6 +35,64 @@ template <int G, int NEQ> constexpr int trace_wg()
 // code out of the step loop's register budget.
 constexpr int RAW = 25;
 
+// Ray-path output (write_rays / write_caustics of run_prop, the
+// eigenray-N.dat files of find_eigenray).  Records of one item (ray or
+// eigenray) are appended by the lane tracing it into chunks of PCHUNK
+// records taken from a device counter; each chunk carries its item and its
+// order within the item, so the host puts the records in sequence without a
+// sort.  Per-item record counts are kept also beyond the capacity, so a
+// caller can size a second pass exactly.
+constexpr int PCHUNK = 32;
+
+struct PathOut {
+    ga_path_pt *pts;           // [max_chunks][PCHUNK]
+    int32_t *chunk_item;       // item of each chunk
+    int32_t *chunk_seq;        // position of the chunk within its item
+    int32_t *chunk_fill;       // records in the chunk
+    int32_t *item_pts;         // records per item (all counted)
+    unsigned int *ctr;         // chunks handed out
+    unsigned int max_chunks;
+    int flags;                 // GA_PATH_RAYS | GA_PATH_CAUSTICS
+};
+
+// Per-lane cursor into PathOut.
+struct PathLane {
+    unsigned int chunk;        // current chunk (valid when fill > 0 or n > 0)
+    int fill, seq, n;
+};
+
+GA_HD void path_reset(PathLane &c)
+{
+    c.chunk = 0xffffffffu;
+    c.fill = PCHUNK;
+    c.seq = 0;
+    c.n = 0;
+}
+
+GA_HD void path_put(const PathOut &P, PathLane &c, int item, const ga_path_pt &r)
+{
+    if (c.fill == PCHUNK) {
+        if (c.chunk < P.max_chunks) P.chunk_fill[c.chunk] = c.fill;
+        c.chunk = atomicAdd(P.ctr, 1u);
+        c.fill = 0;
+        if (c.chunk < P.max_chunks) {
+            P.chunk_item[c.chunk] = item;
+            P.chunk_seq[c.chunk] = c.seq;
+        }
+        ++c.seq;
+    }
+    if (c.chunk < P.max_chunks) P.pts[(size_t)c.chunk * PCHUNK + c.fill] = r;
+    ++c.fill;
+    ++c.n;
+}
+
+GA_HD void path_close(const PathOut &P, PathLane &c, int item)
+{
+    if (c.n > 0 && c.chunk < P.max_chunks) P.chunk_fill[c.chunk] = c.fill;
+    P.item_pts[item] = c.n;
+    path_reset(c);
+}
+
 struct FanArgs {
     Medium M;
     Limits L;
@@ -55,6 +113,7 @@ struct FanArgs {
     int32_t *counts;
     int32_t *steps;
     unsigned int *queue;
+    PathOut path;              // path output of the write-mode kernels
 };
 
 // ---------------------------------------------------------------------------
@@ -166,6 +225,67 @@ GA_HD double jacobian(const double *y, double c, double u, double v)
     return dxs * (y[7] * y[14] - y[13] * y[8]) - y[6] * (dys * y[14] - dzs * y[13]) + y[12] * (dys * y[8] - dzs * y[7]);
 }
 
+// 20 log10 of the transport-equation amplitude at row y, whose medium
+// values are c, u, v, rho (geoac::amp, geoac.eqset.sph.strat.cpp:825-849,
+// geoac.eqset.3d.strat.cpp:725-748).
+template <int G>
+GA_HD double amp_db(const FanArgs &A, const Medium &M, const double *y, double c, double u, double v, double rho,
+                    double c0, double nu0s, double th, double ph, int &cur)
+{
+    double src[3] = {A.src0, A.src1, A.src2};
+    double cs, us, vs, rhos;
+    point_medium<G>(M, src, cur, cs, us, vs, rhos);
+    double jac = jacobian<G>(y, c, u, v);
+    double st, ct, sp, cp;
+    sincos(th, &st, &ct);
+    sincos(ph, &sp, &cp);
+    double num, den;
+    if (G == GEOM_SPH) {
+        double nm = (c0 - y[4] * v - y[5] * u) / c;
+        double q0 = c0 * (nu0s * st) / nu0s, q1 = c0 * (nu0s * ct * sp) / nu0s + vs, q2 = c0 * (nu0s * ct * cp) / nu0s + us;
+        double p0 = c * y[3] / nm, p1 = c * y[4] / nm + v, p2 = c * y[5] / nm + u;
+        double pm = sqrt(p0 * p0 + p1 * p1 + p2 * p2), qm = sqrt(q0 * q0 + q1 * q1 + q2 * q2);
+        num = rho * nm * (c * c * c) * qm * ct;
+        den = rhos * nu0s * (cs * cs * cs) * pm * jac;
+    } else {
+        double nm = sqrt(y[3] * y[3] + y[4] * y[4] + y[5] * y[5]);
+        double nm0 = 1.0 - (y[3] * us + y[4] * vs) / c0;
+        double p0 = c * y[3] / nm + u, p1 = c * y[4] / nm + v, p2 = c * y[5] / nm;
+        double q0 = c0 * ct * cp + us, q1 = c0 * ct * sp + vs, q2 = c0 * st;
+        double pm = sqrt(p0 * p0 + p1 * p1 + p2 * p2), qm = sqrt(q0 * q0 + q1 * q1 + q2 * q2);
+        num = rho * nm * (c * c * c) * qm * ct;
+        den = rhos * nm0 * (c0 * c0 * c0) * pm * jac;
+    }
+    return 20.0 * log10(sqrt(fabs(num / den)));
+}
+
+// One ray-path / caustic record of row y (raypath and caustics columns of
+// main.infraga.sph.cpp:471-490, main.infraga.3d.cpp:458-478): sph lat, lon
+// [deg] and r - r0; 3D x, y and max(z, ground).
+template <int G>
+GA_HD ga_path_pt path_row(const Medium &M, const double *y, int item, int leg, int m, int kind, double tl,
+                          double att, double tt)
+{
+    ga_path_pt P;
+    P.launch_idx = item;
+    P.leg = leg;
+    P.step = m;
+    P.kind = kind;
+    if (G == GEOM_SPH) {
+        P.pos0 = y[1] * (180.0 / PI);
+        P.pos1 = y[2] * (180.0 / PI);
+        P.z = y[0] - M.r0;
+    } else {
+        P.pos0 = y[0];
+        P.pos1 = y[1];
+        P.z = y[2] > M.zg ? y[2] : M.zg;
+    }
+    P.tl = tl;
+    P.absorption = -2.0 * att;
+    P.time = tt;
+    return P;
+}
+
 // Arrival record (main.infraga.sph.cpp:523-544, main.infraga.3d.cpp:511-530).
 template <int G, int NEQ>
 GA_HD ga_arrival arrival_row(const FanArgs &A, const Medium &M, const double *y, int ray, int bnc,
@@ -203,34 +323,7 @@ GA_HD ga_arrival arrival_row(const FanArgs &A, const Medium &M, const double *y,
     R.time = tt;
     R.turn_ht = rmax;
     R.absorption = -2.0 * att;
-    R.tl = 0.0;
-    if (NEQ == 18) {
-        double src[3] = {A.src0, A.src1, A.src2};
-        double cs, us, vs, rhos;
-        point_medium<G>(M, src, cur, cs, us, vs, rhos);
-        double jac = jacobian<G>(y, c, u, v);
-        double st, ct, sp, cp;
-        sincos(th, &st, &ct);
-        sincos(ph, &sp, &cp);
-        double num, den;
-        if (G == GEOM_SPH) {
-            double nm = (c0 - y[4] * v - y[5] * u) / c;
-            double q0 = c0 * (nu0s * st) / nu0s, q1 = c0 * (nu0s * ct * sp) / nu0s + vs, q2 = c0 * (nu0s * ct * cp) / nu0s + us;
-            double p0 = c * y[3] / nm, p1 = c * y[4] / nm + v, p2 = c * y[5] / nm + u;
-            double pm = sqrt(p0 * p0 + p1 * p1 + p2 * p2), qm = sqrt(q0 * q0 + q1 * q1 + q2 * q2);
-            num = rho * nm * (c * c * c) * qm * ct;
-            den = rhos * nu0s * (cs * cs * cs) * pm * jac;
-        } else {
-            double nm = sqrt(y[3] * y[3] + y[4] * y[4] + y[5] * y[5]);
-            double nm0 = 1.0 - (y[3] * us + y[4] * vs) / c0;
-            double p0 = c * y[3] / nm + u, p1 = c * y[4] / nm + v, p2 = c * y[5] / nm;
-            double q0 = c0 * ct * cp + us, q1 = c0 * ct * sp + vs, q2 = c0 * st;
-            double pm = sqrt(p0 * p0 + p1 * p1 + p2 * p2), qm = sqrt(q0 * q0 + q1 * q1 + q2 * q2);
-            num = rho * nm * (c * c * c) * qm * ct;
-            den = rhos * nm0 * (c0 * c0 * c0) * pm * jac;
-        }
-        R.tl = 20.0 * log10(sqrt(fabs(num / den)));
-    }
+    R.tl = (NEQ == 18) ? amp_db<G>(A, M, y, c, u, v, rho, c0, nu0s, th, ph, cur) : 0.0;
     return R;
 }
 
@@ -292,7 +385,15 @@ GA_HD void reflect(const Medium &M, const double *yk2, const double *yk1, double
 
 // Per-lane trace loop: pulls rays from next_ray() until it returns a value
 // >= A.n_rays.  Shared by the fan kernel and host-side builds of this code.
-template <int G, int NEQ, class NextRay>
+// PATHS: the write_rays / write_caustics bookkeeping of run_prop
+// (main.infraga.sph.cpp:456-496, main.infraga.3d.cpp:442-487): the segments
+// of rows 1..k-1 of each leg are added one by one to the running travel time
+// and absorption (the segment into the sub-ground row k is not), rows m == 1
+// or m % 15 == 0 are written to A.path (GA_PATH_RAYS), and a sign change of
+// the Jacobian determinant between rows m-1 and m is a caustic
+// (GA_PATH_CAUSTICS, 18 equations).  Otherwise each leg is summed whole
+// (rows 0..k) and added to the totals at its end (write_rays=false).
+template <int G, int NEQ, bool PATHS = false, class NextRay>
 GA_HD void lane_loop(const FanArgs &A, const Medium &M, int lane, NextRay next_ray)
 {
     const Limits L = A.L;
@@ -303,6 +404,9 @@ GA_HD void lane_loop(const FanArgs &A, const Medium &M, int lane, NextRay next_r
     double c0 = 0.0, nu0s = 0.0, th = 0.0, ph = 0.0, incl = 0.0, az = 0.0;
     int k = 0, bnc = 0, nst = 0, ray = 0, cur = 0, narr = 0;
     bool fetch = true;
+    PathLane pl;
+    double d_prev = 0.0;
+    if (PATHS) path_reset(pl);
 #pragma unroll
     for (int i = 0; i < NEQ; ++i) y[i] = 0.0;
 
@@ -328,10 +432,36 @@ GA_HD void lane_loop(const FanArgs &A, const Medium &M, int lane, NextRay next_r
         else if (k >= A.max_steps) { brk = true; end = true; }
         else if (!(y1[0] == y1[0])) { brk = true; end = true; }   // non-finite state: stop the ray
 
-        double dt, da;
-        seg_book<G>(M, y, y1, A.freq, cur, dt, da);
-        ttl += dt;
-        attl += da;
+        if (PATHS) {
+            if (!end) {
+                double dt, da;
+                seg_book<G>(M, y, y1, A.freq, cur, dt, da);
+                tt += dt;
+                att += da;
+                const bool caus = NEQ == 18 && (A.path.flags & GA_PATH_CAUSTICS);
+                const bool samp = (A.path.flags & GA_PATH_RAYS) && (k == 1 || k % 15 == 0);
+                if (caus || samp) {
+                    double c, u, v, rho;
+                    point_medium<G>(M, y1, cur, c, u, v, rho);
+                    if (samp) {
+                        const double tl = NEQ == 18 ? amp_db<G>(A, M, y1, c, u, v, rho, c0, nu0s, th, ph, cur) : 0.0;
+                        path_put(A.path, pl, ray, path_row<G>(M, y1, ray, bnc, k, GA_PT_RAY, tl, att, tt));
+                    }
+                    if (caus) {
+                        const double d = jacobian<G>(y1, c, u, v);
+                        if (k == 1) d_prev = d;
+                        if (d * d_prev < 0.0)
+                            path_put(A.path, pl, ray, path_row<G>(M, y1, ray, bnc, k, GA_PT_CAUSTIC, 0.0, att, tt));
+                        d_prev = d;
+                    }
+                }
+            }
+        } else {
+            double dt, da;
+            seg_book<G>(M, y, y1, A.freq, cur, dt, da);
+            ttl += dt;
+            attl += da;
+        }
         rmax = fmax(rmax, zcoord<G>(M, y));
 
         if (!end) {
@@ -375,6 +505,7 @@ GA_HD void lane_loop(const FanArgs &A, const Medium &M, int lane, NextRay next_r
         if (done) {
             A.counts[ray] = narr;
             A.steps[ray] = nst;
+            if (PATHS) path_close(A.path, pl, ray);
             fetch = true;
         }
     }

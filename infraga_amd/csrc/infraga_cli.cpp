@@ -12,12 +12,10 @@
 //
 // Differences from the reference drivers, by design:
 //  - an unknown key is an error (exit 2) instead of an interactive prompt;
-//  - write_rays=true writes the arrivals but not <id>.raypaths.dat (ray path
-//    sampling is not part of this engine); write_caustics and write_topo are
-//    rejected, and so are topo_file / topo_use_BLw outside the
-//    range-dependent binary;
-//  - -eig_search writes <id>.arrivals.dat but not the per-eigenray path
-//    files <id>.eigenray-N.dat; verbose=true prints no trace-by-trace log.
+//  - write_topo is rejected, and so are topo_file / topo_use_BLw outside the
+//    range-dependent binary; write_caustics needs calc_amp=true (the
+//    reference would read auxiliary columns it never integrated);
+//  - verbose=true prints no trace-by-trace log.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -88,6 +86,80 @@ std::string strip_ext(char *path)
 
 double profile_bottom(const char *path);
 
+// The fan of run_prop with write_rays / write_caustics
+// (main.infraga.sph.cpp:388-413, 456-496, 549; main.infraga.3d.cpp:378-400,
+// 442-487; main.infraga.sph.rngdep.cpp:382-406, 451-490, 544): rays go to the
+// engine in batches; <id>.raypaths.dat gets each ray's sampled rows and a
+// blank line, <id>.caustics-<leg>.dat the caustic points of each leg.  The
+// reference's sph / range-dependent writers print at precision 8 (the
+// setprecision of the first column sticks), the 3D ones at the stream
+// default 6.  arr receives the arrivals in launch order.
+int fan_with_paths(ga_ctx *ctx, int aid, const ga_prop_params &p, const std::vector<double> &launches,
+                   bool write_rays, bool write_caustics, const std::string &id, bool sph,
+                   std::vector<ga_arrival> &arr, int &n_out, ga_run_stats &st)
+{
+    const int n = (int)(launches.size() / 2);
+    const int slots = p.bounces + 1;
+    std::ofstream raypath;
+    std::vector<std::ofstream> caus;
+    if (write_rays) {
+        raypath.open(id + ".raypaths.dat");
+        raypath << (sph ? "# lat [deg]" : "# x [km]") << '\t' << (sph ? "lon [deg]" : "y [km]") << '\t' << "z [km]"
+                << '\t' << "trans. coeff. [dB]" << '\t' << "absorption [dB]" << '\t' << "time [s]" << '\n';
+        if (sph) raypath << std::setprecision(8);
+    }
+    if (write_caustics) {
+        for (int b = 0; b <= p.bounces; ++b) {
+            caus.emplace_back(id + ".caustics-" + std::to_string(b) + ".dat");
+            caus.back() << (sph ? "# lat [deg]" : "# x [km]") << '\t' << (sph ? "lon [deg]" : "y [km]") << '\t'
+                        << "z [km]" << '\t' << "time [s]" << '\n';
+            if (sph) caus.back() << std::setprecision(8);
+        }
+    }
+    const int flags = (write_rays ? GA_PATH_RAYS : 0) | (write_caustics ? GA_PATH_CAUSTICS : 0);
+    const int batch = 4096;
+    int64_t cap = (int64_t)1 << 21;
+    std::vector<ga_path_pt> pts;
+    arr.assign((size_t)std::max(1, n) * slots, ga_arrival{});
+    n_out = 0;
+    st = ga_run_stats{};
+    for (int r0 = 0; r0 < n; r0 += batch) {
+        const int nb = std::min(batch, n - r0);
+        int nb_out = 0;
+        int64_t n_pts = 0;
+        ga_run_stats bs;
+        int rc;
+        for (;;) {
+            pts.resize((size_t)cap);
+            rc = ga_fan_run_paths(ctx, aid, &p, launches.data() + 2 * (size_t)r0, nb, flags, arr.data() + n_out,
+                                  nb * slots, &nb_out, pts.data(), cap, &n_pts, &bs);
+            if (rc == GA_E_CAP && n_pts > cap) { cap = n_pts + n_pts / 4; continue; }
+            break;
+        }
+        if (rc != GA_OK) return rc;
+        for (int i = 0; i < nb_out; ++i) arr[n_out + i].launch_idx += r0;
+        n_out += nb_out;
+        st.ray_steps += bs.ray_steps;
+        st.kernel_ms += bs.kernel_ms;
+        st.n_rays += bs.n_rays;
+        st.n_launches += bs.n_launches;
+        int64_t w = 0;
+        for (int r = 0; r < nb; ++r) {
+            for (; w < n_pts && pts[w].launch_idx == r; ++w) {
+                const ga_path_pt &P = pts[w];
+                if (P.kind == GA_PT_RAY && write_rays) {
+                    raypath << P.pos0 << '\t' << P.pos1 << '\t' << P.z << '\t' << P.tl << '\t' << P.absorption
+                            << '\t' << P.time << '\n';
+                } else if (P.kind == GA_PT_CAUSTIC && write_caustics && P.leg >= 0 && P.leg < (int)caus.size()) {
+                    caus[P.leg] << P.pos0 << '\t' << P.pos1 << '\t' << P.z << '\t' << P.time << '\n';
+                }
+            }
+            if (write_rays) raypath << '\n';
+        }
+    }
+    return GA_OK;
+}
+
 int run_prop(int argc, char **argv)
 {
     const bool sph = GA_CLI_GEOM == GA_GEOM_SPH;
@@ -95,7 +167,7 @@ int run_prop(int argc, char **argv)
     double phi_min = -90.0, phi_max = -90.0, phi_step = 1.0;
     int bounces = 10;
     double s0 = sph ? 30.0 : 0.0, s1 = 0.0, z_src = 0.0;
-    bool write_rays = true, reverse_winds = false, custom_id = false;
+    bool write_rays = true, write_caustics = false, reverse_winds = false, custom_id = false;
     double freq = 0.1, turn_ht_min = 0.2;
     const char *prof_format = "zTuvdp";
     const char *output_id = nullptr;
@@ -144,8 +216,8 @@ int run_prop(int argc, char **argv)
         else if (key(a, "output_id=", v)) { custom_id = true; output_id = v; }
         else if (key(a, "z_grnd=", v)) z_grnd = std::atof(v);
         else if (key(a, "print_resid=", v)) { /* diagnostics only */ }
-        else if (key(a, "write_caustics=", v) || key(a, "write_topo=", v) || key(a, "topo_file=", v) ||
-                 key(a, "topo_use_BLw=", v)) {
+        else if (key(a, "write_caustics=", v)) write_caustics = str2bool(v);
+        else if (key(a, "write_topo=", v) || key(a, "topo_file=", v) || key(a, "topo_use_BLw=", v)) {
             if (std::strcmp(v, "false") != 0 && std::strcmp(v, "False") != 0 && std::strcmp(v, "0") != 0) {
                 std::cerr << GA_CLI_NAME << ": " << a << " is not supported by this engine\n";
                 return 2;
@@ -156,6 +228,10 @@ int run_prop(int argc, char **argv)
         }
     }
 
+    if (write_caustics && !p.calc_amp) {
+        std::cerr << GA_CLI_NAME << ": write_caustics=true needs calc_amp=true\n";
+        return 2;
+    }
     ga_ctx *ctx = nullptr;
     if (ga_ctx_create(&ctx, 0) != GA_OK) return die(nullptr, "no usable GPU");
     int aid = 0;
@@ -189,10 +265,14 @@ int run_prop(int argc, char **argv)
     std::vector<ga_arrival> arr((size_t)std::max(1, n) * (bounces + 1));
     int n_out = 0;
     ga_run_stats st;
-    if (ga_fan_run(ctx, aid, &p, launches.data(), n, arr.data(), (int)arr.size(), &n_out, &st) != GA_OK)
-        return die(ctx, "fan failed");
-
     std::string id = custom_id ? std::string(output_id) : strip_ext(argv[2]);
+    if (write_rays || write_caustics) {
+        if (fan_with_paths(ctx, aid, p, launches, write_rays, write_caustics, id, sph, arr, n_out, st) != GA_OK)
+            return die(ctx, "fan failed");
+    } else if (ga_fan_run(ctx, aid, &p, launches.data(), n, arr.data(), (int)arr.size(), &n_out, &st) != GA_OK) {
+        return die(ctx, "fan failed");
+    }
+
     std::ofstream results(id + ".arrivals.dat");
     results << "# " << GA_CLI_NAME << " -prop summary:" << '\n';
     results << "#" << '\t' << "profile: " << argv[2] << '\n';
@@ -233,8 +313,6 @@ int run_prop(int argc, char **argv)
         if (r + 1 == az_end[az_i]) { results << '\n'; ++az_i; }
     }
     results.close();
-    if (write_rays)
-        std::cout << GA_CLI_NAME << ": note: ray paths are not written by this engine (arrivals only)\n";
     std::cout << GA_CLI_NAME << ": " << n << " rays, " << n_out << " arrivals, " << st.ray_steps << " ray-steps, "
               << st.kernel_ms << " ms on the GPU\n";
     ga_ctx_destroy(ctx);
@@ -253,7 +331,7 @@ int run_prop_rd(int argc, char **argv)
     double phi_min = -90.0, phi_max = -90.0, phi_step = 1.0;
     int bounces = 2;
     double lat_src = NAN, lon_src = NAN, z_src = 0.0;
-    bool write_rays = true, reverse_winds = false, custom_id = false;
+    bool write_rays = true, write_caustics = false, reverse_winds = false, custom_id = false;
     double freq = 0.1, turn_ht_min = 0.2;
     const char *prof_format = "zTuvdp";
     const char *topo_file = nullptr;
@@ -300,7 +378,8 @@ int run_prop_rd(int argc, char **argv)
         else if (key(a, "topo_file=", v)) topo_file = v;
         else if (key(a, "topo_use_BLw=", v)) p.topo_use_blw = str2bool(v) ? 1 : 0;
         else if (key(a, "print_resid=", v)) { /* diagnostics only */ }
-        else if (key(a, "write_caustics=", v) || key(a, "write_topo=", v)) {
+        else if (key(a, "write_caustics=", v)) write_caustics = str2bool(v);
+        else if (key(a, "write_topo=", v)) {
             if (str2bool(v)) {
                 std::cerr << GA_CLI_NAME << ": " << a << " is not supported by this engine\n";
                 return 2;
@@ -311,6 +390,10 @@ int run_prop_rd(int argc, char **argv)
         }
     }
 
+    if (write_caustics && !p.calc_amp) {
+        std::cerr << GA_CLI_NAME << ": write_caustics=true needs calc_amp=true\n";
+        return 2;
+    }
     ga_ctx *ctx = nullptr;
     if (ga_ctx_create(&ctx, 0) != GA_OK) return die(nullptr, "no usable GPU");
     int rid = 0;
@@ -366,10 +449,14 @@ int run_prop_rd(int argc, char **argv)
     std::vector<ga_arrival> arr((size_t)std::max(1, n) * (bounces + 1));
     int n_out = 0;
     ga_run_stats st;
-    if (ga_fan_run(ctx, rid, &p, launches.data(), n, arr.data(), (int)arr.size(), &n_out, &st) != GA_OK)
-        return die(ctx, "fan failed");
-
     std::string id = custom_id ? std::string(output_id) : std::string(argv[2]);
+    if (write_rays || write_caustics) {
+        if (fan_with_paths(ctx, rid, p, launches, write_rays, write_caustics, id, true, arr, n_out, st) != GA_OK)
+            return die(ctx, "fan failed");
+    } else if (ga_fan_run(ctx, rid, &p, launches.data(), n, arr.data(), (int)arr.size(), &n_out, &st) != GA_OK) {
+        return die(ctx, "fan failed");
+    }
+
     std::ofstream results(id + ".arrivals.dat");
     results << "# " << GA_CLI_NAME << " -prop summary:" << '\n';
     results << "#" << '\t' << "profile: " << argv[2] << '\n';
@@ -396,8 +483,6 @@ int run_prop_rd(int argc, char **argv)
         if (r + 1 == az_end[az_i]) { results << '\n'; ++az_i; }
     }
     results.close();
-    if (write_rays)
-        std::cout << GA_CLI_NAME << ": note: ray paths are not written by this engine (arrivals only)\n";
     std::cout << GA_CLI_NAME << ": " << n << " rays, " << n_out << " arrivals, " << st.ray_steps << " ray-steps, "
               << st.kernel_ms << " ms on the GPU\n";
     ga_ctx_destroy(ctx);
@@ -503,17 +588,41 @@ int run_eig_search(int argc, char **argv)
     p.src[2] = z_src;
     const double rcvr[2] = {r0, r1};
     std::vector<ga_eigenray> eig(64);
+    std::vector<ga_path_pt> pts((size_t)1 << 16);
     int n_out = 0;
+    int64_t n_pts = 0;
     double lat_ms = 0.0;
     ga_run_stats st;
-    int rc = ga_eig_search(ctx, aid, &p, &e, rcvr, 1, eig.data(), (int)eig.size(), &n_out, &lat_ms, &st);
+    int rc = ga_eig_search_paths(ctx, aid, &p, &e, rcvr, 1, eig.data(), (int)eig.size(), &n_out, &lat_ms,
+                                 pts.data(), (int64_t)pts.size(), &n_pts, &st);
     if (rc == GA_E_CAP) {
-        eig.resize(n_out);
-        rc = ga_eig_search(ctx, aid, &p, &e, rcvr, 1, eig.data(), (int)eig.size(), &n_out, &lat_ms, &st);
+        eig.resize(std::max<size_t>(eig.size(), (size_t)n_out));
+        pts.resize(std::max<size_t>(pts.size(), (size_t)n_pts));
+        rc = ga_eig_search_paths(ctx, aid, &p, &e, rcvr, 1, eig.data(), (int)eig.size(), &n_out, &lat_ms,
+                                 pts.data(), (int64_t)pts.size(), &n_pts, &st);
     }
     if (rc != GA_OK) return die(ctx, "eigenray search failed");
 
     std::string id = custom_id ? std::string(output_id) : strip_ext(argv[2]);
+    // <id>.eigenray-N.dat, N counting the eigenrays in output order
+    // (geoac.eigenray.3d.cpp:219-276, geoac.eigenray.sph.cpp:227-271): sph
+    // rows at precision 8; 3D rows at the stream default, the absorption
+    // column after the first reflection printed as the bare integral
+    // (3d.cpp:263, without the -2 factor of the first leg).
+    {
+        int64_t w = 0;
+        for (int i = 0; i < n_out; ++i) {
+            std::ofstream ray(id + ".eigenray-" + std::to_string(i) + ".dat");
+            ray << (sph ? "# lat [deg]" : "# x [km]") << '\t' << (sph ? "lon [deg]" : "y [km]") << '\t' << "z [km]"
+                << '\t' << "trans. coeff. [dB]" << '\t' << "absorption [dB]" << '\t' << "time [s]" << '\n';
+            if (sph) ray << std::setprecision(8);
+            for (; w < n_pts && pts[w].launch_idx == i; ++w) {
+                const ga_path_pt &P = pts[w];
+                const double ab = (!sph && P.leg > 0) ? P.absorption * -0.5 : P.absorption;
+                ray << P.pos0 << '\t' << P.pos1 << '\t' << P.z << '\t' << P.tl << '\t' << ab << '\t' << P.time << '\n';
+            }
+        }
+    }
     std::ofstream out(id + ".arrivals.dat");
     // header (main.infraga.3d.cpp:897-930, main.infraga.sph.cpp:931-960)
     out << "# " << GA_CLI_NAME << " -eig_search summary:" << '\n';

@@ -26,7 +26,11 @@ EXPORTED = ["ga_prop_params_default", "ga_eig_params_default", "ga_ctx_create", 
             "ga_last_error", "ga_atmo_load_strat", "ga_atmo_load_file", "ga_atmo_release", "ga_fan_run",
             "ga_fan_run_device", "ga_eig_search", "ga_server_create", "ga_server_submit", "ga_server_poll",
             "ga_server_stats", "ga_server_destroy", "ga_rngdep_load", "ga_topo_load", "ga_rngdep_eval",
-            "ga_rngdep_release", "ga_rngdep_load_files", "ga_rngdep_info"]
+            "ga_rngdep_release", "ga_rngdep_load_files", "ga_rngdep_info", "ga_fan_run_paths",
+            "ga_eig_search_paths"]
+
+GA_PATH_RAYS, GA_PATH_CAUSTICS = 1, 2
+GA_PT_RAY, GA_PT_CAUSTIC = 0, 1
 
 _dp = C.POINTER(C.c_double)
 
@@ -52,6 +56,15 @@ class Arrival(C.Structure):
                 ("pos0", C.c_double), ("pos1", C.c_double), ("time", C.c_double), ("cel", C.c_double),
                 ("turn_ht", C.c_double), ("incl_arr", C.c_double), ("back_az", C.c_double),
                 ("tl", C.c_double), ("absorption", C.c_double)]
+
+
+class PathPt(C.Structure):
+    _fields_ = [("launch_idx", C.c_int32), ("leg", C.c_int32), ("step", C.c_int32), ("kind", C.c_int32),
+                ("pos0", C.c_double), ("pos1", C.c_double), ("z", C.c_double), ("tl", C.c_double),
+                ("absorption", C.c_double), ("time", C.c_double)]
+
+
+PATH_DTYPE = np.dtype([(f, "i4" if t is C.c_int32 else "f8") for f, t in PathPt._fields_])
 
 
 class Request(C.Structure):
@@ -108,6 +121,12 @@ def load_library(path: str | None = None):
                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     L.ga_eig_search.argtypes = [C.c_void_p, C.c_int, C.POINTER(PropParams), C.POINTER(EigParams), _dp, C.c_int,
                                 C.POINTER(Arrival), C.c_int, C.POINTER(C.c_int), _dp, C.POINTER(RunStats)]
+    L.ga_fan_run_paths.argtypes = [C.c_void_p, C.c_int, C.POINTER(PropParams), _dp, C.c_int, C.c_int,
+                                   C.POINTER(Arrival), C.c_int, C.POINTER(C.c_int), C.POINTER(PathPt), C.c_int64,
+                                   C.POINTER(C.c_int64), C.POINTER(RunStats)]
+    L.ga_eig_search_paths.argtypes = [C.c_void_p, C.c_int, C.POINTER(PropParams), C.POINTER(EigParams), _dp,
+                                      C.c_int, C.POINTER(Arrival), C.c_int, C.POINTER(C.c_int), _dp,
+                                      C.POINTER(PathPt), C.c_int64, C.POINTER(C.c_int64), C.POINTER(RunStats)]
     L.ga_rngdep_load.argtypes = [C.c_void_p, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, _dp, _dp, _dp,
                                  C.POINTER(C.c_int)]
     L.ga_topo_load.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp]
@@ -205,6 +224,30 @@ class Engine:
                                       out.ctypes.data_as(C.POINTER(Arrival)), cap, C.byref(n_out), C.byref(st)))
         return out[: n_out.value].copy(), st
 
+    def fan_paths(self, aid, params: PropParams, launches, flags=GA_PATH_RAYS, pts_cap=1 << 20):
+        """Trace a fan with the write_rays / write_caustics bookkeeping;
+        returns (arrivals, path rows, RunStats).  A too-small pts_cap is
+        raised to the count the call reports and the fan traced again."""
+        Lc = np.ascontiguousarray(launches, dtype=np.float64).reshape(-1, 2)
+        n = Lc.shape[0]
+        cap = max(1, n * (params.bounces + 1))
+        out = np.zeros(cap, dtype=ARRIVAL_DTYPE)
+        n_out = C.c_int(0)
+        n_pts = C.c_int64(0)
+        st = RunStats()
+        while True:
+            pts = np.zeros(max(1, pts_cap), dtype=PATH_DTYPE)
+            rc = self.L.ga_fan_run_paths(self.h, aid, C.byref(params), Lc.ctypes.data_as(_dp), n, int(flags),
+                                         out.ctypes.data_as(C.POINTER(Arrival)), cap, C.byref(n_out),
+                                         pts.ctypes.data_as(C.POINTER(PathPt)), pts_cap, C.byref(n_pts),
+                                         C.byref(st))
+            if rc == GA_E_CAP and n_pts.value > pts_cap:
+                pts_cap = n_pts.value
+                continue
+            break
+        self._check(rc)
+        return out[: n_out.value].copy(), pts[: n_pts.value].copy(), st
+
     def fan_device(self, aid, params: PropParams, d_launches: int, n: int, d_out: int, d_counts: int,
                    d_steps: int, stream: int = 0):
         """Enqueue a fan on device-resident buffers (raw device pointers)."""
@@ -273,6 +316,32 @@ class Engine:
                                       lat.ctypes.data_as(_dp), C.byref(st))
         self._check(rc)
         return out[: n_out.value].copy(), lat, st
+
+
+    def eig_search_paths(self, aid, params: PropParams, eig: EigParams, rcvr_xy, pts_cap=1 << 18):
+        """eig_search plus the eigenray-N.dat rows of every eigenray found
+        (launch_idx of a row = index of its eigenray in the result)."""
+        R = np.ascontiguousarray(rcvr_xy, dtype=np.float64).reshape(-1, 2)
+        n = R.shape[0]
+        cap = max(16, 8 * n * (eig.bnc_max - eig.bnc_min + 1))
+        lat = np.zeros(n)
+        n_out = C.c_int(0)
+        n_pts = C.c_int64(0)
+        st = RunStats()
+        while True:
+            out = np.zeros(cap, dtype=ARRIVAL_DTYPE)
+            pts = np.zeros(max(1, pts_cap), dtype=PATH_DTYPE)
+            rc = self.L.ga_eig_search_paths(self.h, aid, C.byref(params), C.byref(eig), R.ctypes.data_as(_dp), n,
+                                            out.ctypes.data_as(C.POINTER(Arrival)), cap, C.byref(n_out),
+                                            lat.ctypes.data_as(_dp), pts.ctypes.data_as(C.POINTER(PathPt)),
+                                            pts_cap, C.byref(n_pts), C.byref(st))
+            if rc == GA_E_CAP and (n_out.value > cap or n_pts.value > pts_cap):
+                cap = max(cap, n_out.value)
+                pts_cap = max(pts_cap, n_pts.value)
+                continue
+            break
+        self._check(rc)
+        return out[: n_out.value].copy(), pts[: n_pts.value].copy(), lat, st
 
 
 class Server:

@@ -1148,6 +1148,137 @@ int go_fan(go_atmo *a, const go_params *p, const go_prop *pr,
     return cnt > out_cap ? -1 : 0;
 }
 
+/* ======================================================================
+ * -prop fan with write_rays / write_caustics: main.infraga.sph.cpp:450-496
+ * (3D: main.infraga.3d.cpp:442-487).  Everything else as go_fan.
+ * ==================================================================== */
+typedef struct {
+    go_path_pt *pts;
+    long long cap, n;
+} path_sink;
+
+static void sink_pt(path_sink *k, int li, int leg, int m, int kind, const go_atmo *a, const double *y,
+                    double tl, double att, double tt)
+{
+    if (!k) return;
+    if (k->n < k->cap) {
+        go_path_pt *P = k->pts + k->n;
+        P->launch_idx = li;
+        P->leg = leg;
+        P->step = m;
+        P->kind = kind;
+        if (a->geom == GA_GEOM_SPH) {
+            P->pos0 = y[1] * (180.0 / GO_PI);
+            P->pos1 = y[2] * (180.0 / GO_PI);
+            P->z = y[0] - a->r0;
+        } else {
+            P->pos0 = y[0];
+            P->pos1 = y[1];
+            P->z = y[2] > ground(a) ? y[2] : ground(a);     /* max(z, topo::z(x, y)) */
+        }
+        P->tl = tl;
+        P->absorption = -2.0 * att;
+        P->time = tt;
+    }
+    k->n++;
+}
+
+int go_fan_paths(go_atmo *a, const go_params *p, const go_prop *pr,
+                 const double *launches, int n_launch, int flags,
+                 go_arrival *out, int out_cap, int *n_out,
+                 go_path_pt *pts, long long pts_cap, long long *n_pts, long long *ray_steps)
+{
+    const int sph = (a->geom == GA_GEOM_SPH);
+    const int rays = (flags & GO_PATH_RAYS) != 0, caus = (flags & GO_PATH_CAUSTICS) != 0 && p->calc_amp;
+    int length = (int)(p->s_max / p->ds_min);
+    double *rows = get_rows(length);
+    go_src s;
+    int cnt = 0;
+    long long steps = 0;
+    double lat_src = 0, lon_src = 0, z_src;
+    path_sink sink = {pts, pts_cap, 0};
+
+    if (sph) {
+        lat_src = pr->src0 * (GO_PI / 180.0);
+        lon_src = pr->src1 * (GO_PI / 180.0);
+        z_src = pr->src_alt;
+        double zg = ground(a) - a->r0;
+        z_src = z_src > zg ? z_src : zg;
+        s.src[0] = a->r0 + z_src; s.src[1] = lat_src; s.src[2] = lon_src;
+    } else {
+        z_src = pr->src_alt > ground(a) ? pr->src_alt : ground(a);
+        s.src[0] = pr->src0; s.src[1] = pr->src1; s.src[2] = z_src;
+    }
+
+    for (int li = 0; li < n_launch; li++) {
+        double incl = launches[2 * li], az = launches[2 * li + 1];
+        s.phi = GO_PI / 2.0 - az * (GO_PI / 180.0);
+        s.theta = incl * (GO_PI / 180.0);
+        go_set_initial(a, p, &s, rows);
+        double tt = 0.0, att = 0.0;
+        double rmax = sph ? 0.0 : a->z0;
+        int brk = 0, k;
+        for (int bnc = 0; bnc <= pr->bounces; bnc++) {
+            k = go_prop_rk4(a, p, &s, rows, length, &brk);
+            steps += k;
+            /* sph :456-496 / 3d :442-487: segments of rows 1..k-1 one by one */
+            double D = 0.0, D_prev = caus ? go_jacobian(a, rows + 18) : 0.0;
+            for (int m = 1; m < k; m++) {
+                const double *ym = rows + 18 * m;
+                if (caus) D = go_jacobian(a, ym);
+                tt += go_travel_time_seg(a, rows + 18 * (m - 1), ym);
+                att += go_atten_seg(a, rows + 18 * (m - 1), ym, pr->freq);
+                if (rays && (m == 1 || m % 15 == 0))
+                    sink_pt(&sink, li, bnc, m, 0, a, ym, p->calc_amp ? 20.0 * log10(go_amp(a, &s, ym)) : 0.0, att, tt);
+                if (caus && D * D_prev < 0.0) sink_pt(&sink, li, bnc, m, 1, a, ym, 0.0, att, tt);
+                if (caus) D_prev = D;
+            }
+            for (int m = 0; m < k; m++) {
+                double zz = sph ? rows[18 * m] - a->r0 : rows[18 * m + 2];
+                rmax = rmax > zz ? rmax : zz;
+            }
+            if (rmax < (a->z0 + pr->turn_ht_min)) break;
+            if (brk || k < 2) break;
+            if (sph && (tt / 3600.0) > p->time_max) break;
+            double *yk = rows + 18 * k;
+            go_arrival A;
+            memset(&A, 0, sizeof(A));
+            A.launch_idx = li;
+            A.n_bounce = bnc;
+            A.incl_launch = incl;
+            A.az_launch = az;
+            double baz;
+            if (sph) {
+                A.incl_arr = -asin(go_c(a, yk[0], yk[1], yk[2]) / go_c(a, a->r0 + z_src, lat_src, lon_src) * yk[3]) * (180.0 / GO_PI);
+                baz = 90.0 - atan2(-yk[4], -yk[5]) * (180.0 / GO_PI);
+                A.pos0 = yk[1] * (180.0 / GO_PI);
+                A.pos1 = yk[2] * (180.0 / GO_PI);
+                A.cel = go_gc_dist(a->r0, yk[1], yk[2], lat_src, lon_src) / tt;
+            } else {
+                A.incl_arr = -asin(go_c(a, yk[0], yk[1], ground(a)) / go_c(a, s.src[0], s.src[1], s.src[2]) * yk[5]) * 180.0 / GO_PI;
+                baz = 90.0 - atan2(-yk[4], -yk[3]) * 180.0 / GO_PI;
+                A.pos0 = yk[0];
+                A.pos1 = yk[1];
+                A.cel = sqrt(sq(yk[0] - s.src[0]) + sq(yk[1] - s.src[1])) / tt;
+            }
+            if (baz < -180.0) baz += 360.0;
+            else if (baz > 180.0) baz -= 360.0;
+            A.back_az = baz;
+            A.time = tt;
+            A.turn_ht = rmax;
+            A.tl = p->calc_amp ? 20.0 * log10(go_amp(a, &s, yk)) : 0.0;
+            A.absorption = -2.0 * att;
+            if (cnt < out_cap) out[cnt] = A;
+            cnt++;
+            go_set_refl(a, p, &s, rows + 18 * (k - 2), rows + 18 * (k - 1), rows);
+        }
+    }
+    *n_out = cnt;
+    if (n_pts) *n_pts = sink.n;
+    if (ray_steps) *ray_steps = steps;
+    return (cnt > out_cap || sink.n > pts_cap) ? -1 : 0;
+}
+
 #ifndef GO_OPCOUNT
 /* ======================================================================
  * Eigenray search, 3D: geoac.eigenray.3d.cpp:34-366 and the driver loop of
@@ -1245,7 +1376,7 @@ static int est_eigenray(eig_ctx *E, const double src[3], const double rcvr[2], d
  * segments 1..k-1 of each leg over bnc_cnt reflections, the row taken at
  * row k of the last leg.  Returns the RK4 steps taken. */
 static long long eig_accept_3d(go_atmo *a, const go_params *p, go_src *s, const double src[3], int bnc_cnt,
-                               double freq, double *rows, int length, go_arrival *A)
+                               double freq, double *rows, int length, go_arrival *A, path_sink *ps)
 {
     double att = 0.0, tt = 0.0, zmax = 0.0;
     int brk, k;
@@ -1257,6 +1388,8 @@ static long long eig_accept_3d(go_atmo *a, const go_params *p, go_src *s, const 
         tt += go_travel_time_seg(a, rows + 18 * (m - 1), rows + 18 * m);
         att += go_atten_seg(a, rows + 18 * (m - 1), rows + 18 * m, freq);
         zmax = zmax > rows[18 * m + 2] ? zmax : rows[18 * m + 2];
+        if (ps && (m == 1 || m % 15 == 0))     /* 3d.cpp:240-247 */
+            sink_pt(ps, 0, 0, m, 0, a, rows + 18 * m, 20.0 * log10(go_amp(a, s, rows + 18 * m)), att, tt);
     }
     for (int b = 1; b <= bnc_cnt; b++) {
         if (k < 2) break;   /* row k-2 < 0: no further reflection */
@@ -1267,6 +1400,8 @@ static long long eig_accept_3d(go_atmo *a, const go_params *p, go_src *s, const 
             tt += go_travel_time_seg(a, rows + 18 * (m - 1), rows + 18 * m);
             att += go_atten_seg(a, rows + 18 * (m - 1), rows + 18 * m, freq);
             zmax = zmax > rows[18 * m + 2] ? zmax : rows[18 * m + 2];
+            if (ps && (m == 1 || m % 15 == 0))     /* 3d.cpp:258-265 (z printed without the ground floor) */
+                sink_pt(ps, 0, b, m, 0, a, rows + 18 * m, 20.0 * log10(go_amp(a, s, rows + 18 * m)), att, tt);
         }
     }
     const double *yk = rows + 18 * k;
@@ -1301,7 +1436,7 @@ int go_eig_row_3d(go_atmo *a, const go_params *p, const double src[3], double in
     s.src[0] = src[0]; s.src[1] = src[1]; s.src[2] = src[2];
     s.theta = incl_deg * (GO_PI / 180.0);
     s.phi = (90.0 - az_deg) * (GO_PI / 180.0);
-    eig_accept_3d(a, p, &s, src, bnc_cnt, freq, rows, length, A);
+    eig_accept_3d(a, p, &s, src, bnc_cnt, freq, rows, length, A, NULL);
     return 0;
 }
 
@@ -1344,7 +1479,7 @@ static int find_eigenray(eig_ctx *E, const double src[3], const double rcvr[2], 
             y = yk[1]; dy = rcvr[1] - y;
             dr = (double)sqrtl(powl(dx, 2) + powl(dy, 2));
             if (dr < E->tolerance) {
-                E->steps += eig_accept_3d(a, p, &s, src, bnc_cnt, freq, rows, length, A);
+                E->steps += eig_accept_3d(a, p, &s, src, bnc_cnt, freq, rows, length, A, NULL);
                 eig_cnt[GO_EIG_FINAL_TRACES]++;
                 return 1;
             } else if (n > 0 && dr > dr_prev) {
@@ -1474,6 +1609,84 @@ static int est_eigenray_sph(eig_ctx *E, const double src[3], const double rcvr[2
     return 0;
 }
 
+/* The accepted spherical eigenray's row: re-trace with per-step bookkeeping
+ * over rows 1..k-1 of every leg (geoac.eigenray.sph.cpp:219-271), path rows
+ * m == 1 or m % 15 == 0 into ps when given.  src = (alt [km], lat, lon
+ * [rad]).  Returns the RK4 steps taken. */
+static long long eig_accept_sph(go_atmo *a, const go_params *p, go_src *s, const double src[3], int bnc_cnt,
+                                double freq, double *rows, int length, go_arrival *A, path_sink *ps)
+{
+    const double r0 = a->r0;
+    double att = 0.0, tt = 0.0, rmax = 0.0;
+    int brk, k;
+    long long steps = 0;
+    go_set_initial(a, p, s, rows);
+    k = go_prop_rk4(a, p, s, rows, length, &brk);
+    steps += k;
+    for (int b = 0; b <= bnc_cnt; b++) {
+        if (b > 0) {
+            if (k < 2) break;   /* row k-2 < 0: no further reflection */
+            go_set_refl(a, p, s, rows + 18 * (k - 2), rows + 18 * (k - 1), rows);
+            k = go_prop_rk4(a, p, s, rows, length, &brk);
+            steps += k;
+        }
+        for (int m = 1; m < k; m++) {
+            tt += go_travel_time_seg(a, rows + 18 * (m - 1), rows + 18 * m);
+            att += go_atten_seg(a, rows + 18 * (m - 1), rows + 18 * m, freq);
+            rmax = rmax > rows[18 * m] - r0 ? rmax : rows[18 * m] - r0;
+            if (ps && (m == 1 || m % 15 == 0))     /* sph.cpp:250-257, 266-273 */
+                sink_pt(ps, 0, b, m, 0, a, rows + 18 * m, 20.0 * log10(go_amp(a, s, rows + 18 * m)), att, tt);
+        }
+    }
+    const double *yk = rows + 18 * k;
+    memset(A, 0, sizeof(*A));
+    A->n_bounce = bnc_cnt;
+    A->incl_launch = s->theta * (180.0 / GO_PI);
+    A->az_launch = 90.0 - s->phi * (180.0 / GO_PI);
+    A->pos0 = yk[1] * 180.0 / GO_PI;
+    A->pos1 = yk[2] * 180.0 / GO_PI;
+    A->time = tt;
+    A->cel = go_gc_dist(r0, yk[1], yk[2], src[1], src[2]) / tt;
+    A->turn_ht = rmax;
+    /* atmo::c(src[0], ...) takes the source altitude as a radius:
+     * the profile's bottom value (sph.cpp:273) */
+    A->incl_arr = -asin(go_c(a, yk[0], yk[1], yk[2]) / go_c(a, src[0], src[1], src[2]) * yk[3]) * 180.0 / GO_PI;
+    double baz = 90.0 - atan2(-yk[4], -yk[5]) * (180.0 / GO_PI);
+    while (baz < -180.0) baz += 360.0;
+    while (baz > 180.0) baz -= 360.0;
+    A->back_az = baz;
+    A->tl = 20.0 * log10(go_amp(a, s, yk));
+    A->absorption = -2.0 * att;
+    return steps;
+}
+
+long long go_eig_path(go_atmo *a, const go_params *p, const double src_in[3], double incl_deg, double az_deg,
+                      int bnc_cnt, double freq, go_path_pt *pts, long long cap, go_arrival *A)
+{
+    go_params p18 = *p;
+    p18.calc_amp = 1;
+    p18.eq_cnt = 18;
+    int length = (int)(p18.s_max / p18.ds_min);
+    double *rows = get_rows(length);
+    path_sink sink = {pts, cap, 0};
+    go_src s;
+    s.theta = incl_deg * (GO_PI / 180.0);
+    s.phi = (90.0 - az_deg) * (GO_PI / 180.0);
+    if (a->geom == GA_GEOM_SPH) {
+        double src[3];
+        src[1] = src_in[0] * (GO_PI / 180.0);
+        src[2] = src_in[1] * (GO_PI / 180.0);
+        src[0] = src_in[2] > ground(a) - a->r0 ? src_in[2] : ground(a) - a->r0;
+        s.src[0] = a->r0 + src[0]; s.src[1] = src[1]; s.src[2] = src[2];
+        eig_accept_sph(a, &p18, &s, src, bnc_cnt, freq, rows, length, A, &sink);
+    } else {
+        double src[3] = {src_in[0], src_in[1], src_in[2] > ground(a) ? src_in[2] : ground(a)};
+        s.src[0] = src[0]; s.src[1] = src[1]; s.src[2] = src[2];
+        eig_accept_3d(a, &p18, &s, src, bnc_cnt, freq, rows, length, A, &sink);
+    }
+    return sink.n;
+}
+
 static int find_eigenray_sph(eig_ctx *E, const double src[3], const double rcvr[2], double th_est, double ph_est,
                              double freq, int bnc_cnt, int iterate_limit, go_arrival *A)
 {
@@ -1499,47 +1712,8 @@ static int find_eigenray_sph(eig_ctx *E, const double src[3], const double rcvr[
             double *yk = rows + 18 * k;
             dr = go_gc_dist(r0, yk[1], yk[2], rcvr[0], rcvr[1]);
             if (dr < E->tolerance) {
-                /* accept: re-trace with per-step bookkeeping (sph.cpp:219-271) */
-                double att = 0.0, tt = 0.0, rmax = 0.0;
-                go_set_initial(a, p, &s, rows);
-                k = go_prop_rk4(a, p, &s, rows, length, &brk);
-                E->steps += k;
+                E->steps += eig_accept_sph(a, p, &s, src, bnc_cnt, freq, rows, length, A, NULL);
                 eig_cnt[GO_EIG_FINAL_TRACES]++;
-                for (int m = 1; m < k; m++) {
-                    tt += go_travel_time_seg(a, rows + 18 * (m - 1), rows + 18 * m);
-                    att += go_atten_seg(a, rows + 18 * (m - 1), rows + 18 * m, freq);
-                    rmax = rmax > rows[18 * m] - r0 ? rmax : rows[18 * m] - r0;
-                }
-                for (int b = 1; b <= bnc_cnt; b++) {
-                    if (k < 2) break;   /* row k-2 < 0: no further reflection */
-                    go_set_refl(a, p, &s, rows + 18 * (k - 2), rows + 18 * (k - 1), rows);
-                    k = go_prop_rk4(a, p, &s, rows, length, &brk);
-                    E->steps += k;
-                    for (int m = 1; m < k; m++) {
-                        tt += go_travel_time_seg(a, rows + 18 * (m - 1), rows + 18 * m);
-                        att += go_atten_seg(a, rows + 18 * (m - 1), rows + 18 * m, freq);
-                        rmax = rmax > rows[18 * m] - r0 ? rmax : rows[18 * m] - r0;
-                    }
-                }
-                yk = rows + 18 * k;
-                memset(A, 0, sizeof(*A));
-                A->n_bounce = bnc_cnt;
-                A->incl_launch = s.theta * (180.0 / GO_PI);
-                A->az_launch = 90.0 - s.phi * (180.0 / GO_PI);
-                A->pos0 = yk[1] * 180.0 / GO_PI;
-                A->pos1 = yk[2] * 180.0 / GO_PI;
-                A->time = tt;
-                A->cel = go_gc_dist(r0, yk[1], yk[2], src[1], src[2]) / tt;
-                A->turn_ht = rmax;
-                /* atmo::c(src[0], ...) takes the source altitude as a radius:
-                 * the profile's bottom value (sph.cpp:273) */
-                A->incl_arr = -asin(go_c(a, yk[0], yk[1], yk[2]) / go_c(a, src[0], src[1], src[2]) * yk[3]) * 180.0 / GO_PI;
-                double baz = 90.0 - atan2(-yk[4], -yk[5]) * (180.0 / GO_PI);
-                while (baz < -180.0) baz += 360.0;
-                while (baz > 180.0) baz -= 360.0;
-                A->back_az = baz;
-                A->tl = 20.0 * log10(go_amp(a, &s, yk));
-                A->absorption = -2.0 * att;
                 return 1;
             } else if (n > 0 && dr > dr_prev) {
                 s.theta -= dth * step_sc; s.phi -= dph * step_sc; step_sc /= 2.0;
@@ -1652,6 +1826,13 @@ void go_h_params(void *h, int calc_amp, go_params *p)
     go_atmo *a = (go_atmo *)h;
     go_params_default(p, a->geom, a, calc_amp);
 }
+int go_h_fan_paths(void *h, const go_params *p, const go_prop *pr, const double *launches, int n, int flags,
+                   go_arrival *out, int cap, int *n_out, go_path_pt *pts, long long pts_cap, long long *n_pts,
+                   long long *steps)
+{
+    return go_fan_paths((go_atmo *)h, p, pr, launches, n, flags, out, cap, n_out, pts, pts_cap, n_pts, steps);
+}
+
 int go_h_fan(void *h, const go_params *p, const go_prop *pr, const double *launches, int n,
              go_arrival *out, int cap, int *n_out, long long *steps)
 {

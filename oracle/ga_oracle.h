@@ -88,6 +88,22 @@ typedef struct {
     double src0, src1, src_alt;      /* sph: lat, lon [deg], alt [km]; 3D: x, y, z [km] */
 } go_prop;
 
+/* One sampled ray-path row (kind 0: <id>.raypaths.dat, <id>.eigenray-N.dat)
+ * or caustic point (kind 1: <id>.caustics-N.dat). */
+typedef struct {
+    int launch_idx;
+    int leg;                         /* reflections before the row */
+    int step;                        /* row index m within the leg */
+    int kind;
+    double pos0, pos1;               /* sph: lat, lon [deg]; 3D: x, y [km] */
+    double z;                        /* sph: r - r0; 3D: max(z, ground) [km] */
+    double tl;                       /* 20 log10(amp) [dB]; 0 without amplitudes or for caustics */
+    double absorption;               /* -2 x absorption integral so far [dB] */
+    double time;                     /* s */
+} go_path_pt;
+
+enum { GO_PATH_RAYS = 1, GO_PATH_CAUSTICS = 2 };
+
 /* ---- atmosphere set-up ------------------------------------------------ */
 int    go_atmo_init(go_atmo *a, int geom, const double *z, const double *c,
                     const double *u, const double *v, const double *rho, int n);
@@ -150,6 +166,31 @@ int    go_fan(go_atmo *a, const go_params *p, const go_prop *pr,
               const double *launches, int n_launch,
               go_arrival *out, int out_cap, int *n_out, long long *ray_steps);
 
+/* -prop fan with write_rays / write_caustics bookkeeping
+ * (main.infraga.sph.cpp:450-496, main.infraga.3d.cpp:442-487): per leg the
+ * segments of rows 1..k-1 are added one by one to the running travel time
+ * and absorption (the segment into the sub-ground row k is not), rows
+ * m == 1 or m % 15 == 0 are sampled (flags & GO_PATH_RAYS), and a sign
+ * change of the Jacobian determinant between rows m-1 and m is a caustic
+ * (flags & GO_PATH_CAUSTICS, needs calc_amp).  Arrivals as go_fan otherwise.
+ * Returns 0, or -1 when out_cap or pts_cap was too small (counts still
+ * complete). */
+int    go_fan_paths(go_atmo *a, const go_params *p, const go_prop *pr,
+                    const double *launches, int n_launch, int flags,
+                    go_arrival *out, int out_cap, int *n_out,
+                    go_path_pt *pts, long long pts_cap, long long *n_pts, long long *ray_steps);
+
+/* The accepted eigenray's path file <id>.eigenray-N.dat
+ * (geoac.eigenray.3d.cpp:223-276, geoac.eigenray.sph.cpp:228-271) and its
+ * arrival row, for launch angles in degrees.  src as for the searches:
+ * 3D (x, y, z) [km]; sph (lat [deg], lon [deg], alt [km]).  Rows m == 1 or
+ * m % 15 == 0 of every leg; absorption is stored as -2 x the integral in
+ * every leg (the 3D file prints the bare integral after the first
+ * reflection: a text-format matter of the writer).  Returns the number of
+ * rows (all counted, at most cap stored). */
+long long go_eig_path(go_atmo *a, const go_params *p, const double src[3], double incl_deg, double az_deg,
+                      int bnc_cnt, double freq, go_path_pt *pts, long long cap, go_arrival *A);
+
 /* Eigenray search for one receiver, 3D geometry (main.infraga.3d.cpp:944-951,
  * geoac.eigenray.3d.cpp).  Returns the number of eigenrays written to out. */
 /* The row find_eigenray writes for an accepted 3D eigenray with these launch
@@ -187,6 +228,9 @@ void   go_h_set_tweak(void *h, double t);
 void   go_h_set_r0(void *h, double r0);
 void   go_h_tables(void *h, int which, double *x, double *f, double *slope);
 void   go_h_params(void *h, int calc_amp, go_params *p);
+int    go_h_fan_paths(void *h, const go_params *p, const go_prop *pr, const double *launches, int n, int flags,
+                      go_arrival *out, int cap, int *n_out, go_path_pt *pts, long long pts_cap, long long *n_pts,
+                      long long *steps);
 int    go_h_fan(void *h, const go_params *p, const go_prop *pr, const double *launches, int n,
                 go_arrival *out, int cap, int *n_out, long long *steps);
 

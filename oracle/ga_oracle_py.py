@@ -50,6 +50,23 @@ class Arrival(C.Structure):
 ARRIVAL_FIELDS = [f for f, _ in Arrival._fields_]
 
 
+class PathPt(C.Structure):
+    _fields_ = [("launch_idx", C.c_int), ("leg", C.c_int), ("step", C.c_int), ("kind", C.c_int),
+                ("pos0", C.c_double), ("pos1", C.c_double), ("z", C.c_double),
+                ("tl", C.c_double), ("absorption", C.c_double), ("time", C.c_double)]
+
+
+PATH_DTYPE = np.dtype([("launch_idx", "i4"), ("leg", "i4"), ("step", "i4"), ("kind", "i4"),
+                       ("pos0", "f8"), ("pos1", "f8"), ("z", "f8"), ("tl", "f8"), ("absorption", "f8"),
+                       ("time", "f8")])
+PATH_RAYS, PATH_CAUSTICS = 1, 2
+
+
+def paths_to_array(buf, n):
+    """Structured numpy array from a ctypes PathPt buffer."""
+    return np.frombuffer(C.string_at(C.addressof(buf), C.sizeof(PathPt) * n), dtype=PATH_DTYPE).copy()
+
+
 def arrivals_to_array(arr, n):
     """Structured numpy array from a ctypes Arrival buffer."""
     dt = np.dtype([("launch_idx", "i4"), ("n_bounce", "i4")] + [(f, "f8") for f in ARRIVAL_FIELDS[2:]])
@@ -74,6 +91,13 @@ class Oracle:
         L.go_h_fan.argtypes = [C.c_void_p, C.POINTER(Params), C.POINTER(Prop), _dp, C.c_int,
                                C.POINTER(Arrival), C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_longlong)]
         L.go_h_fan.restype = C.c_int
+        L.go_h_fan_paths.argtypes = [C.c_void_p, C.POINTER(Params), C.POINTER(Prop), _dp, C.c_int, C.c_int,
+                                     C.POINTER(Arrival), C.c_int, C.POINTER(C.c_int), C.POINTER(PathPt),
+                                     C.c_longlong, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
+        L.go_h_fan_paths.restype = C.c_int
+        L.go_eig_path.argtypes = [C.c_void_p, C.POINTER(Params), _dp, C.c_double, C.c_double, C.c_int,
+                                  C.c_double, C.POINTER(PathPt), C.c_longlong, C.POINTER(Arrival)]
+        L.go_eig_path.restype = C.c_longlong
         for name in ("go_c", "go_rho", "go_u", "go_v"):
             getattr(L, name).argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
             getattr(L, name).restype = C.c_double
@@ -163,6 +187,40 @@ class Medium:
         if rc != 0:
             raise RuntimeError("oracle fan overflow")
         return arrivals_to_array(out, n_out.value), steps.value
+
+    def fan_paths(self, launches, params, flags=PATH_RAYS, bounces=2, freq=0.1, turn_ht_min=0.2,
+                  src=(30.0, 0.0, 0.0)):
+        """-prop with write_rays / write_caustics bookkeeping (go_fan_paths):
+        (arrivals, path rows, ray-steps)."""
+        L = np.ascontiguousarray(launches, dtype=np.float64).reshape(-1, 2)
+        pr = Prop(bounces, freq, turn_ht_min, src[0], src[1], src[2])
+        cap = max(1, L.shape[0] * (bounces + 1))
+        out = (Arrival * cap)()
+        n_out = C.c_int(0)
+        steps = C.c_longlong(0)
+        n_pts = C.c_longlong(0)
+        pcap = 1
+        while True:
+            pts = (PathPt * pcap)()
+            rc = self.o.lib.go_h_fan_paths(self.h, C.byref(params), C.byref(pr), _ptr(L), L.shape[0], int(flags),
+                                           out, cap, C.byref(n_out), pts, pcap, C.byref(n_pts), C.byref(steps))
+            if n_pts.value <= pcap:
+                break
+            pcap = n_pts.value
+        if rc != 0:
+            raise RuntimeError("oracle fan overflow")
+        return arrivals_to_array(out, n_out.value), paths_to_array(pts, n_pts.value), steps.value
+
+    def eig_path(self, params, src, incl_deg, az_deg, bounces, freq=0.1):
+        """The accepted eigenray's path rows and arrival row (go_eig_path)."""
+        s = np.ascontiguousarray(src, dtype=np.float64)
+        A = Arrival()
+        n = self.o.lib.go_eig_path(self.h, C.byref(params), _ptr(s), float(incl_deg), float(az_deg), int(bounces),
+                                   freq, None, 0, C.byref(A))
+        pts = (PathPt * max(1, n))()
+        n = self.o.lib.go_eig_path(self.h, C.byref(params), _ptr(s), float(incl_deg), float(az_deg), int(bounces),
+                                   freq, pts, n, C.byref(A))
+        return paths_to_array(pts, n), arrivals_to_array((Arrival * 1)(A), 1)[0]
 
     def eig_search(self, params, src, rcvr, incl_min=0.5, incl_max=45.0, bnc_min=0, bnc_max=0,
                    iterations=25, damping=1.0e-3, tolerance=0.1, az_dev_lim=2.0, incl_step_max=0.1,
@@ -296,6 +354,10 @@ def _rd_bind(L):
     L.go_rd_fan.restype = C.c_int
     L.go_rd_fan.argtypes = [C.c_void_p, C.POINTER(Params), C.POINTER(Prop), _dp, C.c_int,
                             C.POINTER(Arrival), C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_longlong)]
+    L.go_rd_fan_paths.restype = C.c_int
+    L.go_rd_fan_paths.argtypes = [C.c_void_p, C.POINTER(Params), C.POINTER(Prop), _dp, C.c_int, C.c_int,
+                                  C.POINTER(Arrival), C.c_int, C.POINTER(C.c_int), C.POINTER(PathPt), C.c_longlong,
+                                  C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
     L._rd_bound = True
 
 
@@ -457,6 +519,30 @@ class RdMedium:
         if rc != 0:
             raise RuntimeError("oracle fan overflow")
         return arrivals_to_array(out, n_out.value), steps.value
+
+    def fan_paths(self, launches, params, flags=PATH_RAYS, bounces=2, freq=0.1, turn_ht_min=0.2, src=None):
+        """write_rays / write_caustics run_prop (go_rd_fan_paths): (arrivals, path rows, ray-steps)."""
+        if src is None:
+            la, lo = self.centre()
+            src = (la, lo, 0.0)
+        Lr = np.ascontiguousarray(launches, dtype=np.float64).reshape(-1, 2)
+        pr = Prop(bounces, freq, turn_ht_min, src[0], src[1], src[2])
+        cap = max(1, Lr.shape[0] * (bounces + 1))
+        out = (Arrival * cap)()
+        n_out = C.c_int(0)
+        steps = C.c_longlong(0)
+        n_pts = C.c_longlong(0)
+        pcap = 1
+        while True:
+            pts = (PathPt * pcap)()
+            rc = self.o.lib.go_rd_fan_paths(self.h, C.byref(params), C.byref(pr), _ptr(Lr), Lr.shape[0], int(flags),
+                                            out, cap, C.byref(n_out), pts, pcap, C.byref(n_pts), C.byref(steps))
+            if n_pts.value <= pcap:
+                break
+            pcap = n_pts.value
+        if rc != 0:
+            raise RuntimeError("oracle fan overflow")
+        return arrivals_to_array(out, n_out.value), paths_to_array(pts, n_pts.value), steps.value
 
 
 def write_rd_files(d, lat_deg, lon_deg, z, c_ms, u_ms, v_ms, rho, topo=None, prefix="prof"):

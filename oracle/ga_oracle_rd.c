@@ -1050,6 +1050,102 @@ int go_rd_fan(go_rd *a, const go_params *p, const go_prop *pr, const double *lau
     return cnt > out_cap ? -1 : 0;
 }
 
+static void rd_pt(go_path_pt *pts, long long cap, long long *n, const go_rd *a, int li, int leg, int m, int kind,
+                  const double *y, double tl, double att, double tt)
+{
+    if (*n < cap) {
+        go_path_pt *P = pts + *n;
+        P->launch_idx = li;
+        P->leg = leg;
+        P->step = m;
+        P->kind = kind;
+        P->pos0 = y[1] * (180.0 / RD_PI);
+        P->pos1 = y[2] * (180.0 / RD_PI);
+        P->z = y[0] - a->r0;
+        P->tl = tl;
+        P->absorption = -2.0 * att;
+        P->time = tt;
+    }
+    (*n)++;
+}
+
+int go_rd_fan_paths(go_rd *a, const go_params *p, const go_prop *pr, const double *launches, int n_launch,
+                    int flags, go_arrival *out, int out_cap, int *n_out, go_path_pt *pts, long long pts_cap,
+                    long long *n_pts, long long *ray_steps)
+{
+    const int rays = (flags & GO_PATH_RAYS) != 0, caus = (flags & GO_PATH_CAUSTICS) != 0 && p->calc_amp;
+    int length = (int)(p->s_max / p->ds_min);
+    if (rd_rows_len < length) {
+        free(rd_rows);
+        rd_rows = (double *)malloc(sizeof(double) * 18 * (size_t)length);
+        rd_rows_len = length;
+    }
+    double *rows = rd_rows;
+    go_src s;
+    int cnt = 0;
+    long long steps = 0, np = 0;
+    double lat_src = pr->src0, lon_src = pr->src1, z_src = pr->src_alt;
+    lat_src *= RD_PI / 180.0;
+    lon_src *= RD_PI / 180.0;
+    z_src = dmax(z_src, go_rd_topo_z(a, lat_src, lon_src) - a->r0);
+    s.src[0] = a->r0 + z_src; s.src[1] = lat_src; s.src[2] = lon_src;
+
+    for (int li = 0; li < n_launch; li++) {
+        double incl = launches[2 * li], az = launches[2 * li + 1];
+        s.phi = RD_PI / 2.0 - az * (RD_PI / 180.0);
+        s.theta = incl * (RD_PI / 180.0);
+        go_rd_set_initial(a, p, &s, rows);
+        double tt = 0.0, att = 0.0, rmax = 0.0;
+        int brk = 0, k;
+        for (int bnc = 0; bnc <= pr->bounces; bnc++) {
+            k = go_rd_prop_rk4(a, p, &s, rows, length, &brk);
+            steps += k;
+            /* :451-490 */
+            double D = 0.0, D_prev = caus ? go_rd_jacobian(a, rows + 18) : 0.0;
+            for (int m = 1; m < k; m++) {
+                const double *ym = rows + 18 * m;
+                if (caus) D = go_rd_jacobian(a, ym);
+                tt += go_rd_travel_time_seg(a, rows + 18 * (m - 1), ym);
+                att += go_rd_atten_seg(a, rows + 18 * (m - 1), ym, pr->freq);
+                if (rays && (m == 1 || m % 15 == 0))
+                    rd_pt(pts, pts_cap, &np, a, li, bnc, m, 0, ym,
+                          p->calc_amp ? 20.0 * log10(go_rd_amp(a, &s, ym)) : 0.0, att, tt);
+                if (caus && D * D_prev < 0.0) rd_pt(pts, pts_cap, &np, a, li, bnc, m, 1, ym, 0.0, att, tt);
+                if (caus) D_prev = D;
+            }
+            for (int m = 0; m < k; m++) rmax = dmax(rmax, rows[18 * m] - a->r0);
+            if (!a->is_topo && rmax < (a->z0 + pr->turn_ht_min)) break;
+            if (brk || k < 2 || (tt / 3600.0) > p->time_max) break;
+            double *yk = rows + 18 * k;
+            go_arrival A;
+            memset(&A, 0, sizeof(A));
+            A.launch_idx = li;
+            A.n_bounce = bnc;
+            A.incl_launch = incl;
+            A.az_launch = az;
+            A.incl_arr = asin(go_rd_c(a, yk[0], yk[1], yk[2]) / go_rd_c(a, a->r0 + z_src, lat_src, lon_src) * yk[3]) * 180.0 / RD_PI;
+            double baz = 90.0 - atan2(-yk[4], -yk[5]) * 180.0 / RD_PI;
+            if (baz < -180.0) baz += 360.0;
+            else if (baz > 180.0) baz -= 360.0;
+            A.back_az = baz;
+            A.pos0 = yk[1] * (180.0 / RD_PI);
+            A.pos1 = yk[2] * (180.0 / RD_PI);
+            A.time = tt;
+            A.cel = go_gc_dist(a->r0, yk[1], yk[2], lat_src, lon_src) / tt;
+            A.turn_ht = rmax;
+            A.tl = p->calc_amp ? 20.0 * log10(go_rd_amp(a, &s, yk)) : 0.0;
+            A.absorption = -2.0 * att;
+            if (cnt < out_cap) out[cnt] = A;
+            cnt++;
+            go_rd_set_refl(a, p, &s, rows + 18 * (k - 2), rows + 18 * (k - 1), rows);
+        }
+    }
+    *n_out = cnt;
+    if (n_pts) *n_pts = np;
+    if (ray_steps) *ray_steps = steps;
+    return (cnt > out_cap || np > pts_cap) ? -1 : 0;
+}
+
 /* ---- handles ------------------------------------------------------------ */
 void *go_rd_new(const double *lat_deg, int nlat, const double *lon_deg, int nlon, const double *z, int nz,
                 const double *c, const double *u, const double *v, const double *rho)

@@ -106,6 +106,14 @@ double go_rd_theta_grnd(go_rd *a, double lat_src, double lon_src, double z_src, 
 int    go_rd_fan(go_rd *a, const go_params *p, const go_prop *pr, const double *launches,
                  int n_launch, go_arrival *out, int out_cap, int *n_out, long long *ray_steps);
 
+/* run_prop with write_rays / write_caustics (main.infraga.sph.rngdep.cpp:451-490):
+ * segments of rows 1..k-1 added one by one, rows m == 1 or m % 15 == 0
+ * sampled (flags & GO_PATH_RAYS), Jacobian sign changes recorded
+ * (flags & GO_PATH_CAUSTICS, needs calc_amp).  As go_fan_paths otherwise. */
+int    go_rd_fan_paths(go_rd *a, const go_params *p, const go_prop *pr, const double *launches,
+                       int n_launch, int flags, go_arrival *out, int out_cap, int *n_out,
+                       go_path_pt *pts, long long pts_cap, long long *n_pts, long long *ray_steps);
+
 /* ---- opaque handles for ctypes ------------------------------------------ */
 void  *go_rd_new(const double *lat_deg, int nlat, const double *lon_deg, int nlon, const double *z,
                  int nz, const double *c, const double *u, const double *v, const double *rho);

@@ -101,3 +101,40 @@ def compare(gpu, ref, geom_sph, src=(0.0, 0.0), rel=1e-6, db=0.01, near=None, ke
         fails["values"] = bad[:10]
     fails_summary = {"worst": worst, "n_common": len(set(g) & set(r))}
     return fails, fails_summary
+
+
+def compare_paths(gpu, ref, geom_sph, src=(0.0, 0.0, 0.0), rel=1e-6, db=0.01):
+    """Path-row sets (ga_path_pt / go_path_pt): keys (ray, leg, step, kind)
+    must be equal and in the same order; the row position within rel of the
+    range from the source (plus 1 km), the altitude within rel of the same
+    scale, time within rel, TL and absorption within db."""
+    fails = {}
+    kg = np.stack([gpu[f] for f in ("launch_idx", "leg", "step", "kind")], axis=1) if len(gpu) else np.zeros((0, 4))
+    kr = np.stack([ref[f] for f in ("launch_idx", "leg", "step", "kind")], axis=1) if len(ref) else np.zeros((0, 4))
+    if kg.shape != kr.shape or not (kg == kr).all():
+        sg, sr = set(map(tuple, kg.tolist())), set(map(tuple, kr.tolist()))
+        fails["keys"] = {"n_gpu": len(kg), "n_ref": len(kr), "missing": sorted(sr - sg)[:10],
+                         "extra": sorted(sg - sr)[:10]}
+        return fails, {"n": 0}
+    if geom_sph:
+        rng = gc_range(ref["pos0"], ref["pos1"], src[0], src[1])
+        dpos = R0 * np.radians(np.hypot(gpu["pos0"] - ref["pos0"],
+                                        (gpu["pos1"] - ref["pos1"]) * np.cos(np.radians(ref["pos0"]))))
+    else:
+        rng = np.hypot(ref["pos0"] - src[0], ref["pos1"] - src[1])
+        dpos = np.hypot(gpu["pos0"] - ref["pos0"], gpu["pos1"] - ref["pos1"])
+    scale = rng + 1.0
+    e = {
+        "pos": dpos / scale,
+        "z": np.abs(gpu["z"] - ref["z"]) / scale,
+        "time": np.abs(gpu["time"] - ref["time"]) / np.maximum(np.abs(ref["time"]), 1e-9),
+        "tl": np.abs(gpu["tl"] - ref["tl"]),
+        "absorption": np.abs(gpu["absorption"] - ref["absorption"]),
+    }
+    worst = {k: float(v.max()) if len(v) else 0.0 for k, v in e.items()}
+    bad = (e["pos"] > rel) | (e["z"] > rel) | (e["time"] > rel) | (e["tl"] > db) | (e["absorption"] > db)
+    if bad.any():
+        idx = np.nonzero(bad)[0][:10]
+        fails["values"] = [(tuple(kg[i].tolist()), {k: float(v[i]) for k, v in e.items()}) for i in idx]
+        fails["n_bad"] = int(bad.sum())
+    return fails, {"worst": worst, "n": len(kg)}

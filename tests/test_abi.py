@@ -37,7 +37,8 @@ def test_struct_layouts_match_the_c_header(tmp_path):
     field offsets for include/infraga_gpu.h."""
     hdr = os.path.join(ROOT, "include", "infraga_gpu.h")
     structs = {"ga_prop_params": engine.PropParams, "ga_arrival": engine.Arrival, "ga_run_stats": engine.RunStats,
-               "ga_eig_params": engine.EigParams, "ga_request": engine.Request, "ga_result": engine.Result}
+               "ga_eig_params": engine.EigParams, "ga_request": engine.Request, "ga_result": engine.Result,
+               "ga_path_pt": engine.PathPt}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "%s"' % hdr, "int main(void) {"]
     for cname, cls in structs.items():
         lines.append('printf("%s size %%zu\\n", sizeof(%s));' % (cname, cname))
