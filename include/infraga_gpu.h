@@ -28,6 +28,9 @@
  *                          -prop loop (<id>.raypaths.dat, <id>.caustics-N.dat)
  *                          src/main.infraga.sph.cpp:388-413, 456-496, 549;
  *                          src/main.infraga.3d.cpp:378-400, 442-487
+ *   ga_eig_direct       <- run_eig_direct: geoac::find_eigenray from a
+ *                          launch estimate, src/main.infraga.3d.cpp:955-1086,
+ *                          src/main.infraga.sph.cpp:998-1139
  *   ga_eig_search_paths <- the same search, plus find_eigenray's
  *                          <id>.eigenray-N.dat rows
  *                          src/geoac/geoac.eigenray.3d.cpp:223-276,
@@ -205,6 +208,19 @@ int ga_fan_run_paths(ga_ctx *ctx, int atmo_id, const ga_prop_params *p,
 int ga_eig_search(ga_ctx *ctx, int atmo_id, const ga_prop_params *p, const ga_eig_params *e,
                   const double *rcvr_xy, int n_rcvr,
                   ga_eigenray *out, int out_cap, int *n_out, double *rcvr_latency_ms,
+                  ga_run_stats *stats);
+
+/* -eig_direct: the Levenberg-Marquardt eigenray iteration of find_eigenray
+ * alone, from the launch estimate (incl_est_deg, az_est_deg) with `bounces`
+ * reflections toward one receiver (rcvr_xy as for ga_eig_search), using
+ * e->iterations, e->damping and e->tolerance.  az_est_deg NaN: toward the
+ * receiver (sph: great-circle bearing; 3D: straight line), the reference's
+ * default.  *found = 1 and out[0] hold the eigenray when the iteration
+ * converges, with its eigenray-0.dat rows in pts (GA_E_CAP with *n_pts set
+ * when pts_cap is too small); *found = 0 otherwise. */
+int ga_eig_direct(ga_ctx *ctx, int atmo_id, const ga_prop_params *p, const ga_eig_params *e,
+                  const double *rcvr_xy, double incl_est_deg, double az_est_deg, int bounces,
+                  ga_eigenray *out, int *found, ga_path_pt *pts, int64_t pts_cap, int64_t *n_pts,
                   ga_run_stats *stats);
 
 /* ga_eig_search plus the path rows of every eigenray found (the rows

@@ -40,13 +40,19 @@ def build_frontends():
     if not os.path.exists(src):
         return []
     outs = []
-    for name, geom in (("infraga-sph", "GA_GEOM_SPH"), ("infraga-3d", "GA_GEOM_3D"),
-                       ("infraga-sph-rngdep", "GA_GEOM_SPH_RNGDEP")):
-        out = os.path.join(BIN, name)
-        _run(["g++", "-O2", "-std=c++17", f"-DGA_CLI_GEOM={geom}", f"-DGA_CLI_NAME=\"{name}\"",
-              "-I", os.path.join(ROOT, "include"), "-o", out, src, "-L", HERE, "-linfraga_gpu",
-              f"-Wl,-rpath,$ORIGIN/.."])
-        outs.append(out)
+    # infraga-accel-* are the names the reference's Python front end runs
+    # under mpirun (cpu_cnt > 1); they take the MPI drivers' launch lists and
+    # let rank 0 do the whole job.
+    for accel in (0, 1):
+        for name, geom in (("infraga-sph", "GA_GEOM_SPH"), ("infraga-3d", "GA_GEOM_3D"),
+                           ("infraga-sph-rngdep", "GA_GEOM_SPH_RNGDEP")):
+            if accel:
+                name = name.replace("infraga-", "infraga-accel-")
+            out = os.path.join(BIN, name)
+            _run(["g++", "-O2", "-std=c++17", f"-DGA_CLI_GEOM={geom}", f"-DGA_CLI_NAME=\"{name}\"",
+                  f"-DGA_CLI_ACCEL={accel}", "-I", os.path.join(ROOT, "include"), "-o", out, src, "-L", HERE,
+                  "-linfraga_gpu", f"-Wl,-rpath,$ORIGIN/.."])
+            outs.append(out)
     return outs
 
 

@@ -130,6 +130,13 @@ public:
         live_.resize(n_rcvr_, 0);
         done_ms_.resize(n_rcvr_, 0.0);
         for (int r = base; r < n_rcvr_; ++r) {
+            const Direct *d = nullptr;
+            for (const Direct &q : direct_)
+                if (q.rcvr == r) d = &q;
+            if (d) {
+                spawn(S_.geom == GA_GEOM_SPH ? find_sph(r, d->nb, 0, d->th, d->ph) : find(r, d->nb, 0, d->th, d->ph), r);
+                continue;
+            }
             for (int nb = S_.bnc_min; nb <= S_.bnc_max; ++nb)
                 spawn(S_.geom == GA_GEOM_SPH ? chain_sph(r, nb) : chain(r, nb), r);
             if (S_.bnc_max < S_.bnc_min) receiver_done(r);
@@ -137,6 +144,19 @@ public:
         return base;
     }
     void set_clock(std::chrono::steady_clock::time_point t0) { t0_ = t0; }
+
+    // -eig_direct (run_eig_direct, src/main.infraga.3d.cpp:955-1086,
+    // src/main.infraga.sph.cpp:998-1139): one find_eigenray from the launch
+    // estimate (th_est, ph_est) [rad, geoac convention] with nb reflections
+    // toward the receiver xy (search units); no inclination sweep.  Returns
+    // the receiver index; call before start().
+    int add_direct(const double *xy, double th_est, double ph_est, int nb)
+    {
+        const int r = (int)(init_xy_.size() / 2);
+        direct_.push_back(Direct{r, nb, th_est, ph_est});
+        init_xy_.insert(init_xy_.end(), xy, xy + 2);
+        return r;
+    }
 
     // Receivers whose searches all finished since the last call.
     std::vector<int> take_done()
@@ -665,6 +685,11 @@ private:
         return a.seq < b.seq;
     }
 
+    struct Direct {
+        int rcvr, nb;
+        double th, ph;
+    };
+    std::vector<Direct> direct_;
     EigSetup S_;
     int n_rcvr_;
     int stride_;

@@ -1736,10 +1736,17 @@ int ga_fan_run_paths(ga_ctx *ctx, int atmo_id, const ga_prop_params *p, const do
     return prc;
 }
 
+// One -eig_direct request: launch estimate [deg] (az NaN: toward the
+// receiver) and reflections.
+struct EigDirect {
+    double incl_deg, az_deg;
+    int bounces;
+};
+
 static int eig_search_impl(ga_ctx *ctx, int atmo_id, const ga_prop_params *p, const ga_eig_params *e,
                            const double *rcvr_xy, int n_rcvr, ga_eigenray *out, int out_cap, int *n_out,
                            double *rcvr_latency_ms, ga_run_stats *stats, bool paths, ga_path_pt *pts,
-                           int64_t pts_cap, int64_t *n_pts)
+                           int64_t pts_cap, int64_t *n_pts, const EigDirect *direct = nullptr)
 {
     auto t0 = std::chrono::steady_clock::now();
     if (!ctx || !p || !e || !n_out || (n_rcvr > 0 && !rcvr_xy)) return fail(ctx, GA_E_ARG, "null argument");
@@ -1764,7 +1771,23 @@ static int eig_search_impl(ga_ctx *ctx, int atmo_id, const ga_prop_params *p, co
     std::vector<double> lat;
     EigSearchStats es;
     {
-        EigSearch search(S, rcvr.data(), n_rcvr);
+        EigSearch search(S, direct ? nullptr : rcvr.data(), direct ? 0 : n_rcvr);
+        if (direct) {
+            // run_eig_direct: theta_est from incl_est; phi_est = pi/2 - az_est,
+            // by default the direction of the receiver (sph: pi/2 - bearing,
+            // main.infraga.sph.cpp:1112; 3D: atan2(dy, dx), main.infraga.3d.cpp:1050)
+            const double th = direct->incl_deg * (PI / 180.0);
+            double ph;
+            if (!std::isnan(direct->az_deg)) ph = PI / 2.0 - direct->az_deg * (PI / 180.0);
+            else if (S.geom == GA_GEOM_SPH) {
+                const double t1 = std::sin(rcvr[1] - S.src[2]);
+                const double t2 = std::cos(S.src[1]) * std::tan(rcvr[0]) - std::sin(S.src[1]) * std::cos(rcvr[1] - S.src[2]);
+                ph = PI / 2.0 - std::atan2(t1, t2);
+            } else {
+                ph = std::atan2(rcvr[1] - S.src[1], rcvr[0] - S.src[0]);
+            }
+            search.add_direct(rcvr.data(), th, ph, direct->bounces);
+        }
         rc = search.run([&](const EigBatch &B, EigBatchOut &O) { return gpu(B, O); }, found, lat, es);
         search.result_jobs(jobs);
     }
@@ -1799,6 +1822,23 @@ int ga_eig_search(ga_ctx *ctx, int atmo_id, const ga_prop_params *p, const ga_ei
 {
     return eig_search_impl(ctx, atmo_id, p, e, rcvr_xy, n_rcvr, out, out_cap, n_out, rcvr_latency_ms, stats, false,
                            nullptr, 0, nullptr);
+}
+
+int ga_eig_direct(ga_ctx *ctx, int atmo_id, const ga_prop_params *p, const ga_eig_params *e, const double *rcvr_xy,
+                  double incl_est_deg, double az_est_deg, int bounces, ga_eigenray *out, int *found,
+                  ga_path_pt *pts, int64_t pts_cap, int64_t *n_pts, ga_run_stats *stats)
+{
+    if (!ctx || !p || !e || !rcvr_xy || !out || !found || !n_pts) return fail(ctx, GA_E_ARG, "null argument");
+    *found = 0;
+    *n_pts = 0;
+    if (bounces < 0) return fail(ctx, GA_E_ARG, "bounces must be >= 0");
+    if (pts_cap < 0) return fail(ctx, GA_E_ARG, "negative path capacity");
+    if (!std::isfinite(incl_est_deg)) return fail(ctx, GA_E_ARG, "inclination estimate must be finite");
+    ga_eig_params e2 = *e;
+    e2.bnc_min = e2.bnc_max = bounces;
+    const EigDirect d{incl_est_deg, az_est_deg, bounces};
+    return eig_search_impl(ctx, atmo_id, p, &e2, rcvr_xy, 1, out, 1, found, nullptr, stats, true, pts, pts_cap,
+                           n_pts, &d);
 }
 
 int ga_eig_search_paths(ga_ctx *ctx, int atmo_id, const ga_prop_params *p, const ga_eig_params *e,

@@ -71,6 +71,7 @@ void usage()
     std::cout << "Usage: " << GA_CLI_NAME << " [option] profile.met [parameters]\n"
               << "  -prop        ray paths for a fan of inclinations and azimuths\n"
               << "  -eig_search  eigenrays between the source and a receiver\n"
+              << "  -eig_direct  one eigenray from a launch-angle estimate (incl_est, az_est)\n"
               << "Parameters use the key=value names of the reference infraga drivers.\n";
 }
 
@@ -85,6 +86,57 @@ std::string strip_ext(char *path)
 }
 
 double profile_bottom(const char *path);
+
+#ifndef GA_CLI_ACCEL
+#define GA_CLI_ACCEL 0
+#endif
+
+// Rank and size of an mpirun launch (infraga-accel-*: MPICH/Hydra PMI_*,
+// Open MPI OMPI_COMM_WORLD_*, PMIx PMIX_RANK); 0 and 1 otherwise.
+int env_int(const char *const *names, int dflt)
+{
+    for (; *names; ++names)
+        if (const char *v = std::getenv(*names)) return std::atoi(v);
+    return dflt;
+}
+int mpi_rank()
+{
+    static const char *const n[] = {"PMI_RANK", "OMPI_COMM_WORLD_RANK", "PMIX_RANK", "MV2_COMM_WORLD_RANK", nullptr};
+    return GA_CLI_ACCEL ? env_int(n, 0) : 0;
+}
+int mpi_size()
+{
+    static const char *const n[] = {"PMI_SIZE", "OMPI_COMM_WORLD_SIZE", "MV2_COMM_WORLD_SIZE", nullptr};
+    return GA_CLI_ACCEL ? std::max(1, env_int(n, 1)) : 1;
+}
+
+// The inclinations launched at one azimuth.  Serial drivers accumulate
+// theta += step up to max(theta_max, theta_grnd) (main.infraga.sph.cpp:434,
+// main.infraga.3d.cpp:421); the MPI drivers step by step x ranks and launch
+// theta + step x j on rank j while that stays <= theta_max, with the outer
+// bound max(theta_max, theta_grnd) (main.infraga.sph.mpi.cpp:388-399,
+// main.infraga.sph.rngdep.mpi.cpp:381-392) or theta_max in 3D
+// (main.infraga.3d.mpi.cpp:369-380).
+void incl_loop(double theta_min, double theta_max, double theta_grnd, double step, double phi,
+               std::vector<double> &launches)
+{
+    const int world = mpi_size();
+    const double top = (world > 1 && GA_CLI_GEOM == GA_GEOM_3D) ? theta_max : std::max(theta_max, theta_grnd);
+    if (world <= 1) {
+        for (double theta = std::max(theta_min, theta_grnd); theta <= top; theta += step) {
+            launches.push_back(theta);
+            launches.push_back(phi);
+        }
+        return;
+    }
+    for (double theta = std::max(theta_min, theta_grnd); theta <= top; theta += step * world) {
+        for (int j = 0; j < world; ++j) {
+            if (theta + step * j > theta_max) continue;
+            launches.push_back(theta + step * j);
+            launches.push_back(phi);
+        }
+    }
+}
 
 // The fan of run_prop with write_rays / write_caustics
 // (main.infraga.sph.cpp:388-413, 456-496, 549; main.infraga.3d.cpp:378-400,
@@ -255,10 +307,7 @@ int run_prop(int argc, char **argv)
     std::vector<int> az_end;      // launch count after each azimuth (blank lines)
     for (double phi = phi_min; phi <= phi_max; phi += phi_step) {
         double theta_grnd = (z_src - zg_alt < 0.01) ? 0.1 : -89.9;
-        for (double theta = std::max(theta_min, theta_grnd); theta <= std::max(theta_max, theta_grnd); theta += theta_step) {
-            launches.push_back(theta);
-            launches.push_back(phi);
-        }
+        incl_loop(theta_min, theta_max, theta_grnd, theta_step, phi, launches);
         az_end.push_back((int)(launches.size() / 2));
     }
     const int n = (int)(launches.size() / 2);
@@ -439,10 +488,7 @@ int run_prop_rd(int argc, char **argv)
         } else {
             theta_grnd = -89.9;
         }
-        for (double theta = std::max(theta_min, theta_grnd); theta <= std::max(theta_max, theta_grnd); theta += theta_step) {
-            launches.push_back(theta);
-            launches.push_back(phi);
-        }
+        incl_loop(theta_min, theta_max, theta_grnd, theta_step, phi, launches);
         az_end.push_back((int)(launches.size() / 2));
     }
     const int n = (int)(launches.size() / 2);
@@ -669,10 +715,135 @@ int run_eig_search(int argc, char **argv)
     return 0;
 }
 
+// -eig_direct (run_eig_direct, main.infraga.3d.cpp:955-1086,
+// main.infraga.sph.cpp:998-1139): find_eigenray from a launch estimate;
+// writes <id>.eigenray-0.dat when it converges (the reference opens no
+// arrivals file for this option).
+int run_eig_direct(int argc, char **argv)
+{
+    const bool sph = GA_CLI_GEOM == GA_GEOM_SPH;
+    double s0 = sph ? 30.0 : 0.0, s1 = 0.0, z_src = 0.0;
+    double r0 = sph ? 30.0 : -250.0, r1 = sph ? -2.5 : 0.0;
+    double incl_est = 10.0, az_est = NAN;
+    int bounces = 0;
+    bool reverse_winds = false, custom_id = false;
+    double freq = 0.1;
+    const char *prof_format = "zTuvdp";
+    const char *output_id = nullptr;
+    ga_prop_params p;
+    ga_prop_params_default(&p, GA_CLI_GEOM);
+    ga_eig_params e;
+    ga_eig_params_default(&e);
+    double z_grnd = NAN;
+
+    for (int i = 3; i < argc; i++) {
+        const char *a = argv[i], *v = nullptr;
+        if (key2(a, "incl_est=", "est_incl=", v)) incl_est = std::atof(v);
+        else if (key2(a, "az_est=", "est_az=", v)) az_est = std::atof(v);
+        else if (key(a, "bounces=", v)) bounces = std::atoi(v);
+        else if (key2(a, "src_alt=", "alt_src=", v)) z_src = std::atof(v);
+        else if (sph && key2(a, "src_lat=", "lat_src=", v)) s0 = std::atof(v);
+        else if (sph && key2(a, "src_lon=", "lon_src=", v)) s1 = std::atof(v);
+        else if (sph && key2(a, "rcvr_lat=", "lat_rcvr=", v)) r0 = std::atof(v);
+        else if (sph && key2(a, "rcvr_lon=", "lon_rcvr=", v)) r1 = std::atof(v);
+        else if (!sph && key2(a, "src_x=", "x_src=", v)) s0 = std::atof(v);
+        else if (!sph && key2(a, "src_y=", "y_src_y=", v)) s1 = std::atof(v);
+        else if (!sph && key2(a, "rcvr_x=", "x_rcvr=", v)) r0 = std::atof(v);
+        else if (!sph && key2(a, "rcvr_y=", "y_rcvr=", v)) r1 = std::atof(v);
+        else if (key(a, "verbose=", v)) { /* trace-by-trace log not produced */ }
+        else if (key(a, "iterations=", v)) e.iterations = (int)std::atof(v);
+        else if (key(a, "damping=", v)) e.damping = std::atof(v);
+        else if (key(a, "tolerance=", v)) e.tolerance = std::atof(v);
+        else if (key(a, "freq=", v)) freq = std::atof(v);
+        else if (key(a, "abs_coeff=", v)) p.abs_coeff = std::max(0.0, std::atof(v));
+        else if (key2(a, "max_alt=", "alt_max=", v)) p.alt_max = std::atof(v);
+        else if (key2(a, "max_rng=", "rng_max=", v)) p.rng_max = std::atof(v);
+        else if (sph && key2(a, "max_time=", "time_max=", v)) p.time_max = std::atof(v);
+        else if (sph && key2(a, "min_lat=", "lat_min=", v)) p.bounds[0] = std::atof(v);
+        else if (sph && key2(a, "max_lat=", "lat_max=", v)) p.bounds[1] = std::atof(v);
+        else if (sph && key2(a, "min_lon=", "lon_min=", v)) p.bounds[2] = std::atof(v);
+        else if (sph && key2(a, "max_lon=", "lon_max=", v)) p.bounds[3] = std::atof(v);
+        else if (!sph && key2(a, "min_x=", "x_min=", v)) p.bounds[0] = std::atof(v);
+        else if (!sph && key2(a, "max_x=", "x_max=", v)) p.bounds[1] = std::atof(v);
+        else if (!sph && key2(a, "min_y=", "y_min=", v)) p.bounds[2] = std::atof(v);
+        else if (!sph && key2(a, "max_y=", "y_max=", v)) p.bounds[3] = std::atof(v);
+        else if (key2(a, "min_ds=", "ds_min=", v)) p.ds_min = std::atof(v);
+        else if (key2(a, "max_ds=", "ds_max=", v)) p.ds_max = std::atof(v);
+        else if (key2(a, "max_s=", "s_max=", v)) p.s_max = std::atof(v);
+        else if (key(a, "write_atmo=", v)) { /* profile dump: not produced */ }
+        else if (key(a, "prof_format=", v)) prof_format = v;
+        else if (key(a, "reverse_winds=", v)) reverse_winds = str2bool(v);
+        else if (key(a, "output_id=", v)) { custom_id = true; output_id = v; }
+        else if (key(a, "z_grnd=", v)) z_grnd = std::atof(v);
+        else if (key(a, "topo_file=", v) || key(a, "topo_use_BLw=", v)) {
+            if (str2bool(v) || std::strncmp(a, "topo_file=", 10) == 0) {
+                std::cerr << GA_CLI_NAME << ": " << a << " is not supported by this engine\n";
+                return 2;
+            }
+        } else {
+            std::cerr << GA_CLI_NAME << ": unrecognized parameter entry: " << a << '\n';
+            return 2;
+        }
+    }
+    if (bounces < 0) {
+        std::cerr << GA_CLI_NAME << ": bounces must be >= 0\n";
+        return 2;
+    }
+
+    ga_ctx *ctx = nullptr;
+    if (ga_ctx_create(&ctx, 0) != GA_OK) return die(nullptr, "no usable GPU");
+    int aid = 0;
+    if (ga_atmo_load_file(ctx, argv[2], prof_format, reverse_winds ? 1 : 0, &aid) != GA_OK)
+        return die(ctx, "cannot load the atmosphere");
+    const double z0 = std::isnan(z_grnd) ? profile_bottom(argv[2]) : z_grnd;
+    p.z_grnd = z0;
+    p.freq = freq;
+    z_src = std::max(z_src, z0);
+    p.src[0] = s0;
+    p.src[1] = s1;
+    p.src[2] = z_src;
+    const double rcvr[2] = {r0, r1};
+    ga_eigenray eig;
+    int found = 0;
+    std::vector<ga_path_pt> pts((size_t)1 << 16);
+    int64_t n_pts = 0;
+    ga_run_stats st;
+    int rc = ga_eig_direct(ctx, aid, &p, &e, rcvr, incl_est, az_est, bounces, &eig, &found, pts.data(),
+                           (int64_t)pts.size(), &n_pts, &st);
+    if (rc == GA_E_CAP) {
+        pts.resize((size_t)n_pts);
+        rc = ga_eig_direct(ctx, aid, &p, &e, rcvr, incl_est, az_est, bounces, &eig, &found, pts.data(),
+                           (int64_t)pts.size(), &n_pts, &st);
+    }
+    if (rc != GA_OK) return die(ctx, "eigenray iteration failed");
+    std::string id = custom_id ? std::string(output_id) : strip_ext(argv[2]);
+    if (found) {
+        std::ofstream ray(id + ".eigenray-0.dat");
+        ray << (sph ? "# lat [deg]" : "# x [km]") << '\t' << (sph ? "lon [deg]" : "y [km]") << '\t' << "z [km]"
+            << '\t' << "trans. coeff. [dB]" << '\t' << "absorption [dB]" << '\t' << "time [s]" << '\n';
+        if (sph) ray << std::setprecision(8);
+        for (int64_t w = 0; w < n_pts; ++w) {
+            const ga_path_pt &P = pts[w];
+            const double ab = (!sph && P.leg > 0) ? P.absorption * -0.5 : P.absorption;
+            ray << P.pos0 << '\t' << P.pos1 << '\t' << P.z << '\t' << P.tl << '\t' << ab << '\t' << P.time << '\n';
+        }
+        std::cout << '\t' << "Eigenray identified:" << '\t' << "theta, phi = " << std::setprecision(8) << eig.incl << ", "
+                  << eig.az << " degrees." << '\n';
+    } else {
+        std::cout << GA_CLI_NAME << ": no eigenray identified\n";
+    }
+    std::cout << GA_CLI_NAME << ": " << st.n_rays << " ray traces, " << st.ray_steps << " ray-steps, " << st.kernel_ms
+              << " ms on the GPU\n";
+    ga_ctx_destroy(ctx);
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char **argv)
 {
+    // under mpirun every rank starts a copy: rank 0 does the whole job on the GPU
+    if (mpi_rank() > 0) return 0;
     if (argc < 3) { usage(); return argc < 2 ? 0 : 2; }
     if (GA_CLI_GEOM == GA_GEOM_SPH_RNGDEP) {
         if (std::strncmp(argv[1], "-prop", 5) == 0) return run_prop_rd(argc, argv);
@@ -681,6 +852,7 @@ int main(int argc, char **argv)
     }
     if (std::strncmp(argv[1], "-prop", 5) == 0) return run_prop(argc, argv);
     if (std::strncmp(argv[1], "-eig_search", 11) == 0) return run_eig_search(argc, argv);
+    if (std::strncmp(argv[1], "-eig_direct", 11) == 0) return run_eig_direct(argc, argv);
     std::cerr << GA_CLI_NAME << ": option " << argv[1] << " is not supported by this engine\n";
     usage();
     return 2;

@@ -27,7 +27,7 @@ EXPORTED = ["ga_prop_params_default", "ga_eig_params_default", "ga_ctx_create", 
             "ga_fan_run_device", "ga_eig_search", "ga_server_create", "ga_server_submit", "ga_server_poll",
             "ga_server_stats", "ga_server_destroy", "ga_rngdep_load", "ga_topo_load", "ga_rngdep_eval",
             "ga_rngdep_release", "ga_rngdep_load_files", "ga_rngdep_info", "ga_fan_run_paths",
-            "ga_eig_search_paths"]
+            "ga_eig_search_paths", "ga_eig_direct"]
 
 GA_PATH_RAYS, GA_PATH_CAUSTICS = 1, 2
 GA_PT_RAY, GA_PT_CAUSTIC = 0, 1
@@ -127,6 +127,9 @@ def load_library(path: str | None = None):
     L.ga_eig_search_paths.argtypes = [C.c_void_p, C.c_int, C.POINTER(PropParams), C.POINTER(EigParams), _dp,
                                       C.c_int, C.POINTER(Arrival), C.c_int, C.POINTER(C.c_int), _dp,
                                       C.POINTER(PathPt), C.c_int64, C.POINTER(C.c_int64), C.POINTER(RunStats)]
+    L.ga_eig_direct.argtypes = [C.c_void_p, C.c_int, C.POINTER(PropParams), C.POINTER(EigParams), _dp, C.c_double,
+                                C.c_double, C.c_int, C.POINTER(Arrival), C.POINTER(C.c_int), C.POINTER(PathPt),
+                                C.c_int64, C.POINTER(C.c_int64), C.POINTER(RunStats)]
     L.ga_rngdep_load.argtypes = [C.c_void_p, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, _dp, _dp, _dp,
                                  C.POINTER(C.c_int)]
     L.ga_topo_load.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp]
@@ -342,6 +345,28 @@ class Engine:
             break
         self._check(rc)
         return out[: n_out.value].copy(), pts[: n_pts.value].copy(), lat, st
+
+
+    def eig_direct(self, aid, params: PropParams, eig: EigParams, rcvr_xy, incl_est, az_est=math.nan,
+                   bounces=0, pts_cap=1 << 16):
+        """-eig_direct: (eigenray row or None, its path rows, RunStats)."""
+        R = np.ascontiguousarray(rcvr_xy, dtype=np.float64).reshape(2)
+        out = np.zeros(1, dtype=ARRIVAL_DTYPE)
+        found = C.c_int(0)
+        n_pts = C.c_int64(0)
+        st = RunStats()
+        while True:
+            pts = np.zeros(max(1, pts_cap), dtype=PATH_DTYPE)
+            rc = self.L.ga_eig_direct(self.h, aid, C.byref(params), C.byref(eig), R.ctypes.data_as(_dp),
+                                      float(incl_est), float(az_est), int(bounces),
+                                      out.ctypes.data_as(C.POINTER(Arrival)), C.byref(found),
+                                      pts.ctypes.data_as(C.POINTER(PathPt)), pts_cap, C.byref(n_pts), C.byref(st))
+            if rc == GA_E_CAP and n_pts.value > pts_cap:
+                pts_cap = n_pts.value
+                continue
+            break
+        self._check(rc)
+        return (out[0].copy() if found.value else None), pts[: n_pts.value].copy(), st
 
 
 class Server:

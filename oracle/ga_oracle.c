@@ -1796,6 +1796,49 @@ int go_eig_search_sph(go_atmo *a, const go_params *p, const double src_in[3],
     return cnt;
 }
 
+/* run_eig_direct (main.infraga.3d.cpp:955-1086, main.infraga.sph.cpp:998-1139):
+ * find_eigenray from a launch estimate, no sweep. */
+int go_eig_direct(go_atmo *a, const go_params *p, const double src_in[3], const double rcvr_in[2],
+                  double incl_est_deg, double az_est_deg, int bnc, int iterations, double damping,
+                  double tolerance, double freq, go_arrival *A, long long *ray_steps)
+{
+    eig_ctx E;
+    E.a = a;
+    E.p6 = *p; E.p6.calc_amp = 0; E.p6.eq_cnt = 6;
+    E.p18 = *p; E.p18.calc_amp = 1; E.p18.eq_cnt = 18;
+    E.dth_big = E.dth_sml = 0.0;
+    E.damping = damping;
+    E.tolerance = tolerance;
+    E.steps = 0;
+    memset(eig_cnt, 0, sizeof(eig_cnt));
+    const double th_est = incl_est_deg * (GO_PI / 180.0);
+    int ok;
+    if (a->geom == GA_GEOM_SPH) {
+        double src[3], rcvr[2];
+        src[1] = src_in[0] * (GO_PI / 180.0);
+        src[2] = src_in[1] * (GO_PI / 180.0);
+        rcvr[0] = rcvr_in[0] * (GO_PI / 180.0);
+        rcvr[1] = rcvr_in[1] * (GO_PI / 180.0);
+        src[0] = src_in[2] > ground(a) - a->r0 ? src_in[2] : ground(a) - a->r0;   /* sph.cpp:1106 */
+        double ph_est;
+        if (isnan(az_est_deg)) {                                                   /* sph.cpp:1110 */
+            double t1 = sin(rcvr[1] - src[2]);
+            double t2 = cos(src[1]) * tan(rcvr[0]) - sin(src[1]) * cos(rcvr[1] - src[2]);
+            ph_est = GO_PI / 2.0 - atan2(t1, t2);
+        } else {
+            ph_est = GO_PI / 2.0 - az_est_deg * (GO_PI / 180.0);
+        }
+        ok = find_eigenray_sph(&E, src, rcvr, th_est, ph_est, freq, bnc, iterations, A);
+    } else {
+        double src[3] = {src_in[0], src_in[1], src_in[2] > ground(a) ? src_in[2] : ground(a)};   /* 3d.cpp:1048 */
+        double ph_est = isnan(az_est_deg) ? atan2(rcvr_in[1] - src[1], rcvr_in[0] - src[0])     /* 3d.cpp:1050 */
+                                          : GO_PI / 2.0 - az_est_deg * (GO_PI / 180.0);
+        ok = find_eigenray(&E, src, rcvr_in, th_est, ph_est, freq, bnc, iterations, A);
+    }
+    if (ray_steps) *ray_steps = E.steps;
+    return ok;
+}
+
 void go_eig_counts(long long out[GO_EIG_NCOUNT])
 {
     for (int i = 0; i < GO_EIG_NCOUNT; i++) out[i] = eig_cnt[i];

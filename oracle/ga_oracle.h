@@ -214,6 +214,14 @@ int    go_eig_search_sph(go_atmo *a, const go_params *p, const double src[3],
                          double dth_sml_deg, double freq,
                          go_arrival *out, int out_cap, long long *ray_steps);
 
+/* -eig_direct: find_eigenray from the launch estimate (degrees; az_est_deg
+ * NaN = toward the receiver) with bnc reflections.  src and rcvr as for
+ * the searches of the geometry.  Returns 1 with *A filled when it
+ * converges, else 0. */
+int    go_eig_direct(go_atmo *a, const go_params *p, const double src[3], const double rcvr[2],
+                     double incl_est_deg, double az_est_deg, int bnc, int iterations, double damping,
+                     double tolerance, double freq, go_arrival *A, long long *ray_steps);
+
 /* Work counters of the last go_eig_search_3d / go_eig_search_sph call. */
 enum { GO_EIG_EST, GO_EIG_SWEEP_TRACES, GO_EIG_SWEEP_CORR, GO_EIG_SWEEP_ADAPT, GO_EIG_LM_TRACES,
        GO_EIG_FINAL_TRACES, GO_EIG_FINDS, GO_EIG_FOUND, GO_EIG_NCOUNT };

@@ -135,6 +135,10 @@ class Oracle:
                                     C.c_double, C.POINTER(Arrival)]
         L.go_eig_row_3d.restype = C.c_int
         L.go_eig_counts.argtypes = [C.POINTER(C.c_longlong)]
+        L.go_eig_direct.argtypes = [C.c_void_p, C.POINTER(Params), _dp, _dp, C.c_double, C.c_double, C.c_int,
+                                    C.c_int, C.c_double, C.c_double, C.c_double, C.POINTER(Arrival),
+                                    C.POINTER(C.c_longlong)]
+        L.go_eig_direct.restype = C.c_int
 
     EIG_COUNT_NAMES = ("est_calls", "sweep_traces", "sweep_traces_corrected", "sweep_traces_adaptive",
                        "lm_traces", "final_traces", "finds", "found")
@@ -236,6 +240,18 @@ class Medium:
                incl_step_max, incl_step_min, freq, out, cap, C.byref(steps))
         return arrivals_to_array(out, min(n, cap)), steps.value
 
+
+    def eig_direct(self, params, src, rcvr, incl_est, az_est=float("nan"), bounces=0, iterations=25,
+                   damping=1.0e-3, tolerance=0.1, freq=0.1):
+        """-eig_direct (go_eig_direct): (row or None, ray-steps)."""
+        s = np.ascontiguousarray(src, dtype=np.float64)
+        r = np.ascontiguousarray(rcvr, dtype=np.float64)
+        A = Arrival()
+        steps = C.c_longlong(0)
+        ok = self.o.lib.go_eig_direct(self.h, C.byref(params), _ptr(s), _ptr(r), float(incl_est), float(az_est),
+                                      int(bounces), int(iterations), damping, tolerance, freq, C.byref(A),
+                                      C.byref(steps))
+        return (arrivals_to_array((Arrival * 1)(A), 1)[0] if ok else None), steps.value
 
     def eig_row(self, params, src, incl_deg, az_deg, bounces, freq=0.1):
         """3D: the row find_eigenray writes for an accepted eigenray with these

@@ -39,6 +39,23 @@ def test_rejected_arguments(args):
     assert r.returncode == 2, (r.stdout, r.stderr)
 
 
+@pytest.mark.parametrize("name", ["infraga-accel-sph", "infraga-accel-3d", "infraga-accel-sph-rngdep"])
+def test_accel_ranks_above_zero_exit_quietly(name, tmp_path):
+    """Under mpirun every rank starts the binary; ranks above 0 leave at once
+    (before any GPU or file work), rank 0 does the whole job."""
+    env = dict(os.environ, PMI_RANK="3", PMI_SIZE="8")
+    exe = os.path.join(BIN, name)
+    if not os.path.exists(exe):
+        pytest.skip(f"{exe} not built")
+    r = subprocess.run([exe, "-prop", "missing.met"], capture_output=True, text=True, cwd=tmp_path, env=env,
+                       timeout=60)
+    assert r.returncode == 0 and r.stdout == "" and r.stderr == ""
+    assert list(tmp_path.iterdir()) == []
+    r = subprocess.run([exe, "-prop", "missing.met", "no_such_key=1"], capture_output=True, text=True, cwd=tmp_path,
+                       env=dict(os.environ, OMPI_COMM_WORLD_RANK="0", OMPI_COMM_WORLD_SIZE="2"), timeout=60)
+    assert r.returncode == 2
+
+
 @pytest.fixture
 def toy_met(tmp_path, toyatmo_cols):
     path = tmp_path / "ToyAtmo.met"
@@ -63,6 +80,42 @@ def test_eig_search_known_answer_text(toy_met, tmp_path):
     rows = [l.split("\t") for l in open(tmp_path / "ToyAtmo.arrivals.dat") if l.strip() and not l.startswith("#")]
     assert [x[0] for x in rows] == ["34.559248", "4.1385204", "30.078055"]
     assert [x[2] for x in rows] == ["0", "1", "1"]
+    for i in range(3):
+        txt = open(tmp_path / f"ToyAtmo.eigenray-{i}.dat").read().split("\n")
+        assert txt[0] == "# lat [deg]\tlon [deg]\tz [km]\ttrans. coeff. [dB]\tabsorption [dB]\ttime [s]"
+        assert len([l for l in txt[1:] if l.strip()]) > 100
+
+
+@pytest.mark.gpu
+def test_eig_direct_text(toy_met, tmp_path):
+    """-eig_direct from an estimate near the reference's 1-bounce eigenray
+    converges to it (launch angles as -eig_search prints them, within the
+    acceptance tolerance) and writes eigenray-0.dat only."""
+    r = run("infraga-sph", "-eig_direct", toy_met, "src_lat=30.0", "src_lon=-100.0", "rcvr_lat=30.25",
+            "rcvr_lon=-104.25", "incl_est=4.0", "bounces=1", cwd=str(tmp_path))
+    assert r.returncode == 0, r.stderr
+    assert "Eigenray identified" in r.stdout
+    names = sorted(p.name for p in tmp_path.iterdir())
+    assert names == ["ToyAtmo.eigenray-0.dat", "ToyAtmo.met"]
+    rows = [l for l in open(tmp_path / "ToyAtmo.eigenray-0.dat") if l.strip() and not l.startswith("#")]
+    assert len(rows) > 100
+
+
+@pytest.mark.gpu
+def test_accel_launch_list_is_the_mpi_drivers(toy_met, tmp_path):
+    """infraga-accel-sph as rank 0 of 8: inclinations 0.1..45 step 0.1 give
+    450 launches per azimuth (the MPI driver's list; the serial loop gives
+    449), with the MPI driver's theta + step * j values."""
+    env = dict(os.environ, PMI_RANK="0", PMI_SIZE="8")
+    exe = os.path.join(BIN, "infraga-accel-sph")
+    r = subprocess.run([exe, "-prop", toy_met, "incl_min=0.1", "incl_max=45", "incl_step=0.1", "azimuth=-90",
+                        "bounces=0", "max_rng=60", "write_rays=false", "src_lat=30", "src_lon=-100"],
+                       capture_output=True, text=True, cwd=tmp_path, env=env, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert "450 rays" in r.stdout, r.stdout
+    r = run("infraga-sph", "-prop", toy_met, "incl_min=0.1", "incl_max=45", "incl_step=0.1", "azimuth=-90",
+            "bounces=0", "max_rng=60", "write_rays=false", "src_lat=30", "src_lon=-100", cwd=str(tmp_path))
+    assert r.returncode == 0 and "449 rays" in r.stdout, r.stdout
 
 
 @pytest.mark.parametrize("args", [

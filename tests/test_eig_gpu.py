@@ -152,3 +152,42 @@ def test_argument_errors(engine, base3d):
         engine.eig_search(base3d, prop_params(GA_GEOM_3D), eig_params(bnc_min=2, bnc_max=1), [(100.0, 0.0)])
     with pytest.raises(EngineError):
         engine.eig_search(base3d + 1000, prop_params(GA_GEOM_3D), eig_params(), [(100.0, 0.0)])
+
+
+DIRECT = [
+    # (incl_est, az_est, bounces): ToyAtmo, source 30 N 100 W, receiver 30.25 N 104.25 W
+    (4.0, float("nan"), 1),
+    (34.0, float("nan"), 0),
+    (30.5, -84.9, 1),
+    (10.0, float("nan"), 0),    # does not converge within 25 iterations (oracle alike)
+]
+
+
+@pytest.mark.parametrize("incl,az,nb", DIRECT, ids=["b1_4deg", "b0_34deg", "b1_30.5deg_az", "b0_10deg_none"])
+def test_eig_direct_matches_oracle(engine, oracle, toyatmo_units, incl, az, nb):
+    """-eig_direct (ga_eig_direct) against the oracle's go_eig_direct from the
+    same estimate: same outcome; launch angles within 1e-6 relative; the row
+    and its eigenray-0.dat rows against the oracle's accepted trace at the
+    GPU's launch angles (1e-6 relative, 0.01 dB)."""
+    from ga_oracle_py import GEOM_SPH, Medium
+    from parity import compare_paths
+    src, rcvr = (30.0, -100.0, 0.0), (30.25, -104.25)
+    aid = engine.load_atmo(*toyatmo_units)
+    row, pts, st = engine.eig_direct(aid, prop_params(GA_GEOM_SPH, src=src), eig_params(), rcvr, incl, az, nb)
+    m = Medium(oracle, GEOM_SPH, *toyatmo_units)
+    p = m.params(True)
+    ref, _ = m.eig_direct(p, src, rcvr, incl, az, nb)
+    engine.release_atmo(aid)
+    assert (row is None) == (ref is None)
+    if ref is None:
+        assert len(pts) == 0
+        return
+    assert abs(row["incl"] - ref["incl_launch"]) <= 1e-6 * abs(ref["incl_launch"])
+    assert abs(row["az"] - ref["az_launch"]) <= 1e-6 * 360.0
+    rp, rrow = m.eig_path(p, src, row["incl"], row["az"], nb)
+    for f in ("time", "turn_ht"):
+        assert abs(row[f] - rrow[f]) <= 1e-6 * abs(rrow[f]), f
+    for f in ("tl", "absorption"):
+        assert abs(row[f] - rrow[f]) <= 0.01, f
+    pf, ps = compare_paths(pts, rp, True, src=src)
+    assert not pf, (pf, ps)
