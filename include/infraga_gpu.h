@@ -28,6 +28,9 @@
  *                          -prop loop (<id>.raypaths.dat, <id>.caustics-N.dat)
  *                          src/main.infraga.sph.cpp:388-413, 456-496, 549;
  *                          src/main.infraga.3d.cpp:378-400, 442-487
+ *   ga_fan_run(_paths) with GA_GEOM_2D
+ *                       <- infraga-2d run_prop (src/main.infraga.2d.cpp:127-400)
+ *                          over geoac.eqset.2d.cpp
  *   ga_eig_direct       <- run_eig_direct: geoac::find_eigenray from a
  *                          launch estimate, src/main.infraga.3d.cpp:955-1086,
  *                          src/main.infraga.sph.cpp:998-1139
@@ -59,6 +62,10 @@ extern "C" {
 #define GA_GEOM_SPH_RNGDEP 2   /* spherical, range-dependent atmosphere and
                                   terrain (infraga-sph-rngdep); atmo_id is a
                                   ga_rngdep_load id */
+#define GA_GEOM_2D     3   /* 2D effective sound speed in the vertical plane
+                              of each launch azimuth (infraga-2d); arrivals:
+                              pos0 = range r [km], pos1 = out-of-plane drift
+                              of the last leg [km], back_az unused (0) */
 
 typedef struct ga_ctx ga_ctx;
 
@@ -85,6 +92,8 @@ typedef struct {
     double r_earth;        /* km; the reference hard-codes 6370 */
     int32_t topo_use_blw;  /* range-dependent: boundary-layer vertical wind
                               over terrain (topo_use_BLw, default 0) */
+    double refl_alt;       /* 2D: altitude of the elevated (partial) reflection
+                              [km]; NaN = none (profile top + 1 km) */
 } ga_prop_params;
 
 /* One row of <id>.arrivals.dat (src/main.infraga.sph.cpp:531-544). */
@@ -232,6 +241,18 @@ int ga_eig_search_paths(ga_ctx *ctx, int atmo_id, const ga_prop_params *p, const
                         const double *rcvr_xy, int n_rcvr,
                         ga_eigenray *out, int out_cap, int *n_out, double *rcvr_latency_ms,
                         ga_path_pt *pts, int64_t pts_cap, int64_t *n_pts, ga_run_stats *stats);
+
+/* Terrain profile z(r) [km] of the 2D geometry (infraga-2d topo_file=, read
+ * by set_region, src/atmo/atmo_io.2d.cpp:138-156, into topo::spline): n >= 3
+ * strictly increasing ranges.  Attached to a stratified atmosphere; used by
+ * GA_GEOM_2D runs on that id (ground, boundary layer and range limit follow
+ * it as set_limits / set_bndlyr do, atmo_io.2d.cpp:23-33,
+ * atmo_state.2d.cpp:34-45). */
+int ga_topo_load_2d(ga_ctx *ctx, int atmo_id, const double *r_km, const double *z_km, int n);
+
+/* topo::z, dz, ddz of the 2D geometry at range r [km] (atmo_state.2d.cpp:47-69):
+ * out = (z, dz/dr, d2z/dr2); without terrain (z0 of the profile bottom, 0, 0). */
+int ga_topo_eval_2d(ga_ctx *ctx, int atmo_id, double r_km, double out[3]);
 
 /* ---- Range-dependent atmosphere and topography --------------------------
  *

@@ -29,6 +29,7 @@
 #include <vector>
 
 #include "../../include/infraga_gpu.h"
+#include "ga_2d.hpp"
 #include "ga_eig_search.hpp"
 #include "ga_rdtrace.hpp"
 #include "ga_rngdep.hpp"
@@ -83,6 +84,24 @@ __global__ void __launch_bounds__(256) arrival_kernel(FanArgs A)
 }
 
 __global__ void reset_counter(unsigned int *q) { *q = 0u; }
+
+// 2D fans (infraga-2d): persistent lanes, the stratified table in LDS when it fits.
+#ifndef GA_WG_2D
+#define GA_WG_2D 512
+#endif
+template <int NEQ, bool LDS>
+__global__ void __launch_bounds__(GA_WG_2D) fan2d_kernel(Fan2DArgs A)
+{
+    extern __shared__ double lds_tab[];
+    Medium M = A.M;
+    if constexpr (LDS) {
+        const int nd = M.n * NODE;
+        for (int i = threadIdx.x; i < nd; i += blockDim.x) lds_tab[i] = A.M.tab[i];
+        __syncthreads();
+        M.tab = lds_tab;
+    }
+    lane_loop_2d<NEQ>(A, M, [&]() { return (int)atomicAdd(A.queue, 1u); });
+}
 
 // Range-dependent atmosphere and topography at n points (lat, lon [rad], z
 // [km]): per point 4 fields x {f, fx, fy, fz, fxx, fyy, fzz, fxy, fxz, fyz}
@@ -204,6 +223,8 @@ struct DevAtmo {
     double *d_tab = nullptr;
     int n = 0;
     std::vector<double> z, c, u, v, rho;
+    double *d_topo2d = nullptr;        // 2D terrain spline [n][3] (r, z, slope)
+    std::vector<double> topo_r, topo_z;
 };
 
 // Range-dependent atmosphere (hybrid-spline tables) and optional topography.
@@ -635,6 +656,108 @@ int rd_fan_enqueue(ga_ctx *ctx, FanBufs &fb, int rd_id, const ga_prop_params *p,
     return GA_OK;
 }
 
+// 2D fan (infraga-2d run_prop): always the write_rays bookkeeping; path
+// records only for the flags of `path` (nullptr: none).
+int fan2d_enqueue(ga_ctx *ctx, FanBufs &fb, int atmo_id, const ga_prop_params *p, const double *d_launches, int n,
+                  ga_arrival *d_out, int32_t *d_counts, int32_t *d_steps, hipStream_t st, const PathOut *path)
+{
+    auto it = ctx->atmos.find(atmo_id);
+    if (it == ctx->atmos.end()) return fail(ctx, GA_E_ATMO, "unknown atmosphere id %d", atmo_id);
+    if (p->bounces < 0) return fail(ctx, GA_E_ARG, "bounces must be >= 0");
+    if (!(p->ds_min > 0.0) || !(p->ds_max >= p->ds_min) || !(p->s_max > 0.0))
+        return fail(ctx, GA_E_ARG, "invalid step parameters ds_min=%g ds_max=%g s_max=%g", p->ds_min, p->ds_max, p->s_max);
+    const DevAtmo &a = it->second;
+    const bool topo = a.d_topo2d != nullptr;
+    Fan2DArgs A;
+    std::memset(&A, 0, sizeof(A));
+    Medium &M = A.M;
+    M.tab = a.d_tab;
+    M.n = a.n;
+    M.zlo = a.z.front();
+    M.zhi = a.z.back();
+    M.r0 = 0.0;
+    // set_limits / set_bndlyr (atmo_io.2d.cpp:23-33, atmo_state.2d.cpp:34-45);
+    // z_grnd= only without terrain (main.infraga.2d.cpp:192-198)
+    M.z0 = (!topo && !std::isnan(p->z_grnd)) ? p->z_grnd : a.z.front();
+    if (topo) {
+        double zm = 0.0;
+        for (double zz : a.topo_z) zm = std::max(zm, zz);
+        M.z_max = zm + 0.1;
+    } else {
+        M.z_max = M.z0;
+    }
+    M.zg = M.z0;
+    M.z_bndlyr = M.z_max + 2.0;
+    M.tweak = std::max(0.0, p->abs_coeff);
+    M.sbc[0] = std::pow(10.0, -0.67887);
+    M.sbc[1] = std::pow(10.0, -0.10744);
+    M.sbc[2] = std::pow(10.0, -3.3979);
+    A.T.tab = a.d_topo2d;
+    A.T.n = topo ? (int)a.topo_r.size() : 0;
+    A.s_max = p->s_max;
+    A.ds_min = p->ds_min;
+    A.ds_max = p->ds_max;
+    A.alt_max = std::isnan(p->alt_max) ? a.z.back() : p->alt_max;
+    A.rng_max = !std::isnan(p->rng_max) ? p->rng_max : (topo ? a.topo_r.back() : 2000.0);
+    A.z_reflect = std::isnan(p->refl_alt) ? a.z.back() + 1.0 : p->refl_alt;
+    // source at r = 0, raised to the ground there (:222)
+    double zg0 = M.z0;
+    if (topo) {
+        // natural spline of the terrain at r = 0 on the host (same Hermite form)
+        const std::vector<double> &r = a.topo_r, &zt = a.topo_z;
+        std::vector<double> sl;
+        spline_slopes(r, zt, sl);
+        double rr = std::min(std::max(0.0, r.front()), r.back());
+        size_t k = 0;
+        while (k + 2 < r.size() && rr > r[k + 1]) ++k;
+        const double dx = r[k + 1] - r[k], X = (rr - r[k]) / dx, d = zt[k + 1] - zt[k];
+        const double Aa = sl[k] * dx - d, B = -sl[k + 1] * dx + d;
+        zg0 = zt[k] + X * (d + (1.0 - X) * (Aa * (1.0 - X) + B * X));
+    }
+    A.z_src = std::max(p->src[2], zg0);
+    A.c_src = host_c_at(a, A.z_src);
+    {
+        // rho at the source: the same spline through the host copies
+        std::vector<double> sl;
+        spline_slopes(a.z, a.rho, sl);
+        const double zz = std::min(std::max(A.z_src, a.z.front()), a.z.back());
+        size_t k = 0;
+        while (k + 2 < a.z.size() && zz > a.z[k + 1]) ++k;
+        const double dx = a.z[k + 1] - a.z[k], X = (zz - a.z[k]) / dx, d = a.rho[k + 1] - a.rho[k];
+        const double Aa = sl[k] * dx - d, B = -sl[k + 1] * dx + d;
+        A.rho_src = a.rho[k] + X * (d + (1.0 - X) * (Aa * (1.0 - X) + B * X));
+    }
+    A.freq = p->freq;
+    A.bounces = p->bounces;
+    A.n_rays = n;
+    long long rows = (long long)(p->s_max / p->ds_min);
+    A.max_steps = (int)std::min<long long>(rows - 1, 2000000000LL);
+    A.launches = d_launches;
+    A.out = d_out;
+    A.counts = d_counts;
+    A.steps = d_steps;
+    if (path) A.path = *path;
+    if (!fb.queue && hipMalloc(&fb.queue, sizeof(unsigned int)) != hipSuccess)
+        return fail(ctx, GA_E_NOMEM, "queue counter");
+    A.queue = fb.queue;
+    if (n == 0) return GA_OK;
+    const size_t lds = (size_t)a.n * NODE * sizeof(double);
+    const bool use_lds = lds <= ctx->max_lds;
+    auto kern = p->calc_amp ? (use_lds ? fan2d_kernel<8, true> : fan2d_kernel<8, false>)
+                            : (use_lds ? fan2d_kernel<4, true> : fan2d_kernel<4, false>);
+    const size_t shm = use_lds ? lds : 0;
+    if (shm > 65536) HIPCHK(ctx, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+    int per_cu = 0;
+    HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, GA_WG_2D, shm));
+    per_cu = std::max(per_cu, 1);
+    const int need = (n + GA_WG_2D - 1) / GA_WG_2D;
+    const int grid = std::max(1, std::min(per_cu * ctx->n_cu, need));
+    hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, st, A.queue);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(GA_WG_2D), shm, st, A);
+    HIPCHK(ctx, hipGetLastError());
+    return GA_OK;
+}
+
 // path: write-mode bookkeeping with path records (nullptr: write_rays=false sums).
 int fan_enqueue(ga_ctx *ctx, FanBufs &fb, int atmo_id, const ga_prop_params *p, const double *d_launches, int n,
                 ga_arrival *d_out, int32_t *d_counts, int32_t *d_steps, hipStream_t st,
@@ -642,6 +765,8 @@ int fan_enqueue(ga_ctx *ctx, FanBufs &fb, int atmo_id, const ga_prop_params *p, 
 {
     if (p->geometry == GA_GEOM_SPH_RNGDEP)
         return rd_fan_enqueue(ctx, fb, atmo_id, p, d_launches, n, d_out, d_counts, d_steps, st, path);
+    if (p->geometry == GA_GEOM_2D)
+        return fan2d_enqueue(ctx, fb, atmo_id, p, d_launches, n, d_out, d_counts, d_steps, st, path);
     auto it = ctx->atmos.find(atmo_id);
     if (it == ctx->atmos.end()) return fail(ctx, GA_E_ATMO, "unknown atmosphere id %d", atmo_id);
     if (p->geometry != GA_GEOM_SPH && p->geometry != GA_GEOM_3D) return fail(ctx, GA_E_ARG, "bad geometry %d", p->geometry);
@@ -1461,6 +1586,16 @@ void ga_prop_params_default(ga_prop_params *p, int geometry)
     p->alt_max = NAN;
     p->time_max = 10.0;
     p->r_earth = 6370.0;
+    p->refl_alt = NAN;
+    if (geometry == GA_GEOM_2D) {
+        // run_prop of infraga-2d (main.infraga.2d.cpp:129-141, geoac.params.2d.cpp,
+        // atmo_io.2d.cpp:23-33): source at r = 0, range limit 2000 km (the
+        // terrain's end with terrain: NaN selects that default)
+        p->src[0] = 0.0; p->src[1] = 0.0; p->src[2] = 0.0;
+        p->rng_max = NAN;
+        for (double &b : p->bounds) b = NAN;
+        return;
+    }
     if (geometry == GA_GEOM_SPH) {
         p->src[0] = 30.0; p->src[1] = 0.0; p->src[2] = 0.0;
         p->rng_max = 1000.0;
@@ -1521,7 +1656,10 @@ void ga_ctx_destroy(ga_ctx *ctx)
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    for (auto &kv : ctx->atmos) (void)hipFree(kv.second.d_tab);
+    for (auto &kv : ctx->atmos) {
+        (void)hipFree(kv.second.d_tab);
+        if (kv.second.d_topo2d) (void)hipFree(kv.second.d_topo2d);
+    }
     for (auto &kv : ctx->rds) kv.second.release();
     ctx->fb.release();
     (void)hipStreamDestroy(ctx->stream);
@@ -1610,6 +1748,7 @@ int ga_atmo_release(ga_ctx *ctx, int atmo_id)
     HIPCHK(ctx, hipSetDevice(ctx->device));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     HIPCHK(ctx, hipFree(it->second.d_tab));
+    if (it->second.d_topo2d) HIPCHK(ctx, hipFree(it->second.d_topo2d));
     ctx->atmos.erase(it);
     return GA_OK;
 }
@@ -1980,6 +2119,59 @@ int ga_topo_load(ga_ctx *ctx, int rd_id, const double *lat_deg, int nlat, const 
     r.thy = y;
     r.t_max = 0.0;                                   // set_bndlyr starts from 0 (atmo_state.sph.rngdep.cpp:44-48)
     for (size_t q = 0; q < (size_t)nlat * nlon; ++q) r.t_max = std::max(r.t_max, zg_km[q]);
+    return GA_OK;
+}
+
+int ga_topo_load_2d(ga_ctx *ctx, int atmo_id, const double *r_km, const double *z_km, int n)
+{
+    if (!ctx || !r_km || !z_km) return fail(ctx, GA_E_ARG, "null argument");
+    auto it = ctx->atmos.find(atmo_id);
+    if (it == ctx->atmos.end()) return fail(ctx, GA_E_ATMO, "unknown atmosphere id %d", atmo_id);
+    if (n < 3) return fail(ctx, GA_E_ARG, "terrain needs at least 3 nodes");
+    if (!increasing(r_km, n)) return fail(ctx, GA_E_ARG, "terrain ranges must increase strictly");
+    DevAtmo &a = it->second;
+    a.topo_r.assign(r_km, r_km + n);
+    a.topo_z.assign(z_km, z_km + n);
+    std::vector<double> sl;
+    spline_slopes(a.topo_r, a.topo_z, sl);
+    std::vector<double> tab((size_t)n * 3);
+    for (int i = 0; i < n; ++i) {
+        tab[3 * i] = r_km[i];
+        tab[3 * i + 1] = z_km[i];
+        tab[3 * i + 2] = sl[i];
+    }
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (a.d_topo2d) HIPCHK(ctx, hipFree(a.d_topo2d));
+    a.d_topo2d = nullptr;
+    if (hipMalloc(&a.d_topo2d, tab.size() * sizeof(double)) != hipSuccess) return fail(ctx, GA_E_NOMEM, "terrain table");
+    HIPCHK(ctx, hipMemcpy(a.d_topo2d, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+    return GA_OK;
+}
+
+int ga_topo_eval_2d(ga_ctx *ctx, int atmo_id, double r, double out[3])
+{
+    if (!ctx || !out) return fail(ctx, GA_E_ARG, "null argument");
+    auto it = ctx->atmos.find(atmo_id);
+    if (it == ctx->atmos.end()) return fail(ctx, GA_E_ATMO, "unknown atmosphere id %d", atmo_id);
+    const DevAtmo &a = it->second;
+    if (a.topo_r.empty()) {
+        out[0] = a.z.front();
+        out[1] = 0.0;
+        out[2] = 0.0;
+        return GA_OK;
+    }
+    const std::vector<double> &x = a.topo_r, &f = a.topo_z;
+    std::vector<double> sl;
+    spline_slopes(x, f, sl);
+    const double rr = std::min(std::max(r, x.front()), x.back());
+    size_t k = 0;
+    while (k + 2 < x.size() && rr > x[k + 1]) ++k;
+    const double dx = x[k + 1] - x[k], X = (rr - x[k]) / dx, d = f[k + 1] - f[k];
+    const double A = sl[k] * dx - d, B = -sl[k + 1] * dx + d;
+    out[0] = f[k] + X * (d + (1.0 - X) * (A * (1.0 - X) + B * X));
+    out[1] = d / dx + (1.0 - 2.0 * X) * (A * (1.0 - X) + B * X) / dx + X * (1.0 - X) * (B - A) / dx;
+    out[2] = 2.0 * (B - 2.0 * A + (A - B) * 3.0 * X) / (dx * dx);
     return GA_OK;
 }
 

@@ -89,7 +89,7 @@ GA_HD void path_put(const PathOut &P, PathLane &c, int item, const ga_path_pt &r
 GA_HD void path_close(const PathOut &P, PathLane &c, int item)
 {
     if (c.n > 0 && c.chunk < P.max_chunks) P.chunk_fill[c.chunk] = c.fill;
-    P.item_pts[item] = c.n;
+    if (P.item_pts) P.item_pts[item] = c.n;
     path_reset(c);
 }
 

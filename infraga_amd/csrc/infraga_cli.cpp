@@ -715,6 +715,185 @@ int run_eig_search(int argc, char **argv)
     return 0;
 }
 
+// infraga-2d -prop (run_prop, main.infraga.2d.cpp:127-400): one azimuth,
+// the effective sound speed in its vertical plane.  Always writes
+// <id>.raypaths.dat; caustics-N.dat with write_caustics; all files at the
+// stream's default precision.
+int run_prop_2d(int argc, char **argv)
+{
+    double theta_min = 0.5, theta_max = 45.0, theta_step = 0.5, azimuth = -90.0;
+    double z_src = 0.0, freq = 0.1;
+    int bounces = 10;
+    bool write_caustics = false, reverse_winds = false, custom_id = false;
+    const char *prof_format = "zTuvdp";
+    const char *topo_file = nullptr;
+    const char *output_id = nullptr;
+    ga_prop_params p;
+    ga_prop_params_default(&p, GA_GEOM_2D);
+    double z_grnd = NAN;
+
+    for (int i = 3; i < argc; i++) {
+        const char *a = argv[i], *v = nullptr;
+        if (key2(a, "incl_min=", "min_incl=", v)) theta_min = std::atof(v);
+        else if (key2(a, "incl_max=", "max_incl=", v)) theta_max = std::atof(v);
+        else if (key(a, "incl_step=", v)) theta_step = std::atof(v);
+        else if (key(a, "inclination=", v)) { theta_min = theta_max = std::atof(v); theta_step = 1.0; }
+        else if (key(a, "azimuth=", v)) azimuth = std::atof(v);
+        else if (key(a, "bounces=", v)) bounces = std::atoi(v);
+        else if (key(a, "src_alt=", v)) z_src = std::atof(v);
+        else if (key(a, "freq=", v)) freq = std::atof(v);
+        else if (key(a, "abs_coeff=", v)) p.abs_coeff = std::max(0.0, std::atof(v));
+        else if (key(a, "write_caustics=", v)) write_caustics = str2bool(v);
+        else if (key(a, "calc_amp=", v)) p.calc_amp = str2bool(v) ? 1 : 0;
+        else if (key2(a, "max_rng=", "rng_max=", v)) p.rng_max = std::atof(v);
+        else if (key2(a, "max_alt=", "alt_max=", v)) p.alt_max = std::atof(v);
+        else if (key2(a, "max_ds=", "ds_max=", v)) p.ds_max = std::atof(v);
+        else if (key2(a, "min_ds=", "ds_min=", v)) p.ds_min = std::atof(v);
+        else if (key2(a, "max_s=", "s_max=", v)) p.s_max = std::atof(v);
+        else if (key(a, "write_atmo=", v)) { /* profile dump: not produced */ }
+        else if (key(a, "prof_format=", v)) prof_format = v;
+        else if (key(a, "reverse_winds=", v)) reverse_winds = str2bool(v);
+        else if (key2(a, "refl_alt=", "alt_refl=", v)) p.refl_alt = std::atof(v);
+        else if (key(a, "output_id=", v)) { custom_id = true; output_id = v; }
+        else if (key(a, "z_grnd=", v)) z_grnd = std::atof(v);
+        else if (key(a, "topo_file=", v)) topo_file = v;
+        else if (key(a, "print_resid=", v)) { /* diagnostics only */ }
+        else {
+            std::cerr << GA_CLI_NAME << ": unrecognized parameter entry: " << a << '\n';
+            return 2;
+        }
+    }
+    if (write_caustics) p.calc_amp = 1;                     // :215
+
+    ga_ctx *ctx = nullptr;
+    if (ga_ctx_create(&ctx, 0) != GA_OK) return die(nullptr, "no usable GPU");
+    int aid = 0;
+    if (ga_atmo_load_file(ctx, argv[2], prof_format, reverse_winds ? 1 : 0, &aid) != GA_OK)
+        return die(ctx, "cannot load the atmosphere");
+    if (topo_file) {
+        std::ifstream tf(topo_file);
+        if (!tf) return die(ctx, "cannot read the terrain file");
+        std::vector<double> r, zt;
+        std::string line;
+        while (std::getline(tf, line)) {
+            if (line.find('#') == 0) continue;
+            double a = 0.0, b = 0.0;
+            if (std::sscanf(line.c_str(), "%lf %lf", &a, &b) == 2) { r.push_back(a); zt.push_back(b); }
+        }
+        if (ga_topo_load_2d(ctx, aid, r.data(), zt.data(), (int)r.size()) != GA_OK) return die(ctx, "terrain");
+    }
+    const double zb = profile_bottom(argv[2]);
+    const double z0 = (!topo_file && !std::isnan(z_grnd)) ? z_grnd : zb;
+    p.z_grnd = topo_file ? NAN : z0;
+    double t0[3];
+    if (ga_topo_eval_2d(ctx, aid, 0.0, t0) != GA_OK) return die(ctx, "terrain");
+    const double zg_src = topo_file ? t0[0] : z0;
+    z_src = std::max(z_src, zg_src);                        // :222
+    p.src[2] = z_src;
+    p.bounces = bounces;
+    p.freq = freq;
+    // profile top: the altitude limit and, by default, the reflection altitude above it
+    double z_top = zb;
+    {
+        std::ifstream probe(argv[2]);
+        std::string line;
+        while (std::getline(probe, line)) {
+            if (line.find('#') == 0) continue;
+            double zz;
+            if (std::sscanf(line.c_str(), "%lf", &zz) == 1) z_top = zz;
+        }
+    }
+    const double alt_max = std::isnan(p.alt_max) ? z_top : p.alt_max;
+    const double z_reflect = std::isnan(p.refl_alt) ? z_top + 1.0 : p.refl_alt;
+
+    // launch list (:310-318)
+    double theta_grnd;
+    if (z_src - t0[0] < 0.01) theta_grnd = topo_file ? std::atan(t0[1]) * (180.0 / Pi) + 0.1 : 0.1;
+    else theta_grnd = -89.9;
+    std::vector<double> launches;
+    for (double theta = std::max(theta_min, theta_grnd); theta <= std::max(theta_max, theta_grnd); theta += theta_step) {
+        launches.push_back(theta);
+        launches.push_back(azimuth);
+    }
+    const int n = (int)(launches.size() / 2);
+    std::string id = custom_id ? std::string(output_id) : strip_ext(argv[2]);
+
+    std::ofstream results(id + ".arrivals.dat");
+    results << "# infraga-2d -prop summary:" << '\n';
+    results << "#" << '\t' << "profile: " << argv[2] << '\n';
+    results << "#" << '\t' << "inclination: " << theta_min << ", " << theta_max << ", " << theta_step << '\n';
+    results << "#" << '\t' << "azimuth: " << azimuth << '\n';
+    results << "#" << '\t' << "bounces: " << bounces << '\n';
+    results << "#" << '\t' << "source location: " << 0.0 << ", " << z_src << '\n';
+    if (!topo_file) results << "#" << '\t' << "ground elevation: " << z0 << '\n';
+    else results << "#" << '\t' << "topo file:" << topo_file << '\n';
+    if (z_reflect < alt_max) results << '#' << '\t' << "reflection altitude: " << z_reflect << '\n';
+    results << "#" << '\t' << "frequency: " << freq << '\n';
+    results << "#" << '\t' << "S&B atten coeff: " << p.abs_coeff << '\n' << '\n';
+    results << "# incl [deg]" << '\t' << "az [deg]" << '\t' << "n_b" << '\t' << "r_0 [km]" << '\t' << "time [s]" << '\t'
+            << "celerity [km/s]" << '\t' << "turning ht [km]" << '\t' << "arrival incl. [deg]" << '\t'
+            << "trans. coeff. [dB]" << '\t' << "absorption [dB]" << '\t' << "perp. dist. [km]" << '\n';
+    std::ofstream raypath(id + ".raypaths.dat");
+    raypath << "# r [km]" << '\t' << "z [km]" << '\t' << "trans. coeff. [dB]" << '\t' << "absorption [dB]" << '\t'
+            << "time [s]" << '\n';
+    std::vector<std::ofstream> caus;
+    if (write_caustics) {
+        for (int b = 0; b <= bounces; ++b) {
+            caus.emplace_back(id + ".caustics-" + std::to_string(b) + ".dat");
+            caus.back() << "# r [km]" << '\t' << "z [km]" << '\t' << "time [s]" << '\n';
+        }
+    }
+    const int slots = bounces + 1;
+    const int flags = GA_PATH_RAYS | (write_caustics ? GA_PATH_CAUSTICS : 0);
+    std::vector<ga_arrival> arr;
+    std::vector<ga_path_pt> pts;
+    int64_t cap = (int64_t)1 << 21;
+    ga_run_stats tot{};
+    int n_arr = 0;
+    for (int r0 = 0; r0 < n; r0 += 4096) {
+        const int nb = std::min(4096, n - r0);
+        arr.assign((size_t)nb * slots, ga_arrival{});
+        int nb_out = 0;
+        int64_t n_pts = 0;
+        ga_run_stats bs;
+        int rc;
+        for (;;) {
+            pts.resize((size_t)cap);
+            rc = ga_fan_run_paths(ctx, aid, &p, launches.data() + 2 * (size_t)r0, nb, flags, arr.data(), nb * slots,
+                                  &nb_out, pts.data(), cap, &n_pts, &bs);
+            if (rc == GA_E_CAP && n_pts > cap) { cap = n_pts + n_pts / 4; continue; }
+            break;
+        }
+        if (rc != GA_OK) return die(ctx, "fan failed");
+        tot.ray_steps += bs.ray_steps;
+        tot.kernel_ms += bs.kernel_ms;
+        n_arr += nb_out;
+        int w = 0;
+        int64_t q = 0;
+        for (int r = 0; r < nb; ++r) {
+            for (; q < n_pts && pts[q].launch_idx == r; ++q) {
+                const ga_path_pt &P = pts[q];
+                if (P.kind == GA_PT_RAY) {
+                    raypath << P.pos0 << '\t' << P.z << '\t' << P.tl << '\t' << P.absorption << '\t' << P.time << '\n';
+                } else if (write_caustics && P.leg >= 0 && P.leg < (int)caus.size()) {
+                    caus[P.leg] << P.pos0 << '\t' << P.z << '\t' << P.time << '\n' << '\n';
+                }
+            }
+            raypath << '\n';
+            for (; w < nb_out && arr[w].launch_idx == r; ++w) {
+                const ga_arrival &A = arr[w];
+                results << A.incl << '\t' << A.az << '\t' << A.n_bounce << '\t' << A.pos0 << '\t' << A.time << '\t'
+                        << A.cel << '\t' << A.turn_ht << '\t' << A.incl_arr << '\t' << A.tl << '\t' << A.absorption
+                        << '\t' << A.pos1 << '\n';
+            }
+        }
+    }
+    std::cout << GA_CLI_NAME << ": " << n << " rays, " << n_arr << " arrivals, " << tot.ray_steps << " ray-steps, "
+              << tot.kernel_ms << " ms on the GPU\n";
+    ga_ctx_destroy(ctx);
+    return 0;
+}
+
 // -eig_direct (run_eig_direct, main.infraga.3d.cpp:955-1086,
 // main.infraga.sph.cpp:998-1139): find_eigenray from a launch estimate;
 // writes <id>.eigenray-0.dat when it converges (the reference opens no
@@ -845,6 +1024,11 @@ int main(int argc, char **argv)
     // under mpirun every rank starts a copy: rank 0 does the whole job on the GPU
     if (mpi_rank() > 0) return 0;
     if (argc < 3) { usage(); return argc < 2 ? 0 : 2; }
+    if (GA_CLI_GEOM == GA_GEOM_2D) {
+        if (std::strncmp(argv[1], "-prop", 5) == 0) return run_prop_2d(argc, argv);
+        std::cerr << GA_CLI_NAME << ": option " << argv[1] << " is not supported by this engine\n";
+        return 2;
+    }
     if (GA_CLI_GEOM == GA_GEOM_SPH_RNGDEP) {
         if (std::strncmp(argv[1], "-prop", 5) == 0) return run_prop_rd(argc, argv);
         std::cerr << GA_CLI_NAME << ": option " << argv[1] << " is not supported by this engine\n";

@@ -15,7 +15,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libinfraga_gpu.so")
 
-GA_GEOM_3D, GA_GEOM_SPH, GA_GEOM_SPH_RNGDEP = 0, 1, 2
+GA_GEOM_3D, GA_GEOM_SPH, GA_GEOM_SPH_RNGDEP, GA_GEOM_2D = 0, 1, 2, 3
 GA_OK, GA_E_ARG, GA_E_HIP, GA_E_NOMEM, GA_E_CAP, GA_E_NODEV, GA_E_ATMO, GA_E_IO = 0, -1, -2, -3, -4, -5, -6, -7
 
 GA_REQ_FAN, GA_REQ_EIG = 0, 1
@@ -27,7 +27,7 @@ EXPORTED = ["ga_prop_params_default", "ga_eig_params_default", "ga_ctx_create", 
             "ga_fan_run_device", "ga_eig_search", "ga_server_create", "ga_server_submit", "ga_server_poll",
             "ga_server_stats", "ga_server_destroy", "ga_rngdep_load", "ga_topo_load", "ga_rngdep_eval",
             "ga_rngdep_release", "ga_rngdep_load_files", "ga_rngdep_info", "ga_fan_run_paths",
-            "ga_eig_search_paths", "ga_eig_direct"]
+            "ga_eig_search_paths", "ga_eig_direct", "ga_topo_load_2d", "ga_topo_eval_2d"]
 
 GA_PATH_RAYS, GA_PATH_CAUSTICS = 1, 2
 GA_PT_RAY, GA_PT_CAUSTIC = 0, 1
@@ -41,7 +41,7 @@ class PropParams(C.Structure):
                 ("src", C.c_double * 3), ("z_grnd", C.c_double), ("s_max", C.c_double),
                 ("ds_min", C.c_double), ("ds_max", C.c_double), ("alt_max", C.c_double),
                 ("rng_max", C.c_double), ("time_max", C.c_double), ("bounds", C.c_double * 4),
-                ("r_earth", C.c_double), ("topo_use_blw", C.c_int32)]
+                ("r_earth", C.c_double), ("topo_use_blw", C.c_int32), ("refl_alt", C.c_double)]
 
 
 class EigParams(C.Structure):
@@ -127,6 +127,8 @@ def load_library(path: str | None = None):
     L.ga_eig_search_paths.argtypes = [C.c_void_p, C.c_int, C.POINTER(PropParams), C.POINTER(EigParams), _dp,
                                       C.c_int, C.POINTER(Arrival), C.c_int, C.POINTER(C.c_int), _dp,
                                       C.POINTER(PathPt), C.c_int64, C.POINTER(C.c_int64), C.POINTER(RunStats)]
+    L.ga_topo_load_2d.argtypes = [C.c_void_p, C.c_int, _dp, _dp, C.c_int]
+    L.ga_topo_eval_2d.argtypes = [C.c_void_p, C.c_int, C.c_double, _dp]
     L.ga_eig_direct.argtypes = [C.c_void_p, C.c_int, C.POINTER(PropParams), C.POINTER(EigParams), _dp, C.c_double,
                                 C.c_double, C.c_int, C.POINTER(Arrival), C.POINTER(C.c_int), C.POINTER(PathPt),
                                 C.c_int64, C.POINTER(C.c_int64), C.POINTER(RunStats)]
@@ -257,6 +259,12 @@ class Engine:
         self._check(self.L.ga_fan_run_device(self.h, aid, C.byref(params), C.c_void_p(d_launches), n,
                                              C.c_void_p(d_out), C.c_void_p(d_counts), C.c_void_p(d_steps),
                                              C.c_void_p(stream) if stream else None))
+
+    def topo_load_2d(self, aid, r_km, z_km):
+        """Terrain profile z(r) for GA_GEOM_2D runs on a stratified atmosphere."""
+        a = [np.ascontiguousarray(q, dtype=np.float64) for q in (r_km, z_km)]
+        self._check(self.L.ga_topo_load_2d(self.h, aid, a[0].ctypes.data_as(_dp), a[1].ctypes.data_as(_dp),
+                                           len(a[0])))
 
     def rngdep_load(self, lat_deg, lon_deg, z_km, c, u, v, rho) -> int:
         """Range-dependent atmosphere: node lat/lon [deg], z [km], fields [nlat][nlon][nz]."""

@@ -1888,3 +1888,396 @@ void go_h_tables(void *h, int which, double *x, double *f, double *slope)
     go_spline *sp[4] = {&a->c, &a->u, &a->v, &a->rho};
     for (int i = 0; i < a->n; i++) { x[i] = a->z[i]; f[i] = sp[which]->f[i]; slope[i] = sp[which]->slope[i]; }
 }
+
+#ifndef GO_OPCOUNT
+/* ======================================================================
+ * 2D effective-sound-speed geometry: infraga-2d -prop
+ * (geoac.eqset.2d.cpp, atmo_state.2d.cpp, atmo_io.2d.cpp:27-33 set_limits,
+ * main.infraga.2d.cpp:127-400).  Equations: r, z, nu_r, nu_z and, with
+ * amplitudes, R, Z, mu_r, mu_z; c_eff = c + u cos(phi) + v sin(phi) with no
+ * boundary-layer wind scaling.
+ * ==================================================================== */
+static double g2_topo_z(go_2d *g, double r)           /* atmo_state.2d.cpp:47-53 */
+{
+    if (!g->is_topo) return g->a->z0;
+    return go_spline_f(&g->topo, in_interval(r, g->topo.x[0], g->topo.x[g->topo.n - 1]));
+}
+static double g2_topo_dz(go_2d *g, double r)          /* :55-61 */
+{
+    if (!g->is_topo) return 0.0;
+    double f, df, ddf;
+    go_spline_all(&g->topo, in_interval(r, g->topo.x[0], g->topo.x[g->topo.n - 1]), &f, &df, &ddf);
+    return df;
+}
+static double g2_topo_ddz(go_2d *g, double r)         /* :63-69 */
+{
+    if (!g->is_topo) return 0.0;
+    double f, df, ddf;
+    go_spline_all(&g->topo, in_interval(r, g->topo.x[0], g->topo.x[g->topo.n - 1]), &f, &df, &ddf);
+    return ddf;
+}
+static double g2_pt(go_2d *g, go_spline *s, double z)   /* atmo::c/u/v/rho, :75-131 */
+{
+    go_atmo *a = g->a;
+    return go_spline_f(s, in_interval(in_interval(z, a->z[0], a->z[a->n - 1]), a->z[0], a->z[a->n - 1]));
+}
+static double g2_dpt(go_2d *g, go_spline *s, double z)  /* atmo::dc/du/dv with n = 2 */
+{
+    go_atmo *a = g->a;
+    double f, df, ddf;
+    go_spline_all(s, in_interval(in_interval(z, a->z[0], a->z[a->n - 1]), a->z[0], a->z[a->n - 1]), &f, &df, &ddf);
+    return df;
+}
+static double g2_ceff(go_2d *g, double z)
+{
+    return g2_pt(g, &g->a->c, z) + g2_pt(g, &g->a->u, z) * cos(g->phi) + g2_pt(g, &g->a->v, z) * sin(g->phi);
+}
+
+void go2_init(go_2d *g, go_atmo *a)
+{
+    memset(g, 0, sizeof(*g));
+    g->a = a;
+    a->z0 = a->z[0];                                  /* set_limits (atmo_io.2d.cpp:27-33) */
+    g->alt_max = a->z[a->n - 1];
+    g->rng_max = 2000.0;
+    g->z_reflect = g->alt_max + 1.0;
+    a->z_max = a->z0;                                 /* set_bndlyr without terrain (:34-45) */
+    a->z_bndlyr = a->z_max + 2.0;
+    g->s_max = 1000.0; g->ds_min = 0.001; g->ds_max = 0.05;   /* geoac.params.2d.cpp */
+    g->calc_amp = 1;
+}
+
+int go2_set_topo(go_2d *g, const double *r, const double *z, int n)
+{
+    if (n < 3) return -1;
+    g->topo.n = n;
+    g->topo.x = (double *)malloc(sizeof(double) * (size_t)n);
+    g->topo.f = (double *)malloc(sizeof(double) * (size_t)n);
+    g->topo.slope = (double *)malloc(sizeof(double) * (size_t)n);
+    g->topo.accel = 0;
+    memcpy((double *)g->topo.x, r, sizeof(double) * (size_t)n);
+    memcpy(g->topo.f, z, sizeof(double) * (size_t)n);
+    go_spline_set(&g->topo);
+    g->is_topo = 1;
+    g->rng_max = r[n - 1];                            /* set_limits with terrain */
+    double zm = 0.0;                                  /* set_bndlyr with terrain */
+    for (int i = 0; i < n; i++) zm = zm > z[i] ? zm : z[i];
+    g->a->z_max = zm + 0.1;
+    g->a->z_bndlyr = g->a->z_max + 2.0;
+    return 0;
+}
+
+void go2_set_ground(go_2d *g, double z0)            /* main.infraga.2d.cpp z_grnd= */
+{
+    if (g->is_topo) return;
+    g->a->z0 = z0;
+    g->a->z_max = z0;
+    g->a->z_bndlyr = z0 + 2.0;
+}
+
+void go2_free(go_2d *g)
+{
+    if (g->is_topo) {
+        free((double *)g->topo.x);
+        free(g->topo.f);
+        free(g->topo.slope);
+    }
+    g->is_topo = 0;
+}
+
+/* update_refs + eval_src_eq (geoac.eqset.2d.cpp:176-239) at state y into f. */
+static void g2_rhs(go_2d *g, const double *y, double *f)
+{
+    go_atmo *a = g->a;
+    double z_eval = in_interval(y[1], a->z[0], a->z[a->n - 1]);
+    double c, dc, ddc, u, du, ddu, v, dv, ddv;
+    go_spline_all(&a->c, z_eval, &c, &dc, &ddc);
+    go_spline_all(&a->u, z_eval, &u, &du, &ddu);
+    go_spline_all(&a->v, z_eval, &v, &dv, &ddv);
+    double ce = c + u * cos(g->phi) + v * sin(g->phi);
+    double dce = dc + du * cos(g->phi) + dv * sin(g->phi);
+    double ddce = ddc + ddu * cos(g->phi) + ddv * sin(g->phi);
+    double nu_r = y[2], nu_z = y[3], c0 = g->c_eff0;
+    double dc_dr = 0.0, dc_dz = dce;
+    f[0] = ce / c0 * nu_r;
+    f[1] = ce / c0 * nu_z;
+    f[2] = -c0 / pow(ce, 2) * dc_dr;
+    f[3] = -c0 / pow(ce, 2) * dc_dz;
+    if (g->calc_amp) {
+        double R = y[4], Z = y[5], mu_r = y[6], mu_z = y[7];
+        double ddc_ddr = 0.0, ddc_ddz = ddce, ddc_drdz = 0.0;
+        double dc_dth = dc_dr * R + dc_dz * Z;
+        double ddc_drdth = ddc_ddr * R + ddc_drdz * Z;
+        double ddc_dzdth = ddc_drdz * R + ddc_ddz * Z;
+        f[4] = dc_dth / c0 * nu_r + ce / c0 * mu_r;
+        f[5] = dc_dth / c0 * nu_z + ce / c0 * mu_z;
+        f[6] = 2.0 * c0 / pow(ce, 3) * dc_dr * dc_dth - c0 / pow(ce, 2) * ddc_drdth;
+        f[7] = 2.0 * c0 / pow(ce, 3) * dc_dz * dc_dth - c0 / pow(ce, 2) * ddc_dzdth;
+    }
+}
+
+static double g2_set_ds(go_2d *g, const double *y)   /* :165-171 */
+{
+    double result = g->ds_max;
+    if (y[1] < g->a->z_bndlyr) result -= (g->ds_max - g->ds_min) * exp(-(y[1] - g2_topo_z(g, y[0])) / 0.2);
+    return result;
+}
+
+/* rk4solver.cpp:14-85 with the 2D checks (geoac.eqset.2d.cpp:258-281):
+ * break r > rng_max or z > alt_max; ground z <= z_max and z < topo::z(r);
+ * elevated reflection z > z_reflect. */
+int go2_prop_rk4(go_2d *g, double *rows, int length, int *check)
+{
+    const int n = g->calc_amp ? 8 : 4;
+    double t1[8], t2[8], t3[8], t4[8], p1[8], p2[8], p3[8], f[8];
+    double s = 0.0, ds;
+    int k;
+    *check = 0;
+    for (k = 0; k < length - 1; k++) {
+        const double *y = rows + 8 * k;
+        double *y1 = rows + 8 * (k + 1);
+        ds = g2_set_ds(g, y);
+        s += ds;
+        g2_rhs(g, y, f);
+        for (int i = 0; i < n; i++) { t1[i] = ds * f[i]; p1[i] = y[i] + t1[i] / 2.0; }
+        g2_rhs(g, p1, f);
+        for (int i = 0; i < n; i++) { t2[i] = ds * f[i]; p2[i] = y[i] + t2[i] / 2.0; }
+        g2_rhs(g, p2, f);
+        for (int i = 0; i < n; i++) { t3[i] = ds * f[i]; p3[i] = y[i] + t3[i]; }
+        g2_rhs(g, p3, f);
+        for (int i = 0; i < n; i++) {
+            t4[i] = ds * f[i];
+            y1[i] = y[i] + t1[i] / 6.0 + t2[i] / 3.0 + t3[i] / 3.0 + t4[i] / 6.0;
+        }
+        if (s > g->s_max) { *check = 1; break; }
+        if (y1[0] > g->rng_max || y1[1] > g->alt_max) { *check = 1; break; }
+        if (y1[1] <= g->a->z_max && y1[1] < g2_topo_z(g, y1[0])) { *check = 0; break; }
+        if (y1[1] > g->z_reflect) { *check = 0; break; }
+    }
+    return k + 1;
+}
+
+/* set_initial (:46-68). */
+void go2_set_initial(go_2d *g, double *y, double r0, double z0)
+{
+    g->z_src = z0;
+    g->c_eff0 = g2_ceff(g, z0);
+    memset(y, 0, sizeof(double) * 8);
+    y[0] = r0; y[1] = z0;
+    y[2] = cos(g->theta); y[3] = sin(g->theta);
+    if (g->calc_amp) { y[4] = 0.0; y[5] = 0.0; y[6] = -sin(g->theta); y[7] = cos(g->theta); }
+}
+
+/* approx_intercept + set_refl (:73-160); rk2, rk1 = rows k-2, k-1; out = row 0.
+ * dC1 uses the C2 of this reflection (the reference reads C2 before setting
+ * it, an uninitialised value). */
+void go2_set_refl(go_2d *g, const double *rk2, const double *rk1, double *out)
+{
+    const int n = g->calc_amp ? 8 : 4;
+    double prev[8];
+    if (rk1[1] < g->a->z_bndlyr) {
+        double dz_grnd = g2_topo_z(g, rk1[0]) - rk1[1];
+        double dz_k = (rk1[1] - rk2[1]) - g2_topo_dz(g, rk1[0]) * (rk1[0] - rk2[0]);
+        for (int i = 0; i < n; i++) prev[i] = rk1[i] + (rk1[i] - rk2[i]) / dz_k * dz_grnd;
+    } else {
+        double dz_grnd = g->z_reflect - rk1[1];
+        double dz_k = (rk1[1] - rk2[1]);
+        for (int i = 0; i < n; i++) prev[i] = rk1[i] + (rk1[i] - rk2[i]) / dz_k * dz_grnd;
+    }
+    double zg, c_eff, c_eff_diff, C1, C2, dC1, dC2, ds0_dtheta = 0.0;
+    go_atmo *a = g->a;
+    if (prev[1] < a->z_bndlyr) {
+        zg = g2_topo_z(g, prev[0]);
+        c_eff = g2_pt(g, &a->c, zg) + g2_pt(g, &a->u, zg) * cos(g->phi) + g2_pt(g, &a->v, zg) * sin(g->phi);
+        c_eff_diff = g2_dpt(g, &a->c, zg) + g2_dpt(g, &a->u, zg) * cos(g->phi) + g2_dpt(g, &a->v, zg) * sin(g->phi);
+        if (g->is_topo) {
+            double ta = g2_topo_dz(g, prev[0]);
+            double norm = 1.0 + pow(ta, 2);
+            double da_dtheta = (prev[4] * prev[3] - prev[5] * prev[2]) / (prev[3] - ta * prev[2]) * g2_topo_ddz(g, prev[0]);
+            C1 = (1.0 - pow(ta, 2)) / norm;
+            C2 = 2.0 * ta / norm;
+            dC1 = -2.0 * da_dtheta / norm * C2;
+            dC2 = 2.0 * da_dtheta / norm * C1;
+            ds0_dtheta = -g->c_eff0 / c_eff * (prev[5] - ta * prev[4]) / (prev[3] - ta * prev[2]);
+        } else {
+            C1 = 1.0; dC1 = 0.0; C2 = 0.0; dC2 = 0.0;
+            ds0_dtheta = -g->c_eff0 / c_eff * prev[5] / prev[3];
+        }
+    } else {
+        zg = g->z_reflect;
+        c_eff = g2_pt(g, &a->c, zg) + g2_pt(g, &a->u, zg) * cos(g->phi) + g2_pt(g, &a->v, zg) * sin(g->phi);
+        c_eff_diff = g2_dpt(g, &a->c, zg) + g2_dpt(g, &a->u, zg) * cos(g->phi) + g2_dpt(g, &a->v, zg) * sin(g->phi);
+        C1 = 1.0; dC1 = 0.0; C2 = 0.0; dC2 = 0.0;
+        ds0_dtheta = -g->c_eff0 / c_eff * prev[5] / prev[3];
+    }
+    double dnur_ds = 0.0, dnuz_ds = -g->c_eff0 / pow(c_eff, 2) * c_eff_diff;
+    memset(out, 0, sizeof(double) * 8);
+    out[0] = prev[0];
+    out[1] = zg;
+    out[2] = C1 * prev[2] + C2 * prev[3];
+    out[3] = -C1 * prev[3] + C2 * prev[2];
+    if (g->calc_amp) {
+        out[4] = C1 * prev[4] + C2 * prev[5];
+        out[5] = -C1 * prev[5] + C2 * prev[4];
+        out[6] = C1 * prev[6] + C2 * prev[7] + dC1 * prev[2] + dC2 * prev[3] + ((C1 - 1.0) * dnur_ds + C2 * dnuz_ds) * ds0_dtheta;
+        out[7] = -C1 * prev[7] + C2 * prev[6] - dC1 * prev[3] + dC2 * prev[2] + (-(C1 + 1.0) * dnuz_ds + C2 * dnur_ds) * ds0_dtheta;
+    }
+}
+
+double go2_travel_time_seg(go_2d *g, const double *y0, const double *y1)   /* :283-296 */
+{
+    double dr = y1[0] - y0[0], dz = y1[1] - y0[1];
+    double z_avg = y0[1] + dz / 2.0;
+    double c_eff = g2_ceff(g, z_avg);
+    double ds = sqrt(pow(dr, 2) + pow(dz, 2));
+    return ds / c_eff;
+}
+
+double go2_atten_seg(go_2d *g, const double *y0, const double *y1, double freq)   /* :313-326 */
+{
+    double dr = y1[0] - y0[0], dz = y1[1] - y0[1];
+    double z_avg = y0[1] + dz / 2.0;
+    double ds = sqrt(pow(dr, 2) + pow(dz, 2));
+    go_atmo *a = g->a;
+    return go_sb_alpha_at(g2_pt(g, &a->c, z_avg), g2_pt(g, &a->rho, z_avg), z_avg, 0, a->tweak_abs, freq) * ds;
+}
+
+double go2_jacobian(go_2d *g, const double *y)        /* :345-351 */
+{
+    double r = y[0];
+    double drds = g2_pt(g, &g->a->c, y[1]) / g->c_eff0 * y[2], drdth = y[4];
+    double dzds = g2_pt(g, &g->a->c, y[1]) / g->c_eff0 * y[3], dzdth = y[5];
+    return r * (drds * dzdth - dzds * drdth);
+}
+
+double go2_amp(go_2d *g, const double *y)             /* :354-361 */
+{
+    double z = y[1];
+    double num = g2_pt(g, &g->a->rho, z) * g2_pt(g, &g->a->c, z) * cos(g->theta);
+    double den = g2_pt(g, &g->a->rho, g->z_src) * g->c_eff0 * go2_jacobian(g, y);
+    return sqrt(fabs(num / den));
+}
+
+/* est_dev (:366-380): out-of-plane drift over rows 0..k of the leg. */
+double go2_est_dev(go_2d *g, const double *rows, int k)
+{
+    double dev = 0.0;
+    go_atmo *a = g->a;
+    for (int n = 0; n < k; n++) {
+        const double *y0 = rows + 8 * n, *y1 = rows + 8 * (n + 1);
+        double ds = sqrt(pow(y1[0] - y0[0], 2) + pow(y1[1] - y0[1], 2));
+        double z_avg = (y1[1] + y0[1]) / 2.0;
+        double c_eff = g2_ceff(g, z_avg);
+        double uu = g2_pt(g, &a->u, z_avg), vv = g2_pt(g, &a->v, z_avg);
+        double v_perp = -uu * sin(g->phi) + vv * cos(g->phi);
+        dev += v_perp / c_eff * 1.0 / sqrt(1.0 + 2.0 * v_perp / c_eff + (pow(uu, 2) + pow(vv, 2)) / pow(c_eff, 2)) * ds;
+    }
+    return dev;
+}
+
+/* run_prop of infraga-2d (main.infraga.2d.cpp:318-395): raypath rows of
+ * every leg (always written), caustics when flags & GO_PATH_CAUSTICS,
+ * elevated reflections above the boundary layer not counted as bounces.
+ * Arrival rows: pos0 = r, pos1 = perpendicular drift of the last leg,
+ * back_az = 0. */
+int go2_fan(go_2d *g, const double *launches, int n_launch, int bounces, double src_alt, double freq, int flags,
+            go_arrival *out, int out_cap, int *n_out, go_path_pt *pts, long long pts_cap, long long *n_pts,
+            long long *ray_steps)
+{
+    const int caus = (flags & GO_PATH_CAUSTICS) != 0 && g->calc_amp;
+    int length = (int)(g->s_max / g->ds_min);
+    double *rows = get_rows(length);          /* 18 doubles per row: room for 8 */
+    double *r8 = rows;
+    int cnt = 0;
+    long long steps = 0, np = 0;
+    const double r_src = 0.0;
+    double z_src = src_alt > g2_topo_z(g, r_src) ? src_alt : g2_topo_z(g, r_src);   /* :222 */
+    for (int li = 0; li < n_launch; li++) {
+        double incl = launches[2 * li], az = launches[2 * li + 1];
+        g->theta = incl * GO_PI / 180.0;
+        g->phi = GO_PI / 2.0 - az * GO_PI / 180.0;
+        go2_set_initial(g, r8, r_src, z_src);
+        double tt = 0.0, att = 0.0, zmax = 0.0;
+        int brk = 0, k;
+        for (int bnc = 0; bnc <= bounces; bnc++) {
+            k = go2_prop_rk4(g, r8, length, &brk);
+            steps += k;
+            double D = 0.0, D_prev = caus ? go2_jacobian(g, r8 + 8) : 0.0;
+            for (int m = 1; m < k; m++) {
+                const double *ym = r8 + 8 * m;
+                tt += go2_travel_time_seg(g, r8 + 8 * (m - 1), ym);
+                att += go2_atten_seg(g, r8 + 8 * (m - 1), ym, freq);
+                if (caus) D = go2_jacobian(g, ym);
+                if (m == 1 || m % 15 == 0) {
+                    if (np < pts_cap) {
+                        go_path_pt *P = pts + np;
+                        double zt = g2_topo_z(g, ym[0]);
+                        P->launch_idx = li; P->leg = bnc; P->step = m; P->kind = 0;
+                        P->pos0 = ym[0]; P->pos1 = 0.0; P->z = ym[1] > zt ? ym[1] : zt;
+                        P->tl = g->calc_amp ? 20.0 * log10(go2_amp(g, ym)) : 0.0;
+                        P->absorption = -2.0 * att; P->time = tt;
+                    }
+                    np++;
+                }
+                if (caus && D * D_prev < 0.0) {
+                    if (np < pts_cap) {
+                        go_path_pt *P = pts + np;
+                        P->launch_idx = li; P->leg = bnc; P->step = m; P->kind = 1;
+                        P->pos0 = ym[0]; P->pos1 = 0.0; P->z = ym[1];
+                        P->tl = 0.0; P->absorption = -2.0 * att; P->time = tt;
+                    }
+                    np++;
+                }
+                if (caus) D_prev = D;
+            }
+            for (int m = 0; m < k; m++) zmax = zmax > r8[8 * m + 1] ? zmax : r8[8 * m + 1];
+            if (brk || k < 2) break;
+            if (r8[8 * (k - 1) + 1] > g->a->z_bndlyr) {
+                bnc -= 1;                                     /* elevated reflection (:371-372) */
+            } else {
+                const double *yk = r8 + 8 * k;
+                go_arrival A;
+                memset(&A, 0, sizeof(A));
+                A.launch_idx = li;
+                A.n_bounce = bnc;
+                A.incl_launch = incl;
+                A.az_launch = az;
+                A.pos0 = yk[0];
+                A.time = tt;
+                A.cel = yk[0] / tt;
+                A.turn_ht = zmax;
+                A.incl_arr = -asin(g2_pt(g, &g->a->c, g2_topo_z(g, yk[0])) / g2_pt(g, &g->a->c, z_src) * yk[3]) * (180.0 / GO_PI);
+                A.tl = g->calc_amp ? 20.0 * log10(go2_amp(g, yk)) : 0.0;
+                A.absorption = -2.0 * att;
+                A.pos1 = go2_est_dev(g, r8, k);
+                A.back_az = 0.0;
+                if (cnt < out_cap) out[cnt] = A;
+                cnt++;
+            }
+            double rk2[8], rk1[8];
+            memcpy(rk2, r8 + 8 * (k - 2), sizeof(rk2));
+            memcpy(rk1, r8 + 8 * (k - 1), sizeof(rk1));
+            go2_set_refl(g, rk2, rk1, r8);
+        }
+    }
+    *n_out = cnt;
+    if (n_pts) *n_pts = np;
+    if (ray_steps) *ray_steps = steps;
+    return (cnt > out_cap || np > pts_cap) ? -1 : 0;
+}
+
+void *go2_h_new(void *atmo_h) { go_2d *g = (go_2d *)malloc(sizeof(go_2d)); go2_init(g, (go_atmo *)atmo_h); return g; }
+void go2_h_free(void *h) { if (h) { go2_free((go_2d *)h); free(h); } }
+void go2_h_config(void *h, int calc_amp, double z_reflect, double alt_max, double rng_max, double s_max, double ds_min,
+                  double ds_max)
+{
+    go_2d *g = (go_2d *)h;
+    g->calc_amp = calc_amp;
+    if (!isnan(z_reflect)) g->z_reflect = z_reflect;
+    if (!isnan(alt_max)) g->alt_max = alt_max;
+    if (!isnan(rng_max)) g->rng_max = rng_max;
+    if (!isnan(s_max)) g->s_max = s_max;
+    if (!isnan(ds_min)) g->ds_min = ds_min;
+    if (!isnan(ds_max)) g->ds_max = ds_max;
+}
+#endif

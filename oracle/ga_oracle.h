@@ -227,6 +227,37 @@ enum { GO_EIG_EST, GO_EIG_SWEEP_TRACES, GO_EIG_SWEEP_CORR, GO_EIG_SWEEP_ADAPT, G
        GO_EIG_FINAL_TRACES, GO_EIG_FINDS, GO_EIG_FOUND, GO_EIG_NCOUNT };
 void   go_eig_counts(long long out[GO_EIG_NCOUNT]);
 
+/* ---- 2D effective-sound-speed geometry (infraga-2d) ------------------- */
+typedef struct {
+    go_atmo *a;          /* stratified splines; a->z0, z_max, z_bndlyr as set here */
+    go_spline topo;      /* terrain z(r) when is_topo */
+    int is_topo;
+    int calc_amp;        /* 8 equations with R, Z, mu_r, mu_z; 4 without */
+    double z_reflect, alt_max, rng_max, s_max, ds_min, ds_max;
+    double theta, phi;   /* launch inclination, phi = pi/2 - azimuth [rad] */
+    double z_src, c_eff0;
+} go_2d;
+
+void   go2_init(go_2d *g, go_atmo *a);
+int    go2_set_topo(go_2d *g, const double *r, const double *z, int n);
+void   go2_set_ground(go_2d *g, double z0);
+void   go2_free(go_2d *g);
+void   go2_set_initial(go_2d *g, double *y, double r0, double z0);
+int    go2_prop_rk4(go_2d *g, double *rows, int length, int *check);
+void   go2_set_refl(go_2d *g, const double *rk2, const double *rk1, double *out);
+double go2_travel_time_seg(go_2d *g, const double *y0, const double *y1);
+double go2_atten_seg(go_2d *g, const double *y0, const double *y1, double freq);
+double go2_jacobian(go_2d *g, const double *y);
+double go2_amp(go_2d *g, const double *y);
+double go2_est_dev(go_2d *g, const double *rows, int k);
+int    go2_fan(go_2d *g, const double *launches, int n_launch, int bounces, double src_alt, double freq, int flags,
+               go_arrival *out, int out_cap, int *n_out, go_path_pt *pts, long long pts_cap, long long *n_pts,
+               long long *ray_steps);
+void  *go2_h_new(void *atmo_h);
+void   go2_h_free(void *h);
+void   go2_h_config(void *h, int calc_amp, double z_reflect, double alt_max, double rng_max, double s_max,
+                    double ds_min, double ds_max);
+
 /* ---- opaque-handle helpers for test bindings ------------------------- */
 void  *go_h_atmo(int geom, const double *z, const double *c, const double *u,
                  const double *v, const double *rho, int n);

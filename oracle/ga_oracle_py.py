@@ -139,6 +139,16 @@ class Oracle:
                                     C.c_int, C.c_double, C.c_double, C.c_double, C.POINTER(Arrival),
                                     C.POINTER(C.c_longlong)]
         L.go_eig_direct.restype = C.c_int
+        L.go2_h_new.restype = C.c_void_p
+        L.go2_h_new.argtypes = [C.c_void_p]
+        L.go2_h_free.argtypes = [C.c_void_p]
+        L.go2_h_config.argtypes = [C.c_void_p, C.c_int] + [C.c_double] * 6
+        L.go2_set_topo.argtypes = [C.c_void_p, _dp, _dp, C.c_int]
+        L.go2_set_ground.argtypes = [C.c_void_p, C.c_double]
+        L.go2_fan.argtypes = [C.c_void_p, _dp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
+                              C.POINTER(Arrival), C.c_int, C.POINTER(C.c_int), C.POINTER(PathPt), C.c_longlong,
+                              C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
+        L.go2_fan.restype = C.c_int
 
     EIG_COUNT_NAMES = ("est_calls", "sweep_traces", "sweep_traces_corrected", "sweep_traces_adaptive",
                        "lm_traces", "final_traces", "finds", "found")
@@ -263,6 +273,53 @@ class Medium:
         self.o.lib.go_eig_row_3d(self.h, C.byref(params), _ptr(s), float(incl_deg), float(az_deg), int(bounces),
                                  freq, C.byref(A))
         return arrivals_to_array((Arrival * 1)(A), 1)[0]
+
+
+class Medium2D:
+    """The oracle's infraga-2d geometry over a stratified profile (engine
+    units).  Ground defaults to the profile bottom, as set_limits does."""
+
+    def __init__(self, oracle: Oracle, z, c, u, v, rho, calc_amp=True, refl_alt=float("nan"),
+                 alt_max=float("nan"), rng_max=float("nan"), s_max=float("nan"), ds_min=float("nan"),
+                 ds_max=float("nan"), z_grnd=None, topo=None):
+        self.o = oracle
+        self.m = Medium(oracle, GEOM_3D, z, c, u, v, rho)
+        L = oracle.lib
+        self.h = L.go2_h_new(self.m.h)
+        if topo is not None:
+            r, zt = (np.ascontiguousarray(a, dtype=np.float64) for a in topo)
+            self._topo = (r, zt)
+            L.go2_set_topo(self.h, _ptr(r), _ptr(zt), len(r))
+        if z_grnd is not None:
+            L.go2_set_ground(self.h, float(z_grnd))
+        L.go2_h_config(self.h, int(calc_amp), refl_alt, alt_max, rng_max, s_max, ds_min, ds_max)
+
+    def __del__(self):
+        try:
+            self.o.lib.go2_h_free(self.h)
+        except Exception:
+            pass
+
+    def fan(self, launches, bounces=2, src_alt=0.0, freq=0.1, flags=0):
+        """(arrivals, path rows, ray-steps); arrivals pos0 = r, pos1 = perpendicular drift."""
+        Lr = np.ascontiguousarray(launches, dtype=np.float64).reshape(-1, 2)
+        cap = max(1, Lr.shape[0] * (bounces + 1))
+        out = (Arrival * cap)()
+        n_out = C.c_int(0)
+        steps = C.c_longlong(0)
+        n_pts = C.c_longlong(0)
+        pcap = 1
+        while True:
+            pts = (PathPt * pcap)()
+            rc = self.o.lib.go2_fan(self.h, _ptr(Lr), Lr.shape[0], bounces, src_alt, freq, int(flags), out, cap,
+                                    C.byref(n_out), pts, pcap, C.byref(n_pts), C.byref(steps))
+            if n_pts.value <= pcap and n_out.value <= cap:
+                break
+            pcap = max(pcap, n_pts.value)
+            if n_out.value > cap:
+                cap = n_out.value
+                out = (Arrival * cap)()
+        return arrivals_to_array(out, n_out.value), paths_to_array(pts, n_pts.value), steps.value
 
 
 class Ref:

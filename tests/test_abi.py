@@ -13,7 +13,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def declared():
     txt = open(os.path.join(ROOT, "include", "infraga_gpu.h")).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"\b(ga_[a-z_]+)\s*\(", txt)))
+    return sorted(set(re.findall(r"\b(ga_[a-z0-9_]+)\s*\(", txt)))
 
 
 def test_header_declares_expected_api():
