@@ -30,8 +30,10 @@ struct EigJob {
 // Result layout per job (stride doubles):
 //   EIG_SWEEP: row K's (x, y) [3D] or (lat, lon) [sph] of legs 0..legs
 //                                                     stride 2 * (legs_max + 1)
-//   EIG_LM / EIG_FINAL: row K (18), tt, att, zmax    stride EIG_FULL
-constexpr int EIG_FULL = 21;
+//   EIG_LM / EIG_FINAL: row K (18), tt, att, zmax, and for the
+//              range-dependent medium topo::z, topo::dz (lat, lon) and
+//              atmo::c at the ground under row K       stride EIG_FULL
+constexpr int EIG_FULL = 25;
 
 struct EigArgs {
     FanArgs F;               // medium, limits, source, freq, max_steps, ypbuf, queue

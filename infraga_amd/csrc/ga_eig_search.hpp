@@ -52,6 +52,8 @@ struct EigSetup {
     double rng_max;                // km
     double c_src, c_grnd;          // c at the source and at the ground [km/s] (LM update)
     int spec_chunk;                // sweep traces requested per miss after one azimuth correction
+    int rngdep;                    // sph over a range-dependent medium and terrain: the LM update
+                                   // reads the ground under row K from the trace result
 };
 
 // One round of traces.  sweep: EIG_SWEEP jobs; full: EIG_LM jobs followed by
@@ -648,11 +650,19 @@ private:
                     step_sc = std::min(1.0, step_sc * 1.25);
                     dlat = rcvr[0] - yk[1];
                     dlon = rcvr[1] - yk[2];
-                    r_grnd = S_.r0 + S_.src_ground;     // topo::z: globe radius + ground height
-                    c_grnd = S_.c_grnd;
                     c_src = S_.c_src;
-                    dzg_dlat = 0.0 / r_grnd;             // flat ground: topo::dz = 0
-                    dzg_dlon = 0.0 / (r_grnd * std::cos(yk[1]));
+                    if (S_.rngdep) {
+                        // topo::z, topo::dz and atmo::c at the ground under row K
+                        r_grnd = yk[21];
+                        c_grnd = yk[24];
+                        dzg_dlat = (long double)yk[22] / r_grnd;
+                        dzg_dlon = (long double)yk[23] / (r_grnd * std::cos(yk[1]));
+                    } else {
+                        r_grnd = S_.r0 + S_.src_ground;     // topo::z: globe radius + ground height
+                        c_grnd = S_.c_grnd;
+                        dzg_dlat = 0.0 / r_grnd;             // flat ground: topo::dz = 0
+                        dzg_dlon = 0.0 / (r_grnd * std::cos(yk[1]));
+                    }
                     ds_norm = yk[3] - dzg_dlat * yk[4] - dzg_dlon * yk[5];
                     ds_dth = -c_src / c_grnd * (yk[6] - dzg_dlat * r_grnd * yk[7] - dzg_dlon * (r_grnd * std::cos(yk[1])) * yk[8]) / ds_norm;
                     ds_dph = -c_src / c_grnd * (yk[12] - dzg_dlat * r_grnd * yk[13] - dzg_dlon * (r_grnd * std::cos(yk[1])) * yk[14]) / ds_norm;

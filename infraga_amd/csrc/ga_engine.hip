@@ -31,6 +31,7 @@
 #include "../../include/infraga_gpu.h"
 #include "ga_2d.hpp"
 #include "ga_eig_search.hpp"
+#include "ga_rdeig.hpp"
 #include "ga_rdtrace.hpp"
 #include "ga_rngdep.hpp"
 #include "ga_trace.hpp"
@@ -169,6 +170,26 @@ __global__ void __launch_bounds__(256) rd_arrival_kernel(RdFanArgs A)
 }
 
 // Terrain height above sea level at one point (the source of a fan).
+// Eigenray traces over the range-dependent medium (same job set as eig_kernel).
+template <int NEQ, bool BLW, bool PATHS>
+__global__ void __launch_bounds__(GA_WG_RD) rd_eig_kernel(RdEigArgs E)
+{
+    const int lane = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ double rows[3 * NEQ * GA_WG_RD];
+    const LaneVec<GA_WG_RD> ya{rows + threadIdx.x}, yb{rows + NEQ * GA_WG_RD + threadIdx.x},
+        ys{rows + 2 * NEQ * GA_WG_RD + threadIdx.x};
+    rd_eig_lane_loop<NEQ, BLW, PATHS>(E, lane, [&]() { return (int)atomicAdd(E.F.queue, 1u); }, ya, yb, ys);
+}
+
+// Arrival rows of the range-dependent EIG_FINAL jobs [first, first + n).
+__global__ void __launch_bounds__(256) rd_eig_arrival_kernel(RdEigArgs E, int first, int n, ga_arrival *out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const EigJob J = E.jobs[first + i];
+    out[i] = rd_eig_arrival_row(E, J, E.res + (size_t)(first + i) * EIG_FULL);
+}
+
 __global__ void rd_ground_kernel(RdMed M, double lat, double lon, double *out)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) out[0] = rd_ground(M, lat, lon) - M.r0;
@@ -521,9 +542,9 @@ bool rd_region(const DevRd &R, double z_grnd, RdRegion &G)
 // them (atmo_io.sph.rngdep.cpp:27-43, atmo_state.sph.rngdep.cpp:42-55), the
 // source and the reference's z_grnd handling from run_prop
 // (main.infraga.sph.rngdep.cpp:229-315).
-int rd_fan_enqueue(ga_ctx *ctx, FanBufs &fb, int rd_id, const ga_prop_params *p, const double *d_launches, int n,
-                   ga_arrival *d_out, int32_t *d_counts, int32_t *d_steps, hipStream_t st,
-                   const PathOut *path = nullptr)
+// Medium, limits and source of a range-dependent run (run_prop /
+// run_eig_search set-up, main.infraga.sph.rngdep.cpp:229-315, 877-880).
+int rd_setup(ga_ctx *ctx, FanBufs &fb, int rd_id, const ga_prop_params *p, RdFanArgs &A, hipStream_t st)
 {
     auto it = ctx->rds.find(rd_id);
     if (it == ctx->rds.end()) return fail(ctx, GA_E_ATMO, "unknown range-dependent atmosphere id %d", rd_id);
@@ -531,7 +552,6 @@ int rd_fan_enqueue(ga_ctx *ctx, FanBufs &fb, int rd_id, const ga_prop_params *p,
     if (p->bounces < 0) return fail(ctx, GA_E_ARG, "bounces must be >= 0");
     if (!(p->ds_min > 0.0) || !(p->ds_max >= p->ds_min) || !(p->s_max > 0.0))
         return fail(ctx, GA_E_ARG, "invalid step parameters ds_min=%g ds_max=%g s_max=%g", p->ds_min, p->ds_max, p->s_max);
-    RdFanArgs A;
     std::memset(&A, 0, sizeof(A));
     RdMed &M = A.M;
     M.g = rd_grid_of(R);
@@ -584,6 +604,21 @@ int rd_fan_enqueue(ga_ctx *ctx, FanBufs &fb, int rd_id, const ga_prop_params *p,
         zsrc = std::max(zsrc, M.z0);
     }
     A.src0 = M.r0 + zsrc;
+    A.freq = p->freq;
+    A.turn_ht_min = p->turn_ht_min;
+    A.bounces = p->bounces;
+    long long rows = (long long)(p->s_max / p->ds_min);
+    A.max_steps = (int)std::min<long long>(rows - 1, 2000000000LL);
+    return GA_OK;
+}
+
+int rd_fan_enqueue(ga_ctx *ctx, FanBufs &fb, int rd_id, const ga_prop_params *p, const double *d_launches, int n,
+                   ga_arrival *d_out, int32_t *d_counts, int32_t *d_steps, hipStream_t st,
+                   const PathOut *path = nullptr)
+{
+    RdFanArgs A;
+    int rc = rd_setup(ctx, fb, rd_id, p, A, st);
+    if (rc != GA_OK) return rc;
     A.freq = p->freq;
     A.turn_ht_min = p->turn_ht_min;
     A.bounces = p->bounces;
@@ -961,6 +996,8 @@ struct EigGpu {
     ga_ctx *ctx = nullptr;
     int geom = GA_GEOM_3D;
     FanArgs F{};                 // medium, limits, source, c_src, freq, max_steps, lds
+    bool rd = false;             // range-dependent medium: RF instead of F
+    RdFanArgs RF{};
     hipStream_t st = nullptr, s2 = nullptr;
     bool own_st = false;
     DevBuf<EigJob> jobs6, jobs18;
@@ -1025,6 +1062,38 @@ struct EigGpu {
         return GA_OK;
     }
 
+    // Range-dependent traces: persistent 64-lane workgroups (RK4 rows in LDS).
+    template <int NEQ, bool PATHS = false>
+    int launch_rd(const EigJob *jobs, int n_jobs, int stride, double *res, int32_t *brk, int32_t *stp,
+                  DevBuf<double> &yp, unsigned int *queue, hipStream_t s, const PathOut *path = nullptr)
+    {
+        const bool blw = RF.M.is_topo && RF.M.use_blw;
+        auto kern = blw ? rd_eig_kernel<NEQ, true, PATHS> : rd_eig_kernel<NEQ, false, PATHS>;
+        int per_cu = 0;
+        HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, GA_WG_RD, 0));
+        per_cu = std::max(per_cu, 1);
+        const int need = (n_jobs + GA_WG_RD - 1) / GA_WG_RD;
+        const int grid = std::max(1, std::min(per_cu * ctx->n_cu, need));
+        if (!yp.ensure((size_t)grid * GA_WG_RD * 18)) return fail(ctx, GA_E_NOMEM, "eigenray lane buffer");
+        RdEigArgs E{};
+        E.F = RF;
+        E.F.ypbuf = yp.p;
+        E.F.nlanes = grid * GA_WG_RD;
+        E.F.queue = queue;
+        if (path) E.F.path = *path;
+        E.jobs = jobs;
+        E.n_jobs = n_jobs;
+        E.stride = stride;
+        E.res = res;
+        E.brk_leg = brk;
+        E.steps = stp;
+        hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, s, queue);
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(GA_WG_RD), 0, s, E);
+        HIPCHK(ctx, hipGetLastError());
+        launches += 2;
+        return GA_OK;
+    }
+
     int enqueue(const EigBatch &B)
     {
         n6 = B.sweep.size();
@@ -1044,7 +1113,13 @@ struct EigGpu {
         if (n18) HIPCHK(ctx, hipMemcpyAsync(jobs18.p, h_jobs18.p, n18 * sizeof(EigJob), hipMemcpyHostToDevice, st));
         HIPCHK(ctx, hipEventRecord(e0, st));
         HIPCHK(ctx, hipStreamWaitEvent(s2, e0, 0));
-        if (n6) {
+        if (n6 && rd) {
+            int rc = launch_rd<6>(jobs6.p, (int)n6, stride, res6.p, brk6.p, st6.p, yp6, q.p, st);
+            if (rc != GA_OK) return rc;
+            HIPCHK(ctx, hipMemcpyAsync(h_res6.p, res6.p, n6 * stride * sizeof(double), hipMemcpyDeviceToHost, st));
+            HIPCHK(ctx, hipMemcpyAsync(h_brk6.p, brk6.p, n6 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+            HIPCHK(ctx, hipMemcpyAsync(h_st6.p, st6.p, n6 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        } else if (n6) {
             EigArgs E{};
             E.F = F;
             E.jobs = jobs6.p;
@@ -1060,7 +1135,26 @@ struct EigGpu {
             HIPCHK(ctx, hipMemcpyAsync(h_brk6.p, brk6.p, n6 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
             HIPCHK(ctx, hipMemcpyAsync(h_st6.p, st6.p, n6 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
         }
-        if (n18) {
+        if (n18 && rd) {
+            int rc = launch_rd<18>(jobs18.p, (int)n18, EIG_FULL, res18.p, brk18.p, st18.p, yp18, q.p + 1, s2);
+            if (rc != GA_OK) return rc;
+            if (nf) {
+                RdEigArgs E{};
+                E.F = RF;
+                E.jobs = jobs18.p;
+                E.n_jobs = (int)n18;
+                E.stride = EIG_FULL;
+                E.res = res18.p;
+                hipLaunchKernelGGL(rd_eig_arrival_kernel, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, s2, E,
+                                   B.n_lm, (int)nf, arr.p);
+                HIPCHK(ctx, hipGetLastError());
+                launches += 1;
+                HIPCHK(ctx, hipMemcpyAsync(h_arr.p, arr.p, nf * sizeof(ga_arrival), hipMemcpyDeviceToHost, s2));
+            }
+            HIPCHK(ctx, hipMemcpyAsync(h_res18.p, res18.p, n18 * EIG_FULL * sizeof(double), hipMemcpyDeviceToHost, s2));
+            HIPCHK(ctx, hipMemcpyAsync(h_brk18.p, brk18.p, n18 * sizeof(int32_t), hipMemcpyDeviceToHost, s2));
+            HIPCHK(ctx, hipMemcpyAsync(h_st18.p, st18.p, n18 * sizeof(int32_t), hipMemcpyDeviceToHost, s2));
+        } else if (n18) {
             EigArgs E{};
             E.F = F;
             E.jobs = jobs18.p;
@@ -1103,7 +1197,12 @@ struct EigGpu {
         PathOut P{};
         int rc = pd.setup(ctx, cap, (int)n, GA_PATH_RAYS, s2, P);
         if (rc != GA_OK) return rc;
-        if (n) {
+        if (n && rd) {
+            std::memcpy(h_jobs18.p, jobs.data(), n * sizeof(EigJob));
+            HIPCHK(ctx, hipMemcpyAsync(jobs18.p, h_jobs18.p, n * sizeof(EigJob), hipMemcpyHostToDevice, s2));
+            rc = launch_rd<18, true>(jobs18.p, (int)n, EIG_FULL, res18.p, brk18.p, st18.p, yp18, q.p + 1, s2, &P);
+            if (rc != GA_OK) return rc;
+        } else if (n) {
             std::memcpy(h_jobs18.p, jobs.data(), n * sizeof(EigJob));
             HIPCHK(ctx, hipMemcpyAsync(jobs18.p, h_jobs18.p, n * sizeof(EigJob), hipMemcpyHostToDevice, s2));
             EigArgs E{};
@@ -1207,12 +1306,71 @@ int eig_prepare(ga_ctx *ctx, const DevAtmo &a, const ga_prop_params *p, const ga
     S.rng_max = p->rng_max;
     S.c_grnd = host_c_at(a, F.M.z0);
     S.spec_chunk = 64;
+    S.rngdep = 0;
+    return GA_OK;
+}
+
+// Search set-up over a range-dependent medium with terrain (run_eig_search
+// of main.infraga.sph.rngdep.cpp:877-965): the spherical search with the
+// medium's c at the source and the terrain's lowest inclination.
+int eig_prepare_rd(ga_ctx *ctx, int rd_id, const ga_prop_params *p, const ga_eig_params *e, RdFanArgs &RF,
+                   EigSetup &S, std::vector<double> &rcvr)
+{
+    int rc = rd_setup(ctx, ctx->fb, rd_id, p, RF, ctx->stream);
+    if (rc != GA_OK) return rc;
+    const RdMed &M = RF.M;
+    S.geom = GA_GEOM_SPH;
+    S.r0 = M.r0;
+    S.src_ground = M.z0;
+    const double zsrc = RF.src0 - M.r0;
+    S.src[0] = zsrc;
+    S.src[1] = RF.src1;
+    S.src[2] = RF.src2;
+    for (double &x : rcvr) x *= (PI / 180.0);
+    double pt[3] = {RF.src1, RF.src2, zsrc}, aout[40], tout[10];
+    rc = ga_rngdep_eval(ctx, rd_id, pt, 1, aout, tout);
+    if (rc != GA_OK) return rc;
+    S.c_src = aout[0];                 // atmo::c(src[0] + r0, src[1], src[2]) (geoac.eigenray.sph.cpp:338)
+    S.c_grnd = 0.0;                    // read per arrival from the trace result
+    const double zg = M.is_topo ? tout[0] : M.z0;
+    double th_grnd;
+    if (zsrc - zg < 0.01) {
+        if (M.is_topo) {
+            // :940-946: the steepest terrain slope over azimuths 0, 3, ..., 357
+            // in degrees plus 0.1, then compared with the radian bounds as it stands
+            th_grnd = 0.0;
+            const double dzgdx = 1.0 / (M.r0 * std::cos(RF.src1)) * tout[2];
+            const double dzgdy = 1.0 / M.r0 * tout[1];
+            for (double ph = 0.0; ph < 360.0; ph += 3.0)
+                th_grnd = std::max(th_grnd, std::atan(dzgdx * std::sin(ph * PI / 180.0) + dzgdy * std::cos(ph * PI / 180.0)) *
+                                                (180.0 / PI) + 0.1);
+        } else {
+            th_grnd = 0.1 * (PI / 180.0);
+        }
+    } else {
+        th_grnd = -89.9 * (PI / 180.0);
+    }
+    S.th_min = e->incl_min * (PI / 180.0);
+    S.th_max = e->incl_max * (PI / 180.0);
+    S.th_start = std::max(S.th_min, th_grnd);
+    S.bnc_min = e->bnc_min;
+    S.bnc_max = e->bnc_max;
+    S.iterations = e->iterations;
+    S.damping = e->damping;
+    S.tolerance = e->tolerance;
+    S.az_err_lim = e->az_dev_lim;
+    S.dth_big = e->incl_step_max * (PI / 180.0);
+    S.dth_sml = e->incl_step_min * (PI / 180.0);
+    S.rng_max = p->rng_max;
+    S.spec_chunk = 64;
+    S.rngdep = 1;
     return GA_OK;
 }
 
 int eig_check(ga_ctx *ctx, const ga_prop_params *p, const ga_eig_params *e)
 {
-    if (p->geometry != GA_GEOM_3D && p->geometry != GA_GEOM_SPH) return fail(ctx, GA_E_ARG, "bad geometry %d", p->geometry);
+    if (p->geometry != GA_GEOM_3D && p->geometry != GA_GEOM_SPH && p->geometry != GA_GEOM_SPH_RNGDEP)
+        return fail(ctx, GA_E_ARG, "bad geometry %d", p->geometry);
     if (e->bnc_min < 0 || e->bnc_max < e->bnc_min) return fail(ctx, GA_E_ARG, "bad bounce range %d..%d", e->bnc_min, e->bnc_max);
     if (e->iterations < 0) return fail(ctx, GA_E_ARG, "negative iteration limit");
     if (!(e->incl_step_max > 0.0) || !(e->incl_step_min > 0.0))
@@ -1893,15 +2051,22 @@ static int eig_search_impl(ga_ctx *ctx, int atmo_id, const ga_prop_params *p, co
     if (n_rcvr < 0) return fail(ctx, GA_E_ARG, "negative receiver count");
     int rc = eig_check(ctx, p, e);
     if (rc != GA_OK) return rc;
-    auto it = ctx->atmos.find(atmo_id);
-    if (it == ctx->atmos.end()) return fail(ctx, GA_E_ATMO, "unknown atmosphere id %d", atmo_id);
     HIPCHK(ctx, hipSetDevice(ctx->device));
     EigGpu gpu;
     gpu.ctx = ctx;
     gpu.geom = p->geometry;
     EigSetup S;
+    std::memset(&S, 0, sizeof(S));
     std::vector<double> rcvr(rcvr_xy, rcvr_xy + 2 * (size_t)n_rcvr);
-    rc = eig_prepare(ctx, it->second, p, e, gpu.F, S, rcvr);
+    if (p->geometry == GA_GEOM_SPH_RNGDEP) {
+        gpu.rd = true;
+        gpu.geom = GA_GEOM_SPH;
+        rc = eig_prepare_rd(ctx, atmo_id, p, e, gpu.RF, S, rcvr);
+    } else {
+        auto it = ctx->atmos.find(atmo_id);
+        if (it == ctx->atmos.end()) return fail(ctx, GA_E_ATMO, "unknown atmosphere id %d", atmo_id);
+        rc = eig_prepare(ctx, it->second, p, e, gpu.F, S, rcvr);
+    }
     if (rc != GA_OK) return rc;
     rc = gpu.init(ctx->stream);
     if (rc != GA_OK) return rc;

@@ -715,6 +715,161 @@ int run_eig_search(int argc, char **argv)
     return 0;
 }
 
+// infraga-sph-rngdep -eig_search prefix lats.dat lons.dat [key=value ...]
+// (run_eig_search, main.infraga.sph.rngdep.cpp:777-965): the spherical
+// search over the range-dependent medium and terrain; writes
+// <prefix>.arrivals.dat and <prefix>.eigenray-N.dat.
+int run_eig_search_rd(int argc, char **argv)
+{
+    if (argc < 5) {
+        std::cerr << GA_CLI_NAME << ": usage: -eig_search <profile prefix> <lat nodes> <lon nodes> [parameters]\n";
+        return 2;
+    }
+    double lat_src = 30.0, lon_src = 0.0, z_src = 0.0, rlat = 30.0, rlon = -2.5;
+    bool src_lat_set = false, src_lon_set = false, rlat_set = false, rlon_set = false;
+    bool reverse_winds = false, custom_id = false;
+    double freq = 0.1;
+    const char *prof_format = "zTuvdp";
+    const char *topo_file = nullptr;
+    const char *output_id = nullptr;
+    ga_prop_params p;
+    ga_prop_params_default(&p, GA_GEOM_SPH_RNGDEP);
+    ga_eig_params e;
+    ga_eig_params_default(&e);
+    double z_grnd = NAN;
+    for (int i = 5; i < argc; i++) {
+        const char *a = argv[i], *v = nullptr;
+        if (key2(a, "incl_min=", "min_incl=", v)) e.incl_min = std::atof(v);
+        else if (key2(a, "incl_max=", "max_incl=", v)) e.incl_max = std::atof(v);
+        else if (key(a, "bnc_min=", v)) e.bnc_min = std::atoi(v);
+        else if (key(a, "bnc_max=", v)) e.bnc_max = std::atoi(v);
+        else if (key(a, "bounces=", v)) e.bnc_min = e.bnc_max = std::atoi(v);
+        else if (key2(a, "src_alt=", "alt_src=", v)) z_src = std::atof(v);
+        else if (key2(a, "src_lat=", "lat_src=", v)) { lat_src = std::atof(v); src_lat_set = true; }
+        else if (key2(a, "src_lon=", "lon_src=", v)) { lon_src = std::atof(v); src_lon_set = true; }
+        else if (key2(a, "rcvr_lat=", "lat_rcvr=", v)) { rlat = std::atof(v); rlat_set = true; }
+        else if (key2(a, "rcvr_lon=", "lon_rcvr=", v)) { rlon = std::atof(v); rlon_set = true; }
+        else if (key(a, "verbose=", v)) { /* trace-by-trace log not produced */ }
+        else if (key(a, "iterations=", v)) e.iterations = (int)std::atof(v);
+        else if (key(a, "damping=", v)) e.damping = std::atof(v);
+        else if (key(a, "tolerance=", v)) e.tolerance = std::atof(v);
+        else if (key(a, "az_dev_lim=", v)) e.az_dev_lim = std::atof(v);
+        else if (key2(a, "incl_step_min=", "min_incl_step=", v)) e.incl_step_min = std::atof(v);
+        else if (key2(a, "incl_step_max=", "max_incl_step=", v)) e.incl_step_max = std::atof(v);
+        else if (key(a, "freq=", v)) freq = std::atof(v);
+        else if (key(a, "abs_coeff=", v)) p.abs_coeff = std::max(0.0, std::atof(v));
+        else if (key2(a, "max_alt=", "alt_max=", v)) p.alt_max = std::atof(v);
+        else if (key2(a, "max_rng=", "rng_max=", v)) p.rng_max = std::atof(v);
+        else if (key2(a, "max_time=", "time_max=", v)) p.time_max = std::atof(v);
+        else if (key2(a, "min_lat=", "lat_min=", v)) p.bounds[0] = std::atof(v);
+        else if (key2(a, "max_lat=", "lat_max=", v)) p.bounds[1] = std::atof(v);
+        else if (key2(a, "min_lon=", "lon_min=", v)) p.bounds[2] = std::atof(v);
+        else if (key2(a, "max_lon=", "lon_max=", v)) p.bounds[3] = std::atof(v);
+        else if (key2(a, "min_ds=", "ds_min=", v)) p.ds_min = std::atof(v);
+        else if (key2(a, "max_ds=", "ds_max=", v)) p.ds_max = std::atof(v);
+        else if (key2(a, "max_s=", "s_max=", v)) p.s_max = std::atof(v);
+        else if (key(a, "write_atmo=", v)) { /* profile dump: not produced */ }
+        else if (key(a, "prof_format=", v)) prof_format = v;
+        else if (key(a, "reverse_winds=", v)) reverse_winds = str2bool(v);
+        else if (key(a, "output_id=", v)) { custom_id = true; output_id = v; }
+        else if (key(a, "z_grnd=", v)) z_grnd = std::atof(v);
+        else if (key(a, "topo_file=", v)) topo_file = v;
+        else if (key(a, "topo_use_BLw=", v)) p.topo_use_blw = str2bool(v) ? 1 : 0;
+        else {
+            std::cerr << GA_CLI_NAME << ": unrecognized parameter entry: " << a << '\n';
+            return 2;
+        }
+    }
+    ga_ctx *ctx = nullptr;
+    if (ga_ctx_create(&ctx, 0) != GA_OK) return die(nullptr, "no usable GPU");
+    int rid = 0;
+    if (ga_rngdep_load_files(ctx, argv[2], argv[3], argv[4], topo_file, prof_format, reverse_winds ? 1 : 0, &rid) != GA_OK)
+        return die(ctx, "cannot load the range-dependent atmosphere");
+    double info[8];
+    if (ga_rngdep_info(ctx, rid, info) != GA_OK) return die(ctx, "region");
+    const bool is_topo = topo_file != nullptr;
+    if (is_topo) {
+        // :812-815: source at the region centre, receiver 5 deg east of it
+        const double clat = (info[1] + info[0]) / 2.0 * (180.0 / Pi), clon = (info[3] + info[2]) / 2.0 * (180.0 / Pi);
+        if (!src_lat_set) lat_src = clat;
+        if (!src_lon_set) lon_src = clon;
+        if (!rlat_set) rlat = clat + 0.0;
+        if (!rlon_set) rlon = clon + 5.0;
+    }
+    const double z0 = (!is_topo && !std::isnan(z_grnd)) ? z_grnd : info[5];
+    if (!is_topo) p.z_grnd = z0;
+    double pts[6] = {lat_src * (Pi / 180.0), lon_src * (Pi / 180.0), 0.0, rlat * (Pi / 180.0), rlon * (Pi / 180.0), 0.0};
+    double aout[80], tout[20];
+    if (ga_rngdep_eval(ctx, rid, pts, 2, aout, tout) != GA_OK) return die(ctx, "terrain");
+    const double zg_src = is_topo ? tout[0] : z0, zg_rcvr = is_topo ? tout[10] : z0;
+    z_src = std::max(z_src, zg_src);                                     // :880
+    p.freq = freq;
+    p.src[0] = lat_src;
+    p.src[1] = lon_src;
+    p.src[2] = z_src;
+    const double rcvr[2] = {rlat, rlon};
+    std::vector<ga_eigenray> eig(64);
+    std::vector<ga_path_pt> pts_out((size_t)1 << 16);
+    int n_out = 0;
+    int64_t n_pts = 0;
+    double lat_ms = 0.0;
+    ga_run_stats st;
+    int rc = ga_eig_search_paths(ctx, rid, &p, &e, rcvr, 1, eig.data(), (int)eig.size(), &n_out, &lat_ms,
+                                 pts_out.data(), (int64_t)pts_out.size(), &n_pts, &st);
+    if (rc == GA_E_CAP) {
+        eig.resize(std::max<size_t>(eig.size(), (size_t)n_out));
+        pts_out.resize(std::max<size_t>(pts_out.size(), (size_t)n_pts));
+        rc = ga_eig_search_paths(ctx, rid, &p, &e, rcvr, 1, eig.data(), (int)eig.size(), &n_out, &lat_ms,
+                                 pts_out.data(), (int64_t)pts_out.size(), &n_pts, &st);
+    }
+    if (rc != GA_OK) return die(ctx, "eigenray search failed");
+    std::string id = custom_id ? std::string(output_id) : std::string(argv[2]);
+    {
+        int64_t w = 0;
+        for (int i = 0; i < n_out; ++i) {
+            std::ofstream ray(id + ".eigenray-" + std::to_string(i) + ".dat");
+            ray << "# lat [deg]" << '\t' << "lon [deg]" << '\t' << "z [km]" << '\t' << "trans. coeff. [dB]" << '\t'
+                << "absorption [dB]" << '\t' << "time [s]" << '\n' << std::setprecision(8);
+            for (; w < n_pts && pts_out[w].launch_idx == i; ++w) {
+                const ga_path_pt &P = pts_out[w];
+                ray << P.pos0 << '\t' << P.pos1 << '\t' << P.z << '\t' << P.tl << '\t' << P.absorption << '\t' << P.time
+                    << '\n';
+            }
+        }
+    }
+    std::ofstream out(id + ".arrivals.dat");
+    out << "# infraga-sph-rngdep -eig_search summary:" << '\n';
+    out << "# " << '\t' << "profile prefix: " << argv[2] << '\n';
+    out << "# " << '\t' << "source location (lat, lon, alt): " << (lat_src * (Pi / 180.0)) * (180.0 / Pi) << ", "
+        << (lon_src * (Pi / 180.0)) * (180.0 / Pi) << ", " << z_src << '\n';
+    out << "# " << '\t' << "receiver location (lat, lon, alt): " << (rlat * (Pi / 180.0)) * (180.0 / Pi) << ", "
+        << (rlon * (Pi / 180.0)) * (180.0 / Pi) << ", " << zg_rcvr << '\n';
+    out << "# " << '\t' << "inclination range: " << (e.incl_min * (Pi / 180.0)) * (180.0 / Pi) << ", "
+        << (e.incl_max * (Pi / 180.0)) * (180.0 / Pi) << '\n';
+    out << "# " << '\t' << "bounces: " << e.bnc_min << ", " << e.bnc_max << '\n';
+    if (!is_topo) out << "#" << '\t' << "ground elevation: " << z0 << '\n';
+    else out << "#" << '\t' << "topo file:" << topo_file << '\n';
+    out << "# " << '\t' << "damping: " << e.damping << '\n';
+    out << "# " << '\t' << "frequency: " << freq << '\n';
+    out << "# " << '\t' << "S&B atten coeff: " << p.abs_coeff << '\n';
+    out << "# incl [deg]" << '\t' << "az [deg]" << '\t' << "n_b" << '\t' << "lat_0 [deg]" << '\t' << "lon_0 [deg]" << '\t'
+        << "time [s]" << '\t' << "cel [km/s]" << '\t' << "turning ht [km]" << '\t' << "inclination [deg]" << '\t'
+        << "back azimuth [deg]" << '\t' << "trans. coeff. [dB]" << '\t' << "absorption [dB]" << '\n';
+    for (int i = 0; i < n_out; ++i) {
+        const ga_eigenray &A = eig[i];
+        out << std::setprecision(8) << A.incl << '\t' << std::setprecision(8) << A.az << '\t' << A.n_bounce;
+        out << '\t' << A.pos0 << '\t' << A.pos1 << '\t' << A.time << '\t' << A.cel << '\t' << A.turn_ht;
+        out << '\t' << A.incl_arr << '\t' << A.back_az << '\t' << A.tl << '\t' << A.absorption << '\n';
+        std::cout << '\t' << "Eigenray identified:" << '\t' << "theta, phi = " << std::setprecision(8) << A.incl << ", "
+                  << A.az << " degrees." << '\n';
+    }
+    out.close();
+    std::cout << GA_CLI_NAME << ": " << n_out << " eigenray(s), " << st.n_rays << " ray traces, " << st.ray_steps
+              << " ray-steps, " << st.kernel_ms << " ms on the GPU\n";
+    ga_ctx_destroy(ctx);
+    return 0;
+}
+
 // infraga-2d -prop (run_prop, main.infraga.2d.cpp:127-400): one azimuth,
 // the effective sound speed in its vertical plane.  Always writes
 // <id>.raypaths.dat; caustics-N.dat with write_caustics; all files at the
@@ -1031,6 +1186,7 @@ int main(int argc, char **argv)
     }
     if (GA_CLI_GEOM == GA_GEOM_SPH_RNGDEP) {
         if (std::strncmp(argv[1], "-prop", 5) == 0) return run_prop_rd(argc, argv);
+        if (std::strncmp(argv[1], "-eig_search", 11) == 0) return run_eig_search_rd(argc, argv);
         std::cerr << GA_CLI_NAME << ": option " << argv[1] << " is not supported by this engine\n";
         return 2;
     }

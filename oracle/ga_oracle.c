@@ -1289,6 +1289,7 @@ typedef struct {
     go_params p6, p18;
     double dth_big, dth_sml, damping, tolerance;
     long long steps;
+    const go_eig_med *med;    /* spherical searches: the medium's operations */
 } eig_ctx;
 
 /* Work counters of the last go_eig_search_3d call (see go_eig_counts). */
@@ -1557,11 +1558,53 @@ static double go_bearing(double lat1, double lon1, double lat2, double lon2)   /
     return atan2(term1, term2);
 }
 
+/* The stratified spherical medium as a go_eig_med (topo::z = r0 + z0, flat). */
+static void sm_init(void *m, const go_params *p, go_src *s, double *y) { go_set_initial((go_atmo *)m, p, s, y); }
+static int sm_prop(void *m, const go_params *p, const go_src *s, double *rows, int length, int *check)
+{
+    return go_prop_rk4((go_atmo *)m, p, s, rows, length, check);
+}
+static void sm_refl(void *m, const go_params *p, const go_src *s, const double *a, const double *b, double *o)
+{
+    go_set_refl((go_atmo *)m, p, s, a, b, o);
+}
+static double sm_tt(void *m, const double *a, const double *b) { return go_travel_time_seg((go_atmo *)m, a, b); }
+static double sm_att(void *m, const double *a, const double *b, double f) { return go_atten_seg((go_atmo *)m, a, b, f); }
+static double sm_c(void *m, double r, double t, double p) { return go_c((go_atmo *)m, r, t, p); }
+static double sm_amp(void *m, const go_src *s, const double *y) { return go_amp((go_atmo *)m, s, y); }
+static double sm_tz(void *m, double t, double p) { (void)t; (void)p; return ground((go_atmo *)m); }
+static double sm_tdz(void *m, double t, double p, int n) { (void)m; (void)t; (void)p; (void)n; return 0.0; }
+
+static go_eig_med strat_med(go_atmo *a)
+{
+    go_eig_med M = {a, a->r0, sm_init, sm_prop, sm_refl, sm_tt, sm_att, sm_c, sm_amp, sm_tz, sm_tdz};
+    return M;
+}
+
+/* trace_legs over the search's medium operations (spherical searches). */
+static int trace_legs_m(eig_ctx *E, const go_params *p, go_src *s, double *rows, int length, int nb, int *brk)
+{
+    const go_eig_med *M = E->med;
+    M->set_initial(M->m, p, s, rows);
+    int k = M->prop_rk4(M->m, p, s, rows, length, brk);
+    E->steps += k;
+    if (!*brk) {
+        for (int b = 1; b <= nb; b++) {
+            if (k < 2) { *brk = 1; break; }
+            M->set_refl(M->m, p, s, rows + 18 * (k - 2), rows + 18 * (k - 1), rows);
+            k = M->prop_rk4(M->m, p, s, rows, length, brk);
+            E->steps += k;
+            if (*brk) break;
+        }
+    }
+    return k;
+}
+
 static int est_eigenray_sph(eig_ctx *E, const double src[3], const double rcvr[2], double th_min, double th_max,
                             double *th_est, double *ph_est, double *th_next, int bncs, double az_err_lim)
 {
     const go_params *p = &E->p6;
-    const double r0 = E->a->r0;
+    const double r0 = E->med->r0;
     double rcvr_rng = go_gc_dist(r0, src[1], src[2], rcvr[0], rcvr[1]);
     double rcvr_az = go_bearing(src[1], src[2], rcvr[0], rcvr[1]);
     int iterations = 0, k, length = (int)(p->s_max / p->ds_min);
@@ -1578,7 +1621,7 @@ static int est_eigenray_sph(eig_ctx *E, const double src[3], const double rcvr[2
         for (double th = th_min; th < th_max; th += dth) {          /* sph: strict bound */
             if (th + dth >= th_max) th_max_reached = 1;
             s.theta = th;
-            k = trace_legs(E, p, &s, rows, length, bncs, &brk, 0);
+            k = trace_legs_m(E, p, &s, rows, length, bncs, &brk);
             eig_cnt[GO_EIG_SWEEP_TRACES]++;
             if (iterations >= 1) eig_cnt[GO_EIG_SWEEP_CORR]++;
             if (iterations >= 3) eig_cnt[GO_EIG_SWEEP_ADAPT]++;
@@ -1609,33 +1652,47 @@ static int est_eigenray_sph(eig_ctx *E, const double src[3], const double rcvr[2
     return 0;
 }
 
+/* A spherical path row (lat, lon [deg], r - r0) into a path sink. */
+static void sink_pt_sph(path_sink *k, int leg, int m, double r0, const double *y, double tl, double att, double tt)
+{
+    if (k->n < k->cap) {
+        go_path_pt *P = k->pts + k->n;
+        P->launch_idx = 0; P->leg = leg; P->step = m; P->kind = 0;
+        P->pos0 = y[1] * (180.0 / GO_PI);
+        P->pos1 = y[2] * (180.0 / GO_PI);
+        P->z = y[0] - r0;
+        P->tl = tl; P->absorption = -2.0 * att; P->time = tt;
+    }
+    k->n++;
+}
+
 /* The accepted spherical eigenray's row: re-trace with per-step bookkeeping
  * over rows 1..k-1 of every leg (geoac.eigenray.sph.cpp:219-271), path rows
  * m == 1 or m % 15 == 0 into ps when given.  src = (alt [km], lat, lon
  * [rad]).  Returns the RK4 steps taken. */
-static long long eig_accept_sph(go_atmo *a, const go_params *p, go_src *s, const double src[3], int bnc_cnt,
-                                double freq, double *rows, int length, go_arrival *A, path_sink *ps)
+static long long eig_accept_sph(const go_eig_med *M, const go_params *p, go_src *s, const double src[3],
+                                int bnc_cnt, double freq, double *rows, int length, go_arrival *A, path_sink *ps)
 {
-    const double r0 = a->r0;
+    const double r0 = M->r0;
     double att = 0.0, tt = 0.0, rmax = 0.0;
     int brk, k;
     long long steps = 0;
-    go_set_initial(a, p, s, rows);
-    k = go_prop_rk4(a, p, s, rows, length, &brk);
+    M->set_initial(M->m, p, s, rows);
+    k = M->prop_rk4(M->m, p, s, rows, length, &brk);
     steps += k;
     for (int b = 0; b <= bnc_cnt; b++) {
         if (b > 0) {
             if (k < 2) break;   /* row k-2 < 0: no further reflection */
-            go_set_refl(a, p, s, rows + 18 * (k - 2), rows + 18 * (k - 1), rows);
-            k = go_prop_rk4(a, p, s, rows, length, &brk);
+            M->set_refl(M->m, p, s, rows + 18 * (k - 2), rows + 18 * (k - 1), rows);
+            k = M->prop_rk4(M->m, p, s, rows, length, &brk);
             steps += k;
         }
         for (int m = 1; m < k; m++) {
-            tt += go_travel_time_seg(a, rows + 18 * (m - 1), rows + 18 * m);
-            att += go_atten_seg(a, rows + 18 * (m - 1), rows + 18 * m, freq);
+            tt += M->tt_seg(M->m, rows + 18 * (m - 1), rows + 18 * m);
+            att += M->att_seg(M->m, rows + 18 * (m - 1), rows + 18 * m, freq);
             rmax = rmax > rows[18 * m] - r0 ? rmax : rows[18 * m] - r0;
             if (ps && (m == 1 || m % 15 == 0))     /* sph.cpp:250-257, 266-273 */
-                sink_pt(ps, 0, b, m, 0, a, rows + 18 * m, 20.0 * log10(go_amp(a, s, rows + 18 * m)), att, tt);
+                sink_pt_sph(ps, b, m, r0, rows + 18 * m, 20.0 * log10(M->amp(M->m, s, rows + 18 * m)), att, tt);
         }
     }
     const double *yk = rows + 18 * k;
@@ -1650,12 +1707,12 @@ static long long eig_accept_sph(go_atmo *a, const go_params *p, go_src *s, const
     A->turn_ht = rmax;
     /* atmo::c(src[0], ...) takes the source altitude as a radius:
      * the profile's bottom value (sph.cpp:273) */
-    A->incl_arr = -asin(go_c(a, yk[0], yk[1], yk[2]) / go_c(a, src[0], src[1], src[2]) * yk[3]) * 180.0 / GO_PI;
+    A->incl_arr = -asin(M->c(M->m, yk[0], yk[1], yk[2]) / M->c(M->m, src[0], src[1], src[2]) * yk[3]) * 180.0 / GO_PI;
     double baz = 90.0 - atan2(-yk[4], -yk[5]) * (180.0 / GO_PI);
     while (baz < -180.0) baz += 360.0;
     while (baz > 180.0) baz -= 360.0;
     A->back_az = baz;
-    A->tl = 20.0 * log10(go_amp(a, s, yk));
+    A->tl = 20.0 * log10(M->amp(M->m, s, yk));
     A->absorption = -2.0 * att;
     return steps;
 }
@@ -1678,7 +1735,8 @@ long long go_eig_path(go_atmo *a, const go_params *p, const double src_in[3], do
         src[2] = src_in[1] * (GO_PI / 180.0);
         src[0] = src_in[2] > ground(a) - a->r0 ? src_in[2] : ground(a) - a->r0;
         s.src[0] = a->r0 + src[0]; s.src[1] = src[1]; s.src[2] = src[2];
-        eig_accept_sph(a, &p18, &s, src, bnc_cnt, freq, rows, length, A, &sink);
+        go_eig_med M = strat_med(a);
+        eig_accept_sph(&M, &p18, &s, src, bnc_cnt, freq, rows, length, A, &sink);
     } else {
         double src[3] = {src_in[0], src_in[1], src_in[2] > ground(a) ? src_in[2] : ground(a)};
         s.src[0] = src[0]; s.src[1] = src[1]; s.src[2] = src[2];
@@ -1691,8 +1749,8 @@ static int find_eigenray_sph(eig_ctx *E, const double src[3], const double rcvr[
                              double freq, int bnc_cnt, int iterate_limit, go_arrival *A)
 {
     const go_params *p = &E->p18;
-    go_atmo *a = E->a;
-    const double r0 = a->r0;
+    const go_eig_med *M = E->med;
+    const double r0 = M->r0;
     int brk, k, length = (int)(p->s_max / p->ds_min);
     double dr, dr_prev = 10000.0, step_sc = 1.0;
     long double r_grnd, c_src, c_grnd, dzg_dlat, dzg_dlon, ds_norm, ds_dth, ds_dph, dlat, dlon;
@@ -1705,14 +1763,14 @@ static int find_eigenray_sph(eig_ctx *E, const double src[3], const double rcvr[
     s.phi = ph_est;
     for (int n = 0; n <= iterate_limit; n++) {
         if (n == iterate_limit) break;
-        k = trace_legs(E, p, &s, rows, length, bnc_cnt, &brk, 0);    /* eigenray.sph.cpp:178-201 */
+        k = trace_legs_m(E, p, &s, rows, length, bnc_cnt, &brk);    /* eigenray.sph.cpp:178-201 */
         eig_cnt[GO_EIG_LM_TRACES]++;
         if (brk) { s.theta -= dth * step_sc; s.phi -= dph * step_sc; step_sc /= 2.0; }
         if (!brk) {
             double *yk = rows + 18 * k;
             dr = go_gc_dist(r0, yk[1], yk[2], rcvr[0], rcvr[1]);
             if (dr < E->tolerance) {
-                E->steps += eig_accept_sph(a, p, &s, src, bnc_cnt, freq, rows, length, A, NULL);
+                E->steps += eig_accept_sph(M, p, &s, src, bnc_cnt, freq, rows, length, A, NULL);
                 eig_cnt[GO_EIG_FINAL_TRACES]++;
                 return 1;
             } else if (n > 0 && dr > dr_prev) {
@@ -1721,11 +1779,11 @@ static int find_eigenray_sph(eig_ctx *E, const double src[3], const double rcvr[
                 step_sc = (1.0 < step_sc * 1.25) ? 1.0 : step_sc * 1.25;
                 dlat = rcvr[0] - yk[1];
                 dlon = rcvr[1] - yk[2];
-                r_grnd = ground(a);                                 /* topo::z: r0 + z0 */
-                c_grnd = go_c(a, r_grnd, yk[1], yk[2]);
-                c_src = go_c(a, src[0] + r0, src[1], src[2]);
-                dzg_dlat = 0.0 / r_grnd;                            /* flat ground: topo::dz = 0 */
-                dzg_dlon = 0.0 / (r_grnd * cos(yk[1]));
+                r_grnd = M->topo_z(M->m, yk[1], yk[2]);            /* topo::z (a radius) */
+                c_grnd = M->c(M->m, r_grnd, yk[1], yk[2]);
+                c_src = M->c(M->m, src[0] + r0, src[1], src[2]);
+                dzg_dlat = M->topo_dz(M->m, yk[1], yk[2], 0) / r_grnd;
+                dzg_dlon = M->topo_dz(M->m, yk[1], yk[2], 1) / (r_grnd * cos(yk[1]));
                 ds_norm = yk[3] - dzg_dlat * yk[4] - dzg_dlon * yk[5];
                 ds_dth = -c_src / c_grnd * (yk[6] - dzg_dlat * r_grnd * yk[7] - dzg_dlon * (r_grnd * cos(yk[1])) * yk[8]) / ds_norm;
                 ds_dph = -c_src / c_grnd * (yk[12] - dzg_dlat * r_grnd * yk[13] - dzg_dlon * (r_grnd * cos(yk[1])) * yk[14]) / ds_norm;
@@ -1747,15 +1805,19 @@ static int find_eigenray_sph(eig_ctx *E, const double src[3], const double rcvr[
     return 0;
 }
 
-int go_eig_search_sph(go_atmo *a, const go_params *p, const double src_in[3],
-                      const double rcvr_in[2], double th_min_deg, double th_max_deg,
-                      int bnc_min, int bnc_max, int iterations, double damping,
-                      double tolerance, double az_err_lim, double dth_big_deg,
-                      double dth_sml_deg, double freq,
-                      go_arrival *out, int out_cap, long long *ray_steps)
+/* The spherical search of run_eig_search (main.infraga.sph.cpp:962-989) over
+ * a medium: src = (alt [km], lat, lon [rad]) raised to the ground by the
+ * caller, rcvr [rad], th_grnd the lowest inclination [rad] (or whatever the
+ * caller's driver computes there). */
+static int eig_search_sph_core(const go_eig_med *M, const go_params *p, const double src[3], const double rcvr[2],
+                               double th_min, double th_max, double th_grnd, int bnc_min, int bnc_max,
+                               int iterations, double damping, double tolerance, double az_err_lim,
+                               double dth_big_deg, double dth_sml_deg, double freq,
+                               go_arrival *out, int out_cap, long long *ray_steps)
 {
     eig_ctx E;
-    E.a = a;
+    E.a = NULL;
+    E.med = M;
     E.p6 = *p; E.p6.calc_amp = 0; E.p6.eq_cnt = 6;
     E.p18 = *p; E.p18.calc_amp = 1; E.p18.eq_cnt = 18;
     E.dth_big = dth_big_deg * (GO_PI / 180.0);
@@ -1764,16 +1826,6 @@ int go_eig_search_sph(go_atmo *a, const go_params *p, const double src_in[3],
     E.tolerance = tolerance;
     E.steps = 0;
     memset(eig_cnt, 0, sizeof(eig_cnt));
-    /* main.infraga.sph.cpp:906-909 */
-    double src[3], rcvr[2];
-    src[1] = src_in[0] * (GO_PI / 180.0);
-    src[2] = src_in[1] * (GO_PI / 180.0);
-    rcvr[0] = rcvr_in[0] * (GO_PI / 180.0);
-    rcvr[1] = rcvr_in[1] * (GO_PI / 180.0);
-    src[0] = src_in[2] > ground(a) - a->r0 ? src_in[2] : ground(a) - a->r0;
-    double th_min = th_min_deg * (GO_PI / 180.0), th_max = th_max_deg * (GO_PI / 180.0);
-    /* ground source: theta_grnd = 0.1 deg (sph.cpp:966-979) */
-    double th_grnd = (src[0] - (ground(a) - a->r0) < 0.01) ? 0.1 * (GO_PI / 180.0) : -89.9 * (GO_PI / 180.0);
     int cnt = 0;
     for (int nb = bnc_min; nb <= bnc_max; nb++) {
         double th_start = th_min > th_grnd ? th_min : th_grnd;
@@ -1794,6 +1846,54 @@ int go_eig_search_sph(go_atmo *a, const go_params *p, const double src_in[3],
     if (ray_steps) *ray_steps = E.steps;
     eig_cnt[GO_EIG_FOUND] = cnt;
     return cnt;
+}
+
+int go_eig_search_sph(go_atmo *a, const go_params *p, const double src_in[3],
+                      const double rcvr_in[2], double th_min_deg, double th_max_deg,
+                      int bnc_min, int bnc_max, int iterations, double damping,
+                      double tolerance, double az_err_lim, double dth_big_deg,
+                      double dth_sml_deg, double freq,
+                      go_arrival *out, int out_cap, long long *ray_steps)
+{
+    /* main.infraga.sph.cpp:906-909 */
+    double src[3], rcvr[2];
+    src[1] = src_in[0] * (GO_PI / 180.0);
+    src[2] = src_in[1] * (GO_PI / 180.0);
+    rcvr[0] = rcvr_in[0] * (GO_PI / 180.0);
+    rcvr[1] = rcvr_in[1] * (GO_PI / 180.0);
+    src[0] = src_in[2] > ground(a) - a->r0 ? src_in[2] : ground(a) - a->r0;
+    double th_min = th_min_deg * (GO_PI / 180.0), th_max = th_max_deg * (GO_PI / 180.0);
+    /* ground source: theta_grnd = 0.1 deg (sph.cpp:966-979) */
+    double th_grnd = (src[0] - (ground(a) - a->r0) < 0.01) ? 0.1 * (GO_PI / 180.0) : -89.9 * (GO_PI / 180.0);
+    go_eig_med M = strat_med(a);
+    return eig_search_sph_core(&M, p, src, rcvr, th_min, th_max, th_grnd, bnc_min, bnc_max, iterations, damping,
+                               tolerance, az_err_lim, dth_big_deg, dth_sml_deg, freq, out, out_cap, ray_steps);
+}
+
+int go_eig_search_sph_med(const go_eig_med *M, const go_params *p, const double src[3], const double rcvr[2],
+                          double th_min, double th_max, double th_grnd, int bnc_min, int bnc_max, int iterations,
+                          double damping, double tolerance, double az_err_lim, double dth_big_deg,
+                          double dth_sml_deg, double freq, go_arrival *out, int out_cap, long long *ray_steps)
+{
+    return eig_search_sph_core(M, p, src, rcvr, th_min, th_max, th_grnd, bnc_min, bnc_max, iterations, damping,
+                               tolerance, az_err_lim, dth_big_deg, dth_sml_deg, freq, out, out_cap, ray_steps);
+}
+
+long long go_eig_path_sph_med(const go_eig_med *M, const go_params *p, const double src[3], double th, double ph,
+                              int bnc_cnt, double freq, go_path_pt *pts, long long cap, go_arrival *A)
+{
+    go_params p18 = *p;
+    p18.calc_amp = 1;
+    p18.eq_cnt = 18;
+    int length = (int)(p18.s_max / p18.ds_min);
+    double *rows = get_rows(length);
+    path_sink sink = {pts, cap, 0};
+    go_src s;
+    s.theta = th;
+    s.phi = ph;
+    s.src[0] = M->r0 + src[0]; s.src[1] = src[1]; s.src[2] = src[2];
+    eig_accept_sph(M, &p18, &s, src, bnc_cnt, freq, rows, length, A, &sink);
+    return sink.n;
 }
 
 /* run_eig_direct (main.infraga.3d.cpp:955-1086, main.infraga.sph.cpp:998-1139):
@@ -1828,6 +1928,8 @@ int go_eig_direct(go_atmo *a, const go_params *p, const double src_in[3], const 
         } else {
             ph_est = GO_PI / 2.0 - az_est_deg * (GO_PI / 180.0);
         }
+        go_eig_med M = strat_med(a);
+        E.med = &M;
         ok = find_eigenray_sph(&E, src, rcvr, th_est, ph_est, freq, bnc, iterations, A);
     } else {
         double src[3] = {src_in[0], src_in[1], src_in[2] > ground(a) ? src_in[2] : ground(a)};   /* 3d.cpp:1048 */

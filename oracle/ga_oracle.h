@@ -222,6 +222,36 @@ int    go_eig_direct(go_atmo *a, const go_params *p, const double src[3], const 
                      double incl_est_deg, double az_est_deg, int bnc, int iterations, double damping,
                      double tolerance, double freq, go_arrival *A, long long *ray_steps);
 
+/* A spherical medium's operations for the eigenray search (the stratified
+ * medium and the range-dependent one of ga_oracle_rd.c). */
+typedef struct go_eig_med {
+    void *m;
+    double r0;
+    void   (*set_initial)(void *m, const go_params *p, go_src *s, double *y0);
+    int    (*prop_rk4)(void *m, const go_params *p, const go_src *s, double *rows, int length, int *check);
+    void   (*set_refl)(void *m, const go_params *p, const go_src *s, const double *rk2, const double *rk1,
+                       double *out);
+    double (*tt_seg)(void *m, const double *y0, const double *y1);
+    double (*att_seg)(void *m, const double *y0, const double *y1, double freq);
+    double (*c)(void *m, double r, double lat, double lon);
+    double (*amp)(void *m, const go_src *s, const double *y);
+    double (*topo_z)(void *m, double lat, double lon);          /* topo::z: a radius */
+    double (*topo_dz)(void *m, double lat, double lon, int n);  /* topo::dz, n = 0 lat, 1 lon */
+} go_eig_med;
+
+/* The spherical search over a medium: src = (alt [km], lat, lon [rad]) already
+ * raised to the ground, rcvr [rad], th_* [rad]; th_grnd is the lowest
+ * inclination the driver starts from.  Returns the eigenray count. */
+int    go_eig_search_sph_med(const go_eig_med *M, const go_params *p, const double src[3], const double rcvr[2],
+                             double th_min, double th_max, double th_grnd, int bnc_min, int bnc_max,
+                             int iterations, double damping, double tolerance, double az_err_lim,
+                             double dth_big_deg, double dth_sml_deg, double freq,
+                             go_arrival *out, int out_cap, long long *ray_steps);
+/* The accepted spherical eigenray's path rows (pts may be NULL) and row for
+ * launch angles th, ph [rad] over a medium. */
+long long go_eig_path_sph_med(const go_eig_med *M, const go_params *p, const double src[3], double th, double ph,
+                              int bnc_cnt, double freq, go_path_pt *pts, long long cap, go_arrival *A);
+
 /* Work counters of the last go_eig_search_3d / go_eig_search_sph call. */
 enum { GO_EIG_EST, GO_EIG_SWEEP_TRACES, GO_EIG_SWEEP_CORR, GO_EIG_SWEEP_ADAPT, GO_EIG_LM_TRACES,
        GO_EIG_FINAL_TRACES, GO_EIG_FINDS, GO_EIG_FOUND, GO_EIG_NCOUNT };

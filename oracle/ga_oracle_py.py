@@ -428,6 +428,15 @@ def _rd_bind(L):
     L.go_rd_fan.argtypes = [C.c_void_p, C.POINTER(Params), C.POINTER(Prop), _dp, C.c_int,
                             C.POINTER(Arrival), C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_longlong)]
     L.go_rd_fan_paths.restype = C.c_int
+    L.go_rd_eig_search.restype = C.c_int
+    L.go_rd_eig_search.argtypes = [C.c_void_p, C.POINTER(Params), _dp, _dp, C.c_double, C.c_double, C.c_int, C.c_int,
+                                   C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                   C.POINTER(Arrival), C.c_int, C.POINTER(C.c_longlong)]
+    L.go_rd_eig_theta_grnd.restype = C.c_double
+    L.go_rd_eig_theta_grnd.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
+    L.go_rd_eig_path.restype = C.c_longlong
+    L.go_rd_eig_path.argtypes = [C.c_void_p, C.POINTER(Params), _dp, C.c_double, C.c_double, C.c_int, C.c_double,
+                                 C.POINTER(PathPt), C.c_longlong, C.POINTER(Arrival)]
     L.go_rd_fan_paths.argtypes = [C.c_void_p, C.POINTER(Params), C.POINTER(Prop), _dp, C.c_int, C.c_int,
                                   C.POINTER(Arrival), C.c_int, C.POINTER(C.c_int), C.POINTER(PathPt), C.c_longlong,
                                   C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
@@ -592,6 +601,30 @@ class RdMedium:
         if rc != 0:
             raise RuntimeError("oracle fan overflow")
         return arrivals_to_array(out, n_out.value), steps.value
+
+    def eig_search(self, params, src, rcvr, incl_min=0.5, incl_max=45.0, bnc_min=0, bnc_max=0, iterations=25,
+                   damping=1.0e-3, tolerance=0.1, az_dev_lim=2.0, incl_step_max=0.1, incl_step_min=0.001, freq=0.1):
+        """infraga-sph-rngdep -eig_search (go_rd_eig_search): (eigenrays, ray-steps)."""
+        s = np.ascontiguousarray(src, dtype=np.float64)
+        r = np.ascontiguousarray(rcvr, dtype=np.float64)
+        cap = 64
+        out = (Arrival * cap)()
+        steps = C.c_longlong(0)
+        n = self.o.lib.go_rd_eig_search(self.h, C.byref(params), _ptr(s), _ptr(r), incl_min, incl_max, bnc_min,
+                                        bnc_max, iterations, damping, tolerance, az_dev_lim, incl_step_max,
+                                        incl_step_min, freq, out, cap, C.byref(steps))
+        return arrivals_to_array(out, min(n, cap)), steps.value
+
+    def eig_path(self, params, src, incl_deg, az_deg, bounces, freq=0.1):
+        """The accepted eigenray's path rows and row (go_rd_eig_path)."""
+        s = np.ascontiguousarray(src, dtype=np.float64)
+        A = Arrival()
+        n = self.o.lib.go_rd_eig_path(self.h, C.byref(params), _ptr(s), float(incl_deg), float(az_deg), int(bounces),
+                                      freq, None, 0, C.byref(A))
+        pts = (PathPt * max(1, n))()
+        n = self.o.lib.go_rd_eig_path(self.h, C.byref(params), _ptr(s), float(incl_deg), float(az_deg), int(bounces),
+                                      freq, pts, n, C.byref(A))
+        return paths_to_array(pts, n), arrivals_to_array((Arrival * 1)(A), 1)[0]
 
     def fan_paths(self, launches, params, flags=PATH_RAYS, bounces=2, freq=0.1, turn_ht_min=0.2, src=None):
         """write_rays / write_caustics run_prop (go_rd_fan_paths): (arrivals, path rows, ray-steps)."""

@@ -1146,6 +1146,85 @@ int go_rd_fan_paths(go_rd *a, const go_params *p, const go_prop *pr, const doubl
     return (cnt > out_cap || np > pts_cap) ? -1 : 0;
 }
 
+/* ---- eigenray search (infraga-sph-rngdep -eig_search / -eig_direct) ---- */
+static void rm_init(void *m, const go_params *p, go_src *s, double *y) { go_rd_set_initial((go_rd *)m, p, s, y); }
+static int rm_prop(void *m, const go_params *p, const go_src *s, double *rows, int length, int *check)
+{
+    return go_rd_prop_rk4((go_rd *)m, p, s, rows, length, check);
+}
+static void rm_refl(void *m, const go_params *p, const go_src *s, const double *a, const double *b, double *o)
+{
+    go_rd_set_refl((go_rd *)m, p, s, a, b, o);
+}
+static double rm_tt(void *m, const double *a, const double *b) { return go_rd_travel_time_seg((go_rd *)m, a, b); }
+static double rm_att(void *m, const double *a, const double *b, double f) { return go_rd_atten_seg((go_rd *)m, a, b, f); }
+static double rm_c(void *m, double r, double t, double p) { return go_rd_c((go_rd *)m, r, t, p); }
+static double rm_amp(void *m, const go_src *s, const double *y) { return go_rd_amp((go_rd *)m, s, y); }
+static double rm_tz(void *m, double t, double p) { return go_rd_topo_z((go_rd *)m, t, p); }
+static double rm_tdz(void *m, double t, double p, int n) { return go_rd_topo_dz((go_rd *)m, t, p, n); }
+
+static go_eig_med rd_med(go_rd *a)
+{
+    go_eig_med M = {a, a->r0, rm_init, rm_prop, rm_refl, rm_tt, rm_att, rm_c, rm_amp, rm_tz, rm_tdz};
+    return M;
+}
+
+/* Source conversion of run_eig_search (main.infraga.sph.rngdep.cpp:877-880):
+ * degrees to radians, altitude raised to the terrain. */
+static void rd_eig_src(go_rd *a, const double src_in[3], const double rcvr_in[2], double src[3], double rcvr[2])
+{
+    src[1] = src_in[0] * (RD_PI / 180.0);
+    src[2] = src_in[1] * (RD_PI / 180.0);
+    rcvr[0] = rcvr_in[0] * (RD_PI / 180.0);
+    rcvr[1] = rcvr_in[1] * (RD_PI / 180.0);
+    src[0] = dmax(src_in[2], go_rd_topo_z(a, src[1], src[2]) - a->r0);
+}
+
+/* theta_grnd of run_eig_search (:938-954): with terrain the largest slope
+ * angle over azimuths 0, 3, ..., 357 in DEGREES plus 0.1, compared with the
+ * radian inclination bounds as it stands; without terrain 0.1 deg in
+ * radians; -89.9 deg in radians for an elevated source. */
+double go_rd_eig_theta_grnd(go_rd *a, double src_alt, double lat, double lon)
+{
+    if (src_alt - (go_rd_topo_z(a, lat, lon) - a->r0) < 0.01) {
+        if (a->is_topo) {
+            double th = 0.0;
+            for (double ph = 0.0; ph < 360.0; ph += 3.0) {
+                double dzgdx = 1.0 / (a->r0 * cos(lat)) * go_rd_topo_dz(a, lat, lon, 1);
+                double dzgdy = 1.0 / a->r0 * go_rd_topo_dz(a, lat, lon, 0);
+                th = dmax(th, atan(dzgdx * sin(ph * RD_PI / 180.0) + dzgdy * cos(ph * RD_PI / 180.0)) * (180.0 / RD_PI) + 0.1);
+            }
+            return th;
+        }
+        return 0.1 * (RD_PI / 180.0);
+    }
+    return -89.9 * (RD_PI / 180.0);
+}
+
+int go_rd_eig_search(go_rd *a, const go_params *p, const double src_in[3], const double rcvr_in[2],
+                     double th_min_deg, double th_max_deg, int bnc_min, int bnc_max, int iterations,
+                     double damping, double tolerance, double az_err_lim, double dth_big_deg, double dth_sml_deg,
+                     double freq, go_arrival *out, int out_cap, long long *ray_steps)
+{
+    double src[3], rcvr[2];
+    rd_eig_src(a, src_in, rcvr_in, src, rcvr);
+    go_eig_med M = rd_med(a);
+    double th_grnd = go_rd_eig_theta_grnd(a, src[0], src[1], src[2]);
+    return go_eig_search_sph_med(&M, p, src, rcvr, th_min_deg * (RD_PI / 180.0), th_max_deg * (RD_PI / 180.0),
+                                 th_grnd, bnc_min, bnc_max, iterations, damping, tolerance, az_err_lim, dth_big_deg,
+                                 dth_sml_deg, freq, out, out_cap, ray_steps);
+}
+
+long long go_rd_eig_path(go_rd *a, const go_params *p, const double src_in[3], double incl_deg, double az_deg,
+                         int bnc_cnt, double freq, go_path_pt *pts, long long cap, go_arrival *A)
+{
+    double src[3], rcvr[2], dummy[2] = {0.0, 0.0};
+    rd_eig_src(a, src_in, dummy, src, rcvr);
+    go_eig_med M = rd_med(a);
+    return go_eig_path_sph_med(&M, p, src, incl_deg * (RD_PI / 180.0), (90.0 - az_deg) * (RD_PI / 180.0), bnc_cnt,
+                               freq, pts, cap, A);
+}
+
 /* ---- handles ------------------------------------------------------------ */
 void *go_rd_new(const double *lat_deg, int nlat, const double *lon_deg, int nlon, const double *z, int nz,
                 const double *c, const double *u, const double *v, const double *rho)

@@ -114,6 +114,19 @@ int    go_rd_fan_paths(go_rd *a, const go_params *p, const go_prop *pr, const do
                        int n_launch, int flags, go_arrival *out, int out_cap, int *n_out,
                        go_path_pt *pts, long long pts_cap, long long *n_pts, long long *ray_steps);
 
+/* infraga-sph-rngdep -eig_search (main.infraga.sph.rngdep.cpp:777-965): the
+ * spherical search of geoac.eigenray.sph.cpp over this medium and terrain;
+ * src = (lat, lon [deg], alt [km]), rcvr = (lat, lon) [deg]. */
+int    go_rd_eig_search(go_rd *a, const go_params *p, const double src_in[3], const double rcvr_in[2],
+                        double th_min_deg, double th_max_deg, int bnc_min, int bnc_max, int iterations,
+                        double damping, double tolerance, double az_err_lim, double dth_big_deg,
+                        double dth_sml_deg, double freq, go_arrival *out, int out_cap, long long *ray_steps);
+/* The search's lowest inclination (with terrain: degrees, as the reference computes it). */
+double go_rd_eig_theta_grnd(go_rd *a, double src_alt, double lat, double lon);
+/* The accepted eigenray's path rows and row for launch angles in degrees. */
+long long go_rd_eig_path(go_rd *a, const go_params *p, const double src_in[3], double incl_deg, double az_deg,
+                         int bnc_cnt, double freq, go_path_pt *pts, long long cap, go_arrival *A);
+
 /* ---- opaque handles for ctypes ------------------------------------------ */
 void  *go_rd_new(const double *lat_deg, int nlat, const double *lon_deg, int nlon, const double *z,
                  int nz, const double *c, const double *u, const double *v, const double *rho);

@@ -122,7 +122,9 @@ def test_accel_launch_list_is_the_mpi_drivers(toy_met, tmp_path):
     ("-prop", "prefix", "lats.dat"),
     ("-prop", "prefix", "lats.dat", "lons.dat", "no_such_key=1"),
     ("-prop", "prefix", "lats.dat", "lons.dat", "write_topo=true"),
-    ("-eig_search", "prefix", "lats.dat", "lons.dat"),
+    ("-eig_search", "prefix", "lats.dat"),
+    ("-eig_search", "prefix", "lats.dat", "lons.dat", "no_such_key=1"),
+    ("-eig_direct", "prefix", "lats.dat", "lons.dat"),
 ])
 def test_rngdep_rejected_arguments(args):
     r = run("infraga-sph-rngdep", *args)

@@ -346,3 +346,30 @@ def test_theta_grnd_follows_terrain_slope(oracle):
         slope = (t[2] / (R0 * math.cos(la)) * math.sin(math.radians(az)) + t[1] / R0 * math.cos(math.radians(az)))
         assert m.theta_grnd(la, lo, zs, az) == pytest.approx(math.degrees(math.atan(slope)) + 0.1, rel=1e-12)
     assert m.theta_grnd(la, lo, zs + 1.0, 0.0) == -89.9
+
+
+def test_rngdep_eigenray_search_reduces_to_stratified(oracle, toyatmo_units):
+    """The range-dependent eigenray search (go_rd_eig_search: the spherical
+    search over the range-dependent medium) on a grid of identical ToyAtmo
+    columns over flat ground finds the stratified search's eigenray with the
+    same ray-steps, launch angles and arrival row."""
+    z, c, u, v, rho = toyatmo_units
+    lat = np.array([20.0, 30.0, 40.0])
+    lon = np.array([-110.0, -100.0, -90.0, -80.0])
+
+    def rep(f):
+        return np.ascontiguousarray(np.broadcast_to(f, (3, 4, f.size)))
+    rd = RdMedium(oracle, lat, lon, z, rep(c), rep(u), rep(v), rep(rho))
+    st = Medium(oracle, GEOM_SPH, z, c, u, v, rho)
+    pr, ps = rd.params(True), st.params(True)
+    for k in ("lat_min", "lat_max", "lon_min", "lon_max", "alt_max"):
+        setattr(ps, k, getattr(pr, k))
+    kw = dict(incl_min=9.5, incl_max=10.6, bnc_min=0, bnc_max=0)
+    src, rcvr = (30.0, -100.0, 0.0), (29.985815, -101.96135)   # the reference's 10 deg ToyAtmo arrival
+    a1, s1 = rd.eig_search(pr, src, rcvr, **kw)
+    a2, s2 = st.eig_search(ps, src, rcvr, **kw)
+    assert s1 == s2 and len(a1) == len(a2) == 1
+    for f in ("incl_launch", "az_launch", "pos0", "pos1", "time", "turn_ht", "back_az", "absorption"):
+        np.testing.assert_allclose(a1[f], a2[f], rtol=1e-11, atol=1e-11, err_msg=f)
+    np.testing.assert_allclose(a1["tl"], a2["tl"], atol=1e-8)
+    np.testing.assert_allclose(a1["incl_arr"], a2["incl_arr"], atol=1e-8)
