@@ -66,6 +66,11 @@ extern "C" {
                               of each launch azimuth (infraga-2d); arrivals:
                               pos0 = range r [km], pos1 = out-of-plane drift
                               of the last leg [km], back_az unused (0) */
+#define GA_GEOM_3D_RNGDEP 4  /* 3D Cartesian range-dependent (infraga-3d-rngdep):
+                                atmo_id a ga_rngdep_load_xy id; pos0, pos1 =
+                                x, y [km]; src[0..1] x, y [km] (NaN: the
+                                region centre); bounds x_min, x_max, y_min,
+                                y_max [km] (NaN: the grid's) */
 
 typedef struct ga_ctx ga_ctx;
 
@@ -274,6 +279,15 @@ int ga_topo_eval_2d(ga_ctx *ctx, int atmo_id, double r_km, double out[3]);
 int ga_rngdep_load(ga_ctx *ctx, const double *lat_deg, int nlat, const double *lon_deg, int nlon,
                    const double *z_km, int nz, const double *c, const double *u, const double *v,
                    const double *rho, int *rd_id);
+/* ga_rngdep_load_xy replaces the loader of the 3D Cartesian range-dependent
+ * geometry (set_region, src/atmo/atmo_io.3d.rngdep.cpp:56-225): the same
+ * node grid with x / y node coordinates in km; a topography loaded on such
+ * an id (ga_topo_load) takes its node coordinates in km too.
+ * ga_rngdep_load_files_xy reads the reference's x / y node files. */
+int ga_rngdep_load_xy(ga_ctx *ctx, const double *x_km, int nx, const double *y_km, int ny, const double *z_km, int nz,
+                      const double *c, const double *u, const double *v, const double *rho, int *rd_id);
+int ga_rngdep_load_files_xy(ga_ctx *ctx, const char *prefix, const char *x_path, const char *y_path,
+                            const char *topo_path, const char *format, int reverse_winds, int *rd_id);
 int ga_topo_load(ga_ctx *ctx, int rd_id, const double *lat_deg, int nlat, const double *lon_deg, int nlon,
                  const double *zg_km);
 int ga_rngdep_eval(ga_ctx *ctx, int rd_id, const double *pts, int n, double *atmo_out, double *topo_out);

@@ -133,7 +133,7 @@ __global__ void __launch_bounds__(256) rd_eval_kernel(RdGrid g, TopoGrid t, int 
 #ifndef GA_WG_RD
 #define GA_WG_RD 64
 #endif
-template <int NEQ, bool ONE, bool BLW, bool PATHS = false>
+template <int NEQ, bool ONE, bool BLW, bool PATHS = false, bool CART = false>
 __global__ void __launch_bounds__(GA_WG_RD) rd_fan_kernel(RdFanArgs A)
 {
     // three RK4 rows per lane (state, accumulator, stage point), lane-interleaved
@@ -143,17 +143,17 @@ __global__ void __launch_bounds__(GA_WG_RD) rd_fan_kernel(RdFanArgs A)
         ys{rows + 2 * NEQ * GA_WG_RD + threadIdx.x};
     if constexpr (ONE) {
         bool first = true;
-        rd_lane_loop<NEQ, BLW>(A, lane, [&]() {
+        rd_lane_loop<NEQ, BLW, false, CART>(A, lane, [&]() {
             const int r = first ? lane : A.n_rays;
             first = false;
             return r;
         }, ya, yb, ys);
     } else {
-        rd_lane_loop<NEQ, BLW, PATHS>(A, lane, [&]() { return (int)atomicAdd(A.queue, 1u); }, ya, yb, ys);
+        rd_lane_loop<NEQ, BLW, PATHS, CART>(A, lane, [&]() { return (int)atomicAdd(A.queue, 1u); }, ya, yb, ys);
     }
 }
 
-template <int NEQ>
+template <int NEQ, bool CART = false>
 __global__ void __launch_bounds__(256) rd_arrival_kernel(RdFanArgs A)
 {
     const long long slot = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -165,8 +165,12 @@ __global__ void __launch_bounds__(256) rd_arrival_kernel(RdFanArgs A)
     double y[18];
 #pragma unroll
     for (int i = 0; i < 18; ++i) y[i] = (i < NEQ) ? r[i] : 0.0;
-    A.out[slot] = rd_arrival_row<NEQ>(A, y, ray, bnc, A.launches[2 * ray], A.launches[2 * ray + 1], r[18], r[19],
-                                      r[20], r[23], r[24], r[21], r[22]);
+    if (CART)
+        A.out[slot] = rd3_arrival_row<NEQ>(A, y, ray, bnc, A.launches[2 * ray], A.launches[2 * ray + 1], r[18], r[19],
+                                           r[20], r[23], r[21], r[22]);
+    else
+        A.out[slot] = rd_arrival_row<NEQ>(A, y, ray, bnc, A.launches[2 * ray], A.launches[2 * ray + 1], r[18], r[19],
+                                          r[20], r[23], r[24], r[21], r[22]);
 }
 
 // Terrain height above sea level at one point (the source of a fan).
@@ -259,6 +263,7 @@ struct DevRd {
     std::vector<double> hx, hy, hz;    // host copies of the node axes [rad, rad, km]
     std::vector<double> thx, thy;      // terrain axes [rad]
     double t_max = 0.0;                // terrain maximum, floored at 0 (set_bndlyr)
+    bool cart = false;                 // 3D Cartesian grid: node axes in km
     void release()
     {
         for (double *p : {d_axes, d_tab, d_taxes, d_ttab})
@@ -515,7 +520,7 @@ struct RdRegion {
 
 bool rd_region(const DevRd &R, double z_grnd, RdRegion &G)
 {
-    const double buffer = 1.0e-3 * (PI / 180.0);
+    const double buffer = R.cart ? 1.0e-3 : 1.0e-3 * (PI / 180.0);   // atmo_io.3d.rngdep.cpp:25
     G.lat_min = R.hx.front() + buffer;
     G.lat_max = R.hx.back() - buffer;
     G.lon_min = R.hy.front() + buffer;
@@ -549,6 +554,12 @@ int rd_setup(ga_ctx *ctx, FanBufs &fb, int rd_id, const ga_prop_params *p, RdFan
     auto it = ctx->rds.find(rd_id);
     if (it == ctx->rds.end()) return fail(ctx, GA_E_ATMO, "unknown range-dependent atmosphere id %d", rd_id);
     const DevRd &R = it->second;
+    const bool cart = p->geometry == GA_GEOM_3D_RNGDEP;
+    if (cart != R.cart)
+        return fail(ctx, GA_E_ARG, cart ? "GA_GEOM_3D_RNGDEP needs a ga_rngdep_load_xy atmosphere"
+                                        : "a ga_rngdep_load_xy atmosphere needs GA_GEOM_3D_RNGDEP");
+    if (cart && p->topo_use_blw)
+        return fail(ctx, GA_E_ARG, "topo_use_BLw is not supported in the 3D range-dependent geometry");
     if (p->bounces < 0) return fail(ctx, GA_E_ARG, "bounces must be >= 0");
     if (!(p->ds_min > 0.0) || !(p->ds_max >= p->ds_min) || !(p->s_max > 0.0))
         return fail(ctx, GA_E_ARG, "invalid step parameters ds_min=%g ds_max=%g s_max=%g", p->ds_min, p->ds_max, p->s_max);
@@ -558,7 +569,7 @@ int rd_setup(ga_ctx *ctx, FanBufs &fb, int rd_id, const ga_prop_params *p, RdFan
     M.t = topo_grid_of(R);
     M.is_topo = R.d_ttab ? 1 : 0;
     M.use_blw = p->topo_use_blw ? 1 : 0;
-    M.r0 = p->r_earth;
+    M.r0 = cart ? 0.0 : p->r_earth;
     M.tweak = std::max(0.0, p->abs_coeff);
     M.sbc[0] = std::pow(10.0, -0.67887);
     M.sbc[1] = std::pow(10.0, -0.10744);
@@ -577,20 +588,19 @@ int rd_setup(ga_ctx *ctx, FanBufs &fb, int rd_id, const ga_prop_params *p, RdFan
     L.rng_max = p->rng_max;
     L.time_max = p->time_max;
     L.z_keep = M.z_max + 4.0 * p->ds_max;
-    L.b0 = std::isnan(p->bounds[0]) ? lat_min : p->bounds[0] * (PI / 180.0);
-    L.b1 = std::isnan(p->bounds[1]) ? lat_max : p->bounds[1] * (PI / 180.0);
-    L.b2 = std::isnan(p->bounds[2]) ? lon_min : p->bounds[2] * (PI / 180.0);
-    L.b3 = std::isnan(p->bounds[3]) ? lon_max : p->bounds[3] * (PI / 180.0);
-    {
+    const double deg = cart ? 1.0 : PI / 180.0;   // bounds and source in degrees (sph) or km (3D)
+    L.b0 = std::isnan(p->bounds[0]) ? lat_min : p->bounds[0] * deg;
+    L.b1 = std::isnan(p->bounds[1]) ? lat_max : p->bounds[1] * deg;
+    L.b2 = std::isnan(p->bounds[2]) ? lon_min : p->bounds[2] * deg;
+    L.b3 = std::isnan(p->bounds[3]) ? lon_max : p->bounds[3] * deg;
+    if (!cart) {
         double q = p->rng_max / (2.0 * M.r0);
         double sq = q >= PI / 2.0 ? 2.0 : std::sin(q);
         L.gc_thresh = sq >= 1.0 ? 2.0 : sq * sq;
     }
     // source: region centre by default, at max(z_src, terrain) (:242-243, :311-315)
-    double lat_deg = std::isnan(p->src[0]) ? (lat_max + lat_min) / 2.0 * (180.0 / PI) : p->src[0];
-    double lon_deg = std::isnan(p->src[1]) ? (lon_max + lon_min) / 2.0 * (180.0 / PI) : p->src[1];
-    A.src1 = lat_deg * (PI / 180.0);
-    A.src2 = lon_deg * (PI / 180.0);
+    A.src1 = std::isnan(p->src[0]) ? (lat_max + lat_min) / 2.0 : p->src[0] * deg;
+    A.src2 = std::isnan(p->src[1]) ? (lon_max + lon_min) / 2.0 : p->src[1] * deg;
     double zsrc = p->src[2];
     if (M.is_topo) {
         if (!fb.scalar && hipMalloc(&fb.scalar, sizeof(double)) != hipSuccess) return fail(ctx, GA_E_NOMEM, "scalar");
@@ -661,7 +671,13 @@ int rd_fan_enqueue(ga_ctx *ctx, FanBufs &fb, int rd_id, const ga_prop_params *p,
     // [amp][blw], write mode
     const RdKern pkerns[2][2] = {{rd_fan_kernel<6, false, false, true>, rd_fan_kernel<6, false, true, true>},
                                  {rd_fan_kernel<18, false, false, true>, rd_fan_kernel<18, false, true, true>}};
-    const RdKern kern = path ? pkerns[amp ? 1 : 0][blw ? 1 : 0] : kerns[amp ? 1 : 0][A.one_shot ? 1 : 0][blw ? 1 : 0];
+    // 3D Cartesian: [amp][one_shot], write mode [amp]
+    const RdKern ckerns[2][2] = {{rd_fan_kernel<6, false, false, false, true>, rd_fan_kernel<6, true, false, false, true>},
+                                 {rd_fan_kernel<18, false, false, false, true>, rd_fan_kernel<18, true, false, false, true>}};
+    const RdKern cpkerns[2] = {rd_fan_kernel<6, false, false, true, true>, rd_fan_kernel<18, false, false, true, true>};
+    const bool cart = p->geometry == GA_GEOM_3D_RNGDEP;
+    const RdKern kern = cart ? (path ? cpkerns[amp ? 1 : 0] : ckerns[amp ? 1 : 0][A.one_shot ? 1 : 0])
+                             : (path ? pkerns[amp ? 1 : 0][blw ? 1 : 0] : kerns[amp ? 1 : 0][A.one_shot ? 1 : 0][blw ? 1 : 0]);
     int per_cu = 0;
     constexpr int W = GA_WG_RD;
     HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, W, 0));
@@ -685,7 +701,10 @@ int rd_fan_enqueue(ga_ctx *ctx, FanBufs &fb, int rd_id, const ga_prop_params *p,
     HIPCHK(ctx, hipGetLastError());
     long long slots = (long long)n * (p->bounces + 1);
     int agrid = (int)((slots + 255) / 256);
-    if (amp) hipLaunchKernelGGL(rd_arrival_kernel<18>, dim3(agrid), dim3(256), 0, st, A);
+    if (cart) {
+        if (amp) hipLaunchKernelGGL((rd_arrival_kernel<18, true>), dim3(agrid), dim3(256), 0, st, A);
+        else hipLaunchKernelGGL((rd_arrival_kernel<6, true>), dim3(agrid), dim3(256), 0, st, A);
+    } else if (amp) hipLaunchKernelGGL(rd_arrival_kernel<18>, dim3(agrid), dim3(256), 0, st, A);
     else hipLaunchKernelGGL(rd_arrival_kernel<6>, dim3(agrid), dim3(256), 0, st, A);
     HIPCHK(ctx, hipGetLastError());
     return GA_OK;
@@ -798,7 +817,7 @@ int fan_enqueue(ga_ctx *ctx, FanBufs &fb, int atmo_id, const ga_prop_params *p, 
                 ga_arrival *d_out, int32_t *d_counts, int32_t *d_steps, hipStream_t st,
                 const PathOut *path = nullptr)
 {
-    if (p->geometry == GA_GEOM_SPH_RNGDEP)
+    if (p->geometry == GA_GEOM_SPH_RNGDEP || p->geometry == GA_GEOM_3D_RNGDEP)
         return rd_fan_enqueue(ctx, fb, atmo_id, p, d_launches, n, d_out, d_counts, d_steps, st, path);
     if (p->geometry == GA_GEOM_2D)
         return fan2d_enqueue(ctx, fb, atmo_id, p, d_launches, n, d_out, d_counts, d_steps, st, path);
@@ -1765,6 +1784,13 @@ void ga_prop_params_default(ga_prop_params *p, int geometry)
         p->bounces = 2;
         p->rng_max = 1000.0;
         for (double &b : p->bounds) b = NAN;
+    } else if (geometry == GA_GEOM_3D_RNGDEP) {
+        // run_prop (main.infraga.3d.rngdep.cpp:205-243): source at the
+        // region centre, ten bounces, rng_max of geoac.params.3d.cpp:36
+        p->src[0] = NAN; p->src[1] = NAN; p->src[2] = 0.0;
+        p->bounces = 10;
+        p->rng_max = 1000.0;
+        for (double &b : p->bounds) b = NAN;
     } else {
         p->src[0] = 0.0; p->src[1] = 0.0; p->src[2] = 0.0;
         p->rng_max = 2000.0;   // atmo_io.3d.strat.cpp:30
@@ -2168,8 +2194,8 @@ static bool increasing(const double *v, int n)
     return true;
 }
 
-int ga_rngdep_load(ga_ctx *ctx, const double *lat_deg, int nlat, const double *lon_deg, int nlon, const double *z_km,
-                   int nz, const double *c, const double *u, const double *v, const double *rho, int *rd_id)
+static int rngdep_load(ga_ctx *ctx, const double *lat_deg, int nlat, const double *lon_deg, int nlon, const double *z_km,
+                       int nz, const double *c, const double *u, const double *v, const double *rho, int *rd_id, bool cart)
 {
     if (!ctx || !lat_deg || !lon_deg || !z_km || !c || !u || !v || !rho || !rd_id) return fail(ctx, GA_E_ARG, "null argument");
     if (nlat < 2 || nlon < 2 || nz < 3) return fail(ctx, GA_E_ARG, "grid needs >= 2 x 2 nodes and >= 3 levels");
@@ -2178,8 +2204,11 @@ int ga_rngdep_load(ga_ctx *ctx, const double *lat_deg, int nlat, const double *l
     const size_t N = (size_t)nlat * nlon * nz;
     std::vector<double> x(nlat), y(nlon), z(z_km, z_km + nz);
     // node files hold degrees; the splines run in radians (atmo_io.sph.rngdep.cpp)
-    for (int i = 0; i < nlat; ++i) x[i] = lat_deg[i] * (PI / 180.0);
-    for (int j = 0; j < nlon; ++j) y[j] = lon_deg[j] * (PI / 180.0);
+    // node files hold degrees and the splines run in radians
+    // (atmo_io.sph.rngdep.cpp); the 3D grid stays in km
+    const double deg = cart ? 1.0 : PI / 180.0;
+    for (int i = 0; i < nlat; ++i) x[i] = lat_deg[i] * deg;
+    for (int j = 0; j < nlon; ++j) y[j] = lon_deg[j] * deg;
     std::vector<double> tab(4 * N * RD_Q);
     const double *fields[4] = {c, u, v, rho};
     std::vector<double> col(nz), sl, dcol(nz), dsl;
@@ -2216,6 +2245,7 @@ int ga_rngdep_load(ga_ctx *ctx, const double *lat_deg, int nlat, const double *l
     DevRd r;
     r.nx = nlat; r.ny = nlon; r.nz = nz;
     r.hx = x; r.hy = y; r.hz = z;
+    r.cart = cart;
     if (hipMalloc(&r.d_axes, axes.size() * sizeof(double)) != hipSuccess ||
         hipMalloc(&r.d_tab, tab.size() * sizeof(double)) != hipSuccess) {
         r.release();
@@ -2232,6 +2262,18 @@ int ga_rngdep_load(ga_ctx *ctx, const double *lat_deg, int nlat, const double *l
     return GA_OK;
 }
 
+int ga_rngdep_load(ga_ctx *ctx, const double *lat_deg, int nlat, const double *lon_deg, int nlon, const double *z_km,
+                   int nz, const double *c, const double *u, const double *v, const double *rho, int *rd_id)
+{
+    return rngdep_load(ctx, lat_deg, nlat, lon_deg, nlon, z_km, nz, c, u, v, rho, rd_id, false);
+}
+
+int ga_rngdep_load_xy(ga_ctx *ctx, const double *x_km, int nx, const double *y_km, int ny, const double *z_km, int nz,
+                      const double *c, const double *u, const double *v, const double *rho, int *rd_id)
+{
+    return rngdep_load(ctx, x_km, nx, y_km, ny, z_km, nz, c, u, v, rho, rd_id, true);
+}
+
 int ga_topo_load(ga_ctx *ctx, int rd_id, const double *lat_deg, int nlat, const double *lon_deg, int nlon,
                  const double *zg_km)
 {
@@ -2241,8 +2283,9 @@ int ga_topo_load(ga_ctx *ctx, int rd_id, const double *lat_deg, int nlat, const 
     if (nlat < 2 || nlon < 2) return fail(ctx, GA_E_ARG, "topography needs >= 2 x 2 nodes");
     if (!increasing(lat_deg, nlat) || !increasing(lon_deg, nlon)) return fail(ctx, GA_E_ARG, "topography nodes must increase");
     std::vector<double> x(nlat), y(nlon);
-    for (int i = 0; i < nlat; ++i) x[i] = lat_deg[i] * (PI / 180.0);
-    for (int j = 0; j < nlon; ++j) y[j] = lon_deg[j] * (PI / 180.0);
+    const double deg = it->second.cart ? 1.0 : PI / 180.0;
+    for (int i = 0; i < nlat; ++i) x[i] = lat_deg[i] * deg;
+    for (int j = 0; j < nlon; ++j) y[j] = lon_deg[j] * deg;
     // per node: z, y slope of its row, x slope of its column of z, x slope of
     // its column of y slopes
     std::vector<double> fy((size_t)nlat * nlon), tab((size_t)nlat * nlon * TP_Q), row(nlon), sl, colv(nlat), csl;
@@ -2386,8 +2429,8 @@ static bool file_2d_dims(const std::string &path, int &n1, int &n2)
     return true;
 }
 
-int ga_rngdep_load_files(ga_ctx *ctx, const char *prefix, const char *lat_path, const char *lon_path,
-                         const char *topo_path, const char *format, int reverse_winds, int *rd_id)
+static int rngdep_load_files(ga_ctx *ctx, const char *prefix, const char *lat_path, const char *lon_path,
+                             const char *topo_path, const char *format, int reverse_winds, int *rd_id, bool cart)
 {
     if (!ctx || !prefix || !lat_path || !lon_path || !rd_id) return fail(ctx, GA_E_ARG, "null argument");
     const std::string fmt = format ? format : "zTuvdp";
@@ -2435,8 +2478,8 @@ int ga_rngdep_load_files(ga_ctx *ctx, const char *prefix, const char *lat_path, 
             }
             if (k < nz) return fail(ctx, GA_E_IO, "%s has %d rows, expected %d", path.c_str(), k, nz);
         }
-    int rc = ga_rngdep_load(ctx, lat.data(), nlat, lon.data(), nlon, z.data(), nz, c.data(), u.data(), v.data(),
-                            rho.data(), rd_id);
+    int rc = rngdep_load(ctx, lat.data(), nlat, lon.data(), nlon, z.data(), nz, c.data(), u.data(), v.data(),
+                            rho.data(), rd_id, cart);
     if (rc != GA_OK || !topo_path || !topo_path[0]) return rc;
     int n1 = 0, n2 = 0;
     if (!file_2d_dims(topo_path, n1, n2) || n1 < 2 || n2 < 2) {
@@ -2456,6 +2499,20 @@ int ga_rngdep_load_files(ga_ctx *ctx, const char *prefix, const char *lat_path, 
     rc = ga_topo_load(ctx, *rd_id, tl.data(), n1, to.data(), n2, tz.data());
     if (rc != GA_OK) ga_rngdep_release(ctx, *rd_id);
     return rc;
+}
+
+int ga_rngdep_load_files(ga_ctx *ctx, const char *prefix, const char *lat_path, const char *lon_path,
+                         const char *topo_path, const char *format, int reverse_winds, int *rd_id)
+{
+    return rngdep_load_files(ctx, prefix, lat_path, lon_path, topo_path, format, reverse_winds, rd_id, false);
+}
+
+// The 3D Cartesian node files (set_region, atmo_io.3d.rngdep.cpp:56-225): the
+// same layout with x / y node coordinates and terrain rows in km.
+int ga_rngdep_load_files_xy(ga_ctx *ctx, const char *prefix, const char *x_path, const char *y_path,
+                            const char *topo_path, const char *format, int reverse_winds, int *rd_id)
+{
+    return rngdep_load_files(ctx, prefix, x_path, y_path, topo_path, format, reverse_winds, rd_id, true);
 }
 
 int ga_rngdep_eval(ga_ctx *ctx, int rd_id, const double *pts, int n, double *atmo_out, double *topo_out)

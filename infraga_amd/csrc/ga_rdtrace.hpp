@@ -280,7 +280,9 @@ GA_HD int p6(int n, int m)   // packed index of the symmetric pair (n, m)
 
 // BLW: the boundary-layer vertical wind is on (a separate instantiation:
 // its terms keep the terrain derivatives live across the spline fields).
-template <bool SECOND, bool BLW>
+// CART: the 3D Cartesian medium, whose c comes from atmo::c / dc / ddc at
+// the clamped point (no f_zz patch).
+template <bool SECOND, bool BLW, bool CART = false>
 GA_HD void rd_medium_rhs(const RdMed &M, double r, double lat, double lon, RdAtm &A)
 {
     constexpr int MODE = SECOND ? 2 : 1;
@@ -322,6 +324,7 @@ GA_HD void rd_medium_rhs(const RdMed &M, double r, double lat, double lon, RdAtm
     // patch does not); u and v at the clamped point: the same cell with the
     // f_zz patch at the clamped height.
     RdCell cl = rd_cell(M.g, lat, lon, r - M.r0);
+    if (CART) cl.Xz_u = cl.Xz;
     double f, df[3], ddf[6];
     rd_field_eval<MODE>(M.g, cl, F_C, f, df, ddf);
     A.c = f;
@@ -815,6 +818,327 @@ GA_HD ga_arrival rd_arrival_row(const RdFanArgs &A, const double *y, int ray, in
     return R;
 }
 
+// --------------------------------------------------------------------------
+// 3D Cartesian range-dependent geometry (infraga-3d-rngdep:
+// geoac.eqset.3d.rngdep.cpp over atmo_state.3d.rngdep.cpp).  The medium is
+// the same hybrid-spline table with node coordinates in km and r0 = 0, so
+// the point functions above take (z, x, y) in the places of (r, lat, lon)
+// (medium derivative index 0 z, 1 x, 2 y).  The state row is the
+// reference's (x, y, z, nu, X_th, mu_th, X_ph, mu_ph); the source is kept
+// as src0 = z, src1 = x, src2 = y.
+// --------------------------------------------------------------------------
+GA_HD int c3(int n) { return n == 2 ? 0 : n + 1; }   // Cartesian index -> medium index
+
+// update_refs + eval_src_eq (geoac.eqset.3d.rngdep.cpp:226-382): c and its
+// derivatives at the clamped point (atmo::c, dc, ddc), winds of calc_uvw,
+// the full symmetric second-derivative contraction and the refs.dv[3]
+// accumulation of X_ph (:288), which leaves refs.dv[4] at zero.
+template <int NEQ, class V, class SV>
+GA_HD void rd3_stage(const RdMed &M, const Stage &S, const V &y, const SV &stg, const V &acc)
+{
+    const double nu0 = stg[3], nu1 = stg[4], nu2 = stg[5];
+    RdAtm A;
+    rd_medium_rhs<NEQ == 18, false, true>(M, stg[2], stg[0], stg[1], A);
+    value_fence(A);
+    double dc[3], du[3], dv[3];
+#pragma unroll
+    for (int n = 0; n < 3; ++n) {
+        dc[n] = A.dc[c3(n)];
+        du[n] = A.du[c3(n)];
+        dv[n] = A.dv[c3(n)];
+    }
+    const double nu_mag = sqrt(nu0 * nu0 + nu1 * nu1 + nu2 * nu2);
+    const double inu = 1.0 / nu_mag;
+    const double cg0 = A.c * nu0 * inu + A.u;
+    const double cg1 = A.c * nu1 * inu + A.v;
+    const double cg2 = A.c * nu2 * inu;
+    const double cg_mag = sqrt(cg0 * cg0 + cg1 * cg1 + cg2 * cg2);
+    const double icg = 1.0 / cg_mag;
+    double Sg[3];
+#pragma unroll
+    for (int n = 0; n < 3; ++n) Sg[n] = nu_mag * dc[n] + nu0 * du[n] + nu1 * dv[n];
+    const double cgv[3] = {cg0, cg1, cg2};
+
+    if (NEQ == 18) {
+        const double icg2 = icg * icg;
+        const double inu2 = inu * inu;
+        const double Xt[3] = {stg[6], stg[7], stg[8]}, Xp[3] = {stg[12], stg[13], stg[14]};
+        double dv3 = 0.0;
+#pragma unroll
+        for (int n = 0; n < 3; ++n) dv3 += Xt[n] * dv[n] + Xp[n] * dv[n];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int oX = j ? 12 : 6, om = j ? 15 : 9;
+            const double Xj[3] = {stg[oX], stg[oX + 1], stg[oX + 2]};
+            const double mj[3] = {stg[om], stg[om + 1], stg[om + 2]};
+            double dcj = 0.0, duj = 0.0;
+            double ddc[3], ddu[3], ddv[3];
+#pragma unroll
+            for (int n = 0; n < 3; ++n) {
+                dcj += Xj[n] * dc[n];
+                duj += Xj[n] * du[n];
+                ddc[n] = ddu[n] = ddv[n] = 0.0;
+#pragma unroll
+                for (int m = 0; m < 3; ++m) {
+                    const int q = p6(c3(n), c3(m));
+                    ddc[n] += Xj[m] * A.ddc[q];
+                    ddu[n] += Xj[m] * A.ddu[q];
+                    ddv[n] += Xj[m] * A.ddv[q];
+                }
+            }
+            const double dvj = j ? 0.0 : dv3;
+            const double dnu = (nu0 * mj[0] + nu1 * mj[1] + nu2 * mj[2]) * inu;
+            const double cnd = A.c * inu2 * dnu;
+            const double dcg[3] = {nu0 * inu * dcj + A.c * mj[0] * inu - cnd * nu0 + duj,
+                                   nu1 * inu * dcj + A.c * mj[1] * inu - cnd * nu1 + dvj,
+                                   nu2 * inu * dcj + A.c * mj[2] * inu - cnd * nu2};
+            const double dcgm = (cg0 * dcg[0] + cg1 * dcg[1] + cg2 * dcg[2]) * icg;
+            const double cgm2 = dcgm * icg2;
+#pragma unroll
+            for (int n = 0; n < 3; ++n) {
+                const double fX = dcg[n] * icg - cgv[n] * cgm2;
+                const double fm = cgm2 * Sg[n] - icg * (dnu * dc[n] + nu_mag * ddc[n] + mj[0] * du[n] + mj[1] * dv[n] +
+                                                        nu0 * ddu[n] + nu1 * ddv[n]);
+                rd_upd(S, y, stg, acc, oX + n, fX);
+                rd_upd(S, y, stg, acc, om + n, fm);
+            }
+        }
+    }
+#pragma unroll
+    for (int n = 0; n < 3; ++n) {
+        rd_upd(S, y, stg, acc, n, cgv[n] * icg);
+        rd_upd(S, y, stg, acc, 3 + n, -icg * Sg[n]);
+    }
+}
+
+template <int NEQ, class V, class SV>
+GA_HD double rd3_rk4_step(const RdMed &M, const Limits &L, const V &y, const V &y1, SV &stg)
+{
+    const double ds = rd_set_ds(M, L, y[2], y[0], y[1]);
+    const double h6 = ds * (1.0 / 6.0), h3 = ds * (1.0 / 3.0), h2 = ds * 0.5;
+#pragma unroll
+    for (int i = 0; i < NEQ; ++i) { stg[i] = y[i]; y1[i] = y[i]; }
+#pragma clang loop unroll(disable)
+    for (int st = 0; st < 4; ++st) {
+        const double ha = (st == 0 || st == 3) ? h6 : h3;
+        const double hn = (st == 2) ? ds : h2;
+        rd3_stage<NEQ>(M, Stage{ha, hn, false}, y, stg, y1);
+    }
+    return ds;
+}
+
+// break_check (:434-442): horizontal range from the source, the x / y
+// region (L.b0..b3) and the altitude ceiling.
+GA_HD bool rd3_leaves_region(const RdFanArgs &A, const Limits &L, const double *y)
+{
+    const double dx = y[0] - A.src1, dy = y[1] - A.src2;
+    if (sqrt(dx * dx + dy * dy) > L.rng_max) return true;
+    if (y[0] > L.b1 || y[0] < L.b0) return true;
+    if (y[1] > L.b3 || y[1] < L.b2) return true;
+    return y[2] > L.alt_max;
+}
+
+GA_HD bool rd3_below_ground(const RdMed &M, const double *y)
+{
+    return (y[2] <= M.z_max) && (y[2] < rd_ground(M, y[0], y[1]));
+}
+
+// travel_time / atten of one segment at its midpoint (:460-485, Cartesian
+// Sutherland-Bass absorption).
+GA_HD void rd3_seg_book(const RdMed &M, const double *y0, const double *y1, double freq, double &dt, double &datt)
+{
+    const double dx = y1[0] - y0[0], dy = y1[1] - y0[1], dz = y1[2] - y0[2];
+    const double x = y0[0] + dx * 0.5, yy = y0[1] + dy * 0.5, z = y0[2] + dz * 0.5;
+    const double ds = sqrt(dx * dx + dy * dy + dz * dz);
+    const double n0 = y0[3] + (y1[3] - y0[3]) * 0.5;
+    const double n1 = y0[4] + (y1[4] - y0[4]) * 0.5;
+    const double n2 = y0[5] + (y1[5] - y0[5]) * 0.5;
+    const double inm = 1.0 / sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+    RdPt P;
+    rd_point(M, z, x, yy, P);
+    value_fence(P);
+    const double c0 = P.c * n0 * inm + P.u, c1 = P.c * n1 * inm + P.v, c2 = P.c * n2 * inm + P.w;
+    dt = ds / sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+    datt = sb_alpha<false>(z, P.c, P.rho, freq, M.tweak, M.sbc) * ds;
+}
+
+// set_initial (:58-105); th, ph in radians.
+template <int NEQ>
+GA_HD void rd3_init_ray(const RdFanArgs &A, double th, double ph, double *y, double &c0, double &nu0s)
+{
+    RdPt P;
+    rd_point(A.M, A.src0, A.src1, A.src2, P);
+    c0 = P.c;
+    double st, ct, sp, cp;
+    sincos(th, &st, &ct);
+    sincos(ph, &sp, &cp);
+    const double mach[3] = {P.u / c0, P.v / c0, P.w / c0};
+    const double n0[3] = {ct * cp, ct * sp, st};
+    const double mt[3] = {-st * cp, -st * sp, ct};
+    const double mp[3] = {-ct * sp, ct * cp, 0.0};
+    const double msc = 1.0 + (n0[0] * mach[0] + n0[1] * mach[1] + n0[2] * mach[2]);
+    const double imsc = 1.0 / msc;
+    nu0s = imsc;
+    y[0] = A.src1; y[1] = A.src2; y[2] = A.src0;
+    for (int i = 0; i < 3; ++i) y[3 + i] = n0[i] * imsc;
+    if (NEQ == 18) {
+        const double a = mt[0] * mach[0] + mt[1] * mach[1] + mt[2] * mach[2];
+        const double b = mp[0] * mach[0] + mp[1] * mach[1] + mp[2] * mach[2];
+        for (int i = 0; i < 3; ++i) {
+            y[6 + i] = 0.0;
+            y[12 + i] = 0.0;
+            y[9 + i] = mt[i] * imsc - n0[i] * (imsc * imsc) * a;
+            y[15 + i] = mp[i] * imsc - n0[i] * (imsc * imsc) * b;
+        }
+    }
+}
+
+// approx_intercept + set_refl (:108-213): rows K-2 and K-1 of the leg in,
+// row 0 of the next leg out; terrain slopes a = dz/dx, b = dz/dy.
+template <int NEQ>
+GA_HD void rd3_reflect(const RdMed &M, const double *yk2, const double *yk1, double *y, double c0)
+{
+    double prev[NEQ];
+    {
+        Terr T1;
+        rd_terrain(M, yk1[0], yk1[1], T1);
+        const double dz_grnd = T1.zg - yk1[2];
+        const double dz_k = (yk1[2] - yk2[2]) - (yk1[0] - yk2[0]) * T1.d[0] - (yk1[1] - yk2[1]) * T1.d[1];
+#pragma unroll
+        for (int i = 0; i < NEQ; ++i) prev[i] = yk1[i] + (yk1[i] - yk2[i]) / dz_k * dz_grnd;
+    }
+    Terr T;
+    rd_terrain(M, prev[0], prev[1], T);
+    const double zg = T.zg, a = T.d[0], b = T.d[1];
+    const double norm1 = 1.0 + a * a + b * b, in1 = 1.0 / norm1;
+    const double C1x = (1.0 - a * a + b * b) * in1, C2x = 2.0 * a * in1;
+    const double C1y = (1.0 + a * a - b * b) * in1, C2y = 2.0 * b * in1;
+    const double C1z = (1.0 - a * a - b * b) * in1;
+    y[0] = prev[0];
+    y[1] = prev[1];
+    y[2] = zg;
+    y[3] = C1x * prev[3] + C2x * (prev[5] - prev[4] * b);
+    y[4] = C1y * prev[4] + C2y * (prev[5] - prev[3] * a);
+    y[5] = -C1z * prev[5] + prev[3] * C2x + prev[4] * C2y;
+    if (NEQ < 18) return;
+
+    RdPtD P;
+    rd_point_d(M, zg, prev[0], prev[1], P);
+    value_fence(P);
+    const double cg = P.c;
+    const double norm2 = prev[5] - a * prev[3] - b * prev[4];
+    double da[2], db[2], ds0[2];
+    for (int j = 0; j < 2; ++j) {
+        const int o = j ? 12 : 6;
+        const double k1 = (prev[o] * (prev[5] - b * prev[4]) - prev[3] * (prev[o + 2] - b * prev[o + 1])) / norm2;
+        const double k2 = (prev[o + 1] * (prev[5] - a * prev[3]) - prev[4] * (prev[o + 2] - a * prev[o])) / norm2;
+        da[j] = k1 * T.dd[0][0] + k2 * T.dd[0][1];
+        db[j] = k1 * T.dd[1][0] + k2 * T.dd[1][1];
+        ds0[j] = -c0 / cg * (prev[o + 2] - a * prev[o] - b * prev[o + 1]) / norm2;
+    }
+    const double in12 = in1 * in1;
+    double dC1x[2], dC1y[2], dC1z[2], dC2x[2], dC2y[2];
+    for (int j = 0; j < 2; ++j) {
+        dC1x[j] = -4.0 * a * in12 * ((1.0 + b * b) * da[j] - a * b * db[j]);
+        dC1y[j] = -4.0 * b * in12 * ((1.0 + a * a) * db[j] - a * b * da[j]);
+        dC1z[j] = -4.0 * in12 * (a * da[j] + b * db[j]);
+        dC2x[j] = 2.0 * in12 * ((1.0 - a * a + b * b) * da[j] - 2.0 * a * b * db[j]);
+        dC2y[j] = 2.0 * in12 * ((1.0 + a * a - b * b) * db[j] - 2.0 * a * b * da[j]);
+    }
+    // medium derivative index: 1 x, 2 y, 0 z
+    const double icg = -1.0 / cg, cr = c0 / cg;
+    const double dnx = icg * (cr * P.dc[1] + prev[3] * P.du[1] + prev[4] * P.dv[1] + prev[5] * P.dw[1]);
+    const double dny = icg * (cr * P.dc[2] + prev[3] * P.du[2] + prev[4] * P.dv[2] + prev[5] * P.dw[2]);
+    const double dnz = icg * (cr * P.dc[0] + prev[3] * P.du[0] + prev[4] * P.dv[0] + prev[5] * P.dw[0]);
+    for (int j = 0; j < 2; ++j) {
+        const int o = j ? 12 : 6;
+        y[o] = C1x * prev[o] + C2x * (prev[o + 2] - prev[o + 1] * b);
+        y[o + 1] = C1y * prev[o + 1] + C2y * (prev[o + 2] - prev[o] * a);
+        y[o + 2] = -C1z * prev[o + 2] + prev[o] * C2x + prev[o + 1] * C2y;
+        y[o + 3] = C1x * prev[o + 3] + C2x * (prev[o + 5] - b * prev[o + 4] - prev[4] * db[j]) + dC1x[j] * prev[3] +
+                   dC2x[j] * (prev[5] - b * prev[4]) + ((C1x - 1.0) * dnx + C2x * (dnz - b * dny)) * ds0[j];
+        y[o + 4] = C1y * prev[o + 4] + C2y * (prev[o + 5] - a * prev[o + 3] - prev[3] * da[j]) + dC1y[j] * prev[4] +
+                   dC2y[j] * (prev[5] - a * prev[3]) + ((C1y - 1.0) * dny + C2y * (dnz - a * dnx)) * ds0[j];
+        y[o + 5] = -C1z * prev[o + 5] + C2x * prev[o + 3] + C2y * prev[o + 4] - dC1z[j] * prev[5] + dC2x[j] * prev[3] +
+                   dC2y[j] * prev[4] + (-(C1z + 1.0) * dnz + C2x * dnx + C2y * dny) * ds0[j];
+    }
+}
+
+// jacobian (:704-721) at row y with medium values P.
+GA_HD double rd3_jacobian(const double *y, const RdPt &P)
+{
+    const double nm = sqrt(y[3] * y[3] + y[4] * y[4] + y[5] * y[5]);
+    const double p0 = P.c * y[3] / nm + P.u, p1 = P.c * y[4] / nm + P.v, p2 = P.c * y[5] / nm + P.w;
+    const double pm = sqrt(p0 * p0 + p1 * p1 + p2 * p2);
+    const double dxs = p0 / pm, dys = p1 / pm, dzs = p2 / pm;
+    return dxs * (y[7] * y[14] - y[13] * y[8]) - y[6] * (dys * y[14] - dzs * y[13]) + y[12] * (dys * y[8] - dzs * y[7]);
+}
+
+// 20 log10 of amp (:723-746); P at y, S at the source.
+GA_HD double rd3_amp_db(const double *y, const RdPt &P, const RdPt &S, double c0, double th, double ph)
+{
+    const double jac = rd3_jacobian(y, P);
+    double st, ct, sp, cp;
+    sincos(th, &st, &ct);
+    sincos(ph, &sp, &cp);
+    const double nm = sqrt(y[3] * y[3] + y[4] * y[4] + y[5] * y[5]);
+    const double nm0 = 1.0 - (y[3] * S.u + y[4] * S.v + y[5] * S.w) / c0;
+    const double p0 = P.c * y[3] / nm + P.u, p1 = P.c * y[4] / nm + P.v, p2 = P.c * y[5] / nm + P.w;
+    const double q0 = c0 * ct * cp + S.u, q1 = c0 * ct * sp + S.v, q2 = c0 * st + S.w;
+    const double pm = sqrt(p0 * p0 + p1 * p1 + p2 * p2), qm = sqrt(q0 * q0 + q1 * q1 + q2 * q2);
+    const double num = P.rho * nm * (P.c * P.c * P.c) * qm * ct;
+    const double den = S.rho * nm0 * (c0 * c0 * c0) * pm * jac;
+    return 20.0 * log10(sqrt(fabs(num / den)));
+}
+
+GA_HD double rd3_tl(const RdFanArgs &A, const double *y, const RdPt &P, double c0, double th, double ph)
+{
+    RdPt S;
+    rd_point(A.M, A.src0, A.src1, A.src2, S);
+    return rd3_amp_db(y, P, S, c0, th, ph);
+}
+
+// Arrival row (main.infraga.3d.rngdep.cpp:494-525): inclination from c on
+// the terrain under the arrival over c at the source, with the minus sign.
+template <int NEQ>
+GA_HD ga_arrival rd3_arrival_row(const RdFanArgs &A, const double *y, int ray, int bnc, double incl, double az,
+                                 double tt, double att, double rmax, double c0, double th, double ph)
+{
+    ga_arrival R;
+    R.launch_idx = ray;
+    R.n_bounce = bnc;
+    R.incl = incl;
+    R.az = az;
+    RdPt P, G, S;
+    rd_point(A.M, y[2], y[0], y[1], P);
+    rd_point(A.M, rd_ground(A.M, y[0], y[1]), y[0], y[1], G);
+    rd_point(A.M, A.src0, A.src1, A.src2, S);
+    R.incl_arr = -asin(G.c / S.c * y[5]) * 180.0 / PI;
+    double baz = 90.0 - atan2(-y[4], -y[3]) * 180.0 / PI;
+    if (baz < -180.0) baz += 360.0;
+    else if (baz > 180.0) baz -= 360.0;
+    R.back_az = baz;
+    R.pos0 = y[0];
+    R.pos1 = y[1];
+    const double dx = y[0] - A.src1, dy = y[1] - A.src2;
+    R.cel = sqrt(dx * dx + dy * dy) / tt;
+    R.time = tt;
+    R.turn_ht = rmax;
+    R.absorption = -2.0 * att;
+    R.tl = (NEQ == 18) ? rd3_amp_db(y, P, S, c0, th, ph) : 0.0;
+    return R;
+}
+
+// Raypath columns of a 3D row (main.infraga.3d.rngdep.cpp:454-456): x, y
+// and max(z, terrain).
+GA_HD void rd3_path_xyz(const RdMed &M, const double *y, ga_path_pt &R)
+{
+    const double zg = rd_ground(M, y[0], y[1]);
+    R.pos0 = y[0];
+    R.pos1 = y[1];
+    R.z = y[2] > zg ? y[2] : zg;
+}
+
 // Per-lane trace loop (the range-dependent run_prop with write_rays=false).
 // BLW: instantiation for M.use_blw on a terrain (rd_medium_rhs).  ya, yb:
 // the lane's two state rows (current row and the RK4 accumulator, which
@@ -823,7 +1147,7 @@ GA_HD ga_arrival rd_arrival_row(const RdFanArgs &A, const double *y, int ray, in
 // (main.infraga.sph.rngdep.cpp:451-490): segments of rows 1..k-1 added one
 // by one, rows m == 1 or m % 15 == 0 and Jacobian sign changes written to
 // A.path.
-template <int NEQ, bool BLW, bool PATHS = false, class V, class NextRay>
+template <int NEQ, bool BLW, bool PATHS = false, bool CART = false, class V, class NextRay>
 GA_HD void rd_lane_loop(const RdFanArgs &A, int lane, NextRay next_ray, V ya, V yb, V ys)
 {
     const Limits L = A.L;
@@ -845,14 +1169,16 @@ GA_HD void rd_lane_loop(const RdFanArgs &A, int lane, NextRay next_ray, V ya, V 
             ph = PI / 2.0 - A.launches[2 * ray + 1] * (PI / 180.0);
             th = A.launches[2 * ray] * (PI / 180.0);
             double y0[NEQ];
-            rd_init_ray<NEQ>(A, th, ph, y0, c0, nu0s);
+            if (CART) rd3_init_ray<NEQ>(A, th, ph, y0, c0, nu0s);
+            else rd_init_ray<NEQ>(A, th, ph, y0, c0, nu0s);
 #pragma unroll
             for (int i = 0; i < NEQ; ++i) y[i] = y0[i];
             s = 0.0; tt = 0.0; att = 0.0; ttl = 0.0; attl = 0.0; rmax = 0.0;
             k = 0; bnc = 0; nst = 0; narr = 0;
             fetch = false;
         }
-        s += rd_rk4_step<NEQ, BLW>(M, L, y, y1, ys);
+        if (CART) s += rd3_rk4_step<NEQ>(M, L, y, y1, ys);
+        else s += rd_rk4_step<NEQ, BLW>(M, L, y, y1, ys);
         ++k;
         ++nst;
         double n[NEQ];   // the new row
@@ -860,8 +1186,8 @@ GA_HD void rd_lane_loop(const RdFanArgs &A, int lane, NextRay next_ray, V ya, V 
         for (int i = 0; i < NEQ; ++i) n[i] = y1[i];
         bool brk = false, end = false;
         if (s > L.s_max) { brk = true; end = true; }
-        else if (rd_leaves_region(A, L, n)) { brk = true; end = true; }
-        else if (rd_below_ground(M, n)) { end = true; }
+        else if (CART ? rd3_leaves_region(A, L, n) : rd_leaves_region(A, L, n)) { brk = true; end = true; }
+        else if (CART ? rd3_below_ground(M, n) : rd_below_ground(M, n)) { end = true; }
         else if (k >= A.max_steps) { brk = true; end = true; }
         else if (!(n[0] == n[0])) { brk = true; end = true; }   // non-finite state: stop the ray
         y1[2] = n[2];   // longitude wrapped by rd_leaves_region
@@ -872,35 +1198,39 @@ GA_HD void rd_lane_loop(const RdFanArgs &A, int lane, NextRay next_ray, V ya, V 
         if (PATHS) {
             if (!end) {
                 double dt, da;
-                rd_seg_book(M, o, n, A.freq, dt, da);
+                if (CART) rd3_seg_book(M, o, n, A.freq, dt, da);
+                else rd_seg_book(M, o, n, A.freq, dt, da);
                 tt += dt;
                 att += da;
                 const bool caus = NEQ == 18 && (A.path.flags & GA_PATH_CAUSTICS);
                 const bool samp = (A.path.flags & GA_PATH_RAYS) && (k == 1 || k % 15 == 0);
                 if (caus || samp) {
                     RdPt P;
-                    rd_point(M, n[0], n[1], n[2], P);
+                    if (CART) rd_point(M, n[2], n[0], n[1], P);
+                    else rd_point(M, n[0], n[1], n[2], P);
                     if (samp) {
                         double tl = 0.0;
                         if (NEQ == 18) {
                             RdPt S;
                             rd_point(M, A.src0, A.src1, A.src2, S);
-                            tl = rd_amp_db(n, P, S, c0, nu0s, th, ph);
+                            tl = CART ? rd3_amp_db(n, P, S, c0, th, ph) : rd_amp_db(n, P, S, c0, nu0s, th, ph);
                         }
                         ga_path_pt R;
                         R.launch_idx = ray; R.leg = bnc; R.step = k; R.kind = GA_PT_RAY;
                         R.pos0 = n[1] * (180.0 / PI); R.pos1 = n[2] * (180.0 / PI); R.z = n[0] - M.r0;
                         R.tl = tl; R.absorption = -2.0 * att; R.time = tt;
+                        if (CART) rd3_path_xyz(M, n, R);
                         path_put(A.path, pl, ray, R);
                     }
                     if (caus) {
-                        const double d = rd_jacobian(n, P);
+                        const double d = CART ? rd3_jacobian(n, P) : rd_jacobian(n, P);
                         if (k == 1) d_prev = d;
                         if (d * d_prev < 0.0) {
                             ga_path_pt R;
                             R.launch_idx = ray; R.leg = bnc; R.step = k; R.kind = GA_PT_CAUSTIC;
                             R.pos0 = n[1] * (180.0 / PI); R.pos1 = n[2] * (180.0 / PI); R.z = n[0] - M.r0;
                             R.tl = 0.0; R.absorption = -2.0 * att; R.time = tt;
+                            if (CART) rd3_path_xyz(M, n, R);
                             path_put(A.path, pl, ray, R);
                         }
                         d_prev = d;
@@ -909,14 +1239,16 @@ GA_HD void rd_lane_loop(const RdFanArgs &A, int lane, NextRay next_ray, V ya, V 
             }
         } else {
             double dt, da;
-            rd_seg_book(M, o, n, A.freq, dt, da);
+            if (CART) rd3_seg_book(M, o, n, A.freq, dt, da);
+            else rd_seg_book(M, o, n, A.freq, dt, da);
             ttl += dt;
             attl += da;
         }
-        rmax = fmax(rmax, o[0] - M.r0);
+        const double ho = CART ? o[2] : o[0] - M.r0;
+        rmax = fmax(rmax, ho);
 
         if (!end) {
-            if (o[0] - M.r0 < L.z_keep) {
+            if (ho < L.z_keep) {
 #pragma unroll
                 for (int i = 0; i < NEQ; ++i) ypl[i] = y[i];
             }
@@ -948,7 +1280,8 @@ GA_HD void rd_lane_loop(const RdFanArgs &A, int lane, NextRay next_ray, V ya, V 
                     yp[i] = ypl[i];
                     yk[i] = y[i];
                 }
-                rd_reflect<NEQ>(M, yp, yk, yr, c0);
+                if (CART) rd3_reflect<NEQ>(M, yp, yk, yr, c0);
+                else rd_reflect<NEQ>(M, yp, yk, yr, c0);
 #pragma unroll
                 for (int i = 0; i < NEQ; ++i) y[i] = yr[i];
                 ++bnc;

@@ -15,7 +15,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libinfraga_gpu.so")
 
-GA_GEOM_3D, GA_GEOM_SPH, GA_GEOM_SPH_RNGDEP, GA_GEOM_2D = 0, 1, 2, 3
+GA_GEOM_3D, GA_GEOM_SPH, GA_GEOM_SPH_RNGDEP, GA_GEOM_2D, GA_GEOM_3D_RNGDEP = 0, 1, 2, 3, 4
 GA_OK, GA_E_ARG, GA_E_HIP, GA_E_NOMEM, GA_E_CAP, GA_E_NODEV, GA_E_ATMO, GA_E_IO = 0, -1, -2, -3, -4, -5, -6, -7
 
 GA_REQ_FAN, GA_REQ_EIG = 0, 1
@@ -25,7 +25,8 @@ GA_PENDING = 1
 EXPORTED = ["ga_prop_params_default", "ga_eig_params_default", "ga_ctx_create", "ga_ctx_destroy",
             "ga_last_error", "ga_atmo_load_strat", "ga_atmo_load_file", "ga_atmo_release", "ga_fan_run",
             "ga_fan_run_device", "ga_eig_search", "ga_server_create", "ga_server_submit", "ga_server_poll",
-            "ga_server_stats", "ga_server_destroy", "ga_rngdep_load", "ga_topo_load", "ga_rngdep_eval",
+            "ga_server_stats", "ga_server_destroy", "ga_rngdep_load", "ga_rngdep_load_xy", "ga_rngdep_load_files_xy",
+            "ga_topo_load", "ga_rngdep_eval",
             "ga_rngdep_release", "ga_rngdep_load_files", "ga_rngdep_info", "ga_fan_run_paths",
             "ga_eig_search_paths", "ga_eig_direct", "ga_topo_load_2d", "ga_topo_eval_2d"]
 
@@ -273,6 +274,16 @@ class Engine:
         self._check(self.L.ga_rngdep_load(self.h, a[0].ctypes.data_as(_dp), len(a[0]), a[1].ctypes.data_as(_dp),
                                           len(a[1]), a[2].ctypes.data_as(_dp), len(a[2]),
                                           *[q.ctypes.data_as(_dp) for q in a[3:]], C.byref(rid)))
+        self._keep_rd = a
+        return rid.value
+
+    def rngdep_load_xy(self, x_km, y_km, z_km, c, u, v, rho) -> int:
+        """3D Cartesian range-dependent atmosphere (GA_GEOM_3D_RNGDEP): node x/y, z [km], fields [nx][ny][nz]."""
+        a = [np.ascontiguousarray(q, dtype=np.float64) for q in (x_km, y_km, z_km, c, u, v, rho)]
+        rid = C.c_int(0)
+        self._check(self.L.ga_rngdep_load_xy(self.h, a[0].ctypes.data_as(_dp), len(a[0]), a[1].ctypes.data_as(_dp),
+                                             len(a[1]), a[2].ctypes.data_as(_dp), len(a[2]),
+                                             *[q.ctypes.data_as(_dp) for q in a[3:]], C.byref(rid)))
         self._keep_rd = a
         return rid.value
 

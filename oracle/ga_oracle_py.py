@@ -440,6 +440,13 @@ def _rd_bind(L):
     L.go_rd_fan_paths.argtypes = [C.c_void_p, C.POINTER(Params), C.POINTER(Prop), _dp, C.c_int, C.c_int,
                                   C.POINTER(Arrival), C.c_int, C.POINTER(C.c_int), C.POINTER(PathPt), C.c_longlong,
                                   C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
+    L.go_rd_new_xy.restype = C.c_void_p
+    L.go_rd_new_xy.argtypes = [_dp, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, _dp, _dp, _dp]
+    L.go_rd3_fan.restype = C.c_int
+    L.go_rd3_fan.argtypes = [C.c_void_p, C.POINTER(Params), C.POINTER(Prop), _dp, C.c_int, C.c_int, C.c_int,
+                             C.POINTER(Arrival), C.c_int, C.POINTER(C.c_int), C.POINTER(PathPt), C.c_longlong,
+                             C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
+    L.go_rd3_rhs.argtypes = [C.c_void_p, C.POINTER(Params), _dp, _dp]
     L._rd_bound = True
 
 
@@ -741,3 +748,46 @@ class RefRd:
 
     def sb_alpha(self, r, t, p, freq):
         return self.lib.ref_rd_sb_alpha(r, t, p, freq)
+
+
+class Rd3Medium(RdMedium):
+    """One 3D Cartesian range-dependent atmosphere of the oracle
+    (infraga-3d-rngdep): node coordinates x, y in km, fields [nx, ny, nz];
+    a topography set on it takes x, y in km."""
+
+    def __init__(self, oracle: Oracle, x_km, y_km, z, c, u, v, rho):
+        self.o = oracle
+        L = oracle.lib
+        _rd_bind(L)
+        self._keep = [np.ascontiguousarray(a, dtype=np.float64) for a in (x_km, y_km, z, c, u, v, rho)]
+        xx, yy, zz, cc, uu, vv, rr = self._keep
+        if any(f.size != xx.size * yy.size * zz.size for f in (cc, uu, vv, rr)):
+            raise ValueError("field shape must be [nx, ny, nz]")
+        self.h = L.go_rd_new_xy(_ptr(xx), xx.size, _ptr(yy), yy.size, _ptr(zz), zz.size, _ptr(cc), _ptr(uu), _ptr(vv),
+                                _ptr(rr))
+        if not self.h:
+            raise ValueError("oracle rejected the range-dependent atmosphere")
+
+    def rhs(self, params, y):
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        f = np.zeros(18)
+        self.o.lib.go_rd3_rhs(self.h, C.byref(params), _ptr(y), _ptr(f))
+        return f[:params.eq_cnt]
+
+    def fan3(self, launches, params, bounces=2, write=False, flags=0, freq=0.1, turn_ht_min=0.2,
+             src=(float("nan"), float("nan"), 0.0)):
+        """run_prop (go_rd3_fan): (arrivals, path rows, ray-steps)."""
+        Lr = np.ascontiguousarray(launches, dtype=np.float64).reshape(-1, 2)
+        pr = Prop(bounces, freq, turn_ht_min, src[0], src[1], src[2])
+        cap = max(1, Lr.shape[0] * (bounces + 1))
+        out = (Arrival * cap)()
+        n_out, steps, n_pts = C.c_int(0), C.c_longlong(0), C.c_longlong(0)
+        pcap = 1
+        while True:
+            pts = (PathPt * pcap)()
+            self.o.lib.go_rd3_fan(self.h, C.byref(params), C.byref(pr), _ptr(Lr), Lr.shape[0], int(write), int(flags),
+                                  out, cap, C.byref(n_out), pts, pcap, C.byref(n_pts), C.byref(steps))
+            if n_pts.value <= pcap:
+                break
+            pcap = n_pts.value
+        return arrivals_to_array(out, n_out.value), paths_to_array(pts, n_pts.value), steps.value

@@ -31,7 +31,7 @@ static double dmin(double a, double b) { return b < a ? b : a; }   /* std::min *
  * ==================================================================== */
 static void set_limits(go_rd *a)
 {
-    double buffer = 1.0e-3 * (RD_PI / 180.0);
+    double buffer = a->cart ? 1.0e-3 : 1.0e-3 * (RD_PI / 180.0);   /* atmo_io.3d.rngdep.cpp:25 */
     const go_h3d *c = &a->c;
     a->lat_min = c->x[0] + buffer;  a->lat_max = c->x[c->nx - 1] - buffer;
     a->lon_min = c->y[0] + buffer;  a->lon_max = c->y[c->ny - 1] - buffer;
@@ -91,8 +91,8 @@ int go_rd_set_topo(go_rd *a, const double *lat_deg, int nlat, const double *lon_
 {
     double *x = (double *)malloc(sizeof(double) * (size_t)nlat);
     double *y = (double *)malloc(sizeof(double) * (size_t)nlon);
-    for (int i = 0; i < nlat; i++) { x[i] = lat_deg[i]; x[i] *= RD_PI / 180.0; }   /* :231 */
-    for (int j = 0; j < nlon; j++) { y[j] = lon_deg[j]; y[j] *= RD_PI / 180.0; }   /* :232 */
+    for (int i = 0; i < nlat; i++) { x[i] = lat_deg[i]; if (!a->cart) x[i] *= RD_PI / 180.0; }   /* :231 */
+    for (int j = 0; j < nlon; j++) { y[j] = lon_deg[j]; if (!a->cart) y[j] *= RD_PI / 180.0; }   /* :232 */
     if (a->is_topo) go_s2d_free(&a->topo);
     int rc = go_s2d_init(&a->topo, x, nlat, y, nlon, zg);
     free(x);
@@ -1245,4 +1245,417 @@ void go_rd_h_limits(void *h, double out[8])
     go_rd *a = (go_rd *)h;
     out[0] = a->lat_min; out[1] = a->lat_max; out[2] = a->lon_min; out[3] = a->lon_max;
     out[4] = a->alt_max; out[5] = a->z0; out[6] = a->z_max; out[7] = a->z_bndlyr;
+}
+
+/* ======================================================================
+ * 3D Cartesian range-dependent geometry (infraga-3d-rngdep):
+ * geoac.eqset.3d.rngdep.cpp and run_prop of main.infraga.3d.rngdep.cpp over
+ * the medium above with r0 = 0, node coordinates in km, and (z, x, y) in
+ * the places of (r, lat, lon) (atmo_state.3d.rngdep.cpp has the same
+ * functions with those arguments; its boundary-layer vertical wind, whose
+ * geometric factors differ, is not restated: use_blw stays 0).  State rows
+ * are the reference's (x, y, z, nu, X_th, mu_th, X_ph, mu_ph).
+ * ==================================================================== */
+static int mi3(int n) { return n == 2 ? 0 : n + 1; }   /* Cartesian -> medium derivative index */
+
+int go_rd_init_xy(go_rd *a, const double *x_km, int nx, const double *y_km, int ny, const double *z, int nz,
+                  const double *c, const double *u, const double *v, const double *rho)
+{
+    memset(a, 0, sizeof(*a));
+    if (nx < 2 || ny < 2 || nz < 3) return -1;
+    int rc = go_h3d_init(&a->c, x_km, nx, y_km, ny, z, nz, c);   /* atmo_io.3d.rngdep.cpp:93-160 */
+    rc |= go_h3d_init(&a->u, x_km, nx, y_km, ny, z, nz, u);
+    rc |= go_h3d_init(&a->v, x_km, nx, y_km, ny, z, nz, v);
+    rc |= go_h3d_init(&a->rho, x_km, nx, y_km, ny, z, nz, rho);
+    if (rc) return -1;
+    a->cart = 1;
+    a->r0 = 0.0;
+    a->bl_param = 0.1;             /* atmo_state.3d.rngdep.cpp:81 */
+    a->vert_wind_grad = 1.0;       /* :82 */
+    a->tweak_abs = 1.0;
+    set_limits(a);
+    set_bndlyr(a);
+    return 0;
+}
+
+typedef struct {
+    double c, dc[5], u, du[5], v, dv[5], w, dw[5];
+    double ddc[3][2], ddu[3][2], ddv[3][2], ddw[3][2];
+    double nu_mag, dnu_mag[2], cg[3], cg_mag, dcg[3][2], dcg_mag[2];
+} r3_refs;
+
+/* update_refs, geoac.eqset.3d.rngdep.cpp:226-323 */
+static void refs3(go_rd *a, const go_params *p, const double *y, r3_refs *R)
+{
+    const double *nu = y + 3;
+    double xe, ye, ze, du_t[3], dv_t[3], dw_t[3], ddu_t[6], ddv_t[6], ddw_t[6];
+    double dd_c[3][3], dd_u[3][3], dd_v[3][3], dd_w[3][3];
+    memset(R, 0, sizeof(*R));
+    R->nu_mag = sqrt(sq(nu[0]) + sq(nu[1]) + sq(nu[2]));
+    ev(a, &a->c, y[2], y[0], y[1], &xe, &ye, &ze);
+    R->c = go_h3d_f(&a->c, xe, ye, ze);                                  /* atmo::c, :148-154 */
+    for (int n = 0; n < 3; n++) R->dc[n] = go_h3d_df(&a->c, xe, ye, ze, n);
+    go_rd_calc_uvw(a, y[2], y[0], y[1], &R->u, &R->v, &R->w, du_t, dv_t, dw_t,
+                   p->calc_amp ? ddu_t : NULL, p->calc_amp ? ddv_t : NULL, p->calc_amp ? ddw_t : NULL);
+    for (int n = 0; n < 3; n++) { R->du[n] = du_t[mi3(n)]; R->dv[n] = dv_t[mi3(n)]; R->dw[n] = dw_t[mi3(n)]; }
+    for (int n = 0; n < 3; n++) R->cg[n] = R->c * nu[n] / R->nu_mag + (n == 0 ? R->u : (n == 1 ? R->v : R->w));
+    R->cg_mag = sqrt(sq(R->cg[0]) + sq(R->cg[1]) + sq(R->cg[2]));
+    if (!p->calc_amp) return;
+
+    /* packed medium second derivatives: zz, xx, yy, zx, zy, xy */
+    for (int n = 0; n < 3; n++)
+        for (int m = 0; m < 3; m++) {
+            int i = mi3(n), j = mi3(m), q = (i == j) ? i : (i + j == 1 ? 3 : (i + j == 2 ? 4 : 5));
+            dd_c[n][m] = go_h3d_ddf(&a->c, xe, ye, ze, n, m);
+            dd_u[n][m] = ddu_t[q]; dd_v[n][m] = ddv_t[q]; dd_w[n][m] = ddw_t[q];
+        }
+    const double *Xt = y + 6, *mt = y + 9, *Xp = y + 12, *mp = y + 15;
+    for (int n = 0; n < 3; n++) {
+        R->dc[3] += Xt[n] * R->dc[n]; R->du[3] += Xt[n] * R->du[n]; R->dv[3] += Xt[n] * R->dv[n]; R->dw[3] += Xt[n] * R->dw[n];
+        R->dc[4] += Xp[n] * R->dc[n]; R->du[4] += Xp[n] * R->du[n]; R->dv[3] += Xp[n] * R->dv[n]; R->dw[4] += Xp[n] * R->dw[n];   /* :288 */
+        for (int m = 0; m < 3; m++) {
+            R->ddc[n][0] += Xt[m] * dd_c[n][m]; R->ddc[n][1] += Xp[m] * dd_c[n][m];
+            R->ddu[n][0] += Xt[m] * dd_u[n][m]; R->ddu[n][1] += Xp[m] * dd_u[n][m];
+            R->ddv[n][0] += Xt[m] * dd_v[n][m]; R->ddv[n][1] += Xp[m] * dd_v[n][m];
+            R->ddw[n][0] += Xt[m] * dd_w[n][m]; R->ddw[n][1] += Xp[m] * dd_w[n][m];
+        }
+    }
+    R->dnu_mag[0] = (nu[0] * mt[0] + nu[1] * mt[1] + nu[2] * mt[2]) / R->nu_mag;
+    R->dnu_mag[1] = (nu[0] * mp[0] + nu[1] * mp[1] + nu[2] * mp[2]) / R->nu_mag;
+    for (int n = 0; n < 3; n++) {
+        R->dcg[n][0] = nu[n] / R->nu_mag * R->dc[3] + R->c * mt[n] / R->nu_mag - R->c * nu[n] / sq(R->nu_mag) * R->dnu_mag[0];
+        R->dcg[n][1] = nu[n] / R->nu_mag * R->dc[4] + R->c * mp[n] / R->nu_mag - R->c * nu[n] / sq(R->nu_mag) * R->dnu_mag[1];
+    }
+    R->dcg[0][0] += R->du[3]; R->dcg[0][1] += R->du[4];
+    R->dcg[1][0] += R->dv[3]; R->dcg[1][1] += R->dv[4];
+    R->dcg[2][0] += R->dw[3]; R->dcg[2][1] += R->dw[4];
+    for (int j = 0; j < 2; j++)
+        R->dcg_mag[j] = (R->cg[0] * R->dcg[0][j] + R->cg[1] * R->dcg[1][j] + R->cg[2] * R->dcg[2][j]) / R->cg_mag;
+}
+
+/* eval_src_eq, :329-382 */
+void go_rd3_rhs(go_rd *a, const go_params *p, const double *y, double *f)
+{
+    r3_refs R;
+    refs3(a, p, y, &R);
+    const double *nu = y + 3;
+    for (int n = 0; n < 3; n++) {
+        f[n] = R.cg[n] / R.cg_mag;
+        f[3 + n] = -1.0 / R.cg_mag * (R.nu_mag * R.dc[n] + nu[0] * R.du[n] + nu[1] * R.dv[n] + nu[2] * R.dw[n]);
+    }
+    if (p->eq_cnt < 18) return;
+    for (int j = 0; j < 2; j++) {
+        const double *mj = y + (j ? 15 : 9);
+        for (int n = 0; n < 3; n++) {
+            f[(j ? 12 : 6) + n] = R.dcg[n][j] / R.cg_mag - R.cg[n] / sq(R.cg_mag) * R.dcg_mag[j];
+            f[(j ? 15 : 9) + n] = 1.0 / sq(R.cg_mag) * R.dcg_mag[j] * (R.nu_mag * R.dc[n] + nu[0] * R.du[n] + nu[1] * R.dv[n] + nu[2] * R.dw[n])
+                                  - 1.0 / R.cg_mag * (R.dnu_mag[j] * R.dc[n] + R.nu_mag * R.ddc[n][j]
+                                                      + mj[0] * R.du[n] + mj[1] * R.dv[n] + mj[2] * R.dw[n]
+                                                      + nu[0] * R.ddu[n][j] + nu[1] * R.ddv[n][j] + nu[2] * R.ddw[n][j]);
+        }
+    }
+}
+
+/* set_initial, :58-105; s->src = (x, y, z) */
+void go_rd3_set_initial(go_rd *a, const go_params *p, go_src *s, double *y)
+{
+    double x0 = s->src[0], y0 = s->src[1], z0 = s->src[2], th = s->theta, ph = s->phi;
+    s->c0 = go_rd_c(a, z0, x0, y0);
+    double mach[3] = {go_rd_u(a, z0, x0, y0) / s->c0, go_rd_v(a, z0, x0, y0) / s->c0, go_rd_w(a, z0, x0, y0) / s->c0};
+    double nu0[3] = {cos(th) * cos(ph), cos(th) * sin(ph), sin(th)};
+    double mt[3] = {-sin(th) * cos(ph), -sin(th) * sin(ph), cos(th)};
+    double mp[3] = {-cos(th) * sin(ph), cos(th) * cos(ph), 0.0};
+    double msc = 1.0 + (nu0[0] * mach[0] + nu0[1] * mach[1] + nu0[2] * mach[2]);
+    y[0] = x0; y[1] = y0; y[2] = z0;
+    for (int n = 0; n < 3; n++) y[3 + n] = nu0[n] / msc;
+    if (p->eq_cnt < 18) return;
+    for (int n = 0; n < 3; n++) {
+        y[6 + n] = 0.0;
+        y[12 + n] = 0.0;
+        y[9 + n] = mt[n] / msc - nu0[n] / sq(msc) * (mt[0] * mach[0] + mt[1] * mach[1] + mt[2] * mach[2]);
+        y[15 + n] = mp[n] / msc - nu0[n] / sq(msc) * (mp[0] * mach[0] + mp[1] * mach[1] + mp[2] * mach[2]);
+    }
+}
+
+static double rd3_set_ds(go_rd *a, const go_params *p, const double *y)   /* :215-221 */
+{
+    double result = p->ds_max;
+    if (y[2] < a->z_bndlyr) result -= (p->ds_max - p->ds_min) * exp(-(y[2] - go_rd_topo_z(a, y[0], y[1])) / 0.2);
+    return result;
+}
+
+static int rd3_break(go_rd *a, const go_params *p, const go_src *s, const double *y)   /* :434-442 */
+{
+    if (sqrt(sq(y[0] - s->src[0]) + sq(y[1] - s->src[1])) > p->rng_max) return 1;
+    if (y[0] > p->lat_max || y[0] < p->lat_min) return 1;
+    if (y[1] > p->lon_max || y[1] < p->lon_min) return 1;
+    if (y[2] > p->alt_max) return 1;
+    return 0;
+}
+
+static int rd3_ground(go_rd *a, const double *y)   /* :444-451 */
+{
+    if (y[2] <= a->z_max)
+        if (y[2] < go_rd_topo_z(a, y[0], y[1])) return 1;
+    return 0;
+}
+
+static int rd3_prop_rk4(go_rd *a, const go_params *p, const go_src *s, double *rows, int length, int *check)
+{
+    int k, n = p->eq_cnt;
+    double sum_s = 0.0;
+    *check = 0;
+    for (k = 0; k < length - 1; k++) {
+        double *y = rows + (size_t)k * 18, *y1 = rows + (size_t)(k + 1) * 18;
+        double t1[18], t2[18], t3[18], t4[18], pa[18], f[18];
+        double ds = rd3_set_ds(a, p, y);
+        go_rd3_rhs(a, p, y, f);
+        for (int i = 0; i < n; i++) { t1[i] = ds * f[i]; pa[i] = y[i] + t1[i] / 2.0; }
+        go_rd3_rhs(a, p, pa, f);
+        for (int i = 0; i < n; i++) t2[i] = ds * f[i];
+        for (int i = 0; i < n; i++) pa[i] = y[i] + t2[i] / 2.0;
+        go_rd3_rhs(a, p, pa, f);
+        for (int i = 0; i < n; i++) t3[i] = ds * f[i];
+        for (int i = 0; i < n; i++) pa[i] = y[i] + t3[i];
+        go_rd3_rhs(a, p, pa, f);
+        for (int i = 0; i < n; i++) {
+            t4[i] = ds * f[i];
+            y1[i] = y[i] + t1[i] / 6.0 + t2[i] / 3.0 + t3[i] / 3.0 + t4[i] / 6.0;
+        }
+        sum_s += ds;
+        if (sum_s > p->s_max) { *check = 1; break; }
+        if (rd3_break(a, p, s, y1)) { *check = 1; break; }
+        if (rd3_ground(a, y1)) { *check = 0; break; }
+    }
+    return k + 1;
+}
+
+/* approx_intercept + set_refl, :108-213 */
+void go_rd3_set_refl(go_rd *a, const go_params *p, const go_src *s, const double *rk2, const double *rk1, double *y0)
+{
+    int neq = p->eq_cnt;
+    double prev[18];
+    double dz_grnd = go_rd_topo_z(a, rk1[0], rk1[1]) - rk1[2];
+    double dz_k = (rk1[2] - rk2[2]) - (rk1[0] - rk2[0]) * go_rd_topo_dz(a, rk1[0], rk1[1], 0)
+                  - (rk1[1] - rk2[1]) * go_rd_topo_dz(a, rk1[0], rk1[1], 1);
+    for (int n = 0; n < neq; n++) prev[n] = rk1[n] + (rk1[n] - rk2[n]) / dz_k * dz_grnd;
+
+    double zg = go_rd_topo_z(a, prev[0], prev[1]);
+    double c_grnd = go_rd_c(a, zg, prev[0], prev[1]);
+    double A = go_rd_topo_dz(a, prev[0], prev[1], 0), B = go_rd_topo_dz(a, prev[0], prev[1], 1);
+    double norm1 = 1.0 + sq(A) + sq(B);
+    double C1x = (1.0 - sq(A) + sq(B)) / norm1, C2x = 2.0 * A / norm1;
+    double C1y = (1.0 + sq(A) - sq(B)) / norm1, C2y = 2.0 * B / norm1;
+    double C1z = (1.0 - sq(A) - sq(B)) / norm1;
+    double da[2] = {0, 0}, db[2] = {0, 0}, ds0[2] = {0, 0}, dC1x[2] = {0, 0}, dC1y[2] = {0, 0}, dC1z[2] = {0, 0};
+    double dC2x[2] = {0, 0}, dC2y[2] = {0, 0}, dnx = 0, dny = 0, dnz = 0;
+    if (p->calc_amp) {
+        double norm2 = prev[5] - A * prev[3] - B * prev[4];
+        for (int j = 0; j < 2; j++) {
+            int o = j ? 12 : 6;
+            double k1 = (prev[o] * (prev[5] - B * prev[4]) - prev[3] * (prev[o + 2] - B * prev[o + 1])) / norm2;
+            double k2 = (prev[o + 1] * (prev[5] - A * prev[3]) - prev[4] * (prev[o + 2] - A * prev[o])) / norm2;
+            da[j] = k1 * go_rd_topo_ddz(a, prev[0], prev[1], 0, 0) + k2 * go_rd_topo_ddz(a, prev[0], prev[1], 0, 1);
+            db[j] = k1 * go_rd_topo_ddz(a, prev[0], prev[1], 1, 0) + k2 * go_rd_topo_ddz(a, prev[0], prev[1], 1, 1);
+            dC1x[j] = -4.0 * A / sq(norm1) * ((1.0 + sq(B)) * da[j] - A * B * db[j]);
+            dC1y[j] = -4.0 * B / sq(norm1) * ((1.0 + sq(A)) * db[j] - A * B * da[j]);
+            dC1z[j] = -4.0 / sq(norm1) * (A * da[j] + B * db[j]);
+            dC2x[j] = 2.0 / sq(norm1) * ((1.0 - sq(A) + sq(B)) * da[j] - 2.0 * A * B * db[j]);
+            dC2y[j] = 2.0 / sq(norm1) * ((1.0 + sq(A) - sq(B)) * db[j] - 2.0 * A * B * da[j]);
+            ds0[j] = -s->c0 / c_grnd * (prev[o + 2] - A * prev[o] - B * prev[o + 1]) / norm2;
+        }
+        /* atmo::dc(x, y, zg, n), n = 0 x, 1 y, 2 z */
+        double dn[3];
+        for (int n = 0; n < 3; n++)
+            dn[n] = -1.0 / c_grnd * (s->c0 / c_grnd * go_rd_dc(a, zg, prev[0], prev[1], mi3(n))
+                                     + prev[3] * go_rd_du(a, zg, prev[0], prev[1], mi3(n))
+                                     + prev[4] * go_rd_dv(a, zg, prev[0], prev[1], mi3(n))
+                                     + prev[5] * go_rd_dw(a, zg, prev[0], prev[1], mi3(n)));
+        dnx = dn[0]; dny = dn[1]; dnz = dn[2];
+    }
+    y0[0] = prev[0];
+    y0[1] = prev[1];
+    y0[2] = zg;
+    for (int j = -1; j < 2; j++) {
+        int o = j < 0 ? 3 : (j ? 12 : 6);
+        if (j >= 0 && neq < 18) break;
+        y0[o] = C1x * prev[o] + C2x * (prev[o + 2] - prev[o + 1] * B);
+        y0[o + 1] = C1y * prev[o + 1] + C2y * (prev[o + 2] - prev[o] * A);
+        y0[o + 2] = -C1z * prev[o + 2] + prev[o] * C2x + prev[o + 1] * C2y;
+    }
+    if (neq < 18) return;
+    for (int j = 0; j < 2; j++) {
+        int o = j ? 15 : 9;
+        y0[o] = C1x * prev[o] + C2x * (prev[o + 2] - B * prev[o + 1] - prev[4] * db[j]) + dC1x[j] * prev[3]
+                + dC2x[j] * (prev[5] - B * prev[4]) + ((C1x - 1.0) * dnx + C2x * (dnz - B * dny)) * ds0[j];
+        y0[o + 1] = C1y * prev[o + 1] + C2y * (prev[o + 2] - A * prev[o] - prev[3] * da[j]) + dC1y[j] * prev[4]
+                    + dC2y[j] * (prev[5] - A * prev[3]) + ((C1y - 1.0) * dny + C2y * (dnz - A * dnx)) * ds0[j];
+        y0[o + 2] = -C1z * prev[o + 2] + C2x * prev[o] + C2y * prev[o + 1] - dC1z[j] * prev[5] + dC2x[j] * prev[3]
+                    + dC2y[j] * prev[4] + (-(C1z + 1.0) * dnz + C2x * dnx + C2y * dny) * ds0[j];
+    }
+}
+
+/* travel_time / atten of one segment, :460-485 (Cartesian Sutherland-Bass) */
+static void rd3_seg(go_rd *a, const double *y0, const double *y1, double freq, double *dt, double *datt)
+{
+    double dx = y1[0] - y0[0], dy = y1[1] - y0[1], dz = y1[2] - y0[2];
+    double x = y0[0] + dx / 2.0, y = y0[1] + dy / 2.0, z = y0[2] + dz / 2.0;
+    double ds = sqrt(dx * dx + dy * dy + dz * dz), nu[3], cp[3];
+    for (int n = 0; n < 3; n++) nu[n] = y0[3 + n] + (y1[3 + n] - y0[3 + n]) / 2.0;
+    double nm = sqrt(nu[0] * nu[0] + nu[1] * nu[1] + nu[2] * nu[2]);
+    double c = go_rd_c(a, z, x, y);
+    cp[0] = c * nu[0] / nm + go_rd_u(a, z, x, y);
+    cp[1] = c * nu[1] / nm + go_rd_v(a, z, x, y);
+    cp[2] = c * nu[2] / nm + go_rd_w(a, z, x, y);
+    *dt = ds / sqrt(sq(cp[0]) + sq(cp[1]) + sq(cp[2]));
+    *datt = go_sb_alpha_at(c, go_rd_rho(a, z, x, y), z, 0, a->tweak_abs, freq) * ds;
+}
+
+/* jacobian, :704-721 */
+double go_rd3_jacobian(go_rd *a, const double *y)
+{
+    double c = go_rd_c(a, y[2], y[0], y[1]), nm = sqrt(sq(y[3]) + sq(y[4]) + sq(y[5])), cp[3];
+    cp[0] = c * y[3] / nm + go_rd_u(a, y[2], y[0], y[1]);
+    cp[1] = c * y[4] / nm + go_rd_v(a, y[2], y[0], y[1]);
+    cp[2] = c * y[5] / nm + go_rd_w(a, y[2], y[0], y[1]);
+    double cm = sqrt(sq(cp[0]) + sq(cp[1]) + sq(cp[2]));
+    double dxs = cp[0] / cm, dys = cp[1] / cm, dzs = cp[2] / cm;
+    return dxs * (y[7] * y[14] - y[13] * y[8]) - y[6] * (dys * y[14] - dzs * y[13]) + y[12] * (dys * y[8] - dzs * y[7]);
+}
+
+/* amp, :723-746 */
+double go_rd3_amp(go_rd *a, const go_src *s, const double *y)
+{
+    double x0 = s->src[0], y0 = s->src[1], z0 = s->src[2];
+    double c = go_rd_c(a, y[2], y[0], y[1]);
+    double w[3] = {go_rd_u(a, y[2], y[0], y[1]), go_rd_v(a, y[2], y[0], y[1]), go_rd_w(a, y[2], y[0], y[1])};
+    double w0[3] = {go_rd_u(a, z0, x0, y0), go_rd_v(a, z0, x0, y0), go_rd_w(a, z0, x0, y0)};
+    double nm = sqrt(sq(y[3]) + sq(y[4]) + sq(y[5]));
+    double nm0 = 1.0 - (y[3] * w0[0] + y[4] * w0[1] + y[5] * w0[2]) / s->c0;
+    double cp[3], cp0[3];
+    for (int n = 0; n < 3; n++) cp[n] = c * y[3 + n] / nm + w[n];
+    cp0[0] = s->c0 * cos(s->theta) * cos(s->phi) + w0[0];
+    cp0[1] = s->c0 * cos(s->theta) * sin(s->phi) + w0[1];
+    cp0[2] = s->c0 * sin(s->theta) + w0[2];
+    double cm = sqrt(sq(cp[0]) + sq(cp[1]) + sq(cp[2])), cm0 = sqrt(sq(cp0[0]) + sq(cp0[1]) + sq(cp0[2]));
+    double num = go_rd_rho(a, y[2], y[0], y[1]) * nm * pow(c, 3) * cm0 * cos(s->theta);
+    double den = go_rd_rho(a, z0, x0, y0) * nm0 * pow(s->c0, 3) * cm * go_rd3_jacobian(a, y);
+    return sqrt(fabs(num / den));
+}
+
+static void rd3_pt(go_path_pt *pts, long long cap, long long *n, go_rd *a, int li, int leg, int m, int kind,
+                   const double *y, double tl, double att, double tt)
+{
+    if (*n < cap) {
+        go_path_pt *P = pts + *n;
+        P->launch_idx = li; P->leg = leg; P->step = m; P->kind = kind;
+        P->pos0 = y[0]; P->pos1 = y[1];
+        P->z = dmax(y[2], go_rd_topo_z(a, y[0], y[1]));                  /* :456 */
+        P->tl = tl; P->absorption = -2.0 * att; P->time = tt;
+    }
+    (*n)++;
+}
+
+/* run_prop, main.infraga.3d.rngdep.cpp:400-540.  write = 0: the
+ * write_rays=false sums (travel_time / atten over rows 0..k-1 of a leg);
+ * write = 1: the write_rays / write_caustics bookkeeping (rows 1..k-1 one
+ * segment at a time, rows m == 1 or m % 15 == 0 and Jacobian sign changes
+ * written to pts as flags select).  pr->src0, src1: x, y [km] (NaN: the
+ * region centre). */
+int go_rd3_fan(go_rd *a, const go_params *p, const go_prop *pr, const double *launches, int n_launch, int write,
+               int flags, go_arrival *out, int out_cap, int *n_out, go_path_pt *pts, long long pts_cap,
+               long long *n_pts, long long *ray_steps)
+{
+    int length = (int)(p->s_max / p->ds_min);
+    if (rd_rows_len < length) {
+        free(rd_rows);
+        rd_rows = (double *)malloc(sizeof(double) * 18 * (size_t)length);
+        rd_rows_len = length;
+    }
+    double *rows = rd_rows;
+    go_src s;
+    int cnt = 0;
+    long long steps = 0, np = 0;
+    double x_src = pr->src0, y_src = pr->src1;
+    if (isnan(x_src)) x_src = (p->lat_max + p->lat_min) / 2.0;           /* :37-38 */
+    if (isnan(y_src)) y_src = (p->lon_max + p->lon_min) / 2.0;
+    double z_src = dmax(pr->src_alt, go_rd_topo_z(a, x_src, y_src));    /* :305 */
+    s.src[0] = x_src; s.src[1] = y_src; s.src[2] = z_src;
+
+    for (int li = 0; li < n_launch; li++) {
+        double incl = launches[2 * li], az = launches[2 * li + 1];
+        s.phi = (90.0 - az) * (RD_PI / 180.0);
+        s.theta = incl * (RD_PI / 180.0);
+        go_rd3_set_initial(a, p, &s, rows);
+        double tt = 0.0, att = 0.0, zmax = 0.0;
+        int brk = 0, k;
+        for (int bnc = 0; bnc <= pr->bounces; bnc++) {
+            k = rd3_prop_rk4(a, p, &s, rows, length, &brk);
+            steps += k;
+            if (write) {
+                double d_prev = (p->calc_amp && (flags & GO_PATH_CAUSTICS)) ? go_rd3_jacobian(a, rows + 18) : 0.0;
+                for (int m = 1; m < k; m++) {
+                    double dt, da, *ym = rows + 18 * m;
+                    rd3_seg(a, rows + 18 * (m - 1), ym, pr->freq, &dt, &da);
+                    tt += dt;
+                    att += da;
+                    if ((flags & GO_PATH_RAYS) && (m == 1 || m % 15 == 0))
+                        rd3_pt(pts, pts_cap, &np, a, li, bnc, m, 0, ym,
+                               p->calc_amp ? 20.0 * log10(go_rd3_amp(a, &s, ym)) : 0.0, att, tt);
+                    if (p->calc_amp && (flags & GO_PATH_CAUSTICS)) {
+                        double d = go_rd3_jacobian(a, ym);
+                        if (d * d_prev < 0.0) rd3_pt(pts, pts_cap, &np, a, li, bnc, m, 1, ym, 0.0, att, tt);
+                        d_prev = d;
+                    }
+                }
+            } else {
+                double tleg = 0.0, aleg = 0.0, dt, da;
+                for (int n = 0; n < k; n++) {
+                    rd3_seg(a, rows + 18 * n, rows + 18 * (n + 1), pr->freq, &dt, &da);
+                    tleg += dt;
+                    aleg += da;
+                }
+                tt += tleg;
+                att += aleg;
+            }
+            for (int m = 0; m < k; m++) zmax = dmax(zmax, rows[18 * m + 2]);      /* :490-492 */
+            if (!a->is_topo && zmax < (a->z0 + pr->turn_ht_min)) break;         /* :494 */
+            if (brk || k < 2 || (tt / 3600.0) > p->time_max) break;
+            double *yk = rows + 18 * k;
+            go_arrival A;
+            memset(&A, 0, sizeof(A));
+            A.launch_idx = li;
+            A.n_bounce = bnc;
+            A.incl_launch = incl;
+            A.az_launch = az;
+            A.incl_arr = -asin(go_rd_c(a, go_rd_topo_z(a, yk[0], yk[1]), yk[0], yk[1]) / go_rd_c(a, z_src, x_src, y_src)
+                               * yk[5]) * 180.0 / RD_PI;                       /* :502 */
+            double baz = 90.0 - atan2(-yk[4], -yk[3]) * 180.0 / RD_PI;
+            if (baz < -180.0) baz += 360.0;
+            else if (baz > 180.0) baz -= 360.0;
+            A.back_az = baz;
+            A.pos0 = yk[0];
+            A.pos1 = yk[1];
+            A.time = tt;
+            A.cel = sqrt(sq(yk[0] - x_src) + sq(yk[1] - y_src)) / tt;
+            A.turn_ht = zmax;
+            A.tl = p->calc_amp ? 20.0 * log10(go_rd3_amp(a, &s, yk)) : 0.0;
+            A.absorption = -2.0 * att;
+            if (cnt < out_cap) out[cnt] = A;
+            cnt++;
+            go_rd3_set_refl(a, p, &s, rows + 18 * (k - 2), rows + 18 * (k - 1), rows);
+        }
+    }
+    *n_out = cnt;
+    if (n_pts) *n_pts = np;
+    if (ray_steps) *ray_steps = steps;
+    return (cnt > out_cap || np > pts_cap) ? -1 : 0;
+}
+
+void *go_rd_new_xy(const double *x_km, int nx, const double *y_km, int ny, const double *z, int nz,
+                   const double *c, const double *u, const double *v, const double *rho)
+{
+    go_rd *a = (go_rd *)malloc(sizeof(go_rd));
+    if (go_rd_init_xy(a, x_km, nx, y_km, ny, z, nz, c, u, v, rho)) { free(a); return NULL; }
+    return a;
 }

@@ -30,6 +30,7 @@ typedef struct {
     go_s2d topo;              /* terrain height above sea level [km] */
     int is_topo;              /* geoac::is_topo */
     int use_blw;              /* topo::use_BLw (boundary-layer vertical wind) */
+    int cart;                 /* 3D Cartesian grid: node axes in km, r0 = 0 */
     double r0;                /* globe radius, 6370 km */
     double z0, z_max, z_bndlyr;
     double bl_param;          /* topo::bndlyr_param, 0.1 km */
@@ -128,6 +129,20 @@ long long go_rd_eig_path(go_rd *a, const go_params *p, const double src_in[3], d
                          int bnc_cnt, double freq, go_path_pt *pts, long long cap, go_arrival *A);
 
 /* ---- opaque handles for ctypes ------------------------------------------ */
+/* 3D Cartesian range-dependent geometry (infraga-3d-rngdep) */
+int    go_rd_init_xy(go_rd *a, const double *x_km, int nx, const double *y_km, int ny, const double *z, int nz,
+                     const double *c, const double *u, const double *v, const double *rho);
+void   go_rd3_rhs(go_rd *a, const go_params *p, const double *y, double *f);
+void   go_rd3_set_initial(go_rd *a, const go_params *p, go_src *s, double *y);
+void   go_rd3_set_refl(go_rd *a, const go_params *p, const go_src *s, const double *rk2, const double *rk1, double *y0);
+double go_rd3_jacobian(go_rd *a, const double *y);
+double go_rd3_amp(go_rd *a, const go_src *s, const double *y);
+int    go_rd3_fan(go_rd *a, const go_params *p, const go_prop *pr, const double *launches, int n_launch, int write,
+                  int flags, go_arrival *out, int out_cap, int *n_out, go_path_pt *pts, long long pts_cap,
+                  long long *n_pts, long long *ray_steps);
+void  *go_rd_new_xy(const double *x_km, int nx, const double *y_km, int ny, const double *z, int nz,
+                    const double *c, const double *u, const double *v, const double *rho);
+
 void  *go_rd_new(const double *lat_deg, int nlat, const double *lon_deg, int nlon, const double *z,
                  int nz, const double *c, const double *u, const double *v, const double *rho);
 void   go_rd_delete(void *h);

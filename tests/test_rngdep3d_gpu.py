@@ -1,0 +1,86 @@
+"""GPU parity of the 3D Cartesian range-dependent geometry
+(GA_GEOM_3D_RNGDEP, infraga-3d-rngdep -prop) against the oracle's
+go_rd3_fan, whose outputs for this workload are the committed fixture
+tests/golden/rd3.npz (make_rd3_golden.py).
+
+Workload: a 6 x 6 node grid over +-500 km with horizontally varying c, u,
+v, rho (seeded), terrain 0-2 km on an 81 x 81 grid, source at the region
+centre, 18 and 6 equations, write_rays=false sums and the write-mode
+bookkeeping with ray-path and caustic rows.  Bar: arrival keys and RK4
+step counts exact, positions, time, turning height within 1e-6 relative,
+TL and absorption within 0.01 dB; path-row keys exact, values at the same
+bar (compare_paths).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from infraga_amd import synth
+from infraga_amd.engine import GA_GEOM_3D_RNGDEP, GA_PATH_CAUSTICS, GA_PATH_RAYS, prop_params
+from parity import compare, compare_paths
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(HERE, "golden", "rd3.npz")
+
+
+def workload():
+    """(units, topo, launches): node x, y [km], z, c, u, v, rho in engine
+    units; terrain (x, y, height); (inclination, azimuth) launches."""
+    grid = synth.rngdep_grid(3, lat=np.linspace(30, 50, 6), lon=np.linspace(-110, -90, 6), dz=0.5, z_top=120.0)
+    _, _, z, c, u, v, rho = synth.rngdep_engine_units(*grid)
+    x = np.linspace(-500.0, 500.0, 6)
+    units = (x, x.copy(), z, c, u, v, rho)
+    t = synth.fractal_topo(5, lat=np.linspace(30, 50, 81), lon=np.linspace(-110, -90, 81), amp_km=2.0)
+    tx = np.linspace(-520.0, 520.0, 81)
+    topo = (tx, tx.copy(), t[2])
+    launches = np.array([[th, az] for az in (-90.0, 45.0, 160.0) for th in (4.0, 12.0, 19.0)])
+    return units, topo, launches
+
+
+CASES = [("amp18", 1, 0), ("amp6", 0, 0), ("paths18", 1, GA_PATH_RAYS | GA_PATH_CAUSTICS)]
+
+
+@pytest.fixture(scope="module")
+def rd3(engine):
+    units, topo, launches = workload()
+    rid = engine.rngdep_load_xy(*units)
+    engine.topo_load(rid, *topo)
+    yield rid, launches
+    engine.rngdep_release(rid)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case,amp,flags", CASES, ids=[c[0] for c in CASES])
+def test_fan_matches_oracle(engine, rd3, case, amp, flags):
+    rid, launches = rd3
+    g = np.load(GOLDEN)
+    ref, steps = g[case], int(g[case + "_steps"])
+    p = prop_params(GA_GEOM_3D_RNGDEP, calc_amp=amp, bounces=2)
+    if flags:
+        arr, pts, st = engine.fan_paths(rid, p, launches, flags)
+    else:
+        arr, st = engine.fan(rid, p, launches)
+    print(f"\n{case}: {len(arr)} arrivals (oracle {len(ref)}), {st.ray_steps} ray-steps (oracle {steps}), "
+          f"{st.kernel_ms:.1f} ms")
+    assert len(ref) > 0 and st.ray_steps == steps
+    fails, summ = compare(arr, ref, False, src=(0.0, 0.0))
+    assert not fails, (fails, summ)
+    np.testing.assert_allclose(arr["incl_arr"], ref["incl_arr"], rtol=0, atol=1e-6)
+    np.testing.assert_allclose(arr["back_az"], ref["back_az"], rtol=0, atol=1e-6)
+    if flags:
+        rp = g[case + "_pts"]
+        pf, ps = compare_paths(pts, rp, False, src=(0.0, 0.0, 0.0))
+        print(ps)
+        assert not pf, (pf, ps)
+
+
+@pytest.mark.gpu
+def test_geometry_and_grid_must_agree(engine, rd3):
+    """A Cartesian grid with the spherical geometry (and the converse) is an error."""
+    from infraga_amd.engine import GA_GEOM_SPH_RNGDEP, EngineError
+    rid, launches = rd3
+    with pytest.raises(EngineError):
+        engine.fan(rid, prop_params(GA_GEOM_SPH_RNGDEP), launches[:1])
+    with pytest.raises(EngineError):
+        engine.fan(rid, prop_params(GA_GEOM_3D_RNGDEP, topo_use_blw=1), launches[:1])
