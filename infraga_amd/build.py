@@ -45,7 +45,8 @@ def build_frontends():
     # let rank 0 do the whole job.
     for accel in (0, 1):
         for name, geom in (("infraga-sph", "GA_GEOM_SPH"), ("infraga-3d", "GA_GEOM_3D"),
-                           ("infraga-sph-rngdep", "GA_GEOM_SPH_RNGDEP"), ("infraga-2d", "GA_GEOM_2D")):
+                           ("infraga-sph-rngdep", "GA_GEOM_SPH_RNGDEP"), ("infraga-2d", "GA_GEOM_2D"),
+                           ("infraga-3d-rngdep", "GA_GEOM_3D_RNGDEP")):
             if accel and geom == "GA_GEOM_2D":
                 continue
             if accel:

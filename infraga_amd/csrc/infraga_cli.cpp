@@ -372,13 +372,17 @@ int run_prop(int argc, char **argv)
 // (run_prop, src/main.infraga.sph.rngdep.cpp:205-562).
 int run_prop_rd(int argc, char **argv)
 {
+    // infraga-3d-rngdep (main.infraga.3d.rngdep.cpp:205-540): the same run
+    // over x / y node files in km
+    constexpr bool cart = GA_CLI_GEOM == GA_GEOM_3D_RNGDEP;
     if (argc < 5) {
-        std::cerr << GA_CLI_NAME << ": usage: -prop <profile prefix> <lat nodes> <lon nodes> [parameters]\n";
+        std::cerr << GA_CLI_NAME << (cart ? ": usage: -prop <profile prefix> <x nodes> <y nodes> [parameters]\n"
+                                          : ": usage: -prop <profile prefix> <lat nodes> <lon nodes> [parameters]\n");
         return 2;
     }
     double theta_min = 0.5, theta_max = 45.0, theta_step = 0.5;
     double phi_min = -90.0, phi_max = -90.0, phi_step = 1.0;
-    int bounces = 2;
+    int bounces = cart ? 10 : 2;
     double lat_src = NAN, lon_src = NAN, z_src = 0.0;
     bool write_rays = true, write_caustics = false, reverse_winds = false, custom_id = false;
     double freq = 0.1, turn_ht_min = 0.2;
@@ -386,7 +390,7 @@ int run_prop_rd(int argc, char **argv)
     const char *topo_file = nullptr;
     const char *output_id = nullptr;
     ga_prop_params p;
-    ga_prop_params_default(&p, GA_GEOM_SPH_RNGDEP);
+    ga_prop_params_default(&p, cart ? GA_GEOM_3D_RNGDEP : GA_GEOM_SPH_RNGDEP);
     p.calc_amp = 1;                                    // run_prop :226
     double z_grnd = NAN;
 
@@ -402,20 +406,26 @@ int run_prop_rd(int argc, char **argv)
         else if (key(a, "azimuth=", v)) { phi_min = phi_max = std::atof(v); phi_step = 1.0; }
         else if (key(a, "bounces=", v)) bounces = std::atoi(v);
         else if (key2(a, "src_alt=", "alt_src=", v)) z_src = std::atof(v);
-        else if (key2(a, "src_lat=", "lat_src=", v)) lat_src = std::atof(v);
-        else if (key2(a, "src_lon=", "lon_src=", v)) lon_src = std::atof(v);
+        else if (!cart && key2(a, "src_lat=", "lat_src=", v)) lat_src = std::atof(v);
+        else if (!cart && key2(a, "src_lon=", "lon_src=", v)) lon_src = std::atof(v);
+        else if (cart && key2(a, "src_x=", "x_src=", v)) lat_src = std::atof(v);
+        else if (cart && key2(a, "src_y=", "y_src=", v)) lon_src = std::atof(v);
         else if (key(a, "turn_ht_min=", v)) turn_ht_min = std::atof(v);
         else if (key(a, "write_rays=", v)) write_rays = str2bool(v);
         else if (key(a, "freq=", v)) freq = std::atof(v);
         else if (key(a, "abs_coeff=", v)) p.abs_coeff = std::max(0.0, std::atof(v));
-        else if (key(a, "calc_amp=", v)) p.calc_amp = str2bool(v) ? 1 : 0;
+        else if (key(a, "calc_amp=", v)) {
+            // the 3D binary compares 8 characters and parses "=<value>",
+            // which string2bool reads as false (main.infraga.3d.rngdep.cpp:269)
+            p.calc_amp = (!cart && str2bool(v)) ? 1 : 0;
+        }
         else if (key2(a, "max_alt=", "alt_max=", v)) p.alt_max = std::atof(v);
         else if (key2(a, "max_rng=", "rng_max=", v)) p.rng_max = std::atof(v);
         else if (key2(a, "max_time=", "time_max=", v)) p.time_max = std::atof(v);
-        else if (key2(a, "min_lat=", "lat_min=", v)) p.bounds[0] = std::atof(v);
-        else if (key2(a, "max_lat=", "lat_max=", v)) p.bounds[1] = std::atof(v);
-        else if (key2(a, "min_lon=", "lon_min=", v)) p.bounds[2] = std::atof(v);
-        else if (key2(a, "max_lon=", "lon_max=", v)) p.bounds[3] = std::atof(v);
+        else if (key2(a, cart ? "min_x=" : "min_lat=", cart ? "x_min=" : "lat_min=", v)) p.bounds[0] = std::atof(v);
+        else if (key2(a, cart ? "max_x=" : "max_lat=", cart ? "x_max=" : "lat_max=", v)) p.bounds[1] = std::atof(v);
+        else if (key2(a, cart ? "min_y=" : "min_lon=", cart ? "y_min=" : "lon_min=", v)) p.bounds[2] = std::atof(v);
+        else if (key2(a, cart ? "max_y=" : "max_lon=", cart ? "y_max=" : "lon_max=", v)) p.bounds[3] = std::atof(v);
         else if (key2(a, "min_ds=", "ds_min=", v)) p.ds_min = std::atof(v);
         else if (key2(a, "max_ds=", "ds_max=", v)) p.ds_max = std::atof(v);
         else if (key2(a, "max_s=", "s_max=", v)) p.s_max = std::atof(v);
@@ -446,18 +456,20 @@ int run_prop_rd(int argc, char **argv)
     ga_ctx *ctx = nullptr;
     if (ga_ctx_create(&ctx, 0) != GA_OK) return die(nullptr, "no usable GPU");
     int rid = 0;
-    if (ga_rngdep_load_files(ctx, argv[2], argv[3], argv[4], topo_file, prof_format, reverse_winds ? 1 : 0, &rid) != GA_OK)
+    if ((cart ? ga_rngdep_load_files_xy : ga_rngdep_load_files)(ctx, argv[2], argv[3], argv[4], topo_file, prof_format,
+                                                               reverse_winds ? 1 : 0, &rid) != GA_OK)
         return die(ctx, "cannot load the range-dependent atmosphere");
     double info[8];
     if (ga_rngdep_info(ctx, rid, info) != GA_OK) return die(ctx, "region");
     const bool is_topo = topo_file != nullptr;
-    if (std::isnan(lat_src)) lat_src = (info[1] + info[0]) / 2.0 * (180.0 / Pi);   // :242-243
-    if (std::isnan(lon_src)) lon_src = (info[3] + info[2]) / 2.0 * (180.0 / Pi);
+    const double deg = cart ? 1.0 : Pi / 180.0;   // region in radians (sph) or km (3D)
+    if (std::isnan(lat_src)) lat_src = (info[1] + info[0]) / 2.0 / deg;   // :242-243
+    if (std::isnan(lon_src)) lon_src = (info[3] + info[2]) / 2.0 / deg;
     const double z0 = (!is_topo && !std::isnan(z_grnd)) ? z_grnd : info[5];
     if (!is_topo) p.z_grnd = z0;
 
     // ground under the source and its slope (topo::z, topo::dz)
-    const double la = lat_src * (Pi / 180.0), lo = lon_src * (Pi / 180.0);
+    const double la = lat_src * deg, lo = lon_src * deg;
     double pt[3] = {la, lo, 0.0}, atmo_out[40], topo_out[10];
     if (ga_rngdep_eval(ctx, rid, pt, 1, atmo_out, topo_out) != GA_OK) return die(ctx, "terrain at the source");
     const double zg = is_topo ? topo_out[0] : z0;
@@ -479,8 +491,8 @@ int run_prop_rd(int argc, char **argv)
         double theta_grnd;
         if (z_src - zg < 0.01) {
             if (is_topo) {
-                const double dzgdx = 1.0 / (r0 * std::cos(la)) * topo_out[2];
-                const double dzgdy = 1.0 / r0 * topo_out[1];
+                const double dzgdx = cart ? topo_out[1] : 1.0 / (r0 * std::cos(la)) * topo_out[2];
+                const double dzgdy = cart ? topo_out[2] : 1.0 / r0 * topo_out[1];
                 theta_grnd = std::atan(dzgdx * std::sin(phi * Pi / 180.0) + dzgdy * std::cos(phi * Pi / 180.0)) * (180.0 / Pi) + 0.1;
             } else {
                 theta_grnd = 0.1;
@@ -497,7 +509,7 @@ int run_prop_rd(int argc, char **argv)
     ga_run_stats st;
     std::string id = custom_id ? std::string(output_id) : std::string(argv[2]);
     if (write_rays || write_caustics) {
-        if (fan_with_paths(ctx, rid, p, launches, write_rays, write_caustics, id, true, arr, n_out, st) != GA_OK)
+        if (fan_with_paths(ctx, rid, p, launches, write_rays, write_caustics, id, !cart, arr, n_out, st) != GA_OK)
             return die(ctx, "fan failed");
     } else if (ga_fan_run(ctx, rid, &p, launches.data(), n, arr.data(), (int)arr.size(), &n_out, &st) != GA_OK) {
         return die(ctx, "fan failed");
@@ -509,12 +521,13 @@ int run_prop_rd(int argc, char **argv)
     results << "#" << '\t' << "inclination: " << theta_min << ", " << theta_max << ", " << theta_step << '\n';
     results << "#" << '\t' << "azimuth: " << phi_min << ", " << phi_max << ", " << phi_step << '\n';
     results << "#" << '\t' << "bounces: " << bounces << '\n';
-    results << "#" << '\t' << "source location (lat, lon, alt): " << lat_src << ", " << lon_src << ", " << z_src << '\n';
+    results << "#" << '\t' << (cart ? "source location (x, y, alt): " : "source location (lat, lon, alt): ") << lat_src << ", " << lon_src << ", " << z_src << '\n';
     if (!is_topo) results << "#" << '\t' << "ground elevation: " << z0 << '\n';
     else results << "#" << '\t' << "topo file:" << topo_file << '\n';
     results << "#" << '\t' << "frequency: " << freq << '\n';
     results << "#" << '\t' << "S&B atten coeff: " << p.abs_coeff << '\n' << '\n';
-    results << "# incl [deg]" << '\t' << "az [deg]" << '\t' << "n_b" << '\t' << "lat_0 [deg]" << '\t' << "lon_0 [deg]"
+    results << "# incl [deg]" << '\t' << "az [deg]" << '\t' << "n_b" << '\t' << (cart ? "x_0 [km]" : "lat_0 [deg]") << '\t'
+            << (cart ? "y_0 [km]" : "lon_0 [deg]")
             << '\t' << "time [s]" << '\t' << "cel [km/s]" << '\t' << "turning ht [km]" << '\t' << "inclination [deg]"
             << '\t' << "back azimuth [deg]" << '\t' << "trans. coeff. [dB]" << '\t' << "absorption [dB]" << '\n';
     int w = 0, az_i = 0;
@@ -1181,6 +1194,11 @@ int main(int argc, char **argv)
     if (argc < 3) { usage(); return argc < 2 ? 0 : 2; }
     if (GA_CLI_GEOM == GA_GEOM_2D) {
         if (std::strncmp(argv[1], "-prop", 5) == 0) return run_prop_2d(argc, argv);
+        std::cerr << GA_CLI_NAME << ": option " << argv[1] << " is not supported by this engine\n";
+        return 2;
+    }
+    if (GA_CLI_GEOM == GA_GEOM_3D_RNGDEP) {
+        if (std::strncmp(argv[1], "-prop", 5) == 0) return run_prop_rd(argc, argv);
         std::cerr << GA_CLI_NAME << ": option " << argv[1] << " is not supported by this engine\n";
         return 2;
     }

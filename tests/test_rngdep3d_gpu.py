@@ -84,3 +84,34 @@ def test_geometry_and_grid_must_agree(engine, rd3):
         engine.fan(rid, prop_params(GA_GEOM_SPH_RNGDEP), launches[:1])
     with pytest.raises(EngineError):
         engine.fan(rid, prop_params(GA_GEOM_3D_RNGDEP, topo_use_blw=1), launches[:1])
+
+
+@pytest.mark.gpu
+def test_cli_3d_rngdep_prop(tmp_path, oracle):
+    """infraga-3d-rngdep -prop over the reference's node-file layout: the
+    arrivals file equals the oracle's rows at the same launch angles to the
+    printed digits (source 3 km up, above the terrain, so the inclinations
+    start at incl_min).  calc_amp= is given, which the reference's 3D binary
+    reads as false (TL column 0)."""
+    import subprocess
+    from ga_oracle_py import Rd3Medium, write_rd_files
+    units, topo, _ = workload()
+    x, y, z, c, u, v, rho = units
+    pre, xs, ys, tp = write_rd_files(str(tmp_path), x, y, z, c * 1000.0, u * 1000.0, v * 1000.0, rho, topo=topo)
+    exe = os.path.join(os.path.dirname(HERE), "infraga_amd", "bin", "infraga-3d-rngdep")
+    r = subprocess.run([exe, "-prop", pre, xs, ys, f"topo_file={tp}", "prof_format=zcuvd", "write_rays=false",
+                        "incl_min=6", "incl_max=20", "incl_step=7", "azimuth=45", "bounces=2", "calc_amp=true",
+                        "src_alt=3.0"],
+                       capture_output=True, text=True, cwd=tmp_path, timeout=120)
+    assert r.returncode == 0, r.stderr
+    rows = [[float(t) for t in l.split("\t")] for l in open(pre + ".arrivals.dat") if l.strip() and not l.startswith("#")]
+    assert len(rows) > 0
+    m = Rd3Medium(oracle, *units)
+    m.set_topo(*topo)
+    L = sorted({(row[0], row[1]) for row in rows})
+    ref, _, _ = m.fan3(L, m.params(False), bounces=2, src=(float("nan"), float("nan"), 3.0))
+    assert len(ref) == len(rows)
+    for row, a in zip(rows, ref):
+        exp = [a["incl_launch"], a["az_launch"], a["n_bounce"], a["pos0"], a["pos1"], a["time"], a["cel"],
+               a["turn_ht"], a["incl_arr"], a["back_az"], 0.0, a["absorption"]]
+        np.testing.assert_allclose(row, exp, rtol=1e-5, atol=1e-5)
