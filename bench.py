@@ -172,6 +172,48 @@ def rngdep_leg(eng, with_cpu):
     return out
 
 
+def rngdep3d_leg(eng, with_cpu, target_s=8.0):
+    """3D Cartesian range-dependent fan (infraga-3d-rngdep): the grid and
+    terrain of the spherical range-dependent leg with node coordinates read
+    as km over +-800 km."""
+    from infraga_amd import synth
+    from infraga_amd.engine import GA_GEOM_3D_RNGDEP, prop_params
+    _, _, z, c, u, v, rho = synth.rngdep_engine_units(*synth.rngdep_grid())
+    x = np.linspace(-800.0, 800.0, c.shape[0])
+    units = (x, np.linspace(-800.0, 800.0, c.shape[1]), z, c, u, v, rho)
+    t = synth.fractal_topo()
+    topo = (np.linspace(-820.0, 820.0, len(t[0])), np.linspace(-820.0, 820.0, len(t[1])), t[2])
+    L = synth.fan_launches(0.2, 40.0, 0.2, 0.0, 359.0, 1.0)
+    rid = eng.rngdep_load_xy(*units)
+    eng.topo_load(rid, *topo)
+    p = prop_params(GA_GEOM_3D_RNGDEP, calc_amp=1, bounces=4)
+    eng.fan(rid, p, L[::997])                        # warm-up (buffers, kernel load)
+    t0 = time.perf_counter()
+    arr, st = eng.fan(rid, p, L)
+    wall = time.perf_counter() - t0
+    eng.rngdep_release(rid)
+    out = {"workload": f"3D range-dependent: {len(x)} x {len(units[1])} profiles x {len(z)} levels over +-800 km, "
+                       f"{len(topo[0])} x {len(topo[1])} terrain 0-2 km, 200 incl x 360 az = {len(L)} rays, 4 bounces, 18 eq",
+           "rays": int(len(L)), "ray_steps": int(st.ray_steps), "arrivals": int(len(arr)),
+           "kernel_ms": st.kernel_ms, "wall_ms": wall * 1e3, "ray_steps_per_s": st.ray_steps / wall}
+    if with_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        from ga_oracle_py import Oracle, Rd3Medium
+        m = Rd3Medium(Oracle(), *units)
+        m.set_topo(*topo)
+        op = m.params(True)
+        steps, n, t0 = 0, 0, time.process_time()
+        for i in range(0, len(L), 4801):
+            steps += m.fan3(L[i:i + 1], op, bounces=4)[2]
+            n += 1
+            if time.process_time() - t0 > target_s:
+                break
+        dt = time.process_time() - t0
+        out["cpu_baseline"] = {"value": steps / dt, "unit": "ray-steps/s", "cores": 1, "kind": "port",
+                               "sample": f"{n} rays (every 4801st launch) of the same fan, {steps} ray-steps, {dt:.1f} s"}
+    return out
+
+
 def serve_trace(n_req=24, qps=1.5, seed=11):
     """Seeded request trace: (arrival_s, kind, atmosphere index, size)."""
     rng = np.random.Generator(np.random.PCG64(seed))
@@ -369,6 +411,7 @@ def main():
         line["served"] = serve_leg(eng, serve_trace())
     if not args.no_rngdep:
         line["rngdep"] = rngdep_leg(eng, not args.no_cpu)
+        line["rngdep3d"] = rngdep3d_leg(eng, not args.no_cpu)
     eng.close()
     print(json.dumps(line))
 
