@@ -198,6 +198,22 @@ int ga_fan_run_device(ga_ctx *ctx, int atmo_id, const ga_prop_params *p,
                       const double *d_launches, int n_launch,
                       ga_arrival *d_out, int32_t *d_counts, int32_t *d_steps, void *stream);
 
+/* ga_fan_run_device without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The work queue counter, the per-lane row buffer and the raw
+ * arrival records live in a fixed 48 MiB device array that the library
+ * holds on each GPU it runs on until the process ends; the call makes no
+ * hipMalloc / hipFree and launches the same kernels as ga_fan_run_device.
+ * Stratified geometries only (GA_GEOM_SPH, GA_GEOM_3D); n_launch x
+ * (bounces + 1) may be at most 157,285 arrival slots (GA_E_ARG beyond, with
+ * nothing launched).  Every call on one GPU shares that array: calls that
+ * overlap on one GPU (other streams, other contexts or threads) race, so
+ * serialise them.  The caller's buffers are as for ga_fan_run_device. */
+int ga_fan_run_device_noalloc(ga_ctx *ctx, int atmo_id, const ga_prop_params *p,
+                              const double *d_launches, int n_launch,
+                              ga_arrival *d_out, int32_t *d_counts, int32_t *d_steps, void *stream);
+
 /* ga_fan_run with the reference's write_rays / write_caustics bookkeeping
  * (path_flags = GA_PATH_RAYS | GA_PATH_CAUSTICS): travel time and
  * absorption are summed segment by segment over rows 1..k-1 of each leg, so

@@ -284,8 +284,14 @@ struct FanBufs {
     size_t raw_cap = 0;            // doubles
     double *yp = nullptr;
     size_t yp_cap = 0;             // doubles
+    // fixed: the buffers are views of preallocated memory that must never be
+    // grown or freed (a launch that needs more fails instead); max_lanes
+    // (> 0) caps the persistent grid so that the lane buffer fits.
+    bool fixed = false;
+    size_t max_lanes = 0;
     void release()
     {
+        if (fixed) return;
         if (queue) (void)hipFree(queue);
         if (scalar) (void)hipFree(scalar);
         if (raw) (void)hipFree(raw);
@@ -294,8 +300,23 @@ struct FanBufs {
     }
 };
 
+// Device scratch of ga_fan_run_device_noalloc: the queue counter, the
+// per-lane row k-1 buffer of NOALLOC_LANES lanes and the raw arrival
+// records.  Each device holds its own copy of this array for the life of
+// the process; host code reaches it only through the address that
+// hipGetSymbolAddress returns on that device (scratch_addr), never through
+// the array's name.
+constexpr size_t NOALLOC_BYTES = size_t(48) << 20;
+constexpr size_t NOALLOC_LANES = 131072;                  // 256 CUs x 512 lanes
+constexpr size_t NOALLOC_YP_OFF = 256;                    // after the queue counter
+constexpr size_t NOALLOC_RAW_OFF = NOALLOC_YP_OFF + NOALLOC_LANES * 18 * sizeof(double);
+constexpr size_t NOALLOC_RAW_DOUBLES = (NOALLOC_BYTES - NOALLOC_RAW_OFF) / sizeof(double);
+static_assert(NOALLOC_RAW_OFF % 256 == 0, "raw records stay 256-byte aligned");
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
 struct ga_ctx {
     int device = 0;
+    unsigned char *scratch_ptr = nullptr;   // this device's address of scratch
     hipStream_t stream = nullptr;
     FanBufs fb;                    // scratch of ga_fan_run / ga_fan_run_device
     int n_cu = 0;
@@ -326,6 +347,22 @@ int fail(ga_ctx *ctx, int code, const char *fmt, ...)
         hipError_t e_ = (call);                                                               \
         if (e_ != hipSuccess) return fail((ctx), GA_E_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
     } while (0)
+
+// Address of scratch on device dev, looked up once per device with that
+// device current (the caller has made it current); nullptr if the lookup
+// fails.
+unsigned char *scratch_addr(int dev)
+{
+    static std::mutex mu;
+    static std::map<int, unsigned char *> addr;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = addr.find(dev);
+    if (it != addr.end()) return it->second;
+    void *ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return nullptr;
+    addr[dev] = (unsigned char *)ptr;
+    return (unsigned char *)ptr;
+}
 
 // Natural cubic spline slopes by the Thomas algorithm, in the operation
 // order of interp::set (src/util/interpolation.cpp:103-136).
@@ -432,10 +469,12 @@ int launch_fan(ga_ctx *ctx, FanBufs &fb, const FanArgs &A, hipStream_t st, bool 
     int grid = per_cu * ctx->n_cu;
     int need = (A.n_rays + W - 1) / W;
     grid = (A.one_shot && !paths) ? std::max(1, need) : std::max(1, std::min(grid, need));
+    if (fb.max_lanes > 0 && !A.one_shot) grid = std::max(1, std::min(grid, (int)(fb.max_lanes / W)));
     FanArgs B = A;
     B.nlanes = grid * W;
     size_t ypn = (size_t)B.nlanes * 18;
     if (ypn > fb.yp_cap) {
+        if (fb.fixed) return fail(ctx, GA_E_CAP, "lane buffer of %zu doubles exceeds the fixed %zu", ypn, fb.yp_cap);
         HIPCHK(ctx, hipStreamSynchronize(st));
         HIPCHK(ctx, hipFree(fb.yp));
         fb.yp = nullptr;
@@ -857,6 +896,7 @@ int fan_enqueue(ga_ctx *ctx, FanBufs &fb, int atmo_id, const ga_prop_params *p, 
     {
         size_t need = (size_t)std::max(n, 1) * (size_t)(p->bounces + 1) * RAW;
         if (need > fb.raw_cap) {
+            if (fb.fixed) return fail(ctx, GA_E_CAP, "raw arrival records of %zu doubles exceed the fixed %zu", need, fb.raw_cap);
             HIPCHK(ctx, hipStreamSynchronize(st));
             HIPCHK(ctx, hipFree(fb.raw));
             fb.raw = nullptr;
@@ -870,10 +910,10 @@ int fan_enqueue(ga_ctx *ctx, FanBufs &fb, int atmo_id, const ga_prop_params *p, 
     A.out = d_out;
     A.counts = d_counts;
     A.steps = d_steps;
-    if (!fb.queue && hipMalloc(&fb.queue, sizeof(unsigned int)) != hipSuccess)
+    if (!fb.queue && (fb.fixed || hipMalloc(&fb.queue, sizeof(unsigned int)) != hipSuccess))
         return fail(ctx, GA_E_NOMEM, "queue counter");
     A.queue = fb.queue;
-    A.one_shot = serve_one_shot;
+    A.one_shot = fb.fixed ? 0 : serve_one_shot;
     if (path) A.path = *path;
     const bool pw = path != nullptr;
     size_t lds = (size_t)a.n * NODE * sizeof(double);
@@ -1831,6 +1871,13 @@ int ga_ctx_create(ga_ctx **out, int dev)
         delete ctx;
         return GA_E_NOMEM;
     }
+    ctx->scratch_ptr = scratch_addr(dev);
+    if (!ctx->scratch_ptr) {
+        ctx->fb.release();
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return GA_E_HIP;
+    }
     *out = ctx;
     return GA_OK;
 }
@@ -1945,6 +1992,32 @@ int ga_fan_run_device(ga_ctx *ctx, int atmo_id, const ga_prop_params *p, const d
     HIPCHK(ctx, hipSetDevice(ctx->device));
     hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
     return fan_enqueue(ctx, ctx->fb, atmo_id, p, d_launches, n, d_out, d_counts, d_steps, st);
+}
+
+int ga_fan_run_device_noalloc(ga_ctx *ctx, int atmo_id, const ga_prop_params *p, const double *d_launches, int n,
+                              ga_arrival *d_out, int32_t *d_counts, int32_t *d_steps, void *stream)
+{
+    if (!ctx || !p || (n > 0 && (!d_launches || !d_out || !d_counts || !d_steps))) return fail(ctx, GA_E_ARG, "null argument");
+    if (n < 0) return fail(ctx, GA_E_ARG, "negative ray count");
+    if (p->geometry != GA_GEOM_SPH && p->geometry != GA_GEOM_3D)
+        return fail(ctx, GA_E_ARG, "the no-allocation fan takes stratified geometries only (got %d)", p->geometry);
+    if (p->bounces < 0) return fail(ctx, GA_E_ARG, "bounces must be >= 0");
+    const size_t slots = (size_t)std::max(n, 1) * (size_t)(p->bounces + 1);
+    if (slots > NOALLOC_RAW_DOUBLES / RAW)
+        return fail(ctx, GA_E_ARG, "rays x (bounces + 1) = %zu exceeds the no-allocation limit of %zu arrival slots",
+                    slots, NOALLOC_RAW_DOUBLES / RAW);
+    if (!ctx->scratch_ptr) return fail(ctx, GA_E_HIP, "device scratch address unavailable");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+    FanBufs fb;
+    fb.fixed = true;
+    fb.queue = (unsigned int *)ctx->scratch_ptr;
+    fb.yp = (double *)(ctx->scratch_ptr + NOALLOC_YP_OFF);
+    fb.yp_cap = NOALLOC_LANES * 18;
+    fb.max_lanes = NOALLOC_LANES;
+    fb.raw = (double *)(ctx->scratch_ptr + NOALLOC_RAW_OFF);
+    fb.raw_cap = NOALLOC_RAW_DOUBLES;
+    return fan_enqueue(ctx, fb, atmo_id, p, d_launches, n, d_out, d_counts, d_steps, st);
 }
 
 int ga_fan_run(ga_ctx *ctx, int atmo_id, const ga_prop_params *p, const double *launches, int n,

@@ -28,7 +28,8 @@ EXPORTED = ["ga_prop_params_default", "ga_eig_params_default", "ga_ctx_create", 
             "ga_server_stats", "ga_server_destroy", "ga_rngdep_load", "ga_rngdep_load_xy", "ga_rngdep_load_files_xy",
             "ga_topo_load", "ga_rngdep_eval",
             "ga_rngdep_release", "ga_rngdep_load_files", "ga_rngdep_info", "ga_fan_run_paths",
-            "ga_eig_search_paths", "ga_eig_direct", "ga_topo_load_2d", "ga_topo_eval_2d"]
+            "ga_eig_search_paths", "ga_eig_direct", "ga_topo_load_2d", "ga_topo_eval_2d",
+            "ga_fan_run_device_noalloc"]
 
 GA_PATH_RAYS, GA_PATH_CAUSTICS = 1, 2
 GA_PT_RAY, GA_PT_CAUSTIC = 0, 1
@@ -120,6 +121,7 @@ def load_library(path: str | None = None):
                              C.c_int, C.POINTER(C.c_int), C.POINTER(RunStats)]
     L.ga_fan_run_device.argtypes = [C.c_void_p, C.c_int, C.POINTER(PropParams), C.c_void_p, C.c_int,
                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.ga_fan_run_device_noalloc.argtypes = L.ga_fan_run_device.argtypes
     L.ga_eig_search.argtypes = [C.c_void_p, C.c_int, C.POINTER(PropParams), C.POINTER(EigParams), _dp, C.c_int,
                                 C.POINTER(Arrival), C.c_int, C.POINTER(C.c_int), _dp, C.POINTER(RunStats)]
     L.ga_fan_run_paths.argtypes = [C.c_void_p, C.c_int, C.POINTER(PropParams), _dp, C.c_int, C.c_int,
@@ -260,6 +262,13 @@ class Engine:
         self._check(self.L.ga_fan_run_device(self.h, aid, C.byref(params), C.c_void_p(d_launches), n,
                                              C.c_void_p(d_out), C.c_void_p(d_counts), C.c_void_p(d_steps),
                                              C.c_void_p(stream) if stream else None))
+
+    def fan_device_noalloc(self, aid, params: PropParams, d_launches: int, n: int, d_out: int, d_counts: int,
+                           d_steps: int, stream: int = 0):
+        """fan_device through the entry point that allocates no device memory."""
+        self._check(self.L.ga_fan_run_device_noalloc(self.h, aid, C.byref(params), C.c_void_p(d_launches), n,
+                                                     C.c_void_p(d_out), C.c_void_p(d_counts), C.c_void_p(d_steps),
+                                                     C.c_void_p(stream) if stream else None))
 
     def topo_load_2d(self, aid, r_km, z_km):
         """Terrain profile z(r) for GA_GEOM_2D runs on a stratified atmosphere."""
