@@ -617,6 +617,7 @@ int rd_setup(ga_ctx *ctx, FanBufs &fb, int rd_id, const ga_prop_params *p, RdFan
     if (!rd_region(R, p->z_grnd, G)) return fail(ctx, GA_E_ARG, "atmosphere grid and topography grid have no overlap");
     const double lat_min = G.lat_min, lat_max = G.lat_max, lon_min = G.lon_min, lon_max = G.lon_max;
     M.z0 = G.z0;
+    M.zflat = cart ? 0.0 : G.z0;   // the Cartesian topo::z ignores z0 (atmo_state.3d.rngdep.cpp:45-52)
     M.z_max = G.z_max;
     M.z_bndlyr = G.z_bndlyr;
     Limits &L = A.L;
@@ -650,7 +651,7 @@ int rd_setup(ga_ctx *ctx, FanBufs &fb, int rd_id, const ga_prop_params *p, RdFan
         HIPCHK(ctx, hipStreamSynchronize(st));
         zsrc = std::max(zsrc, zg);
     } else {
-        zsrc = std::max(zsrc, M.z0);
+        zsrc = std::max(zsrc, M.zflat);   // z_src = max(z_src, topo::z) (main.infraga.3d.rngdep.cpp:308)
     }
     A.src0 = M.r0 + zsrc;
     A.freq = p->freq;

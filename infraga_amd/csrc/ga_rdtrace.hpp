@@ -40,6 +40,7 @@ struct RdMed {
     int is_topo;
     int use_blw;
     double r0, z0, z_max, z_bndlyr;
+    double zflat;       // ground height without terrain: z0 (sph), 0 (3D Cartesian)
     double tweak;
     double sbc[3];
 };
@@ -100,7 +101,9 @@ GA_HD void rd_upd(const Stage &S, const V &y, const SV &stg, const V &acc, int i
 
 // --------------------------------------------------------------------------
 // Terrain: z_g = r0 + h(lat, lon) (topo::z, dz, ddz, dddz with clamped
-// arguments); flat ground r0 + z0 without a terrain file.
+// arguments); without a terrain file flat ground at r0 + z0 in the
+// spherical geometry (atmo_state.sph.rngdep.cpp:57-66) and at z = 0 in the
+// Cartesian one, whose topo::z ignores z0 (atmo_state.3d.rngdep.cpp:45-52).
 // --------------------------------------------------------------------------
 struct Terr {
     double zg;          // r0 + height
@@ -111,7 +114,7 @@ struct Terr {
 
 GA_HD double rd_ground(const RdMed &M, double lat, double lon)
 {
-    if (!M.is_topo) return M.r0 + M.z0;
+    if (!M.is_topo) return M.r0 + M.zflat;
     double o[10];
     topo_all(M.t, lat, lon, o);
     return M.r0 + o[0];
@@ -120,7 +123,7 @@ GA_HD double rd_ground(const RdMed &M, double lat, double lon)
 GA_HD void rd_terrain(const RdMed &M, double lat, double lon, Terr &T)
 {
     if (!M.is_topo) {
-        T.zg = M.r0 + M.z0;
+        T.zg = M.r0 + M.zflat;
         T.d[0] = T.d[1] = 0.0;
         for (int i = 0; i < 2; ++i)
             for (int j = 0; j < 2; ++j) {
@@ -1230,7 +1233,10 @@ GA_HD void rd_lane_loop(const RdFanArgs &A, int lane, NextRay next_ray, V ya, V 
                             R.launch_idx = ray; R.leg = bnc; R.step = k; R.kind = GA_PT_CAUSTIC;
                             R.pos0 = n[1] * (180.0 / PI); R.pos1 = n[2] * (180.0 / PI); R.z = n[0] - M.r0;
                             R.tl = 0.0; R.absorption = -2.0 * att; R.time = tt;
-                            if (CART) rd3_path_xyz(M, n, R);
+                            if (CART) {   // caustic rows keep the raw z (main.infraga.3d.rngdep.cpp:467-471)
+                                rd3_path_xyz(M, n, R);
+                                R.z = n[2];
+                            }
                             path_put(A.path, pl, ray, R);
                         }
                         d_prev = d;
@@ -1263,7 +1269,9 @@ GA_HD void rd_lane_loop(const RdFanArgs &A, int lane, NextRay next_ray, V ya, V 
         attl = 0.0;
         bool done = false;
         if (!M.is_topo && rmax < M.z0 + A.turn_ht_min) done = true;
-        else if (brk || k < 2 || (tt / 3600.0) > L.time_max) done = true;
+        // the 3D range-dependent driver has no travel-time limit
+        // (main.infraga.3d.rngdep.cpp:498)
+        else if (brk || k < 2 || (!CART && (tt / 3600.0) > L.time_max)) done = true;
         else {
             {
                 double *r = A.raw + ((size_t)ray * (A.bounces + 1) + bnc) * RAW;

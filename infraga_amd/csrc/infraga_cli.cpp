@@ -13,8 +13,7 @@
 // Differences from the reference drivers, by design:
 //  - an unknown key is an error (exit 2) instead of an interactive prompt;
 //  - write_topo is rejected, and so are topo_file / topo_use_BLw outside the
-//    range-dependent binary; write_caustics needs calc_amp=true (the
-//    reference would read auxiliary columns it never integrated);
+//    range-dependent binary;
 //  - verbose=true prints no trace-by-trace log.
 #include <cmath>
 #include <cstdio>
@@ -148,7 +147,7 @@ void incl_loop(double theta_min, double theta_max, double theta_grnd, double ste
 // default 6.  arr receives the arrivals in launch order.
 int fan_with_paths(ga_ctx *ctx, int aid, const ga_prop_params &p, const std::vector<double> &launches,
                    bool write_rays, bool write_caustics, const std::string &id, bool sph,
-                   std::vector<ga_arrival> &arr, int &n_out, ga_run_stats &st)
+                   std::vector<ga_arrival> &arr, int &n_out, ga_run_stats &st, bool rd3 = false)
 {
     const int n = (int)(launches.size() / 2);
     const int slots = p.bounces + 1;
@@ -203,7 +202,11 @@ int fan_with_paths(ga_ctx *ctx, int aid, const ga_prop_params &p, const std::vec
                     raypath << P.pos0 << '\t' << P.pos1 << '\t' << P.z << '\t' << P.tl << '\t' << P.absorption
                             << '\t' << P.time << '\n';
                 } else if (P.kind == GA_PT_CAUSTIC && write_caustics && P.leg >= 0 && P.leg < (int)caus.size()) {
-                    caus[P.leg] << P.pos0 << '\t' << P.pos1 << '\t' << P.z << '\t' << P.time << '\n';
+                    // the 3D range-dependent writer puts a 0.0 column before the time
+                    // (main.infraga.3d.rngdep.cpp:467-471)
+                    caus[P.leg] << P.pos0 << '\t' << P.pos1 << '\t' << P.z << '\t';
+                    if (rd3) caus[P.leg] << 0.0 << '\t';
+                    caus[P.leg] << P.time << '\n';
                 }
             }
             if (write_rays) raypath << '\n';
@@ -280,10 +283,7 @@ int run_prop(int argc, char **argv)
         }
     }
 
-    if (write_caustics && !p.calc_amp) {
-        std::cerr << GA_CLI_NAME << ": write_caustics=true needs calc_amp=true\n";
-        return 2;
-    }
+    if (write_caustics) p.calc_amp = 1;   // caustics need the Jacobian (main.infraga.sph.cpp:315, 3d.cpp:308)
     ga_ctx *ctx = nullptr;
     if (ga_ctx_create(&ctx, 0) != GA_OK) return die(nullptr, "no usable GPU");
     int aid = 0;
@@ -421,7 +421,7 @@ int run_prop_rd(int argc, char **argv)
         }
         else if (key2(a, "max_alt=", "alt_max=", v)) p.alt_max = std::atof(v);
         else if (key2(a, "max_rng=", "rng_max=", v)) p.rng_max = std::atof(v);
-        else if (key2(a, "max_time=", "time_max=", v)) p.time_max = std::atof(v);
+        else if (!cart && key2(a, "max_time=", "time_max=", v)) p.time_max = std::atof(v);   // no time limit in 3D (:498)
         else if (key2(a, cart ? "min_x=" : "min_lat=", cart ? "x_min=" : "lat_min=", v)) p.bounds[0] = std::atof(v);
         else if (key2(a, cart ? "max_x=" : "max_lat=", cart ? "x_max=" : "lat_max=", v)) p.bounds[1] = std::atof(v);
         else if (key2(a, cart ? "min_y=" : "min_lon=", cart ? "y_min=" : "lon_min=", v)) p.bounds[2] = std::atof(v);
@@ -449,10 +449,7 @@ int run_prop_rd(int argc, char **argv)
         }
     }
 
-    if (write_caustics && !p.calc_amp) {
-        std::cerr << GA_CLI_NAME << ": write_caustics=true needs calc_amp=true\n";
-        return 2;
-    }
+    if (write_caustics) p.calc_amp = 1;   // caustics need the Jacobian (main.infraga.sph.cpp:315, 3d.cpp:308)
     ga_ctx *ctx = nullptr;
     if (ga_ctx_create(&ctx, 0) != GA_OK) return die(nullptr, "no usable GPU");
     int rid = 0;
@@ -472,8 +469,10 @@ int run_prop_rd(int argc, char **argv)
     const double la = lat_src * deg, lo = lon_src * deg;
     double pt[3] = {la, lo, 0.0}, atmo_out[40], topo_out[10];
     if (ga_rngdep_eval(ctx, rid, pt, 1, atmo_out, topo_out) != GA_OK) return die(ctx, "terrain at the source");
-    const double zg = is_topo ? topo_out[0] : z0;
-    z_src = std::max(z_src, zg);                                                 // :315
+    // without terrain the spherical ground is z0, the Cartesian topo::z is 0
+    // (atmo_state.3d.rngdep.cpp:45-52)
+    const double zg = is_topo ? topo_out[0] : (cart ? 0.0 : z0);
+    z_src = std::max(z_src, zg);                                                 // :315, 3D :308
 
     p.bounces = bounces;
     p.freq = freq;
@@ -509,7 +508,7 @@ int run_prop_rd(int argc, char **argv)
     ga_run_stats st;
     std::string id = custom_id ? std::string(output_id) : std::string(argv[2]);
     if (write_rays || write_caustics) {
-        if (fan_with_paths(ctx, rid, p, launches, write_rays, write_caustics, id, !cart, arr, n_out, st) != GA_OK)
+        if (fan_with_paths(ctx, rid, p, launches, write_rays, write_caustics, id, !cart, arr, n_out, st, cart) != GA_OK)
             return die(ctx, "fan failed");
     } else if (ga_fan_run(ctx, rid, &p, launches.data(), n, arr.data(), (int)arr.size(), &n_out, &st) != GA_OK) {
         return die(ctx, "fan failed");
@@ -517,25 +516,34 @@ int run_prop_rd(int argc, char **argv)
 
     std::ofstream results(id + ".arrivals.dat");
     results << "# " << GA_CLI_NAME << " -prop summary:" << '\n';
-    results << "#" << '\t' << "profile: " << argv[2] << '\n';
+    if (cart) {   // main.infraga.3d.rngdep.cpp:346-354
+        results << "#" << '\t' << "profile prefix: " << argv[2] << '\n';
+        results << "#" << '\t' << "profile grid files: " << argv[3] << ", " << argv[4] << '\n';
+    } else {
+        results << "#" << '\t' << "profile: " << argv[2] << '\n';
+    }
     results << "#" << '\t' << "inclination: " << theta_min << ", " << theta_max << ", " << theta_step << '\n';
     results << "#" << '\t' << "azimuth: " << phi_min << ", " << phi_max << ", " << phi_step << '\n';
     results << "#" << '\t' << "bounces: " << bounces << '\n';
-    results << "#" << '\t' << (cart ? "source location (x, y, alt): " : "source location (lat, lon, alt): ") << lat_src << ", " << lon_src << ", " << z_src << '\n';
+    results << "#" << '\t' << (cart ? "source location: " : "source location (lat, lon, alt): ") << lat_src << ", " << lon_src << ", " << z_src << '\n';
     if (!is_topo) results << "#" << '\t' << "ground elevation: " << z0 << '\n';
     else results << "#" << '\t' << "topo file:" << topo_file << '\n';
     results << "#" << '\t' << "frequency: " << freq << '\n';
     results << "#" << '\t' << "S&B atten coeff: " << p.abs_coeff << '\n' << '\n';
     results << "# incl [deg]" << '\t' << "az [deg]" << '\t' << "n_b" << '\t' << (cart ? "x_0 [km]" : "lat_0 [deg]") << '\t'
             << (cart ? "y_0 [km]" : "lon_0 [deg]")
-            << '\t' << "time [s]" << '\t' << "cel [km/s]" << '\t' << "turning ht [km]" << '\t' << "inclination [deg]"
-            << '\t' << "back azimuth [deg]" << '\t' << "trans. coeff. [dB]" << '\t' << "absorption [dB]" << '\n';
+            << '\t' << "time [s]" << '\t' << "cel [km/s]" << '\t' << "turning ht [km]" << '\t'
+            << (cart ? "arrival incl [deg]" : "inclination [deg]") << '\t' << (cart ? "back az [deg]" : "back azimuth [deg]")
+            << '\t' << "trans. coeff. [dB]" << '\t' << "absorption [dB]" << '\n';
     int w = 0, az_i = 0;
     for (int r = 0; r < n; ++r) {
         while (w < n_out && arr[w].launch_idx == r) {
             const ga_arrival &A = arr[w++];
             results << A.incl << '\t' << A.az << '\t' << A.n_bounce;
-            results << '\t' << std::setprecision(8) << A.pos0 << '\t' << std::setprecision(8) << A.pos1;
+            // the spherical writer's setprecision(8) sticks; the 3D one prints
+            // at the stream default 6 (main.infraga.3d.rngdep.cpp:506-520)
+            if (!cart) results << std::setprecision(8);
+            results << '\t' << A.pos0 << '\t' << A.pos1;
             results << '\t' << A.time << '\t' << A.cel << '\t' << A.turn_ht << '\t' << A.incl_arr << '\t' << A.back_az;
             results << '\t' << A.tl << '\t' << A.absorption << '\n';
         }

@@ -144,11 +144,16 @@ void go_rd_centre(const go_rd *a, double *lat_deg, double *lon_deg)
 static double tx(go_rd *a, double lat) { return in_iv(lat, a->topo.x[0], a->topo.x[a->topo.nx - 1]); }
 static double ty(go_rd *a, double lon) { return in_iv(lon, a->topo.y[0], a->topo.y[a->topo.ny - 1]); }
 
+/* Without terrain the spherical ground sits at r0 + z0
+ * (atmo_state.sph.rngdep.cpp:57-66); the Cartesian topo::z returns 0.0
+ * whatever z0 is (atmo_state.3d.rngdep.cpp:45-52), so the 3D geometry
+ * reflects at z = 0 while z_max, the boundary-layer top and the
+ * turning-height test still follow z0. */
 double go_rd_topo_z(go_rd *a, double lat, double lon)
 {
     double result = a->r0;
     if (a->is_topo) result += go_s2d_f(&a->topo, tx(a, lat), ty(a, lon));
-    else result += a->z0;
+    else if (!a->cart) result += a->z0;
     return result;
 }
 double go_rd_topo_dz(go_rd *a, double lat, double lon, int n)
@@ -1550,7 +1555,8 @@ static void rd3_pt(go_path_pt *pts, long long cap, long long *n, go_rd *a, int l
         go_path_pt *P = pts + *n;
         P->launch_idx = li; P->leg = leg; P->step = m; P->kind = kind;
         P->pos0 = y[0]; P->pos1 = y[1];
-        P->z = dmax(y[2], go_rd_topo_z(a, y[0], y[1]));                  /* :456 */
+        /* raypath rows clip z at the terrain (:456), caustic rows do not (:469) */
+        P->z = kind == 0 ? dmax(y[2], go_rd_topo_z(a, y[0], y[1])) : y[2];
         P->tl = tl; P->absorption = -2.0 * att; P->time = tt;
     }
     (*n)++;
@@ -1620,7 +1626,7 @@ int go_rd3_fan(go_rd *a, const go_params *p, const go_prop *pr, const double *la
             }
             for (int m = 0; m < k; m++) zmax = dmax(zmax, rows[18 * m + 2]);      /* :490-492 */
             if (!a->is_topo && zmax < (a->z0 + pr->turn_ht_min)) break;         /* :494 */
-            if (brk || k < 2 || (tt / 3600.0) > p->time_max) break;
+            if (brk || k < 2) break;                                   /* :498, no travel-time limit */
             double *yk = rows + 18 * k;
             go_arrival A;
             memset(&A, 0, sizeof(A));

@@ -30,7 +30,6 @@ def test_usage_without_arguments():
 @pytest.mark.parametrize("args", [
     ("-prop", "x.met", "no_such_key=1"),
     ("-prop", "x.met", "write_topo=true"),
-    ("-prop", "x.met", "write_caustics=true", "calc_amp=false"),
     ("-eig_search", "x.met", "topo_file=t.dat"),
     ("-back_proj", "x.met"),
 ])

@@ -135,3 +135,51 @@ def test_sloped_plane_image_source(oracle, a, b, az):
     for row in q1[::3]:
         s = np.linalg.norm(np.array([row["pos0"], row["pos1"], row["z"]]) - Si)
         assert row["tl"] == pytest.approx(20.0 * math.log10(1.0 / s), abs=1e-6), row["step"]
+
+
+def linear_columns(z_bottom=0.5, c_a=0.33, g=0.001):
+    """Windless columns with c = c_a + g (z - z_bottom) from z_bottom up; the
+    spline holds c(z_bottom) below the profile (clamped lookups)."""
+    z = np.arange(z_bottom, 120.0 + 1e-9, 0.5)
+    c = c_a + g * (z - z_bottom)
+    rho = 1.2e-3 * np.exp(-z / 7.0)
+    return columns(z, c, 0.0 * z, 0.0 * z, rho), c_a, g
+
+
+def test_ground_at_zero_without_terrain(oracle):
+    """Without terrain the Cartesian topo::z is 0.0 whatever z_grnd is
+    (atmo_state.3d.rngdep.cpp:45-52): the ground test, the reflection and
+    the source clamp use z = 0 while the shadow-zone test still uses z0.
+    With c = c_a + g z, no wind, z_grnd = 1.0 and the source at z = 0, a ray
+    launched at theta is a circle of radius c_a / (g cos theta) through the
+    source and each hop covers 2 (c_a / g) tan(theta) at z = 0; reflecting
+    at z0 = 1 km would shorten each hop by 2 / tan(theta) km or more."""
+    cols, c_a, g = linear_columns(z_bottom=0.0)
+    m = Rd3Medium(oracle, *cols)
+    m.set_ground(1.0)
+    thetas = (8.0, 15.0, 25.0)
+    L = [(th, 90.0) for th in thetas]
+    arr, _, _ = m.fan3(L, m.params(False), bounces=2, src=(0.0, 0.0, 0.0))
+    assert len(arr) == 3 * len(thetas)
+    for a in arr:
+        th = math.radians(a["incl_launch"])
+        hop = 2.0 * (c_a / g) * math.tan(th)
+        rng = math.hypot(a["pos0"], a["pos1"])
+        # the arrival is the first RK4 row below z = 0, within a few ds_min of the intercept
+        assert abs(rng - (a["n_bounce"] + 1) * hop) < 0.01, (a["incl_launch"], a["n_bounce"], rng, hop)
+        apex = (c_a / g) * (1.0 / math.cos(th) - 1.0)
+        assert abs(a["turn_ht"] - apex) < 1e-3
+
+
+def test_no_travel_time_limit(oracle):
+    """infraga-3d-rngdep stops rays on region, ground and step limits only
+    (main.infraga.3d.rngdep.cpp:494-500): a travel-time limit far below the
+    arrival times changes nothing."""
+    cols, _, _ = linear_columns()
+    m = Rd3Medium(oracle, *cols)
+    L = [(th, 90.0) for th in (8.0, 15.0)]
+    p = m.params(False)
+    a1, _, s1 = m.fan3(L, p, bounces=2, src=(0.0, 0.0, 0.0))
+    p.time_max = 1.0e-3          # hours
+    a2, _, s2 = m.fan3(L, p, bounces=2, src=(0.0, 0.0, 0.0))
+    assert s1 == s2 and len(a1) == len(a2) == 6 and (a2["time"] > 3.6).all()

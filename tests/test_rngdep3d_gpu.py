@@ -115,3 +115,71 @@ def test_cli_3d_rngdep_prop(tmp_path, oracle):
         exp = [a["incl_launch"], a["az_launch"], a["n_bounce"], a["pos0"], a["pos1"], a["time"], a["cel"],
                a["turn_ht"], a["incl_arr"], a["back_az"], 0.0, a["absorption"]]
         np.testing.assert_allclose(row, exp, rtol=1e-5, atol=1e-5)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("z_bottom", [0.0, 0.5])
+def test_ground_at_zero_with_z_grnd(engine, oracle, z_bottom):
+    """Without terrain the 3D range-dependent ground is z = 0 whatever
+    z_grnd is (atmo_state.3d.rngdep.cpp:45-52); z_grnd = 1.0 moves only the
+    shadow-zone test and the boundary-layer top.  Linear c from z_bottom
+    (held below the profile), GPU against the oracle at the parity bar."""
+    from ga_oracle_py import Rd3Medium
+    from test_oracle_rd3 import linear_columns
+    cols, _, _ = linear_columns(z_bottom=z_bottom)
+    L = np.array([[th, az] for az in (90.0, 200.0) for th in (8.0, 15.0, 25.0)])
+    rid = engine.rngdep_load_xy(*cols)
+    try:
+        for amp in (1, 0):
+            p = prop_params(GA_GEOM_3D_RNGDEP, calc_amp=amp, bounces=2, z_grnd=1.0, src=(0.0, 0.0, 0.0))
+            arr, st = engine.fan(rid, p, L)
+            m = Rd3Medium(oracle, *cols)
+            m.set_ground(1.0)
+            ref, _, steps = m.fan3(L, m.params(bool(amp)), bounces=2, src=(0.0, 0.0, 0.0))
+            assert len(ref) == 18 and st.ray_steps == steps
+            fails, summ = compare(arr, ref, False, src=(0.0, 0.0))
+            assert not fails, (fails, summ)
+    finally:
+        engine.rngdep_release(rid)
+
+
+@pytest.mark.gpu
+def test_no_travel_time_limit(engine):
+    """The 3D range-dependent geometry ignores time_max (main.infraga.3d.rngdep.cpp:494-500)."""
+    from test_oracle_rd3 import linear_columns
+    cols, _, _ = linear_columns(z_bottom=0.0)
+    L = np.array([[th, 90.0] for th in (8.0, 15.0)])
+    rid = engine.rngdep_load_xy(*cols)
+    try:
+        a1, s1 = engine.fan(rid, prop_params(GA_GEOM_3D_RNGDEP, calc_amp=0, bounces=2, src=(0.0, 0.0, 0.0)), L)
+        a2, s2 = engine.fan(rid, prop_params(GA_GEOM_3D_RNGDEP, calc_amp=0, bounces=2, src=(0.0, 0.0, 0.0),
+                                             time_max=1.0e-3), L)
+        assert len(a1) == len(a2) == 6 and s1.ray_steps == s2.ray_steps and (a2["time"] > 3.6).all()
+    finally:
+        engine.rngdep_release(rid)
+
+
+@pytest.mark.gpu
+def test_cli_3d_rngdep_caustics_force_amplitudes(tmp_path):
+    """write_caustics=true turns amplitudes on (main.infraga.3d.rngdep.cpp:310)
+    even though the 3D binary reads calc_amp=... as false; caustic rows are
+    x, y, z, 0.0, time (:467-471) and the arrivals header is the 3D
+    range-dependent one (:346-368)."""
+    import subprocess
+    from ga_oracle_py import write_rd_files
+    units, topo, _ = workload()
+    x, y, z, c, u, v, rho = units
+    pre, xs, ys, tp = write_rd_files(str(tmp_path), x, y, z, c * 1000.0, u * 1000.0, v * 1000.0, rho, topo=topo)
+    exe = os.path.join(os.path.dirname(HERE), "infraga_amd", "bin", "infraga-3d-rngdep")
+    r = subprocess.run([exe, "-prop", pre, xs, ys, f"topo_file={tp}", "prof_format=zcuvd", "write_rays=false",
+                        "write_caustics=true", "calc_amp=false", "incl_min=4", "incl_max=30", "incl_step=2",
+                        "azimuth=45", "bounces=1", "src_alt=3.0"],
+                       capture_output=True, text=True, cwd=tmp_path, timeout=120)
+    assert r.returncode == 0, r.stderr
+    head = [l for l in open(pre + ".arrivals.dat") if l.startswith("#")]
+    assert any("profile prefix: " in l for l in head) and any("profile grid files: " in l for l in head)
+    assert any("arrival incl [deg]" in l and "back az [deg]" in l for l in head)
+    rows = [[float(t) for t in l.split("\t")] for l in open(pre + ".arrivals.dat") if l.strip() and not l.startswith("#")]
+    assert len(rows) > 0 and any(row[10] != 0.0 for row in rows)     # TL computed
+    caus = [l.split("\t") for l in open(pre + ".caustics-0.dat") if l.strip() and not l.startswith("#")]
+    assert len(caus) > 0 and all(len(cr) == 5 and float(cr[3]) == 0.0 for cr in caus)
