@@ -37,6 +37,7 @@
 #ifndef GA_HD
 #define GA_HD __device__ __forceinline__
 #endif
+#include "ga_math.hpp"
 
 namespace ga {
 
@@ -233,7 +234,7 @@ template <int G> GA_HD double hcoord(const double *y)
 GA_HD double bl_sc(const Medium &M, double z, double h)
 {
     if (z < M.z_bndlyr) {
-        double X = exp(-(h - M.zg) * 10.0);
+        double X = ga_exp(-(h - M.zg) * 10.0);
         return (1.0 - X) / (1.0 + X);
     }
     return 1.0;
@@ -263,7 +264,7 @@ GA_HD void medium_rhs(const Medium &M, double z, double h, int &cur, Atm &m)
         herm_fd(s, 5, v0, dv0);
         m.ddc = 0.0;
     }
-    double X = exp(-(h - M.zg) * 10.0);
+    double X = ga_exp(-(h - M.zg) * 10.0);
     double opx = 1.0 + X;
     double iop = ga_rcp(opx);
     double iop2 = iop * iop;
@@ -473,7 +474,7 @@ template <int G>
 GA_HD double set_ds(const Medium &M, const Limits &L, const double *y)
 {
     double ds = L.ds_max;
-    if (zcoord<G>(M, y) < M.z_bndlyr) ds -= (L.ds_max - L.ds_min) * exp(-(hcoord<G>(y) - M.zg) * 5.0);
+    if (zcoord<G>(M, y) < M.z_bndlyr) ds -= (L.ds_max - L.ds_min) * ga_exp(-(hcoord<G>(y) - M.zg) * 5.0);
     return ds;
 }
 
@@ -516,22 +517,24 @@ GA_HD double sb_alpha(double z, double c, double rho, double freq, double tweak,
     const double z2 = z * z, z3 = z2 * z, z4 = z3 * z, z5 = z4 * z;
     double X0, X1, X3, X4, X5, X6;
     const double X2 = sbc[2];
-    X0 = (z > 90.0) ? exp10(49.296 - (1.5524 * z) + (1.8714E-2 * z2) - (1.1069E-4 * z3) + (3.199E-7 * z4) - (3.6211E-10 * z5))
+    X0 = (z > 90.0) ? ga_exp10(49.296 - (1.5524 * z) + (1.8714E-2 * z2) - (1.1069E-4 * z3) + (3.199E-7 * z4) - (3.6211E-10 * z5))
                     : sbc[0];
-    X1 = (z > 76.0) ? exp10((1.3972E-1) - (5.6269E-3 * z) + (3.9407E-5 * z2) - (1.0737E-7 * z3))
+    X1 = (z > 76.0) ? ga_exp10((1.3972E-1) - (5.6269E-3 * z) + (3.9407E-5 * z2) - (1.0737E-7 * z3))
                     : sbc[1];
-    X3 = (z > 80.0) ? exp10(-4.234 - (3.0975E-2 * z))
-                    : exp10(-19.027 + (1.3093 * z) - (4.6496E-2 * z2) + (7.8543E-4 * z3) - (6.5169E-6 * z4) + (2.1343E-8 * z5));
-    X4 = (z > 95.0) ? exp10(-3.2456 + (4.6642E-2 * z) - (2.6894E-4 * z2) + (5.264E-7 * z3))
-                    : exp10(-11.195 + (1.5408E-1 * z) - (1.4348E-3 * z2) + (1.0166E-5 * z3));
-    if (SPH) X5 = exp10(-53.746 + (1.5439 * z) - (1.8824E-2 * z2) + (1.1587E-4 * z3) - (3.5399E-7 * z4) + (4.2609E-10 * z5));
-    else     X5 = exp10(-53.746 + (1.5439 * z) - (1.8824e-2 * z2) + (1.1587e-4 * z3) - (3.5399e-7 * z2) + (4.2609e-10 * z5));
+    // one exp10 of the selected exponent per species: lanes of a wave on
+    // either side of a branch height then share the call
+    X3 = ga_exp10((z > 80.0) ? (-4.234 - (3.0975E-2 * z))
+                          : (-19.027 + (1.3093 * z) - (4.6496E-2 * z2) + (7.8543E-4 * z3) - (6.5169E-6 * z4) + (2.1343E-8 * z5)));
+    X4 = ga_exp10((z > 95.0) ? (-3.2456 + (4.6642E-2 * z) - (2.6894E-4 * z2) + (5.264E-7 * z3))
+                          : (-11.195 + (1.5408E-1 * z) - (1.4348E-3 * z2) + (1.0166E-5 * z3)));
+    if (SPH) X5 = ga_exp10(-53.746 + (1.5439 * z) - (1.8824E-2 * z2) + (1.1587E-4 * z3) - (3.5399E-7 * z4) + (4.2609E-10 * z5));
+    else     X5 = ga_exp10(-53.746 + (1.5439 * z) - (1.8824e-2 * z2) + (1.1587e-4 * z3) - (3.5399e-7 * z2) + (4.2609e-10 * z5));
     // the reference's "z > 100" branch is unreachable behind "z > 30"
-    X6 = (z > 30.0) ? exp10(-4.2563 + (7.6245E-2 * z) - (2.1824E-3 * z2) - (2.3010E-6 * z3) + (2.4265E-7 * z4) - (1.2500E-09 * z5))
-                    : exp10(-1.7491 + (4.4986E-2 * z) - (6.8549E-2 * z2) + (5.4639E-3 * z3) - (1.5539E-4 * z4) + (1.5063E-06 * z5));
+    X6 = ga_exp10((z > 30.0) ? (-4.2563 + (7.6245E-2 * z) - (2.1824E-3 * z2) - (2.3010E-6 * z3) + (2.4265E-7 * z4) - (1.2500E-09 * z5))
+                          : (-1.7491 + (4.4986E-2 * z) - (6.8549E-2 * z2) + (5.4639E-3 * z3) - (1.5539E-4 * z4) + (1.5063E-06 * z5)));
     const double X_ON = (X0 + X1) / 0.9903;
     const double Tm13 = rcbrt(T_z);
-    const double Z_rot = 1.0 / ((X1 / (63.3 * exp(-16.7 * Tm13))) + (X0 / (54.1 * exp(-17.3 * Tm13))));
+    const double Z_rot = 1.0 / ((X1 / (63.3 * ga_exp(-16.7 * Tm13))) + (X0 / (54.1 * ga_exp(-17.3 * Tm13))));
     const double sigma = 5.0 / sqrt(21.0);
     const double nn = (4.0 / 5.0) * sqrt(3.0 / 7.0) * Z_rot;
     const double chi = 3.0 * nn * nu / 4.0;
@@ -553,19 +556,19 @@ GA_HD double sb_alpha(double z, double c, double rho, double freq, double tweak,
         a_rot = alpha_2 * beta_1 * X_ON;
     }
     const double Tr = rcbrt(T_z / T_o) - 1.0;
-    const double A1 = (X0 + X1) * 24.0 * exp(-9.16 * Tr);
+    const double A1 = (X0 + X1) * 24.0 * ga_exp(-9.16 * Tr);
     const double A2 = (X4 + X5) * 2400.0;
-    const double B = 40400.0 * exp(10.0 * Tr);
-    const double C = 0.02 * exp(-11.2 * Tr);
-    const double D = 0.391 * exp(8.41 * Tr);
-    const double E = 9.0 * exp(-19.9 * Tr);
+    const double B = 40400.0 * ga_exp(10.0 * Tr);
+    const double C = 0.02 * ga_exp(-11.2 * Tr);
+    const double D = 0.391 * ga_exp(8.41 * Tr);
+    const double E = 9.0 * ga_exp(-19.9 * Tr);
     const double F = 60000.0;
-    const double G = 28000.0 * exp(-4.17 * Tr);
-    const double H = 22000.0 * exp(-7.68 * Tr);
-    const double I = 15100.0 * exp(-10.4 * Tr);
-    const double J = 11500.0 * exp(-9.17 * Tr);
-    const double K = (8.48E08) * exp(9.17 * Tr);
-    const double L = exp(-7.72 * Tr);
+    const double G = 28000.0 * ga_exp(-4.17 * Tr);
+    const double H = 22000.0 * ga_exp(-7.68 * Tr);
+    const double I = 15100.0 * ga_exp(-10.4 * Tr);
+    const double J = 11500.0 * ga_exp(-9.17 * Tr);
+    const double K = (8.48E08) * ga_exp(9.17 * Tr);
+    const double L = ga_exp(-7.72 * Tr);
     const double ZZ = H * X2 + I * (X0 + 0.5 * X4) + J * (X1 + 0.5 * X5) + K * (X6 + X3);
     const double hu = 100.0 * (X3 + X6);
     const double pf = (P_z / P_o) * (mu_o / mu);
@@ -579,7 +582,7 @@ GA_HD double sb_alpha(double z, double c, double rho, double freq, double tweak,
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
         const double tt = Th[m] * iT;
-        const double e = exp(-tt);
+        const double e = ga_exp(-tt);
         const double ome = 1.0 - e;
         const double C_R = (tt * tt * e) / (ome * ome);
         const double A_max = (Xm[m] * (PI / 2) * C_R) / (Cp[m] * (Cv[m] + C_R));
@@ -604,7 +607,7 @@ GA_HD void seg_book(const Medium &M, const double *y0, const double *y1, double 
         double dr = y1[0] - y0[0], dt_ = y1[1] - y0[1], dp = y1[2] - y0[2];
         if (dp > PI) dp -= 2.0 * PI;
         else if (dp < -PI) dp += 2.0 * PI;
-        double a = m0 * dt_, b = m0 * cos(m1) * dp;
+        double a = m0 * dt_, b = m0 * ga_cos(m1) * dp;
         ds = sqrt(dr * dr + a * a + b * b);
     } else {
         double dx = y1[0] - y0[0], dy = y1[1] - y0[1], dz = y1[2] - y0[2];

@@ -86,6 +86,71 @@ __global__ void __launch_bounds__(256) arrival_kernel(FanArgs A)
 
 __global__ void reset_counter(unsigned int *q) { *q = 0u; }
 
+// Ring control words: head and tail of the pending records, and the append
+// counter on a cache line of its own (the slice kernel's atomics).
+constexpr int CTL_HEAD = 0, CTL_TAIL = 1, CTL_APPEND = 32, CTL_WORDS = 64;
+
+// Every ray of the fan as a record that has not started, head 0, tail n.
+__global__ void ring_init(RayRing R, int n)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        R.iv[i] = (int)i;
+        R.iv[R.cap + i] = -1;
+    }
+    if (i == 0) {
+        R.ctl[CTL_HEAD] = 0;
+        R.ctl[CTL_TAIL] = (unsigned long long)n;
+        R.ctl[CTL_APPEND] = (unsigned long long)n;
+    }
+}
+
+// One slice: up to R.lanes records from the ring head, spread evenly over
+// the workgroups (workgroup b takes records [b * per, (b + 1) * per) of
+// the slice), each ray advanced by at most R.slice_steps RK4 steps; the
+// records of unfinished rays are appended at the tail, packed per wave.
+template <int G, int NEQ, bool LDS>
+__global__ void __launch_bounds__((trace_wg<G, NEQ>())) slice_kernel(FanArgs A, RayRing R)
+{
+    extern __shared__ double lds_tab[];
+    const unsigned long long head = R.ctl[CTL_HEAD], tail = R.ctl[CTL_TAIL];
+    const long long m = (long long)min(tail - head, (unsigned long long)R.lanes);
+    const long long per = (m + gridDim.x - 1) / gridDim.x;
+    if ((long long)blockIdx.x * per >= m) return;        // whole workgroup idle this slice
+    Medium M = A.M;
+    if constexpr (LDS) {
+        const int nd = M.n * NODE;
+        for (int i = threadIdx.x; i < nd; i += blockDim.x) lds_tab[i] = A.M.tab[i];
+        __syncthreads();
+        M.tab = lds_tab;
+    }
+    const long long j = (long long)blockIdx.x * per + threadIdx.x;
+    if ((long long)threadIdx.x >= per || j >= m) return;
+    RayRec<NEQ> r;
+    ring_load(R, (head + (unsigned long long)j) % R.cap, r);
+    const bool alive = slice_trace<G, NEQ>(A, M, r, R.slice_steps);
+    const unsigned long long act = __ballot(1), live = __ballot(alive);
+    const int lane = (int)__lane_id();
+    const int leader = __ffsll((long long)act) - 1;
+    unsigned long long base = 0;
+    if (lane == leader && live) base = atomicAdd(R.ctl + CTL_APPEND, (unsigned long long)__popcll(live));
+    base = __shfl(base, leader);
+    if (alive) {
+        const unsigned long long below = live & ((1ull << lane) - 1ull);
+        ring_store(R, (base + (unsigned long long)__popcll(below)) % R.cap, r);
+    }
+}
+
+// After a slice: the records it took leave the head, the appended ones
+// extend the tail.
+__global__ void slice_ctl(RayRing R)
+{
+    const unsigned long long head = R.ctl[CTL_HEAD], tail = R.ctl[CTL_TAIL];
+    const unsigned long long m = min(tail - head, (unsigned long long)R.lanes);
+    R.ctl[CTL_HEAD] = head + m;
+    R.ctl[CTL_TAIL] = R.ctl[CTL_APPEND];
+}
+
 // 2D fans (infraga-2d): persistent lanes, the stratified table in LDS when it fits.
 #ifndef GA_WG_2D
 #define GA_WG_2D 512
@@ -289,9 +354,33 @@ struct FanBufs {
     // (> 0) caps the persistent grid so that the lane buffer fits.
     bool fixed = false;
     size_t max_lanes = 0;
+    // rotating slice scheduler: ring records, control words, pinned readback
+    // of head / tail, a capture stream and the cached graph of one batch of
+    // slices (re-captured when the launch or its arguments change)
+    double *ring_d = nullptr;      // [(18 + RING_SC)][cap]
+    int32_t *ring_i = nullptr;     // [RING_IV][cap]
+    size_t ring_cap = 0;
+    unsigned long long *ctl = nullptr;
+    unsigned long long *h_ctl = nullptr;
+    hipStream_t cap_stream = nullptr;
+    hipGraphExec_t graph = nullptr;
+    std::vector<unsigned char> graph_key;
+    void drop_graph()
+    {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        graph = nullptr;
+        graph_key.clear();
+    }
     void release()
     {
         if (fixed) return;
+        drop_graph();
+        if (ring_d) (void)hipFree(ring_d);
+        if (ring_i) (void)hipFree(ring_i);
+        if (ctl) (void)hipFree(ctl);
+        if (h_ctl) (void)hipHostFree(h_ctl);
+        if (cap_stream) (void)hipStreamDestroy(cap_stream);
+        ring_d = nullptr; ring_i = nullptr; ring_cap = 0; ctl = nullptr; h_ctl = nullptr; cap_stream = nullptr;
         if (queue) (void)hipFree(queue);
         if (scalar) (void)hipFree(scalar);
         if (raw) (void)hipFree(raw);
@@ -486,6 +575,132 @@ int launch_fan(ga_ctx *ctx, FanBufs &fb, const FanArgs &A, hipStream_t st, bool 
     hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, st, A.queue);
     hipLaunchKernelGGL(kern, dim3(grid), dim3(W), lds, st, B);
     HIPCHK(ctx, hipGetLastError());
+    long long slots = (long long)A.n_rays * (A.bounces + 1);
+    int agrid = (int)((slots + 255) / 256);
+    hipLaunchKernelGGL((arrival_kernel<G, NEQ>), dim3(agrid), dim3(256), 0, st, A);
+    HIPCHK(ctx, hipGetLastError());
+    return GA_OK;
+}
+
+// Steps per slice and slices per captured graph of the rotating scheduler;
+// GA_FAN_SLICE (steps, 0 = one persistent launch with a ray queue),
+// GA_FAN_GRAPH (slices per graph) and GA_FAN_GRID (at most that many
+// workgroups) override them.
+int env_int(const char *name, int dflt)
+{
+    const char *v = std::getenv(name);
+    return (v && *v) ? std::atoi(v) : dflt;
+}
+constexpr int SLICE_STEPS = 64;
+constexpr int SLICES_PER_GRAPH = 32;
+
+// The fan through the rotating slice scheduler (write_rays=false sums):
+// ring of ray records, batches of slices captured once into a hipGraph and
+// replayed until the ring is empty (one readback of head and tail per
+// batch), then the arrival kernel.  The per-ray arithmetic is lane_loop's.
+template <int G, int NEQ>
+int launch_fan_sliced(ga_ctx *ctx, FanBufs &fb, const FanArgs &A, hipStream_t st, int slice_steps)
+{
+    size_t lds = A.lds ? (size_t)A.M.n * NODE * sizeof(double) : 0;
+    auto kern = A.lds ? slice_kernel<G, NEQ, true> : slice_kernel<G, NEQ, false>;
+    if (lds > 65536) HIPCHK(ctx, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int per_cu = 0;
+    constexpr int W = trace_wg<G, NEQ>();
+    HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, W, lds));
+    if (per_cu < 1) per_cu = 1;
+    int grid = per_cu * ctx->n_cu;
+    const int grid_cap = env_int("GA_FAN_GRID", 0);       // fewer workgroups (tests, tuning)
+    if (grid_cap > 0) grid = std::min(grid, grid_cap);
+    const int lanes = grid * W;
+    const int n = A.n_rays;
+    const size_t cap = (size_t)n + (size_t)lanes;
+    if (cap > fb.ring_cap) {
+        HIPCHK(ctx, hipStreamSynchronize(st));
+        fb.drop_graph();
+        if (fb.ring_d) HIPCHK(ctx, hipFree(fb.ring_d));
+        if (fb.ring_i) HIPCHK(ctx, hipFree(fb.ring_i));
+        fb.ring_d = nullptr;
+        fb.ring_i = nullptr;
+        fb.ring_cap = 0;
+        if (hipMalloc(&fb.ring_d, cap * (18 + RING_SC) * sizeof(double)) != hipSuccess ||
+            hipMalloc(&fb.ring_i, cap * RING_IV * sizeof(int32_t)) != hipSuccess)
+            return fail(ctx, GA_E_NOMEM, "ray ring of %zu records", cap);
+        fb.ring_cap = cap;
+    }
+    if (!fb.ctl && hipMalloc(&fb.ctl, CTL_WORDS * sizeof(unsigned long long)) != hipSuccess)
+        return fail(ctx, GA_E_NOMEM, "ring control words");
+    if (!fb.h_ctl && hipHostMalloc((void **)&fb.h_ctl, 2 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess) {
+        fb.h_ctl = nullptr;
+        return fail(ctx, GA_E_NOMEM, "pinned readback");
+    }
+    if (!fb.cap_stream) HIPCHK(ctx, hipStreamCreateWithFlags(&fb.cap_stream, hipStreamNonBlocking));
+    const size_t ypn = (size_t)std::max(n, 1) * 18;   // row k-1 per ray
+    if (ypn > fb.yp_cap) {
+        HIPCHK(ctx, hipStreamSynchronize(st));
+        fb.drop_graph();
+        HIPCHK(ctx, hipFree(fb.yp));
+        fb.yp = nullptr;
+        fb.yp_cap = 0;
+        if (hipMalloc(&fb.yp, ypn * sizeof(double)) != hipSuccess) return fail(ctx, GA_E_NOMEM, "row buffer");
+        fb.yp_cap = ypn;
+    }
+    FanArgs B = A;
+    B.ypbuf = fb.yp;
+    B.nlanes = lanes;
+    RayRing R;
+    R.cap = cap;
+    R.y = fb.ring_d;
+    R.sc = fb.ring_d + 18 * cap;
+    R.iv = fb.ring_i;
+    R.ctl = fb.ctl;
+    R.lanes = lanes;
+    R.slice_steps = std::max(1, slice_steps);
+    const int K = std::max(1, env_int("GA_FAN_GRAPH", SLICES_PER_GRAPH));
+
+    // one batch of K slices as a graph, re-captured when anything it bakes in changes
+    std::vector<unsigned char> key(sizeof(void *) + sizeof(int) * 3 + sizeof(size_t) + sizeof(FanArgs) + sizeof(RayRing));
+    {
+        unsigned char *k = key.data();
+        const void *kp = (const void *)kern;
+        std::memcpy(k, &kp, sizeof(void *)); k += sizeof(void *);
+        std::memcpy(k, &grid, sizeof(int)); k += sizeof(int);
+        const int w = W;
+        std::memcpy(k, &w, sizeof(int)); k += sizeof(int);
+        std::memcpy(k, &K, sizeof(int)); k += sizeof(int);
+        std::memcpy(k, &lds, sizeof(size_t)); k += sizeof(size_t);
+        std::memcpy(k, &B, sizeof(FanArgs)); k += sizeof(FanArgs);
+        std::memcpy(k, &R, sizeof(RayRing));
+    }
+    if (!fb.graph || fb.graph_key != key) {
+        fb.drop_graph();
+        hipGraph_t g = nullptr;
+        HIPCHK(ctx, hipStreamBeginCapture(fb.cap_stream, hipStreamCaptureModeThreadLocal));
+        for (int i = 0; i < K; ++i) {
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(W), lds, fb.cap_stream, B, R);
+            hipLaunchKernelGGL(slice_ctl, dim3(1), dim3(1), 0, fb.cap_stream, R);
+        }
+        hipError_t ce = hipStreamEndCapture(fb.cap_stream, &g);
+        if (ce != hipSuccess) return fail(ctx, GA_E_HIP, "graph capture of the slice batch: %s", hipGetErrorString(ce));
+        hipGraphExec_t ex = nullptr;
+        ce = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ce != hipSuccess) return fail(ctx, GA_E_HIP, "graph instantiation: %s", hipGetErrorString(ce));
+        fb.graph = ex;
+        fb.graph_key = key;
+    }
+    hipLaunchKernelGGL(ring_init, dim3((n + 255) / 256), dim3(256), 0, st, R, n);
+    HIPCHK(ctx, hipGetLastError());
+    // every batch advances each live ray by at least one step, so the loop
+    // ends; the bound guards against a ring that never drains
+    const long long max_batches = 2 + ((long long)A.max_steps * (A.bounces + 1) + (long long)K * R.slice_steps - 1) /
+                                          ((long long)K * R.slice_steps) * (((long long)n + lanes - 1) / lanes);
+    for (long long b = 0;; ++b) {
+        if (b >= max_batches) return fail(ctx, GA_E_HIP, "slice scheduler did not drain after %lld batches", b);
+        HIPCHK(ctx, hipGraphLaunch(fb.graph, st));
+        HIPCHK(ctx, hipMemcpyAsync(fb.h_ctl, fb.ctl, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+        HIPCHK(ctx, hipStreamSynchronize(st));
+        if (fb.h_ctl[0] == fb.h_ctl[1]) break;
+    }
     long long slots = (long long)A.n_rays * (A.bounces + 1);
     int agrid = (int)((slots + 255) / 256);
     hipLaunchKernelGGL((arrival_kernel<G, NEQ>), dim3(agrid), dim3(256), 0, st, A);
@@ -920,6 +1135,16 @@ int fan_enqueue(ga_ctx *ctx, FanBufs &fb, int atmo_id, const ga_prop_params *p, 
     size_t lds = (size_t)a.n * NODE * sizeof(double);
     A.lds = lds <= ctx->max_lds ? 1 : 0;
     if (n == 0) return GA_OK;
+    // the rotating slice scheduler for plain fans; the persistent ray queue
+    // for write-mode fans, one-ray-per-lane server launches and the fixed
+    // buffers of the no-allocation entry point
+    const int slice = env_int("GA_FAN_SLICE", SLICE_STEPS);
+    if (!pw && !A.one_shot && !fb.fixed && slice > 0) {
+        if (sph) return p->calc_amp ? launch_fan_sliced<GEOM_SPH, 18>(ctx, fb, A, st, slice)
+                                    : launch_fan_sliced<GEOM_SPH, 6>(ctx, fb, A, st, slice);
+        return p->calc_amp ? launch_fan_sliced<GEOM_3D, 18>(ctx, fb, A, st, slice)
+                           : launch_fan_sliced<GEOM_3D, 6>(ctx, fb, A, st, slice);
+    }
     if (sph) return p->calc_amp ? launch_fan<GEOM_SPH, 18>(ctx, fb, A, st, pw) : launch_fan<GEOM_SPH, 6>(ctx, fb, A, st, pw);
     return p->calc_amp ? launch_fan<GEOM_3D, 18>(ctx, fb, A, st, pw) : launch_fan<GEOM_3D, 6>(ctx, fb, A, st, pw);
 }

@@ -608,10 +608,15 @@ GA_HD bool rd_leaves_region(const RdFanArgs &A, const Limits &L, double *y)
     else if (y[2] < -PI) y[2] += 2.0 * PI;
     if (y[1] < -PI / 2.0 || y[1] > PI / 2.0) return true;
     if (y[0] - A.M.r0 > L.alt_max) return true;
-    double s1 = sin((A.src1 - y[1]) * 0.5);
-    double s2 = sin((A.src2 - y[2]) * 0.5);
-    double h = s1 * s1 + cos(y[1]) * cos(A.src1) * (s2 * s2);
-    if (h > L.gc_thresh) return true;
+    // exact haversine test only when the bound (d_lat^2 + d_lon^2) / 4 >= h
+    // is near the threshold (as in the stratified leaves_region)
+    const double d1 = A.src1 - y[1], d2 = A.src2 - y[2];
+    if ((d1 * d1 + d2 * d2) * (0.25 * (1.0 + 1.0e-9)) >= L.gc_thresh) {
+        double s1 = sin(d1 * 0.5);
+        double s2 = sin(d2 * 0.5);
+        double h = s1 * s1 + cos(y[1]) * cos(A.src1) * (s2 * s2);
+        if (h > L.gc_thresh) return true;
+    }
     if (y[1] < L.b0 || y[1] > L.b1) return true;
     if (y[2] < L.b2 || y[2] > L.b3) return true;
     return false;

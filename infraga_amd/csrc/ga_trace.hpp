@@ -182,10 +182,19 @@ GA_HD bool leaves_region(const FanArgs &A, const Medium &M, const Limits &L, dou
         else if (y[2] < -PI) y[2] += 2.0 * PI;
         if (y[1] < -PI / 2.0 || y[1] > PI / 2.0) return true;
         if (y[0] - M.r0 > L.alt_max) return true;
-        double s1 = sin((A.src1 - y[1]) * 0.5);
-        double s2 = sin((A.src2 - y[2]) * 0.5);
-        double h = s1 * s1 + cos(y[1]) * cos(A.src1) * (s2 * s2);
-        if (h > L.gc_thresh) return true;
+        // haversine range test gc_dist > rng_max as h > sin^2(rng_max / 2 r0).
+        // Since sin^2(x) <= x^2 and cos(lat) cos(lat_src) <= 1, h is at most
+        // (d_lat^2 + d_lon^2) / 4; while that bound stays clearly below the
+        // threshold the ray is inside and the sines need not be evaluated
+        // (the margin dwarfs the rounding of the exact form, so both forms
+        // decide alike).
+        const double d1 = A.src1 - y[1], d2 = A.src2 - y[2];
+        if ((d1 * d1 + d2 * d2) * (0.25 * (1.0 + 1.0e-9)) >= L.gc_thresh) {
+            double s1 = sin(d1 * 0.5);
+            double s2 = sin(d2 * 0.5);
+            double h = s1 * s1 + cos(y[1]) * cos(A.src1) * (s2 * s2);
+            if (h > L.gc_thresh) return true;
+        }
         if (y[1] < L.b0 || y[1] > L.b1) return true;
         if (y[2] < L.b2 || y[2] > L.b3) return true;
         return false;
@@ -509,6 +518,151 @@ GA_HD void lane_loop(const FanArgs &A, const Medium &M, int lane, NextRay next_r
             fetch = true;
         }
     }
+}
+
+
+// ---------------------------------------------------------------------------
+// Rotating slice scheduler of the fan (write_rays=false bookkeeping).
+//
+// Every ray that is not finished sits in a ring of records in HBM.  One slice
+// takes up to `lanes` records from the head of the ring, spread evenly over
+// the workgroups, advances each ray by at most `slice_steps` RK4 steps and
+// appends the records of the rays still running at the tail, packed by a
+// wavefront ballot (one atomic per wave).  Rays are therefore visited round
+// robin, every lane works on a live ray while at least `lanes` rays are
+// left, and once fewer are left the survivors occupy whole waves spread over
+// all CUs instead of a few live lanes in every wave.  The per-ray arithmetic
+// is that of lane_loop step for step: results do not depend on the slice
+// length or on the lane a ray lands in.
+// ---------------------------------------------------------------------------
+constexpr int RING_SC = 8;     // s, tt, att, ttl, attl, rmax, c0, nu0s
+constexpr int RING_IV = 6;     // ray, k (< 0: not started), bnc, nst, narr, cur
+
+struct RayRing {
+    double *y;                 // [18][cap] state row of each record
+    double *sc;                // [RING_SC][cap]
+    int32_t *iv;               // [RING_IV][cap]
+    unsigned long long *ctl;   // head, tail, append counter
+    unsigned long long cap;    // records the ring holds (>= rays + lanes)
+    int lanes;                 // records one slice takes at most
+    int slice_steps;           // RK4 steps per ray per slice
+};
+
+// One ray's record in registers.
+template <int NEQ>
+struct RayRec {
+    double y[NEQ];
+    double s, tt, att, ttl, attl, rmax, c0, nu0s;
+    int ray, k, bnc, nst, narr, cur;
+};
+
+template <int NEQ>
+GA_HD void ring_load(const RayRing &R, unsigned long long q, RayRec<NEQ> &r)
+{
+    const unsigned long long c = R.cap;
+    r.ray = R.iv[q];
+    r.k = R.iv[c + q];
+    if (r.k < 0) return;
+#pragma unroll
+    for (int i = 0; i < NEQ; ++i) r.y[i] = R.y[i * c + q];
+    r.s = R.sc[q]; r.tt = R.sc[c + q]; r.att = R.sc[2 * c + q]; r.ttl = R.sc[3 * c + q];
+    r.attl = R.sc[4 * c + q]; r.rmax = R.sc[5 * c + q]; r.c0 = R.sc[6 * c + q]; r.nu0s = R.sc[7 * c + q];
+    r.bnc = R.iv[2 * c + q]; r.nst = R.iv[3 * c + q]; r.narr = R.iv[4 * c + q]; r.cur = R.iv[5 * c + q];
+}
+
+template <int NEQ>
+GA_HD void ring_store(const RayRing &R, unsigned long long q, const RayRec<NEQ> &r)
+{
+    const unsigned long long c = R.cap;
+#pragma unroll
+    for (int i = 0; i < NEQ; ++i) R.y[i * c + q] = r.y[i];
+    R.sc[q] = r.s; R.sc[c + q] = r.tt; R.sc[2 * c + q] = r.att; R.sc[3 * c + q] = r.ttl;
+    R.sc[4 * c + q] = r.attl; R.sc[5 * c + q] = r.rmax; R.sc[6 * c + q] = r.c0; R.sc[7 * c + q] = r.nu0s;
+    R.iv[q] = r.ray; R.iv[c + q] = r.k; R.iv[2 * c + q] = r.bnc; R.iv[3 * c + q] = r.nst;
+    R.iv[4 * c + q] = r.narr; R.iv[5 * c + q] = r.cur;
+}
+
+// Advances ray record r by at most `budget` RK4 steps with the leg, arrival
+// and reflection handling of lane_loop (write_rays=false sums).  Row k-1 of
+// a ray near the ground is kept in A.ypbuf at the ray's index.  Returns
+// true while the ray is unfinished; a finished ray has its arrival and step
+// counts written.
+template <int G, int NEQ>
+GA_HD bool slice_trace(const FanArgs &A, const Medium &M, RayRec<NEQ> &r, int budget)
+{
+    const Limits L = A.L;
+    const double incl = A.launches[2 * r.ray], az = A.launches[2 * r.ray + 1];
+    const double ph = PI / 2.0 - az * (PI / 180.0), th = incl * (PI / 180.0);
+    double *ypl = A.ypbuf + (size_t)r.ray * 18;
+    if (r.k < 0) {
+        r.cur = 0;
+        init_ray_rad<G, NEQ>(A, M, th, ph, r.y, r.c0, r.nu0s, r.cur);
+        r.s = 0.0; r.tt = 0.0; r.att = 0.0; r.ttl = 0.0; r.attl = 0.0; r.rmax = A.rmax_init;
+        r.k = 0; r.bnc = 0; r.nst = 0; r.narr = 0;
+    }
+    for (int it = 0; it < budget; ++it) {
+        double y1[NEQ];
+        r.s += rk4_step<G, NEQ>(M, L, r.y, y1, r.cur);
+        ++r.k;
+        ++r.nst;
+        bool brk = false, end = false;
+        if (r.s > L.s_max) { brk = true; end = true; }
+        else if (leaves_region<G>(A, M, L, y1, 0.0)) { brk = true; end = true; }
+        else if (below_ground<G>(M, y1)) { end = true; }
+        else if (r.k >= A.max_steps) { brk = true; end = true; }
+        else if (!(y1[0] == y1[0])) { brk = true; end = true; }   // non-finite state: stop the ray
+        {
+            double dt, da;
+            seg_book<G>(M, r.y, y1, A.freq, r.cur, dt, da);
+            r.ttl += dt;
+            r.attl += da;
+        }
+        r.rmax = fmax(r.rmax, zcoord<G>(M, r.y));
+        if (!end) {
+            if (zcoord<G>(M, r.y) < L.z_keep) {
+#pragma unroll
+                for (int i = 0; i < NEQ; ++i) ypl[i] = r.y[i];
+            }
+#pragma unroll
+            for (int i = 0; i < NEQ; ++i) r.y[i] = y1[i];
+            continue;
+        }
+        // leg finished at row K = k (y1); rows K-1 = r.y, K-2 = ypl
+        r.tt += r.ttl;
+        r.att += r.attl;
+        r.ttl = 0.0;
+        r.attl = 0.0;
+        bool done = false;
+        if (r.rmax < M.z0 + A.turn_ht_min) done = true;
+        else if (brk || r.k < 2 || (G == GEOM_SPH && (r.tt / 3600.0) > L.time_max)) done = true;
+        else {
+            {
+                double *w = A.raw + ((size_t)r.ray * (A.bounces + 1) + r.bnc) * RAW;
+#pragma unroll
+                for (int i = 0; i < NEQ; ++i) w[i] = y1[i];
+                w[18] = r.tt; w[19] = r.att; w[20] = r.rmax; w[21] = th; w[22] = ph; w[23] = r.c0; w[24] = r.nu0s;
+            }
+            ++r.narr;
+            if (r.bnc == A.bounces) done = true;
+            else {
+                double yr[NEQ], yp[NEQ];
+#pragma unroll
+                for (int i = 0; i < NEQ; ++i) yp[i] = ypl[i];
+                reflect<G, NEQ>(M, yp, r.y, yr, r.c0, r.cur);
+#pragma unroll
+                for (int i = 0; i < NEQ; ++i) r.y[i] = yr[i];
+                ++r.bnc;
+                r.s = 0.0;
+                r.k = 0;
+            }
+        }
+        if (done) {
+            A.counts[r.ray] = r.narr;
+            A.steps[r.ray] = r.nst;
+            return false;
+        }
+    }
+    return true;
 }
 
 }  // namespace ga
