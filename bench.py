@@ -48,10 +48,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 F_STEP = 5690.9          # algorithmic FP64 flops per ray-step (oracle/opcount.cpp, 400-ray sample)
-# HBM bytes per fan_kernel dispatch on this workload, from rocprofv3 --pmc
-# FETCH_SIZE / WRITE_SIZE passes scaled by a 256 MiB copy (factors 2.00 and
-# 1.00): 65.4 MB read + 655.0 MB written (profiles/fan_kernel_sph162k_pmc.md)
-TRAFFIC_BYTES = 65.4e6 + 655.0e6
+# HBM bytes of one pass (all slice_kernel dispatches) on this workload, from
+# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs scaled by a 256 MiB copy
+# (factors 2.00 and 1.00): 5.35 GB read + 20.34 GB written, mostly the ray
+# records' round trip through the ring (232 B each way per ray and slice)
+TRAFFIC_BYTES = 10443239.75 / 2.00 * 1024 + 19866050.0 * 1024
 TRANS_STEP = 167.8       # transcendental calls per ray-step (same count, not included in F_STEP)
 PEAK_FP64 = 78.6e12      # MI355X FP64 vector / matrix peak
 METRIC = "ray-steps/s on 162k-ray sph fan (FP64)"
@@ -64,44 +65,117 @@ def workload():
     return units, L
 
 
-def cpu_baseline(units, L, target_s=12.0):
-    """The oracle restatement (one host core) on a bounded, evenly strided
-    sample of the same launch list."""
+def host_cpu():
+    """Cores this process may use (capped at the 16-core share of a one-GPU
+    box), nproc, CPU model and sockets of the host."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    model, sockets = "unknown", set()
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name") and model == "unknown":
+                model = line.split(":", 1)[1].strip()
+            elif line.startswith("physical id"):
+                sockets.add(line.split(":", 1)[1].strip())
+    except OSError:
+        pass
+    return {"cores": max(1, min(avail, 16)), "nproc": os.cpu_count(), "affinity": avail, "cpu_model": model,
+            "sockets": max(1, len(sockets))}
+
+
+def _pool_map(fn, items, cores):
+    """fn over items on `cores` worker processes (spawned: this process holds
+    the GPU); returns the results and the wall time of the compute phase
+    (first worker start to last worker end)."""
+    import multiprocessing as mp
+    with mp.get_context("spawn").Pool(cores) as pool:
+        res = pool.map(fn, items, chunksize=1)
+    t_start = min(r["t0"] for r in res)
+    t_end = max(r["t1"] for r in res)
+    return res, t_end - t_start
+
+
+def _fan_worker(job):
+    """Oracle restatement of the spherical fan on a chunk of the launch list."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from ga_oracle_py import GEOM_SPH, Medium, Oracle
+    units, idx, L = job
     m = Medium(Oracle(), GEOM_SPH, *units)
     p = m.params(True)
-    n = 24
-    total_steps, total_t, used = 0, 0.0, 0
-    idx = list(range(0, len(L), max(1, len(L) // 480)))
-    while used < len(idx) and total_t < target_s:
-        sel = L[idx[used: used + n]]
-        t0 = time.perf_counter()
-        _, steps = m.fan(sel, p, bounces=5, src=(30.0, 0.0, 0.0))
-        total_t += time.perf_counter() - t0
-        total_steps += steps
-        used += len(sel)
-    return {"value": total_steps / total_t, "unit": "ray-steps/s", "cores": 1, "kind": "port",
-            "sample": f"{used} rays (every {max(1, len(L) // 480)}th of the 162k launch list), "
-                      f"{total_steps} ray-steps in {total_t:.1f} s, oracle restatement, 1 thread"}
+    t0 = time.time()
+    arr, steps = m.fan(L, p, bounces=5, src=(30.0, 0.0, 0.0))
+    t1 = time.time()
+    arr = arr.copy()
+    arr["launch_idx"] = np.asarray(idx)[arr["launch_idx"]]
+    return {"arr": arr, "steps": steps, "t0": t0, "t1": t1}
 
 
-def eig_cpu_baseline(pair=6):
-    """The oracle's serial search (one host core) for one pair of the batch."""
+def cpu_baseline(units, L, cores, per_core=480):
+    """The oracle restatement on all host cores (one process per core) over
+    an evenly strided sample of the same launch list (per_core rays per
+    core, about 12 s of CPU each), beside the single-core rate.  Returns the
+    baseline and the oracle's arrivals of the sampled rays."""
+    n = per_core * cores
+    stride = max(1, len(L) // n)
+    idx = np.arange(0, len(L), stride)[:n]
+    chunks = [(units, idx[k::cores], L[idx[k::cores]]) for k in range(cores)]
+    res, wall = _pool_map(_fan_worker, chunks, cores)
+    steps = sum(r["steps"] for r in res)
+    per_core_rates = [r["steps"] / (r["t1"] - r["t0"]) for r in res]
+    arr = np.concatenate([r["arr"] for r in res])
+    base = {"value": steps / wall, "unit": "ray-steps/s", "cores": cores, "kind": "port",
+            "per_core": float(np.mean(per_core_rates)),
+            "sample": f"{len(idx)} rays (every {stride}th of the 162k launch list) on {cores} processes, "
+                      f"{steps} ray-steps in {wall:.1f} s wall; oracle restatement, one thread per process"}
+    return base, idx, arr
+
+
+def sample_parity(gpu_rows, ref_rows, geom_sph, src):
+    """GPU rows of the sampled rays against the oracle's at the parity bar
+    (keys exact; range, time, turning height 1e-6 relative; TL, absorption
+    0.01 dB)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from parity import compare
+    fails, summ = compare(gpu_rows, ref_rows, geom_sph, src=src)
+    worst = {k: float(v) for k, v in summ["worst"].items()}
+    return {"ok": not fails, "arrivals_gpu": int(len(gpu_rows)), "arrivals_oracle": int(len(ref_rows)),
+            "key_mismatches": len(fails.get("missing", [])) + len(fails.get("extra", [])), "worst": worst,
+            "failures": {k: str(v)[:200] for k, v in fails.items()}}
+
+
+def _eig_worker(i):
+    """The oracle's serial search for pair i (one process)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from ga_oracle_py import GEOM_3D, Medium, Oracle
     from infraga_amd import synth
     m = Medium(Oracle(), GEOM_3D, *synth.to_engine_units(*synth.base_profile()))
     R = synth.eig_pairs()
-    t0 = time.perf_counter()
-    arr, steps = m.eig_search(m.params(True), (0.0, 0.0, 0.0), R[pair], bnc_min=0, bnc_max=3)
-    dt = time.perf_counter() - t0
-    return {"value": dt, "unit": "s per receiver", "cores": 1, "kind": "port",
-            "sample": f"pair {pair} ({float(np.hypot(*R[pair])):.0f} km), bounces 0..3: {len(arr)} eigenrays, "
-                      f"{steps} ray-steps in {dt:.1f} s, oracle restatement, 1 thread"}
+    t0 = time.time()
+    arr, steps = m.eig_search(m.params(True), (0.0, 0.0, 0.0), R[i], bnc_min=0, bnc_max=3)
+    t1 = time.time()
+    arr = arr.copy()
+    arr["launch_idx"] = i
+    return {"arr": arr, "steps": steps, "t0": t0, "t1": t1}
 
 
-def eig_leg(eng, with_cpu):
+def eig_cpu_baseline(cores):
+    """The oracle's serial search on all host cores, one pair per process
+    (pairs 0, 32, 64, ... of the batch)."""
+    pairs = list(range(0, 512, max(1, 512 // cores)))[:cores]
+    res, wall = _pool_map(_eig_worker, pairs, cores)
+    per_pair = [r["t1"] - r["t0"] for r in res]
+    arr = np.concatenate([r["arr"] for r in res])
+    return {"value": len(pairs) / wall, "unit": "pairs/s", "cores": cores, "kind": "port",
+            "s_per_pair_mean": float(np.mean(per_pair)), "s_per_pair_max": float(np.max(per_pair)),
+            "eigenrays": int(len(arr)),
+            "sample": f"pairs {pairs[0]}..{pairs[-1]} step {pairs[1] - pairs[0] if len(pairs) > 1 else 0} "
+                      f"({len(pairs)} pairs, bounces 0..3) on {cores} processes, {wall:.1f} s wall; "
+                      f"oracle restatement"}, pairs, arr
+
+
+def eig_leg(eng, with_cpu, cores=1):
     from infraga_amd import synth
     from infraga_amd.engine import GA_GEOM_3D, eig_params, prop_params
     aid = eng.load_atmo(*synth.to_engine_units(*synth.base_profile()))
@@ -119,7 +193,22 @@ def eig_leg(eng, with_cpu):
            "ray_steps": int(st.ray_steps), "traces": int(st.n_rays), "kernel_launches": int(st.n_launches),
            "ray_steps_per_s": st.ray_steps / wall}
     if with_cpu:
-        out["cpu_baseline"] = eig_cpu_baseline()
+        base, pairs, ref = eig_cpu_baseline(cores)
+        out["cpu_baseline"] = base
+        # per (pair, bounces) eigenray counts of the GPU batch against the
+        # oracle's serial search on the sampled pairs
+        sel = arr[np.isin(arr["launch_idx"], pairs)]
+        cg, co = {}, {}
+        for r in sel:
+            k = (int(r["launch_idx"]), int(r["n_bounce"]))
+            cg[k] = cg.get(k, 0) + 1
+        for r in ref:
+            k = (int(r["launch_idx"]), int(r["n_bounce"]))
+            co[k] = co.get(k, 0) + 1
+        keys = set(cg) | set(co)
+        out["sample_parity"] = {"pairs": len(pairs), "groups": len(keys),
+                                "groups_count_equal": sum(1 for k in keys if cg.get(k, 0) == co.get(k, 0)),
+                                "count_differences": sorted([list(k) for k in keys if cg.get(k, 0) != co.get(k, 0)])}
     return out
 
 
@@ -131,27 +220,43 @@ def rngdep_workload():
     return units, topo, L
 
 
-def rngdep_cpu_baseline(units, topo, L, target_s=12.0):
-    """The oracle's range-dependent fan (one host core) on evenly strided
-    rays of the same launch list until target_s of CPU time is spent."""
+def _rd_worker(job):
+    """Oracle range-dependent fan (spherical or 3D) on a chunk of rays."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    from ga_oracle_py import Oracle, RdMedium
-    m = RdMedium(Oracle(), *units)
+    from ga_oracle_py import Oracle, Rd3Medium, RdMedium
+    cart, units, topo, idx, L = job
+    m = (Rd3Medium if cart else RdMedium)(Oracle(), *units)
     m.set_topo(*topo)
     p = m.params(True)
-    steps, n, t0 = 0, 0, time.process_time()
-    for i in range(0, len(L), 4801):
-        _, st = m.fan(L[i:i + 1], p, bounces=4, src=None)
-        steps += st
-        n += 1
-        if time.process_time() - t0 > target_s:
-            break
-    dt = time.process_time() - t0
-    return {"value": steps / dt, "unit": "ray-steps/s", "cores": 1, "kind": "port",
-            "sample": f"{n} rays (every 4801st launch) of the 72k range-dependent fan, {steps} ray-steps, {dt:.1f} s"}
+    t0 = time.time()
+    if cart:
+        arr, _, steps = m.fan3(L, p, bounces=4)
+    else:
+        arr, steps = m.fan(L, p, bounces=4, src=None)
+    t1 = time.time()
+    arr = arr.copy()
+    arr["launch_idx"] = np.asarray(idx)[arr["launch_idx"]]
+    return {"arr": arr, "steps": steps, "t0": t0, "t1": t1}
 
 
-def rngdep_leg(eng, with_cpu):
+def rd_cpu_baseline(cart, units, topo, L, cores, per_core=6):
+    """The oracle's range-dependent fan on all host cores over evenly strided
+    rays of the same launch list (per_core rays per core, about 10 s each)."""
+    n = per_core * cores
+    stride = max(1, len(L) // n)
+    idx = np.arange(0, len(L), stride)[:n]
+    chunks = [(cart, units, topo, idx[k::cores], L[idx[k::cores]]) for k in range(cores)]
+    res, wall = _pool_map(_rd_worker, chunks, cores)
+    steps = sum(r["steps"] for r in res)
+    arr = np.concatenate([r["arr"] for r in res])
+    per_core_rates = [r["steps"] / (r["t1"] - r["t0"]) for r in res]
+    return {"value": steps / wall, "unit": "ray-steps/s", "cores": cores, "kind": "port",
+            "per_core": float(np.mean(per_core_rates)),
+            "sample": f"{len(idx)} rays (every {stride}th launch) on {cores} processes, {steps} ray-steps in "
+                      f"{wall:.1f} s wall; oracle restatement"}, idx, arr
+
+
+def rngdep_leg(eng, with_cpu, cores=1):
     from infraga_amd.engine import GA_GEOM_SPH_RNGDEP, prop_params
     units, topo, L = rngdep_workload()
     rid = eng.rngdep_load(*units)
@@ -168,11 +273,15 @@ def rngdep_leg(eng, with_cpu):
            "kernel_ms": st.kernel_ms, "wall_ms": wall * 1e3, "ray_steps_per_s": st.ray_steps / wall,
            "reflections": int((arr["n_bounce"] > 0).sum())}
     if with_cpu:
-        out["cpu_baseline"] = rngdep_cpu_baseline(units, topo, L)
+        base, idx, ref = rd_cpu_baseline(False, units, topo, L, cores)
+        out["cpu_baseline"] = base
+        lat0 = (units[0][0] + units[0][-1]) / 2.0
+        lon0 = (units[1][0] + units[1][-1]) / 2.0
+        out["sample_parity"] = sample_parity(arr[np.isin(arr["launch_idx"], idx)], ref, True, (lat0, lon0))
     return out
 
 
-def rngdep3d_leg(eng, with_cpu, target_s=8.0):
+def rngdep3d_leg(eng, with_cpu, cores=1):
     """3D Cartesian range-dependent fan (infraga-3d-rngdep): the grid and
     terrain of the spherical range-dependent leg with node coordinates read
     as km over +-800 km."""
@@ -197,20 +306,9 @@ def rngdep3d_leg(eng, with_cpu, target_s=8.0):
            "rays": int(len(L)), "ray_steps": int(st.ray_steps), "arrivals": int(len(arr)),
            "kernel_ms": st.kernel_ms, "wall_ms": wall * 1e3, "ray_steps_per_s": st.ray_steps / wall}
     if with_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        from ga_oracle_py import Oracle, Rd3Medium
-        m = Rd3Medium(Oracle(), *units)
-        m.set_topo(*topo)
-        op = m.params(True)
-        steps, n, t0 = 0, 0, time.process_time()
-        for i in range(0, len(L), 4801):
-            steps += m.fan3(L[i:i + 1], op, bounces=4)[2]
-            n += 1
-            if time.process_time() - t0 > target_s:
-                break
-        dt = time.process_time() - t0
-        out["cpu_baseline"] = {"value": steps / dt, "unit": "ray-steps/s", "cores": 1, "kind": "port",
-                               "sample": f"{n} rays (every 4801st launch) of the same fan, {steps} ray-steps, {dt:.1f} s"}
+        base, idx, ref = rd_cpu_baseline(True, units, topo, L, cores)
+        out["cpu_baseline"] = base
+        out["sample_parity"] = sample_parity(arr[np.isin(arr["launch_idx"], idx)], ref, False, (0.0, 0.0))
     return out
 
 
@@ -301,6 +399,7 @@ def main():
     ap.add_argument("--no-eig", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-rngdep", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="skip the scheduler autotuner (library defaults)")
     ap.add_argument("--one-pass", action="store_true",
                     help="one untimed fan pass (for hardware-counter runs); no JSON line")
     ap.add_argument("--leg", choices=("fan", "rngdep"), default="fan",
@@ -363,6 +462,9 @@ def main():
         eng.close()
         return
 
+    # the scheduler's steps per slice and slices per graph, picked by the
+    # library's autotuner on this launch list before the timed region
+    tuning = eng.fan_autotune(aid, p, d_l.data_ptr(), n, stream) if not args.no_tune else None
     for _ in range(args.warmup):
         one_pass()
     torch.cuda.synchronize()
@@ -396,24 +498,48 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded perturbed stratified profile)",
         "config": {"workload": "sph fan 450 incl x 360 az = 162000 rays, 5 bounces, 1000 km, 18 eq",
-                   "rays": n, "ray_steps_per_pass": steps_per_pass, "arrivals_per_pass": arrivals},
+                   "rays": n, "ray_steps_per_pass": steps_per_pass, "arrivals_per_pass": arrivals,
+                   "scheduler": ({"slice_steps": tuning.slice_steps, "slices_per_graph": tuning.slices_per_graph,
+                                  "tuned_pass_ms": tuning.pass_ms} if tuning else "defaults")},
         "roofline": {"bound": "mfma", "pipe": "fp64 (VALU; MI355X FP64 vector = matrix peak)",
+                     "bound_note": "compute-bound on the FP64 VALU pipe (no MFMA instructions); 'mfma' is the "
+                                   "compute roof of the hbm|mfma enumeration, priced at the FP64 peak",
+                     "per": "one pass = all slice_kernel launches of the fan (ring_init, slice batches, "
+                            "arrival_kernel)",
                      "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64, "traffic": TRAFFIC_BYTES,
                      "kernel_ms_events": kern_ms, "flops_per_step": F_STEP,
                      "transcendentals_per_step": TRANS_STEP},
     }
+    host = host_cpu()
+    ok = True
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(units, L)
+        base, idx, ref = cpu_baseline(units, L, host["cores"])
+        base.update({k: host[k] for k in ("nproc", "affinity", "cpu_model", "sockets")})
+        base["gpu_over_cpu"] = value / base["value"]
+        line["cpu_baseline"] = base
+        raw = d_out.cpu().numpy().view(ARRIVAL_DTYPE)
+        cnt = d_cnt.cpu().numpy()
+        nb = bounces + 1
+        gpu_rows = np.concatenate([raw[i * nb: i * nb + cnt[i]] for i in idx])
+        line["sample_parity"] = sample_parity(gpu_rows, ref, True, (30.0, 0.0))
+        ok = ok and line["sample_parity"]["ok"]
     if not args.no_eig:
-        line["eigenray"] = eig_leg(eng, not args.no_cpu)
+        line["eigenray"] = eig_leg(eng, not args.no_cpu, host["cores"])
     if not args.no_serve:
         line["served"] = serve_leg(eng, serve_trace())
     if not args.no_rngdep:
-        line["rngdep"] = rngdep_leg(eng, not args.no_cpu)
-        line["rngdep3d"] = rngdep3d_leg(eng, not args.no_cpu)
+        line["rngdep"] = rngdep_leg(eng, not args.no_cpu, host["cores"])
+        line["rngdep3d"] = rngdep3d_leg(eng, not args.no_cpu, host["cores"])
+        for leg in ("rngdep", "rngdep3d"):
+            if "sample_parity" in line[leg]:
+                ok = ok and line[leg]["sample_parity"]["ok"]
     eng.close()
     print(json.dumps(line))
+    if not ok:
+        # the full-size passes disagree with the oracle on their sampled rays
+        print("bench: sample parity failed (see sample_parity in the line)", file=sys.stderr)
+        sys.exit(3)
 
 
 if __name__ == "__main__":
