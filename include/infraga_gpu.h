@@ -193,10 +193,33 @@ int ga_fan_run(ga_ctx *ctx, int atmo_id, const ga_prop_params *p,
  * (2*n doubles), d_out (n * (bounces + 1) slots, slot r*(bounces+1)+b holds
  * bounce b of ray r), d_counts (n ints, arrivals per ray) and d_steps (n
  * ints, RK4 iterations per ray) are device pointers; stream is a hipStream_t
- * or NULL for the context stream.  Enqueues only: no synchronisation. */
+ * or NULL for the context stream.  Plain stratified fans run on the
+ * rotating slice scheduler: the call enqueues batches of slices (a captured
+ * graph) and waits on the stream after each batch until every ray is
+ * finished, then enqueues the arrival kernel and returns without waiting
+ * for it; the other cases only enqueue. */
 int ga_fan_run_device(ga_ctx *ctx, int atmo_id, const ga_prop_params *p,
                       const double *d_launches, int n_launch,
                       ga_arrival *d_out, int32_t *d_counts, int32_t *d_steps, void *stream);
+
+/* Scheduler settings of the plain (write_rays=false) fan: RK4 steps a ray
+ * advances per slice of the rotating scheduler, and slices captured per
+ * graph launch. */
+typedef struct {
+    int32_t slice_steps;
+    int32_t slices_per_graph;
+    double  pass_ms;       /* device time of the fastest candidate pass */
+} ga_fan_tuning;
+
+/* Autotuner of the fan scheduler.  No counterpart in the reference, whose
+ * MPI fan has no such knobs (rank count aside): it traces the given
+ * HBM-resident launch list (d_launches, 2*n doubles) once per candidate
+ * (steps per slice 32..512 x slices per graph 8, 32), keeps the fastest in
+ * the context for this geometry and equation count (later ga_fan_run /
+ * ga_fan_run_device calls use it) and returns it in *best.  Synchronises
+ * stream (NULL: the context stream).  Stratified geometries only. */
+int ga_fan_autotune(ga_ctx *ctx, int atmo_id, const ga_prop_params *p, const double *d_launches, int n_launch,
+                    void *stream, ga_fan_tuning *best);
 
 /* ga_fan_run_device without any device allocation.  No counterpart in the
  * reference: it is for callers that cannot allocate device memory (code

@@ -406,6 +406,10 @@ static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
 struct ga_ctx {
     int device = 0;
     unsigned char *scratch_ptr = nullptr;   // this device's address of scratch
+    // slice scheduler settings per [geometry sph?][18 equations?] chosen by
+    // ga_fan_autotune (0: the built-in defaults)
+    int tuned_slice[2][2] = {{0, 0}, {0, 0}};
+    int tuned_graph[2][2] = {{0, 0}, {0, 0}};
     hipStream_t stream = nullptr;
     FanBufs fb;                    // scratch of ga_fan_run / ga_fan_run_device
     int n_cu = 0;
@@ -599,7 +603,7 @@ constexpr int SLICES_PER_GRAPH = 32;
 // replayed until the ring is empty (one readback of head and tail per
 // batch), then the arrival kernel.  The per-ray arithmetic is lane_loop's.
 template <int G, int NEQ>
-int launch_fan_sliced(ga_ctx *ctx, FanBufs &fb, const FanArgs &A, hipStream_t st, int slice_steps)
+int launch_fan_sliced(ga_ctx *ctx, FanBufs &fb, const FanArgs &A, hipStream_t st, int slice_steps, int graph_k)
 {
     size_t lds = A.lds ? (size_t)A.M.n * NODE * sizeof(double) : 0;
     auto kern = A.lds ? slice_kernel<G, NEQ, true> : slice_kernel<G, NEQ, false>;
@@ -655,7 +659,7 @@ int launch_fan_sliced(ga_ctx *ctx, FanBufs &fb, const FanArgs &A, hipStream_t st
     R.ctl = fb.ctl;
     R.lanes = lanes;
     R.slice_steps = std::max(1, slice_steps);
-    const int K = std::max(1, env_int("GA_FAN_GRAPH", SLICES_PER_GRAPH));
+    const int K = std::max(1, graph_k);
 
     // one batch of K slices as a graph, re-captured when anything it bakes in changes
     std::vector<unsigned char> key(sizeof(void *) + sizeof(int) * 3 + sizeof(size_t) + sizeof(FanArgs) + sizeof(RayRing));
@@ -1138,12 +1142,14 @@ int fan_enqueue(ga_ctx *ctx, FanBufs &fb, int atmo_id, const ga_prop_params *p, 
     // the rotating slice scheduler for plain fans; the persistent ray queue
     // for write-mode fans, one-ray-per-lane server launches and the fixed
     // buffers of the no-allocation entry point
-    const int slice = env_int("GA_FAN_SLICE", SLICE_STEPS);
+    const int ts = ctx->tuned_slice[sph][p->calc_amp ? 1 : 0], tk = ctx->tuned_graph[sph][p->calc_amp ? 1 : 0];
+    const int slice = env_int("GA_FAN_SLICE", ts > 0 ? ts : SLICE_STEPS);
+    const int gk = env_int("GA_FAN_GRAPH", tk > 0 ? tk : SLICES_PER_GRAPH);
     if (!pw && !A.one_shot && !fb.fixed && slice > 0) {
-        if (sph) return p->calc_amp ? launch_fan_sliced<GEOM_SPH, 18>(ctx, fb, A, st, slice)
-                                    : launch_fan_sliced<GEOM_SPH, 6>(ctx, fb, A, st, slice);
-        return p->calc_amp ? launch_fan_sliced<GEOM_3D, 18>(ctx, fb, A, st, slice)
-                           : launch_fan_sliced<GEOM_3D, 6>(ctx, fb, A, st, slice);
+        if (sph) return p->calc_amp ? launch_fan_sliced<GEOM_SPH, 18>(ctx, fb, A, st, slice, gk)
+                                    : launch_fan_sliced<GEOM_SPH, 6>(ctx, fb, A, st, slice, gk);
+        return p->calc_amp ? launch_fan_sliced<GEOM_3D, 18>(ctx, fb, A, st, slice, gk)
+                           : launch_fan_sliced<GEOM_3D, 6>(ctx, fb, A, st, slice, gk);
     }
     if (sph) return p->calc_amp ? launch_fan<GEOM_SPH, 18>(ctx, fb, A, st, pw) : launch_fan<GEOM_SPH, 6>(ctx, fb, A, st, pw);
     return p->calc_amp ? launch_fan<GEOM_3D, 18>(ctx, fb, A, st, pw) : launch_fan<GEOM_3D, 6>(ctx, fb, A, st, pw);
@@ -1753,13 +1759,23 @@ struct ga_server {
     int launches = 0;
     double kernel_ms = 0.0;
 
+    // Device bytes a request holds while admitted: a fan's launch list, its
+    // output slots (arrival rows and raw records), per-ray counts and the
+    // row k-1 buffer of its one-ray-per-lane launch (lanes rounded up to a
+    // 512-lane workgroup), plus 1 MB of small buffers; an eigenray query
+    // about 1 MB per receiver (sweep and Levenberg-Marquardt trace jobs and
+    // their results) plus 16 MB.  The atmosphere tables are resident and
+    // counted when loaded, not per request.
     double footprint(const SrvReq &r) const
     {
         if (r.req.kind == GA_REQ_FAN) {
-            const double slots_ = (double)r.req.n * (r.req.prop.bounces + 1);
-            return slots_ * (sizeof(ga_arrival) + RAW * sizeof(double)) + r.req.n * 24.0 + 64.0e6;
+            const double n = (double)r.req.n;
+            const double slots_ = n * (r.req.prop.bounces + 1);
+            const double lanes = std::ceil(n / 512.0) * 512.0;
+            return n * 2.0 * sizeof(double) + slots_ * (sizeof(ga_arrival) + RAW * sizeof(double)) +
+                   n * 2.0 * sizeof(int32_t) + lanes * 18.0 * sizeof(double) + 1.0e6;
         }
-        return r.req.n * 1.0e6 + 64.0e6;
+        return r.req.n * 1.0e6 + 16.0e6;
     }
 
     void finish(SrvReq *r, int status)
@@ -2244,6 +2260,64 @@ int ga_fan_run_device_noalloc(ga_ctx *ctx, int atmo_id, const ga_prop_params *p,
     fb.raw = (double *)(ctx->scratch_ptr + NOALLOC_RAW_OFF);
     fb.raw_cap = NOALLOC_RAW_DOUBLES;
     return fan_enqueue(ctx, fb, atmo_id, p, d_launches, n, d_out, d_counts, d_steps, st);
+}
+
+int ga_fan_autotune(ga_ctx *ctx, int atmo_id, const ga_prop_params *p, const double *d_launches, int n,
+                    void *stream, ga_fan_tuning *best)
+{
+    if (!ctx || !p || !best || (n > 0 && !d_launches)) return fail(ctx, GA_E_ARG, "null argument");
+    if (n <= 0) return fail(ctx, GA_E_ARG, "autotuning needs a launch list");
+    if (p->geometry != GA_GEOM_SPH && p->geometry != GA_GEOM_3D)
+        return fail(ctx, GA_E_ARG, "autotuning covers the stratified geometries (got %d)", p->geometry);
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+    const int slots = p->bounces + 1;
+    DevBuf<ga_arrival> d_o;
+    DevBuf<int32_t> d_c, d_s;
+    if (!d_o.alloc((size_t)n * slots) || !d_c.alloc((size_t)n) || !d_s.alloc((size_t)n))
+        return fail(ctx, GA_E_NOMEM, "autotune buffers for %d rays", n);
+    Events ev;
+    if (!ev.ok) return fail(ctx, GA_E_HIP, "hipEventCreate failed");
+    const bool sph = p->geometry == GA_GEOM_SPH;
+    const int amp = p->calc_amp ? 1 : 0;
+    const int save_s = ctx->tuned_slice[sph][amp], save_g = ctx->tuned_graph[sph][amp];
+    static const int cand_slice[] = {32, 64, 128, 256, 512};
+    static const int cand_graph[] = {8, 32};
+    double best_ms = 1e300;
+    int bs = SLICE_STEPS, bg = SLICES_PER_GRAPH;
+    int rc = GA_OK;
+    // one untimed pass with the defaults: buffers, graph capture, code load
+    ctx->tuned_slice[sph][amp] = SLICE_STEPS;
+    ctx->tuned_graph[sph][amp] = SLICES_PER_GRAPH;
+    rc = fan_enqueue(ctx, ctx->fb, atmo_id, p, d_launches, n, d_o.p, d_c.p, d_s.p, st);
+    for (int cg : cand_graph)
+        for (int cs : cand_slice) {
+            if (rc != GA_OK) break;
+            ctx->tuned_slice[sph][amp] = cs;
+            ctx->tuned_graph[sph][amp] = cg;
+            hipError_t e0 = hipEventRecord(ev.e0, st);
+            rc = fan_enqueue(ctx, ctx->fb, atmo_id, p, d_launches, n, d_o.p, d_c.p, d_s.p, st);
+            if (rc != GA_OK) break;
+            hipError_t e1 = hipEventRecord(ev.e1, st);
+            if (e0 != hipSuccess || e1 != hipSuccess || hipStreamSynchronize(st) != hipSuccess) {
+                rc = fail(ctx, GA_E_HIP, "autotune timing failed");
+                break;
+            }
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, ev.e0, ev.e1) != hipSuccess) { rc = fail(ctx, GA_E_HIP, "event time"); break; }
+            if (ms < best_ms) { best_ms = ms; bs = cs; bg = cg; }
+        }
+    if (rc != GA_OK) {
+        ctx->tuned_slice[sph][amp] = save_s;
+        ctx->tuned_graph[sph][amp] = save_g;
+        return rc;
+    }
+    ctx->tuned_slice[sph][amp] = bs;
+    ctx->tuned_graph[sph][amp] = bg;
+    best->slice_steps = bs;
+    best->slices_per_graph = bg;
+    best->pass_ms = best_ms;
+    return GA_OK;
 }
 
 int ga_fan_run(ga_ctx *ctx, int atmo_id, const ga_prop_params *p, const double *launches, int n,

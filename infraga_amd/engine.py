@@ -29,7 +29,7 @@ EXPORTED = ["ga_prop_params_default", "ga_eig_params_default", "ga_ctx_create", 
             "ga_topo_load", "ga_rngdep_eval",
             "ga_rngdep_release", "ga_rngdep_load_files", "ga_rngdep_info", "ga_fan_run_paths",
             "ga_eig_search_paths", "ga_eig_direct", "ga_topo_load_2d", "ga_topo_eval_2d",
-            "ga_fan_run_device_noalloc"]
+            "ga_fan_run_device_noalloc", "ga_fan_autotune"]
 
 GA_PATH_RAYS, GA_PATH_CAUSTICS = 1, 2
 GA_PT_RAY, GA_PT_CAUSTIC = 0, 1
@@ -79,6 +79,10 @@ class Result(C.Structure):
                 ("total_ms", C.c_double), ("ray_steps", C.c_int64)]
 
 
+class FanTuning(C.Structure):
+    _fields_ = [("slice_steps", C.c_int32), ("slices_per_graph", C.c_int32), ("pass_ms", C.c_double)]
+
+
 class RunStats(C.Structure):
     _fields_ = [("ray_steps", C.c_int64), ("kernel_ms", C.c_double), ("wall_ms", C.c_double),
                 ("n_rays", C.c_int32), ("n_launches", C.c_int32)]
@@ -122,6 +126,8 @@ def load_library(path: str | None = None):
     L.ga_fan_run_device.argtypes = [C.c_void_p, C.c_int, C.POINTER(PropParams), C.c_void_p, C.c_int,
                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     L.ga_fan_run_device_noalloc.argtypes = L.ga_fan_run_device.argtypes
+    L.ga_fan_autotune.argtypes = [C.c_void_p, C.c_int, C.POINTER(PropParams), C.c_void_p, C.c_int, C.c_void_p,
+                                  C.POINTER(FanTuning)]
     L.ga_eig_search.argtypes = [C.c_void_p, C.c_int, C.POINTER(PropParams), C.POINTER(EigParams), _dp, C.c_int,
                                 C.POINTER(Arrival), C.c_int, C.POINTER(C.c_int), _dp, C.POINTER(RunStats)]
     L.ga_fan_run_paths.argtypes = [C.c_void_p, C.c_int, C.POINTER(PropParams), _dp, C.c_int, C.c_int,
@@ -262,6 +268,13 @@ class Engine:
         self._check(self.L.ga_fan_run_device(self.h, aid, C.byref(params), C.c_void_p(d_launches), n,
                                              C.c_void_p(d_out), C.c_void_p(d_counts), C.c_void_p(d_steps),
                                              C.c_void_p(stream) if stream else None))
+
+    def fan_autotune(self, aid, params: PropParams, d_launches: int, n: int, stream: int = 0) -> FanTuning:
+        """Pick the slice scheduler's settings on this HBM-resident launch list."""
+        best = FanTuning()
+        self._check(self.L.ga_fan_autotune(self.h, aid, C.byref(params), C.c_void_p(d_launches), n,
+                                           C.c_void_p(stream) if stream else None, C.byref(best)))
+        return best
 
     def fan_device_noalloc(self, aid, params: PropParams, d_launches: int, n: int, d_out: int, d_counts: int,
                            d_steps: int, stream: int = 0):

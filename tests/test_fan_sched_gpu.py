@@ -120,3 +120,24 @@ def test_graph_replays_equal_eager(pert_units):
     finally:
         hip.free_all()
         eng.close()
+
+
+def test_autotune_keeps_results(pert_units):
+    """ga_fan_autotune picks candidate settings and later calls with them
+    give the persistent queue's results bit for bit."""
+    from hip_mem import Hip
+    eng = Engine(0)
+    hip = Hip()
+    try:
+        aid = eng.load_atmo(*pert_units)
+        p = prop_params(GA_GEOM_SPH, calc_amp=1, bounces=2, src=(30.0, -100.0, 0.0), rng_max=700.0)
+        L = synth.fan_launches(1.0, 40.0, 1.0, 0.0, 340.0, 20.0)
+        ref = _fan(eng, hip, aid, p, L, {"GA_FAN_SLICE": 0})
+        d_l = hip.upload(np.asarray(L, dtype=np.float64))
+        best = eng.fan_autotune(aid, p, d_l, len(L))
+        assert best.slice_steps in (32, 64, 128, 256, 512) and best.slices_per_graph in (8, 32)
+        assert best.pass_ms > 0.0
+        _same(_fan(eng, hip, aid, p, L, {}), ref, "after autotune")
+    finally:
+        hip.free_all()
+        eng.close()

@@ -93,3 +93,41 @@ def test_server_reports_request_errors(engine, atmos):
         assert res.status == 0 and len(rows) == 0
     finally:
         srv.close()
+
+
+def test_admission_by_footprint_queues_and_results_equal_alone(engine, atmos):
+    """A budget below any two requests' footprints admits one request at a
+    time: each later request is admitted only after the one before it has
+    finished (submission-relative times from the results), and every result
+    still equals the same request run alone.  With the default budget the
+    same requests overlap (some admission precedes an earlier completion)."""
+    import time
+    reqs = _requests(atmos)
+    alone = [_alone(engine, *r) for r in reqs]
+
+    def run(budget):
+        srv = Server(engine, slots=4, merge=True, budget_bytes=budget)
+        try:
+            t_sub, tickets = [], []
+            t0 = time.perf_counter()
+            for kind, aid, p, e, data in reqs:
+                t_sub.append((time.perf_counter() - t0) * 1e3)
+                tickets.append(srv.submit_fan(aid, p, data) if kind == "fan" else srv.submit_eig(aid, p, e, data))
+            out = [srv.poll(t, timeout_ms=120000) for t in tickets]
+        finally:
+            srv.close()
+        adm = [ts + r.queue_ms for ts, (_, r) in zip(t_sub, out)]
+        done = [ts + r.total_ms for ts, (_, r) in zip(t_sub, out)]
+        return out, adm, done
+
+    out, adm, done = run(1.0e6)             # below every request's footprint
+    for i in range(1, len(reqs)):
+        assert adm[i] >= max(done[:i]) - 1.0, (i, adm[i], max(done[:i]))
+    for (kind, aid, p, e, data), ref, (rows, res) in zip(reqs, alone, out):
+        assert res.status == 0
+        sph = p.geometry in (GA_GEOM_SPH, GA_GEOM_SPH_RNGDEP)
+        fails, summ = (compare if kind == "fan" else compare_eig)(rows, ref, sph, src=(p.src[0], p.src[1]))
+        assert not fails, (kind, fails, summ)
+        assert len(rows) == len(ref) and len(rows) > 0
+    _, adm2, done2 = run(16e9)
+    assert any(adm2[i] < max(done2[:i]) - 1.0 for i in range(1, len(reqs)))
