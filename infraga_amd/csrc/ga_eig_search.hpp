@@ -27,6 +27,8 @@
 #include <cmath>
 #include <coroutine>
 #include <cstdint>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <exception>
 #include <functional>
@@ -425,6 +427,19 @@ private:
         return std::atan2(term1, term2);
     }
 
+    // Decision log of the 3D search for diagnosis (GA_EIG_LOG=<path>,
+    // appended): "E ri nb seq th_est ph_est" per accepted estimate and
+    // "L ri nb seq n theta phi dr act" per Levenberg-Marquardt iteration
+    // (act: b break, a accept, h halve, s step), angles in radians.
+    static FILE *decision_log()
+    {
+        static FILE *f = []() -> FILE * {
+            const char *path = std::getenv("GA_EIG_LOG");
+            return (path && *path) ? std::fopen(path, "a") : nullptr;
+        }();
+        return f;
+    }
+
     double mod_dth(double dr, double dr_dtheta) const   // geoac.eigenray.3d.cpp:32
     {
         return S_.dth_big - (S_.dth_big - S_.dth_sml) * std::exp(-1.0 / 2.0 * sq(dr / dr_dtheta));
@@ -498,6 +513,8 @@ private:
         const double *src = S_.src;
         const double rcvr[2] = {rcvr_[2 * ri], rcvr_[2 * ri + 1]};
         double theta = th_est, phi = ph_est;
+        FILE *lg = decision_log();
+        if (lg) std::fprintf(lg, "E %d %d %d %.17g %.17g\n", ri, nb, seq, th_est, ph_est);
         double dr, dr_prev = 10000.0, step_sc = 1.0;
         long double x, y, c_src, c_grnd, ds_dth, ds_dph, dx, dy, dx_dth, dy_dth, dx_dph, dy_dph;
         long double det, dth = 0.0L, dph = 0.0L;   // uninitialised in the reference; zero here
@@ -508,6 +525,7 @@ private:
             if (n == S_.iterations) break;
             FullHit h = co_await full(theta, phi, lm_legs, EIG_LM);
             const bool brk = h.brk;
+            if (brk && lg) std::fprintf(lg, "L %d %d %d %d %.17g %.17g nan b\n", ri, nb, seq, n, theta, phi);
             if (brk) {
                 theta -= dth * step_sc;
                 phi -= dph * step_sc;
@@ -520,6 +538,9 @@ private:
                 y = yk[1];
                 dy = rcvr[1] - y;
                 dr = (double)std::sqrt(std::pow(dx, 2) + std::pow(dy, 2));
+                if (lg)
+                    std::fprintf(lg, "L %d %d %d %d %.17g %.17g %.17g %c\n", ri, nb, seq, n, theta, phi, dr,
+                                 dr < S_.tolerance ? 'a' : (n > 0 && dr > dr_prev) ? 'h' : 's');
                 if (dr < S_.tolerance) {
                     FullHit f = co_await full(theta, phi, nb, EIG_FINAL);
                     ga_arrival row = f.row;

@@ -9,6 +9,7 @@
  * Scope: flat ground (no terrain file), topo_use_BLw=false (vertical wind
  * w == 0 everywhere), 3D Cartesian and spherical stratified geometries.
  */
+#include <stdio.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -1295,6 +1296,28 @@ typedef struct {
 /* Work counters of the last go_eig_search_3d call (see go_eig_counts). */
 static long long eig_cnt[GO_EIG_NCOUNT];
 
+/* Decision log of the 3D search for diagnosis (GO_EIG_LOG=<path>, appended):
+ * "E nb seq th_est ph_est" per accepted estimate, "L nb seq n theta phi dr act"
+ * per Levenberg-Marquardt iteration (act: b break, a accept, h halve, s step),
+ * "R x y" at the start of a receiver's search; angles in radians. */
+static FILE *eig_log(void)
+{
+    static int tried = 0;
+    static FILE *f = NULL;
+    if (!tried) {
+        tried = 1;
+        const char *path = getenv("GO_EIG_LOG");
+        if (path && *path) f = fopen(path, "a");
+        if (f) setvbuf(f, NULL, _IOLBF, 0);
+    }
+    return f;
+}
+static int eig_seq = 0;
+/* Trajectory of the last find_eigenray started by go_eig_find_3d (theta,
+ * phi, dr per iteration; dr NaN for a break), for conditioning probes. */
+static double *eig_traj = NULL;
+static int eig_traj_cap = 0, eig_traj_n = 0;
+
 static double mod_dth(const eig_ctx *E, double dr, double dr_dtheta)   /* eigenray.3d.cpp:32 */
 {
     return E->dth_big - (E->dth_big - E->dth_sml) * exp(-1.0 / 2.0 * sq(dr / dr_dtheta));
@@ -1474,11 +1497,23 @@ static int find_eigenray(eig_ctx *E, const double src[3], const double rcvr[2], 
                 break;
             }
         }
+        if (brk && eig_log()) fprintf(eig_log(), "L %d %d %d %.17g %.17g nan b\n", bnc_cnt, eig_seq, n, s.theta, s.phi);
+        if (eig_traj && eig_traj_n < eig_traj_cap) {
+            double *t = eig_traj + 3 * eig_traj_n++;
+            t[0] = s.theta; t[1] = s.phi; t[2] = NAN;
+            if (!brk) {
+                const double *yk = rows + 18 * k;
+                t[2] = sqrt(sq(rcvr[0] - yk[0]) + sq(rcvr[1] - yk[1]));
+            }
+        }
         if (!brk) {
             double *yk = rows + 18 * k;
             x = yk[0]; dx = rcvr[0] - x;
             y = yk[1]; dy = rcvr[1] - y;
             dr = (double)sqrtl(powl(dx, 2) + powl(dy, 2));
+            FILE *lg = eig_log();
+            if (lg) fprintf(lg, "L %d %d %d %.17g %.17g %.17g %c\n", bnc_cnt, eig_seq, n, s.theta, s.phi, dr,
+                            dr < E->tolerance ? 'a' : (n > 0 && dr > dr_prev) ? 'h' : 's');
             if (dr < E->tolerance) {
                 E->steps += eig_accept_3d(a, p, &s, src, bnc_cnt, freq, rows, length, A, NULL);
                 eig_cnt[GO_EIG_FINAL_TRACES]++;
@@ -1506,6 +1541,42 @@ static int find_eigenray(eig_ctx *E, const double src[3], const double rcvr[2], 
     return 0;
 }
 
+/* find_eigenray alone from a launch estimate in radians (the search's
+ * starting points, for sensitivity probes): 1 and the row in *A when it
+ * accepts, 0 otherwise. */
+int go_eig_find_3d(go_atmo *a, const go_params *p, const double src_in[3], const double rcvr[2], double th_est,
+                   double ph_est, int bnc, int iterations, double damping, double tolerance, double freq,
+                   go_arrival *A, long long *ray_steps)
+{
+    eig_ctx E;
+    E.a = a;
+    E.p6 = *p; E.p6.calc_amp = 0; E.p6.eq_cnt = 6;
+    E.p18 = *p; E.p18.calc_amp = 1; E.p18.eq_cnt = 18;
+    E.damping = damping;
+    E.tolerance = tolerance;
+    E.steps = 0;
+    double src[3] = {src_in[0], src_in[1], src_in[2] > ground(a) ? src_in[2] : ground(a)};
+    int ok = find_eigenray(&E, src, rcvr, th_est, ph_est, freq, bnc, iterations, A);
+    if (ray_steps) *ray_steps = E.steps;
+    return ok;
+}
+
+/* go_eig_find_3d that also records the iterates (theta, phi, dr) into traj
+ * (up to cap rows); *n_traj receives the number recorded. */
+int go_eig_find_3d_traj(go_atmo *a, const go_params *p, const double src_in[3], const double rcvr[2], double th_est,
+                        double ph_est, int bnc, int iterations, double damping, double tolerance, double freq,
+                        go_arrival *A, double *traj, int cap, int *n_traj)
+{
+    eig_traj = traj;
+    eig_traj_cap = cap;
+    eig_traj_n = 0;
+    int ok = go_eig_find_3d(a, p, src_in, rcvr, th_est, ph_est, bnc, iterations, damping, tolerance, freq, A, NULL);
+    *n_traj = eig_traj_n;
+    eig_traj = NULL;
+    eig_traj_cap = 0;
+    return ok;
+}
+
 int go_eig_search_3d(go_atmo *a, const go_params *p, const double src_in[3],
                      const double rcvr[2], double th_min_deg, double th_max_deg,
                      int bnc_min, int bnc_max, int iterations, double damping,
@@ -1526,17 +1597,21 @@ int go_eig_search_3d(go_atmo *a, const go_params *p, const double src_in[3],
     double src[3] = {src_in[0], src_in[1], src_in[2] > ground(a) ? src_in[2] : ground(a)};
     double th_min = th_min_deg * (GO_PI / 180.0), th_max = th_max_deg * (GO_PI / 180.0);
     int cnt = 0;
+    if (eig_log()) fprintf(eig_log(), "R %.17g %.17g\n", rcvr[0], rcvr[1]);
     for (int nb = bnc_min; nb <= bnc_max; nb++) {
         double th_start = th_min, th_next = th_min, th_est = 0, ph_est = 0;
+        eig_seq = 0;
         while (th_start < th_max) {
             int ok = est_eigenray(&E, src, rcvr, th_start, th_max, &th_est, &ph_est, &th_next, nb, az_err_lim);
             if (ok) {
                 go_arrival A;
                 eig_cnt[GO_EIG_FINDS]++;
+                if (eig_log()) fprintf(eig_log(), "E %d %d %.17g %.17g\n", nb, eig_seq, th_est, ph_est);
                 if (find_eigenray(&E, src, rcvr, th_est, ph_est, freq, nb, iterations, &A)) {
                     if (cnt < out_cap) out[cnt] = A;
                     cnt++;
                 }
+                eig_seq++;
             }
             th_start = th_next;
         }

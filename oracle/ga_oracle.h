@@ -197,6 +197,12 @@ long long go_eig_path(go_atmo *a, const go_params *p, const double src[3], doubl
  * angles (degrees) and bounces. */
 int    go_eig_row_3d(go_atmo *a, const go_params *p, const double src[3], double incl_deg, double az_deg,
                      int bnc_cnt, double freq, go_arrival *A);
+int    go_eig_find_3d(go_atmo *a, const go_params *p, const double src_in[3], const double rcvr[2], double th_est,
+                      double ph_est, int bnc, int iterations, double damping, double tolerance, double freq,
+                      go_arrival *A, long long *ray_steps);
+int    go_eig_find_3d_traj(go_atmo *a, const go_params *p, const double src_in[3], const double rcvr[2],
+                           double th_est, double ph_est, int bnc, int iterations, double damping, double tolerance,
+                           double freq, go_arrival *A, double *traj, int cap, int *n_traj);
 int    go_eig_search_3d(go_atmo *a, const go_params *p, const double src[3],
                         const double rcvr[2], double th_min_deg, double th_max_deg,
                         int bnc_min, int bnc_max, int iterations, double damping,

@@ -131,6 +131,14 @@ class Oracle:
         L.go_eig_search_3d.restype = C.c_int
         L.go_eig_search_sph.argtypes = L.go_eig_search_3d.argtypes
         L.go_eig_search_sph.restype = C.c_int
+        L.go_eig_find_3d.argtypes = [C.c_void_p, C.POINTER(Params), _dp, _dp, C.c_double, C.c_double, C.c_int,
+                                     C.c_int, C.c_double, C.c_double, C.c_double, C.POINTER(Arrival),
+                                     C.POINTER(C.c_longlong)]
+        L.go_eig_find_3d.restype = C.c_int
+        L.go_eig_find_3d_traj.argtypes = [C.c_void_p, C.POINTER(Params), _dp, _dp, C.c_double, C.c_double, C.c_int,
+                                          C.c_int, C.c_double, C.c_double, C.c_double, C.POINTER(Arrival), _dp,
+                                          C.c_int, C.POINTER(C.c_int)]
+        L.go_eig_find_3d_traj.restype = C.c_int
         L.go_eig_row_3d.argtypes = [C.c_void_p, C.POINTER(Params), _dp, C.c_double, C.c_double, C.c_int,
                                     C.c_double, C.POINTER(Arrival)]
         L.go_eig_row_3d.restype = C.c_int
@@ -262,6 +270,32 @@ class Medium:
                                       int(bounces), int(iterations), damping, tolerance, freq, C.byref(A),
                                       C.byref(steps))
         return (arrivals_to_array((Arrival * 1)(A), 1)[0] if ok else None), steps.value
+
+    def eig_find(self, params, src, rcvr, th_rad, ph_rad, bounces, iterations=25, damping=1.0e-3, tolerance=0.1,
+                 freq=0.1):
+        """3D: find_eigenray alone from (theta, phi) in radians (go_eig_find_3d):
+        (row or None, ray-steps)."""
+        s = np.ascontiguousarray(src, dtype=np.float64)
+        r = np.ascontiguousarray(rcvr, dtype=np.float64)
+        A = Arrival()
+        steps = C.c_longlong(0)
+        ok = self.o.lib.go_eig_find_3d(self.h, C.byref(params), _ptr(s), _ptr(r), float(th_rad), float(ph_rad),
+                                       int(bounces), int(iterations), damping, tolerance, freq, C.byref(A),
+                                       C.byref(steps))
+        return (arrivals_to_array((Arrival * 1)(A), 1)[0] if ok else None), steps.value
+
+    def eig_find_traj(self, params, src, rcvr, th_rad, ph_rad, bounces, iterations=25, damping=1.0e-3,
+                      tolerance=0.1, freq=0.1):
+        """eig_find that also returns the iterates: (row or None, array of (theta, phi, dr))."""
+        s = np.ascontiguousarray(src, dtype=np.float64)
+        r = np.ascontiguousarray(rcvr, dtype=np.float64)
+        A = Arrival()
+        traj = np.zeros((iterations + 1, 3))
+        n = C.c_int(0)
+        ok = self.o.lib.go_eig_find_3d_traj(self.h, C.byref(params), _ptr(s), _ptr(r), float(th_rad), float(ph_rad),
+                                            int(bounces), int(iterations), damping, tolerance, freq, C.byref(A),
+                                            _ptr(traj), traj.shape[0], C.byref(n))
+        return (arrivals_to_array((Arrival * 1)(A), 1)[0] if ok else None), traj[: n.value].copy()
 
     def eig_row(self, params, src, incl_deg, az_deg, bounces, freq=0.1):
         """3D: the row find_eigenray writes for an accepted eigenray with these

@@ -5,13 +5,15 @@
 - 3D: the 512 seeded source-receiver pairs of the batched workload (base
   profile, bounces 0..3) against the oracle's serial search, stored in
   tests/golden/eig3d_base.npz by tests/golden/make_eig_golden.py.  The
-  search is ill-conditioned at the level of its 0.1 km acceptance disc: the
-  same oracle compiled with FMA contraction (eig3d_base_fma.npz, same
-  script with --fma) changes the eigenray count of 5 and the rows of 28 of
-  the 249 (receiver, bounces) groups.  So every GPU row is checked tightly
-  against the oracle's accepted-eigenray trace at the GPU's own launch
-  angles (go_eig_row_3d), and the search result against the serial search
-  within the spread that contraction alone produces;
+  Levenberg-Marquardt iteration is chaotic near some starts: a 1e-13
+  relative change of the start changes whether it accepts
+  (tests/golden/make_eig_sensitivity.py classifies every start of the batch),
+  and the same oracle compiled with FMA contraction (eig3d_base_fma.npz)
+  changes the eigenray count of 5 groups, all of them holding such a start.
+  So every GPU row is checked tightly against the oracle's accepted-eigenray
+  trace at the GPU's own launch angles (go_eig_row_3d), and the eigenray
+  count of every (receiver, bounces) group whose starts are all determined
+  at FP64 precision must equal the serial search's;
 - a receiver's eigenrays are the same alone and inside a batch;
 - argument errors.
 Bar (rows): arrival position, travel time and turning height within 1e-6
@@ -32,6 +34,7 @@ pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
 GOLDEN = os.path.join(HERE, "golden", "eig3d_base.npz")
 GOLDEN_FMA = os.path.join(HERE, "golden", "eig3d_base_fma.npz")
+SENS = os.path.join(HERE, "golden", "eig3d_sensitivity.npz")
 
 
 def printed(x):
@@ -112,16 +115,21 @@ def test_3d_batch_matches_oracle(engine, base3d, oracle, base_units):
         if miss(rows, k[0]) > 0.1:
             assert k in O and miss(O[k], k[0]) > 0.1, (k, miss(rows, k[0]))
 
-    # 3. search: eigenray counts per (receiver, bounces) equal to the serial
-    #    search's, except in at most twice as many groups as the FMA build of
-    #    the oracle itself changes (groups where the two builds disagree are
-    #    left out); where the counts agree, launch angles within 0.1 degree
-    #    (the LM stops anywhere inside the tolerance disc)
+    # 3. search: eigenray counts per (receiver, bounces) identical to the
+    #    serial search's in every group whose starts are all determined at
+    #    FP64 precision (eig3d_sensitivity.npz); groups holding a start that
+    #    a 1e-13 relative change moves may differ, as the FMA build of the
+    #    oracle itself does.  Where the counts agree, launch angles within
+    #    0.1 degree (the LM stops anywhere inside the tolerance disc)
+    sens = np.load(SENS)
+    ung = {(int(s[0]), int(s[1])) for s, u in zip(sens["start"], sens["unstable"]) if u}
     keys = set(G) | set(O) | set(F)
     n_ref_flips = sum(1 for k in keys if len(O.get(k, [])) != len(F.get(k, [])))
-    flips = [k for k in keys if len(O.get(k, [])) == len(F.get(k, [])) and len(G.get(k, [])) != len(O.get(k, []))]
-    print(f"count differences: GPU vs oracle {len(flips)} {sorted(flips)}, oracle vs FMA build {n_ref_flips}")
-    assert n_ref_flips > 0 and len(flips) <= 2 * n_ref_flips, flips
+    flips = sorted(k for k in keys if len(G.get(k, [])) != len(O.get(k, [])))
+    hard = [k for k in flips if k not in ung]
+    print(f"count differences: GPU vs oracle {len(flips)} {flips} (all in the {len(ung)} groups with a start not "
+          f"determined at FP64 precision: {not hard}), oracle vs FMA build {n_ref_flips}")
+    assert not hard, hard
     for k in set(G) & set(O):
         if len(G[k]) != len(O[k]):
             continue
