@@ -49,10 +49,12 @@ sys.path.insert(0, ROOT)
 
 F_STEP = 5690.9          # algorithmic FP64 flops per ray-step (oracle/opcount.cpp, 400-ray sample)
 # HBM bytes of one pass (all slice_kernel dispatches) on this workload, from
-# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs scaled by a 256 MiB copy
-# (factors 2.00 and 1.00): 5.35 GB read + 20.34 GB written, mostly the ray
-# records' round trip through the ring (232 B each way per ray and slice)
-TRAFFIC_BYTES = 10443239.75 / 2.00 * 1024 + 19866050.0 * 1024
+# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs, each scaled by what the counter
+# reads for a 256 MiB device copy (FETCH_SIZE reads half the copy's bytes:
+# factor 2.00; WRITE_SIZE factor 1.00): 21.4 GB read + 20.4 GB written.  The
+# ray records' round trip through the ring is 232 B each way per ray and
+# slice, 12.9 GB each way over the pass's 5.56e7 ray-slices.
+TRAFFIC_BYTES = 10445547.0 * 2.00 * 1024 + 19894452.625 * 1.00 * 1024
 TRANS_STEP = 167.8       # transcendental calls per ray-step (same count, not included in F_STEP)
 PEAK_FP64 = 78.6e12      # MI355X FP64 vector / matrix peak
 METRIC = "ray-steps/s on 162k-ray sph fan (FP64)"

@@ -501,7 +501,10 @@ GA_HD double rk4_step(const Medium &M, const Limits &L, const double *y, double 
 
 // --------------------------------------------------------------------------
 // Sutherland & Bass absorption [dB/km] at height z with sound speed c [km/s]
-// and density rho [g/cm^3].  SPH selects the spherical variant.
+// and density rho [g/cm^3].  SPH selects the spherical variant.  Quotients
+// are products with refined reciprocals (ga_rcp), shared where a divisor
+// repeats (1/T, 1/c, 1/f_vib): within a few ulp of the divisions, about half
+// their instructions.
 // --------------------------------------------------------------------------
 template <bool SPH>
 GA_HD double sb_alpha(double z, double c, double rho, double freq, double tweak,
@@ -510,10 +513,13 @@ GA_HD double sb_alpha(double z, double c, double rho, double freq, double tweak,
     const double gam = 1.4, Rg = 287.058;
     const double mu_o = 18.192e-6, T_o = 293.15, P_o = 101325.0, S = 117.0;
     const double cms = c * 1.0e3;
-    const double T_z = cms * cms / (Rg * gam);
-    const double P_z = rho * 1.0e3 * (cms * cms) / gam;
-    const double mu = mu_o * sqrt(T_z / T_o) * ((1.0 + S / T_o) / (1.0 + S / T_z));
-    const double nu = (8.0 * PI * freq * mu) / (3.0 * P_z);
+    const double icms = ga_rcp(cms);
+    const double T_z = cms * cms * (1.0 / (Rg * gam));
+    const double iTz = ga_rcp(T_z);
+    const double P_z = rho * 1.0e3 * (cms * cms) * (1.0 / gam);
+    const double mu = mu_o * sqrt(T_z * (1.0 / T_o)) * ((1.0 + S / T_o) * ga_rcp(1.0 + S * iTz));
+    const double imu = ga_rcp(mu);
+    const double nu = (8.0 * PI * freq * mu) * ga_rcp(3.0 * P_z);
     const double z2 = z * z, z3 = z2 * z, z4 = z3 * z, z5 = z4 * z;
     double X0, X1, X3, X4, X5, X6;
     const double X2 = sbc[2];
@@ -532,20 +538,20 @@ GA_HD double sb_alpha(double z, double c, double rho, double freq, double tweak,
     // the reference's "z > 100" branch is unreachable behind "z > 30"
     X6 = ga_exp10((z > 30.0) ? (-4.2563 + (7.6245E-2 * z) - (2.1824E-3 * z2) - (2.3010E-6 * z3) + (2.4265E-7 * z4) - (1.2500E-09 * z5))
                           : (-1.7491 + (4.4986E-2 * z) - (6.8549E-2 * z2) + (5.4639E-3 * z3) - (1.5539E-4 * z4) + (1.5063E-06 * z5)));
-    const double X_ON = (X0 + X1) / 0.9903;
+    const double X_ON = (X0 + X1) * (1.0 / 0.9903);
     const double Tm13 = rcbrt(T_z);
-    const double Z_rot = 1.0 / ((X1 / (63.3 * ga_exp(-16.7 * Tm13))) + (X0 / (54.1 * ga_exp(-17.3 * Tm13))));
+    const double Z_rot = ga_rcp((X1 * ga_rcp(63.3 * ga_exp(-16.7 * Tm13))) + (X0 * ga_rcp(54.1 * ga_exp(-17.3 * Tm13))));
     const double sigma = 5.0 / sqrt(21.0);
     const double nn = (4.0 / 5.0) * sqrt(3.0 / 7.0) * Z_rot;
     const double chi = 3.0 * nn * nu / 4.0;
     const double cchi = 2.36 * chi;
-    const double beta_0 = 2.0 * PI * freq / cms;
+    const double beta_0 = 2.0 * PI * freq * icms;
     const double nu2p1 = 1.0 + nu * nu;
     const double snu = sqrt(nu2p1);
     double a_cl, a_rot;
     if (SPH) {
-        a_cl = beta_0 * sqrt(0.5 * (snu - 1.0) * (1.0 + cchi * cchi) / (nu2p1 * (1.0 + (sigma * cchi) * (sigma * cchi))));
-        a_rot = beta_0 * X_ON * ((sigma * sigma - 1.0) * chi / (2 * sigma)) * sqrt(0.5 * (snu + 1.0) / (nu2p1 * (1.0 + cchi * cchi)));
+        a_cl = beta_0 * sqrt(0.5 * (snu - 1.0) * (1.0 + cchi * cchi) * ga_rcp(nu2p1 * (1.0 + (sigma * cchi) * (sigma * cchi))));
+        a_rot = beta_0 * X_ON * ((sigma * sigma - 1.0) * (1.0 / (2 * sigma)) * chi) * sqrt(0.5 * (snu + 1.0) * ga_rcp(nu2p1 * (1.0 + cchi * cchi)));
     } else {
         const double beta_1 = sqrt(0.5 * (snu + 1.0) / nu2p1);
         const double beta_2 = sqrt((1.0 + cchi * cchi) / (1.0 + (sigma * cchi) * (sigma * cchi)));
@@ -555,7 +561,7 @@ GA_HD double sb_alpha(double z, double c, double rho, double freq, double tweak,
         a_cl = alpha_1 * beta_2;
         a_rot = alpha_2 * beta_1 * X_ON;
     }
-    const double Tr = rcbrt(T_z / T_o) - 1.0;
+    const double Tr = rcbrt(T_z * (1.0 / T_o)) - 1.0;
     const double A1 = (X0 + X1) * 24.0 * ga_exp(-9.16 * Tr);
     const double A2 = (X4 + X5) * 2400.0;
     const double B = 40400.0 * ga_exp(10.0 * Tr);
@@ -571,23 +577,25 @@ GA_HD double sb_alpha(double z, double c, double rho, double freq, double tweak,
     const double L = ga_exp(-7.72 * Tr);
     const double ZZ = H * X2 + I * (X0 + 0.5 * X4) + J * (X1 + 0.5 * X5) + K * (X6 + X3);
     const double hu = 100.0 * (X3 + X6);
-    const double pf = (P_z / P_o) * (mu_o / mu);
+    const double pf = (P_z * (1.0 / P_o)) * (mu_o * imu);
     const double fv[4] = {pf * (A1 + A2 + B * hu * (C + hu) * (D + hu)), pf * (E + F * X3 + G * X6), pf * ZZ, pf * (1.2E5) * L};
     const double Xm[4] = {X0, X1, X2, X3};
     const double Th[4] = {2239.1, 3352.0, 915.0, 1037.0};
     const double Cv[4] = {2.5, 2.5, 3.0, 3.0};
     const double Cp[4] = {3.5, 3.5, 4.0, 4.0};
-    const double iT = 1.0 / T_z;
+    const double iT = iTz;
     double a_vib = 0.0;
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
         const double tt = Th[m] * iT;
         const double e = ga_exp(-tt);
         const double ome = 1.0 - e;
-        const double C_R = (tt * tt * e) / (ome * ome);
-        const double A_max = (Xm[m] * (PI / 2) * C_R) / (Cp[m] * (Cv[m] + C_R));
-        const double fr = freq / fv[m];
-        a_vib += (A_max / cms) * ((2 * (freq * freq) / fv[m]) / (1.0 + fr * fr));
+        const double iome = ga_rcp(ome);
+        const double C_R = (tt * tt * e) * (iome * iome);
+        const double A_max = (Xm[m] * (PI / 2) * C_R) * ga_rcp(Cp[m] * (Cv[m] + C_R));
+        const double ifv = ga_rcp(fv[m]);
+        const double fr = freq * ifv;
+        a_vib += (A_max * icms) * ((2 * (freq * freq) * ifv) * ga_rcp(1.0 + fr * fr));
     }
     if (SPH) return (a_cl + a_rot + 0.003 * a_cl + a_vib) * tweak * 8.685889e3;
     return (a_cl * 1.003 + a_rot + a_vib) * tweak * 8.685889e3;
@@ -616,7 +624,7 @@ GA_HD void seg_book(const Medium &M, const double *y0, const double *y1, double 
     double n0 = y0[3] + (y1[3] - y0[3]) * 0.5;
     double n1 = y0[4] + (y1[4] - y0[4]) * 0.5;
     double n2 = y0[5] + (y1[5] - y0[5]) * 0.5;
-    double inm = 1.0 / sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+    double inm = ga_rcp(sqrt(n0 * n0 + n1 * n1 + n2 * n2));
     double z = (G == GEOM_SPH) ? m0 - M.r0 : m2;
     double h = (G == GEOM_SPH) ? m0 : m2;
     SegPt s = seg_point(M, clampz(M, z), cur);
@@ -629,7 +637,7 @@ GA_HD void seg_book(const Medium &M, const double *y0, const double *y1, double 
     double cp0, cp1, cp2;
     if (G == GEOM_SPH) { cp0 = c * n0 * inm; cp1 = c * n1 * inm + vv; cp2 = c * n2 * inm + uu; }
     else               { cp0 = c * n0 * inm + uu; cp1 = c * n1 * inm + vv; cp2 = c * n2 * inm; }
-    dt = ds / sqrt(cp0 * cp0 + cp1 * cp1 + cp2 * cp2);
+    dt = ds * ga_rcp(sqrt(cp0 * cp0 + cp1 * cp1 + cp2 * cp2));
     datt = sb_alpha<G == GEOM_SPH>(z, c, rho, freq, M.tweak, M.sbc) * ds;
 }
 
