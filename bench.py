@@ -51,10 +51,15 @@ F_STEP = 5690.9          # algorithmic FP64 flops per ray-step (oracle/opcount.c
 # HBM bytes of one pass (all slice_kernel dispatches) on this workload, from
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs, each scaled by what the counter
 # reads for a 256 MiB device copy (FETCH_SIZE reads half the copy's bytes:
-# factor 2.00; WRITE_SIZE factor 1.00): 21.4 GB read + 20.4 GB written.  The
+# factor 2.00; WRITE_SIZE factor 1.00): 22.3 GB read + 20.0 GB written.  The
 # ray records' round trip through the ring is 232 B each way per ray and
 # slice, 12.9 GB each way over the pass's 5.56e7 ray-slices.
-TRAFFIC_BYTES = 10445547.0 * 2.00 * 1024 + 19894452.625 * 1.00 * 1024
+TRAFFIC_BYTES = 10909465.1875 * 2.00 * 1024 + 19569994.84375 * 1.00 * 1024
+# FP64 flops the hardware executes per ray-step on this workload (rocprofv3
+# SQ_INSTS_VALU_{FMA,MUL,ADD}_F64 over one pass, FMA = 2, x 64 lanes / ray-steps).
+# Below F_STEP: the engine evaluates the smooth Sutherland-Bass terms from a
+# per-frequency table of piecewise polynomials instead of the formula.
+F_STEP_EXEC = 3305.2
 TRANS_STEP = 167.8       # transcendental calls per ray-step (same count, not included in F_STEP)
 PEAK_FP64 = 78.6e12      # MI355X FP64 vector / matrix peak
 METRIC = "ray-steps/s on 162k-ray sph fan (FP64)"
@@ -511,7 +516,12 @@ def main():
                      "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64, "traffic": TRAFFIC_BYTES,
                      "kernel_ms_events": kern_ms, "flops_per_step": F_STEP,
-                     "transcendentals_per_step": TRANS_STEP},
+                     "transcendentals_per_step": TRANS_STEP,
+                     "executed_flops_per_step": F_STEP_EXEC,
+                     "frac_executed": F_STEP_EXEC * steps_per_pass / (kern_ms / 1e3) / PEAK_FP64,
+                     "flops_note": "achieved/frac count the reference algorithm's FP64 work per ray-step "
+                                   "(op-counted oracle); frac_executed counts the FP64 work the hardware runs "
+                                   "(rocprof), smaller because Sutherland-Bass comes from a table"},
     }
     host = host_cpu()
     ok = True

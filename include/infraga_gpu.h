@@ -170,7 +170,11 @@ const char *ga_last_error(const ga_ctx *ctx);
 
 /* Upload a stratified profile (n >= 3 strictly increasing z).  The natural
  * cubic spline slopes are solved on the host exactly as interp::set does
- * (src/util/interpolation.cpp:103-136). */
+ * (src/util/interpolation.cpp:103-136).  The first run at a frequency and
+ * geometry also builds the atmosphere's absorption table (piecewise
+ * polynomials of the smooth Sutherland-Bass terms, 176 B per 0.05 km of
+ * profile), kept until ga_atmo_release; GA_ABS_TABLE=0 in the environment
+ * evaluates absorption directly instead. */
 int ga_atmo_load_strat(ga_ctx *ctx, const double *z_km, const double *c_kms,
                        const double *u_kms, const double *v_kms, const double *rho_gcc,
                        int n, int *atmo_id);
@@ -232,7 +236,12 @@ int ga_fan_autotune(ga_ctx *ctx, int atmo_id, const ga_prop_params *p, const dou
  * (bounces + 1) may be at most 157,285 arrival slots (GA_E_ARG beyond, with
  * nothing launched).  Every call on one GPU shares that array: calls that
  * overlap on one GPU (other streams, other contexts or threads) race, so
- * serialise them.  The caller's buffers are as for ga_fan_run_device. */
+ * serialise them.  The caller's buffers are as for ga_fan_run_device.
+ * Absorption uses the atmosphere's absorption table for the run's
+ * frequency and geometry when an earlier allocating call (ga_fan_run*,
+ * ga_eig_search*) has built it, and evaluates Sutherland-Bass directly
+ * otherwise; the two agree within 1e-12 relative per step, so results
+ * equal ga_fan_run_device's bit for bit once such a call has run. */
 int ga_fan_run_device_noalloc(ga_ctx *ctx, int atmo_id, const ga_prop_params *p,
                               const double *d_launches, int n_launch,
                               ga_arrival *d_out, int32_t *d_counts, int32_t *d_steps, void *stream);

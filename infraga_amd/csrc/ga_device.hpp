@@ -33,6 +33,10 @@
 // GA_HOST_BUILD before including this header.
 #ifndef GA_HOST_BUILD
 #include <hip/hip_runtime.h>
+#else
+#include <cmath>
+#include <cstring>
+using std::fma; using std::sqrt; using std::rint; using std::fabs; using std::fmin; using std::fmax;
 #endif
 #ifndef GA_HD
 #define GA_HD __device__ __forceinline__
@@ -57,6 +61,8 @@ struct Medium {
     double z_bndlyr;     // boundary-layer top
     double tweak;        // Sutherland-Bass scale
     double sbc[3];       // 10^-0.67887, 10^-0.10744, 10^-3.3979 (host libm pow)
+    const double *abs;   // absorption table at the run's frequency (nullptr: direct)
+    int abs_per;         // table pieces per node interval
 };
 
 struct Limits {
@@ -81,6 +87,12 @@ struct Limits {
 // rcp: hardware reciprocal estimate refined by two Newton steps (error below
 // 1 ulp, not always the correctly rounded quotient).
 // --------------------------------------------------------------------------
+#ifdef GA_HOST_BUILD
+GA_HD double ga_rcp(double x) { return 1.0 / x; }
+GA_HD double ga_rcbrt(double x) { return 1.0 / std::cbrt(x); }
+GA_HD long long __double_as_longlong(double x) { long long b; std::memcpy(&b, &x, 8); return b; }
+GA_HD double __longlong_as_double(long long b) { double x; std::memcpy(&x, &b, 8); return x; }
+#else
 GA_HD double ga_rcp(double x)
 {
     double r = __builtin_amdgcn_rcp(x);
@@ -89,6 +101,8 @@ GA_HD double ga_rcp(double x)
     e = fma(-x, r, 1.0);
     return fma(r, e, r);
 }
+GA_HD double ga_rcbrt(double x) { return rcbrt(x); }
+#endif
 
 GA_HD double k_sin(double x)
 {
@@ -505,21 +519,50 @@ GA_HD double rk4_step(const Medium &M, const Limits &L, const double *y, double 
 // are products with refined reciprocals (ga_rcp), shared where a divisor
 // repeats (1/T, 1/c, 1/f_vib): within a few ulp of the divisions, about half
 // their instructions.
+//
+// The sum is written as  (1.003 A Q + R) * tweak * 8.685889e3  with
+//   A = beta_0 sqrt((sqrt(1 + nu^2) - 1) / (2 (1 + nu^2)))   (classical),
+//   Q = sqrt((1 + cchi^2) / (1 + (sigma cchi)^2)),
+//   R = a_rot + a_vib,
+// which is the reference's a_cl + a_rot + 0.003 a_cl + a_vib (sph) and
+// a_cl * 1.003 + a_rot + a_vib (cart) with a_cl = A Q.  A depends on c and
+// rho alone and carries the cancellation of sqrt(1 + nu^2) - 1 (nu ~ 1e-7
+// at 65 km: 1 + nu^2 is a few dozen ulp above 1, so A steps with rho and c
+// as the reference's does); Q and R are smooth functions of the height.
 // --------------------------------------------------------------------------
-template <bool SPH>
-GA_HD double sb_alpha(double z, double c, double rho, double freq, double tweak,
-                                           const double *sbc)
+struct SbGas {
+    double cms, icms, T_z, iTz, P_z, mu, imu, nu, nu2p1, snu;
+};
+
+GA_HD SbGas sb_gas(double c, double rho, double freq)
 {
     const double gam = 1.4, Rg = 287.058;
-    const double mu_o = 18.192e-6, T_o = 293.15, P_o = 101325.0, S = 117.0;
-    const double cms = c * 1.0e3;
-    const double icms = ga_rcp(cms);
-    const double T_z = cms * cms * (1.0 / (Rg * gam));
-    const double iTz = ga_rcp(T_z);
-    const double P_z = rho * 1.0e3 * (cms * cms) * (1.0 / gam);
-    const double mu = mu_o * sqrt(T_z * (1.0 / T_o)) * ((1.0 + S / T_o) * ga_rcp(1.0 + S * iTz));
-    const double imu = ga_rcp(mu);
-    const double nu = (8.0 * PI * freq * mu) * ga_rcp(3.0 * P_z);
+    const double mu_o = 18.192e-6, T_o = 293.15, S = 117.0;
+    SbGas g;
+    g.cms = c * 1.0e3;
+    g.icms = ga_rcp(g.cms);
+    g.T_z = g.cms * g.cms * (1.0 / (Rg * gam));
+    g.iTz = ga_rcp(g.T_z);
+    g.P_z = rho * 1.0e3 * (g.cms * g.cms) * (1.0 / gam);
+    g.mu = mu_o * sqrt(g.T_z * (1.0 / T_o)) * ((1.0 + S / T_o) * ga_rcp(1.0 + S * g.iTz));
+    g.imu = ga_rcp(g.mu);
+    g.nu = (8.0 * PI * freq * g.mu) * ga_rcp(3.0 * g.P_z);
+    g.nu2p1 = 1.0 + g.nu * g.nu;
+    g.snu = sqrt(g.nu2p1);
+    return g;
+}
+
+// A: the classical term without the rotational factor Q.
+GA_HD double sb_cl(const SbGas &g, double freq)
+{
+    return (2.0 * PI * freq * g.icms) * sqrt(0.5 * (g.snu - 1.0) * ga_rcp(g.nu2p1));
+}
+
+// Q and R at height z.
+template <bool SPH>
+GA_HD void sb_smooth(double z, const SbGas &g, double freq, const double *sbc, double &Q, double &R)
+{
+    const double T_o = 293.15, P_o = 101325.0, mu_o = 18.192e-6;
     const double z2 = z * z, z3 = z2 * z, z4 = z3 * z, z5 = z4 * z;
     double X0, X1, X3, X4, X5, X6;
     const double X2 = sbc[2];
@@ -539,29 +582,25 @@ GA_HD double sb_alpha(double z, double c, double rho, double freq, double tweak,
     X6 = ga_exp10((z > 30.0) ? (-4.2563 + (7.6245E-2 * z) - (2.1824E-3 * z2) - (2.3010E-6 * z3) + (2.4265E-7 * z4) - (1.2500E-09 * z5))
                           : (-1.7491 + (4.4986E-2 * z) - (6.8549E-2 * z2) + (5.4639E-3 * z3) - (1.5539E-4 * z4) + (1.5063E-06 * z5)));
     const double X_ON = (X0 + X1) * (1.0 / 0.9903);
-    const double Tm13 = rcbrt(T_z);
+    const double Tm13 = ga_rcbrt(g.T_z);
     const double Z_rot = ga_rcp((X1 * ga_rcp(63.3 * ga_exp(-16.7 * Tm13))) + (X0 * ga_rcp(54.1 * ga_exp(-17.3 * Tm13))));
     const double sigma = 5.0 / sqrt(21.0);
     const double nn = (4.0 / 5.0) * sqrt(3.0 / 7.0) * Z_rot;
-    const double chi = 3.0 * nn * nu / 4.0;
+    const double chi = 3.0 * nn * g.nu / 4.0;
     const double cchi = 2.36 * chi;
-    const double beta_0 = 2.0 * PI * freq * icms;
-    const double nu2p1 = 1.0 + nu * nu;
-    const double snu = sqrt(nu2p1);
-    double a_cl, a_rot;
+    const double beta_0 = 2.0 * PI * freq * g.icms;
+    Q = sqrt((1.0 + cchi * cchi) * ga_rcp(1.0 + (sigma * cchi) * (sigma * cchi)));
+    double a_rot;
     if (SPH) {
-        a_cl = beta_0 * sqrt(0.5 * (snu - 1.0) * (1.0 + cchi * cchi) * ga_rcp(nu2p1 * (1.0 + (sigma * cchi) * (sigma * cchi))));
-        a_rot = beta_0 * X_ON * ((sigma * sigma - 1.0) * (1.0 / (2 * sigma)) * chi) * sqrt(0.5 * (snu + 1.0) * ga_rcp(nu2p1 * (1.0 + cchi * cchi)));
+        a_rot = beta_0 * X_ON * ((sigma * sigma - 1.0) * (1.0 / (2 * sigma)) * chi) *
+                sqrt(0.5 * (g.snu + 1.0) * ga_rcp(g.nu2p1 * (1.0 + cchi * cchi)));
     } else {
-        const double beta_1 = sqrt(0.5 * (snu + 1.0) / nu2p1);
-        const double beta_2 = sqrt((1.0 + cchi * cchi) / (1.0 + (sigma * cchi) * (sigma * cchi)));
-        const double alpha_1 = beta_0 * sqrt(0.5 * (snu - 1.0) / nu2p1);
-        const double alpha_2 = beta_0 * (((sigma / 2.0 - 1.0 / (2.0 * sigma)) * chi) /
-                                         (sqrt((1.0 + cchi * cchi) * (1.0 + (sigma * cchi) * (sigma * cchi)))));
-        a_cl = alpha_1 * beta_2;
+        const double beta_1 = sqrt(0.5 * (g.snu + 1.0) * ga_rcp(g.nu2p1));
+        const double alpha_2 = beta_0 * (((sigma / 2.0 - 1.0 / (2.0 * sigma)) * chi) *
+                                         ga_rcp(sqrt((1.0 + cchi * cchi) * (1.0 + (sigma * cchi) * (sigma * cchi)))));
         a_rot = alpha_2 * beta_1 * X_ON;
     }
-    const double Tr = rcbrt(T_z * (1.0 / T_o)) - 1.0;
+    const double Tr = ga_rcbrt(g.T_z * (1.0 / T_o)) - 1.0;
     const double A1 = (X0 + X1) * 24.0 * ga_exp(-9.16 * Tr);
     const double A2 = (X4 + X5) * 2400.0;
     const double B = 40400.0 * ga_exp(10.0 * Tr);
@@ -577,17 +616,16 @@ GA_HD double sb_alpha(double z, double c, double rho, double freq, double tweak,
     const double L = ga_exp(-7.72 * Tr);
     const double ZZ = H * X2 + I * (X0 + 0.5 * X4) + J * (X1 + 0.5 * X5) + K * (X6 + X3);
     const double hu = 100.0 * (X3 + X6);
-    const double pf = (P_z * (1.0 / P_o)) * (mu_o * imu);
+    const double pf = (g.P_z * (1.0 / P_o)) * (mu_o * g.imu);
     const double fv[4] = {pf * (A1 + A2 + B * hu * (C + hu) * (D + hu)), pf * (E + F * X3 + G * X6), pf * ZZ, pf * (1.2E5) * L};
     const double Xm[4] = {X0, X1, X2, X3};
     const double Th[4] = {2239.1, 3352.0, 915.0, 1037.0};
     const double Cv[4] = {2.5, 2.5, 3.0, 3.0};
     const double Cp[4] = {3.5, 3.5, 4.0, 4.0};
-    const double iT = iTz;
     double a_vib = 0.0;
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
-        const double tt = Th[m] * iT;
+        const double tt = Th[m] * g.iTz;
         const double e = ga_exp(-tt);
         const double ome = 1.0 - e;
         const double iome = ga_rcp(ome);
@@ -595,10 +633,116 @@ GA_HD double sb_alpha(double z, double c, double rho, double freq, double tweak,
         const double A_max = (Xm[m] * (PI / 2) * C_R) * ga_rcp(Cp[m] * (Cv[m] + C_R));
         const double ifv = ga_rcp(fv[m]);
         const double fr = freq * ifv;
-        a_vib += (A_max * icms) * ((2 * (freq * freq) * ifv) * ga_rcp(1.0 + fr * fr));
+        a_vib += (A_max * g.icms) * ((2 * (freq * freq) * ifv) * ga_rcp(1.0 + fr * fr));
     }
-    if (SPH) return (a_cl + a_rot + 0.003 * a_cl + a_vib) * tweak * 8.685889e3;
-    return (a_cl * 1.003 + a_rot + a_vib) * tweak * 8.685889e3;
+    R = a_rot + a_vib;
+}
+
+template <bool SPH>
+GA_HD double sb_alpha(double z, double c, double rho, double freq, double tweak, const double *sbc)
+{
+    const SbGas g = sb_gas(c, rho, freq);
+    double Q, R;
+    sb_smooth<SPH>(z, g, freq, sbc, Q, R);
+    return (1.003 * sb_cl(g, freq) * Q + R) * tweak * 8.685889e3;
+}
+
+// --------------------------------------------------------------------------
+// Absorption table of a stratified atmosphere at one frequency.  Q(z) and
+// R(z) are smooth between the spline nodes (c and rho are cubics there) and
+// between the heights where the composition fits switch formula (30, 76,
+// 80, 90, 95 km), so each node interval is cut into abs_per equal pieces of
+// at most ABS_PIECE km and each piece holds the degree-ABS_DEG polynomials
+// in t = (z - mid) / half-width that interpolate Q and R at the Chebyshev
+// points of the piece (values from sb_smooth itself).  Where a switch height
+// h falls inside a piece, the record covers only the piece's lower side and
+// an extra record the upper side; where h is a piece's lower end, z == h
+// belongs to the piece below (the fits switch for z > h).  The step then
+// evaluates A directly and Q, R by Horner's rule: agreement with sb_alpha
+// within 1e-12 relative on the synthetic profiles, checked for every piece
+// when the table is built (abs_fill_record's error output).  Record:
+//   [0] split_lo, [1] split_hi: z <= split_lo or z > split_hi selects the
+//       record at index [2] instead (-inf / +inf: no split)
+//   [3] piece mid, [4] 1 / half-width, [5] unused
+//   [6..13] R coefficients, [14..21] Q coefficients (ascending powers of t)
+// --------------------------------------------------------------------------
+constexpr int ABS_DEG = 7;
+constexpr int ABS_REC = 22;
+constexpr double ABS_PIECE = 0.05;
+constexpr int ABS_MAX_PER = 64;          // node intervals wider than 3.2 km: no table
+constexpr double ABS_TOL = 1.0e-10;      // largest relative error a table may have
+
+struct AbsBasis {
+    double t[ABS_DEG + 1];                       // Chebyshev points cos(pi (k + 1/2) / 8)
+    double inv[ABS_DEG + 1][ABS_DEG + 1];        // inverse Vandermonde matrix at t
+};
+
+GA_HD double abs_poly(const double *c, double t)
+{
+    double p = c[ABS_DEG];
+#pragma unroll
+    for (int j = ABS_DEG - 1; j >= 0; --j) p = fma(p, t, c[j]);
+    return p;
+}
+
+// Absorption [dB/km] from the table (z inside [zlo, zhi]; s: the spline
+// point of z in node interval cur).
+GA_HD double abs_alpha(const Medium &M, const SegPt &s, int cur, double z, double c, double rho, double freq)
+{
+    const int per = M.abs_per;
+    int j = (int)(s.X * (double)per);
+    j = j < 0 ? 0 : (j >= per ? per - 1 : j);
+    const double *r = M.abs + ((size_t)cur * per + j) * ABS_REC;
+    if (z <= r[0] || z > r[1]) r = M.abs + (size_t)r[2] * ABS_REC;
+    const double t = (z - r[3]) * r[4];
+    const double R = abs_poly(r + 6, t), Q = abs_poly(r + 14, t);
+    const SbGas g = sb_gas(c, rho, freq);
+    return (1.003 * sb_cl(g, freq) * Q + R) * M.tweak * 8.685889e3;
+}
+
+template <bool SPH>
+GA_HD void abs_exact(const Medium &M, double z, double freq, int &cur, double &Q, double &R)
+{
+    const SegPt s = seg_point(M, z, cur);
+    const SbGas g = sb_gas(herm_f(s, 1), herm_f(s, 7), freq);
+    sb_smooth<SPH>(z, g, freq, M.sbc, Q, R);
+}
+
+// One table record for the piece [lo, hi] of node interval cur: the
+// interpolating coefficients, and in *err the largest relative difference
+// between the polynomials and sb_smooth at three points between the
+// Chebyshev points.
+template <bool SPH>
+GA_HD void abs_fill_record(const Medium &M, const AbsBasis &B, double lo, double hi, double split_lo, double split_hi,
+                           double alt, int cur, double freq, double *rec, double *err)
+{
+    const double mid = 0.5 * (lo + hi), hw = 0.5 * (hi - lo);
+    double vq[ABS_DEG + 1], vr[ABS_DEG + 1];
+    for (int k = 0; k <= ABS_DEG; ++k) abs_exact<SPH>(M, mid + hw * B.t[k], freq, cur, vq[k], vr[k]);
+    rec[0] = split_lo;
+    rec[1] = split_hi;
+    rec[2] = alt;
+    rec[3] = mid;
+    rec[4] = 1.0 / hw;
+    rec[5] = 0.0;
+    for (int j = 0; j <= ABS_DEG; ++j) {
+        double r = 0.0, q = 0.0;
+        for (int k = 0; k <= ABS_DEG; ++k) {
+            r = fma(B.inv[j][k], vr[k], r);
+            q = fma(B.inv[j][k], vq[k], q);
+        }
+        rec[6 + j] = r;
+        rec[14 + j] = q;
+    }
+    double e = 0.0;
+    const double tc[3] = {-0.6, 0.1, 0.9};
+    for (int k = 0; k < 3; ++k) {
+        double q, r;
+        abs_exact<SPH>(M, mid + hw * tc[k], freq, cur, q, r);
+        e = fmax(e, fabs(abs_poly(rec + 6, tc[k]) - r) / fabs(r));
+        e = fmax(e, fabs(abs_poly(rec + 14, tc[k]) - q) / fabs(q));
+    }
+    *err = (e == e) ? e : 1.0;
 }
 
 // Travel time and absorption of the segment y0 -> y1 evaluated at the
@@ -638,7 +782,10 @@ GA_HD void seg_book(const Medium &M, const double *y0, const double *y1, double 
     if (G == GEOM_SPH) { cp0 = c * n0 * inm; cp1 = c * n1 * inm + vv; cp2 = c * n2 * inm + uu; }
     else               { cp0 = c * n0 * inm + uu; cp1 = c * n1 * inm + vv; cp2 = c * n2 * inm; }
     dt = ds * ga_rcp(sqrt(cp0 * cp0 + cp1 * cp1 + cp2 * cp2));
-    datt = sb_alpha<G == GEOM_SPH>(z, c, rho, freq, M.tweak, M.sbc) * ds;
+    const double alpha = (M.abs && z >= M.zlo && z <= M.zhi)
+                             ? abs_alpha(M, s, cur, z, c, rho, freq)
+                             : sb_alpha<G == GEOM_SPH>(z, c, rho, freq, M.tweak, M.sbc);
+    datt = alpha * ds;
 }
 
 // Point values c, u, v, rho at a state (atmo::c/u/v/rho with bndlyr_sc).

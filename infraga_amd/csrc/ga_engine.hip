@@ -35,6 +35,7 @@
 #include "ga_rdtrace.hpp"
 #include "ga_rngdep.hpp"
 #include "ga_trace.hpp"
+#include "ga_abs_host.hpp"
 
 using namespace ga;
 
@@ -149,6 +150,17 @@ __global__ void slice_ctl(RayRing R)
     const unsigned long long m = min(tail - head, (unsigned long long)R.lanes);
     R.ctl[CTL_HEAD] = head + m;
     R.ctl[CTL_TAIL] = R.ctl[CTL_APPEND];
+}
+
+// Absorption table records (ga_device.hpp), one per thread.
+template <bool SPH>
+__global__ void __launch_bounds__(64) abs_fill(Medium M, AbsBasis B, const double *geo, int nrec, double freq,
+                                               double *rec, double *err)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nrec) return;
+    const double *g = geo + (size_t)k * ABS_GEO;
+    abs_fill_record<SPH>(M, B, g[0], g[1], g[2], g[3], g[4], (int)g[5], freq, rec + (size_t)k * ABS_REC, err + k);
 }
 
 // 2D fans (infraga-2d): persistent lanes, the stratified table in LDS when it fits.
@@ -309,12 +321,30 @@ __global__ void __launch_bounds__(256) eig_arrival_kernel(EigArgs E, int first, 
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
+// Absorption table of an atmosphere at one frequency and geometry (see
+// ga_device.hpp); ok = false records that no table is used (layout or
+// accuracy check failed).
+struct AbsTable {
+    double freq = 0.0;
+    bool sph = false;
+    bool ok = false;
+    int per = 0;
+    double *d = nullptr;
+};
+
 struct DevAtmo {
     double *d_tab = nullptr;
     int n = 0;
     std::vector<double> z, c, u, v, rho;
     double *d_topo2d = nullptr;        // 2D terrain spline [n][3] (r, z, slope)
     std::vector<double> topo_r, topo_z;
+    std::vector<AbsTable> abs;
+    void release_abs()
+    {
+        for (AbsTable &t : abs)
+            if (t.d) (void)hipFree(t.d);
+        abs.clear();
+    }
 };
 
 // Range-dependent atmosphere (hybrid-spline tables) and optional topography.
@@ -502,7 +532,72 @@ double host_c_at(const DevAtmo &a, double z)
     return a.c[k] + X * (df + (1.0 - X) * (Ac * (1.0 - X) + Bc * X));
 }
 
-bool build_medium(ga_ctx *ctx, const DevAtmo &a, const ga_prop_params *p, Medium &M, Limits &L, int &rc)
+// The absorption table of atmosphere a at (freq, geometry): built on first
+// use (one kernel on a stream of its own, waited for) and kept with the
+// atmosphere; a table whose accuracy check fails, or an atmosphere whose
+// node spacing is too coarse, runs with direct evaluation.  may_alloc =
+// false (callers that allocate nothing) uses a table only if one exists.
+// GA_ABS_TABLE=0 turns tables off.
+void abs_attach(ga_ctx *ctx, DevAtmo &a, bool sph, double freq, bool may_alloc, Medium &M)
+{
+    M.abs = nullptr;
+    M.abs_per = 0;
+    const char *ev = std::getenv("GA_ABS_TABLE");
+    if (ev && *ev && std::atoi(ev) == 0) return;
+    for (const AbsTable &t : a.abs) {
+        if (t.freq == freq && t.sph == sph) {
+            if (t.ok) {
+                M.abs = t.d;
+                M.abs_per = t.per;
+            }
+            return;
+        }
+    }
+    if (!may_alloc) return;
+    AbsTable t;
+    t.freq = freq;
+    t.sph = sph;
+    std::vector<double> geo;
+    if (freq > 0.0 && abs_layout(a.z, geo, t.per)) {
+        const int nrec = (int)(geo.size() / ABS_GEO);
+        const AbsBasis B = abs_basis();
+        double *d_geo = nullptr, *d_err = nullptr;
+        hipStream_t st = nullptr;
+        std::vector<double> err(nrec, 1.0);
+        bool good = hipMalloc(&t.d, (size_t)nrec * ABS_REC * sizeof(double)) == hipSuccess &&
+                    hipMalloc(&d_geo, geo.size() * sizeof(double)) == hipSuccess &&
+                    hipMalloc(&d_err, (size_t)nrec * sizeof(double)) == hipSuccess &&
+                    hipStreamCreate(&st) == hipSuccess &&
+                    hipMemcpyAsync(d_geo, geo.data(), geo.size() * sizeof(double), hipMemcpyHostToDevice, st) == hipSuccess;
+        if (good) {
+            Medium Mf = M;
+            auto kern = sph ? abs_fill<true> : abs_fill<false>;
+            hipLaunchKernelGGL(kern, dim3((nrec + 63) / 64), dim3(64), 0, st, Mf, B, d_geo, nrec, freq, t.d, d_err);
+            good = hipGetLastError() == hipSuccess &&
+                   hipMemcpyAsync(err.data(), d_err, (size_t)nrec * sizeof(double), hipMemcpyDeviceToHost, st) == hipSuccess &&
+                   hipStreamSynchronize(st) == hipSuccess;
+        }
+        if (st) (void)hipStreamDestroy(st);
+        if (d_geo) (void)hipFree(d_geo);
+        if (d_err) (void)hipFree(d_err);
+        double worst = 0.0;
+        for (double e : err) worst = std::max(worst, e);
+        t.ok = good && worst <= ABS_TOL;
+        if (!t.ok && t.d) {
+            (void)hipFree(t.d);
+            t.d = nullptr;
+        }
+        (void)hipGetLastError();
+    }
+    a.abs.push_back(t);
+    if (t.ok) {
+        M.abs = t.d;
+        M.abs_per = t.per;
+    }
+}
+
+bool build_medium(ga_ctx *ctx, DevAtmo &a, const ga_prop_params *p, Medium &M, Limits &L, int &rc,
+                  bool may_alloc = true)
 {
     const bool sph = p->geometry == GA_GEOM_SPH;
     M.tab = a.d_tab;
@@ -544,6 +639,7 @@ bool build_medium(ga_ctx *ctx, const DevAtmo &a, const ga_prop_params *p, Medium
         rc = fail(ctx, GA_E_ARG, "invalid step parameters ds_min=%g ds_max=%g s_max=%g", p->ds_min, p->ds_max, p->s_max);
         return false;
     }
+    abs_attach(ctx, a, sph, p->freq, may_alloc, M);
     return true;
 }
 
@@ -1087,7 +1183,7 @@ int fan_enqueue(ga_ctx *ctx, FanBufs &fb, int atmo_id, const ga_prop_params *p, 
     FanArgs A;
     std::memset(&A, 0, sizeof(A));
     int rc = GA_OK;
-    if (!build_medium(ctx, it->second, p, A.M, A.L, rc)) return rc;
+    if (!build_medium(ctx, it->second, p, A.M, A.L, rc, !fb.fixed)) return rc;
     const bool sph = p->geometry == GA_GEOM_SPH;
     const DevAtmo &a = it->second;
     // source position: z_src = max(z_src, ground) (main.infraga.sph.cpp:314, 3d.cpp:306)
@@ -1540,7 +1636,7 @@ struct EigGpu {
 // Medium, limits and search set-up of an eigenray search (shared by
 // ga_eig_search and the request server).  rcvr: receiver pairs in the
 // caller's units, converted in place to the search's (sph: radians).
-int eig_prepare(ga_ctx *ctx, const DevAtmo &a, const ga_prop_params *p, const ga_eig_params *e, FanArgs &F,
+int eig_prepare(ga_ctx *ctx, DevAtmo &a, const ga_prop_params *p, const ga_eig_params *e, FanArgs &F,
                 EigSetup &S, std::vector<double> &rcvr)
 {
     const bool sph = p->geometry == GA_GEOM_SPH;
@@ -2132,6 +2228,7 @@ void ga_ctx_destroy(ga_ctx *ctx)
     for (auto &kv : ctx->atmos) {
         (void)hipFree(kv.second.d_tab);
         if (kv.second.d_topo2d) (void)hipFree(kv.second.d_topo2d);
+        kv.second.release_abs();
     }
     for (auto &kv : ctx->rds) kv.second.release();
     ctx->fb.release();
@@ -2222,6 +2319,7 @@ int ga_atmo_release(ga_ctx *ctx, int atmo_id)
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     HIPCHK(ctx, hipFree(it->second.d_tab));
     if (it->second.d_topo2d) HIPCHK(ctx, hipFree(it->second.d_topo2d));
+    it->second.release_abs();
     ctx->atmos.erase(it);
     return GA_OK;
 }
