@@ -46,9 +46,8 @@ def test_fan_parity(engine, oracle, pert_units, name, geom, calc_amp, incl, az, 
     print(summ)
     assert len(ref) > 0
     assert not fails, fails
-    # the step sequence is the same, so the ray-step counts agree up to the
-    # rays that end one step apart at a threshold
-    assert abs(st.ray_steps - steps) <= 0.001 * steps
+    # the step sequence is the same, so the ray-step counts are identical
+    assert st.ray_steps == steps
 
 
 def test_empty_fan(engine, pert_units):
