@@ -186,6 +186,26 @@ int ga_atmo_load_file(ga_ctx *ctx, const char *path, const char *format, int rev
 
 int ga_atmo_release(ga_ctx *ctx, int atmo_id);
 
+/* A stratified profile over terrain, as infraga-sph runs it with topo_file=
+ * (set_region(atmo_file, topo_file, format, invert_winds),
+ * src/atmo/atmo_io.sph.strat.cpp:147-200, and the stratified medium with
+ * terrain of src/atmo/atmo_state.sph.strat.cpp:37-330).  The profile is
+ * held as 4 x 4 identical columns spanning the terrain grid, over which the
+ * range-dependent splines reduce to the stratified ones exactly, with the
+ * stratified sources' boundary-layer top (terrain maximum + 0.01 km, then
+ * + 2 km) and the stratified driver's arrival inclination sign; the
+ * returned id is a range-dependent id: run it with p->geometry =
+ * GA_GEOM_SPH_RNGDEP (region, source and ground from the terrain as the
+ * stratified driver sets them).  Terrain heights [nlat][nlon] in km,
+ * nodes in degrees.  ga_atmo_load_file_topo reads the reference's files: a
+ * .met profile (formats as ga_atmo_load_file) and "lat lon z" terrain rows,
+ * longitude fastest.  Release with ga_rngdep_release. */
+int ga_atmo_load_strat_topo(ga_ctx *ctx, const double *z_km, const double *c_kms, const double *u_kms,
+                            const double *v_kms, const double *rho_gcc, int n, const double *lat_deg, int nlat,
+                            const double *lon_deg, int nlon, const double *zg_km, int *atmo_id);
+int ga_atmo_load_file_topo(ga_ctx *ctx, const char *path, const char *topo_path, const char *format,
+                           int reverse_winds, int *atmo_id);
+
 /* Trace a fan.  launches holds n_launch (incl_deg, az_deg) pairs.  Arrivals
  * are returned sorted by (launch_idx, n_bounce).  When out_cap is too small
  * the call returns GA_E_CAP with *n_out set to the count needed. */

@@ -359,6 +359,7 @@ struct DevRd {
     std::vector<double> thx, thy;      // terrain axes [rad]
     double t_max = 0.0;                // terrain maximum, floored at 0 (set_bndlyr)
     bool cart = false;                 // 3D Cartesian grid: node axes in km
+    bool strat = false;                // a stratified profile over terrain (ga_atmo_load_strat_topo)
     void release()
     {
         for (double *p : {d_axes, d_tab, d_taxes, d_ttab})
@@ -887,8 +888,15 @@ bool rd_region(const DevRd &R, double z_grnd, RdRegion &G)
         G.lon_min = std::max(G.lon_min, R.thy.front() + buffer);
         G.lon_max = std::min(G.lon_max, R.thy.back() - buffer);
         G.z_max = R.t_max;
-        G.z_bndlyr = G.z_max + 2.0;
-        G.z_max += 0.01;
+        if (R.strat) {
+            // the stratified set_bndlyr raises z_max before adding 2 km
+            // (atmo_state.sph.strat.cpp:42-54; atmo_state.3d.strat.cpp:36-48: +0.1)
+            G.z_max += R.cart ? 0.1 : 0.01;
+            G.z_bndlyr = G.z_max + 2.0;
+        } else {
+            G.z_bndlyr = G.z_max + 2.0;
+            G.z_max += 0.01;
+        }
         return !(G.lat_max < G.lat_min || G.lon_max < G.lon_min);
     }
     if (!std::isnan(z_grnd)) G.z0 = z_grnd;
@@ -923,6 +931,7 @@ int rd_setup(ga_ctx *ctx, FanBufs &fb, int rd_id, const ga_prop_params *p, RdFan
     M.t = topo_grid_of(R);
     M.is_topo = R.d_ttab ? 1 : 0;
     M.use_blw = p->topo_use_blw ? 1 : 0;
+    M.strat = R.strat ? 1 : 0;
     M.r0 = cart ? 0.0 : p->r_earth;
     M.tweak = std::max(0.0, p->abs_coeff);
     M.sbc[0] = std::pow(10.0, -0.67887);
@@ -2281,13 +2290,14 @@ int ga_atmo_load_strat(ga_ctx *ctx, const double *z, const double *c, const doub
     return GA_OK;
 }
 
-int ga_atmo_load_file(ga_ctx *ctx, const char *path, const char *format, int reverse_winds, int *atmo_id)
+// A .met profile in the reference loader's formats and conversions
+// (atmo_io.sph.strat.cpp:61-145): z [km], c [km/s], u, v [km/s], rho.
+static int read_met(ga_ctx *ctx, const char *path, const char *format, int reverse_winds, std::vector<double> &z,
+                    std::vector<double> &c, std::vector<double> &u, std::vector<double> &v, std::vector<double> &rho)
 {
-    if (!ctx || !path || !atmo_id) return fail(ctx, GA_E_ARG, "null argument");
     std::string fmt = format ? format : "zTuvdp";
     std::ifstream in(path);
     if (!in.is_open()) return fail(ctx, GA_E_IO, "cannot open atmosphere file %s", path);
-    std::vector<double> z, c, u, v, rho;
     std::string line;
     while (std::getline(in, line)) {
         if (line.find('#') == 0) continue;
@@ -2307,6 +2317,15 @@ int ga_atmo_load_file(ga_ctx *ctx, const char *path, const char *format, int rev
         else { uu /= 1000.0; vv /= 1000.0; }
         z.push_back(zz); c.push_back(cc); u.push_back(uu); v.push_back(vv); rho.push_back(rr);
     }
+    return GA_OK;
+}
+
+int ga_atmo_load_file(ga_ctx *ctx, const char *path, const char *format, int reverse_winds, int *atmo_id)
+{
+    if (!ctx || !path || !atmo_id) return fail(ctx, GA_E_ARG, "null argument");
+    std::vector<double> z, c, u, v, rho;
+    int rc = read_met(ctx, path, format, reverse_winds, z, c, u, v, rho);
+    if (rc != GA_OK) return rc;
     return ga_atmo_load_strat(ctx, z.data(), c.data(), u.data(), v.data(), rho.data(), (int)z.size(), atmo_id);
 }
 
@@ -2900,6 +2919,24 @@ static bool file_2d_dims(const std::string &path, int &n1, int &n2)
     return true;
 }
 
+// A terrain file of "x y z" rows, y fastest (file_2d_dims and the loops of
+// set_region, atmo_io.sph.rngdep.cpp:203-230, atmo_io.sph.strat.cpp:147-175).
+static int read_terrain(ga_ctx *ctx, const char *path, std::vector<double> &tl, std::vector<double> &to,
+                        std::vector<double> &tz)
+{
+    int n1 = 0, n2 = 0;
+    if (!file_2d_dims(path, n1, n2) || n1 < 2 || n2 < 2) return fail(ctx, GA_E_IO, "invalid terrain file (%s)", path);
+    tl.assign(n1, 0.0);
+    to.assign(n2, 0.0);
+    tz.assign((size_t)n1 * n2, 0.0);
+    std::ifstream in(path);
+    for (int i = 0; i < n1; ++i)
+        for (int j = 0; j < n2; ++j)
+            if (!(in >> tl[i] >> to[j] >> tz[(size_t)i * n2 + j]))
+                return fail(ctx, GA_E_IO, "cannot read terrain row %d in %s", i * n2 + j, path);
+    return GA_OK;
+}
+
 static int rngdep_load_files(ga_ctx *ctx, const char *prefix, const char *lat_path, const char *lon_path,
                              const char *topo_path, const char *format, int reverse_winds, int *rd_id, bool cart)
 {
@@ -2952,22 +2989,9 @@ static int rngdep_load_files(ga_ctx *ctx, const char *prefix, const char *lat_pa
     int rc = rngdep_load(ctx, lat.data(), nlat, lon.data(), nlon, z.data(), nz, c.data(), u.data(), v.data(),
                             rho.data(), rd_id, cart);
     if (rc != GA_OK || !topo_path || !topo_path[0]) return rc;
-    int n1 = 0, n2 = 0;
-    if (!file_2d_dims(topo_path, n1, n2) || n1 < 2 || n2 < 2) {
-        ga_rngdep_release(ctx, *rd_id);
-        return fail(ctx, GA_E_IO, "invalid terrain file (%s)", topo_path);
-    }
-    std::vector<double> tl(n1), to(n2), tz((size_t)n1 * n2);
-    {
-        std::ifstream in(topo_path);
-        for (int i = 0; i < n1; ++i)
-            for (int j = 0; j < n2; ++j)
-                if (!(in >> tl[i] >> to[j] >> tz[(size_t)i * n2 + j])) {
-                    ga_rngdep_release(ctx, *rd_id);
-                    return fail(ctx, GA_E_IO, "cannot read terrain row %d in %s", i * n2 + j, topo_path);
-                }
-    }
-    rc = ga_topo_load(ctx, *rd_id, tl.data(), n1, to.data(), n2, tz.data());
+    std::vector<double> tl, to, tz;
+    rc = read_terrain(ctx, topo_path, tl, to, tz);
+    if (rc == GA_OK) rc = ga_topo_load(ctx, *rd_id, tl.data(), (int)tl.size(), to.data(), (int)to.size(), tz.data());
     if (rc != GA_OK) ga_rngdep_release(ctx, *rd_id);
     return rc;
 }
@@ -2984,6 +3008,66 @@ int ga_rngdep_load_files_xy(ga_ctx *ctx, const char *prefix, const char *x_path,
                             const char *topo_path, const char *format, int reverse_winds, int *rd_id)
 {
     return rngdep_load_files(ctx, prefix, x_path, y_path, topo_path, format, reverse_winds, rd_id, true);
+}
+
+// A stratified profile over terrain: 4 x 4 identical columns whose grid spans
+// the terrain grid exactly (node spacing as numpy's linspace), the terrain,
+// and the stratified binaries' semantics (DevRd::strat).  Over identical
+// columns the hybrid splines give the 1D splines' values, so the
+// range-dependent kernels run the stratified medium with terrain.
+static int strat_topo_build(ga_ctx *ctx, const double *z, const double *c, const double *u, const double *v,
+                            const double *rho, int nz, const double *tl, int n1, const double *to, int n2,
+                            const double *tz, int *atmo_id)
+{
+    if (!ctx || !z || !c || !u || !v || !rho || !tl || !to || !tz || !atmo_id) return fail(ctx, GA_E_ARG, "null argument");
+    if (nz < 3) return fail(ctx, GA_E_ARG, "profile needs >= 3 levels");
+    if (n1 < 2 || n2 < 2) return fail(ctx, GA_E_ARG, "topography needs >= 2 x 2 nodes");
+    constexpr int NC = 4;
+    double la[NC], lo[NC];
+    const double sa = (tl[n1 - 1] - tl[0]) / (NC - 1), so = (to[n2 - 1] - to[0]) / (NC - 1);
+    for (int i = 0; i < NC; ++i) {
+        la[i] = (i == NC - 1) ? tl[n1 - 1] : i * sa + tl[0];
+        lo[i] = (i == NC - 1) ? to[n2 - 1] : i * so + to[0];
+    }
+    const size_t N = (size_t)NC * NC * nz;
+    std::vector<double> cc(N), uu(N), vv(N), rr(N);
+    for (size_t q = 0; q < (size_t)NC * NC; ++q)
+        for (int k = 0; k < nz; ++k) {
+            cc[q * nz + k] = c[k];
+            uu[q * nz + k] = u[k];
+            vv[q * nz + k] = v[k];
+            rr[q * nz + k] = rho[k];
+        }
+    int id = 0;
+    int rc = rngdep_load(ctx, la, NC, lo, NC, z, nz, cc.data(), uu.data(), vv.data(), rr.data(), &id, false);
+    if (rc != GA_OK) return rc;
+    rc = ga_topo_load(ctx, id, tl, n1, to, n2, tz);
+    if (rc != GA_OK) {
+        ga_rngdep_release(ctx, id);
+        return rc;
+    }
+    ctx->rds[id].strat = true;
+    *atmo_id = id;
+    return GA_OK;
+}
+
+int ga_atmo_load_strat_topo(ga_ctx *ctx, const double *z_km, const double *c_kms, const double *u_kms,
+                            const double *v_kms, const double *rho_gcc, int n, const double *lat_deg, int nlat,
+                            const double *lon_deg, int nlon, const double *zg_km, int *atmo_id)
+{
+    return strat_topo_build(ctx, z_km, c_kms, u_kms, v_kms, rho_gcc, n, lat_deg, nlat, lon_deg, nlon, zg_km, atmo_id);
+}
+
+int ga_atmo_load_file_topo(ga_ctx *ctx, const char *path, const char *topo_path, const char *format,
+                           int reverse_winds, int *atmo_id)
+{
+    if (!ctx || !path || !topo_path || !atmo_id) return fail(ctx, GA_E_ARG, "null argument");
+    std::vector<double> z, c, u, v, rho, tl, to, tz;
+    int rc = read_met(ctx, path, format, reverse_winds, z, c, u, v, rho);
+    if (rc == GA_OK) rc = read_terrain(ctx, topo_path, tl, to, tz);
+    if (rc != GA_OK) return rc;
+    return strat_topo_build(ctx, z.data(), c.data(), u.data(), v.data(), rho.data(), (int)z.size(), tl.data(),
+                            (int)tl.size(), to.data(), (int)to.size(), tz.data(), atmo_id);
 }
 
 int ga_rngdep_eval(ga_ctx *ctx, int rd_id, const double *pts, int n, double *atmo_out, double *topo_out)

@@ -147,7 +147,9 @@ GA_HD void rd_eig_lane_loop(const RdEigArgs &E, int lane, NextJob next_job, V ya
 
 // Arrival row of an accepted eigenray (geoac.eigenray.sph.cpp:273-319):
 // the inclination's divisor is atmo::c at the source altitude read as a
-// radius (the bottom of the grid at the source's latitude and longitude).
+// radius (the bottom of the grid at the source's latitude and longitude);
+// inclination and back azimuth are the eigenray source's expressions for
+// either spherical binary.
 GA_HD ga_arrival rd_eig_arrival_row(const RdEigArgs &E, const EigJob &J, const double *r)
 {
     const RdFanArgs &A = E.F;
@@ -162,6 +164,10 @@ GA_HD ga_arrival rd_eig_arrival_row(const RdEigArgs &E, const EigJob &J, const d
     rd_point(A.M, y[0], y[1], y[2], P);
     rd_point(A.M, A.src0 - A.M.r0, A.src1, A.src2, S);
     R.incl_arr = -asin(P.c / S.c * y[3]) * 180.0 / PI;
+    double baz = 90.0 - atan2(-y[4], -y[5]) * (180.0 / PI);   // :283-289, whatever the driver
+    while (baz < -180.0) baz += 360.0;
+    while (baz > 180.0) baz -= 360.0;
+    R.back_az = baz;
     return R;
 }
 

@@ -43,6 +43,8 @@ struct RdMed {
     double zflat;       // ground height without terrain: z0 (sph), 0 (3D Cartesian)
     double tweak;
     double sbc[3];
+    int strat;          // a stratified profile over terrain (identical columns):
+                        // the stratified driver's arrival inclination sign
 };
 
 struct RdFanArgs {
@@ -810,8 +812,15 @@ GA_HD ga_arrival rd_arrival_row(const RdFanArgs &A, const double *y, int ray, in
     RdPt P, S;
     rd_point(A.M, y[0], y[1], y[2], P);
     rd_point(A.M, A.src0, A.src1, A.src2, S);   // atmo::c(r0 + z_src, lat_src, lon_src)
-    R.incl_arr = asin(P.c / S.c * y[3]) * 180.0 / PI;
-    double baz = 90.0 - atan2(-y[4], -y[5]) * 180.0 / PI;
+    double baz;
+    if (A.M.strat) {
+        // main.infraga.sph.cpp:523-524
+        R.incl_arr = -asin(P.c / S.c * y[3]) * (180.0 / PI);
+        baz = 90.0 - atan2(-y[4], -y[5]) * (180.0 / PI);
+    } else {
+        R.incl_arr = asin(P.c / S.c * y[3]) * 180.0 / PI;
+        baz = 90.0 - atan2(-y[4], -y[5]) * 180.0 / PI;
+    }
     if (baz < -180.0) baz += 360.0;
     else if (baz > 180.0) baz -= 360.0;
     R.back_az = baz;

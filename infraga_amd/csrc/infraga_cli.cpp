@@ -12,8 +12,7 @@
 //
 // Differences from the reference drivers, by design:
 //  - an unknown key is an error (exit 2) instead of an interactive prompt;
-//  - write_topo is rejected, and so are topo_file / topo_use_BLw outside the
-//    range-dependent binary;
+//  - write_topo is rejected, and so is topo_file in the 3D stratified binary;
 //  - verbose=true prints no trace-by-trace log.
 #include <cmath>
 #include <cstdio>
@@ -215,6 +214,56 @@ int fan_with_paths(ga_ctx *ctx, int aid, const ga_prop_params &p, const std::vec
     return GA_OK;
 }
 
+// .arrivals.dat of the stratified drivers (main.infraga.sph.cpp:354-384 and
+// 468-556, main.infraga.3d.cpp): header, then one row per arrival and a blank
+// line after each azimuth; z0 for flat ground, topo_file with terrain.
+void write_arrivals(const std::string &id, const char *profile, double theta_min, double theta_max, double theta_step,
+                    double phi_min, double phi_max, double phi_step, int bounces, double s0, double s1, double z_src,
+                    const double *z0, const char *topo_file, double turn_ht_min, double freq, double abs_coeff,
+                    bool sph, const std::vector<ga_arrival> &arr, int n_out, int n, const std::vector<int> &az_end)
+{
+    std::ofstream results(id + ".arrivals.dat");
+    results << "# " << GA_CLI_NAME << " -prop summary:" << '\n';
+    results << "#" << '\t' << "profile: " << profile << '\n';
+    results << "#" << '\t' << "inclination: " << theta_min << ", " << theta_max << ", " << theta_step << '\n';
+    results << "#" << '\t' << "azimuth: " << phi_min << ", " << phi_max << ", " << phi_step << '\n';
+    results << "#" << '\t' << "bounces: " << bounces << '\n';
+    if (sph) results << "#" << '\t' << "source location (lat, lon, alt): " << s0 << ", " << s1 << ", " << z_src << '\n';
+    else results << "#" << '\t' << "source location: " << s0 << ", " << s1 << ", " << z_src << '\n';
+    if (topo_file) results << "#" << '\t' << "topo file:" << topo_file << '\n';
+    else results << "#" << '\t' << "ground elevation: " << *z0 << '\n';
+    if (sph) results << "#" << '\t' << "turn_ht_min: " << turn_ht_min << '\n';
+    results << "#" << '\t' << "frequency: " << freq << '\n';
+    results << "#" << '\t' << "S&B atten coeff: " << abs_coeff << '\n' << '\n';
+    results << "# incl [deg]" << '\t' << "az [deg]" << '\t' << "n_b";
+    if (sph) results << '\t' << "lat_0 [deg]" << '\t' << "lon_0 [deg]";
+    else results << '\t' << "x_0 [km]" << '\t' << "y_0 [km]";
+    results << '\t' << "time [s]" << '\t' << "cel [km/s]" << '\t' << "turning ht [km]";
+    if (sph) results << '\t' << "inclination [deg]" << '\t' << "back azimuth [deg]";
+    else results << '\t' << "arrival incl [deg]" << '\t' << "back az [deg]";
+    results << '\t' << "trans. coeff. [dB]" << '\t' << "absorption [dB]" << '\n';
+
+    int w = 0, az_i = 0;
+    for (int r = 0; r < n; ++r) {
+        while (w < n_out && arr[w].launch_idx == r) {
+            const ga_arrival &A = arr[w++];
+            results << A.incl;
+            results << '\t' << A.az;
+            results << '\t' << A.n_bounce;
+            if (sph) {
+                results << '\t' << std::setprecision(8) << A.pos0;
+                results << '\t' << std::setprecision(8) << A.pos1;
+            } else {
+                results << '\t' << A.pos0;
+                results << '\t' << A.pos1;
+            }
+            results << '\t' << A.time << '\t' << A.cel << '\t' << A.turn_ht << '\t' << A.incl_arr << '\t' << A.back_az;
+            results << '\t' << A.tl << '\t' << A.absorption << '\n';
+        }
+        if (r + 1 == az_end[az_i]) { results << '\n'; ++az_i; }
+    }
+}
+
 int run_prop(int argc, char **argv)
 {
     const bool sph = GA_CLI_GEOM == GA_GEOM_SPH;
@@ -229,6 +278,8 @@ int run_prop(int argc, char **argv)
     ga_prop_params p;
     ga_prop_params_default(&p, GA_CLI_GEOM);
     double z_grnd = NAN;
+    const char *topo_file = nullptr;
+    bool src_set[2] = {false, false}, bounds_set[4] = {false, false, false, false};
 
     for (int i = 3; i < argc; i++) {
         const char *a = argv[i], *v = nullptr;
@@ -242,8 +293,8 @@ int run_prop(int argc, char **argv)
         else if (key(a, "azimuth=", v)) { phi_min = phi_max = std::atof(v); phi_step = 1.0; }
         else if (key(a, "bounces=", v)) bounces = std::atoi(v);
         else if (key2(a, "src_alt=", "alt_src=", v)) z_src = std::atof(v);
-        else if (sph && key2(a, "src_lat=", "lat_src=", v)) s0 = std::atof(v);
-        else if (sph && key2(a, "src_lon=", "lon_src=", v)) s1 = std::atof(v);
+        else if (sph && key2(a, "src_lat=", "lat_src=", v)) { s0 = std::atof(v); src_set[0] = true; }
+        else if (sph && key2(a, "src_lon=", "lon_src=", v)) { s1 = std::atof(v); src_set[1] = true; }
         else if (!sph && key2(a, "src_x=", "x_src=", v)) s0 = std::atof(v);
         else if (!sph && key2(a, "src_y=", "y_src=", v)) s1 = std::atof(v);
         else if (key(a, "turn_ht_min=", v)) turn_ht_min = std::atof(v);
@@ -254,10 +305,10 @@ int run_prop(int argc, char **argv)
         else if (key2(a, "max_alt=", "alt_max=", v)) p.alt_max = std::atof(v);
         else if (key2(a, "max_rng=", "rng_max=", v)) p.rng_max = std::atof(v);
         else if (sph && key2(a, "max_time=", "time_max=", v)) p.time_max = std::atof(v);
-        else if (sph && key2(a, "min_lat=", "lat_min=", v)) p.bounds[0] = std::atof(v);
-        else if (sph && key2(a, "max_lat=", "lat_max=", v)) p.bounds[1] = std::atof(v);
-        else if (sph && key2(a, "min_lon=", "lon_min=", v)) p.bounds[2] = std::atof(v);
-        else if (sph && key2(a, "max_lon=", "lon_max=", v)) p.bounds[3] = std::atof(v);
+        else if (sph && key2(a, "min_lat=", "lat_min=", v)) { p.bounds[0] = std::atof(v); bounds_set[0] = true; }
+        else if (sph && key2(a, "max_lat=", "lat_max=", v)) { p.bounds[1] = std::atof(v); bounds_set[1] = true; }
+        else if (sph && key2(a, "min_lon=", "lon_min=", v)) { p.bounds[2] = std::atof(v); bounds_set[2] = true; }
+        else if (sph && key2(a, "max_lon=", "lon_max=", v)) { p.bounds[3] = std::atof(v); bounds_set[3] = true; }
         else if (!sph && key2(a, "min_x=", "x_min=", v)) p.bounds[0] = std::atof(v);
         else if (!sph && key2(a, "max_x=", "x_max=", v)) p.bounds[1] = std::atof(v);
         else if (!sph && key2(a, "min_y=", "y_min=", v)) p.bounds[2] = std::atof(v);
@@ -272,8 +323,10 @@ int run_prop(int argc, char **argv)
         else if (key(a, "z_grnd=", v)) z_grnd = std::atof(v);
         else if (key(a, "print_resid=", v)) { /* diagnostics only */ }
         else if (key(a, "write_caustics=", v)) write_caustics = str2bool(v);
-        else if (key(a, "write_topo=", v) || key(a, "topo_file=", v) || key(a, "topo_use_BLw=", v)) {
-            if (std::strcmp(v, "false") != 0 && std::strcmp(v, "False") != 0 && std::strcmp(v, "0") != 0) {
+        else if (key(a, "topo_file=", v)) topo_file = v;
+        else if (key(a, "topo_use_BLw=", v)) p.topo_use_blw = str2bool(v) ? 1 : 0;
+        else if (key(a, "write_topo=", v)) {
+            if (str2bool(v)) {
                 std::cerr << GA_CLI_NAME << ": " << a << " is not supported by this engine\n";
                 return 2;
             }
@@ -284,9 +337,79 @@ int run_prop(int argc, char **argv)
     }
 
     if (write_caustics) p.calc_amp = 1;   // caustics need the Jacobian (main.infraga.sph.cpp:315, 3d.cpp:308)
+    if (topo_file && !sph) {
+        std::cerr << GA_CLI_NAME << ": topo_file is not supported by this engine's 3D binary\n";
+        return 2;
+    }
+    if (!topo_file) p.topo_use_blw = 0;   // the vertical wind needs terrain (atmo::w)
     ga_ctx *ctx = nullptr;
     if (ga_ctx_create(&ctx, 0) != GA_OK) return die(nullptr, "no usable GPU");
     int aid = 0;
+    if (topo_file) {
+        // the stratified profile over terrain (set_region with topo_file,
+        // main.infraga.sph.cpp:231-240): the range-dependent kernels over
+        // identical columns with the stratified semantics
+        if (ga_atmo_load_file_topo(ctx, argv[2], topo_file, prof_format, reverse_winds ? 1 : 0, &aid) != GA_OK)
+            return die(ctx, "cannot load the atmosphere and terrain");
+        ga_prop_params q;
+        ga_prop_params_default(&q, GA_GEOM_SPH_RNGDEP);
+        q.calc_amp = p.calc_amp;
+        q.abs_coeff = p.abs_coeff;
+        q.alt_max = p.alt_max;
+        q.rng_max = p.rng_max;
+        q.time_max = p.time_max;
+        q.ds_min = p.ds_min;
+        q.ds_max = p.ds_max;
+        q.s_max = p.s_max;
+        q.topo_use_blw = p.topo_use_blw;
+        for (int k = 0; k < 4; ++k) q.bounds[k] = bounds_set[k] ? p.bounds[k] : NAN;
+        q.bounces = bounces;
+        q.freq = freq;
+        q.turn_ht_min = turn_ht_min;
+        double info[8];
+        if (ga_rngdep_info(ctx, aid, info) != GA_OK) return die(ctx, "region");
+        // source at the terrain's centre unless given (:236-239)
+        if (!src_set[0]) s0 = (info[1] + info[0]) / 2.0 * (180.0 / Pi);
+        if (!src_set[1]) s1 = (info[3] + info[2]) / 2.0 * (180.0 / Pi);
+        const double la = s0 * (Pi / 180.0), lo = s1 * (Pi / 180.0);
+        double pt[3] = {la, lo, 0.0}, atmo_out[40], topo_out[10];
+        if (ga_rngdep_eval(ctx, aid, pt, 1, atmo_out, topo_out) != GA_OK) return die(ctx, "terrain at the source");
+        const double zg = topo_out[0];
+        z_src = std::max(z_src, zg);                                          // :313
+        q.src[0] = s0;
+        q.src[1] = s1;
+        q.src[2] = z_src;
+        std::vector<double> launches;
+        std::vector<int> az_end;
+        const double r0 = q.r_earth;
+        for (double phi = phi_min; phi <= phi_max; phi += phi_step) {
+            double theta_grnd = -89.9;   // :411-423
+            if (z_src - zg < 0.01) {
+                const double dzgdx = 1.0 / (r0 * std::cos(la)) * topo_out[2];
+                const double dzgdy = 1.0 / r0 * topo_out[1];
+                theta_grnd = std::atan(dzgdx * std::sin(phi * Pi / 180.0) + dzgdy * std::cos(phi * Pi / 180.0)) * (180.0 / Pi) + 0.1;
+            }
+            incl_loop(theta_min, theta_max, theta_grnd, theta_step, phi, launches);
+            az_end.push_back((int)(launches.size() / 2));
+        }
+        const int n = (int)(launches.size() / 2);
+        std::vector<ga_arrival> arr((size_t)std::max(1, n) * (bounces + 1));
+        int n_out = 0;
+        ga_run_stats st;
+        std::string id = custom_id ? std::string(output_id) : strip_ext(argv[2]);
+        if (write_rays || write_caustics) {
+            if (fan_with_paths(ctx, aid, q, launches, write_rays, write_caustics, id, true, arr, n_out, st) != GA_OK)
+                return die(ctx, "fan failed");
+        } else if (ga_fan_run(ctx, aid, &q, launches.data(), n, arr.data(), (int)arr.size(), &n_out, &st) != GA_OK) {
+            return die(ctx, "fan failed");
+        }
+        write_arrivals(id, argv[2], theta_min, theta_max, theta_step, phi_min, phi_max, phi_step, bounces, s0, s1,
+                       z_src, nullptr, topo_file, turn_ht_min, freq, q.abs_coeff, true, arr, n_out, n, az_end);
+        std::cout << GA_CLI_NAME << ": " << n << " rays, " << n_out << " arrivals, " << st.ray_steps << " ray-steps, "
+                  << st.kernel_ms << " ms on the GPU\n";
+        ga_ctx_destroy(ctx);
+        return 0;
+    }
     if (ga_atmo_load_file(ctx, argv[2], prof_format, reverse_winds ? 1 : 0, &aid) != GA_OK)
         return die(ctx, "cannot load the atmosphere");
 
@@ -322,46 +445,8 @@ int run_prop(int argc, char **argv)
         return die(ctx, "fan failed");
     }
 
-    std::ofstream results(id + ".arrivals.dat");
-    results << "# " << GA_CLI_NAME << " -prop summary:" << '\n';
-    results << "#" << '\t' << "profile: " << argv[2] << '\n';
-    results << "#" << '\t' << "inclination: " << theta_min << ", " << theta_max << ", " << theta_step << '\n';
-    results << "#" << '\t' << "azimuth: " << phi_min << ", " << phi_max << ", " << phi_step << '\n';
-    results << "#" << '\t' << "bounces: " << bounces << '\n';
-    if (sph) results << "#" << '\t' << "source location (lat, lon, alt): " << s0 << ", " << s1 << ", " << z_src << '\n';
-    else results << "#" << '\t' << "source location: " << s0 << ", " << s1 << ", " << z_src << '\n';
-    results << "#" << '\t' << "ground elevation: " << z0 << '\n';
-    if (sph) results << "#" << '\t' << "turn_ht_min: " << turn_ht_min << '\n';
-    results << "#" << '\t' << "frequency: " << freq << '\n';
-    results << "#" << '\t' << "S&B atten coeff: " << p.abs_coeff << '\n' << '\n';
-    results << "# incl [deg]" << '\t' << "az [deg]" << '\t' << "n_b";
-    if (sph) results << '\t' << "lat_0 [deg]" << '\t' << "lon_0 [deg]";
-    else results << '\t' << "x_0 [km]" << '\t' << "y_0 [km]";
-    results << '\t' << "time [s]" << '\t' << "cel [km/s]" << '\t' << "turning ht [km]";
-    if (sph) results << '\t' << "inclination [deg]" << '\t' << "back azimuth [deg]";
-    else results << '\t' << "arrival incl [deg]" << '\t' << "back az [deg]";
-    results << '\t' << "trans. coeff. [dB]" << '\t' << "absorption [dB]" << '\n';
-
-    int w = 0, az_i = 0;
-    for (int r = 0; r < n; ++r) {
-        while (w < n_out && arr[w].launch_idx == r) {
-            const ga_arrival &A = arr[w++];
-            results << A.incl;
-            results << '\t' << A.az;
-            results << '\t' << A.n_bounce;
-            if (sph) {
-                results << '\t' << std::setprecision(8) << A.pos0;
-                results << '\t' << std::setprecision(8) << A.pos1;
-            } else {
-                results << '\t' << A.pos0;
-                results << '\t' << A.pos1;
-            }
-            results << '\t' << A.time << '\t' << A.cel << '\t' << A.turn_ht << '\t' << A.incl_arr << '\t' << A.back_az;
-            results << '\t' << A.tl << '\t' << A.absorption << '\n';
-        }
-        if (r + 1 == az_end[az_i]) { results << '\n'; ++az_i; }
-    }
-    results.close();
+    write_arrivals(id, argv[2], theta_min, theta_max, theta_step, phi_min, phi_max, phi_step, bounces, s0, s1, z_src,
+                   &z0, nullptr, turn_ht_min, freq, p.abs_coeff, sph, arr, n_out, n, az_end);
     std::cout << GA_CLI_NAME << ": " << n << " rays, " << n_out << " arrivals, " << st.ray_steps << " ray-steps, "
               << st.kernel_ms << " ms on the GPU\n";
     ga_ctx_destroy(ctx);
@@ -585,6 +670,9 @@ int run_eig_search(int argc, char **argv)
     ga_eig_params e;
     ga_eig_params_default(&e);
     double z_grnd = NAN;
+    const char *topo_file = nullptr;
+    bool set[4] = {false, false, false, false};   // src lat, src lon, rcvr lat, rcvr lon given
+    bool bounds_set[4] = {false, false, false, false};
 
     for (int i = 3; i < argc; i++) {
         const char *a = argv[i], *v = nullptr;
@@ -594,10 +682,10 @@ int run_eig_search(int argc, char **argv)
         else if (key2(a, "bnc_max=", "max_bnc=", v)) e.bnc_max = std::atoi(v);
         else if (key(a, "bounces=", v)) e.bnc_min = e.bnc_max = std::atoi(v);
         else if (key2(a, "src_alt=", "alt_src=", v)) z_src = std::atof(v);
-        else if (sph && key2(a, "src_lat=", "lat_src=", v)) s0 = std::atof(v);
-        else if (sph && key2(a, "src_lon=", "lon_src=", v)) s1 = std::atof(v);
-        else if (sph && key2(a, "rcvr_lat=", "lat_rcvr=", v)) r0 = std::atof(v);
-        else if (sph && key2(a, "rcvr_lon=", "lon_rcvr=", v)) r1 = std::atof(v);
+        else if (sph && key2(a, "src_lat=", "lat_src=", v)) { s0 = std::atof(v); set[0] = true; }
+        else if (sph && key2(a, "src_lon=", "lon_src=", v)) { s1 = std::atof(v); set[1] = true; }
+        else if (sph && key2(a, "rcvr_lat=", "lat_rcvr=", v)) { r0 = std::atof(v); set[2] = true; }
+        else if (sph && key2(a, "rcvr_lon=", "lon_rcvr=", v)) { r1 = std::atof(v); set[3] = true; }
         else if (!sph && key2(a, "src_x=", "x_src=", v)) s0 = std::atof(v);
         else if (!sph && key2(a, "src_y=", "y_src=", v)) s1 = std::atof(v);
         else if (!sph && key2(a, "rcvr_x=", "x_rcvr=", v)) r0 = std::atof(v);
@@ -614,10 +702,10 @@ int run_eig_search(int argc, char **argv)
         else if (key2(a, "max_alt=", "alt_max=", v)) p.alt_max = std::atof(v);
         else if (key2(a, "max_rng=", "rng_max=", v)) p.rng_max = std::atof(v);
         else if (sph && key2(a, "max_time=", "time_max=", v)) p.time_max = std::atof(v);
-        else if (sph && key2(a, "min_lat=", "lat_min=", v)) p.bounds[0] = std::atof(v);
-        else if (sph && key2(a, "max_lat=", "lat_max=", v)) p.bounds[1] = std::atof(v);
-        else if (sph && key2(a, "min_lon=", "lon_min=", v)) p.bounds[2] = std::atof(v);
-        else if (sph && key2(a, "max_lon=", "lon_max=", v)) p.bounds[3] = std::atof(v);
+        else if (sph && key2(a, "min_lat=", "lat_min=", v)) { p.bounds[0] = std::atof(v); bounds_set[0] = true; }
+        else if (sph && key2(a, "max_lat=", "lat_max=", v)) { p.bounds[1] = std::atof(v); bounds_set[1] = true; }
+        else if (sph && key2(a, "min_lon=", "lon_min=", v)) { p.bounds[2] = std::atof(v); bounds_set[2] = true; }
+        else if (sph && key2(a, "max_lon=", "lon_max=", v)) { p.bounds[3] = std::atof(v); bounds_set[3] = true; }
         else if (!sph && key2(a, "min_x=", "x_min=", v)) p.bounds[0] = std::atof(v);
         else if (!sph && key2(a, "max_x=", "x_max=", v)) p.bounds[1] = std::atof(v);
         else if (!sph && key2(a, "min_y=", "y_min=", v)) p.bounds[2] = std::atof(v);
@@ -630,26 +718,62 @@ int run_eig_search(int argc, char **argv)
         else if (key(a, "reverse_winds=", v)) reverse_winds = str2bool(v);
         else if (key(a, "output_id=", v)) { custom_id = true; output_id = v; }
         else if (key(a, "z_grnd=", v)) z_grnd = std::atof(v);
-        else if (key(a, "topo_file=", v) || key(a, "topo_use_BLw=", v)) {
-            if (std::strcmp(v, "false") != 0 && std::strcmp(v, "False") != 0 && std::strcmp(v, "0") != 0) {
-                std::cerr << GA_CLI_NAME << ": " << a << " is not supported by this engine\n";
-                return 2;
-            }
-        } else {
+        else if (key(a, "topo_file=", v)) topo_file = v;
+        else if (key(a, "topo_use_BLw=", v)) p.topo_use_blw = str2bool(v) ? 1 : 0;
+        else {
             std::cerr << GA_CLI_NAME << ": unrecognized parameter entry: " << a << '\n';
             return 2;
         }
     }
 
+    if (topo_file && !sph) {
+        std::cerr << GA_CLI_NAME << ": topo_file is not supported by this engine's 3D binary\n";
+        return 2;
+    }
+    if (!topo_file) p.topo_use_blw = 0;   // the vertical wind needs terrain (atmo::w)
     ga_ctx *ctx = nullptr;
     if (ga_ctx_create(&ctx, 0) != GA_OK) return die(nullptr, "no usable GPU");
     int aid = 0;
-    if (ga_atmo_load_file(ctx, argv[2], prof_format, reverse_winds ? 1 : 0, &aid) != GA_OK)
-        return die(ctx, "cannot load the atmosphere");
-    const double z0 = std::isnan(z_grnd) ? profile_bottom(argv[2]) : z_grnd;
-    p.z_grnd = z0;
+    double z0 = 0.0, z_rcvr = 0.0;
+    if (topo_file) {
+        // the stratified profile over terrain (main.infraga.sph.cpp:819-827,
+        // 891-899): source at the terrain's centre, receiver 2.5 deg east of
+        // it, unless given; the search runs over identical columns
+        if (ga_atmo_load_file_topo(ctx, argv[2], topo_file, prof_format, reverse_winds ? 1 : 0, &aid) != GA_OK)
+            return die(ctx, "cannot load the atmosphere and terrain");
+        double info[8];
+        if (ga_rngdep_info(ctx, aid, info) != GA_OK) return die(ctx, "region");
+        const double clat = (info[1] + info[0]) / 2.0 * (180.0 / Pi), clon = (info[3] + info[2]) / 2.0 * (180.0 / Pi);
+        if (!set[0]) s0 = clat;
+        if (!set[1]) s1 = clon;
+        if (!set[2]) r0 = clat + 0.0;
+        if (!set[3]) r1 = clon + 2.5;
+        double pt[6] = {s0 * (Pi / 180.0), s1 * (Pi / 180.0), 0.0, r0 * (Pi / 180.0), r1 * (Pi / 180.0), 0.0};
+        double aout[80], tout[20];
+        if (ga_rngdep_eval(ctx, aid, pt, 2, aout, tout) != GA_OK) return die(ctx, "terrain");
+        z_src = std::max(z_src, tout[0]);
+        z_rcvr = tout[10];
+        ga_prop_params q;
+        ga_prop_params_default(&q, GA_GEOM_SPH_RNGDEP);
+        q.abs_coeff = p.abs_coeff;
+        q.alt_max = p.alt_max;
+        q.rng_max = p.rng_max;
+        q.time_max = p.time_max;
+        q.ds_min = p.ds_min;
+        q.ds_max = p.ds_max;
+        q.s_max = p.s_max;
+        q.topo_use_blw = p.topo_use_blw;
+        for (int k = 0; k < 4; ++k) q.bounds[k] = bounds_set[k] ? p.bounds[k] : NAN;
+        p = q;
+    } else {
+        if (ga_atmo_load_file(ctx, argv[2], prof_format, reverse_winds ? 1 : 0, &aid) != GA_OK)
+            return die(ctx, "cannot load the atmosphere");
+        z0 = std::isnan(z_grnd) ? profile_bottom(argv[2]) : z_grnd;
+        p.z_grnd = z0;
+        z_src = std::max(z_src, z0);
+        z_rcvr = sph ? (p.r_earth + z0) - p.r_earth : z0;
+    }
     p.freq = freq;
-    z_src = std::max(z_src, z0);
     p.src[0] = s0;
     p.src[1] = s1;
     p.src[2] = z_src;
@@ -698,7 +822,7 @@ int run_eig_search(int argc, char **argv)
         out << "# " << '\t' << "source location (lat, lon, alt): " << (s0 * (Pi / 180.0)) * (180.0 / Pi) << ", "
             << (s1 * (Pi / 180.0)) * (180.0 / Pi) << ", " << z_src << '\n';
         out << "# " << '\t' << "receiver location (lat, lon, alt): " << (r0 * (Pi / 180.0)) * (180.0 / Pi) << ", "
-            << (r1 * (Pi / 180.0)) * (180.0 / Pi) << ", " << (p.r_earth + z0) - p.r_earth << '\n';
+            << (r1 * (Pi / 180.0)) * (180.0 / Pi) << ", " << z_rcvr << '\n';
     } else {
         out << "# " << '\t' << "source location: " << s0 << ", " << s1 << ", " << z_src << '\n';
         out << "# " << '\t' << "receiver location: " << r0 << ", " << r1 << ", " << z0 << '\n';
@@ -706,7 +830,8 @@ int run_eig_search(int argc, char **argv)
     out << "# " << '\t' << "inclination range: " << (e.incl_min * (Pi / 180.0)) * (180.0 / Pi) << ", "
         << (e.incl_max * (Pi / 180.0)) * (180.0 / Pi) << '\n';
     out << "# " << '\t' << "bounces: " << e.bnc_min << ", " << e.bnc_max << '\n';
-    out << "#" << '\t' << "ground elevation: " << z0 << '\n';
+    if (topo_file) out << "#" << '\t' << "topo file:" << topo_file << '\n';
+    else out << "#" << '\t' << "ground elevation: " << z0 << '\n';
     out << "# " << '\t' << "damping: " << e.damping << '\n';
     out << "# " << '\t' << "frequency: " << freq << '\n';
     out << "# " << '\t' << "S&B atten coeff: " << p.abs_coeff << '\n';

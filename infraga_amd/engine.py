@@ -29,7 +29,7 @@ EXPORTED = ["ga_prop_params_default", "ga_eig_params_default", "ga_ctx_create", 
             "ga_topo_load", "ga_rngdep_eval",
             "ga_rngdep_release", "ga_rngdep_load_files", "ga_rngdep_info", "ga_fan_run_paths",
             "ga_eig_search_paths", "ga_eig_direct", "ga_topo_load_2d", "ga_topo_eval_2d",
-            "ga_fan_run_device_noalloc", "ga_fan_autotune"]
+            "ga_fan_run_device_noalloc", "ga_fan_autotune", "ga_atmo_load_strat_topo", "ga_atmo_load_file_topo"]
 
 GA_PATH_RAYS, GA_PATH_CAUSTICS = 1, 2
 GA_PT_RAY, GA_PT_CAUSTIC = 0, 1
@@ -120,6 +120,10 @@ def load_library(path: str | None = None):
     L.ga_last_error.restype = C.c_char_p
     L.ga_atmo_load_strat.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, C.c_int, C.POINTER(C.c_int)]
     L.ga_atmo_load_file.argtypes = [C.c_void_p, C.c_char_p, C.c_char_p, C.c_int, C.POINTER(C.c_int)]
+    L.ga_atmo_load_strat_topo.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int,
+                                          _dp, C.POINTER(C.c_int)]
+    L.ga_atmo_load_file_topo.argtypes = [C.c_void_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_int,
+                                         C.POINTER(C.c_int)]
     L.ga_atmo_release.argtypes = [C.c_void_p, C.c_int]
     L.ga_fan_run.argtypes = [C.c_void_p, C.c_int, C.POINTER(PropParams), _dp, C.c_int, C.POINTER(Arrival),
                              C.c_int, C.POINTER(C.c_int), C.POINTER(RunStats)]
@@ -225,6 +229,22 @@ class Engine:
 
     def release_atmo(self, aid):
         self._check(self.L.ga_atmo_release(self.h, aid))
+
+    def load_strat_topo(self, z, c, u, v, rho, lat_deg, lon_deg, zg) -> int:
+        """A stratified profile (engine units) over terrain zg[nlat][nlon] [km]:
+        a range-dependent id to run with GA_GEOM_SPH_RNGDEP (ga_atmo_load_strat_topo)."""
+        a = [np.ascontiguousarray(q, dtype=np.float64) for q in (z, c, u, v, rho, lat_deg, lon_deg, zg)]
+        rid = C.c_int(0)
+        self._check(self.L.ga_atmo_load_strat_topo(self.h, *[q.ctypes.data_as(_dp) for q in a[:5]], len(a[0]),
+                                                   a[5].ctypes.data_as(_dp), len(a[5]), a[6].ctypes.data_as(_dp),
+                                                   len(a[6]), a[7].ctypes.data_as(_dp), C.byref(rid)))
+        return rid.value
+
+    def load_file_topo(self, path, topo_path, fmt="zTuvdp", reverse_winds=False) -> int:
+        rid = C.c_int(0)
+        self._check(self.L.ga_atmo_load_file_topo(self.h, path.encode(), topo_path.encode(), fmt.encode(),
+                                                  int(reverse_winds), C.byref(rid)))
+        return rid.value
 
     def fan(self, aid, params: PropParams, launches):
         """Trace a fan; returns (arrivals structured array, RunStats)."""

@@ -388,6 +388,35 @@ class Ref:
             L.ref_bearing.argtypes = [C.c_double] * 4
             L.ref_bearing.restype = C.c_double
             L.ref_r0.restype = C.c_double
+            L.ref_limits.argtypes = [_dp]
+        L.ref_load_topo.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p]
+        L.ref_load_topo.restype = C.c_int
+        L.ref_use_blw.argtypes = [C.c_int]
+        L.ref_z_max.restype = C.c_double
+        L.ref_topo.argtypes = [C.c_double, C.c_double, _dp]
+        L.ref_w.argtypes = [C.c_double] * 3
+        L.ref_w.restype = C.c_double
+
+    def load_topo(self, path, topo_path, fmt="zcuvd"):
+        """set_region(atmo, topo, format): the stratified profile over terrain."""
+        if self.lib.ref_load_topo(path.encode(), topo_path.encode(), fmt.encode()) != 1:
+            raise RuntimeError(f"reference set_region failed on {path}, {topo_path}")
+        return self.lib.ref_len()
+
+    def limits(self):
+        out = np.zeros(8)
+        self.lib.ref_limits(_ptr(out))
+        return out
+
+    def topo(self, a, b):
+        out = np.zeros(6)
+        self.lib.ref_topo(a, b, _ptr(out))
+        return out
+
+    def calc_uvw(self, a, b, c, second):
+        out = np.zeros(30 if second else 12)
+        self.lib.ref_calc_uvw(a, b, c, int(second), _ptr(out))
+        return out
 
     def load(self, path, fmt="zcuvd"):
         if self.lib.ref_load(path.encode(), fmt.encode()) != 1:
@@ -417,6 +446,7 @@ def _rd_bind(L):
     L.go_rd_set_ground.argtypes = [C.c_void_p, C.c_double]
     L.go_rd_h_config.argtypes = [C.c_void_p, C.c_int, C.c_double]
     L.go_rd_h_limits.argtypes = [C.c_void_p, _dp]
+    L.go_rd_h_set_strat.argtypes = [C.c_void_p, C.c_int]
     L.go_rd_params.argtypes = [C.c_void_p, C.c_int, C.POINTER(Params)]
     L.go_rd_centre.argtypes = [C.c_void_p, _dp, _dp]
     for name in ("go_rd_c", "go_rd_rho", "go_rd_u", "go_rd_v", "go_rd_w", "go_rd_topo_z", "go_rd_bl_sc"):
@@ -523,6 +553,26 @@ class RdMedium:
 
     def config(self, use_blw=False, tweak_abs=1.0):
         self.o.lib.go_rd_h_config(self.h, int(use_blw), tweak_abs)
+
+    def set_strat(self, on=True):
+        """The stratified binaries' semantics over identical columns
+        (go_rd_h_set_strat)."""
+        self.o.lib.go_rd_h_set_strat(self.h, int(on))
+
+    @classmethod
+    def strat_over_terrain(cls, oracle: "Oracle", z, c, u, v, rho, tlat, tlon, zg, n=4):
+        """infraga-sph with topo_file: the stratified profile (engine units)
+        as n x n identical columns spanning the terrain grid's extent, the
+        terrain, and the stratified semantics."""
+        tlat = np.asarray(tlat, dtype=np.float64)
+        tlon = np.asarray(tlon, dtype=np.float64)
+        lat = np.linspace(tlat[0], tlat[-1], n)
+        lon = np.linspace(tlon[0], tlon[-1], n)
+        col = [np.broadcast_to(np.asarray(f, dtype=np.float64), (n, n, len(z))).copy() for f in (c, u, v, rho)]
+        m = cls(oracle, lat, lon, z, *col)
+        m.set_strat(True)
+        m.set_topo(tlat, tlon, zg)
+        return m
 
     def limits(self):
         out = np.zeros(8)

@@ -50,6 +50,13 @@ static void set_bndlyr(go_rd *a)
         a->z_max = 0.0;
         for (int i = 0; i < a->topo.nx; i++)
             for (int j = 0; j < a->topo.ny; j++) a->z_max = dmax(a->z_max, a->topo.f[i * a->topo.ny + j]);
+        if (a->strat) {
+            /* the stratified sources raise z_max first, then add 2 km:
+             * atmo_state.sph.strat.cpp:42-54 (+0.01), atmo_state.3d.strat.cpp:36-48 (+0.1) */
+            a->z_max += a->cart ? 0.1 : 0.01;
+            a->z_bndlyr = a->z_max + 2.0;
+            return;
+        }
         a->z_bndlyr = a->z_max + 2.0;
         a->z_max += 0.01;
     } else {
@@ -1032,9 +1039,17 @@ int go_rd_fan(go_rd *a, const go_params *p, const go_prop *pr, const double *lau
             A.n_bounce = bnc;
             A.incl_launch = incl;
             A.az_launch = az;
-            /* :515, no leading minus sign (the stratified driver has one) */
-            A.incl_arr = asin(go_rd_c(a, yk[0], yk[1], yk[2]) / go_rd_c(a, a->r0 + z_src, lat_src, lon_src) * yk[3]) * 180.0 / RD_PI;
-            double baz = 90.0 - atan2(-yk[4], -yk[5]) * 180.0 / RD_PI;
+            /* :515, no leading minus sign; the stratified driver has one
+             * (main.infraga.sph.cpp:523) */
+            const double sn = asin(go_rd_c(a, yk[0], yk[1], yk[2]) / go_rd_c(a, a->r0 + z_src, lat_src, lon_src) * yk[3]);
+            double baz;
+            if (a->strat) {   /* main.infraga.sph.cpp:523-524 */
+                A.incl_arr = -sn * (180.0 / RD_PI);
+                baz = 90.0 - atan2(-yk[4], -yk[5]) * (180.0 / RD_PI);
+            } else {
+                A.incl_arr = sn * 180.0 / RD_PI;
+                baz = 90.0 - atan2(-yk[4], -yk[5]) * 180.0 / RD_PI;
+            }
             if (baz < -180.0) baz += 360.0;
             else if (baz > 180.0) baz -= 360.0;
             A.back_az = baz;
@@ -1128,8 +1143,15 @@ int go_rd_fan_paths(go_rd *a, const go_params *p, const go_prop *pr, const doubl
             A.n_bounce = bnc;
             A.incl_launch = incl;
             A.az_launch = az;
-            A.incl_arr = asin(go_rd_c(a, yk[0], yk[1], yk[2]) / go_rd_c(a, a->r0 + z_src, lat_src, lon_src) * yk[3]) * 180.0 / RD_PI;
-            double baz = 90.0 - atan2(-yk[4], -yk[5]) * 180.0 / RD_PI;
+            const double sn = asin(go_rd_c(a, yk[0], yk[1], yk[2]) / go_rd_c(a, a->r0 + z_src, lat_src, lon_src) * yk[3]);
+            double baz;
+            if (a->strat) {   /* main.infraga.sph.cpp:523-524 */
+                A.incl_arr = -sn * (180.0 / RD_PI);
+                baz = 90.0 - atan2(-yk[4], -yk[5]) * (180.0 / RD_PI);
+            } else {
+                A.incl_arr = sn * 180.0 / RD_PI;
+                baz = 90.0 - atan2(-yk[4], -yk[5]) * 180.0 / RD_PI;
+            }
             if (baz < -180.0) baz += 360.0;
             else if (baz > 180.0) baz -= 360.0;
             A.back_az = baz;
@@ -1244,6 +1266,18 @@ void go_rd_h_config(void *h, int use_blw, double tweak_abs)
     go_rd *a = (go_rd *)h;
     a->use_blw = use_blw;
     a->tweak_abs = tweak_abs;
+}
+/* A stratified profile over terrain, as infraga-sph / infraga-3d run it
+ * with topo_file (atmo_io.sph.strat.cpp:147-200, atmo_state.sph.strat.cpp):
+ * the same medium as this restatement over identical columns (the
+ * eigenray sources, the reflection and the RHS are shared or identical), but
+ * z_max / z_bndlyr from the stratified set_bndlyr and the stratified
+ * driver's arrival inclination sign. */
+void go_rd_h_set_strat(void *h, int strat)
+{
+    go_rd *a = (go_rd *)h;
+    a->strat = strat != 0;
+    set_bndlyr(a);
 }
 void go_rd_h_limits(void *h, double out[8])
 {

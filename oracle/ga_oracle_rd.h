@@ -31,6 +31,8 @@ typedef struct {
     int is_topo;              /* geoac::is_topo */
     int use_blw;              /* topo::use_BLw (boundary-layer vertical wind) */
     int cart;                 /* 3D Cartesian grid: node axes in km, r0 = 0 */
+    int strat;                /* a stratified profile over terrain (infraga-sph /
+                                 infraga-3d with topo_file): identical columns */
     double r0;                /* globe radius, 6370 km */
     double z0, z_max, z_bndlyr;
     double bl_param;          /* topo::bndlyr_param, 0.1 km */
@@ -148,6 +150,8 @@ void  *go_rd_new(const double *lat_deg, int nlat, const double *lon_deg, int nlo
 void   go_rd_delete(void *h);
 void   go_rd_h_config(void *h, int use_blw, double tweak_abs);
 void   go_rd_h_limits(void *h, double out[8]);   /* lat/lon limits, alt_max, z0, z_max, z_bndlyr */
+/* Stratified-binary semantics over identical columns (see go_rd_h_set_strat). */
+void   go_rd_h_set_strat(void *h, int strat);
 
 #ifdef __cplusplus
 }

@@ -34,6 +34,36 @@ int ref_load(const char *path, const char *fmt)
     return set_region(pbuf, fbuf, false);
 }
 
+// set_region(atmo_file, topo_file, format, invert_winds): the stratified
+// profile over terrain (atmo_io.*.strat.cpp), as the drivers run it with
+// topo_file=.
+int ref_load_topo(const char *path, const char *topo_path, const char *fmt)
+{
+    static char pbuf[4096], tbuf[4096], fbuf[64];
+    std::strncpy(pbuf, path, sizeof(pbuf) - 1);
+    std::strncpy(tbuf, topo_path, sizeof(tbuf) - 1);
+    std::strncpy(fbuf, fmt, sizeof(fbuf) - 1);
+    geoac::is_topo = true;
+    topo::z0 = 0.0;
+    atmo::tweak_abs = 1.0;
+    return set_region(pbuf, tbuf, fbuf, false);
+}
+
+void ref_use_blw(int on) { topo::use_BLw = on != 0; }
+double ref_z_max() { return topo::z_max; }
+
+// out = {z, dz/dx0, dz/dx1, d2z/dx0^2, d2z/dx0dx1, d2z/dx1^2} of topo::z (which
+// includes r0 in the spherical geometry).
+void ref_topo(double a, double b, double *out)
+{
+    out[0] = topo::z(a, b);
+    out[1] = topo::dz(a, b, 0);
+    out[2] = topo::dz(a, b, 1);
+    out[3] = topo::ddz(a, b, 0, 0);
+    out[4] = topo::ddz(a, b, 0, 1);
+    out[5] = topo::ddz(a, b, 1, 1);
+}
+
 void ref_clear() { clear_region(); }
 
 int ref_len() { return atmo::c_spline.length; }
@@ -54,6 +84,7 @@ void ref_spline_eval(int which, double z, double out[3])
 }
 
 double ref_c(double a, double b, double c) { return atmo::c(a, b, c); }
+double ref_w(double a, double b, double c) { return atmo::w(a, b, c); }
 double ref_rho(double a, double b, double c) { return atmo::rho(a, b, c); }
 double ref_u(double a, double b, double c) { return atmo::u(a, b, c); }
 double ref_v(double a, double b, double c) { return atmo::v(a, b, c); }
@@ -81,6 +112,11 @@ double ref_z0() { return topo::z0; }
 double ref_alt_max() { return geoac::alt_max; }
 
 #ifdef REF_SPH
+void ref_limits(double *out)
+{
+    out[0] = geoac::lat_min; out[1] = geoac::lat_max; out[2] = geoac::lon_min; out[3] = geoac::lon_max;
+    out[4] = geoac::alt_max; out[5] = topo::z0; out[6] = topo::z_max; out[7] = topo::z_bndlyr;
+}
 double ref_gc_dist(double lat1, double lon1, double lat2, double lon2) { return globe::gc_dist(lat1, lon1, lat2, lon2); }
 double ref_bearing(double lat1, double lon1, double lat2, double lon2) { return globe::bearing(lat1, lon1, lat2, lon2); }
 double ref_r0() { return globe::r0; }
