@@ -54,8 +54,10 @@ struct EigSetup {
     double rng_max;                // km
     double c_src, c_grnd;          // c at the source and at the ground [km/s] (LM update)
     int spec_chunk;                // sweep traces requested per miss after one azimuth correction
-    int rngdep;                    // sph over a range-dependent medium and terrain: the LM update
-                                   // reads the ground under row K from the trace result
+    int rngdep;                    // a range-dependent medium: the LM update reads the ground
+                                   // under row K (topo::z, topo::dz, atmo::c there) from the
+                                   // trace result
+    int is_topo;                   // geoac::is_topo (the 3D update's terrain-slope form)
 };
 
 // One round of traces.  sweep: EIG_SWEEP jobs; full: EIG_LM jobs followed by
@@ -516,7 +518,7 @@ private:
         FILE *lg = decision_log();
         if (lg) std::fprintf(lg, "E %d %d %d %.17g %.17g\n", ri, nb, seq, th_est, ph_est);
         double dr, dr_prev = 10000.0, step_sc = 1.0;
-        long double x, y, c_src, c_grnd, ds_dth, ds_dph, dx, dy, dx_dth, dy_dth, dx_dph, dy_dph;
+        long double x, y, c_src, c_grnd, dzg_dx, dzg_dy, ds_dth, ds_dph, dx, dy, dx_dth, dy_dth, dx_dph, dy_dph;
         long double det, dth = 0.0L, dph = 0.0L;   // uninitialised in the reference; zero here
         // the iteration trace reflects at most once (the unconditional break
         // at geoac.eigenray.3d.cpp:206-208)
@@ -555,9 +557,20 @@ private:
                 } else {
                     step_sc = std::min(1.0, step_sc * 1.25);
                     c_src = S_.c_src;
-                    c_grnd = S_.c_grnd;
-                    ds_dth = -c_src / c_grnd * yk[8] / yk[5];
-                    ds_dph = -c_src / c_grnd * yk[14] / yk[5];
+                    // atmo::c at topo::z under row K: the trace result over a
+                    // range-dependent medium, the flat ground's otherwise
+                    c_grnd = S_.rngdep ? yk[24] : S_.c_grnd;
+                    if (S_.is_topo) {   // geoac.eigenray.3d.cpp:337-342
+                        dzg_dx = yk[22];
+                        dzg_dy = yk[23];
+                        ds_dth = -c_src / c_grnd * (yk[8] - dzg_dx * yk[6] - dzg_dy * yk[7]) /
+                                 (yk[5] - dzg_dx * yk[3] - dzg_dy * yk[4]);
+                        ds_dph = -c_src / c_grnd * (yk[14] - dzg_dx * yk[12] - dzg_dy * yk[13]) /
+                                 (yk[5] - dzg_dx * yk[3] - dzg_dy * yk[4]);
+                    } else {
+                        ds_dth = -c_src / c_grnd * yk[8] / yk[5];
+                        ds_dph = -c_src / c_grnd * yk[14] / yk[5];
+                    }
                     dx_dth = yk[6] + yk[3] * ds_dth;
                     dy_dth = yk[7] + yk[4] * ds_dth;
                     dx_dph = yk[12] + yk[3] * ds_dph;

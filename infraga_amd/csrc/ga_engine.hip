@@ -252,23 +252,25 @@ __global__ void __launch_bounds__(256) rd_arrival_kernel(RdFanArgs A)
 
 // Terrain height above sea level at one point (the source of a fan).
 // Eigenray traces over the range-dependent medium (same job set as eig_kernel).
-template <int NEQ, bool BLW, bool PATHS>
+template <int NEQ, bool BLW, bool PATHS, bool CART>
 __global__ void __launch_bounds__(GA_WG_RD) rd_eig_kernel(RdEigArgs E)
 {
     const int lane = blockIdx.x * blockDim.x + threadIdx.x;
     __shared__ double rows[3 * NEQ * GA_WG_RD];
     const LaneVec<GA_WG_RD> ya{rows + threadIdx.x}, yb{rows + NEQ * GA_WG_RD + threadIdx.x},
         ys{rows + 2 * NEQ * GA_WG_RD + threadIdx.x};
-    rd_eig_lane_loop<NEQ, BLW, PATHS>(E, lane, [&]() { return (int)atomicAdd(E.F.queue, 1u); }, ya, yb, ys);
+    rd_eig_lane_loop<NEQ, BLW, PATHS, CART>(E, lane, [&]() { return (int)atomicAdd(E.F.queue, 1u); }, ya, yb, ys);
 }
 
 // Arrival rows of the range-dependent EIG_FINAL jobs [first, first + n).
+template <bool CART>
 __global__ void __launch_bounds__(256) rd_eig_arrival_kernel(RdEigArgs E, int first, int n, ga_arrival *out)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const EigJob J = E.jobs[first + i];
-    out[i] = rd_eig_arrival_row(E, J, E.res + (size_t)(first + i) * EIG_FULL);
+    const double *r = E.res + (size_t)(first + i) * EIG_FULL;
+    out[i] = CART ? rd3_eig_arrival_row(E, J, r) : rd_eig_arrival_row(E, J, r);
 }
 
 __global__ void rd_ground_kernel(RdMed M, double lat, double lon, double *out)
@@ -1393,6 +1395,7 @@ struct EigGpu {
     int geom = GA_GEOM_3D;
     FanArgs F{};                 // medium, limits, source, c_src, freq, max_steps, lds
     bool rd = false;             // range-dependent medium: RF instead of F
+    bool cart = false;           // the 3D Cartesian range-dependent medium
     RdFanArgs RF{};
     hipStream_t st = nullptr, s2 = nullptr;
     bool own_st = false;
@@ -1464,7 +1467,8 @@ struct EigGpu {
                   DevBuf<double> &yp, unsigned int *queue, hipStream_t s, const PathOut *path = nullptr)
     {
         const bool blw = RF.M.is_topo && RF.M.use_blw;
-        auto kern = blw ? rd_eig_kernel<NEQ, true, PATHS> : rd_eig_kernel<NEQ, false, PATHS>;
+        auto kern = cart ? rd_eig_kernel<NEQ, false, PATHS, true>
+                         : (blw ? rd_eig_kernel<NEQ, true, PATHS, false> : rd_eig_kernel<NEQ, false, PATHS, false>);
         int per_cu = 0;
         HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, GA_WG_RD, 0));
         per_cu = std::max(per_cu, 1);
@@ -1541,8 +1545,8 @@ struct EigGpu {
                 E.n_jobs = (int)n18;
                 E.stride = EIG_FULL;
                 E.res = res18.p;
-                hipLaunchKernelGGL(rd_eig_arrival_kernel, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, s2, E,
-                                   B.n_lm, (int)nf, arr.p);
+                hipLaunchKernelGGL(cart ? rd_eig_arrival_kernel<true> : rd_eig_arrival_kernel<false>,
+                                   dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, s2, E, B.n_lm, (int)nf, arr.p);
                 HIPCHK(ctx, hipGetLastError());
                 launches += 1;
                 HIPCHK(ctx, hipMemcpyAsync(h_arr.p, arr.p, nf * sizeof(ga_arrival), hipMemcpyDeviceToHost, s2));
@@ -1703,6 +1707,7 @@ int eig_prepare(ga_ctx *ctx, DevAtmo &a, const ga_prop_params *p, const ga_eig_p
     S.c_grnd = host_c_at(a, F.M.z0);
     S.spec_chunk = 64;
     S.rngdep = 0;
+    S.is_topo = 0;
     return GA_OK;
 }
 
@@ -1715,6 +1720,38 @@ int eig_prepare_rd(ga_ctx *ctx, int rd_id, const ga_prop_params *p, const ga_eig
     int rc = rd_setup(ctx, ctx->fb, rd_id, p, RF, ctx->stream);
     if (rc != GA_OK) return rc;
     const RdMed &M = RF.M;
+    S.is_topo = M.is_topo;
+    S.dth_big = e->incl_step_max * (PI / 180.0);
+    S.dth_sml = e->incl_step_min * (PI / 180.0);
+    S.th_min = e->incl_min * (PI / 180.0);
+    S.th_max = e->incl_max * (PI / 180.0);
+    S.bnc_min = e->bnc_min;
+    S.bnc_max = e->bnc_max;
+    S.iterations = e->iterations;
+    S.damping = e->damping;
+    S.tolerance = e->tolerance;
+    S.az_err_lim = e->az_dev_lim;
+    S.rng_max = p->rng_max;
+    S.spec_chunk = 64;
+    S.rngdep = 1;
+    if (p->geometry == GA_GEOM_3D_RNGDEP) {
+        // run_eig_search / run_eig_direct of main.infraga.3d.rngdep.cpp:754-1052:
+        // the 3D search of geoac.eigenray.3d.cpp from theta_min, the source
+        // raised to the terrain, receivers in km
+        S.geom = GA_GEOM_3D;
+        S.r0 = 0.0;
+        S.src_ground = M.zflat;
+        S.src[0] = RF.src1;
+        S.src[1] = RF.src2;
+        S.src[2] = RF.src0;
+        double pt[3] = {RF.src1, RF.src2, RF.src0}, aout[40], tout[10];
+        rc = ga_rngdep_eval(ctx, rd_id, pt, 1, aout, tout);
+        if (rc != GA_OK) return rc;
+        S.c_src = aout[0];             // atmo::c(src[0], src[1], src[2]) (geoac.eigenray.3d.cpp:334)
+        S.c_grnd = 0.0;                // read per arrival from the trace result
+        S.th_start = S.th_min;
+        return GA_OK;
+    }
     S.geom = GA_GEOM_SPH;
     S.r0 = M.r0;
     S.src_ground = M.z0;
@@ -1765,7 +1802,8 @@ int eig_prepare_rd(ga_ctx *ctx, int rd_id, const ga_prop_params *p, const ga_eig
 
 int eig_check(ga_ctx *ctx, const ga_prop_params *p, const ga_eig_params *e)
 {
-    if (p->geometry != GA_GEOM_3D && p->geometry != GA_GEOM_SPH && p->geometry != GA_GEOM_SPH_RNGDEP)
+    if (p->geometry != GA_GEOM_3D && p->geometry != GA_GEOM_SPH && p->geometry != GA_GEOM_SPH_RNGDEP &&
+        p->geometry != GA_GEOM_3D_RNGDEP)
         return fail(ctx, GA_E_ARG, "bad geometry %d", p->geometry);
     if (e->bnc_min < 0 || e->bnc_max < e->bnc_min) return fail(ctx, GA_E_ARG, "bad bounce range %d..%d", e->bnc_min, e->bnc_max);
     if (e->iterations < 0) return fail(ctx, GA_E_ARG, "negative iteration limit");
@@ -2574,9 +2612,10 @@ static int eig_search_impl(ga_ctx *ctx, int atmo_id, const ga_prop_params *p, co
     EigSetup S;
     std::memset(&S, 0, sizeof(S));
     std::vector<double> rcvr(rcvr_xy, rcvr_xy + 2 * (size_t)n_rcvr);
-    if (p->geometry == GA_GEOM_SPH_RNGDEP) {
+    if (p->geometry == GA_GEOM_SPH_RNGDEP || p->geometry == GA_GEOM_3D_RNGDEP) {
         gpu.rd = true;
-        gpu.geom = GA_GEOM_SPH;
+        gpu.cart = p->geometry == GA_GEOM_3D_RNGDEP;
+        gpu.geom = gpu.cart ? GA_GEOM_3D : GA_GEOM_SPH;
         rc = eig_prepare_rd(ctx, atmo_id, p, e, gpu.RF, S, rcvr);
     } else {
         auto it = ctx->atmos.find(atmo_id);

@@ -1,11 +1,12 @@
 // infraga_cli.cpp -- drop-in front ends infraga-sph / infraga-3d /
-// infraga-sph-rngdep on the GPU engine.
+// infraga-sph-rngdep / infraga-3d-rngdep / infraga-2d on the GPU engine.
 //
 // Same command line as the reference drivers (src/main.infraga.sph.cpp,
-// src/main.infraga.3d.cpp, src/main.infraga.sph.rngdep.cpp): an option
-// (-prop, -eig_search; the range-dependent binary: -prop with the profile
-// prefix and the two node files), the profile path,
-// then key=value parameters with the same names, aliases and defaults.  The
+// src/main.infraga.3d.cpp, src/main.infraga.sph.rngdep.cpp,
+// src/main.infraga.3d.rngdep.cpp, src/main.infraga.2d.cpp): an option
+// (-prop, -eig_search, -eig_direct; the range-dependent binaries take the
+// profile prefix and the two node files), the profile path, then
+// key=value parameters with the same names, aliases and defaults.  The
 // ray tracing runs on the MI355X through the C ABI of libinfraga_gpu.so; this
 // file only parses arguments, builds the launch list with the reference's
 // accumulated loops and writes the output files in the reference's format.
@@ -861,56 +862,76 @@ int run_eig_search(int argc, char **argv)
     return 0;
 }
 
-// infraga-sph-rngdep -eig_search prefix lats.dat lons.dat [key=value ...]
-// (run_eig_search, main.infraga.sph.rngdep.cpp:777-965): the spherical
-// search over the range-dependent medium and terrain; writes
-// <prefix>.arrivals.dat and <prefix>.eigenray-N.dat.
-int run_eig_search_rd(int argc, char **argv)
+// infraga-{sph,3d}-rngdep -eig_search / -eig_direct prefix nodes nodes
+// [key=value ...]: run_eig_search and run_eig_direct of
+// main.infraga.sph.rngdep.cpp:777-1107 (the spherical search of
+// geoac.eigenray.sph.cpp) and main.infraga.3d.rngdep.cpp:755-1050 (the 3D
+// search of geoac.eigenray.3d.cpp, node coordinates and receivers in km)
+// over the range-dependent medium and terrain.  The search writes
+// <id>.arrivals.dat and <id>.eigenray-N.dat, -eig_direct the eigenray file
+// of an accepted iteration.
+int run_eig_rd(int argc, char **argv, bool direct)
 {
+    constexpr bool cart = GA_CLI_GEOM == GA_GEOM_3D_RNGDEP;
     if (argc < 5) {
-        std::cerr << GA_CLI_NAME << ": usage: -eig_search <profile prefix> <lat nodes> <lon nodes> [parameters]\n";
+        std::cerr << GA_CLI_NAME << ": usage: " << argv[1]
+                  << (cart ? " <profile prefix> <x nodes> <y nodes> [parameters]\n"
+                           : " <profile prefix> <lat nodes> <lon nodes> [parameters]\n");
         return 2;
     }
-    double lat_src = 30.0, lon_src = 0.0, z_src = 0.0, rlat = 30.0, rlon = -2.5;
-    bool src_lat_set = false, src_lon_set = false, rlat_set = false, rlon_set = false;
+    // sph: lat / lon [deg]; 3D: x / y [km], the defaults taken from the region
+    double s0 = 30.0, s1 = 0.0, z_src = 0.0, r0 = 30.0, r1 = -2.5;
+    bool s0_set = false, s1_set = false, r0_set = false, r1_set = false;
+    // -eig_direct: inclination estimate (sph 10 deg; the 3D binary starts
+    // from 0.5 taken as radians, main.infraga.3d.rngdep.cpp:937), azimuth
+    // estimate (NaN: toward the receiver)
+    double incl_est = cart ? 0.5 * (180.0 / Pi) : 10.0, az_est = NAN;
+    int bounces = 0;
     bool reverse_winds = false, custom_id = false;
     double freq = 0.1;
     const char *prof_format = "zTuvdp";
     const char *topo_file = nullptr;
     const char *output_id = nullptr;
     ga_prop_params p;
-    ga_prop_params_default(&p, GA_GEOM_SPH_RNGDEP);
+    ga_prop_params_default(&p, cart ? GA_GEOM_3D_RNGDEP : GA_GEOM_SPH_RNGDEP);
     ga_eig_params e;
     ga_eig_params_default(&e);
     double z_grnd = NAN;
     for (int i = 5; i < argc; i++) {
         const char *a = argv[i], *v = nullptr;
-        if (key2(a, "incl_min=", "min_incl=", v)) e.incl_min = std::atof(v);
-        else if (key2(a, "incl_max=", "max_incl=", v)) e.incl_max = std::atof(v);
-        else if (key(a, "bnc_min=", v)) e.bnc_min = std::atoi(v);
-        else if (key(a, "bnc_max=", v)) e.bnc_max = std::atoi(v);
-        else if (key(a, "bounces=", v)) e.bnc_min = e.bnc_max = std::atoi(v);
+        if (!direct && key2(a, "incl_min=", "min_incl=", v)) e.incl_min = std::atof(v);
+        else if (!direct && key2(a, "incl_max=", "max_incl=", v)) e.incl_max = std::atof(v);
+        else if (!direct && key2(a, "bnc_min=", "min_bnc=", v)) e.bnc_min = std::atoi(v);
+        else if (!direct && key2(a, "bnc_max=", "max_bnc=", v)) e.bnc_max = std::atoi(v);
+        else if (!direct && key(a, "bounces=", v)) e.bnc_min = e.bnc_max = std::atoi(v);
+        else if (direct && key2(a, "incl_est=", "est_incl=", v)) incl_est = std::atof(v);
+        else if (direct && key2(a, "az_est=", "est_az=", v)) az_est = std::atof(v);
+        else if (direct && key(a, "bounces=", v)) bounces = std::atoi(v);
         else if (key2(a, "src_alt=", "alt_src=", v)) z_src = std::atof(v);
-        else if (key2(a, "src_lat=", "lat_src=", v)) { lat_src = std::atof(v); src_lat_set = true; }
-        else if (key2(a, "src_lon=", "lon_src=", v)) { lon_src = std::atof(v); src_lon_set = true; }
-        else if (key2(a, "rcvr_lat=", "lat_rcvr=", v)) { rlat = std::atof(v); rlat_set = true; }
-        else if (key2(a, "rcvr_lon=", "lon_rcvr=", v)) { rlon = std::atof(v); rlon_set = true; }
+        else if (!cart && key2(a, "src_lat=", "lat_src=", v)) { s0 = std::atof(v); s0_set = true; }
+        else if (!cart && key2(a, "src_lon=", "lon_src=", v)) { s1 = std::atof(v); s1_set = true; }
+        else if (!cart && key2(a, "rcvr_lat=", "lat_rcvr=", v)) { r0 = std::atof(v); r0_set = true; }
+        else if (!cart && key2(a, "rcvr_lon=", "lon_rcvr=", v)) { r1 = std::atof(v); r1_set = true; }
+        else if (cart && key2(a, "src_x=", "x_src=", v)) { s0 = std::atof(v); s0_set = true; }
+        else if (cart && key2(a, "src_y=", "y_src_y=", v)) { s1 = std::atof(v); s1_set = true; }
+        else if (cart && key2(a, "rcvr_x=", "x_rcvr=", v)) { r0 = std::atof(v); r0_set = true; }
+        else if (cart && key2(a, "rcvr_y=", "y_rcvr=", v)) { r1 = std::atof(v); r1_set = true; }
         else if (key(a, "verbose=", v)) { /* trace-by-trace log not produced */ }
         else if (key(a, "iterations=", v)) e.iterations = (int)std::atof(v);
         else if (key(a, "damping=", v)) e.damping = std::atof(v);
         else if (key(a, "tolerance=", v)) e.tolerance = std::atof(v);
-        else if (key(a, "az_dev_lim=", v)) e.az_dev_lim = std::atof(v);
-        else if (key2(a, "incl_step_min=", "min_incl_step=", v)) e.incl_step_min = std::atof(v);
-        else if (key2(a, "incl_step_max=", "max_incl_step=", v)) e.incl_step_max = std::atof(v);
+        else if (!direct && key(a, "az_dev_lim=", v)) e.az_dev_lim = std::atof(v);
+        else if (!direct && key2(a, "incl_step_min=", "min_incl_step=", v)) e.incl_step_min = std::atof(v);
+        else if (!direct && key2(a, "incl_step_max=", "max_incl_step=", v)) e.incl_step_max = std::atof(v);
         else if (key(a, "freq=", v)) freq = std::atof(v);
         else if (key(a, "abs_coeff=", v)) p.abs_coeff = std::max(0.0, std::atof(v));
         else if (key2(a, "max_alt=", "alt_max=", v)) p.alt_max = std::atof(v);
         else if (key2(a, "max_rng=", "rng_max=", v)) p.rng_max = std::atof(v);
-        else if (key2(a, "max_time=", "time_max=", v)) p.time_max = std::atof(v);
-        else if (key2(a, "min_lat=", "lat_min=", v)) p.bounds[0] = std::atof(v);
-        else if (key2(a, "max_lat=", "lat_max=", v)) p.bounds[1] = std::atof(v);
-        else if (key2(a, "min_lon=", "lon_min=", v)) p.bounds[2] = std::atof(v);
-        else if (key2(a, "max_lon=", "lon_max=", v)) p.bounds[3] = std::atof(v);
+        else if (!cart && key2(a, "max_time=", "time_max=", v)) p.time_max = std::atof(v);
+        else if (key2(a, cart ? "min_x=" : "min_lat=", cart ? "x_min=" : "lat_min=", v)) p.bounds[0] = std::atof(v);
+        else if (key2(a, cart ? "max_x=" : "max_lat=", cart ? "x_max=" : "lat_max=", v)) p.bounds[1] = std::atof(v);
+        else if (key2(a, cart ? "min_y=" : "min_lon=", cart ? "y_min=" : "lon_min=", v)) p.bounds[2] = std::atof(v);
+        else if (key2(a, cart ? "max_y=" : "max_lon=", cart ? "y_max=" : "lon_max=", v)) p.bounds[3] = std::atof(v);
         else if (key2(a, "min_ds=", "ds_min=", v)) p.ds_min = std::atof(v);
         else if (key2(a, "max_ds=", "ds_max=", v)) p.ds_max = std::atof(v);
         else if (key2(a, "max_s=", "s_max=", v)) p.s_max = std::atof(v);
@@ -926,70 +947,112 @@ int run_eig_search_rd(int argc, char **argv)
             return 2;
         }
     }
+    if (direct && bounces < 0) {
+        std::cerr << GA_CLI_NAME << ": bounces must be >= 0\n";
+        return 2;
+    }
     ga_ctx *ctx = nullptr;
     if (ga_ctx_create(&ctx, 0) != GA_OK) return die(nullptr, "no usable GPU");
     int rid = 0;
-    if (ga_rngdep_load_files(ctx, argv[2], argv[3], argv[4], topo_file, prof_format, reverse_winds ? 1 : 0, &rid) != GA_OK)
+    if ((cart ? ga_rngdep_load_files_xy : ga_rngdep_load_files)(ctx, argv[2], argv[3], argv[4], topo_file, prof_format,
+                                                               reverse_winds ? 1 : 0, &rid) != GA_OK)
         return die(ctx, "cannot load the range-dependent atmosphere");
     double info[8];
     if (ga_rngdep_info(ctx, rid, info) != GA_OK) return die(ctx, "region");
     const bool is_topo = topo_file != nullptr;
-    if (is_topo) {
-        // :812-815: source at the region centre, receiver 5 deg east of it
+    const double deg = cart ? 1.0 : Pi / 180.0;   // region in radians (sph) or km (3D)
+    if (cart) {
+        // source at the region centre, receiver 250 km east of it (3d.rngdep.cpp:797-798)
+        const double cx = (info[1] + info[0]) / 2.0, cy = (info[3] + info[2]) / 2.0;
+        if (!s0_set) s0 = cx;
+        if (!s1_set) s1 = cy;
+        if (!r0_set) r0 = cx + 250.0;
+        if (!r1_set) r1 = cy;
+    } else if (is_topo) {
+        // source at the region centre, receiver 5 deg east of it (sph.rngdep.cpp:812-815)
         const double clat = (info[1] + info[0]) / 2.0 * (180.0 / Pi), clon = (info[3] + info[2]) / 2.0 * (180.0 / Pi);
-        if (!src_lat_set) lat_src = clat;
-        if (!src_lon_set) lon_src = clon;
-        if (!rlat_set) rlat = clat + 0.0;
-        if (!rlon_set) rlon = clon + 5.0;
+        if (!s0_set) s0 = clat;
+        if (!s1_set) s1 = clon;
+        if (!r0_set) r0 = clat + 0.0;
+        if (!r1_set) r1 = clon + 5.0;
     }
     const double z0 = (!is_topo && !std::isnan(z_grnd)) ? z_grnd : info[5];
     if (!is_topo) p.z_grnd = z0;
-    double pts[6] = {lat_src * (Pi / 180.0), lon_src * (Pi / 180.0), 0.0, rlat * (Pi / 180.0), rlon * (Pi / 180.0), 0.0};
+    double pts[6] = {s0 * deg, s1 * deg, 0.0, r0 * deg, r1 * deg, 0.0};
     double aout[80], tout[20];
     if (ga_rngdep_eval(ctx, rid, pts, 2, aout, tout) != GA_OK) return die(ctx, "terrain");
-    const double zg_src = is_topo ? tout[0] : z0, zg_rcvr = is_topo ? tout[10] : z0;
-    z_src = std::max(z_src, zg_src);                                     // :880
+    // without terrain the spherical ground is z0, the Cartesian topo::z is 0
+    const double zflat = cart ? 0.0 : z0;
+    const double zg_src = is_topo ? tout[0] : zflat, zg_rcvr = is_topo ? tout[10] : zflat;
+    z_src = std::max(z_src, zg_src);                                     // sph :880, 3D :845
     p.freq = freq;
-    p.src[0] = lat_src;
-    p.src[1] = lon_src;
+    p.src[0] = s0;
+    p.src[1] = s1;
     p.src[2] = z_src;
-    const double rcvr[2] = {rlat, rlon};
+    const double rcvr[2] = {r0, r1};
     std::vector<ga_eigenray> eig(64);
     std::vector<ga_path_pt> pts_out((size_t)1 << 16);
     int n_out = 0;
     int64_t n_pts = 0;
     double lat_ms = 0.0;
     ga_run_stats st;
-    int rc = ga_eig_search_paths(ctx, rid, &p, &e, rcvr, 1, eig.data(), (int)eig.size(), &n_out, &lat_ms,
-                                 pts_out.data(), (int64_t)pts_out.size(), &n_pts, &st);
+    auto call = [&]() {
+        if (direct)
+            return ga_eig_direct(ctx, rid, &p, &e, rcvr, incl_est, az_est, bounces, eig.data(), &n_out, pts_out.data(),
+                                 (int64_t)pts_out.size(), &n_pts, &st);
+        return ga_eig_search_paths(ctx, rid, &p, &e, rcvr, 1, eig.data(), (int)eig.size(), &n_out, &lat_ms,
+                                   pts_out.data(), (int64_t)pts_out.size(), &n_pts, &st);
+    };
+    int rc = call();
     if (rc == GA_E_CAP) {
         eig.resize(std::max<size_t>(eig.size(), (size_t)n_out));
         pts_out.resize(std::max<size_t>(pts_out.size(), (size_t)n_pts));
-        rc = ga_eig_search_paths(ctx, rid, &p, &e, rcvr, 1, eig.data(), (int)eig.size(), &n_out, &lat_ms,
-                                 pts_out.data(), (int64_t)pts_out.size(), &n_pts, &st);
+        rc = call();
     }
-    if (rc != GA_OK) return die(ctx, "eigenray search failed");
+    if (rc != GA_OK) return die(ctx, direct ? "eigenray iteration failed" : "eigenray search failed");
     std::string id = custom_id ? std::string(output_id) : std::string(argv[2]);
     {
+        // <id>.eigenray-N.dat: sph rows at precision 8; 3D rows at the
+        // stream default, the absorption column after the first reflection
+        // the bare integral (geoac.eigenray.3d.cpp:263)
         int64_t w = 0;
         for (int i = 0; i < n_out; ++i) {
             std::ofstream ray(id + ".eigenray-" + std::to_string(i) + ".dat");
-            ray << "# lat [deg]" << '\t' << "lon [deg]" << '\t' << "z [km]" << '\t' << "trans. coeff. [dB]" << '\t'
-                << "absorption [dB]" << '\t' << "time [s]" << '\n' << std::setprecision(8);
+            ray << (cart ? "# x [km]" : "# lat [deg]") << '\t' << (cart ? "y [km]" : "lon [deg]") << '\t' << "z [km]"
+                << '\t' << "trans. coeff. [dB]" << '\t' << "absorption [dB]" << '\t' << "time [s]" << '\n';
+            if (!cart) ray << std::setprecision(8);
             for (; w < n_pts && pts_out[w].launch_idx == i; ++w) {
                 const ga_path_pt &P = pts_out[w];
-                ray << P.pos0 << '\t' << P.pos1 << '\t' << P.z << '\t' << P.tl << '\t' << P.absorption << '\t' << P.time
-                    << '\n';
+                const double ab = (cart && P.leg > 0) ? P.absorption * -0.5 : P.absorption;
+                ray << P.pos0 << '\t' << P.pos1 << '\t' << P.z << '\t' << P.tl << '\t' << ab << '\t' << P.time << '\n';
             }
         }
     }
+    if (direct) {
+        if (n_out)
+            std::cout << '\t' << "Eigenray identified:" << '\t' << "theta, phi = " << std::setprecision(8) << eig[0].incl
+                      << ", " << eig[0].az << " degrees." << '\n';
+        else
+            std::cout << GA_CLI_NAME << ": no eigenray identified\n";
+        std::cout << GA_CLI_NAME << ": " << st.n_rays << " ray traces, " << st.ray_steps << " ray-steps, "
+                  << st.kernel_ms << " ms on the GPU\n";
+        ga_ctx_destroy(ctx);
+        return 0;
+    }
     std::ofstream out(id + ".arrivals.dat");
-    out << "# infraga-sph-rngdep -eig_search summary:" << '\n';
+    out << "# " << GA_CLI_NAME << " -eig_search summary:" << '\n';
     out << "# " << '\t' << "profile prefix: " << argv[2] << '\n';
-    out << "# " << '\t' << "source location (lat, lon, alt): " << (lat_src * (Pi / 180.0)) * (180.0 / Pi) << ", "
-        << (lon_src * (Pi / 180.0)) * (180.0 / Pi) << ", " << z_src << '\n';
-    out << "# " << '\t' << "receiver location (lat, lon, alt): " << (rlat * (Pi / 180.0)) * (180.0 / Pi) << ", "
-        << (rlon * (Pi / 180.0)) * (180.0 / Pi) << ", " << zg_rcvr << '\n';
+    if (cart) {
+        // main.infraga.3d.rngdep.cpp:850-866
+        out << "# " << '\t' << "profile grid files: " << argv[3] << ", " << argv[4] << '\n';
+        out << "# " << '\t' << "source location: " << s0 << ", " << s1 << ", " << z_src << '\n';
+        out << "# " << '\t' << "receiver location: " << r0 << ", " << r1 << ", " << zg_rcvr << '\n';
+    } else {
+        out << "# " << '\t' << "source location (lat, lon, alt): " << (s0 * (Pi / 180.0)) * (180.0 / Pi) << ", "
+            << (s1 * (Pi / 180.0)) * (180.0 / Pi) << ", " << z_src << '\n';
+        out << "# " << '\t' << "receiver location (lat, lon, alt): " << (r0 * (Pi / 180.0)) * (180.0 / Pi) << ", "
+            << (r1 * (Pi / 180.0)) * (180.0 / Pi) << ", " << zg_rcvr << '\n';
+    }
     out << "# " << '\t' << "inclination range: " << (e.incl_min * (Pi / 180.0)) * (180.0 / Pi) << ", "
         << (e.incl_max * (Pi / 180.0)) * (180.0 / Pi) << '\n';
     out << "# " << '\t' << "bounces: " << e.bnc_min << ", " << e.bnc_max << '\n';
@@ -998,9 +1061,14 @@ int run_eig_search_rd(int argc, char **argv)
     out << "# " << '\t' << "damping: " << e.damping << '\n';
     out << "# " << '\t' << "frequency: " << freq << '\n';
     out << "# " << '\t' << "S&B atten coeff: " << p.abs_coeff << '\n';
-    out << "# incl [deg]" << '\t' << "az [deg]" << '\t' << "n_b" << '\t' << "lat_0 [deg]" << '\t' << "lon_0 [deg]" << '\t'
-        << "time [s]" << '\t' << "cel [km/s]" << '\t' << "turning ht [km]" << '\t' << "inclination [deg]" << '\t'
-        << "back azimuth [deg]" << '\t' << "trans. coeff. [dB]" << '\t' << "absorption [dB]" << '\n';
+    if (cart) out << '\n';
+    out << "# incl [deg]" << '\t' << "az [deg]" << '\t' << "n_b";
+    if (cart) out << '\t' << "x_0 [km]" << '\t' << "y_0 [km]";
+    else out << '\t' << "lat_0 [deg]" << '\t' << "lon_0 [deg]";
+    out << '\t' << "time [s]" << '\t' << "cel [km/s]" << '\t' << "turning ht [km]";
+    if (cart) out << '\t' << "arrival incl [deg]" << '\t' << "back az [deg]";
+    else out << '\t' << "inclination [deg]" << '\t' << "back azimuth [deg]";
+    out << '\t' << "trans. coeff. [dB]" << '\t' << "absorption [dB]" << '\n';
     for (int i = 0; i < n_out; ++i) {
         const ga_eigenray &A = eig[i];
         out << std::setprecision(8) << A.incl << '\t' << std::setprecision(8) << A.az << '\t' << A.n_bounce;
@@ -1330,14 +1398,10 @@ int main(int argc, char **argv)
         std::cerr << GA_CLI_NAME << ": option " << argv[1] << " is not supported by this engine\n";
         return 2;
     }
-    if (GA_CLI_GEOM == GA_GEOM_3D_RNGDEP) {
+    if (GA_CLI_GEOM == GA_GEOM_3D_RNGDEP || GA_CLI_GEOM == GA_GEOM_SPH_RNGDEP) {
         if (std::strncmp(argv[1], "-prop", 5) == 0) return run_prop_rd(argc, argv);
-        std::cerr << GA_CLI_NAME << ": option " << argv[1] << " is not supported by this engine\n";
-        return 2;
-    }
-    if (GA_CLI_GEOM == GA_GEOM_SPH_RNGDEP) {
-        if (std::strncmp(argv[1], "-prop", 5) == 0) return run_prop_rd(argc, argv);
-        if (std::strncmp(argv[1], "-eig_search", 11) == 0) return run_eig_search_rd(argc, argv);
+        if (std::strncmp(argv[1], "-eig_search", 11) == 0) return run_eig_rd(argc, argv, false);
+        if (std::strncmp(argv[1], "-eig_direct", 11) == 0) return run_eig_rd(argc, argv, true);
         std::cerr << GA_CLI_NAME << ": option " << argv[1] << " is not supported by this engine\n";
         return 2;
     }

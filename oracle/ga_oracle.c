@@ -1285,12 +1285,36 @@ int go_fan_paths(go_atmo *a, const go_params *p, const go_prop *pr,
  * Eigenray search, 3D: geoac.eigenray.3d.cpp:34-366 and the driver loop of
  * main.infraga.3d.cpp:944-951.
  * ==================================================================== */
+/* A stratified medium (either geometry) as a go_eig_med: topo::z is the flat
+ * ground (sph: r0 + z0, 3D: z0) with zero slopes. */
+static void sm_init(void *m, const go_params *p, go_src *s, double *y) { go_set_initial((go_atmo *)m, p, s, y); }
+static int sm_prop(void *m, const go_params *p, const go_src *s, double *rows, int length, int *check)
+{
+    return go_prop_rk4((go_atmo *)m, p, s, rows, length, check);
+}
+static void sm_refl(void *m, const go_params *p, const go_src *s, const double *a, const double *b, double *o)
+{
+    go_set_refl((go_atmo *)m, p, s, a, b, o);
+}
+static double sm_tt(void *m, const double *a, const double *b) { return go_travel_time_seg((go_atmo *)m, a, b); }
+static double sm_att(void *m, const double *a, const double *b, double f) { return go_atten_seg((go_atmo *)m, a, b, f); }
+static double sm_c(void *m, double r, double t, double p) { return go_c((go_atmo *)m, r, t, p); }
+static double sm_amp(void *m, const go_src *s, const double *y) { return go_amp((go_atmo *)m, s, y); }
+static double sm_tz(void *m, double t, double p) { (void)t; (void)p; return ground((go_atmo *)m); }
+static double sm_tdz(void *m, double t, double p, int n) { (void)m; (void)t; (void)p; (void)n; return 0.0; }
+
+static go_eig_med strat_med(go_atmo *a)
+{
+    go_eig_med M = {a, a->r0, sm_init, sm_prop, sm_refl, sm_tt, sm_att, sm_c, sm_amp, sm_tz, sm_tdz};
+    return M;
+}
+
 typedef struct {
     go_atmo *a;
     go_params p6, p18;
     double dth_big, dth_sml, damping, tolerance;
     long long steps;
-    const go_eig_med *med;    /* spherical searches: the medium's operations */
+    const go_eig_med *med;    /* the medium's operations */
 } eig_ctx;
 
 /* Work counters of the last go_eig_search_3d call (see go_eig_counts). */
@@ -1327,15 +1351,16 @@ static double mod_dth(const eig_ctx *E, double dr, double dr_dtheta)   /* eigenr
 static int trace_legs(eig_ctx *E, const go_params *p, go_src *s, double *rows, int length, int nb, int *brk,
                       int single_reflection)
 {
-    go_set_initial(E->a, p, s, rows);
-    int k = go_prop_rk4(E->a, p, s, rows, length, brk);
+    const go_eig_med *M = E->med;
+    M->set_initial(M->m, p, s, rows);
+    int k = M->prop_rk4(M->m, p, s, rows, length, brk);
     E->steps += k;
     if (!*brk) {
         for (int b = 1; b <= nb; b++) {
             /* the reference reads row k-2 < 0 here (undefined); defined as a break */
             if (k < 2) { *brk = 1; break; }
-            go_set_refl(E->a, p, s, rows + 18 * (k - 2), rows + 18 * (k - 1), rows);
-            k = go_prop_rk4(E->a, p, s, rows, length, brk);
+            M->set_refl(M->m, p, s, rows + 18 * (k - 2), rows + 18 * (k - 1), rows);
+            k = M->prop_rk4(M->m, p, s, rows, length, brk);
             E->steps += k;
             if (*brk) break;
             if (single_reflection) break;   /* eigenray.3d.cpp:206-220: unconditional break */
@@ -1395,37 +1420,55 @@ static int est_eigenray(eig_ctx *E, const double src[3], const double rcvr[2], d
     return 0;
 }
 
+/* A 3D path row (x, y, z floored at the terrain) into a path sink. */
+static void sink_pt_3d(path_sink *k, int leg, int m, const go_eig_med *M, const double *y, double tl, double att,
+                       double tt)
+{
+    if (k->n < k->cap) {
+        go_path_pt *P = k->pts + k->n;
+        const double zg = M->topo_z(M->m, y[0], y[1]);
+        P->launch_idx = 0; P->leg = leg; P->step = m; P->kind = 0;
+        P->pos0 = y[0];
+        P->pos1 = y[1];
+        P->z = y[2] > zg ? y[2] : zg;     /* max(z, topo::z(x, y)) */
+        P->tl = tl; P->absorption = -2.0 * att; P->time = tt;
+    }
+    k->n++;
+}
+
 /* The accepted eigenray's row (find_eigenray, geoac.eigenray.3d.cpp:223-276):
  * a re-trace of launch (s->theta, s->phi) with per-step bookkeeping of the
  * segments 1..k-1 of each leg over bnc_cnt reflections, the row taken at
  * row k of the last leg.  Returns the RK4 steps taken. */
-static long long eig_accept_3d(go_atmo *a, const go_params *p, go_src *s, const double src[3], int bnc_cnt,
+static long long eig_accept_3d(const go_eig_med *M, const go_params *p, go_src *s, const double src[3], int bnc_cnt,
                                double freq, double *rows, int length, go_arrival *A, path_sink *ps)
 {
     double att = 0.0, tt = 0.0, zmax = 0.0;
     int brk, k;
     long long steps = 0;
-    go_set_initial(a, p, s, rows);
-    k = go_prop_rk4(a, p, s, rows, length, &brk);
+    M->set_initial(M->m, p, s, rows);
+    k = M->prop_rk4(M->m, p, s, rows, length, &brk);
     steps += k;
     for (int m = 1; m < k; m++) {
-        tt += go_travel_time_seg(a, rows + 18 * (m - 1), rows + 18 * m);
-        att += go_atten_seg(a, rows + 18 * (m - 1), rows + 18 * m, freq);
+        tt += M->tt_seg(M->m, rows + 18 * (m - 1), rows + 18 * m);
+        att += M->att_seg(M->m, rows + 18 * (m - 1), rows + 18 * m, freq);
         zmax = zmax > rows[18 * m + 2] ? zmax : rows[18 * m + 2];
         if (ps && (m == 1 || m % 15 == 0))     /* 3d.cpp:240-247 */
-            sink_pt(ps, 0, 0, m, 0, a, rows + 18 * m, 20.0 * log10(go_amp(a, s, rows + 18 * m)), att, tt);
+            sink_pt_3d(ps, 0, m, M, rows + 18 * m, 20.0 * log10(M->amp(M->m, s, rows + 18 * m)), att, tt);
     }
     for (int b = 1; b <= bnc_cnt; b++) {
         if (k < 2) break;   /* row k-2 < 0: no further reflection */
-        go_set_refl(a, p, s, rows + 18 * (k - 2), rows + 18 * (k - 1), rows);
-        k = go_prop_rk4(a, p, s, rows, length, &brk);
+        M->set_refl(M->m, p, s, rows + 18 * (k - 2), rows + 18 * (k - 1), rows);
+        k = M->prop_rk4(M->m, p, s, rows, length, &brk);
         steps += k;
         for (int m = 1; m < k; m++) {
-            tt += go_travel_time_seg(a, rows + 18 * (m - 1), rows + 18 * m);
-            att += go_atten_seg(a, rows + 18 * (m - 1), rows + 18 * m, freq);
+            tt += M->tt_seg(M->m, rows + 18 * (m - 1), rows + 18 * m);
+            att += M->att_seg(M->m, rows + 18 * (m - 1), rows + 18 * m, freq);
             zmax = zmax > rows[18 * m + 2] ? zmax : rows[18 * m + 2];
-            if (ps && (m == 1 || m % 15 == 0))     /* 3d.cpp:258-265 (z printed without the ground floor) */
-                sink_pt(ps, 0, b, m, 0, a, rows + 18 * m, 20.0 * log10(go_amp(a, s, rows + 18 * m)), att, tt);
+            /* 3d.cpp:258-265: z printed without the ground floor, which rows
+             * 1..k-1 (above the ground by the ground check) never reach */
+            if (ps && (m == 1 || m % 15 == 0))
+                sink_pt_3d(ps, b, m, M, rows + 18 * m, 20.0 * log10(M->amp(M->m, s, rows + 18 * m)), att, tt);
         }
     }
     const double *yk = rows + 18 * k;
@@ -1438,12 +1481,12 @@ static long long eig_accept_3d(go_atmo *a, const go_params *p, go_src *s, const 
     A->time = tt;
     A->cel = sqrt(sq(yk[0] - src[0]) + sq(yk[1] - src[1])) / tt;
     A->turn_ht = zmax;
-    A->incl_arr = -asin(go_c(a, yk[0], yk[1], yk[2]) / go_c(a, src[0], src[1], src[2]) * yk[5]) * (180.0 / GO_PI);
+    A->incl_arr = -asin(M->c(M->m, yk[0], yk[1], yk[2]) / M->c(M->m, src[0], src[1], src[2]) * yk[5]) * (180.0 / GO_PI);
     double baz = 90.0 - atan2(-yk[4], -yk[3]) * (180.0 / GO_PI);
     while (baz > 180.0) baz -= 360.0;
     while (baz < -180.0) baz += 360.0;
     A->back_az = baz;
-    A->tl = 20.0 * log10(go_amp(a, s, yk));
+    A->tl = 20.0 * log10(M->amp(M->m, s, yk));
     A->absorption = -2.0 * att;
     return steps;
 }
@@ -1460,18 +1503,75 @@ int go_eig_row_3d(go_atmo *a, const go_params *p, const double src[3], double in
     s.src[0] = src[0]; s.src[1] = src[1]; s.src[2] = src[2];
     s.theta = incl_deg * (GO_PI / 180.0);
     s.phi = (90.0 - az_deg) * (GO_PI / 180.0);
-    eig_accept_3d(a, p, &s, src, bnc_cnt, freq, rows, length, A, NULL);
+    go_eig_med M = strat_med(a);
+    eig_accept_3d(&M, p, &s, src, bnc_cnt, freq, rows, length, A, NULL);
     return 0;
+}
+
+/* The arrival's ground-point derivatives d(x, y)/d(theta, phi) that
+ * find_eigenray's update uses (geoac.eigenray.3d.cpp:334-348), in long
+ * double: the auxiliary parameters at row K moved along the ray to the
+ * ground, over flat ground or (is_topo) the terrain's tangent plane.
+ * J = {dx/dth, dy/dth, dx/dph, dy/dph}. */
+static void lm_jac_3d(const go_eig_med *M, const double src[3], const double *yk, long double J[4])
+{
+    long double c_src, c_grnd, dzg_dx, dzg_dy, ds_dth, ds_dph;
+    c_src = M->c(M->m, src[0], src[1], src[2]);
+    c_grnd = M->c(M->m, yk[0], yk[1], M->topo_z(M->m, yk[0], yk[1]));
+    if (M->is_topo) {
+        dzg_dx = M->topo_dz(M->m, yk[0], yk[1], 0);
+        dzg_dy = M->topo_dz(M->m, yk[0], yk[1], 1);
+        ds_dth = -c_src / c_grnd * (yk[8] - dzg_dx * yk[6] - dzg_dy * yk[7]) / (yk[5] - dzg_dx * yk[3] - dzg_dy * yk[4]);
+        ds_dph = -c_src / c_grnd * (yk[14] - dzg_dx * yk[12] - dzg_dy * yk[13]) / (yk[5] - dzg_dx * yk[3] - dzg_dy * yk[4]);
+    } else {
+        ds_dth = -c_src / c_grnd * yk[8] / yk[5];
+        ds_dph = -c_src / c_grnd * yk[14] / yk[5];
+    }
+    J[0] = yk[6] + yk[3] * ds_dth;
+    J[1] = yk[7] + yk[4] * ds_dth;
+    J[2] = yk[12] + yk[3] * ds_dph;
+    J[3] = yk[13] + yk[4] * ds_dph;
+}
+
+/* lm_jac_3d in doubles, for tests. */
+void go_eig_lm_jac_3d(const go_eig_med *M, const double src[3], const double *yk, double J[4])
+{
+    long double L[4];
+    lm_jac_3d(M, src, yk, L);
+    for (int i = 0; i < 4; i++) J[i] = (double)L[i];
+}
+
+/* One leg of an 18-equation trace over a medium from (th, ph): rows k-1 and
+ * k of the first leg (k: the first row below the ground).  Returns k, or -1
+ * when the ray leaves the region. */
+int go_eig_trace_leg_3d(const go_eig_med *M, const go_params *p, const double src[3], double th, double ph,
+                        double y_km1[18], double y_k[18])
+{
+    go_params p18 = *p;
+    p18.calc_amp = 1;
+    p18.eq_cnt = 18;
+    int length = (int)(p18.s_max / p18.ds_min), brk;
+    double *rows = get_rows(length);
+    go_src s;
+    s.src[0] = src[0]; s.src[1] = src[1]; s.src[2] = src[2];
+    s.theta = th;
+    s.phi = ph;
+    M->set_initial(M->m, &p18, &s, rows);
+    int k = M->prop_rk4(M->m, &p18, &s, rows, length, &brk);
+    if (brk || k < 1) return -1;
+    memcpy(y_km1, rows + 18 * (k - 1), 18 * sizeof(double));
+    memcpy(y_k, rows + 18 * k, 18 * sizeof(double));
+    return k;
 }
 
 static int find_eigenray(eig_ctx *E, const double src[3], const double rcvr[2], double th_est, double ph_est,
                          double freq, int bnc_cnt, int iterate_limit, go_arrival *A)
 {
     const go_params *p = &E->p18;
-    go_atmo *a = E->a;
+    const go_eig_med *M = E->med;
     int brk, k, length = (int)(p->s_max / p->ds_min);
     double dr, dr_prev = 10000.0, step_sc = 1.0;
-    long double x, y, c_src, c_grnd, ds_dth, ds_dph, dx, dy, dx_dth, dy_dth, dx_dph, dy_dph;
+    long double x, y, dx, dy, dx_dth, dy_dth, dx_dph, dy_dph;
     long double det, dth = 0.0L, dph = 0.0L;     /* uninitialised in the reference */
     double *rows = get_rows(length);
     go_src s;
@@ -1480,8 +1580,8 @@ static int find_eigenray(eig_ctx *E, const double src[3], const double rcvr[2], 
     s.phi = ph_est;
     for (int n = 0; n <= iterate_limit; n++) {
         if (n == iterate_limit) break;
-        go_set_initial(a, p, &s, rows);
-        k = go_prop_rk4(a, p, &s, rows, length, &brk);
+        M->set_initial(M->m, p, &s, rows);
+        k = M->prop_rk4(M->m, p, &s, rows, length, &brk);
         E->steps += k;
         eig_cnt[GO_EIG_LM_TRACES]++;
         if (!brk && bnc_cnt > 0 && k < 2) brk = 1;   /* row k-2 < 0: defined as a break */
@@ -1490,8 +1590,8 @@ static int find_eigenray(eig_ctx *E, const double src[3], const double rcvr[2], 
         }
         if (!brk) {
             for (int b = 1; b <= bnc_cnt; b++) {
-                go_set_refl(a, p, &s, rows + 18 * (k - 2), rows + 18 * (k - 1), rows);
-                k = go_prop_rk4(a, p, &s, rows, length, &brk);
+                M->set_refl(M->m, p, &s, rows + 18 * (k - 2), rows + 18 * (k - 1), rows);
+                k = M->prop_rk4(M->m, p, &s, rows, length, &brk);
                 E->steps += k;
                 if (brk) { s.theta -= dth * step_sc; s.phi -= dph * step_sc; step_sc /= 2.0; }
                 break;
@@ -1515,19 +1615,16 @@ static int find_eigenray(eig_ctx *E, const double src[3], const double rcvr[2], 
             if (lg) fprintf(lg, "L %d %d %d %.17g %.17g %.17g %c\n", bnc_cnt, eig_seq, n, s.theta, s.phi, dr,
                             dr < E->tolerance ? 'a' : (n > 0 && dr > dr_prev) ? 'h' : 's');
             if (dr < E->tolerance) {
-                E->steps += eig_accept_3d(a, p, &s, src, bnc_cnt, freq, rows, length, A, NULL);
+                E->steps += eig_accept_3d(M, p, &s, src, bnc_cnt, freq, rows, length, A, NULL);
                 eig_cnt[GO_EIG_FINAL_TRACES]++;
                 return 1;
             } else if (n > 0 && dr > dr_prev) {
                 s.theta -= dth * step_sc; s.phi -= dph * step_sc; step_sc /= 2.0;
             } else {
                 step_sc = (1.0 < step_sc * 1.25) ? 1.0 : step_sc * 1.25;
-                c_src = go_c(a, src[0], src[1], src[2]);
-                c_grnd = go_c(a, yk[0], yk[1], ground(a));
-                ds_dth = -c_src / c_grnd * yk[8] / yk[5];
-                ds_dph = -c_src / c_grnd * yk[14] / yk[5];
-                dx_dth = yk[6] + yk[3] * ds_dth;  dy_dth = yk[7] + yk[4] * ds_dth;
-                dx_dph = yk[12] + yk[3] * ds_dph; dy_dph = yk[13] + yk[4] * ds_dph;
+                long double J[4];
+                lm_jac_3d(M, src, yk, J);
+                dx_dth = J[0]; dy_dth = J[1]; dx_dph = J[2]; dy_dph = J[3];
                 det = pow(1.0 + E->damping, 2) * (dx_dth * dy_dph - dx_dph * dy_dth);
                 dth = ((1.0 + E->damping) * dy_dph * dx - dx_dph * dy) / det;
                 dph = ((1.0 + E->damping) * dx_dth * dy - dy_dth * dx) / det;
@@ -1544,21 +1641,31 @@ static int find_eigenray(eig_ctx *E, const double src[3], const double rcvr[2], 
 /* find_eigenray alone from a launch estimate in radians (the search's
  * starting points, for sensitivity probes): 1 and the row in *A when it
  * accepts, 0 otherwise. */
-int go_eig_find_3d(go_atmo *a, const go_params *p, const double src_in[3], const double rcvr[2], double th_est,
-                   double ph_est, int bnc, int iterations, double damping, double tolerance, double freq,
-                   go_arrival *A, long long *ray_steps)
+int go_eig_find_3d_med(const go_eig_med *M, const go_params *p, const double src[3], const double rcvr[2],
+                       double th_est, double ph_est, int bnc, int iterations, double damping, double tolerance,
+                       double freq, go_arrival *A, long long *ray_steps)
 {
     eig_ctx E;
-    E.a = a;
+    memset(&E, 0, sizeof(E));
+    E.med = M;
     E.p6 = *p; E.p6.calc_amp = 0; E.p6.eq_cnt = 6;
     E.p18 = *p; E.p18.calc_amp = 1; E.p18.eq_cnt = 18;
     E.damping = damping;
     E.tolerance = tolerance;
     E.steps = 0;
-    double src[3] = {src_in[0], src_in[1], src_in[2] > ground(a) ? src_in[2] : ground(a)};
     int ok = find_eigenray(&E, src, rcvr, th_est, ph_est, freq, bnc, iterations, A);
     if (ray_steps) *ray_steps = E.steps;
     return ok;
+}
+
+int go_eig_find_3d(go_atmo *a, const go_params *p, const double src_in[3], const double rcvr[2], double th_est,
+                   double ph_est, int bnc, int iterations, double damping, double tolerance, double freq,
+                   go_arrival *A, long long *ray_steps)
+{
+    go_eig_med M = strat_med(a);
+    double src[3] = {src_in[0], src_in[1], src_in[2] > ground(a) ? src_in[2] : ground(a)};
+    return go_eig_find_3d_med(&M, p, src, rcvr, th_est, ph_est, bnc, iterations, damping, tolerance, freq, A,
+                              ray_steps);
 }
 
 /* go_eig_find_3d that also records the iterates (theta, phi, dr) into traj
@@ -1577,15 +1684,14 @@ int go_eig_find_3d_traj(go_atmo *a, const go_params *p, const double src_in[3], 
     return ok;
 }
 
-int go_eig_search_3d(go_atmo *a, const go_params *p, const double src_in[3],
-                     const double rcvr[2], double th_min_deg, double th_max_deg,
-                     int bnc_min, int bnc_max, int iterations, double damping,
-                     double tolerance, double az_err_lim, double dth_big_deg,
-                     double dth_sml_deg, double freq,
-                     go_arrival *out, int out_cap, long long *ray_steps)
+int go_eig_search_3d_med(const go_eig_med *M, const go_params *p, const double src[3], const double rcvr[2],
+                         double th_min, double th_max, int bnc_min, int bnc_max, int iterations, double damping,
+                         double tolerance, double az_err_lim, double dth_big_deg, double dth_sml_deg, double freq,
+                         go_arrival *out, int out_cap, long long *ray_steps)
 {
     eig_ctx E;
-    E.a = a;
+    memset(&E, 0, sizeof(E));
+    E.med = M;
     E.p6 = *p; E.p6.calc_amp = 0; E.p6.eq_cnt = 6;
     E.p18 = *p; E.p18.calc_amp = 1; E.p18.eq_cnt = 18;
     E.dth_big = dth_big_deg * (GO_PI / 180.0);
@@ -1594,8 +1700,6 @@ int go_eig_search_3d(go_atmo *a, const go_params *p, const double src_in[3],
     E.tolerance = tolerance;
     E.steps = 0;
     memset(eig_cnt, 0, sizeof(eig_cnt));
-    double src[3] = {src_in[0], src_in[1], src_in[2] > ground(a) ? src_in[2] : ground(a)};
-    double th_min = th_min_deg * (GO_PI / 180.0), th_max = th_max_deg * (GO_PI / 180.0);
     int cnt = 0;
     if (eig_log()) fprintf(eig_log(), "R %.17g %.17g\n", rcvr[0], rcvr[1]);
     for (int nb = bnc_min; nb <= bnc_max; nb++) {
@@ -1621,6 +1725,37 @@ int go_eig_search_3d(go_atmo *a, const go_params *p, const double src_in[3],
     return cnt;
 }
 
+int go_eig_search_3d(go_atmo *a, const go_params *p, const double src_in[3],
+                     const double rcvr[2], double th_min_deg, double th_max_deg,
+                     int bnc_min, int bnc_max, int iterations, double damping,
+                     double tolerance, double az_err_lim, double dth_big_deg,
+                     double dth_sml_deg, double freq,
+                     go_arrival *out, int out_cap, long long *ray_steps)
+{
+    go_eig_med M = strat_med(a);
+    double src[3] = {src_in[0], src_in[1], src_in[2] > ground(a) ? src_in[2] : ground(a)};
+    return go_eig_search_3d_med(&M, p, src, rcvr, th_min_deg * (GO_PI / 180.0), th_max_deg * (GO_PI / 180.0),
+                                bnc_min, bnc_max, iterations, damping, tolerance, az_err_lim, dth_big_deg,
+                                dth_sml_deg, freq, out, out_cap, ray_steps);
+}
+
+long long go_eig_path_3d_med(const go_eig_med *M, const go_params *p, const double src[3], double th, double ph,
+                             int bnc_cnt, double freq, go_path_pt *pts, long long cap, go_arrival *A)
+{
+    go_params p18 = *p;
+    p18.calc_amp = 1;
+    p18.eq_cnt = 18;
+    int length = (int)(p18.s_max / p18.ds_min);
+    double *rows = get_rows(length);
+    path_sink sink = {pts, cap, 0};
+    go_src s;
+    s.src[0] = src[0]; s.src[1] = src[1]; s.src[2] = src[2];
+    s.theta = th;
+    s.phi = ph;
+    eig_accept_3d(M, &p18, &s, src, bnc_cnt, freq, rows, length, A, &sink);
+    return sink.n;
+}
+
 /* ======================================================================
  * Eigenray search, spherical: geoac.eigenray.sph.cpp:36-380 and the driver
  * loop of main.infraga.sph.cpp:962-989.  src = (z [km], lat, lon [rad]) and
@@ -1631,29 +1766,6 @@ static double go_bearing(double lat1, double lon1, double lat2, double lon2)   /
     double term1 = sin(lon2 - lon1);
     double term2 = cos(lat1) * tan(lat2) - sin(lat1) * cos(lon2 - lon1);
     return atan2(term1, term2);
-}
-
-/* The stratified spherical medium as a go_eig_med (topo::z = r0 + z0, flat). */
-static void sm_init(void *m, const go_params *p, go_src *s, double *y) { go_set_initial((go_atmo *)m, p, s, y); }
-static int sm_prop(void *m, const go_params *p, const go_src *s, double *rows, int length, int *check)
-{
-    return go_prop_rk4((go_atmo *)m, p, s, rows, length, check);
-}
-static void sm_refl(void *m, const go_params *p, const go_src *s, const double *a, const double *b, double *o)
-{
-    go_set_refl((go_atmo *)m, p, s, a, b, o);
-}
-static double sm_tt(void *m, const double *a, const double *b) { return go_travel_time_seg((go_atmo *)m, a, b); }
-static double sm_att(void *m, const double *a, const double *b, double f) { return go_atten_seg((go_atmo *)m, a, b, f); }
-static double sm_c(void *m, double r, double t, double p) { return go_c((go_atmo *)m, r, t, p); }
-static double sm_amp(void *m, const go_src *s, const double *y) { return go_amp((go_atmo *)m, s, y); }
-static double sm_tz(void *m, double t, double p) { (void)t; (void)p; return ground((go_atmo *)m); }
-static double sm_tdz(void *m, double t, double p, int n) { (void)m; (void)t; (void)p; (void)n; return 0.0; }
-
-static go_eig_med strat_med(go_atmo *a)
-{
-    go_eig_med M = {a, a->r0, sm_init, sm_prop, sm_refl, sm_tt, sm_att, sm_c, sm_amp, sm_tz, sm_tdz};
-    return M;
 }
 
 /* trace_legs over the search's medium operations (spherical searches). */
@@ -1815,7 +1927,8 @@ long long go_eig_path(go_atmo *a, const go_params *p, const double src_in[3], do
     } else {
         double src[3] = {src_in[0], src_in[1], src_in[2] > ground(a) ? src_in[2] : ground(a)};
         s.src[0] = src[0]; s.src[1] = src[1]; s.src[2] = src[2];
-        eig_accept_3d(a, &p18, &s, src, bnc_cnt, freq, rows, length, A, &sink);
+        go_eig_med M = strat_med(a);
+        eig_accept_3d(&M, &p18, &s, src, bnc_cnt, freq, rows, length, A, &sink);
     }
     return sink.n;
 }
@@ -1978,6 +2091,7 @@ int go_eig_direct(go_atmo *a, const go_params *p, const double src_in[3], const 
                   double tolerance, double freq, go_arrival *A, long long *ray_steps)
 {
     eig_ctx E;
+    memset(&E, 0, sizeof(E));
     E.a = a;
     E.p6 = *p; E.p6.calc_amp = 0; E.p6.eq_cnt = 6;
     E.p18 = *p; E.p18.calc_amp = 1; E.p18.eq_cnt = 18;
@@ -2010,6 +2124,8 @@ int go_eig_direct(go_atmo *a, const go_params *p, const double src_in[3], const 
         double src[3] = {src_in[0], src_in[1], src_in[2] > ground(a) ? src_in[2] : ground(a)};   /* 3d.cpp:1048 */
         double ph_est = isnan(az_est_deg) ? atan2(rcvr_in[1] - src[1], rcvr_in[0] - src[0])     /* 3d.cpp:1050 */
                                           : GO_PI / 2.0 - az_est_deg * (GO_PI / 180.0);
+        go_eig_med M = strat_med(a);
+        E.med = &M;
         ok = find_eigenray(&E, src, rcvr_in, th_est, ph_est, freq, bnc, iterations, A);
     }
     if (ray_steps) *ray_steps = E.steps;

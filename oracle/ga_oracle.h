@@ -228,8 +228,11 @@ int    go_eig_direct(go_atmo *a, const go_params *p, const double src[3], const 
                      double incl_est_deg, double az_est_deg, int bnc, int iterations, double damping,
                      double tolerance, double freq, go_arrival *A, long long *ray_steps);
 
-/* A spherical medium's operations for the eigenray search (the stratified
- * medium and the range-dependent one of ga_oracle_rd.c). */
+/* A medium's operations for the eigenray search (the stratified media and
+ * the range-dependent ones of ga_oracle_rd.c).  Coordinates are those of the
+ * geometry's rows: spherical (r, lat, lon) with topo_z a radius and topo_dz
+ * n = 0 lat, 1 lon; 3D (x, y, z) with topo_z a height and topo_dz n = 0 x,
+ * 1 y. */
 typedef struct go_eig_med {
     void *m;
     double r0;
@@ -241,9 +244,32 @@ typedef struct go_eig_med {
     double (*att_seg)(void *m, const double *y0, const double *y1, double freq);
     double (*c)(void *m, double r, double lat, double lon);
     double (*amp)(void *m, const go_src *s, const double *y);
-    double (*topo_z)(void *m, double lat, double lon);          /* topo::z: a radius */
-    double (*topo_dz)(void *m, double lat, double lon, int n);  /* topo::dz, n = 0 lat, 1 lon */
+    double (*topo_z)(void *m, double lat, double lon);
+    double (*topo_dz)(void *m, double lat, double lon, int n);
+    int is_topo;      /* geoac::is_topo: the 3D LM update's terrain-slope form */
 } go_eig_med;
+
+/* The 3D search (est_eigenray + find_eigenray of geoac.eigenray.3d.cpp)
+ * over a medium: src = (x, y, z [km]) already raised to the ground, rcvr =
+ * (x, y) [km], th_* [rad].  Returns the eigenray count. */
+int    go_eig_search_3d_med(const go_eig_med *M, const go_params *p, const double src[3], const double rcvr[2],
+                            double th_min, double th_max, int bnc_min, int bnc_max, int iterations,
+                            double damping, double tolerance, double az_err_lim, double dth_big_deg,
+                            double dth_sml_deg, double freq, go_arrival *out, int out_cap, long long *ray_steps);
+/* find_eigenray alone over a medium from (th, ph) [rad]: 1 with *A when it accepts. */
+int    go_eig_find_3d_med(const go_eig_med *M, const go_params *p, const double src[3], const double rcvr[2],
+                          double th, double ph, int bnc, int iterations, double damping, double tolerance,
+                          double freq, go_arrival *A, long long *ray_steps);
+/* find_eigenray's ground-point derivatives at row K (yk):
+ * J = {dx/dth, dy/dth, dx/dph, dy/dph}. */
+void   go_eig_lm_jac_3d(const go_eig_med *M, const double src[3], const double *yk, double J[4]);
+/* Rows k-1 and k of the first leg of an 18-equation trace; k, or -1 on a break. */
+int    go_eig_trace_leg_3d(const go_eig_med *M, const go_params *p, const double src[3], double th, double ph,
+                           double y_km1[18], double y_k[18]);
+/* The accepted 3D eigenray's path rows (pts may be NULL) and row for launch
+ * angles th, ph [rad] over a medium. */
+long long go_eig_path_3d_med(const go_eig_med *M, const go_params *p, const double src[3], double th, double ph,
+                             int bnc_cnt, double freq, go_path_pt *pts, long long cap, go_arrival *A);
 
 /* The spherical search over a medium: src = (alt [km], lat, lon [rad]) already
  * raised to the ground, rcvr [rad], th_* [rad]; th_grnd is the lowest

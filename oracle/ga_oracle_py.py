@@ -511,6 +511,17 @@ def _rd_bind(L):
                              C.POINTER(Arrival), C.c_int, C.POINTER(C.c_int), C.POINTER(PathPt), C.c_longlong,
                              C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
     L.go_rd3_rhs.argtypes = [C.c_void_p, C.POINTER(Params), _dp, _dp]
+    L.go_rd3_eig_search.restype = C.c_int
+    L.go_rd3_eig_search.argtypes = L.go_rd_eig_search.argtypes
+    L.go_rd3_eig_direct.restype = C.c_int
+    L.go_rd3_eig_direct.argtypes = [C.c_void_p, C.POINTER(Params), _dp, _dp, C.c_double, C.c_double, C.c_int,
+                                    C.c_int, C.c_double, C.c_double, C.c_double, C.POINTER(Arrival),
+                                    C.POINTER(C.c_longlong)]
+    L.go_rd3_eig_path.restype = C.c_longlong
+    L.go_rd3_eig_path.argtypes = L.go_rd_eig_path.argtypes
+    L.go_rd3_lm_jac.argtypes = [C.c_void_p, _dp, _dp, _dp]
+    L.go_rd3_trace_leg.restype = C.c_int
+    L.go_rd3_trace_leg.argtypes = [C.c_void_p, C.POINTER(Params), _dp, C.c_double, C.c_double, _dp, _dp]
     L._rd_bound = True
 
 
@@ -875,3 +886,55 @@ class Rd3Medium(RdMedium):
                 break
             pcap = n_pts.value
         return arrivals_to_array(out, n_out.value), paths_to_array(pts, n_pts.value), steps.value
+
+    def eig_search(self, params, src, rcvr, incl_min=0.5, incl_max=45.0, bnc_min=0, bnc_max=0, iterations=25,
+                   damping=1.0e-3, tolerance=0.1, az_dev_lim=2.0, incl_step_max=0.1, incl_step_min=0.001, freq=0.1):
+        """infraga-3d-rngdep -eig_search (go_rd3_eig_search): (eigenrays, ray-steps); src = (x, y, z) km."""
+        s = np.ascontiguousarray(src, dtype=np.float64)
+        r = np.ascontiguousarray(rcvr, dtype=np.float64)
+        cap = 64
+        out = (Arrival * cap)()
+        steps = C.c_longlong(0)
+        n = self.o.lib.go_rd3_eig_search(self.h, C.byref(params), _ptr(s), _ptr(r), incl_min, incl_max, bnc_min,
+                                         bnc_max, iterations, damping, tolerance, az_dev_lim, incl_step_max,
+                                         incl_step_min, freq, out, cap, C.byref(steps))
+        return arrivals_to_array(out, min(n, cap)), steps.value
+
+    def eig_direct(self, params, src, rcvr, incl_est, az_est=float("nan"), bounces=0, iterations=25,
+                   damping=1.0e-3, tolerance=0.1, freq=0.1):
+        """infraga-3d-rngdep -eig_direct (go_rd3_eig_direct): (row or None, ray-steps)."""
+        s = np.ascontiguousarray(src, dtype=np.float64)
+        r = np.ascontiguousarray(rcvr, dtype=np.float64)
+        A = Arrival()
+        steps = C.c_longlong(0)
+        ok = self.o.lib.go_rd3_eig_direct(self.h, C.byref(params), _ptr(s), _ptr(r), float(incl_est), float(az_est),
+                                          int(bounces), int(iterations), damping, tolerance, freq, C.byref(A),
+                                          C.byref(steps))
+        row = arrivals_to_array((Arrival * 1)(A), 1)[0] if ok else None
+        return row, steps.value
+
+    def eig_path(self, params, src, incl_deg, az_deg, bounces, freq=0.1):
+        """The accepted eigenray's path rows and row (go_rd3_eig_path)."""
+        s = np.ascontiguousarray(src, dtype=np.float64)
+        A = Arrival()
+        n = self.o.lib.go_rd3_eig_path(self.h, C.byref(params), _ptr(s), float(incl_deg), float(az_deg),
+                                       int(bounces), freq, None, 0, C.byref(A))
+        pts = (PathPt * max(1, n))()
+        n = self.o.lib.go_rd3_eig_path(self.h, C.byref(params), _ptr(s), float(incl_deg), float(az_deg),
+                                       int(bounces), freq, pts, n, C.byref(A))
+        return paths_to_array(pts, n), arrivals_to_array((Arrival * 1)(A), 1)[0]
+
+    def lm_jac(self, src, yk):
+        """find_eigenray's d(x, y)/d(theta, phi) at row K: (dx/dth, dy/dth, dx/dph, dy/dph)."""
+        s = np.ascontiguousarray(src, dtype=np.float64)
+        y = np.ascontiguousarray(yk, dtype=np.float64)
+        J = np.zeros(4)
+        self.o.lib.go_rd3_lm_jac(self.h, _ptr(s), _ptr(y), _ptr(J))
+        return J
+
+    def trace_leg(self, params, src, th, ph):
+        """Rows k-1 and k of the first leg (18 equations) from (th, ph) [rad]; None on a break."""
+        s = np.ascontiguousarray(src, dtype=np.float64)
+        a, b = np.zeros(18), np.zeros(18)
+        k = self.o.lib.go_rd3_trace_leg(self.h, C.byref(params), _ptr(s), float(th), float(ph), _ptr(a), _ptr(b))
+        return None if k < 0 else (a, b)

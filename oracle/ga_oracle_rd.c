@@ -1692,6 +1692,104 @@ int go_rd3_fan(go_rd *a, const go_params *p, const go_prop *pr, const double *la
     return (cnt > out_cap || np > pts_cap) ? -1 : 0;
 }
 
+/* ---- eigenray search (infraga-3d-rngdep -eig_search / -eig_direct) ----
+ * The 3D search of geoac.eigenray.3d.cpp over the Cartesian medium: rows are
+ * (x, y, z), the medium is read at (z, x, y). */
+static void r3_init(void *m, const go_params *p, go_src *s, double *y) { go_rd3_set_initial((go_rd *)m, p, s, y); }
+static int r3_prop(void *m, const go_params *p, const go_src *s, double *rows, int length, int *check)
+{
+    return rd3_prop_rk4((go_rd *)m, p, s, rows, length, check);
+}
+static void r3_refl(void *m, const go_params *p, const go_src *s, const double *a, const double *b, double *o)
+{
+    go_rd3_set_refl((go_rd *)m, p, s, a, b, o);
+}
+static double r3_tt(void *m, const double *a, const double *b)
+{
+    double dt, da;
+    rd3_seg((go_rd *)m, a, b, 0.1, &dt, &da);
+    return dt;
+}
+static double r3_att(void *m, const double *a, const double *b, double f)
+{
+    double dt, da;
+    rd3_seg((go_rd *)m, a, b, f, &dt, &da);
+    return da;
+}
+static double r3_c(void *m, double x, double y, double z) { return go_rd_c((go_rd *)m, z, x, y); }
+static double r3_amp(void *m, const go_src *s, const double *y) { return go_rd3_amp((go_rd *)m, s, y); }
+static double r3_tz(void *m, double x, double y) { return go_rd_topo_z((go_rd *)m, x, y); }
+static double r3_tdz(void *m, double x, double y, int n) { return go_rd_topo_dz((go_rd *)m, x, y, n); }
+
+static go_eig_med rd3_med(go_rd *a)
+{
+    go_eig_med M = {a, 0.0, r3_init, r3_prop, r3_refl, r3_tt, r3_att, r3_c, r3_amp, r3_tz, r3_tdz, a->is_topo};
+    return M;
+}
+
+/* src = (x, y, z [km]) raised to the terrain (main.infraga.3d.rngdep.cpp:845, 1037). */
+static void rd3_eig_src(go_rd *a, const double src_in[3], double src[3])
+{
+    src[0] = src_in[0];
+    src[1] = src_in[1];
+    src[2] = dmax(src_in[2], go_rd_topo_z(a, src_in[0], src_in[1]));
+}
+
+/* run_eig_search (main.infraga.3d.rngdep.cpp:754-898): the sweep starts at
+ * theta_min for every bounce count (no ground-inclination floor). */
+int go_rd3_eig_search(go_rd *a, const go_params *p, const double src_in[3], const double rcvr[2], double th_min_deg,
+                      double th_max_deg, int bnc_min, int bnc_max, int iterations, double damping, double tolerance,
+                      double az_err_lim, double dth_big_deg, double dth_sml_deg, double freq, go_arrival *out,
+                      int out_cap, long long *ray_steps)
+{
+    double src[3];
+    rd3_eig_src(a, src_in, src);
+    go_eig_med M = rd3_med(a);
+    return go_eig_search_3d_med(&M, p, src, rcvr, th_min_deg * (RD_PI / 180.0), th_max_deg * (RD_PI / 180.0),
+                                bnc_min, bnc_max, iterations, damping, tolerance, az_err_lim, dth_big_deg,
+                                dth_sml_deg, freq, out, out_cap, ray_steps);
+}
+
+/* run_eig_direct (:901-1052): find_eigenray from (incl_est, az_est) [deg];
+ * az_est NaN aims at the receiver (:1039). */
+int go_rd3_eig_direct(go_rd *a, const go_params *p, const double src_in[3], const double rcvr[2], double incl_est_deg,
+                      double az_est_deg, int bnc, int iterations, double damping, double tolerance, double freq,
+                      go_arrival *A, long long *ray_steps)
+{
+    double src[3];
+    rd3_eig_src(a, src_in, src);
+    go_eig_med M = rd3_med(a);
+    double ph = isnan(az_est_deg) ? atan2(rcvr[1] - src[1], rcvr[0] - src[0])
+                                  : RD_PI / 2.0 - az_est_deg * (RD_PI / 180.0);
+    return go_eig_find_3d_med(&M, p, src, rcvr, incl_est_deg * (RD_PI / 180.0), ph, bnc, iterations, damping,
+                              tolerance, freq, A, ray_steps);
+}
+
+/* go_eig_lm_jac_3d / go_eig_trace_leg_3d over this medium (th, ph in radians,
+ * src as given). */
+void go_rd3_lm_jac(go_rd *a, const double src[3], const double *yk, double J[4])
+{
+    go_eig_med M = rd3_med(a);
+    go_eig_lm_jac_3d(&M, src, yk, J);
+}
+int go_rd3_trace_leg(go_rd *a, const go_params *p, const double src[3], double th, double ph, double y_km1[18],
+                     double y_k[18])
+{
+    go_eig_med M = rd3_med(a);
+    return go_eig_trace_leg_3d(&M, p, src, th, ph, y_km1, y_k);
+}
+
+/* The accepted eigenray's path rows and row for launch angles in degrees. */
+long long go_rd3_eig_path(go_rd *a, const go_params *p, const double src_in[3], double incl_deg, double az_deg,
+                          int bnc_cnt, double freq, go_path_pt *pts, long long cap, go_arrival *A)
+{
+    double src[3];
+    rd3_eig_src(a, src_in, src);
+    go_eig_med M = rd3_med(a);
+    return go_eig_path_3d_med(&M, p, src, incl_deg * (RD_PI / 180.0), (90.0 - az_deg) * (RD_PI / 180.0), bnc_cnt,
+                              freq, pts, cap, A);
+}
+
 void *go_rd_new_xy(const double *x_km, int nx, const double *y_km, int ny, const double *z, int nz,
                    const double *c, const double *u, const double *v, const double *rho)
 {

@@ -138,6 +138,20 @@ void   go_rd3_rhs(go_rd *a, const go_params *p, const double *y, double *f);
 void   go_rd3_set_initial(go_rd *a, const go_params *p, go_src *s, double *y);
 void   go_rd3_set_refl(go_rd *a, const go_params *p, const go_src *s, const double *rk2, const double *rk1, double *y0);
 double go_rd3_jacobian(go_rd *a, const double *y);
+/* infraga-3d-rngdep -eig_search / -eig_direct: the 3D search of
+ * geoac.eigenray.3d.cpp over this medium; src = (x, y, z) [km], rcvr = (x, y). */
+int    go_rd3_eig_search(go_rd *a, const go_params *p, const double src_in[3], const double rcvr[2],
+                         double th_min_deg, double th_max_deg, int bnc_min, int bnc_max, int iterations,
+                         double damping, double tolerance, double az_err_lim, double dth_big_deg,
+                         double dth_sml_deg, double freq, go_arrival *out, int out_cap, long long *ray_steps);
+int    go_rd3_eig_direct(go_rd *a, const go_params *p, const double src_in[3], const double rcvr[2],
+                         double incl_est_deg, double az_est_deg, int bnc, int iterations, double damping,
+                         double tolerance, double freq, go_arrival *A, long long *ray_steps);
+void   go_rd3_lm_jac(go_rd *a, const double src[3], const double *yk, double J[4]);
+int    go_rd3_trace_leg(go_rd *a, const go_params *p, const double src[3], double th, double ph, double y_km1[18],
+                        double y_k[18]);
+long long go_rd3_eig_path(go_rd *a, const go_params *p, const double src_in[3], double incl_deg, double az_deg,
+                          int bnc_cnt, double freq, go_path_pt *pts, long long cap, go_arrival *A);
 double go_rd3_amp(go_rd *a, const go_src *s, const double *y);
 int    go_rd3_fan(go_rd *a, const go_params *p, const go_prop *pr, const double *launches, int n_launch, int write,
                   int flags, go_arrival *out, int out_cap, int *n_out, go_path_pt *pts, long long pts_cap,

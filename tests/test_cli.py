@@ -123,7 +123,8 @@ def test_accel_launch_list_is_the_mpi_drivers(toy_met, tmp_path):
     ("-prop", "prefix", "lats.dat", "lons.dat", "write_topo=true"),
     ("-eig_search", "prefix", "lats.dat"),
     ("-eig_search", "prefix", "lats.dat", "lons.dat", "no_such_key=1"),
-    ("-eig_direct", "prefix", "lats.dat", "lons.dat"),
+    ("-eig_direct", "prefix", "lats.dat"),
+    ("-wnl_wvfrm", "prefix", "lats.dat", "lons.dat"),
 ])
 def test_rngdep_rejected_arguments(args):
     r = run("infraga-sph-rngdep", *args)
