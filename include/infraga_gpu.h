@@ -205,6 +205,18 @@ int ga_atmo_load_strat_topo(ga_ctx *ctx, const double *z_km, const double *c_kms
                             const double *lon_deg, int nlon, const double *zg_km, int *atmo_id);
 int ga_atmo_load_file_topo(ga_ctx *ctx, const char *path, const char *topo_path, const char *format,
                            int reverse_winds, int *atmo_id);
+/* The 3D forms (infraga-3d with topo_file; set_region, src/atmo/
+ * atmo_io.3d.strat.cpp:146-257, with atmo_state.3d.strat.cpp): terrain node
+ * coordinates x, y in km ("x y z" rows, y fastest, in the file form); the
+ * id runs with GA_GEOM_3D_RNGDEP under the stratified 3D semantics (boundary
+ * layer top at the terrain maximum + 0.1 + 2 km, the upper-triangle
+ * second-derivative contraction of geoac.eqset.3d.strat.cpp:305-310, turning
+ * height from the profile bottom, caustic rows floored at the terrain). */
+int ga_atmo_load_strat_topo_xy(ga_ctx *ctx, const double *z_km, const double *c_kms, const double *u_kms,
+                               const double *v_kms, const double *rho_gcc, int n, const double *x_km, int nx,
+                               const double *y_km, int ny, const double *zg_km, int *atmo_id);
+int ga_atmo_load_file_topo_xy(ga_ctx *ctx, const char *path, const char *topo_path, const char *format,
+                              int reverse_winds, int *atmo_id);
 
 /* Trace a fan.  launches holds n_launch (incl_deg, az_deg) pairs.  Arrivals
  * are returned sorted by (launch_idx, n_bounce).  When out_cap is too small

@@ -3056,7 +3056,7 @@ int ga_rngdep_load_files_xy(ga_ctx *ctx, const char *prefix, const char *x_path,
 // range-dependent kernels run the stratified medium with terrain.
 static int strat_topo_build(ga_ctx *ctx, const double *z, const double *c, const double *u, const double *v,
                             const double *rho, int nz, const double *tl, int n1, const double *to, int n2,
-                            const double *tz, int *atmo_id)
+                            const double *tz, int *atmo_id, bool cart = false)
 {
     if (!ctx || !z || !c || !u || !v || !rho || !tl || !to || !tz || !atmo_id) return fail(ctx, GA_E_ARG, "null argument");
     if (nz < 3) return fail(ctx, GA_E_ARG, "profile needs >= 3 levels");
@@ -3078,7 +3078,7 @@ static int strat_topo_build(ga_ctx *ctx, const double *z, const double *c, const
             rr[q * nz + k] = rho[k];
         }
     int id = 0;
-    int rc = rngdep_load(ctx, la, NC, lo, NC, z, nz, cc.data(), uu.data(), vv.data(), rr.data(), &id, false);
+    int rc = rngdep_load(ctx, la, NC, lo, NC, z, nz, cc.data(), uu.data(), vv.data(), rr.data(), &id, cart);
     if (rc != GA_OK) return rc;
     rc = ga_topo_load(ctx, id, tl, n1, to, n2, tz);
     if (rc != GA_OK) {
@@ -3107,6 +3107,30 @@ int ga_atmo_load_file_topo(ga_ctx *ctx, const char *path, const char *topo_path,
     if (rc != GA_OK) return rc;
     return strat_topo_build(ctx, z.data(), c.data(), u.data(), v.data(), rho.data(), (int)z.size(), tl.data(),
                             (int)tl.size(), to.data(), (int)to.size(), tz.data(), atmo_id);
+}
+
+// The 3D stratified profile over terrain (infraga-3d with topo_file,
+// set_region of atmo_io.3d.strat.cpp:146-257): node coordinates and terrain
+// rows in km, the Cartesian range-dependent kernels with the stratified 3D
+// semantics (boundary-layer top, upper-triangle contraction, turning height
+// from the profile bottom, floored caustic rows).
+int ga_atmo_load_strat_topo_xy(ga_ctx *ctx, const double *z_km, const double *c_kms, const double *u_kms,
+                               const double *v_kms, const double *rho_gcc, int n, const double *x_km, int nx,
+                               const double *y_km, int ny, const double *zg_km, int *atmo_id)
+{
+    return strat_topo_build(ctx, z_km, c_kms, u_kms, v_kms, rho_gcc, n, x_km, nx, y_km, ny, zg_km, atmo_id, true);
+}
+
+int ga_atmo_load_file_topo_xy(ga_ctx *ctx, const char *path, const char *topo_path, const char *format,
+                              int reverse_winds, int *atmo_id)
+{
+    if (!ctx || !path || !topo_path || !atmo_id) return fail(ctx, GA_E_ARG, "null argument");
+    std::vector<double> z, c, u, v, rho, tx, ty, tz;
+    int rc = read_met(ctx, path, format, reverse_winds, z, c, u, v, rho);
+    if (rc == GA_OK) rc = read_terrain(ctx, topo_path, tx, ty, tz);
+    if (rc != GA_OK) return rc;
+    return strat_topo_build(ctx, z.data(), c.data(), u.data(), v.data(), rho.data(), (int)z.size(), tx.data(),
+                            (int)tx.size(), ty.data(), (int)ty.size(), tz.data(), atmo_id, true);
 }
 
 int ga_rngdep_eval(ga_ctx *ctx, int rd_id, const double *pts, int n, double *atmo_out, double *topo_out)

@@ -849,7 +849,11 @@ GA_HD int c3(int n) { return n == 2 ? 0 : n + 1; }   // Cartesian index -> mediu
 // update_refs + eval_src_eq (geoac.eqset.3d.rngdep.cpp:226-382): c and its
 // derivatives at the clamped point (atmo::c, dc, ddc), winds of calc_uvw,
 // the full symmetric second-derivative contraction and the refs.dv[3]
-// accumulation of X_ph (:288), which leaves refs.dv[4] at zero.
+// accumulation of X_ph (:288), which leaves refs.dv[4] at zero.  A
+// stratified profile over terrain (M.strat, infraga-3d with topo_file)
+// contracts the upper triangle only (geoac.eqset.3d.strat.cpp:305-310): the
+// lower terms enter with weight zero, which leaves every sum as the
+// reference forms it.
 template <int NEQ, class V, class SV>
 GA_HD void rd3_stage(const RdMed &M, const Stage &S, const V &y, const SV &stg, const V &acc)
 {
@@ -898,9 +902,10 @@ GA_HD void rd3_stage(const RdMed &M, const Stage &S, const V &y, const SV &stg, 
 #pragma unroll
                 for (int m = 0; m < 3; ++m) {
                     const int q = p6(c3(n), c3(m));
-                    ddc[n] += Xj[m] * A.ddc[q];
-                    ddu[n] += Xj[m] * A.ddu[q];
-                    ddv[n] += Xj[m] * A.ddv[q];
+                    const double xm = (M.strat && m < n) ? 0.0 : Xj[m];
+                    ddc[n] += xm * A.ddc[q];
+                    ddu[n] += xm * A.ddu[q];
+                    ddv[n] += xm * A.ddv[q];
                 }
             }
             const double dvj = j ? 0.0 : dv3;
@@ -1190,7 +1195,10 @@ GA_HD void rd_lane_loop(const RdFanArgs &A, int lane, NextRay next_ray, V ya, V 
             else rd_init_ray<NEQ>(A, th, ph, y0, c0, nu0s);
 #pragma unroll
             for (int i = 0; i < NEQ; ++i) y[i] = y0[i];
-            s = 0.0; tt = 0.0; att = 0.0; ttl = 0.0; attl = 0.0; rmax = 0.0;
+            s = 0.0; tt = 0.0; att = 0.0; ttl = 0.0; attl = 0.0;
+            // the stratified 3D driver starts the turning height at the
+            // profile bottom (main.infraga.3d.cpp:426)
+            rmax = (CART && M.strat) ? M.z0 : 0.0;
             k = 0; bnc = 0; nst = 0; narr = 0;
             fetch = false;
         }
@@ -1247,9 +1255,11 @@ GA_HD void rd_lane_loop(const RdFanArgs &A, int lane, NextRay next_ray, V ya, V 
                             R.launch_idx = ray; R.leg = bnc; R.step = k; R.kind = GA_PT_CAUSTIC;
                             R.pos0 = n[1] * (180.0 / PI); R.pos1 = n[2] * (180.0 / PI); R.z = n[0] - M.r0;
                             R.tl = 0.0; R.absorption = -2.0 * att; R.time = tt;
-                            if (CART) {   // caustic rows keep the raw z (main.infraga.3d.rngdep.cpp:467-471)
+                            if (CART) {
+                                // caustic rows keep the raw z (main.infraga.3d.rngdep.cpp:467-471);
+                                // the stratified driver floors it at the terrain (main.infraga.3d.cpp:469-472)
                                 rd3_path_xyz(M, n, R);
-                                R.z = n[2];
+                                if (!M.strat) R.z = n[2];
                             }
                             path_put(A.path, pl, ray, R);
                         }

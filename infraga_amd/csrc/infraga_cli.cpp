@@ -13,7 +13,7 @@
 //
 // Differences from the reference drivers, by design:
 //  - an unknown key is an error (exit 2) instead of an interactive prompt;
-//  - write_topo is rejected, and so is topo_file in the 3D stratified binary;
+//  - write_topo is rejected;
 //  - verbose=true prints no trace-by-trace log.
 #include <cmath>
 #include <cstdio>
@@ -280,7 +280,7 @@ int run_prop(int argc, char **argv)
     ga_prop_params_default(&p, GA_CLI_GEOM);
     double z_grnd = NAN;
     const char *topo_file = nullptr;
-    bool src_set[2] = {false, false}, bounds_set[4] = {false, false, false, false};
+    bool src_set[2] = {false, false}, bounds_set[4] = {false, false, false, false}, rng_set = false;
 
     for (int i = 3; i < argc; i++) {
         const char *a = argv[i], *v = nullptr;
@@ -296,24 +296,24 @@ int run_prop(int argc, char **argv)
         else if (key2(a, "src_alt=", "alt_src=", v)) z_src = std::atof(v);
         else if (sph && key2(a, "src_lat=", "lat_src=", v)) { s0 = std::atof(v); src_set[0] = true; }
         else if (sph && key2(a, "src_lon=", "lon_src=", v)) { s1 = std::atof(v); src_set[1] = true; }
-        else if (!sph && key2(a, "src_x=", "x_src=", v)) s0 = std::atof(v);
-        else if (!sph && key2(a, "src_y=", "y_src=", v)) s1 = std::atof(v);
+        else if (!sph && key2(a, "src_x=", "x_src=", v)) { s0 = std::atof(v); src_set[0] = true; }
+        else if (!sph && key2(a, "src_y=", "y_src=", v)) { s1 = std::atof(v); src_set[1] = true; }
         else if (key(a, "turn_ht_min=", v)) turn_ht_min = std::atof(v);
         else if (key(a, "write_rays=", v)) write_rays = str2bool(v);
         else if (key(a, "freq=", v)) freq = std::atof(v);
         else if (key(a, "abs_coeff=", v)) p.abs_coeff = std::max(0.0, std::atof(v));
         else if (key(a, "calc_amp=", v)) p.calc_amp = str2bool(v) ? 1 : 0;
         else if (key2(a, "max_alt=", "alt_max=", v)) p.alt_max = std::atof(v);
-        else if (key2(a, "max_rng=", "rng_max=", v)) p.rng_max = std::atof(v);
+        else if (key2(a, "max_rng=", "rng_max=", v)) { p.rng_max = std::atof(v); rng_set = true; }
         else if (sph && key2(a, "max_time=", "time_max=", v)) p.time_max = std::atof(v);
         else if (sph && key2(a, "min_lat=", "lat_min=", v)) { p.bounds[0] = std::atof(v); bounds_set[0] = true; }
         else if (sph && key2(a, "max_lat=", "lat_max=", v)) { p.bounds[1] = std::atof(v); bounds_set[1] = true; }
         else if (sph && key2(a, "min_lon=", "lon_min=", v)) { p.bounds[2] = std::atof(v); bounds_set[2] = true; }
         else if (sph && key2(a, "max_lon=", "lon_max=", v)) { p.bounds[3] = std::atof(v); bounds_set[3] = true; }
-        else if (!sph && key2(a, "min_x=", "x_min=", v)) p.bounds[0] = std::atof(v);
-        else if (!sph && key2(a, "max_x=", "x_max=", v)) p.bounds[1] = std::atof(v);
-        else if (!sph && key2(a, "min_y=", "y_min=", v)) p.bounds[2] = std::atof(v);
-        else if (!sph && key2(a, "max_y=", "y_max=", v)) p.bounds[3] = std::atof(v);
+        else if (!sph && key2(a, "min_x=", "x_min=", v)) { p.bounds[0] = std::atof(v); bounds_set[0] = true; }
+        else if (!sph && key2(a, "max_x=", "x_max=", v)) { p.bounds[1] = std::atof(v); bounds_set[1] = true; }
+        else if (!sph && key2(a, "min_y=", "y_min=", v)) { p.bounds[2] = std::atof(v); bounds_set[2] = true; }
+        else if (!sph && key2(a, "max_y=", "y_max=", v)) { p.bounds[3] = std::atof(v); bounds_set[3] = true; }
         else if (key2(a, "min_ds=", "ds_min=", v)) p.ds_min = std::atof(v);
         else if (key2(a, "max_ds=", "ds_max=", v)) p.ds_max = std::atof(v);
         else if (key2(a, "max_s=", "s_max=", v)) p.s_max = std::atof(v);
@@ -338,26 +338,25 @@ int run_prop(int argc, char **argv)
     }
 
     if (write_caustics) p.calc_amp = 1;   // caustics need the Jacobian (main.infraga.sph.cpp:315, 3d.cpp:308)
-    if (topo_file && !sph) {
-        std::cerr << GA_CLI_NAME << ": topo_file is not supported by this engine's 3D binary\n";
-        return 2;
-    }
     if (!topo_file) p.topo_use_blw = 0;   // the vertical wind needs terrain (atmo::w)
     ga_ctx *ctx = nullptr;
     if (ga_ctx_create(&ctx, 0) != GA_OK) return die(nullptr, "no usable GPU");
     int aid = 0;
     if (topo_file) {
         // the stratified profile over terrain (set_region with topo_file,
-        // main.infraga.sph.cpp:231-240): the range-dependent kernels over
-        // identical columns with the stratified semantics
-        if (ga_atmo_load_file_topo(ctx, argv[2], topo_file, prof_format, reverse_winds ? 1 : 0, &aid) != GA_OK)
+        // main.infraga.sph.cpp:231-240, main.infraga.3d.cpp:229-238): the
+        // range-dependent kernels over identical columns with the
+        // stratified semantics (terrain nodes in km for the 3D binary)
+        if ((sph ? ga_atmo_load_file_topo : ga_atmo_load_file_topo_xy)(ctx, argv[2], topo_file, prof_format,
+                                                                       reverse_winds ? 1 : 0, &aid) != GA_OK)
             return die(ctx, "cannot load the atmosphere and terrain");
         ga_prop_params q;
-        ga_prop_params_default(&q, GA_GEOM_SPH_RNGDEP);
+        ga_prop_params_default(&q, sph ? GA_GEOM_SPH_RNGDEP : GA_GEOM_3D_RNGDEP);
         q.calc_amp = p.calc_amp;
         q.abs_coeff = p.abs_coeff;
         q.alt_max = p.alt_max;
-        q.rng_max = p.rng_max;
+        // the 3D set_limits leaves rng_max at 1000 with terrain (atmo_io.3d.strat.cpp:26-33)
+        q.rng_max = (sph || rng_set) ? p.rng_max : 1000.0;
         q.time_max = p.time_max;
         q.ds_min = p.ds_min;
         q.ds_max = p.ds_max;
@@ -369,10 +368,11 @@ int run_prop(int argc, char **argv)
         q.turn_ht_min = turn_ht_min;
         double info[8];
         if (ga_rngdep_info(ctx, aid, info) != GA_OK) return die(ctx, "region");
-        // source at the terrain's centre unless given (:236-239)
-        if (!src_set[0]) s0 = (info[1] + info[0]) / 2.0 * (180.0 / Pi);
-        if (!src_set[1]) s1 = (info[3] + info[2]) / 2.0 * (180.0 / Pi);
-        const double la = s0 * (Pi / 180.0), lo = s1 * (Pi / 180.0);
+        // source at the terrain's centre unless given (sph :236-239, 3D :235-238)
+        const double deg = sph ? Pi / 180.0 : 1.0;
+        if (!src_set[0]) s0 = (info[1] + info[0]) / 2.0 / deg;
+        if (!src_set[1]) s1 = (info[3] + info[2]) / 2.0 / deg;
+        const double la = s0 * deg, lo = s1 * deg;
         double pt[3] = {la, lo, 0.0}, atmo_out[40], topo_out[10];
         if (ga_rngdep_eval(ctx, aid, pt, 1, atmo_out, topo_out) != GA_OK) return die(ctx, "terrain at the source");
         const double zg = topo_out[0];
@@ -384,10 +384,10 @@ int run_prop(int argc, char **argv)
         std::vector<int> az_end;
         const double r0 = q.r_earth;
         for (double phi = phi_min; phi <= phi_max; phi += phi_step) {
-            double theta_grnd = -89.9;   // :411-423
+            double theta_grnd = -89.9;   // sph :411-423, 3D :408-417
             if (z_src - zg < 0.01) {
-                const double dzgdx = 1.0 / (r0 * std::cos(la)) * topo_out[2];
-                const double dzgdy = 1.0 / r0 * topo_out[1];
+                const double dzgdx = sph ? 1.0 / (r0 * std::cos(la)) * topo_out[2] : topo_out[1];
+                const double dzgdy = sph ? 1.0 / r0 * topo_out[1] : topo_out[2];
                 theta_grnd = std::atan(dzgdx * std::sin(phi * Pi / 180.0) + dzgdy * std::cos(phi * Pi / 180.0)) * (180.0 / Pi) + 0.1;
             }
             incl_loop(theta_min, theta_max, theta_grnd, theta_step, phi, launches);
@@ -399,13 +399,13 @@ int run_prop(int argc, char **argv)
         ga_run_stats st;
         std::string id = custom_id ? std::string(output_id) : strip_ext(argv[2]);
         if (write_rays || write_caustics) {
-            if (fan_with_paths(ctx, aid, q, launches, write_rays, write_caustics, id, true, arr, n_out, st) != GA_OK)
+            if (fan_with_paths(ctx, aid, q, launches, write_rays, write_caustics, id, sph, arr, n_out, st) != GA_OK)
                 return die(ctx, "fan failed");
         } else if (ga_fan_run(ctx, aid, &q, launches.data(), n, arr.data(), (int)arr.size(), &n_out, &st) != GA_OK) {
             return die(ctx, "fan failed");
         }
         write_arrivals(id, argv[2], theta_min, theta_max, theta_step, phi_min, phi_max, phi_step, bounces, s0, s1,
-                       z_src, nullptr, topo_file, turn_ht_min, freq, q.abs_coeff, true, arr, n_out, n, az_end);
+                       z_src, nullptr, topo_file, turn_ht_min, freq, q.abs_coeff, sph, arr, n_out, n, az_end);
         std::cout << GA_CLI_NAME << ": " << n << " rays, " << n_out << " arrivals, " << st.ray_steps << " ray-steps, "
                   << st.kernel_ms << " ms on the GPU\n";
         ga_ctx_destroy(ctx);
@@ -672,8 +672,8 @@ int run_eig_search(int argc, char **argv)
     ga_eig_params_default(&e);
     double z_grnd = NAN;
     const char *topo_file = nullptr;
-    bool set[4] = {false, false, false, false};   // src lat, src lon, rcvr lat, rcvr lon given
-    bool bounds_set[4] = {false, false, false, false};
+    bool set[4] = {false, false, false, false};   // src lat/x, src lon/y, rcvr lat/x, rcvr lon/y given
+    bool bounds_set[4] = {false, false, false, false}, rng_set = false;
 
     for (int i = 3; i < argc; i++) {
         const char *a = argv[i], *v = nullptr;
@@ -687,10 +687,10 @@ int run_eig_search(int argc, char **argv)
         else if (sph && key2(a, "src_lon=", "lon_src=", v)) { s1 = std::atof(v); set[1] = true; }
         else if (sph && key2(a, "rcvr_lat=", "lat_rcvr=", v)) { r0 = std::atof(v); set[2] = true; }
         else if (sph && key2(a, "rcvr_lon=", "lon_rcvr=", v)) { r1 = std::atof(v); set[3] = true; }
-        else if (!sph && key2(a, "src_x=", "x_src=", v)) s0 = std::atof(v);
-        else if (!sph && key2(a, "src_y=", "y_src=", v)) s1 = std::atof(v);
-        else if (!sph && key2(a, "rcvr_x=", "x_rcvr=", v)) r0 = std::atof(v);
-        else if (!sph && key2(a, "rcvr_y=", "y_rcvr=", v)) r1 = std::atof(v);
+        else if (!sph && key2(a, "src_x=", "x_src=", v)) { s0 = std::atof(v); set[0] = true; }
+        else if (!sph && key2(a, "src_y=", "y_src=", v)) { s1 = std::atof(v); set[1] = true; }
+        else if (!sph && key2(a, "rcvr_x=", "x_rcvr=", v)) { r0 = std::atof(v); set[2] = true; }
+        else if (!sph && key2(a, "rcvr_y=", "y_rcvr=", v)) { r1 = std::atof(v); set[3] = true; }
         else if (key(a, "verbose=", v)) { /* trace-by-trace log not produced */ }
         else if (key(a, "iterations=", v)) e.iterations = (int)std::atof(v);
         else if (key(a, "damping=", v)) e.damping = std::atof(v);
@@ -701,16 +701,16 @@ int run_eig_search(int argc, char **argv)
         else if (key(a, "freq=", v)) freq = std::atof(v);
         else if (key(a, "abs_coeff=", v)) p.abs_coeff = std::max(0.0, std::atof(v));
         else if (key2(a, "max_alt=", "alt_max=", v)) p.alt_max = std::atof(v);
-        else if (key2(a, "max_rng=", "rng_max=", v)) p.rng_max = std::atof(v);
+        else if (key2(a, "max_rng=", "rng_max=", v)) { p.rng_max = std::atof(v); rng_set = true; }
         else if (sph && key2(a, "max_time=", "time_max=", v)) p.time_max = std::atof(v);
         else if (sph && key2(a, "min_lat=", "lat_min=", v)) { p.bounds[0] = std::atof(v); bounds_set[0] = true; }
         else if (sph && key2(a, "max_lat=", "lat_max=", v)) { p.bounds[1] = std::atof(v); bounds_set[1] = true; }
         else if (sph && key2(a, "min_lon=", "lon_min=", v)) { p.bounds[2] = std::atof(v); bounds_set[2] = true; }
         else if (sph && key2(a, "max_lon=", "lon_max=", v)) { p.bounds[3] = std::atof(v); bounds_set[3] = true; }
-        else if (!sph && key2(a, "min_x=", "x_min=", v)) p.bounds[0] = std::atof(v);
-        else if (!sph && key2(a, "max_x=", "x_max=", v)) p.bounds[1] = std::atof(v);
-        else if (!sph && key2(a, "min_y=", "y_min=", v)) p.bounds[2] = std::atof(v);
-        else if (!sph && key2(a, "max_y=", "y_max=", v)) p.bounds[3] = std::atof(v);
+        else if (!sph && key2(a, "min_x=", "x_min=", v)) { p.bounds[0] = std::atof(v); bounds_set[0] = true; }
+        else if (!sph && key2(a, "max_x=", "x_max=", v)) { p.bounds[1] = std::atof(v); bounds_set[1] = true; }
+        else if (!sph && key2(a, "min_y=", "y_min=", v)) { p.bounds[2] = std::atof(v); bounds_set[2] = true; }
+        else if (!sph && key2(a, "max_y=", "y_max=", v)) { p.bounds[3] = std::atof(v); bounds_set[3] = true; }
         else if (key2(a, "min_ds=", "ds_min=", v)) p.ds_min = std::atof(v);
         else if (key2(a, "max_ds=", "ds_max=", v)) p.ds_max = std::atof(v);
         else if (key2(a, "max_s=", "s_max=", v)) p.s_max = std::atof(v);
@@ -727,10 +727,6 @@ int run_eig_search(int argc, char **argv)
         }
     }
 
-    if (topo_file && !sph) {
-        std::cerr << GA_CLI_NAME << ": topo_file is not supported by this engine's 3D binary\n";
-        return 2;
-    }
     if (!topo_file) p.topo_use_blw = 0;   // the vertical wind needs terrain (atmo::w)
     ga_ctx *ctx = nullptr;
     if (ga_ctx_create(&ctx, 0) != GA_OK) return die(nullptr, "no usable GPU");
@@ -738,27 +734,30 @@ int run_eig_search(int argc, char **argv)
     double z0 = 0.0, z_rcvr = 0.0;
     if (topo_file) {
         // the stratified profile over terrain (main.infraga.sph.cpp:819-827,
-        // 891-899): source at the terrain's centre, receiver 2.5 deg east of
-        // it, unless given; the search runs over identical columns
-        if (ga_atmo_load_file_topo(ctx, argv[2], topo_file, prof_format, reverse_winds ? 1 : 0, &aid) != GA_OK)
+        // 891-899; main.infraga.3d.cpp:812-815): source at the terrain's
+        // centre, receiver 2.5 deg east of it (3D: 250 km west), unless
+        // given; the search runs over identical columns
+        if ((sph ? ga_atmo_load_file_topo : ga_atmo_load_file_topo_xy)(ctx, argv[2], topo_file, prof_format,
+                                                                       reverse_winds ? 1 : 0, &aid) != GA_OK)
             return die(ctx, "cannot load the atmosphere and terrain");
         double info[8];
         if (ga_rngdep_info(ctx, aid, info) != GA_OK) return die(ctx, "region");
-        const double clat = (info[1] + info[0]) / 2.0 * (180.0 / Pi), clon = (info[3] + info[2]) / 2.0 * (180.0 / Pi);
+        const double deg = sph ? Pi / 180.0 : 1.0;
+        const double clat = (info[1] + info[0]) / 2.0 / deg, clon = (info[3] + info[2]) / 2.0 / deg;
         if (!set[0]) s0 = clat;
         if (!set[1]) s1 = clon;
-        if (!set[2]) r0 = clat + 0.0;
-        if (!set[3]) r1 = clon + 2.5;
-        double pt[6] = {s0 * (Pi / 180.0), s1 * (Pi / 180.0), 0.0, r0 * (Pi / 180.0), r1 * (Pi / 180.0), 0.0};
+        if (!set[2]) r0 = sph ? clat + 0.0 : clat - 250.0;
+        if (!set[3]) r1 = sph ? clon + 2.5 : clon;
+        double pt[6] = {s0 * deg, s1 * deg, 0.0, r0 * deg, r1 * deg, 0.0};
         double aout[80], tout[20];
         if (ga_rngdep_eval(ctx, aid, pt, 2, aout, tout) != GA_OK) return die(ctx, "terrain");
         z_src = std::max(z_src, tout[0]);
         z_rcvr = tout[10];
         ga_prop_params q;
-        ga_prop_params_default(&q, GA_GEOM_SPH_RNGDEP);
+        ga_prop_params_default(&q, sph ? GA_GEOM_SPH_RNGDEP : GA_GEOM_3D_RNGDEP);
         q.abs_coeff = p.abs_coeff;
         q.alt_max = p.alt_max;
-        q.rng_max = p.rng_max;
+        q.rng_max = (sph || rng_set) ? p.rng_max : 1000.0;
         q.time_max = p.time_max;
         q.ds_min = p.ds_min;
         q.ds_max = p.ds_max;
@@ -826,7 +825,7 @@ int run_eig_search(int argc, char **argv)
             << (r1 * (Pi / 180.0)) * (180.0 / Pi) << ", " << z_rcvr << '\n';
     } else {
         out << "# " << '\t' << "source location: " << s0 << ", " << s1 << ", " << z_src << '\n';
-        out << "# " << '\t' << "receiver location: " << r0 << ", " << r1 << ", " << z0 << '\n';
+        out << "# " << '\t' << "receiver location: " << r0 << ", " << r1 << ", " << z_rcvr << '\n';
     }
     out << "# " << '\t' << "inclination range: " << (e.incl_min * (Pi / 180.0)) * (180.0 / Pi) << ", "
         << (e.incl_max * (Pi / 180.0)) * (180.0 / Pi) << '\n';
@@ -1283,6 +1282,9 @@ int run_eig_direct(int argc, char **argv)
     ga_eig_params e;
     ga_eig_params_default(&e);
     double z_grnd = NAN;
+    const char *topo_file = nullptr;
+    bool set[4] = {false, false, false, false};   // src lat/x, src lon/y, rcvr lat/x, rcvr lon/y given
+    bool bounds_set[4] = {false, false, false, false}, rng_set = false;
 
     for (int i = 3; i < argc; i++) {
         const char *a = argv[i], *v = nullptr;
@@ -1290,14 +1292,14 @@ int run_eig_direct(int argc, char **argv)
         else if (key2(a, "az_est=", "est_az=", v)) az_est = std::atof(v);
         else if (key(a, "bounces=", v)) bounces = std::atoi(v);
         else if (key2(a, "src_alt=", "alt_src=", v)) z_src = std::atof(v);
-        else if (sph && key2(a, "src_lat=", "lat_src=", v)) s0 = std::atof(v);
-        else if (sph && key2(a, "src_lon=", "lon_src=", v)) s1 = std::atof(v);
-        else if (sph && key2(a, "rcvr_lat=", "lat_rcvr=", v)) r0 = std::atof(v);
-        else if (sph && key2(a, "rcvr_lon=", "lon_rcvr=", v)) r1 = std::atof(v);
-        else if (!sph && key2(a, "src_x=", "x_src=", v)) s0 = std::atof(v);
-        else if (!sph && key2(a, "src_y=", "y_src_y=", v)) s1 = std::atof(v);
-        else if (!sph && key2(a, "rcvr_x=", "x_rcvr=", v)) r0 = std::atof(v);
-        else if (!sph && key2(a, "rcvr_y=", "y_rcvr=", v)) r1 = std::atof(v);
+        else if (sph && key2(a, "src_lat=", "lat_src=", v)) { s0 = std::atof(v); set[0] = true; }
+        else if (sph && key2(a, "src_lon=", "lon_src=", v)) { s1 = std::atof(v); set[1] = true; }
+        else if (sph && key2(a, "rcvr_lat=", "lat_rcvr=", v)) { r0 = std::atof(v); set[2] = true; }
+        else if (sph && key2(a, "rcvr_lon=", "lon_rcvr=", v)) { r1 = std::atof(v); set[3] = true; }
+        else if (!sph && key2(a, "src_x=", "x_src=", v)) { s0 = std::atof(v); set[0] = true; }
+        else if (!sph && key2(a, "src_y=", "y_src_y=", v)) { s1 = std::atof(v); set[1] = true; }
+        else if (!sph && key2(a, "rcvr_x=", "x_rcvr=", v)) { r0 = std::atof(v); set[2] = true; }
+        else if (!sph && key2(a, "rcvr_y=", "y_rcvr=", v)) { r1 = std::atof(v); set[3] = true; }
         else if (key(a, "verbose=", v)) { /* trace-by-trace log not produced */ }
         else if (key(a, "iterations=", v)) e.iterations = (int)std::atof(v);
         else if (key(a, "damping=", v)) e.damping = std::atof(v);
@@ -1305,16 +1307,12 @@ int run_eig_direct(int argc, char **argv)
         else if (key(a, "freq=", v)) freq = std::atof(v);
         else if (key(a, "abs_coeff=", v)) p.abs_coeff = std::max(0.0, std::atof(v));
         else if (key2(a, "max_alt=", "alt_max=", v)) p.alt_max = std::atof(v);
-        else if (key2(a, "max_rng=", "rng_max=", v)) p.rng_max = std::atof(v);
+        else if (key2(a, "max_rng=", "rng_max=", v)) { p.rng_max = std::atof(v); rng_set = true; }
         else if (sph && key2(a, "max_time=", "time_max=", v)) p.time_max = std::atof(v);
-        else if (sph && key2(a, "min_lat=", "lat_min=", v)) p.bounds[0] = std::atof(v);
-        else if (sph && key2(a, "max_lat=", "lat_max=", v)) p.bounds[1] = std::atof(v);
-        else if (sph && key2(a, "min_lon=", "lon_min=", v)) p.bounds[2] = std::atof(v);
-        else if (sph && key2(a, "max_lon=", "lon_max=", v)) p.bounds[3] = std::atof(v);
-        else if (!sph && key2(a, "min_x=", "x_min=", v)) p.bounds[0] = std::atof(v);
-        else if (!sph && key2(a, "max_x=", "x_max=", v)) p.bounds[1] = std::atof(v);
-        else if (!sph && key2(a, "min_y=", "y_min=", v)) p.bounds[2] = std::atof(v);
-        else if (!sph && key2(a, "max_y=", "y_max=", v)) p.bounds[3] = std::atof(v);
+        else if (key2(a, sph ? "min_lat=" : "min_x=", sph ? "lat_min=" : "x_min=", v)) { p.bounds[0] = std::atof(v); bounds_set[0] = true; }
+        else if (key2(a, sph ? "max_lat=" : "max_x=", sph ? "lat_max=" : "x_max=", v)) { p.bounds[1] = std::atof(v); bounds_set[1] = true; }
+        else if (key2(a, sph ? "min_lon=" : "min_y=", sph ? "lon_min=" : "y_min=", v)) { p.bounds[2] = std::atof(v); bounds_set[2] = true; }
+        else if (key2(a, sph ? "max_lon=" : "max_y=", sph ? "lon_max=" : "y_max=", v)) { p.bounds[3] = std::atof(v); bounds_set[3] = true; }
         else if (key2(a, "min_ds=", "ds_min=", v)) p.ds_min = std::atof(v);
         else if (key2(a, "max_ds=", "ds_max=", v)) p.ds_max = std::atof(v);
         else if (key2(a, "max_s=", "s_max=", v)) p.s_max = std::atof(v);
@@ -1323,12 +1321,9 @@ int run_eig_direct(int argc, char **argv)
         else if (key(a, "reverse_winds=", v)) reverse_winds = str2bool(v);
         else if (key(a, "output_id=", v)) { custom_id = true; output_id = v; }
         else if (key(a, "z_grnd=", v)) z_grnd = std::atof(v);
-        else if (key(a, "topo_file=", v) || key(a, "topo_use_BLw=", v)) {
-            if (str2bool(v) || std::strncmp(a, "topo_file=", 10) == 0) {
-                std::cerr << GA_CLI_NAME << ": " << a << " is not supported by this engine\n";
-                return 2;
-            }
-        } else {
+        else if (key(a, "topo_file=", v)) topo_file = v;
+        else if (key(a, "topo_use_BLw=", v)) p.topo_use_blw = str2bool(v) ? 1 : 0;
+        else {
             std::cerr << GA_CLI_NAME << ": unrecognized parameter entry: " << a << '\n';
             return 2;
         }
@@ -1338,15 +1333,48 @@ int run_eig_direct(int argc, char **argv)
         return 2;
     }
 
+    if (!topo_file) p.topo_use_blw = 0;   // the vertical wind needs terrain (atmo::w)
     ga_ctx *ctx = nullptr;
     if (ga_ctx_create(&ctx, 0) != GA_OK) return die(nullptr, "no usable GPU");
     int aid = 0;
-    if (ga_atmo_load_file(ctx, argv[2], prof_format, reverse_winds ? 1 : 0, &aid) != GA_OK)
-        return die(ctx, "cannot load the atmosphere");
-    const double z0 = std::isnan(z_grnd) ? profile_bottom(argv[2]) : z_grnd;
-    p.z_grnd = z0;
+    if (topo_file) {
+        // the stratified profile over terrain (main.infraga.sph.cpp:1030-1040,
+        // main.infraga.3d.cpp:992-995): source at the terrain's centre,
+        // receiver 2.5 deg east (3D: 250 km west) of it unless given
+        if ((sph ? ga_atmo_load_file_topo : ga_atmo_load_file_topo_xy)(ctx, argv[2], topo_file, prof_format,
+                                                                       reverse_winds ? 1 : 0, &aid) != GA_OK)
+            return die(ctx, "cannot load the atmosphere and terrain");
+        double info[8];
+        if (ga_rngdep_info(ctx, aid, info) != GA_OK) return die(ctx, "region");
+        const double deg = sph ? Pi / 180.0 : 1.0;
+        const double clat = (info[1] + info[0]) / 2.0 / deg, clon = (info[3] + info[2]) / 2.0 / deg;
+        if (!set[0]) s0 = clat;
+        if (!set[1]) s1 = clon;
+        if (!set[2]) r0 = sph ? clat + 0.0 : clat - 250.0;
+        if (!set[3]) r1 = sph ? clon + 2.5 : clon;
+        double pt[3] = {s0 * deg, s1 * deg, 0.0}, aout[40], tout[10];
+        if (ga_rngdep_eval(ctx, aid, pt, 1, aout, tout) != GA_OK) return die(ctx, "terrain");
+        z_src = std::max(z_src, tout[0]);
+        ga_prop_params q;
+        ga_prop_params_default(&q, sph ? GA_GEOM_SPH_RNGDEP : GA_GEOM_3D_RNGDEP);
+        q.abs_coeff = p.abs_coeff;
+        q.alt_max = p.alt_max;
+        q.rng_max = (sph || rng_set) ? p.rng_max : 1000.0;
+        q.time_max = p.time_max;
+        q.ds_min = p.ds_min;
+        q.ds_max = p.ds_max;
+        q.s_max = p.s_max;
+        q.topo_use_blw = p.topo_use_blw;
+        for (int k = 0; k < 4; ++k) q.bounds[k] = bounds_set[k] ? p.bounds[k] : NAN;
+        p = q;
+    } else {
+        if (ga_atmo_load_file(ctx, argv[2], prof_format, reverse_winds ? 1 : 0, &aid) != GA_OK)
+            return die(ctx, "cannot load the atmosphere");
+        const double z0 = std::isnan(z_grnd) ? profile_bottom(argv[2]) : z_grnd;
+        p.z_grnd = z0;
+        z_src = std::max(z_src, z0);
+    }
     p.freq = freq;
-    z_src = std::max(z_src, z0);
     p.src[0] = s0;
     p.src[1] = s1;
     p.src[2] = z_src;

@@ -29,7 +29,8 @@ EXPORTED = ["ga_prop_params_default", "ga_eig_params_default", "ga_ctx_create", 
             "ga_topo_load", "ga_rngdep_eval",
             "ga_rngdep_release", "ga_rngdep_load_files", "ga_rngdep_info", "ga_fan_run_paths",
             "ga_eig_search_paths", "ga_eig_direct", "ga_topo_load_2d", "ga_topo_eval_2d",
-            "ga_fan_run_device_noalloc", "ga_fan_autotune", "ga_atmo_load_strat_topo", "ga_atmo_load_file_topo"]
+            "ga_fan_run_device_noalloc", "ga_fan_autotune", "ga_atmo_load_strat_topo", "ga_atmo_load_file_topo",
+            "ga_atmo_load_strat_topo_xy", "ga_atmo_load_file_topo_xy"]
 
 GA_PATH_RAYS, GA_PATH_CAUSTICS = 1, 2
 GA_PT_RAY, GA_PT_CAUSTIC = 0, 1
@@ -124,6 +125,8 @@ def load_library(path: str | None = None):
                                           _dp, C.POINTER(C.c_int)]
     L.ga_atmo_load_file_topo.argtypes = [C.c_void_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_int,
                                          C.POINTER(C.c_int)]
+    L.ga_atmo_load_strat_topo_xy.argtypes = L.ga_atmo_load_strat_topo.argtypes
+    L.ga_atmo_load_file_topo_xy.argtypes = L.ga_atmo_load_file_topo.argtypes
     L.ga_atmo_release.argtypes = [C.c_void_p, C.c_int]
     L.ga_fan_run.argtypes = [C.c_void_p, C.c_int, C.POINTER(PropParams), _dp, C.c_int, C.POINTER(Arrival),
                              C.c_int, C.POINTER(C.c_int), C.POINTER(RunStats)]
@@ -230,20 +233,21 @@ class Engine:
     def release_atmo(self, aid):
         self._check(self.L.ga_atmo_release(self.h, aid))
 
-    def load_strat_topo(self, z, c, u, v, rho, lat_deg, lon_deg, zg) -> int:
+    def load_strat_topo(self, z, c, u, v, rho, lat_deg, lon_deg, zg, xy=False) -> int:
         """A stratified profile (engine units) over terrain zg[nlat][nlon] [km]:
-        a range-dependent id to run with GA_GEOM_SPH_RNGDEP (ga_atmo_load_strat_topo)."""
+        a range-dependent id to run with GA_GEOM_SPH_RNGDEP (ga_atmo_load_strat_topo);
+        xy: terrain nodes x, y in km, for GA_GEOM_3D_RNGDEP (ga_atmo_load_strat_topo_xy)."""
         a = [np.ascontiguousarray(q, dtype=np.float64) for q in (z, c, u, v, rho, lat_deg, lon_deg, zg)]
         rid = C.c_int(0)
-        self._check(self.L.ga_atmo_load_strat_topo(self.h, *[q.ctypes.data_as(_dp) for q in a[:5]], len(a[0]),
-                                                   a[5].ctypes.data_as(_dp), len(a[5]), a[6].ctypes.data_as(_dp),
-                                                   len(a[6]), a[7].ctypes.data_as(_dp), C.byref(rid)))
+        fn = self.L.ga_atmo_load_strat_topo_xy if xy else self.L.ga_atmo_load_strat_topo
+        self._check(fn(self.h, *[q.ctypes.data_as(_dp) for q in a[:5]], len(a[0]), a[5].ctypes.data_as(_dp),
+                       len(a[5]), a[6].ctypes.data_as(_dp), len(a[6]), a[7].ctypes.data_as(_dp), C.byref(rid)))
         return rid.value
 
-    def load_file_topo(self, path, topo_path, fmt="zTuvdp", reverse_winds=False) -> int:
+    def load_file_topo(self, path, topo_path, fmt="zTuvdp", reverse_winds=False, xy=False) -> int:
         rid = C.c_int(0)
-        self._check(self.L.ga_atmo_load_file_topo(self.h, path.encode(), topo_path.encode(), fmt.encode(),
-                                                  int(reverse_winds), C.byref(rid)))
+        fn = self.L.ga_atmo_load_file_topo_xy if xy else self.L.ga_atmo_load_file_topo
+        self._check(fn(self.h, path.encode(), topo_path.encode(), fmt.encode(), int(reverse_winds), C.byref(rid)))
         return rid.value
 
     def fan(self, aid, params: PropParams, launches):

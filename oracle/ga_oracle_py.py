@@ -388,7 +388,7 @@ class Ref:
             L.ref_bearing.argtypes = [C.c_double] * 4
             L.ref_bearing.restype = C.c_double
             L.ref_r0.restype = C.c_double
-            L.ref_limits.argtypes = [_dp]
+        L.ref_limits.argtypes = [_dp]
         L.ref_load_topo.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p]
         L.ref_load_topo.restype = C.c_int
         L.ref_use_blw.argtypes = [C.c_int]
@@ -447,6 +447,7 @@ def _rd_bind(L):
     L.go_rd_h_config.argtypes = [C.c_void_p, C.c_int, C.c_double]
     L.go_rd_h_limits.argtypes = [C.c_void_p, _dp]
     L.go_rd_h_set_strat.argtypes = [C.c_void_p, C.c_int]
+    L.go_rd_h_set_upper.argtypes = [C.c_void_p, C.c_int]
     L.go_rd_params.argtypes = [C.c_void_p, C.c_int, C.POINTER(Params)]
     L.go_rd_centre.argtypes = [C.c_void_p, _dp, _dp]
     for name in ("go_rd_c", "go_rd_rho", "go_rd_u", "go_rd_v", "go_rd_w", "go_rd_topo_z", "go_rd_bl_sc"):
@@ -569,6 +570,10 @@ class RdMedium:
         """The stratified binaries' semantics over identical columns
         (go_rd_h_set_strat)."""
         self.o.lib.go_rd_h_set_strat(self.h, int(on))
+
+    def set_upper(self, on=True):
+        """Only the stratified 3D set's upper-triangle contraction (go_rd_h_set_upper)."""
+        self.o.lib.go_rd_h_set_upper(self.h, int(on))
 
     @classmethod
     def strat_over_terrain(cls, oracle: "Oracle", z, c, u, v, rho, tlat, tlon, zg, n=4):

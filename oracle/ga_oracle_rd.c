@@ -419,6 +419,17 @@ void go_rd_calc_uvw(go_rd *a, double r, double lat, double lon, double *u, doubl
                         dddX[k][n][m] *= -X / sq(bp);
                         dddX[k][m][n] = dddX[k][n][m];
                     }
+                } else if (a->strat && a->cart) {
+                    /* atmo_state.3d.strat.cpp:600-625: the reciprocal form,
+                     * filled for n <= m and mirrored */
+                    const int i = n < m ? n : m, j = n < m ? m : n;
+                    ddX[n][m] = X / bp * (ddzg[i - 1][j - 1] + 1.0 / bp * dzg[i - 1] * dzg[j - 1]);
+                    for (int k = 0; k < 2; k++) {
+                        dddX[k][n][m] = dddzg[k][i - 1][j - 1];
+                        dddX[k][n][m] += (ddzg[i - 1][j - 1] * dzg[k] + ddzg[k][j - 1] * dzg[i - 1] + ddzg[k][i - 1] * dzg[j - 1]) / bp;
+                        dddX[k][n][m] += dzg[k] * dzg[i - 1] * dzg[j - 1] / sq(bp);
+                        dddX[k][n][m] *= X / bp;
+                    }
                 } else {
                     ddX[n][m] = ddzg[n - 1][m - 1] + dzg[n - 1] * dzg[m - 1] / bp;
                     ddX[n][m] *= X / bp;
@@ -1277,8 +1288,11 @@ void go_rd_h_set_strat(void *h, int strat)
 {
     go_rd *a = (go_rd *)h;
     a->strat = strat != 0;
+    a->upper = a->strat && a->cart;
     set_bndlyr(a);
 }
+/* The contraction alone, for checks of its effect on one medium. */
+void go_rd_h_set_upper(void *h, int on) { ((go_rd *)h)->upper = on != 0; }
 void go_rd_h_limits(void *h, double out[8])
 {
     go_rd *a = (go_rd *)h;
@@ -1352,7 +1366,9 @@ static void refs3(go_rd *a, const go_params *p, const double *y, r3_refs *R)
     for (int n = 0; n < 3; n++) {
         R->dc[3] += Xt[n] * R->dc[n]; R->du[3] += Xt[n] * R->du[n]; R->dv[3] += Xt[n] * R->dv[n]; R->dw[3] += Xt[n] * R->dw[n];
         R->dc[4] += Xp[n] * R->dc[n]; R->du[4] += Xp[n] * R->du[n]; R->dv[3] += Xp[n] * R->dv[n]; R->dw[4] += Xp[n] * R->dw[n];   /* :288 */
-        for (int m = 0; m < 3; m++) {
+        /* the stratified 3D set over terrain contracts the upper triangle only
+         * (geoac.eqset.3d.strat.cpp:305-310, m = n..2) */
+        for (int m = a->upper ? n : 0; m < 3; m++) {
             R->ddc[n][0] += Xt[m] * dd_c[n][m]; R->ddc[n][1] += Xp[m] * dd_c[n][m];
             R->ddu[n][0] += Xt[m] * dd_u[n][m]; R->ddu[n][1] += Xp[m] * dd_u[n][m];
             R->ddv[n][0] += Xt[m] * dd_v[n][m]; R->ddv[n][1] += Xp[m] * dd_v[n][m];
@@ -1589,8 +1605,9 @@ static void rd3_pt(go_path_pt *pts, long long cap, long long *n, go_rd *a, int l
         go_path_pt *P = pts + *n;
         P->launch_idx = li; P->leg = leg; P->step = m; P->kind = kind;
         P->pos0 = y[0]; P->pos1 = y[1];
-        /* raypath rows clip z at the terrain (:456), caustic rows do not (:469) */
-        P->z = kind == 0 ? dmax(y[2], go_rd_topo_z(a, y[0], y[1])) : y[2];
+        /* raypath rows clip z at the terrain (:456), caustic rows do not (:469);
+         * the stratified driver clips both (main.infraga.3d.cpp:457, 471) */
+        P->z = (kind == 0 || a->strat) ? dmax(y[2], go_rd_topo_z(a, y[0], y[1])) : y[2];
         P->tl = tl; P->absorption = -2.0 * att; P->time = tt;
     }
     (*n)++;
@@ -1627,7 +1644,9 @@ int go_rd3_fan(go_rd *a, const go_params *p, const go_prop *pr, const double *la
         s.phi = (90.0 - az) * (RD_PI / 180.0);
         s.theta = incl * (RD_PI / 180.0);
         go_rd3_set_initial(a, p, &s, rows);
-        double tt = 0.0, att = 0.0, zmax = 0.0;
+        /* the stratified driver starts the turning height at the profile
+         * bottom (main.infraga.3d.cpp:426 with set_limits' z0) */
+        double tt = 0.0, att = 0.0, zmax = a->strat ? a->z0 : 0.0;
         int brk = 0, k;
         for (int bnc = 0; bnc <= pr->bounces; bnc++) {
             k = rd3_prop_rk4(a, p, &s, rows, length, &brk);

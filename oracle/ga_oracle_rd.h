@@ -33,6 +33,9 @@ typedef struct {
     int cart;                 /* 3D Cartesian grid: node axes in km, r0 = 0 */
     int strat;                /* a stratified profile over terrain (infraga-sph /
                                  infraga-3d with topo_file): identical columns */
+    int upper;                /* the stratified 3D set's upper-triangle contraction of the
+                                 second-derivative tensors (geoac.eqset.3d.strat.cpp:305-310);
+                                 set with strat on a Cartesian grid */
     double r0;                /* globe radius, 6370 km */
     double z0, z_max, z_bndlyr;
     double bl_param;          /* topo::bndlyr_param, 0.1 km */

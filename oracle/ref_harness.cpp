@@ -111,6 +111,14 @@ double ref_z_bndlyr() { return topo::z_bndlyr; }
 double ref_z0() { return topo::z0; }
 double ref_alt_max() { return geoac::alt_max; }
 
+#ifdef REF_3D
+void ref_limits(double *out)
+{
+    out[0] = geoac::x_min; out[1] = geoac::x_max; out[2] = geoac::y_min; out[3] = geoac::y_max;
+    out[4] = geoac::alt_max; out[5] = topo::z0; out[6] = topo::z_max; out[7] = topo::z_bndlyr;
+}
+#endif
+
 #ifdef REF_SPH
 void ref_limits(double *out)
 {

@@ -30,7 +30,7 @@ def test_usage_without_arguments():
 @pytest.mark.parametrize("args", [
     ("-prop", "x.met", "no_such_key=1"),
     ("-prop", "x.met", "write_topo=true"),
-    ("-eig_search", "x.met", "topo_file=t.dat"),
+    ("-eig_search", "x.met", "no_such_key=1"),
     ("-back_proj", "x.met"),
 ])
 def test_rejected_arguments(args):
