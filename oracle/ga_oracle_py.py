@@ -943,3 +943,95 @@ class Rd3Medium(RdMedium):
         a, b = np.zeros(18), np.zeros(18)
         k = self.o.lib.go_rd3_trace_leg(self.h, C.byref(params), _ptr(s), float(th), float(ph), _ptr(a), _ptr(b))
         return None if k < 0 else (a, b)
+
+
+# ---- weakly non-linear waveforms (ga_wvfrm.c) --------------------------------
+class WvParams(C.Structure):
+    """go_wv_params: length, geoac::ds_wvfrm and the wvfrm:: filter settings
+    (reference defaults: 2^13 points, ds 0.5, filters (0.5, 4, 0.5) and (0.75, 2, 1))."""
+    _fields_ = [("len", C.c_int), ("ds", C.c_double), ("filt1_g", C.c_double), ("filt1_n1", C.c_double),
+                ("filt1_n2", C.c_double), ("filt2_g", C.c_double), ("filt2_n1", C.c_double),
+                ("filt2_n2", C.c_double)]
+
+    @classmethod
+    def default(cls, **kw):
+        w = cls(8192, 0.5, 0.5, 4.0, 0.5, 0.75, 2.0, 1.0)
+        for k, v in kw.items():
+            setattr(w, k, v)
+        return w
+
+
+def _wv_bind(L):
+    if getattr(L, "_wv_bound", False):
+        return
+    L.go_fft.argtypes = [_dp, _dp, C.c_int, C.c_int]
+    L.go_fft.restype = C.c_int
+    L.go_rfft.argtypes = [_dp, C.c_int, _dp, _dp]
+    L.go_irfft.argtypes = [_dp, _dp, C.c_int, _dp]
+    L.go_wvfrm_build.argtypes = [C.c_int, C.c_double, C.c_double, C.POINTER(C.c_double), C.c_int, C.c_int, C.c_int,
+                                 _dp, _dp]
+    L.go_h_wnl_run.restype = C.c_longlong
+    L.go_h_wnl_run.argtypes = [C.c_void_p, C.POINTER(Params), C.POINTER(WvParams), _dp, C.c_double, C.c_double,
+                               C.c_int, C.c_double, C.c_double, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                               C.POINTER(C.c_int)]
+    L._wv_bound = True
+
+
+def wv_fft(oracle, x, inverse=False):
+    """go_fft of a complex series (length a power of two)."""
+    _wv_bind(oracle.lib)
+    re = np.ascontiguousarray(np.real(x), dtype=np.float64).copy()
+    im = np.ascontiguousarray(np.imag(x), dtype=np.float64).copy()
+    if oracle.lib.go_fft(_ptr(re), _ptr(im), re.size, int(inverse)) != 0:
+        raise ValueError("length must be a power of two")
+    return re + 1j * im
+
+
+def wv_rfft(oracle, x):
+    _wv_bind(oracle.lib)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    re, im = np.zeros(x.size // 2 + 1), np.zeros(x.size // 2 + 1)
+    oracle.lib.go_rfft(_ptr(x), x.size, _ptr(re), _ptr(im))
+    return re + 1j * im
+
+
+def wv_irfft(oracle, X, n):
+    _wv_bind(oracle.lib)
+    re = np.ascontiguousarray(np.real(X), dtype=np.float64)
+    im = np.ascontiguousarray(np.imag(X), dtype=np.float64)
+    out = np.zeros(n)
+    oracle.lib.go_irfft(_ptr(re), _ptr(im), n, _ptr(out))
+    return out
+
+
+WV_OPTIONS = {"impulse": 0, "Uwave": 1, "Nwave": 2}
+
+
+def wv_build(oracle, option="impulse", p0=10.0, t0=0.1, alpha=4.0, length=8192, n_rise=10, n_period=40):
+    """A built-in source waveform (waveforms.cpp build_wvfrm): (t, p, alpha as raised)."""
+    _wv_bind(oracle.lib)
+    t, p = np.zeros(length), np.zeros(length)
+    a = C.c_double(alpha)
+    oracle.lib.go_wvfrm_build(WV_OPTIONS.get(option, 3), p0, t0, C.byref(a), length, n_rise, n_period, _ptr(t),
+                              _ptr(p))
+    return t, p, a.value
+
+
+def wv_run(medium, params, wv, src, incl, az, bounces, t_src, p_src, freq=0.1, wvfrm_ref=1.0):
+    """run_wnl_wvfrm through a stratified Medium: dict with t_init, p_init,
+    t_out, p_out, arrival (pos0, pos1, time, cel, turn_ht, incl_arr, back_az,
+    tl, absorption), ok (ray returned to the ground), steps."""
+    L = medium.o.lib
+    _wv_bind(L)
+    n = wv.len
+    ts = np.ascontiguousarray(t_src, dtype=np.float64)
+    ps = np.ascontiguousarray(p_src, dtype=np.float64)
+    s = np.ascontiguousarray(src, dtype=np.float64)
+    out = {k: np.zeros(n) for k in ("t_init", "p_init", "t_out", "p_out")}
+    arr = np.zeros(9)
+    ok = C.c_int(0)
+    steps = L.go_h_wnl_run(medium.h, C.byref(params), C.byref(wv), _ptr(s), float(incl), float(az), int(bounces),
+                           float(freq), float(wvfrm_ref), _ptr(ts), _ptr(ps), _ptr(out["t_init"]), _ptr(out["p_init"]),
+                           _ptr(out["t_out"]), _ptr(out["p_out"]), _ptr(arr), C.byref(ok))
+    out.update(arrival=arr, ok=bool(ok.value), steps=int(steps))
+    return out
