@@ -38,6 +38,11 @@
  *                          <id>.eigenray-N.dat rows
  *                          src/geoac/geoac.eigenray.3d.cpp:223-276,
  *                          src/geoac/geoac.eigenray.sph.cpp:228-271
+ *   ga_wnl_wvfrm        <- run_wnl_wvfrm (src/main.infraga.sph.cpp:1142-1479)
+ *                          and geoac::wnl_wvfrm
+ *                          (src/geoac/geoac.eqset.sph.strat.cpp:853-1148,
+ *                          src/geoac/geoac.eqset.3d.strat.cpp), one
+ *                          waveform per launch
  */
 #ifndef INFRAGA_GPU_H_
 #define INFRAGA_GPU_H_
@@ -326,6 +331,51 @@ int ga_eig_search_paths(ga_ctx *ctx, int atmo_id, const ga_prop_params *p, const
                         const double *rcvr_xy, int n_rcvr,
                         ga_eigenray *out, int out_cap, int *n_out, double *rcvr_latency_ms,
                         ga_path_pt *pts, int64_t pts_cap, int64_t *n_pts, ga_run_stats *stats);
+
+/* ---- Weakly non-linear waveforms (-wnl_wvfrm) ---------------------------- */
+
+/* Evolution settings: wvfrm_len, wvfrm_ds, wvfrm_ref and the two filters
+ * (defaults of src/util/waveforms.cpp:20-35 and run_wnl_wvfrm: 8192, 0.5,
+ * 1.0, (0.5, 4, 0.5), (0.75, 2, 1)). */
+typedef struct {
+    int32_t len;           /* samples of the source waveform: a power of two, 64 .. 8192 */
+    int32_t reserved;
+    double  ds;            /* wvfrm_ds (geoac::ds_wvfrm) */
+    double  ref;           /* wvfrm_ref: ray length [km] where the source waveform is set */
+    double  filt1_g, filt1_n1, filt1_n2;
+    double  filt2_g, filt2_n1, filt2_n2;
+} ga_wvfrm_params;
+
+typedef struct {
+    int64_t ray_steps;     /* RK4 iterations of the ray traces */
+    int64_t wv_steps;      /* evolution steps summed over the waveforms */
+    double  trace_ms;      /* device time of the ray traces */
+    double  evolve_ms;     /* device time of the evolution kernel */
+    double  wall_ms;       /* host wall time of the call */
+    int32_t n_waveforms;
+    int32_t reserved;
+} ga_wvfrm_stats;
+
+/* -wnl_wvfrm for n launches (incl, az) [deg] from p->src over a stratified
+ * atmosphere (GA_GEOM_SPH or GA_GEOM_3D; p->calc_amp is implied, and, as in
+ * run_wnl_wvfrm, no turning-height or time limit stops a ray).  The source
+ * waveform t_src[len] [s], p_src[len] [Pa] (a built-in one or a loaded
+ * file) is set at ray length w->ref and evolved along every leg to the last
+ * ground return.  Per launch i, rows i*len .. i*len+len-1 of t_init / p_init
+ * receive <id>.wvfrm_init.dat (travel time to the reference point + t, the
+ * source pressure) and of t_out / p_out <id>.wvfrm_out.dat (total travel
+ * time + t, the evolved pressure scaled to the arrival).  arrivals[i] is the
+ * arrival summary of the last ground return; status[i] is 1 when the ray
+ * returned to the ground after every bounce, 0 when it left the region
+ * first (the waveform is evolved over the legs traced; the arrival holds
+ * only launch angles and travel time) and -1 when the ray never reached the
+ * reference point (its rows are NaN).  evolve_steps (may be NULL) receives
+ * the evolution steps per launch.  Lengths that are not powers of two are
+ * refused (GA_E_ARG). */
+int ga_wnl_wvfrm(ga_ctx *ctx, int atmo_id, const ga_prop_params *p, const ga_wvfrm_params *w,
+                 const double *launches, int n, const double *t_src, const double *p_src,
+                 double *t_init, double *p_init, double *t_out, double *p_out,
+                 ga_arrival *arrivals, int32_t *status, int64_t *evolve_steps, ga_wvfrm_stats *stats);
 
 /* Terrain profile z(r) [km] of the 2D geometry (infraga-2d topo_file=, read
  * by set_region, src/atmo/atmo_io.2d.cpp:138-156, into topo::spline): n >= 3

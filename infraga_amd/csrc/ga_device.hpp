@@ -534,6 +534,14 @@ struct SbGas {
     double cms, icms, T_z, iTz, P_z, mu, imu, nu, nu2p1, snu;
 };
 
+// The frequency-dependent members nu, nu2p1, snu of g.
+GA_HD void sb_gas_freq(SbGas &g, double freq)
+{
+    g.nu = (8.0 * PI * freq * g.mu) * ga_rcp(3.0 * g.P_z);
+    g.nu2p1 = 1.0 + g.nu * g.nu;
+    g.snu = sqrt(g.nu2p1);
+}
+
 GA_HD SbGas sb_gas(double c, double rho, double freq)
 {
     const double gam = 1.4, Rg = 287.058;
@@ -546,9 +554,7 @@ GA_HD SbGas sb_gas(double c, double rho, double freq)
     g.P_z = rho * 1.0e3 * (g.cms * g.cms) * (1.0 / gam);
     g.mu = mu_o * sqrt(g.T_z * (1.0 / T_o)) * ((1.0 + S / T_o) * ga_rcp(1.0 + S * g.iTz));
     g.imu = ga_rcp(g.mu);
-    g.nu = (8.0 * PI * freq * g.mu) * ga_rcp(3.0 * g.P_z);
-    g.nu2p1 = 1.0 + g.nu * g.nu;
-    g.snu = sqrt(g.nu2p1);
+    sb_gas_freq(g, freq);
     return g;
 }
 
@@ -558,9 +564,16 @@ GA_HD double sb_cl(const SbGas &g, double freq)
     return (2.0 * PI * freq * g.icms) * sqrt(0.5 * (g.snu - 1.0) * ga_rcp(g.nu2p1));
 }
 
-// Q and R at height z.
+// The factors of Q and R that depend on the height and the gas state but
+// not on the frequency: the composition fits X_ON, the rotational
+// collision number (through nn), the vibrational relaxation strengths
+// A_max and inverse relaxation frequencies 1 / f_vib of O2, N2, CO2 and O3.
+struct SbPoint {
+    double X_ON, nn, A_max[4], ifv[4];
+};
+
 template <bool SPH>
-GA_HD void sb_smooth(double z, const SbGas &g, double freq, const double *sbc, double &Q, double &R)
+GA_HD SbPoint sb_point(double z, const SbGas &g, const double *sbc)
 {
     const double T_o = 293.15, P_o = 101325.0, mu_o = 18.192e-6;
     const double z2 = z * z, z3 = z2 * z, z4 = z3 * z, z5 = z4 * z;
@@ -581,25 +594,11 @@ GA_HD void sb_smooth(double z, const SbGas &g, double freq, const double *sbc, d
     // the reference's "z > 100" branch is unreachable behind "z > 30"
     X6 = ga_exp10((z > 30.0) ? (-4.2563 + (7.6245E-2 * z) - (2.1824E-3 * z2) - (2.3010E-6 * z3) + (2.4265E-7 * z4) - (1.2500E-09 * z5))
                           : (-1.7491 + (4.4986E-2 * z) - (6.8549E-2 * z2) + (5.4639E-3 * z3) - (1.5539E-4 * z4) + (1.5063E-06 * z5)));
-    const double X_ON = (X0 + X1) * (1.0 / 0.9903);
+    SbPoint P;
+    P.X_ON = (X0 + X1) * (1.0 / 0.9903);
     const double Tm13 = ga_rcbrt(g.T_z);
     const double Z_rot = ga_rcp((X1 * ga_rcp(63.3 * ga_exp(-16.7 * Tm13))) + (X0 * ga_rcp(54.1 * ga_exp(-17.3 * Tm13))));
-    const double sigma = 5.0 / sqrt(21.0);
-    const double nn = (4.0 / 5.0) * sqrt(3.0 / 7.0) * Z_rot;
-    const double chi = 3.0 * nn * g.nu / 4.0;
-    const double cchi = 2.36 * chi;
-    const double beta_0 = 2.0 * PI * freq * g.icms;
-    Q = sqrt((1.0 + cchi * cchi) * ga_rcp(1.0 + (sigma * cchi) * (sigma * cchi)));
-    double a_rot;
-    if (SPH) {
-        a_rot = beta_0 * X_ON * ((sigma * sigma - 1.0) * (1.0 / (2 * sigma)) * chi) *
-                sqrt(0.5 * (g.snu + 1.0) * ga_rcp(g.nu2p1 * (1.0 + cchi * cchi)));
-    } else {
-        const double beta_1 = sqrt(0.5 * (g.snu + 1.0) * ga_rcp(g.nu2p1));
-        const double alpha_2 = beta_0 * (((sigma / 2.0 - 1.0 / (2.0 * sigma)) * chi) *
-                                         ga_rcp(sqrt((1.0 + cchi * cchi) * (1.0 + (sigma * cchi) * (sigma * cchi)))));
-        a_rot = alpha_2 * beta_1 * X_ON;
-    }
+    P.nn = (4.0 / 5.0) * sqrt(3.0 / 7.0) * Z_rot;
     const double Tr = ga_rcbrt(g.T_z * (1.0 / T_o)) - 1.0;
     const double A1 = (X0 + X1) * 24.0 * ga_exp(-9.16 * Tr);
     const double A2 = (X4 + X5) * 2400.0;
@@ -622,7 +621,6 @@ GA_HD void sb_smooth(double z, const SbGas &g, double freq, const double *sbc, d
     const double Th[4] = {2239.1, 3352.0, 915.0, 1037.0};
     const double Cv[4] = {2.5, 2.5, 3.0, 3.0};
     const double Cp[4] = {3.5, 3.5, 4.0, 4.0};
-    double a_vib = 0.0;
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
         const double tt = Th[m] * g.iTz;
@@ -630,12 +628,46 @@ GA_HD void sb_smooth(double z, const SbGas &g, double freq, const double *sbc, d
         const double ome = 1.0 - e;
         const double iome = ga_rcp(ome);
         const double C_R = (tt * tt * e) * (iome * iome);
-        const double A_max = (Xm[m] * (PI / 2) * C_R) * ga_rcp(Cp[m] * (Cv[m] + C_R));
-        const double ifv = ga_rcp(fv[m]);
-        const double fr = freq * ifv;
-        a_vib += (A_max * g.icms) * ((2 * (freq * freq) * ifv) * ga_rcp(1.0 + fr * fr));
+        P.A_max[m] = (Xm[m] * (PI / 2) * C_R) * ga_rcp(Cp[m] * (Cv[m] + C_R));
+        P.ifv[m] = ga_rcp(fv[m]);
+    }
+    return P;
+}
+
+// Q and R at frequency freq from the point factors P and the gas state g
+// (whose nu, nu2p1, snu belong to freq).
+template <bool SPH>
+GA_HD void sb_qr(const SbPoint &P, const SbGas &g, double freq, double &Q, double &R)
+{
+    const double sigma = 5.0 / sqrt(21.0);
+    const double chi = 3.0 * P.nn * g.nu / 4.0;
+    const double cchi = 2.36 * chi;
+    const double beta_0 = 2.0 * PI * freq * g.icms;
+    Q = sqrt((1.0 + cchi * cchi) * ga_rcp(1.0 + (sigma * cchi) * (sigma * cchi)));
+    double a_rot;
+    if (SPH) {
+        a_rot = beta_0 * P.X_ON * ((sigma * sigma - 1.0) * (1.0 / (2 * sigma)) * chi) *
+                sqrt(0.5 * (g.snu + 1.0) * ga_rcp(g.nu2p1 * (1.0 + cchi * cchi)));
+    } else {
+        const double beta_1 = sqrt(0.5 * (g.snu + 1.0) * ga_rcp(g.nu2p1));
+        const double alpha_2 = beta_0 * (((sigma / 2.0 - 1.0 / (2.0 * sigma)) * chi) *
+                                         ga_rcp(sqrt((1.0 + cchi * cchi) * (1.0 + (sigma * cchi) * (sigma * cchi)))));
+        a_rot = alpha_2 * beta_1 * P.X_ON;
+    }
+    double a_vib = 0.0;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        const double fr = freq * P.ifv[m];
+        a_vib += (P.A_max[m] * g.icms) * ((2 * (freq * freq) * P.ifv[m]) * ga_rcp(1.0 + fr * fr));
     }
     R = a_rot + a_vib;
+}
+
+// Q and R at height z.
+template <bool SPH>
+GA_HD void sb_smooth(double z, const SbGas &g, double freq, const double *sbc, double &Q, double &R)
+{
+    sb_qr<SPH>(sb_point<SPH>(z, g, sbc), g, freq, Q, R);
 }
 
 template <bool SPH>

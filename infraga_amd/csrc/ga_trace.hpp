@@ -44,6 +44,17 @@ constexpr int RAW = 25;
 // caller can size a second pass exactly.
 constexpr int PCHUNK = 32;
 
+// Waveform rows: an internal path mode of the -wnl_wvfrm trace
+// (run_wnl_wvfrm, main.infraga.sph.cpp:1336-1401, keeps the whole solution
+// array of every leg).  Every row m = 0..K of every leg is written as two
+// records: PT_WV_STATE holds the state y[0..5] in (pos0, pos1, z, tl,
+// absorption, time); PT_WV_MEDIUM holds the Jacobian determinant (pos0),
+// c (pos1), rho (z), the group speed |c nu/|nu| + wind| (tl) and the travel
+// time of the segment from row m-1 into row m (absorption; 0 for row 0).
+constexpr int PATH_WVROWS = 4;
+constexpr int PT_WV_STATE = 2;
+constexpr int PT_WV_MEDIUM = 3;
+
 struct PathOut {
     ga_path_pt *pts;           // [max_chunks][PCHUNK]
     int32_t *chunk_item;       // item of each chunk
@@ -295,6 +306,36 @@ GA_HD ga_path_pt path_row(const Medium &M, const double *y, int item, int leg, i
     return P;
 }
 
+// The two waveform-row records of row y (PATH_WVROWS); dt: travel time of
+// the segment into the row.
+template <int G>
+GA_HD void wv_rows_put(const PathOut &P, PathLane &pl, const Medium &M, const double *y, int item, int leg, int m,
+                       double dt, int &cur)
+{
+    double c, u, v, rho;
+    point_medium<G>(M, y, cur, c, u, v, rho);
+    const double nm = sqrt(y[3] * y[3] + y[4] * y[4] + y[5] * y[5]);
+    double g0, g1, g2;
+    if (G == GEOM_SPH) { g0 = c * y[3] / nm; g1 = c * y[4] / nm + v; g2 = c * y[5] / nm + u; }
+    else               { g0 = c * y[3] / nm + u; g1 = c * y[4] / nm + v; g2 = c * y[5] / nm; }
+    ga_path_pt a, b;
+    a.launch_idx = b.launch_idx = item;
+    a.leg = b.leg = leg;
+    a.step = b.step = m;
+    a.kind = PT_WV_STATE;
+    a.pos0 = y[0]; a.pos1 = y[1]; a.z = y[2];
+    a.tl = y[3]; a.absorption = y[4]; a.time = y[5];
+    b.kind = PT_WV_MEDIUM;
+    b.pos0 = jacobian<G>(y, c, u, v);
+    b.pos1 = c;
+    b.z = rho;
+    b.tl = sqrt(g0 * g0 + g1 * g1 + g2 * g2);
+    b.absorption = dt;
+    b.time = 0.0;
+    path_put(P, pl, item, a);
+    path_put(P, pl, item, b);
+}
+
 // Arrival record (main.infraga.sph.cpp:523-544, main.infraga.3d.cpp:511-530).
 template <int G, int NEQ>
 GA_HD ga_arrival arrival_row(const FanArgs &A, const Medium &M, const double *y, int ray, int bnc,
@@ -400,8 +441,10 @@ GA_HD void reflect(const Medium &M, const double *yk2, const double *yk1, double
 // and absorption (the segment into the sub-ground row k is not), rows m == 1
 // or m % 15 == 0 are written to A.path (GA_PATH_RAYS), and a sign change of
 // the Jacobian determinant between rows m-1 and m is a caustic
-// (GA_PATH_CAUSTICS, 18 equations).  Otherwise each leg is summed whole
-// (rows 0..k) and added to the totals at its end (write_rays=false).
+// (GA_PATH_CAUSTICS, 18 equations); PATH_WVROWS (18 equations) writes
+// every row 0..k of every leg as waveform-row records.  Otherwise each leg
+// is summed whole (rows 0..k) and added to the totals at its end
+// (write_rays=false).
 template <int G, int NEQ, bool PATHS = false, class NextRay>
 GA_HD void lane_loop(const FanArgs &A, const Medium &M, int lane, NextRay next_ray)
 {
@@ -429,6 +472,9 @@ GA_HD void lane_loop(const FanArgs &A, const Medium &M, int lane, NextRay next_r
             s = 0.0; tt = 0.0; att = 0.0; ttl = 0.0; attl = 0.0; rmax = A.rmax_init;
             k = 0; bnc = 0; nst = 0; narr = 0;
             fetch = false;
+            if constexpr (PATHS && NEQ == 18) {
+                if (A.path.flags & PATH_WVROWS) wv_rows_put<G>(A.path, pl, M, y, ray, 0, 0, 0.0, cur);
+            }
         }
         double y1[NEQ];
         s += rk4_step<G, NEQ>(M, L, y, y1, cur);
@@ -442,9 +488,10 @@ GA_HD void lane_loop(const FanArgs &A, const Medium &M, int lane, NextRay next_r
         else if (!(y1[0] == y1[0])) { brk = true; end = true; }   // non-finite state: stop the ray
 
         if (PATHS) {
+            const bool wv = NEQ == 18 && (A.path.flags & PATH_WVROWS);
+            double dt = 0.0, da = 0.0;
+            if (!end || wv) seg_book<G>(M, y, y1, A.freq, cur, dt, da);
             if (!end) {
-                double dt, da;
-                seg_book<G>(M, y, y1, A.freq, cur, dt, da);
                 tt += dt;
                 att += da;
                 const bool caus = NEQ == 18 && (A.path.flags & GA_PATH_CAUSTICS);
@@ -464,6 +511,9 @@ GA_HD void lane_loop(const FanArgs &A, const Medium &M, int lane, NextRay next_r
                         d_prev = d;
                     }
                 }
+            }
+            if constexpr (NEQ == 18) {
+                if (wv) wv_rows_put<G>(A.path, pl, M, y1, ray, bnc, k, dt, cur);
             }
         } else {
             double dt, da;
@@ -509,6 +559,9 @@ GA_HD void lane_loop(const FanArgs &A, const Medium &M, int lane, NextRay next_r
                 ++bnc;
                 s = 0.0;
                 k = 0;
+                if constexpr (PATHS && NEQ == 18) {
+                    if (A.path.flags & PATH_WVROWS) wv_rows_put<G>(A.path, pl, M, y, ray, bnc, 0, 0.0, cur);
+                }
             }
         }
         if (done) {

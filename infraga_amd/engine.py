@@ -30,7 +30,7 @@ EXPORTED = ["ga_prop_params_default", "ga_eig_params_default", "ga_ctx_create", 
             "ga_rngdep_release", "ga_rngdep_load_files", "ga_rngdep_info", "ga_fan_run_paths",
             "ga_eig_search_paths", "ga_eig_direct", "ga_topo_load_2d", "ga_topo_eval_2d",
             "ga_fan_run_device_noalloc", "ga_fan_autotune", "ga_atmo_load_strat_topo", "ga_atmo_load_file_topo",
-            "ga_atmo_load_strat_topo_xy", "ga_atmo_load_file_topo_xy"]
+            "ga_atmo_load_strat_topo_xy", "ga_atmo_load_file_topo_xy", "ga_wnl_wvfrm"]
 
 GA_PATH_RAYS, GA_PATH_CAUSTICS = 1, 2
 GA_PT_RAY, GA_PT_CAUSTIC = 0, 1
@@ -87,6 +87,23 @@ class FanTuning(C.Structure):
 class RunStats(C.Structure):
     _fields_ = [("ray_steps", C.c_int64), ("kernel_ms", C.c_double), ("wall_ms", C.c_double),
                 ("n_rays", C.c_int32), ("n_launches", C.c_int32)]
+
+
+class WvfrmParams(C.Structure):
+    """-wnl_wvfrm settings (waveforms.cpp:20-35 and run_wnl_wvfrm defaults)."""
+    _fields_ = [("len", C.c_int32), ("reserved", C.c_int32), ("ds", C.c_double), ("ref", C.c_double),
+                ("filt1_g", C.c_double), ("filt1_n1", C.c_double), ("filt1_n2", C.c_double),
+                ("filt2_g", C.c_double), ("filt2_n1", C.c_double), ("filt2_n2", C.c_double)]
+
+
+def wvfrm_params(length=8192, ds=0.5, ref=1.0, filt1=(0.5, 4.0, 0.5), filt2=(0.75, 2.0, 1.0)) -> WvfrmParams:
+    return WvfrmParams(int(length), 0, float(ds), float(ref), *map(float, filt1), *map(float, filt2))
+
+
+class WvfrmStats(C.Structure):
+    _fields_ = [("ray_steps", C.c_int64), ("wv_steps", C.c_int64), ("trace_ms", C.c_double),
+                ("evolve_ms", C.c_double), ("wall_ms", C.c_double), ("n_waveforms", C.c_int32),
+                ("reserved", C.c_int32)]
 
 
 ARRIVAL_DTYPE = np.dtype([("launch_idx", "i4"), ("n_bounce", "i4"), ("incl", "f8"), ("az", "f8"),
@@ -156,6 +173,9 @@ def load_library(path: str | None = None):
     L.ga_rngdep_load_files.argtypes = [C.c_void_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p,
                                        C.c_int, C.POINTER(C.c_int)]
     L.ga_rngdep_release.argtypes = [C.c_void_p, C.c_int]
+    L.ga_wnl_wvfrm.argtypes = [C.c_void_p, C.c_int, C.POINTER(PropParams), C.POINTER(WvfrmParams), _dp, C.c_int,
+                               _dp, _dp, _dp, _dp, _dp, _dp, C.POINTER(Arrival), C.POINTER(C.c_int32),
+                               C.POINTER(C.c_int64), C.POINTER(WvfrmStats)]
     L.ga_server_create.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_void_p)]
     L.ga_server_submit.argtypes = [C.c_void_p, C.POINTER(Request), C.POINTER(C.c_uint64)]
     L.ga_server_poll.argtypes = [C.c_void_p, C.c_uint64, C.c_int, C.POINTER(Result), C.POINTER(Arrival), C.c_int]
@@ -285,6 +305,31 @@ class Engine:
             break
         self._check(rc)
         return out[: n_out.value].copy(), pts[: n_pts.value].copy(), st
+
+    def wnl_wvfrm(self, aid, params: PropParams, w: WvfrmParams, launches, t_src, p_src):
+        """-wnl_wvfrm for each (incl, az) launch with the source waveform
+        (t_src, p_src); returns a dict of t_init, p_init, t_out, p_out
+        ([n, len] each), arrivals, status, steps and the WvfrmStats."""
+        Lc = np.ascontiguousarray(launches, dtype=np.float64).reshape(-1, 2)
+        n = Lc.shape[0]
+        ln = int(w.len)
+        t = np.ascontiguousarray(t_src, dtype=np.float64)
+        pz = np.ascontiguousarray(p_src, dtype=np.float64)
+        if t.shape != (ln,) or pz.shape != (ln,):
+            raise ValueError(f"source waveform must have {ln} samples")
+        out = {k: np.zeros((n, ln)) for k in ("t_init", "p_init", "t_out", "p_out")}
+        arr = np.zeros(max(n, 1), dtype=ARRIVAL_DTYPE)
+        status = np.zeros(max(n, 1), dtype=np.int32)
+        steps = np.zeros(max(n, 1), dtype=np.int64)
+        st = WvfrmStats()
+        dp = lambda a: a.ctypes.data_as(_dp)
+        self._check(self.L.ga_wnl_wvfrm(self.h, aid, C.byref(params), C.byref(w), dp(Lc), n, dp(t), dp(pz),
+                                        dp(out["t_init"]), dp(out["p_init"]), dp(out["t_out"]), dp(out["p_out"]),
+                                        arr.ctypes.data_as(C.POINTER(Arrival)),
+                                        status.ctypes.data_as(C.POINTER(C.c_int32)),
+                                        steps.ctypes.data_as(C.POINTER(C.c_int64)), C.byref(st)))
+        out.update(arrivals=arr[:n].copy(), status=status[:n].copy(), steps=steps[:n].copy(), stats=st)
+        return out
 
     def fan_device(self, aid, params: PropParams, d_launches: int, n: int, d_out: int, d_counts: int,
                    d_steps: int, stream: int = 0):
