@@ -4,7 +4,7 @@
 // Same command line as the reference drivers (src/main.infraga.sph.cpp,
 // src/main.infraga.3d.cpp, src/main.infraga.sph.rngdep.cpp,
 // src/main.infraga.3d.rngdep.cpp, src/main.infraga.2d.cpp): an option
-// (-prop, -eig_search, -eig_direct; the range-dependent binaries take the
+// (-prop, -eig_search, -eig_direct, -wnl_wvfrm; the range-dependent binaries take the
 // profile prefix and the two node files), the profile path, then
 // key=value parameters with the same names, aliases and defaults.  The
 // ray tracing runs on the MI355X through the C ABI of libinfraga_gpu.so; this
@@ -22,6 +22,7 @@
 #include <fstream>
 #include <iomanip>
 #include <iostream>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -71,6 +72,7 @@ void usage()
               << "  -prop        ray paths for a fan of inclinations and azimuths\n"
               << "  -eig_search  eigenrays between the source and a receiver\n"
               << "  -eig_direct  one eigenray from a launch-angle estimate (incl_est, az_est)\n"
+              << "  -wnl_wvfrm   weakly non-linear waveform along one ray (infraga-sph, infraga-3d)\n"
               << "Parameters use the key=value names of the reference infraga drivers.\n";
 }
 
@@ -1414,6 +1416,302 @@ int run_eig_direct(int argc, char **argv)
     return 0;
 }
 
+// ---- -wnl_wvfrm ------------------------------------------------------------
+// Built-in source waveforms (src/util/waveforms.cpp:37-145): a shaped
+// impulse p0 x^a (1 - x / (1 + a)) e^-x normalised at its peak x_pk =
+// (1 + a) - sqrt(1 + a), x = t / t0, zero before t = 0; the U-wave is its
+// central difference (eps = 0.01 s) scaled by 1.126 times the slope at 0;
+// the N-wave the ramp -p0 (t - t0/2) / t0 on |t - t0/2| <= t0.  Record
+// lengths: t0 len max(x_pk / N_rise, 2 pi' sqrt(1 + a) / N_period) with the
+// reference's pi' = 3.14159 (min for an unrecognised option), the N-wave
+// t0 / N_period len.  alpha is raised to 0.01 (2 for the U-wave).
+double wv_impulse(double t, double t0, double a)
+{
+    if (t < 0.0) return 0.0;
+    const double x = t / t0, x_pk = (a + 1.0) - std::sqrt(a + 1.0);
+    const double f = std::pow(x, a) * (1.0 - x / (1.0 + a)) * std::exp(-x);
+    return f / (std::pow(x_pk, a) * (1.0 - x_pk / (1.0 + a)) * std::exp(-x_pk));
+}
+
+void wv_build(const char *opt, double p0, double t0, double &alpha, int len, int n_rise, int n_period,
+              std::vector<double> &t, std::vector<double> &p)
+{
+    t.assign((size_t)len, 0.0);
+    p.assign((size_t)len, 0.0);
+    const bool imp = std::strncmp(opt, "impulse", 7) == 0;
+    const bool uw = std::strncmp(opt, "Uwave", 5) == 0 || std::strncmp(opt, "u-wave", 6) == 0 ||
+                    std::strncmp(opt, "U-wave", 6) == 0;
+    const bool nw = std::strncmp(opt, "Nwave", 5) == 0 || std::strncmp(opt, "n-wave", 6) == 0 ||
+                    std::strncmp(opt, "N-wave", 6) == 0;
+    double duration;
+    if (nw) {
+        duration = (t0 / n_period) * len;
+    } else {
+        alpha = std::max(alpha, uw ? 2.0 : 0.01);
+        const double a = alpha;
+        const double r = ((1.0 + a) - std::sqrt(1.0 + a)) / n_rise, q = 2.0 * 3.14159 * std::sqrt(1 + a) / n_period;
+        duration = t0 * len * ((imp || uw) ? std::max(r, q) : std::min(r, q));
+    }
+    const double eps = 1.0e-2;
+    for (int n = 0; n < len; ++n) {
+        const double tn = -duration / 2.0 + duration * n / len;
+        t[n] = tn;
+        if (nw) p[n] = std::fabs(tn - t0 / 2.0) <= t0 ? -p0 * ((tn - t0 / 2.0) / t0) : -p0 * 0.0;
+        else if (uw) {
+            p[n] = p0 * (wv_impulse(tn + eps, t0, alpha) - wv_impulse(tn - eps, t0, alpha)) / (2.0 * eps);
+            p[n] /= 1.126 * (wv_impulse(eps, t0, alpha) - wv_impulse(-eps, t0, alpha)) / (2.0 * eps);
+        } else {
+            p[n] = p0 * wv_impulse(tn, t0, alpha);
+        }
+    }
+}
+
+// wvfrm_file= (waveforms.cpp:64-90, fileIO.cpp:18-35): the record length is
+// the count of lines not starting with '#' less one (the reader's final
+// empty read), then the first that many such lines as "t p".
+bool wv_load(const char *path, std::vector<double> &t, std::vector<double> &p)
+{
+    std::ifstream in(path);
+    if (!in.is_open()) return false;
+    std::vector<std::string> lines;
+    std::string line;
+    int count = 0;
+    while (std::getline(in, line)) {
+        if (line.find('#') != 0) { ++count; lines.push_back(line); }
+    }
+    ++count;   // the reference's eof loop reads one more (empty) line
+    const int len = count - 1;
+    t.assign((size_t)len, 0.0);
+    p.assign((size_t)len, 0.0);
+    for (int n = 0; n < len && n < (int)lines.size(); ++n) {
+        std::istringstream ss(lines[(size_t)n]);
+        ss >> t[n] >> p[n];
+    }
+    return len > 0;
+}
+
+// infraga-sph / infraga-3d -wnl_wvfrm (run_wnl_wvfrm, main.infraga.sph.cpp:
+// 1142-1479, main.infraga.3d.cpp:1087-1430): one ray, its waveform set at
+// wvfrm_ref km along the ray and evolved to the last ground return;
+// <id>.wvfrm_init.dat, <id>.wvfrm_out.dat and, with write_ray=true,
+// <id>.raypaths.dat.  Terrain (topo_file=) and intermediate waveform dumps
+// (wvfrm_out_step < 1e4) are refused; the waveform length must be a power
+// of two (the engine's transform).
+int run_wnl_wvfrm(int argc, char **argv)
+{
+    const bool sph = GA_CLI_GEOM == GA_GEOM_SPH;
+    double incl = 15.0, az = 90.0;
+    int bounces = 0;
+    double s0 = sph ? 30.0 : 0.0, s1 = 0.0, z_src = 0.0;
+    bool write_rays = false, reverse_winds = false, custom_id = false;
+    double freq = 0.1, wvfrm_ref = 1.0, wvfrm_out_step = 1.0e10;
+    double wp0 = 10.0, wt0 = 0.1, walpha = 4.0;
+    int n_rise = 10, n_period = 40;
+    long len = 8192;
+    const char *wvfrm_file = "none", *wvfrm_opt = "impulse";
+    const char *prof_format = "zTuvdp", *output_id = nullptr;
+    ga_wvfrm_params w{};
+    w.ds = 0.5;
+    w.filt1_g = 0.5; w.filt1_n1 = 4.0; w.filt1_n2 = 0.5;
+    w.filt2_g = 0.75; w.filt2_n1 = 2.0; w.filt2_n2 = 1.0;
+    ga_prop_params p;
+    ga_prop_params_default(&p, GA_CLI_GEOM);
+    double z_grnd = NAN;
+    for (int i = 3; i < argc; i++) {
+        const char *a = argv[i], *v = nullptr;
+        if (key(a, "inclination=", v)) incl = std::atof(v);
+        else if (key(a, "azimuth=", v)) az = std::atof(v);
+        else if (key(a, "bounces=", v)) bounces = std::max(0, std::atoi(v));
+        else if (key2(a, "src_alt=", "alt_src=", v)) z_src = std::atof(v);
+        else if (sph && key2(a, "src_lat=", "lat_src=", v)) s0 = std::atof(v);
+        else if (sph && key2(a, "src_lon=", "lon_src=", v)) s1 = std::atof(v);
+        else if (!sph && key2(a, "src_x=", "x_src=", v)) s0 = std::atof(v);
+        // main.infraga.3d.cpp compares "y_src_y=" over 6 characters, so
+        // only src_y= sets y there
+        else if (!sph && key(a, "src_y=", v)) s1 = std::atof(v);
+        else if (key(a, "write_ray=", v)) write_rays = str2bool(v);
+        else if (key(a, "wvfrm_file=", v)) wvfrm_file = v;
+        else if (key(a, "wvfrm_opt=", v)) wvfrm_opt = v;
+        else if (key(a, "wvfrm_p0=", v)) wp0 = std::atof(v);
+        else if (key(a, "wvfrm_t0=", v)) wt0 = std::atof(v);
+        else if (key(a, "wvfrm_alpha=", v)) walpha = std::atof(v);
+        else if (key(a, "wvfrm_ref=", v)) wvfrm_ref = std::atof(v);
+        else if (key(a, "wvfrm_out_step=", v)) wvfrm_out_step = std::atof(v);
+        else if (key(a, "wvfrm_ds=", v)) w.ds = std::atof(v);
+        // the reference reads N_rise from 15 characters in (two past the '=')
+        else if (key(a, "wvfrm_N_rise=", v)) n_rise = (int)std::atof(std::strlen(a) >= 15 ? a + 15 : "");
+        else if (key(a, "wvfrm_N_period=", v)) n_period = (int)std::atof(v);
+        else if (key(a, "wvfrm_len=", v)) len = (long)std::atof(v);
+        else if (key(a, "wvfrm_filt1_g=", v)) w.filt1_g = std::atof(v);
+        else if (key(a, "wvfrm_filt1_n1=", v)) w.filt1_n1 = std::atof(v);
+        else if (key(a, "wvfrm_filt1_n2=", v)) w.filt1_n2 = std::atof(v);
+        else if (key(a, "wvfrm_filt2_g=", v)) w.filt2_g = std::atof(v);
+        else if (key(a, "wvfrm_filt2_n1=", v)) w.filt2_n1 = std::atof(v);
+        else if (key(a, "wvfrm_filt2_n2=", v)) w.filt2_n2 = std::atof(v);
+        else if (key(a, "freq=", v)) freq = std::atof(v);
+        else if (key(a, "abs_coeff=", v)) p.abs_coeff = std::max(0.0, std::atof(v));
+        else if (key2(a, "max_alt=", "alt_max=", v)) p.alt_max = std::atof(v);
+        else if (key2(a, "max_rng=", "rng_max=", v)) p.rng_max = std::atof(v);
+        else if (sph && key2(a, "max_time=", "time_max=", v)) { /* no time limit in -wnl_wvfrm */ }
+        else if (sph && key2(a, "min_lat=", "lat_min=", v)) p.bounds[0] = std::atof(v);
+        else if (sph && key2(a, "max_lat=", "lat_max=", v)) p.bounds[1] = std::atof(v);
+        else if (sph && key2(a, "min_lon=", "lon_min=", v)) p.bounds[2] = std::atof(v);
+        else if (sph && key2(a, "max_lon=", "lon_max=", v)) p.bounds[3] = std::atof(v);
+        else if (!sph && key2(a, "min_x=", "x_min=", v)) p.bounds[0] = std::atof(v);
+        else if (!sph && key2(a, "max_x=", "x_max=", v)) p.bounds[1] = std::atof(v);
+        else if (!sph && key2(a, "min_y=", "y_min=", v)) p.bounds[2] = std::atof(v);
+        else if (!sph && key2(a, "max_y=", "y_max=", v)) p.bounds[3] = std::atof(v);
+        else if (key2(a, "min_ds=", "ds_min=", v)) p.ds_min = std::atof(v);
+        else if (key2(a, "max_ds=", "ds_max=", v)) p.ds_max = std::atof(v);
+        else if (key2(a, "max_s=", "s_max=", v)) p.s_max = std::atof(v);
+        else if (key(a, "write_atmo=", v)) { /* profile dump: not produced */ }
+        else if (key(a, "prof_format=", v)) prof_format = v;
+        else if (key(a, "reverse_winds=", v)) reverse_winds = str2bool(v);
+        else if (key(a, "output_id=", v)) { custom_id = true; output_id = v; }
+        else if (key(a, "z_grnd=", v)) z_grnd = std::atof(v);
+        else if (key(a, "topo_use_BLw=", v)) { /* vertical wind needs terrain */ }
+        else if (key(a, "topo_file=", v)) {
+            std::cerr << GA_CLI_NAME << ": -wnl_wvfrm over terrain is not supported by this engine\n";
+            return 2;
+        } else {
+            std::cerr << GA_CLI_NAME << ": unrecognized parameter entry: " << a << '\n';
+            return 2;
+        }
+    }
+    if (wvfrm_out_step < 1.0e4) {
+        std::cerr << GA_CLI_NAME << ": wvfrm_out_step (intermediate waveform files) is not supported by this engine\n";
+        return 2;
+    }
+    std::vector<double> t, ps;
+    const bool from_file = std::strncmp(wvfrm_file, "none", 4) != 0;
+    if (from_file) {
+        if (!wv_load(wvfrm_file, t, ps)) {
+            std::cerr << GA_CLI_NAME << ": cannot read waveform file " << wvfrm_file << '\n';
+            return 1;
+        }
+        len = (long)t.size();
+    } else {
+        if (len < 1 || len > (1L << 24)) {
+            std::cerr << GA_CLI_NAME << ": bad wvfrm_len\n";
+            return 2;
+        }
+        wv_build(wvfrm_opt, wp0, wt0, walpha, (int)len, n_rise, n_period, t, ps);
+    }
+    w.len = (int32_t)len;
+    w.ref = wvfrm_ref;
+
+    ga_ctx *ctx = nullptr;
+    if (ga_ctx_create(&ctx, 0) != GA_OK) return die(nullptr, "no usable GPU");
+    int aid = 0;
+    if (ga_atmo_load_file(ctx, argv[2], prof_format, reverse_winds ? 1 : 0, &aid) != GA_OK)
+        return die(ctx, "cannot load the atmosphere");
+    const double z0 = std::isnan(z_grnd) ? profile_bottom(argv[2]) : z_grnd;
+    p.z_grnd = z0;
+    p.bounces = bounces;
+    p.freq = freq;
+    p.calc_amp = 1;
+    z_src = std::max(z_src, z0);
+    p.src[0] = s0;
+    p.src[1] = s1;
+    p.src[2] = z_src;
+    const std::string id = custom_id ? std::string(output_id) : strip_ext(argv[2]);
+    const double launch[2] = {incl, az};
+    std::vector<double> ti((size_t)len), pi_((size_t)len), to((size_t)len), po((size_t)len);
+    ga_arrival A{};
+    int32_t status = 0;
+    int64_t steps = 0;
+    ga_wvfrm_stats st{};
+    if (ga_wnl_wvfrm(ctx, aid, &p, &w, launch, 1, t.data(), ps.data(), ti.data(), pi_.data(), to.data(), po.data(),
+                     &A, &status, &steps, &st) != GA_OK)
+        return die(ctx, "waveform failed");
+    if (write_rays) {
+        // the same ray's write_rays rows (m == 1 or m % 15 == 0 of every leg)
+        ga_prop_params q = p;
+        q.turn_ht_min = -1.0e300;
+        q.time_max = 1.0e300;
+        std::vector<ga_arrival> arr((size_t)bounces + 1);
+        int n_out = 0;
+        std::vector<ga_path_pt> pts((size_t)1 << 16);
+        int64_t n_pts = 0;
+        ga_run_stats rs;
+        int rc = ga_fan_run_paths(ctx, aid, &q, launch, 1, GA_PATH_RAYS, arr.data(), bounces + 1, &n_out, pts.data(),
+                                  (int64_t)pts.size(), &n_pts, &rs);
+        if (rc == GA_E_CAP && n_pts > (int64_t)pts.size()) {
+            pts.resize((size_t)n_pts);
+            rc = ga_fan_run_paths(ctx, aid, &q, launch, 1, GA_PATH_RAYS, arr.data(), bounces + 1, &n_out, pts.data(),
+                                  (int64_t)pts.size(), &n_pts, &rs);
+        }
+        if (rc != GA_OK) return die(ctx, "ray path failed");
+        std::ofstream raypath(id + ".raypaths.dat");
+        // the sph writer's header names z first (main.infraga.sph.cpp:1313-1319)
+        raypath << (sph ? "# z [km]" : "# x [km]") << '\t' << (sph ? "lat [deg]" : "y [km]") << '\t'
+                << (sph ? "lon [deg]" : "z [km]") << '\t' << "trans. coeff. [dB]" << '\t' << "absorption [dB]" << '\t'
+                << "time [s]" << '\n';
+        if (sph) raypath << std::setprecision(8);
+        for (int64_t k = 0; k < n_pts; ++k) {
+            const ga_path_pt &P = pts[(size_t)k];
+            raypath << P.pos0 << '\t' << P.pos1 << '\t' << P.z << '\t' << P.tl << '\t' << P.absorption << '\t' << P.time
+                    << '\n';
+        }
+    }
+    if (status < 0) {
+        std::cout << "\n\t\tRay path does not reach the waveform reference location (" << wvfrm_ref << " km).\n\n";
+        ga_ctx_destroy(ctx);
+        return 1;
+    }
+    {
+        std::ofstream f(id + ".wvfrm_init.dat");
+        f << "# t [sec]" << '\t' << "p(t) [Pa]" << '\n';
+        for (long n = 0; n < len; ++n) f << std::setprecision(8) << ti[(size_t)n] << '\t' << pi_[(size_t)n] << '\n';
+    }
+    if (status == 1) {
+        std::cout << '\n' << '\t' << "Arrival summary:" << '\n';
+        if (sph) {
+            std::cout << "\t\tlatitude [deg] = " << std::setprecision(8) << A.pos0 << '\n';
+            std::cout << "\t\tlongitude [deg] = " << std::setprecision(8) << A.pos1 << '\n';
+        } else {
+            std::cout << "\t\tx [km E-W] = " << A.pos0 << '\n';
+            std::cout << "\t\ty [km N-S] = " << A.pos1 << '\n';
+        }
+        std::cout << "\t\ttime [s] = " << A.time << '\n';
+        std::cout << "\t\tcelerity [km/s] = " << A.cel << '\n';
+        std::cout << "\t\tturning height [km] = " << A.turn_ht << '\n';
+        std::cout << "\t\tarrival inclination [deg] = " << A.incl_arr << '\n';
+        std::cout << "\t\tback azimuth" << (sph ? " [deg]" : "") << " = " << A.back_az << '\n';
+        std::cout << (sph ? "\t\ttrans. coeff. [dB] = " : "\t\tattenuation (geometric) [dB] = ") << A.tl << '\n';
+        std::cout << "\t\tabsorption [dB] = " << A.absorption << '\n' << '\n';
+    } else {
+        std::cout << "\n\t\tRay path does not return to the ground.\n\n";
+    }
+    std::ofstream f(id + ".wvfrm_out.dat");
+    f << "# " << GA_CLI_NAME << " -wnl_wvfrm summary:" << '\n';
+    f << "#" << '\t' << "profile: " << argv[2] << '\n';
+    f << "#" << '\t' << "inclination: " << (incl * (Pi / 180.0)) * (180.0 / Pi) << '\n';
+    // the reference prints geoac::phi (90 - azimuth) here
+    f << "#" << '\t' << "azimuth: " << (Pi / 2.0 - az * (Pi / 180.0)) * (180.0 / Pi) << '\n';
+    f << "#" << '\t' << "bounces: " << bounces << '\n';
+    if (sph)
+        f << "#" << '\t' << "source (lat, lon, alt): " << (s0 * (Pi / 180.0)) * (180.0 / Pi) << ", "
+          << (s1 * (Pi / 180.0)) * (180.0 / Pi) << ", " << z_src << '\n';
+    else f << "#" << '\t' << "source location: " << s0 << ", " << s1 << ", " << z_src << '\n';
+    f << "#" << '\t' << "ground elevation: " << z0 << '\n';
+    f << "#" << '\t' << "S&B atten coeff: " << p.abs_coeff << '\n';
+    if (from_file) f << "#" << '\t' << "waveform source file: " << wvfrm_file << '\n';
+    else {
+        f << "#" << '\t' << "waveform option: " << wvfrm_opt << '\n';
+        f << "#" << '\t' << "waveform p0: " << wp0 << '\n';
+        f << "#" << '\t' << "waveform t0: " << wt0 << '\n';
+        f << "#" << '\t' << "waveform alpha: " << walpha << '\n';
+    }
+    f << "#" << '\t' << "waveform reference location: " << wvfrm_ref << '\n' << '\n';
+    f << "# t [sec]" << '\t' << "p(t) [Pa]" << '\n';
+    for (long n = 0; n < len; ++n) f << std::setprecision(8) << to[(size_t)n] << '\t' << po[(size_t)n] << '\n';
+    std::cout << GA_CLI_NAME << ": waveform of " << len << " samples, " << steps << " evolution steps, "
+              << st.ray_steps << " ray-steps, " << st.trace_ms + st.evolve_ms << " ms on the GPU\n";
+    ga_ctx_destroy(ctx);
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char **argv)
@@ -1436,6 +1734,8 @@ int main(int argc, char **argv)
     if (std::strncmp(argv[1], "-prop", 5) == 0) return run_prop(argc, argv);
     if (std::strncmp(argv[1], "-eig_search", 11) == 0) return run_eig_search(argc, argv);
     if (std::strncmp(argv[1], "-eig_direct", 11) == 0) return run_eig_direct(argc, argv);
+    if (std::strncmp(argv[1], "-wnl_wvfrm", 10) == 0 || std::strncmp(argv[1], "-w", 2) == 0)
+        return run_wnl_wvfrm(argc, argv);
     std::cerr << GA_CLI_NAME << ": option " << argv[1] << " is not supported by this engine\n";
     usage();
     return 2;

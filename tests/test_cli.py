@@ -131,6 +131,18 @@ def test_rngdep_rejected_arguments(args):
     assert r.returncode == 2, (r.stdout, r.stderr)
 
 
+@pytest.mark.parametrize("args", [
+    ("-wnl_wvfrm", "x.met", "topo_file=t.dat"),
+    ("-wnl_wvfrm", "x.met", "wvfrm_out_step=10"),
+    ("-wnl_wvfrm", "x.met", "no_such_key=1"),
+])
+def test_wnl_wvfrm_rejected_arguments(args):
+    """Terrain and intermediate waveform dumps are refused before any GPU
+    work, as are unknown keys."""
+    r = run("infraga-sph", *args)
+    assert r.returncode == 2, (r.stdout, r.stderr)
+
+
 @pytest.mark.gpu
 def test_rngdep_prop_text(engine, tmp_path):
     """infraga-sph-rngdep -prop on grid files with terrain: header, the

@@ -124,3 +124,79 @@ def test_argument_errors(engine, oracle, base):
         engine.wnl_wvfrm(base, p, wvfrm_params(1024, ds=0.0), [(12.0, 90.0)], t, ps)
     with pytest.raises(EngineError):
         engine.wnl_wvfrm(base, prop_params(GA_GEOM_SPH_RNGDEP), wvfrm_params(1024), [(12.0, 90.0)], t, ps)
+
+
+def _write_met(d, units):
+    import os
+    z, c, u, v, rho = units
+    met = os.path.join(d, "base.met")
+    with open(met, "w") as f:
+        for row in zip(z, c * 1e3, u * 1e3, v * 1e3, rho):
+            f.write(" ".join("%.17g" % x for x in row) + "\n")
+    # the loader's values: km/s from the file's m/s
+    return met, (z, (c * 1e3) / 1000.0, (u * 1e3) / 1000.0, (v * 1e3) / 1000.0, rho)
+
+
+def _rows(path):
+    return np.array([[float(x) for x in l.split("\t")] for l in open(path) if l.strip() and not l.startswith("#")])
+
+
+@pytest.mark.parametrize("binary,geom,src,extra", [
+    ("infraga-sph", GA_GEOM_SPH, (30.0, 0.0, 0.0), []),
+    ("infraga-3d", GA_GEOM_3D, (0.0, 0.0, 0.0), ["src_x=0.0", "src_y=0.0"]),
+])
+def test_cli_wnl_wvfrm(oracle, base_units, tmp_path, binary, geom, src, extra):
+    """infraga-sph / infraga-3d -wnl_wvfrm: the built-in impulse of
+    wvfrm_init.dat equals the oracle's waveforms.cpp restatement, and
+    wvfrm_out.dat the oracle's run_wnl_wvfrm, at the files' 8 significant
+    digits; header lines as the reference writes them; write_ray=true gives
+    the ray's raypaths.dat."""
+    import os
+    import subprocess
+    met, units = _write_met(str(tmp_path), base_units)
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "infraga_amd", "bin", binary)
+    args = [exe, "-wnl_wvfrm", met, "prof_format=zcuvd", "inclination=12.0", "azimuth=90.0", "bounces=1",
+            "wvfrm_len=1024", "wvfrm_p0=500.0", "write_ray=true", *extra]
+    r = subprocess.run(args, capture_output=True, text=True, cwd=tmp_path, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert "Arrival summary:" in r.stdout
+    t, ps = source(oracle, "impulse", 1024)
+    ref = oracle_run(oracle, units, geom, src, 12.0, 90.0, 1, t, ps, 1024)
+    init = _rows(os.path.join(str(tmp_path), "base.wvfrm_init.dat"))
+    np.testing.assert_allclose(init[:, 0], ref["t_init"], rtol=1e-7, atol=1e-6)
+    np.testing.assert_allclose(init[:, 1], ref["p_init"], rtol=1e-7, atol=1e-12)
+    out_text = open(os.path.join(str(tmp_path), "base.wvfrm_out.dat")).read()
+    assert out_text.startswith(f"# {binary} -wnl_wvfrm summary:\n")
+    assert "#\tazimuth: 0\n" in out_text             # geoac::phi in degrees, as the reference prints it
+    assert "#\twaveform option: impulse\n" in out_text and "#\twaveform p0: 500\n" in out_text
+    out = _rows(os.path.join(str(tmp_path), "base.wvfrm_out.dat"))
+    np.testing.assert_allclose(out[:, 0], ref["t_out"], rtol=1e-7)
+    peak = np.abs(ref["p_out"]).max()
+    np.testing.assert_allclose(out[:, 1], ref["p_out"], rtol=1e-7, atol=1e-7 * peak)
+    rp = _rows(os.path.join(str(tmp_path), "base.raypaths.dat"))
+    assert len(rp) > 10 and rp.shape[1] == 6
+
+
+def test_cli_wnl_wvfrm_nwave_from_file(oracle, base_units, tmp_path):
+    """wvfrm_file= (two columns, '#' comments) through the -w alias: the
+    file's samples are the source waveform."""
+    import os
+    import subprocess
+    met, units = _write_met(str(tmp_path), base_units)
+    t, ps = source(oracle, "Nwave", 2048, p0=200.0, t0=0.5)
+    wf = os.path.join(str(tmp_path), "src.dat")
+    with open(wf, "w") as f:
+        f.write("# t p\n")
+        for a, b in zip(t, ps):
+            f.write("%.17g %.17g\n" % (a, b))
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "infraga_amd", "bin",
+                       "infraga-sph")
+    r = subprocess.run([exe, "-w", met, "prof_format=zcuvd", "inclination=8.0", "azimuth=45.0", "bounces=1",
+                        "wvfrm_file=" + wf, "output_id=nw"], capture_output=True, text=True, cwd=tmp_path,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr
+    ref = oracle_run(oracle, units, GA_GEOM_SPH, (30.0, 0.0, 0.0), 8.0, 45.0, 1, t, ps, 2048)
+    out = _rows(os.path.join(str(tmp_path), "nw.wvfrm_out.dat"))
+    peak = np.abs(ref["p_out"]).max()
+    np.testing.assert_allclose(out[:, 1], ref["p_out"], rtol=1e-7, atol=1e-7 * peak)
+    assert "#\twaveform source file: " + wf + "\n" in open(os.path.join(str(tmp_path), "nw.wvfrm_out.dat")).read()
