@@ -37,6 +37,7 @@ Prints one JSON line.  --gpus N with N > 1 exits: this build runs one GPU.
 from __future__ import annotations
 
 import argparse
+import math
 import json
 import os
 import sys
@@ -397,6 +398,85 @@ def serve_leg(eng, trace):
     return out
 
 
+WV_LEN = 8192
+
+
+def wvfrm_workload():
+    """The reference's own -wnl_wvfrm example (ToyAtmo.met, source 30 N
+    100 W, inclination 4.1314876, impulse p0 = 500 Pa, t0 = 0.1 s, 8192
+    samples, 1 bounce) swept over 256 azimuths within +-40 degrees of its
+    azimuth -84.969455: one waveform per CU."""
+    from infraga_amd import synth, waveforms
+    cols = np.load(os.path.join(ROOT, "tests", "golden", "toyatmo.npz"))["zTuvdp"]
+    units = synth.from_zTuvdp(cols)
+    az = -84.969455 + np.linspace(-40.0, 40.0, 256)
+    L = np.stack([np.full(len(az), 4.1314876), az], axis=1)
+    t, ps = waveforms.build("impulse", p0=500.0, t0=0.1, alpha=4.0, length=WV_LEN)
+    return units, L, t, ps
+
+
+def _wv_worker(job):
+    """The oracle's run_wnl_wvfrm for one launch (one process)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from ga_oracle_py import GEOM_SPH, Medium, Oracle, WvParams, wv_run
+    units, i, incl, az, t, ps = job
+    m = Medium(Oracle(), GEOM_SPH, *units)
+    t0 = time.time()
+    r = wv_run(m, m.params(True), WvParams.default(len=WV_LEN), (30.0, -100.0, 0.0), incl, az, 1, t, ps)
+    t1 = time.time()
+    return {"i": i, "r": r, "t0": t0, "t1": t1}
+
+
+def wvfrm_leg(eng, with_cpu, cores=1):
+    """Weakly non-linear waveforms (ga_wnl_wvfrm) on the example's sweep:
+    trace and evolution times, evolution steps per second, the
+    evolution kernel's LDS roofline; the oracle on all host cores over one
+    waveform per core, with the GPU's waveforms checked against it."""
+    from infraga_amd.engine import GA_GEOM_SPH, prop_params, wvfrm_params
+    units, L, t, ps = wvfrm_workload()
+    aid = eng.load_atmo(*units)
+    p = prop_params(GA_GEOM_SPH, bounces=1, src=(30.0, -100.0, 0.0))
+    w = wvfrm_params(WV_LEN)
+    eng.wnl_wvfrm(aid, p, w, L[:2], t, ps)           # warm-up (kernel load, buffers)
+    t0 = time.perf_counter()
+    g = eng.wnl_wvfrm(aid, p, w, L, t, ps)
+    wall = time.perf_counter() - t0
+    eng.release_atmo(aid)
+    st = g["stats"]
+    lg = int(round(math.log2(WV_LEN)))
+    # LDS bytes per evolution step: three radix-2 transforms of len complex
+    # doubles, each stage reading and writing every element (32 len bytes)
+    lds_step = 3 * 32 * WV_LEN * lg
+    ach = lds_step * st.wv_steps / (st.evolve_ms * 1e-3) / 1e9
+    peak = 256 * 128 * 2.4e9 / 1e9                   # 256 CUs x 128 B/clk x 2.4 GHz
+    out = {"workload": f"{len(L)} waveforms of {WV_LEN} samples: ToyAtmo, source 30N 100W, inclination 4.1314876, "
+                       "azimuth -84.969455 +- 40 deg, impulse p0 500 Pa t0 0.1 s, 1 bounce",
+           "waveforms": int(len(L)), "returned": int((g["status"] == 1).sum()), "wv_steps": int(st.wv_steps),
+           "ray_steps": int(st.ray_steps), "trace_ms": st.trace_ms, "evolve_ms": st.evolve_ms, "wall_ms": wall * 1e3,
+           "waveforms_per_s": len(L) / wall, "wv_steps_per_s": st.wv_steps / (st.evolve_ms * 1e-3),
+           "roofline": {"kernel": "wv_evolve_kernel", "bound": "lds", "achieved": ach, "peak": peak, "unit": "GB/s",
+                        "frac": ach / peak, "traffic": None,
+                        "note": f"{lds_step} LDS bytes per step (3 x 13 radix-2 stages x 32 B x len)"}}
+    if with_cpu:
+        n = max(1, min(cores, len(L)))
+        pick = list(range(0, len(L), max(1, len(L) // n)))[:n]
+        jobs = [(units, i, float(L[i, 0]), float(L[i, 1]), t, ps) for i in pick]
+        res, cwall = _pool_map(_wv_worker, jobs, n)
+        steps = sum(r["r"]["steps"] for r in res)
+        out["cpu_baseline"] = {"value": steps / cwall, "unit": "wv-steps/s", "cores": n, "kind": "port",
+                               "sample": f"{len(pick)} waveforms (every {len(L) // n}th azimuth) on {n} processes, "
+                                         f"{steps} steps in {cwall:.1f} s wall"}
+        worst, same = 0.0, 0
+        for r in res:
+            i, o = r["i"], r["r"]
+            peak_p = np.abs(o["p_out"]).max()
+            worst = max(worst, float(np.abs(g["p_out"][i] - o["p_out"]).max() / peak_p))
+            same += int(g["steps"][i] == o["steps"] and g["status"][i] == (1 if o["ok"] else 0))
+        out["sample_parity"] = {"waveforms": len(res), "steps_and_status_equal": same,
+                                "max_rel_pressure_diff": worst, "ok": bool(same == len(res) and worst <= 1e-6)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -406,6 +486,7 @@ def main():
     ap.add_argument("--no-eig", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-rngdep", action="store_true")
+    ap.add_argument("--no-wvfrm", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the scheduler autotuner (library defaults)")
     ap.add_argument("--one-pass", action="store_true",
                     help="one untimed fan pass (for hardware-counter runs); no JSON line")
@@ -546,6 +627,10 @@ def main():
         for leg in ("rngdep", "rngdep3d"):
             if "sample_parity" in line[leg]:
                 ok = ok and line[leg]["sample_parity"]["ok"]
+    if not args.no_wvfrm:
+        line["wvfrm"] = wvfrm_leg(eng, not args.no_cpu, host["cores"])
+        if "sample_parity" in line["wvfrm"]:
+            ok = ok and line["wvfrm"]["sample_parity"]["ok"]
     eng.close()
     print(json.dumps(line))
     if not ok:

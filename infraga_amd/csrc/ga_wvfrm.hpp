@@ -139,11 +139,14 @@ __global__ void __launch_bounds__(WV_T) wv_evolve_kernel(WvArgs A)
     __shared__ double re[WV_MAX_LEN], im[WV_MAX_LEN];
     __shared__ double red[WV_T / 64];
     __shared__ WvPoint pt;
-    const int tid = threadIdx.x, w = blockIdx.x;
+    const int tid0 = threadIdx.x, w = blockIdx.x;
+    const int tid = tid0;
     if (w >= A.n_wv) return;                      // grid == n_wv: uniform per workgroup
     const int len = A.len, lg = A.lg, half = len >> 1;
     const double *t = A.t;
     const double dt = t[1] - t[0];
+    const double fs = 1.0 / dt;
+    const double ilen = 1.0 / len;                // exact: len is a power of two
     const double c0 = A.c0[w];
     const Medium &M = A.M;
 
@@ -189,7 +192,7 @@ __global__ void __launch_bounds__(WV_T) wv_evolve_kernel(WvArgs A)
                 if (b <= half) {
                     Ur[j] = re[b];
                     Ui[j] = im[b];
-                    fm = fmax(fm, (1.0 / dt * b / len) * sqrt(Ur[j] * Ur[j] + Ui[j] * Ui[j]));
+                    fm = fmax(fm, (fs * b * ilen) * sqrt(Ur[j] * Ur[j] + Ui[j] * Ui[j]));
                 }
             }
             __syncthreads();
@@ -204,6 +207,12 @@ __global__ void __launch_bounds__(WV_T) wv_evolve_kernel(WvArgs A)
             double s = 0.0;
             int ks = 0;
             while (s < s_tot) {
+                // the lane index passes through an empty asm each step, so the
+                // per-lane addresses and frequencies below are recomputed
+                // (a few instructions) instead of being kept live across the
+                // loop, which spills at 1024 threads
+                int tid = tid0;
+                asm volatile("" : "+v"(tid));
                 ks = wv_seek(R, n, ks, s);
                 const double bc = wv_beta(R, ks, s);
                 double ds = A.ds_wv / (PI * Uf_max * bc);
@@ -295,7 +304,7 @@ __global__ void __launch_bounds__(WV_T) wv_evolve_kernel(WvArgs A)
                             const double Br = 0.5 * (zb + zd), Bi = 0.5 * (zc - za);
                             Pr[j] = h1 * Ai + h2 * Bi;
                             Pi[j] = h1 * Ar + h2 * Br;
-                            const double freq = 1.0 / dt * b / len;
+                            const double freq = fs * b * ilen;
                             SbGas g = pt.g;
                             sb_gas_freq(g, freq);
                             double Q, Rr;
@@ -323,7 +332,7 @@ __global__ void __launch_bounds__(WV_T) wv_evolve_kernel(WvArgs A)
                 for (int j = 0; j < WV_BPT; ++j) {
                     const int b = tid + j * WV_T;
                     if (b <= half) {
-                        const double freq = 1.0 / dt * b / len;
+                        const double freq = fs * b * ilen;
                         const double f1 = A.filt[2 * b];
                         const double Cr = re[b], Ci = im[b];
                         Ur[j] -= PI * freq * (Pr[j] + h3 * Ci) * f1;
@@ -356,7 +365,7 @@ __global__ void __launch_bounds__(WV_T) wv_evolve_kernel(WvArgs A)
 #pragma unroll
                 for (int j = 0; j < WV_SPT; ++j) {
                     const int m = tid + j * WV_T;
-                    if (m < len) re[m] = re[m] * A.tukey[m] / len;
+                    if (m < len) re[m] = re[m] * A.tukey[m] * ilen;
                 }
                 Uf_max = wv_block_max(fm, red, tid);   // its barriers publish the new u
                 s += ds;

@@ -198,3 +198,15 @@ def test_nonlinear_steepening_is_earnshaw(oracle):
     # the compression arrives earlier: the peak moves to negative t
     assert t[np.argmax(u)] < -0.5 * sigma
     assert np.abs(u - earnshaw(t, shape, -sigma)).max() > 100 * err
+
+
+@pytest.mark.parametrize("option,alpha", [("impulse", 4.0), ("Uwave", 1.0), ("Nwave", 4.0), ("other", 3.0)])
+def test_package_waveforms_match_oracle(oracle, option, alpha):
+    """infraga_amd.waveforms.build (the package's builder for callers of
+    Engine.wnl_wvfrm) equals the oracle's waveforms.cpp restatement to
+    rounding, for the three built-in shapes and an unrecognised option."""
+    from infraga_amd import waveforms
+    t, p = waveforms.build(option, p0=300.0, t0=0.2, alpha=alpha, length=2048, n_rise=8, n_period=30)
+    to, po, _ = wv_build(oracle, option, p0=300.0, t0=0.2, alpha=alpha, length=2048, n_rise=8, n_period=30)
+    np.testing.assert_allclose(t, to, rtol=1e-15, atol=1e-15)
+    np.testing.assert_allclose(p, po, rtol=1e-13, atol=1e-12)
