@@ -349,7 +349,7 @@ typedef struct {
 typedef struct {
     int64_t ray_steps;     /* RK4 iterations of the ray traces */
     int64_t wv_steps;      /* evolution steps summed over the waveforms */
-    double  trace_ms;      /* device time of the ray traces */
+    double  trace_ms;      /* device time of the ray traces and the per-ray set-up */
     double  evolve_ms;     /* device time of the evolution kernel */
     double  wall_ms;       /* host wall time of the call */
     int32_t n_waveforms;

@@ -451,6 +451,7 @@ struct ga_ctx {
     std::map<int, DevAtmo> atmos;
     std::map<int, DevRd> rds;
     int next_id = 1;
+    int64_t wv_recs_per_ray = 0;   // most waveform-row records one ray needed so far
     std::string err;
 };
 
@@ -2596,75 +2597,18 @@ namespace {
 // coefficients and the output scaling on the host from those rows, the
 // evolution on the GPU (wv_evolve_kernel, one workgroup per waveform).
 // ---------------------------------------------------------------------------
-struct WvRowH {
-    double y[6];
-    double D, c, rho, cg, dt;
-};
-
-// Ray length of the rows a -> b as run_wnl_wvfrm measures it toward the
-// reference point (main.infraga.sph.cpp:1360-1368: midpoint radius and
-// latitude; 3D: the chord).
-double wv_ref_len(bool sph, const WvRowH &a, const WvRowH &b)
+// Records of the waveform trace from their chunks into ray order
+// (one workgroup of PCHUNK threads per chunk; off: first record of each ray).
+__global__ void wv_place_kernel(const ga_path_pt *pts, const int32_t *citem, const int32_t *cseq,
+                                const int32_t *cfill, unsigned int used, const int64_t *off, int n, ga_path_pt *dst)
 {
-    if (sph) {
-        const double dr = b.y[0] - a.y[0], r = a.y[0] + dr / 2.0;
-        const double dth = b.y[1] - a.y[1], th = a.y[1] + dth / 2.0;
-        const double dph = b.y[2] - a.y[2];
-        return std::sqrt(std::pow(dr, 2) + std::pow(r * dth, 2) + std::pow(r * std::cos(th) * dph, 2));
-    }
-    return std::sqrt(std::pow(b.y[0] - a.y[0], 2) + std::pow(b.y[1] - a.y[1], 2) + std::pow(b.y[2] - a.y[2], 2));
-}
-
-// Arc length of the rows y -> z in wnl_wvfrm's splines
-// (geoac.eqset.sph.strat.cpp:903-931: radius and latitude of y; 3D: chord).
-double wv_arc(bool sph, const WvRowH &y, const WvRowH &z)
-{
-    if (sph)
-        return std::sqrt(std::pow(z.y[0] - y.y[0], 2) + std::pow(y.y[0] * (z.y[1] - y.y[1]), 2) +
-                         std::pow(y.y[0] * std::cos(y.y[1]) * (z.y[2] - y.y[2]), 2));
-    return std::sqrt(std::pow(z.y[0] - y.y[0], 2) + std::pow(z.y[1] - y.y[1], 2) + std::pow(z.y[2] - y.y[2], 2));
-}
-
-double wv_nu(const WvRowH &r) { return std::sqrt(std::pow(r.y[3], 2) + std::pow(r.y[4], 2) + std::pow(r.y[5], 2)); }
-
-// Reference-point quantities (main.infraga.sph.cpp:1370-1392).
-struct WvRef {
-    double c0, rho0, nu0, cg0, D0, tref;
-};
-
-// Rows k1..k2-1 of a leg as the spline rows of wv_evolve_kernel
-// (wnl_wvfrm's set-up, geoac.eqset.sph.strat.cpp:903-941): s, position, D,
-// beta with beta0 = 1.2 p0 / (rho0 c0^2) (nu0 c0) / (cg0 c_src) (SI units),
-// and the caustic beta from dD/ds (its first row copies the second).
-void wv_leg_rows(bool sph, const std::vector<WvRowH> &R, int k1, int k2, double c_src, const WvRef &r, double p0,
-                 std::vector<double> &out)
-{
-    const int n = k2 - k1;
-    const size_t o = out.size();
-    out.resize(o + (size_t)n * WV_ROW, 0.0);
-    const double beta0 = 1.2 * p0 / ((r.rho0 * 1.0e3) * std::pow(r.c0 * 1.0e3, 2)) * (r.nu0 * r.c0) / (r.cg0 * c_src);
-    double s = 0.0;
-    for (int k = 0; k < n; ++k) {
-        const WvRowH &y = R[(size_t)(k1 + k)];
-        const double nu = wv_nu(y);
-        double *q = &out[o + (size_t)k * WV_ROW];
-        q[WV_F_S] = s;
-        q[WV_F_POS] = y.y[0];
-        q[WV_F_POS + 1] = y.y[1];
-        q[WV_F_POS + 2] = y.y[2];
-        q[WV_F_D] = y.D;
-        q[WV_F_BETA] = beta0 * std::sqrt(std::fabs((r.D0 * r.rho0 * std::pow(r.cg0, 3)) / (y.D * y.rho * std::pow(y.c, 3)) *
-                                                   (y.c * std::pow(nu, 3)) / (r.c0 * std::pow(r.nu0, 3))));
-        if (k < 1) q[WV_F_BETAC] = 0.0;
-        else {
-            const double *qp = q - WV_ROW;
-            q[WV_F_BETAC] = beta0 * std::sqrt(std::fabs((r.D0 * r.rho0 * std::pow(r.cg0, 3)) /
-                                                        ((q[WV_F_D] - qp[WV_F_D]) / (q[WV_F_S] - qp[WV_F_S]) * y.rho * std::pow(y.c, 3)) *
-                                                        (y.c * std::pow(nu, 3)) / (r.c0 * std::pow(r.nu0, 3))));
-        }
-        if (k + 1 < n) s += wv_arc(sph, y, R[(size_t)(k1 + k + 1)]);
-    }
-    out[o + WV_F_BETAC] = out[o + WV_ROW + WV_F_BETAC];
+    const unsigned int c = blockIdx.x;
+    const int r = threadIdx.x;
+    if (c >= used || r >= cfill[c]) return;
+    const int it = citem[c];
+    if (it < 0 || it >= n) return;
+    const int64_t at = off[it] + (int64_t)cseq[c] * PCHUNK + r;
+    if (at < off[it + 1]) dst[at] = pts[(size_t)c * PCHUNK + r];
 }
 
 // The source waveform's tables: the Tukey window (parameter 0.8), the two
@@ -2745,13 +2689,16 @@ int ga_wnl_wvfrm(ga_ctx *ctx, int atmo_id, const ga_prop_params *p_in, const ga_
     Events ev;
     if (!ev.ok) return fail(ctx, GA_E_HIP, "hipEventCreate failed");
 
-    // 1. the traces, every row of every leg kept; the first guess of the
-    // record count is re-run with the exact count when too small
-    int64_t cap = (int64_t)n * slots * 2 * 2048, total = 0;
-    std::unique_ptr<ga_path_pt[]> rec;
+    // 1. the traces, every row of every leg kept (records stay on the GPU);
+    // the capacity is sized from the largest ray of the previous call and
+    // the trace re-run with the exact count when a ray needs more
+    const int64_t per_ray = std::max<int64_t>((int64_t)slots * 2 * 2048, ctx->wv_recs_per_ray + ctx->wv_recs_per_ray / 8);
+    int64_t cap = (int64_t)n * per_ray, total = 0;
     double trace_ms = 0.0;
+    PathDev pd;
+    std::vector<int32_t> item_pts(nl);
+    unsigned int used = 0;
     for (int pass = 0;; ++pass) {
-        PathDev pd;
         PathOut P{};
         int rc = pd.setup(ctx, cap, n, PATH_WVROWS, ctx->stream, P);
         if (rc != GA_OK) return rc;
@@ -2763,11 +2710,14 @@ int ga_wnl_wvfrm(ga_ctx *ctx, int atmo_id, const ga_prop_params *p_in, const ga_
         float ms = 0.0f;
         HIPCHK(ctx, hipEventElapsedTime(&ms, ev.e0, ev.e1));
         trace_ms += ms;
-        rec.reset(new (std::nothrow) ga_path_pt[(size_t)std::max<int64_t>(cap, 1)]);
-        if (!rec) return fail(ctx, GA_E_NOMEM, "host buffer of %lld waveform rows", (long long)cap);
-        rc = pd.gather(ctx, rec.get(), cap, &total);
-        if (rc == GA_OK) break;
-        if (rc != GA_E_CAP || pass > 0) return rc;
+        HIPCHK(ctx, hipMemcpy(&used, pd.ctr.p, sizeof(unsigned int), hipMemcpyDeviceToHost));
+        HIPCHK(ctx, hipMemcpy(item_pts.data(), pd.item_pts.p, nl * sizeof(int32_t), hipMemcpyDeviceToHost));
+        total = 0;
+        int64_t most = 0;
+        for (int i = 0; i < n; ++i) { total += item_pts[i]; most = std::max<int64_t>(most, item_pts[i]); }
+        ctx->wv_recs_per_ray = std::max(ctx->wv_recs_per_ray, most);
+        if (total <= cap && used <= pd.max_chunks) break;
+        if (pass > 0) return fail(ctx, GA_E_HIP, "waveform rows overflow (%lld of %lld)", (long long)total, (long long)cap);
         cap = total;
     }
     std::vector<int32_t> cnt(nl), stp(nl);
@@ -2775,148 +2725,103 @@ int ga_wnl_wvfrm(ga_ctx *ctx, int atmo_id, const ga_prop_params *p_in, const ga_
     HIPCHK(ctx, hipMemcpy(cnt.data(), d_c.p, nl * sizeof(int32_t), hipMemcpyDeviceToHost));
     HIPCHK(ctx, hipMemcpy(stp.data(), d_s.p, nl * sizeof(int32_t), hipMemcpyDeviceToHost));
     HIPCHK(ctx, hipMemcpy(all.data(), d_o.p, all.size() * sizeof(ga_arrival), hipMemcpyDeviceToHost));
-
-    // 2. per ray: legs, reference point, spline rows of the evolution
-    std::vector<double> tukey, filt, tw, rows;
-    wv_tables(w, tukey, filt, tw);
-    std::vector<int32_t> wv_leg(1, 0);
-    std::vector<int64_t> leg_row(1, 0);
-    std::vector<double> c0s;
-    std::vector<int> wv_ray;                      // evolution index -> ray
-    struct RayOut {
-        WvRef ref;
-        double tt;
-        WvRowH last;
-        bool started;
-    };
-    std::vector<RayOut> ro(nl);
-    std::vector<std::vector<WvRowH>> legs;
-    int64_t at = 0;
+    std::vector<int64_t> off(nl + 1, 0);
     for (int i = 0; i < n; ++i) {
-        legs.clear();
-        for (; at < total && rec[at].launch_idx == i; at += 2) {
-            const ga_path_pt &a = rec[at];
-            if (at + 1 >= total || a.kind != PT_WV_STATE || rec[at + 1].kind != PT_WV_MEDIUM ||
-                rec[at + 1].launch_idx != i || a.leg < 0)
-                return fail(ctx, GA_E_HIP, "waveform rows of ray %d out of order", i);
-            const ga_path_pt &b = rec[at + 1];
-            if ((size_t)a.leg >= legs.size()) legs.resize((size_t)a.leg + 1);
-            auto &L = legs[(size_t)a.leg];
-            if (a.step != (int)L.size()) return fail(ctx, GA_E_HIP, "waveform row %d of ray %d missing", (int)L.size(), i);
-            L.push_back(WvRowH{{a.pos0, a.pos1, a.z, a.tl, a.absorption, a.time}, b.pos0, b.pos1, b.z, b.tl, b.absorption});
-        }
-        RayOut &o = ro[(size_t)i];
-        o.started = false;
-        o.tt = 0.0;
-        if (legs.empty() || legs[0].empty()) return fail(ctx, GA_E_HIP, "no waveform rows for ray %d", i);
-        const double c_src = legs[0][0].c;
-        double ray_length = 0.0;
-        const int32_t first_leg = (int32_t)(leg_row.size() - 1);
-        for (size_t j = 0; j < legs.size(); ++j) {
-            const auto &R = legs[j];
-            const int k = (int)R.size() - 1;
-            for (int m = 1; m < k; ++m) o.tt += R[(size_t)m].dt;
-            int k1 = -1;
-            if (!o.started) {
-                for (int kr = 0; kr < k; ++kr) {
-                    ray_length += wv_ref_len(sph, R[(size_t)kr], R[(size_t)kr + 1]);
-                    if (ray_length >= w->ref) {
-                        const WvRowH &y = R[(size_t)kr + 1];
-                        o.ref = WvRef{y.c, y.rho, wv_nu(y), y.cg, y.D, 0.0};
-                        for (int q = 1; q <= kr + 1; ++q) o.ref.tref += R[(size_t)q].dt;
-                        o.started = true;
-                        k1 = kr + 1;
-                        break;
-                    }
-                }
-            } else {
-                k1 = 0;
-            }
-            if (k1 >= 0 && k - k1 >= 2) {
-                wv_leg_rows(sph, R, k1, k, c_src, o.ref, p0, rows);
-                leg_row.push_back((int64_t)(rows.size() / WV_ROW));
-            }
-        }
-        o.last = legs.back().back();
-        if (o.started) {
-            wv_leg.push_back((int32_t)(leg_row.size() - 1));
-            c0s.push_back(o.ref.c0);
-            wv_ray.push_back(i);
-        } else if ((int32_t)(leg_row.size() - 1) != first_leg) {
-            return fail(ctx, GA_E_HIP, "legs recorded for ray %d without a reference point", i);
-        }
+        if (item_pts[i] < 2 || (item_pts[i] & 1)) return fail(ctx, GA_E_HIP, "waveform rows of ray %d incomplete", i);
+        off[i + 1] = off[i] + item_pts[i];
     }
 
-    // 3. the evolution, one workgroup per waveform
-    const int nq = (int)wv_ray.size();
+    // 2. on the GPU: records in ray order, then per ray the reference point
+    // and the spline rows of the legs it evolves over
+    hipStream_t st = ctx->stream;
+    const size_t n_rows = (size_t)(total / 2);
+    DevBuf<ga_path_pt> d_rec;
+    DevBuf<int64_t> d_off, d_lf;
+    DevBuf<int32_t> d_ln;
+    DevBuf<double> d_rows, d_tw, d_t, d_tk, d_f, d_u0, d_u;
+    DevBuf<WvRay> d_ray;
+    DevBuf<long long> d_st;
+    std::vector<double> tukey, filt, tw;
+    wv_tables(w, tukey, filt, tw);
     std::vector<double> u0((size_t)len);
     for (int m = 0; m < len; ++m) u0[m] = p_src[m] / p0;
-    std::vector<double> uq((size_t)nq * len);
-    std::vector<long long> sq((size_t)nq, 0);
-    double evolve_ms = 0.0;
-    if (nq > 0) {
-        DevBuf<double> d_tw, d_t, d_tk, d_f, d_u0, d_u, d_rows, d_c0;
-        DevBuf<int32_t> d_wl;
-        DevBuf<int64_t> d_lr;
-        DevBuf<long long> d_st;
-        if (!d_tw.alloc(tw.size()) || !d_t.alloc(len) || !d_tk.alloc(len) || !d_f.alloc(filt.size()) ||
-            !d_u0.alloc(len) || !d_u.alloc((size_t)nq * len) || !d_rows.alloc(std::max<size_t>(rows.size(), 1)) ||
-            !d_c0.alloc(nq) || !d_wl.alloc(wv_leg.size()) || !d_lr.alloc(leg_row.size()) || !d_st.alloc(nq))
-            return fail(ctx, GA_E_NOMEM, "device buffers for %d waveforms", nq);
-        hipStream_t st = ctx->stream;
-        HIPCHK(ctx, hipMemcpyAsync(d_tw.p, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice, st));
-        HIPCHK(ctx, hipMemcpyAsync(d_t.p, t_src, (size_t)len * sizeof(double), hipMemcpyHostToDevice, st));
-        HIPCHK(ctx, hipMemcpyAsync(d_tk.p, tukey.data(), (size_t)len * sizeof(double), hipMemcpyHostToDevice, st));
-        HIPCHK(ctx, hipMemcpyAsync(d_f.p, filt.data(), filt.size() * sizeof(double), hipMemcpyHostToDevice, st));
-        HIPCHK(ctx, hipMemcpyAsync(d_u0.p, u0.data(), (size_t)len * sizeof(double), hipMemcpyHostToDevice, st));
-        if (!rows.empty())
-            HIPCHK(ctx, hipMemcpyAsync(d_rows.p, rows.data(), rows.size() * sizeof(double), hipMemcpyHostToDevice, st));
-        HIPCHK(ctx, hipMemcpyAsync(d_c0.p, c0s.data(), (size_t)nq * sizeof(double), hipMemcpyHostToDevice, st));
-        HIPCHK(ctx, hipMemcpyAsync(d_wl.p, wv_leg.data(), wv_leg.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
-        HIPCHK(ctx, hipMemcpyAsync(d_lr.p, leg_row.data(), leg_row.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
-        WvArgs A;
-        std::memset(&A, 0, sizeof(A));
+    if (!d_rec.alloc((size_t)total) || !d_off.alloc(nl + 1) || !d_lf.alloc(nl * slots) || !d_ln.alloc(nl * slots) ||
+        !d_rows.alloc(std::max<size_t>(n_rows, 1) * WV_ROW) || !d_ray.alloc(nl) || !d_tw.alloc(tw.size()) ||
+        !d_t.alloc(len) || !d_tk.alloc(len) || !d_f.alloc(filt.size()) || !d_u0.alloc(len) ||
+        !d_u.alloc(nl * (size_t)len) || !d_st.alloc(nl))
+        return fail(ctx, GA_E_NOMEM, "device buffers for %d waveforms (%zu rows)", n, n_rows);
+    if (slots > WV_MAX_SLOTS) return fail(ctx, GA_E_ARG, "waveforms take at most %d bounces", WV_MAX_SLOTS - 1);
+    HIPCHK(ctx, hipMemcpyAsync(d_off.p, off.data(), off.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    HIPCHK(ctx, hipMemcpyAsync(d_tw.p, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(ctx, hipMemcpyAsync(d_t.p, t_src, (size_t)len * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(ctx, hipMemcpyAsync(d_tk.p, tukey.data(), (size_t)len * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(ctx, hipMemcpyAsync(d_f.p, filt.data(), filt.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(ctx, hipMemcpyAsync(d_u0.p, u0.data(), (size_t)len * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(ctx, hipEventRecord(ev.e0, st));
+    if (used > 0)
+        hipLaunchKernelGGL(wv_place_kernel, dim3(used), dim3(PCHUNK), 0, st, (const ga_path_pt *)pd.pts.p,
+                           (const int32_t *)pd.item.p, (const int32_t *)pd.seq.p, (const int32_t *)pd.fill.p, used,
+                           (const int64_t *)d_off.p, n, d_rec.p);
+    if (sph)
+        hipLaunchKernelGGL(wv_prep_kernel<true>, dim3(n), dim3(WV_PREP_T), 0, st, (const ga_path_pt *)d_rec.p,
+                           (const int64_t *)d_off.p, n, slots, w->ref, p0, d_rows.p, d_lf.p, d_ln.p, d_ray.p);
+    else
+        hipLaunchKernelGGL(wv_prep_kernel<false>, dim3(n), dim3(WV_PREP_T), 0, st, (const ga_path_pt *)d_rec.p,
+                           (const int64_t *)d_off.p, n, slots, w->ref, p0, d_rows.p, d_lf.p, d_ln.p, d_ray.p);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipEventRecord(ev.e1, st));
+
+    // 3. the evolution, one workgroup per waveform (rays without a
+    // reference point have no legs and leave their waveform as it was)
+    Events ev2;
+    if (!ev2.ok) return fail(ctx, GA_E_HIP, "hipEventCreate failed");
+    WvArgs A;
+    std::memset(&A, 0, sizeof(A));
+    {
         Limits L;
         int rc = GA_OK;
         if (!build_medium(ctx, it->second, &p, A.M, L, rc)) return rc;
-        A.M.abs = nullptr;                        // per-bin absorption is evaluated directly
-        A.len = len;
-        A.lg = lg;
-        A.n_wv = nq;
-        A.ds_wv = w->ds;
-        A.tw = d_tw.p;
-        A.t = d_t.p;
-        A.tukey = d_tk.p;
-        A.filt = d_f.p;
-        A.u0 = d_u0.p;
-        A.u = d_u.p;
-        A.wv_leg = d_wl.p;
-        A.leg_row = d_lr.p;
-        A.rows = d_rows.p;
-        A.c0 = d_c0.p;
-        A.steps = d_st.p;
-        HIPCHK(ctx, hipEventRecord(ev.e0, st));
-        if (sph) hipLaunchKernelGGL(wv_evolve_kernel<true>, dim3(nq), dim3(WV_T), 0, st, A);
-        else hipLaunchKernelGGL(wv_evolve_kernel<false>, dim3(nq), dim3(WV_T), 0, st, A);
-        HIPCHK(ctx, hipGetLastError());
-        HIPCHK(ctx, hipEventRecord(ev.e1, st));
-        HIPCHK(ctx, hipStreamSynchronize(st));
-        float ms = 0.0f;
-        HIPCHK(ctx, hipEventElapsedTime(&ms, ev.e0, ev.e1));
-        evolve_ms = ms;
-        HIPCHK(ctx, hipMemcpy(uq.data(), d_u.p, uq.size() * sizeof(double), hipMemcpyDeviceToHost));
-        HIPCHK(ctx, hipMemcpy(sq.data(), d_st.p, sq.size() * sizeof(long long), hipMemcpyDeviceToHost));
     }
+    A.M.abs = nullptr;                            // per-bin absorption is evaluated directly
+    A.len = len;
+    A.lg = lg;
+    A.n_wv = n;
+    A.ds_wv = w->ds;
+    A.tw = d_tw.p;
+    A.t = d_t.p;
+    A.tukey = d_tk.p;
+    A.filt = d_f.p;
+    A.u0 = d_u0.p;
+    A.u = d_u.p;
+    A.slots = slots;
+    A.leg_first = d_lf.p;
+    A.leg_n = d_ln.p;
+    A.rows = d_rows.p;
+    A.ray = d_ray.p;
+    A.steps = d_st.p;
+    HIPCHK(ctx, hipEventRecord(ev2.e0, st));
+    if (sph) hipLaunchKernelGGL(wv_evolve_kernel<true>, dim3(n), dim3(WV_T), 0, st, A);
+    else hipLaunchKernelGGL(wv_evolve_kernel<false>, dim3(n), dim3(WV_T), 0, st, A);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipEventRecord(ev2.e1, st));
+    HIPCHK(ctx, hipStreamSynchronize(st));
+    float prep_ms = 0.0f, evo_ms = 0.0f;
+    HIPCHK(ctx, hipEventElapsedTime(&prep_ms, ev.e0, ev.e1));
+    HIPCHK(ctx, hipEventElapsedTime(&evo_ms, ev2.e0, ev2.e1));
+    const double evolve_ms = evo_ms;
+    std::vector<WvRay> rays(nl);
+    std::vector<double> uq(nl * (size_t)len);
+    std::vector<long long> sq(nl, 0);
+    HIPCHK(ctx, hipMemcpy(rays.data(), d_ray.p, nl * sizeof(WvRay), hipMemcpyDeviceToHost));
+    HIPCHK(ctx, hipMemcpy(uq.data(), d_u.p, uq.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(ctx, hipMemcpy(sq.data(), d_st.p, sq.size() * sizeof(long long), hipMemcpyDeviceToHost));
 
     // 4. outputs: wvfrm_init.dat and wvfrm_out.dat rows (main.infraga.sph.cpp:
     // 1380-1386, 1440-1461), the arrival summary
-    std::vector<int> q_of(nl, -1);
-    for (int q = 0; q < nq; ++q) q_of[(size_t)wv_ray[q]] = q;
     long long ray_steps = 0, wv_steps = 0;
     for (int i = 0; i < n; ++i) {
         ray_steps += stp[(size_t)i];
-        const RayOut &o = ro[(size_t)i];
-        const int q = q_of[(size_t)i];
+        const WvRay &o = rays[(size_t)i];
         const bool ok = cnt[(size_t)i] == slots;
         status[i] = !o.started ? -1 : (ok ? 1 : 0);
         if (ok) arrivals[i] = all[(size_t)i * slots + (slots - 1)];
@@ -2927,27 +2832,23 @@ int ga_wnl_wvfrm(ga_ctx *ctx, int atmo_id, const ga_prop_params *p_in, const ga_
             arrivals[i].az = launches[2 * i + 1];
             arrivals[i].time = o.tt;
         }
-        if (evolve_steps) evolve_steps[i] = q >= 0 ? sq[(size_t)q] : 0;
+        if (evolve_steps) evolve_steps[i] = o.started ? sq[(size_t)i] : 0;
         double *ti = t_init + (size_t)i * len, *pi_ = p_init + (size_t)i * len;
         double *to = t_out + (size_t)i * len, *po = p_out + (size_t)i * len;
-        if (q < 0) {
+        if (!o.started) {
             for (int m = 0; m < len; ++m) ti[m] = pi_[m] = to[m] = po[m] = std::nan("");
             continue;
         }
-        wv_steps += sq[(size_t)q];
-        const WvRef &r = o.ref;
-        const WvRowH &y = o.last;
-        const double nu = wv_nu(y);
-        const double sc = std::sqrt(std::fabs((y.rho * std::pow(y.c, 3) * nu) / (r.rho0 * std::pow(r.c0, 3) * r.nu0) *
-                                              (r.cg0 * r.D0) / (y.cg * y.D)));
-        const double *u = uq.data() + (size_t)q * len;
+        wv_steps += sq[(size_t)i];
+        const double *u = uq.data() + (size_t)i * len;
         for (int m = 0; m < len; ++m) {
-            ti[m] = r.tref + t_src[m];
+            ti[m] = o.tref + t_src[m];
             pi_[m] = p_src[m];
             to[m] = o.tt + t_src[m];
-            po[m] = u[m] * p0 * sc;
+            po[m] = u[m] * p0 * o.scale;
         }
     }
+    trace_ms += prep_ms;
     if (stats) {
         stats->ray_steps = ray_steps;
         stats->wv_steps = wv_steps;

@@ -32,6 +32,7 @@
 #pragma once
 
 #include "ga_device.hpp"
+#include "../../include/infraga_gpu.h"
 
 namespace ga {
 
@@ -43,6 +44,19 @@ constexpr int WV_SPT = WV_MAX_LEN / WV_T;           // samples per thread
 constexpr int WV_ROW = 8;                            // s, pos[3], D, beta, beta_c, pad
 constexpr int WV_F_S = 0, WV_F_POS = 1, WV_F_D = 4, WV_F_BETA = 5, WV_F_BETAC = 6;
 
+// Per-waveform set-up results of wv_prep_kernel.
+struct WvRay {
+    double c0, rho0, nu0, cg0, D0;   // at the reference point (run_wnl_wvfrm, main.infraga.sph.cpp:1370-1379)
+    double tref;                     // travel time to the reference row
+    double tt;                       // travel time of the whole ray (rows 1..K-1 of every leg)
+    double scale;                    // wvfrm_out.dat factor at row K of the last leg (:1440-1461)
+    int32_t started;                 // 1: the reference point was reached
+    int32_t pad;
+};
+
+constexpr int WV_PREP_T = 256;
+constexpr int WV_MAX_SLOTS = 256;    // legs per waveform (bounces + 1)
+
 struct WvArgs {
     Medium M;
     int len, lg, n_wv;
@@ -53,10 +67,11 @@ struct WvArgs {
     const double *filt;          // [len/2 + 1][2]: filter1, filter2
     const double *u0;            // [len] initial scaled waveform
     double *u;                   // [n_wv][len] evolved scaled waveforms
-    const int32_t *wv_leg;       // [n_wv + 1]: legs of waveform w are wv_leg[w] .. wv_leg[w+1]-1
-    const int64_t *leg_row;      // [n_legs + 1]: rows of leg j are leg_row[j] .. leg_row[j+1]-1
+    int slots;                   // legs per waveform (bounces + 1)
+    const int64_t *leg_first;    // [n_wv][slots]: first row of each evolved leg
+    const int32_t *leg_n;        // [n_wv][slots]: its rows (< 2: nothing to evolve)
     const double *rows;          // [n_rows][WV_ROW]
-    const double *c0;            // [n_wv]: c at the reference point
+    const struct WvRay *ray;     // [n_wv]: reference-point quantities (c0)
     long long *steps;            // [n_wv]: evolution steps
 };
 
@@ -147,7 +162,7 @@ __global__ void __launch_bounds__(WV_T) wv_evolve_kernel(WvArgs A)
     const double dt = t[1] - t[0];
     const double fs = 1.0 / dt;
     const double ilen = 1.0 / len;                // exact: len is a power of two
-    const double c0 = A.c0[w];
+    const double c0 = A.ray[w].c0;
     const Medium &M = A.M;
 
     for (int m = tid; m < len; m += WV_T) re[m] = A.u0[m];
@@ -159,9 +174,9 @@ __global__ void __launch_bounds__(WV_T) wv_evolve_kernel(WvArgs A)
     long long nstep = 0;
     int curz = 0;                                 // spline cursor of the absorption point
 
-    for (int leg = A.wv_leg[w]; leg < A.wv_leg[w + 1]; ++leg) {
-        const double *R = A.rows + (size_t)A.leg_row[leg] * WV_ROW;
-        const int n = (int)(A.leg_row[leg + 1] - A.leg_row[leg]);
+    for (int leg = 0; leg < A.slots; ++leg) {
+        const double *R = A.rows + (size_t)A.leg_first[(size_t)w * A.slots + leg] * WV_ROW;
+        const int n = A.leg_n[(size_t)w * A.slots + leg];
         if (n < 2) continue;
         const double s_tot = R[(size_t)(n - 1) * WV_ROW];
 
@@ -375,6 +390,157 @@ __global__ void __launch_bounds__(WV_T) wv_evolve_kernel(WvArgs A)
     }
     for (int m = tid; m < len; m += WV_T) A.u[(size_t)w * len + m] = re[m];
     if (tid == 0) A.steps[w] = nstep;
+}
+
+
+// Set-up of every waveform from its recorded rows (records in ray order,
+// two per row: PT_WV_STATE then PT_WV_MEDIUM), one workgroup per ray:
+// thread 0 walks the legs as run_wnl_wvfrm does (main.infraga.sph.cpp:
+// 1336-1401): the travel time of rows 1..K-1 of each leg, the reference
+// point where the ray length (midpoint radius and latitude; 3D chord) first
+// reaches ref, and the rows each leg evolves over (from the reference row
+// on in its leg, whole legs after it, rows up to K-1), whose arc length,
+// position and D it writes; then all threads fill beta and the caustic beta
+// of those rows (wnl_wvfrm's set-up, geoac.eqset.sph.strat.cpp:903-941).
+template <bool SPH>
+__global__ void __launch_bounds__(WV_PREP_T) wv_prep_kernel(const ga_path_pt *rec, const int64_t *off, int n_wv,
+                                                           int slots, double ref, double p0, double *rows,
+                                                           int64_t *leg_first, int32_t *leg_n, WvRay *ray)
+{
+    __shared__ int64_t s_first[WV_MAX_SLOTS];
+    __shared__ int32_t s_n[WV_MAX_SLOTS];
+    __shared__ WvRay s_ray;
+    __shared__ double s_beta0;
+    const int w = blockIdx.x, tid = threadIdx.x;
+    if (w >= n_wv) return;
+    const ga_path_pt *P = rec + off[w];
+    const int64_t nr = (off[w + 1] - off[w]) / 2;       // rows of this ray
+    const int64_t r0 = off[w] / 2;                      // its first row in the rows array
+    for (int j = tid; j < slots; j += WV_PREP_T) { s_first[j] = 0; s_n[j] = 0; }
+    __syncthreads();
+    if (tid == 0) {
+        WvRay R;
+        R.c0 = R.rho0 = R.nu0 = R.cg0 = R.D0 = R.tref = R.tt = 0.0;
+        R.scale = 0.0;
+        R.started = 0;
+        R.pad = 0;
+        auto y = [&](int64_t r, int k) -> double {
+            const ga_path_pt &a = P[2 * r];
+            return k == 0 ? a.pos0 : k == 1 ? a.pos1 : k == 2 ? a.z : k == 3 ? a.tl : k == 4 ? a.absorption : a.time;
+        };
+        auto nu = [&](int64_t r) { return sqrt(y(r, 3) * y(r, 3) + y(r, 4) * y(r, 4) + y(r, 5) * y(r, 5)); };
+        double ray_length = 0.0;
+        int64_t a = 0;
+        while (a < nr) {
+            const int leg = P[2 * a].leg;
+            int64_t b = a;
+            while (b + 1 < nr && P[2 * (b + 1)].leg == leg) ++b;
+            const int64_t k = b - a;                     // row K of the leg
+            for (int64_t m = 1; m < k; ++m) R.tt += P[2 * (a + m) + 1].absorption;
+            int64_t k1 = -1;
+            if (!R.started) {
+                for (int64_t kr = 0; kr < k; ++kr) {
+                    const int64_t p = a + kr, q = p + 1;
+                    double dl;
+                    if (SPH) {
+                        const double dr = y(q, 0) - y(p, 0), rr = y(p, 0) + dr / 2.0;
+                        const double dth = y(q, 1) - y(p, 1), th = y(p, 1) + dth / 2.0;
+                        const double dph = y(q, 2) - y(p, 2);
+                        dl = sqrt(dr * dr + (rr * dth) * (rr * dth) + (rr * cos(th) * dph) * (rr * cos(th) * dph));
+                    } else {
+                        const double dx = y(q, 0) - y(p, 0), dy = y(q, 1) - y(p, 1), dz = y(q, 2) - y(p, 2);
+                        dl = sqrt(dx * dx + dy * dy + dz * dz);
+                    }
+                    ray_length += dl;
+                    if (ray_length >= ref) {
+                        const ga_path_pt &md = P[2 * q + 1];
+                        R.c0 = md.pos1;
+                        R.rho0 = md.z;
+                        R.nu0 = nu(q);
+                        R.cg0 = md.tl;
+                        R.D0 = md.pos0;
+                        for (int64_t m = 1; m <= kr + 1; ++m) R.tref += P[2 * (a + m) + 1].absorption;
+                        R.started = 1;
+                        k1 = kr + 1;
+                        break;
+                    }
+                }
+            } else {
+                k1 = 0;
+            }
+            if (k1 >= 0 && k - k1 >= 2 && leg >= 0 && leg < slots) {
+                s_first[leg] = r0 + a + k1;
+                s_n[leg] = (int32_t)(k - k1);
+                double sarc = 0.0;
+                for (int64_t m = a + k1; m < a + k; ++m) {
+                    double *o = rows + (size_t)(r0 + m) * WV_ROW;
+                    o[WV_F_S] = sarc;
+                    o[WV_F_POS] = y(m, 0);
+                    o[WV_F_POS + 1] = y(m, 1);
+                    o[WV_F_POS + 2] = y(m, 2);
+                    o[WV_F_D] = P[2 * m + 1].pos0;
+                    if (m + 1 < a + k) {
+                        const double d0 = y(m + 1, 0) - y(m, 0), d1 = y(m + 1, 1) - y(m, 1), d2 = y(m + 1, 2) - y(m, 2);
+                        if (SPH) {
+                            const double e1 = y(m, 0) * d1, e2 = y(m, 0) * cos(y(m, 1)) * d2;
+                            sarc += sqrt(d0 * d0 + e1 * e1 + e2 * e2);
+                        } else {
+                            sarc += sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+                        }
+                    }
+                }
+            }
+            a = b + 1;
+        }
+        if (R.started && nr > 0) {
+            const int64_t kl = nr - 1;                   // row K of the last leg
+            const ga_path_pt &md = P[2 * kl + 1];
+            const double c = md.pos1, rho = md.z, nuk = nu(kl);
+            R.scale = sqrt(fabs((rho * (c * c * c) * nuk) / (R.rho0 * (R.c0 * R.c0 * R.c0) * R.nu0) * (R.cg0 * R.D0) /
+                                (md.tl * md.pos0)));
+            const double c_src = P[1].pos1;
+            s_beta0 = 1.2 * p0 / ((R.rho0 * 1.0e3) * ((R.c0 * 1.0e3) * (R.c0 * 1.0e3))) * (R.nu0 * R.c0) / (R.cg0 * c_src);
+        }
+        s_ray = R;
+        ray[w] = R;
+    }
+    __syncthreads();
+    if (!s_ray.started) {
+        for (int j = tid; j < slots; j += WV_PREP_T) { leg_first[(size_t)w * slots + j] = 0; leg_n[(size_t)w * slots + j] = 0; }
+        return;
+    }
+    const WvRay R = s_ray;
+    const double beta0 = s_beta0;
+    const double num = R.D0 * R.rho0 * (R.cg0 * R.cg0 * R.cg0), den0 = R.c0 * (R.nu0 * R.nu0 * R.nu0);
+    for (int j = 0; j < slots; ++j) {
+        const int32_t nn = s_n[j];
+        const int64_t f = s_first[j];
+        for (int q = tid; q < nn; q += WV_PREP_T) {
+            const int64_t m = f + q;                     // global row
+            const ga_path_pt &md = rec[2 * m + 1];
+            const ga_path_pt &st = rec[2 * m];
+            const double c = md.pos1, rho = md.z;
+            const double nuq = sqrt(st.tl * st.tl + st.absorption * st.absorption + st.time * st.time);
+            double *o = rows + (size_t)m * WV_ROW;
+            const double Dq = o[WV_F_D];
+            o[WV_F_BETA] = beta0 * sqrt(fabs(num / (Dq * rho * (c * c * c)) * (c * (nuq * nuq * nuq)) / den0));
+            if (q > 0) {
+                const double *op = o - WV_ROW;
+                o[WV_F_BETAC] = beta0 * sqrt(fabs(num / ((Dq - op[WV_F_D]) / (o[WV_F_S] - op[WV_F_S]) * rho * (c * c * c)) *
+                                                  (c * (nuq * nuq * nuq)) / den0));
+            }
+            o[WV_ROW - 1] = 0.0;
+        }
+    }
+    __syncthreads();
+    for (int j = tid; j < slots; j += WV_PREP_T) {
+        leg_first[(size_t)w * slots + j] = s_first[j];
+        leg_n[(size_t)w * slots + j] = s_n[j];
+        if (s_n[j] >= 2) {
+            double *o = rows + (size_t)s_first[j] * WV_ROW;
+            o[WV_F_BETAC] = o[WV_ROW + WV_F_BETAC];
+        }
+    }
 }
 
 }  // namespace ga
