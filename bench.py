@@ -444,9 +444,12 @@ def wvfrm_leg(eng, with_cpu, cores=1):
     eng.release_atmo(aid)
     st = g["stats"]
     lg = int(round(math.log2(WV_LEN)))
-    # LDS bytes per evolution step: three radix-2 transforms of len complex
-    # doubles, each stage reading and writing every element (32 len bytes)
-    lds_step = 3 * 32 * WV_LEN * lg
+    # LDS bytes per evolution step: three transforms of len complex doubles
+    # in passes of three fused radix-2 stages, each pass reading and writing
+    # every element (32 len bytes), plus the step's other LDS passes
+    # (u and its neighbours, the two scatters, bin reads, the window pass:
+    # 112 len bytes)
+    lds_step = 3 * 32 * WV_LEN * ((lg + 2) // 3) + 112 * WV_LEN
     ach = lds_step * st.wv_steps / (st.evolve_ms * 1e-3) / 1e9
     peak = 256 * 128 * 2.4e9 / 1e9                   # 256 CUs x 128 B/clk x 2.4 GHz
     out = {"workload": f"{len(L)} waveforms of {WV_LEN} samples: ToyAtmo, source 30N 100W, inclination 4.1314876, "
@@ -456,7 +459,8 @@ def wvfrm_leg(eng, with_cpu, cores=1):
            "waveforms_per_s": len(L) / wall, "wv_steps_per_s": st.wv_steps / (st.evolve_ms * 1e-3),
            "roofline": {"kernel": "wv_evolve_kernel", "bound": "lds", "achieved": ach, "peak": peak, "unit": "GB/s",
                         "frac": ach / peak, "traffic": None,
-                        "note": f"{lds_step} LDS bytes per step (3 x 13 radix-2 stages x 32 B x len)"}}
+                        "note": f"{lds_step} LDS bytes per step (3 transforms x {(lg + 2) // 3} passes x 32 B x len "
+                                f"+ 112 B x len); peak 128 B/clk per CU"}}
     if with_cpu:
         n = max(1, min(cores, len(L)))
         pick = list(range(0, len(L), max(1, len(L) // n)))[:n]

@@ -19,14 +19,14 @@
 // MI355X layout: one workgroup of WV_T threads per waveform, the transform
 // in LDS (two arrays of len doubles, 128 KB at the largest length 8192),
 // radix-2 decimation in time over bit-reversed input with the twiddles of
-// a host-built table.  Each thread keeps the spectrum U of its bins
+// a host-built table, three stages fused per LDS round trip.  Each thread keeps the spectrum U of its bins
 // (b = tid + j WV_T <= len / 2) in registers across the steps; u lives in
 // LDS between steps.  A step is three complex transforms of length len:
 // Z1 = u^2 + i u d(u^2)/dt gives A and B by the even/odd split, Z2 =
 // (d(u^2)/dt)^2 gives C, and the Hermitian U the new u.  The step scalars
 // (beta, ds, the ray position) are computed by every thread from the leg's
 // rows (uniform across the workgroup); the height-dependent factors of the
-// absorption by wave 0 into LDS, the frequency-dependent ones per bin.
+// absorption by the last wave into LDS, the frequency-dependent ones per bin.
 // The rows of a leg are the linear splines of the reference: arc length s,
 // ray position (3), D, beta, beta_caustic (WV_ROW doubles per row).
 #pragma once
@@ -98,33 +98,73 @@ GA_HD double wv_beta(const double *R, int k, double x)
     return fabs(wv_lin(R, k, WV_F_D, x)) >= 0.5 ? wv_lin(R, k, WV_F_BETA, x) : wv_lin(R, k, WV_F_BETAC, x);
 }
 
+// LDS slot of element i: one pad double after every 128, so that the
+// bit-reversed scatter of the transforms' inputs (consecutive lanes writing
+// elements len/64 apart) and the fused passes' strided accesses spread over
+// the banks instead of landing on one.
+constexpr int WV_LDS = WV_MAX_LEN + WV_MAX_LEN / 128;
+__device__ inline int wv_ix(int i) { return i + (i >> 7); }
+
 __device__ inline int wv_rev(int i, int lg) { return (int)(__brev((unsigned int)i) >> (32 - lg)); }
 
-// In-place radix-2 DIT transform over LDS: input in bit-reversed order,
-// output in natural order; inverse: the conjugate twiddles (FFTW's sign
-// convention, unnormalised).  Ends with a barrier.
+// Stages st .. st + RL - 1 of the in-place radix-2 decimation-in-time
+// transform fused in registers: each thread loads the 2^RL elements
+// i0 + m h (h = 2^(st-1) the first fused stage's half size) that those
+// stages combine, applies their butterflies with the stage-by-stage
+// twiddles (element pair (m, m + 2^q) of stage st + q takes W^k, k = the
+// lower bits k0 + (m mod 2^q) h, from the table at k << (lg - st - q)) and
+// stores them back: the same arithmetic as one LDS round trip per stage,
+// with one per RL stages.  Ends with a barrier.
+template <int RL>
+__device__ void wv_fft_pass(double *re, double *im, int len, int lg, int st, const double *tw, bool inv, int tid)
+{
+    constexpr int R = 1 << RL;
+    const int h = 1 << (st - 1);
+    const int groups = len >> RL;
+    for (int g = tid; g < groups; g += WV_T) {
+        const int k0 = g & (h - 1);
+        const int i0 = ((g >> (st - 1)) << (st - 1 + RL)) + k0;
+        double xr[R], xi[R];
+#pragma unroll
+        for (int m = 0; m < R; ++m) {
+            xr[m] = re[wv_ix(i0 + m * h)];
+            xi[m] = im[wv_ix(i0 + m * h)];
+        }
+#pragma unroll
+        for (int q = 0; q < RL; ++q) {
+            const int sh = lg - (st + q);
+#pragma unroll
+            for (int m = 0; m < R; ++m) {
+                if (m & (1 << q)) continue;
+                const int mb = m + (1 << q);
+                const int k = k0 + (m & ((1 << q) - 1)) * h;
+                const double wr = tw[2 * (k << sh)];
+                const double wi = inv ? -tw[2 * (k << sh) + 1] : tw[2 * (k << sh) + 1];
+                const double tr = xr[mb] * wr - xi[mb] * wi, ti = xr[mb] * wi + xi[mb] * wr;
+                xr[mb] = xr[m] - tr;
+                xi[mb] = xi[m] - ti;
+                xr[m] += tr;
+                xi[m] += ti;
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < R; ++m) {
+            re[wv_ix(i0 + m * h)] = xr[m];
+            im[wv_ix(i0 + m * h)] = xi[m];
+        }
+    }
+    __syncthreads();
+}
+
+// In-place transform over LDS: input in bit-reversed order, output in
+// natural order; inverse: the conjugate twiddles (FFTW's sign convention,
+// unnormalised).  Passes of three fused stages, then one of two or one.
 __device__ void wv_fft(double *re, double *im, int len, int lg, const double *tw, bool inv, int tid)
 {
-    const int nb = len >> 1;
-    for (int st = 1; st <= lg; ++st) {
-        const int h = 1 << (st - 1);
-        const int sh = lg - st;
-        for (int b = tid; b < nb; b += WV_T) {
-            const int k = b & (h - 1);
-            const int i = ((b >> (st - 1)) << st) | k;
-            const int j = i + h;
-            const double wr = tw[2 * (k << sh)];
-            const double wi = inv ? -tw[2 * (k << sh) + 1] : tw[2 * (k << sh) + 1];
-            const double xr = re[j], xi = im[j];
-            const double tr = xr * wr - xi * wi, ti = xr * wi + xi * wr;
-            const double ar = re[i], ai = im[i];
-            re[j] = ar - tr;
-            im[j] = ai - ti;
-            re[i] = ar + tr;
-            im[i] = ai + ti;
-        }
-        __syncthreads();
-    }
+    int st = 1;
+    for (; st + 2 <= lg; st += 3) wv_fft_pass<3>(re, im, len, lg, st, tw, inv, tid);
+    if (st + 1 <= lg) wv_fft_pass<2>(re, im, len, lg, st, tw, inv, tid);
+    else if (st <= lg) wv_fft_pass<1>(re, im, len, lg, st, tw, inv, tid);
 }
 
 // Largest v over the workgroup (red: WV_T / 64 doubles of LDS).
@@ -151,7 +191,7 @@ struct WvPoint {
 template <bool SPH>
 __global__ void __launch_bounds__(WV_T) wv_evolve_kernel(WvArgs A)
 {
-    __shared__ double re[WV_MAX_LEN], im[WV_MAX_LEN];
+    __shared__ double re[WV_LDS], im[WV_LDS];
     __shared__ double red[WV_T / 64];
     __shared__ WvPoint pt;
     const int tid0 = threadIdx.x, w = blockIdx.x;
@@ -165,7 +205,7 @@ __global__ void __launch_bounds__(WV_T) wv_evolve_kernel(WvArgs A)
     const double c0 = A.ray[w].c0;
     const Medium &M = A.M;
 
-    for (int m = tid; m < len; m += WV_T) re[m] = A.u0[m];
+    for (int m = tid; m < len; m += WV_T) re[wv_ix(m)] = A.u0[m];
     __syncthreads();
 
     double Ur[WV_BPT], Ui[WV_BPT];
@@ -186,7 +226,7 @@ __global__ void __launch_bounds__(WV_T) wv_evolve_kernel(WvArgs A)
 #pragma unroll
             for (int j = 0; j < WV_SPT; ++j) {
                 const int m = tid + j * WV_T;
-                us[j] = m < len ? re[m] : 0.0;
+                us[j] = m < len ? re[wv_ix(m)] : 0.0;
             }
             __syncthreads();
 #pragma unroll
@@ -194,8 +234,8 @@ __global__ void __launch_bounds__(WV_T) wv_evolve_kernel(WvArgs A)
                 const int m = tid + j * WV_T;
                 if (m < len) {
                     const int r = wv_rev(m, lg);
-                    re[r] = us[j];
-                    im[r] = 0.0;
+                    re[wv_ix(r)] = us[j];
+                    im[wv_ix(r)] = 0.0;
                 }
             }
             __syncthreads();
@@ -205,8 +245,8 @@ __global__ void __launch_bounds__(WV_T) wv_evolve_kernel(WvArgs A)
             for (int j = 0; j < WV_BPT; ++j) {
                 const int b = tid + j * WV_T;
                 if (b <= half) {
-                    Ur[j] = re[b];
-                    Ui[j] = im[b];
+                    Ur[j] = re[wv_ix(b)];
+                    Ui[j] = im[wv_ix(b)];
                     fm = fmax(fm, (fs * b * ilen) * sqrt(Ur[j] * Ur[j] + Ui[j] * Ui[j]));
                 }
             }
@@ -215,7 +255,7 @@ __global__ void __launch_bounds__(WV_T) wv_evolve_kernel(WvArgs A)
 #pragma unroll
             for (int j = 0; j < WV_SPT; ++j) {
                 const int m = tid + j * WV_T;
-                if (m < len) re[m] = us[j];
+                if (m < len) re[wv_ix(m)] = us[j];
             }
             double Uf_max = wv_block_max(fm, red, tid);   // its barriers publish u
             // step loop
@@ -242,8 +282,9 @@ __global__ void __launch_bounds__(WV_T) wv_evolve_kernel(WvArgs A)
                     const double d1 = wv_lin(R, wv_seek(R, n, ks, s + ds), WV_F_D, s + ds);
                     caustic = SPH ? (d0 * d1 < 0.0) : (d0 * d1 <= 0.0);
                 }
-                // absorption point factors at the ray position of s (wave 0)
-                if (tid < 64) {
+                // absorption point factors at the ray position of s (the last
+                // wave: wave 0 holds the extra Nyquist bin)
+                if (tid >= WV_T - 64) {
                     const double pa = wv_lin(R, ks, WV_F_POS, s), pb = wv_lin(R, ks, WV_F_POS + 1, s);
                     const double pc = wv_lin(R, ks, WV_F_POS + 2, s);
                     (void)pb;
@@ -252,7 +293,7 @@ __global__ void __launch_bounds__(WV_T) wv_evolve_kernel(WvArgs A)
                     const double cpt = herm_f(sp, 1), rho = herm_f(sp, 7);
                     const SbGas g = sb_gas(cpt, rho, 0.0);
                     const SbPoint P = sb_point<SPH>(z, g, M.sbc);
-                    if (tid == 0) {
+                    if (tid == WV_T - 64) {
                         pt.g = g;
                         pt.P = P;
                         pt.cpt = cpt;
@@ -267,20 +308,20 @@ __global__ void __launch_bounds__(WV_T) wv_evolve_kernel(WvArgs A)
                     const int m = tid + j * WV_T;
                     z1r[j] = z2[j] = 0.0;
                     if (m < len) {
-                        const double um = re[m];
+                        const double um = re[wv_ix(m)];
                         const double usq = um * um;
                         double du;
                         if (m == 0) {
-                            const double u1 = re[1];
+                            const double u1 = re[wv_ix(1)];
                             du = (u1 * u1 - usq) / (t[1] - t[0]);
                         } else if (m == len - 1) {
-                            const double ua = re[m - 1];
+                            const double ua = re[wv_ix(m - 1)];
                             du = (usq - ua * ua) / (t[m] - t[m - 1]);
                         } else {
-                            const double ua = re[m - 1], ub = re[m + 1];
+                            const double ua = re[wv_ix(m - 1)], ub = re[wv_ix(m + 1)];
                             du = (ub * ub - ua * ua) / (t[m + 1] - t[m - 1]);
                         }
-                        im[wv_rev(m, lg)] = um * du;
+                        im[wv_ix(wv_rev(m, lg))] = um * du;
                         z1r[j] = usq;
                         z2[j] = du * du;
                     }
@@ -289,7 +330,7 @@ __global__ void __launch_bounds__(WV_T) wv_evolve_kernel(WvArgs A)
 #pragma unroll
                 for (int j = 0; j < WV_SPT; ++j) {
                     const int m = tid + j * WV_T;
-                    if (m < len) re[wv_rev(m, lg)] = z1r[j];
+                    if (m < len) re[wv_ix(wv_rev(m, lg))] = z1r[j];
                 }
                 __syncthreads();
                 wv_fft(re, im, len, lg, A.tw, false, tid);
@@ -314,7 +355,7 @@ __global__ void __launch_bounds__(WV_T) wv_evolve_kernel(WvArgs A)
                                 Ui[j] = tmp;
                             }
                             const int nb = (len - b) & (len - 1);
-                            const double za = re[b], zb = im[b], zc = re[nb], zd = im[nb];
+                            const double za = re[wv_ix(b)], zb = im[wv_ix(b)], zc = re[wv_ix(nb)], zd = im[wv_ix(nb)];
                             const double Ar = 0.5 * (za + zc), Ai = 0.5 * (zb - zd);
                             const double Br = 0.5 * (zb + zd), Bi = 0.5 * (zc - za);
                             Pr[j] = h1 * Ai + h2 * Bi;
@@ -335,8 +376,8 @@ __global__ void __launch_bounds__(WV_T) wv_evolve_kernel(WvArgs A)
                     const int m = tid + j * WV_T;
                     if (m < len) {
                         const int r = wv_rev(m, lg);
-                        re[r] = z2[j];
-                        im[r] = 0.0;
+                        re[wv_ix(r)] = z2[j];
+                        im[wv_ix(r)] = 0.0;
                     }
                 }
                 __syncthreads();
@@ -349,7 +390,7 @@ __global__ void __launch_bounds__(WV_T) wv_evolve_kernel(WvArgs A)
                     if (b <= half) {
                         const double freq = fs * b * ilen;
                         const double f1 = A.filt[2 * b];
-                        const double Cr = re[b], Ci = im[b];
+                        const double Cr = re[wv_ix(b)], Ci = im[wv_ix(b)];
                         Ur[j] -= PI * freq * (Pr[j] + h3 * Ci) * f1;
                         Ui[j] += PI * freq * (Pi[j] + h3 * Cr) * f1;
                         Ur[j] *= gf[j];
@@ -366,12 +407,12 @@ __global__ void __launch_bounds__(WV_T) wv_evolve_kernel(WvArgs A)
                     if (b <= half) {
                         const int r = wv_rev(b, lg);
                         const bool edge = (b == 0) || (b == half);
-                        re[r] = Ur[j];
-                        im[r] = edge ? 0.0 : Ui[j];
+                        re[wv_ix(r)] = Ur[j];
+                        im[wv_ix(r)] = edge ? 0.0 : Ui[j];
                         if (!edge) {
                             const int r2 = wv_rev(len - b, lg);
-                            re[r2] = Ur[j];
-                            im[r2] = -Ui[j];
+                            re[wv_ix(r2)] = Ur[j];
+                            im[wv_ix(r2)] = -Ui[j];
                         }
                     }
                 }
@@ -380,7 +421,7 @@ __global__ void __launch_bounds__(WV_T) wv_evolve_kernel(WvArgs A)
 #pragma unroll
                 for (int j = 0; j < WV_SPT; ++j) {
                     const int m = tid + j * WV_T;
-                    if (m < len) re[m] = re[m] * A.tukey[m] * ilen;
+                    if (m < len) re[wv_ix(m)] = re[wv_ix(m)] * A.tukey[m] * ilen;
                 }
                 Uf_max = wv_block_max(fm, red, tid);   // its barriers publish the new u
                 s += ds;
@@ -388,7 +429,7 @@ __global__ void __launch_bounds__(WV_T) wv_evolve_kernel(WvArgs A)
             }
         }
     }
-    for (int m = tid; m < len; m += WV_T) A.u[(size_t)w * len + m] = re[m];
+    for (int m = tid; m < len; m += WV_T) A.u[(size_t)w * len + m] = re[wv_ix(m)];
     if (tid == 0) A.steps[w] = nstep;
 }
 
