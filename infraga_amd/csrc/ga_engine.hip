@@ -2650,8 +2650,10 @@ int ga_wnl_wvfrm(ga_ctx *ctx, int atmo_id, const ga_prop_params *p_in, const ga_
     if (n < 0) return fail(ctx, GA_E_ARG, "negative ray count");
     if (n > 0 && (!launches || !t_src || !p_src || !t_init || !p_init || !t_out || !p_out || !arrivals || !status))
         return fail(ctx, GA_E_ARG, "null argument");
-    if (p_in->geometry != GA_GEOM_SPH && p_in->geometry != GA_GEOM_3D)
-        return fail(ctx, GA_E_ARG, "waveforms run in the stratified geometries (GA_GEOM_SPH, GA_GEOM_3D)");
+    const int geom = p_in->geometry;
+    if (geom != GA_GEOM_SPH && geom != GA_GEOM_3D && geom != GA_GEOM_SPH_RNGDEP && geom != GA_GEOM_3D_RNGDEP)
+        return fail(ctx, GA_E_ARG, "waveforms run in the spherical and 3D geometries (not GA_GEOM_2D)");
+    const bool rdm = geom == GA_GEOM_SPH_RNGDEP || geom == GA_GEOM_3D_RNGDEP;
     if (p_in->bounces < 0) return fail(ctx, GA_E_ARG, "bounces must be >= 0");
     const int len = w->len;
     int lg = 0;
@@ -2662,15 +2664,15 @@ int ga_wnl_wvfrm(ga_ctx *ctx, int atmo_id, const ga_prop_params *p_in, const ga_
     if (!(w->ds > 0.0) || !std::isfinite(w->ds)) return fail(ctx, GA_E_ARG, "wvfrm_ds must be positive");
     if (!std::isfinite(w->ref)) return fail(ctx, GA_E_ARG, "wvfrm_ref must be finite");
     auto it = ctx->atmos.find(atmo_id);
-    if (it == ctx->atmos.end())
-        return fail(ctx, GA_E_ATMO, "unknown atmosphere id %d (waveforms run over stratified profiles)", atmo_id);
+    if (rdm ? ctx->rds.find(atmo_id) == ctx->rds.end() : it == ctx->atmos.end())
+        return fail(ctx, GA_E_ATMO, "unknown %s atmosphere id %d", rdm ? "range-dependent" : "stratified", atmo_id);
     if (n == 0) return GA_OK;
     if (!(t_src[1] > t_src[0])) return fail(ctx, GA_E_ARG, "source waveform times must increase");
     double p0 = 0.0;
     for (int i = 0; i < len; ++i) p0 = std::max(p0, std::fabs(p_src[i]));
     if (!(p0 > 0.0) || !std::isfinite(p0)) return fail(ctx, GA_E_ARG, "source waveform peak must be positive and finite");
     HIPCHK(ctx, hipSetDevice(ctx->device));
-    const bool sph = p_in->geometry == GA_GEOM_SPH;
+    const bool sph = geom == GA_GEOM_SPH || geom == GA_GEOM_SPH_RNGDEP;
     // run_wnl_wvfrm traces with the auxiliary equations and stops a ray
     // only at a break or after the last bounce (no turning-height or time
     // limits)
@@ -2777,12 +2779,23 @@ int ga_wnl_wvfrm(ga_ctx *ctx, int atmo_id, const ga_prop_params *p_in, const ga_
     if (!ev2.ok) return fail(ctx, GA_E_HIP, "hipEventCreate failed");
     WvArgs A;
     std::memset(&A, 0, sizeof(A));
-    {
+    int med = WV_STRAT;
+    if (rdm) {
+        RdFanArgs RA;
+        int rc = rd_setup(ctx, ctx->fb, atmo_id, &p, RA, st);
+        if (rc != GA_OK) return rc;
+        A.RM = RA.M;
+        med = sph ? WV_RD_SPH : WV_RD_CART;
+        // the range-dependent sets floor the step at 1e-6 km; a stratified
+        // profile over terrain runs the stratified sets' wnl_wvfrm
+        A.ds_floor = RA.M.strat ? (sph ? 1.0e-7 : 1.0e-5) : 1.0e-6;
+    } else {
         Limits L;
         int rc = GA_OK;
         if (!build_medium(ctx, it->second, &p, A.M, L, rc)) return rc;
+        A.M.abs = nullptr;                        // per-bin absorption is evaluated directly
+        A.ds_floor = sph ? 1.0e-7 : 1.0e-5;
     }
-    A.M.abs = nullptr;                            // per-bin absorption is evaluated directly
     A.len = len;
     A.lg = lg;
     A.n_wv = n;
@@ -2800,8 +2813,14 @@ int ga_wnl_wvfrm(ga_ctx *ctx, int atmo_id, const ga_prop_params *p_in, const ga_
     A.ray = d_ray.p;
     A.steps = d_st.p;
     HIPCHK(ctx, hipEventRecord(ev2.e0, st));
-    if (sph) hipLaunchKernelGGL(wv_evolve_kernel<true>, dim3(n), dim3(WV_T), 0, st, A);
-    else hipLaunchKernelGGL(wv_evolve_kernel<false>, dim3(n), dim3(WV_T), 0, st, A);
+    if (med == WV_STRAT) {
+        if (sph) hipLaunchKernelGGL((wv_evolve_kernel<true, WV_STRAT>), dim3(n), dim3(WV_T), 0, st, A);
+        else hipLaunchKernelGGL((wv_evolve_kernel<false, WV_STRAT>), dim3(n), dim3(WV_T), 0, st, A);
+    } else if (med == WV_RD_SPH) {
+        hipLaunchKernelGGL((wv_evolve_kernel<true, WV_RD_SPH>), dim3(n), dim3(WV_T), 0, st, A);
+    } else {
+        hipLaunchKernelGGL((wv_evolve_kernel<false, WV_RD_CART>), dim3(n), dim3(WV_T), 0, st, A);
+    }
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipEventRecord(ev2.e1, st));
     HIPCHK(ctx, hipStreamSynchronize(st));

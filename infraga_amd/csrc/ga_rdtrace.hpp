@@ -1161,6 +1161,38 @@ GA_HD void rd3_path_xyz(const RdMed &M, const double *y, ga_path_pt &R)
     R.z = y[2] > zg ? y[2] : zg;
 }
 
+// The two waveform-row records of row y (PATH_WVROWS; ga_trace.hpp): the
+// state, then the Jacobian determinant, c, rho, the group speed
+// |c nu/|nu| + wind| and the travel time dt of the segment into the row.
+template <bool CART>
+GA_HD void rd_wv_rows_put(const PathOut &Po, PathLane &pl, const RdMed &M, const double *y, int item, int leg, int m,
+                          double dt)
+{
+    RdPt P;
+    if (CART) rd_point(M, y[2], y[0], y[1], P);
+    else rd_point(M, y[0], y[1], y[2], P);
+    const double nm = sqrt(y[3] * y[3] + y[4] * y[4] + y[5] * y[5]);
+    double g0, g1, g2;
+    if (CART) { g0 = P.c * y[3] / nm + P.u; g1 = P.c * y[4] / nm + P.v; g2 = P.c * y[5] / nm + P.w; }
+    else      { g0 = P.c * y[3] / nm + P.w; g1 = P.c * y[4] / nm + P.v; g2 = P.c * y[5] / nm + P.u; }
+    ga_path_pt a, b;
+    a.launch_idx = b.launch_idx = item;
+    a.leg = b.leg = leg;
+    a.step = b.step = m;
+    a.kind = PT_WV_STATE;
+    a.pos0 = y[0]; a.pos1 = y[1]; a.z = y[2];
+    a.tl = y[3]; a.absorption = y[4]; a.time = y[5];
+    b.kind = PT_WV_MEDIUM;
+    b.pos0 = CART ? rd3_jacobian(y, P) : rd_jacobian(y, P);
+    b.pos1 = P.c;
+    b.z = P.rho;
+    b.tl = sqrt(g0 * g0 + g1 * g1 + g2 * g2);
+    b.absorption = dt;
+    b.time = 0.0;
+    path_put(Po, pl, item, a);
+    path_put(Po, pl, item, b);
+}
+
 // Per-lane trace loop (the range-dependent run_prop with write_rays=false).
 // BLW: instantiation for M.use_blw on a terrain (rd_medium_rhs).  ya, yb:
 // the lane's two state rows (current row and the RK4 accumulator, which
@@ -1168,7 +1200,8 @@ GA_HD void rd3_path_xyz(const RdMed &M, const double *y, ga_path_pt &R)
 // write_rays / write_caustics bookkeeping of run_prop
 // (main.infraga.sph.rngdep.cpp:451-490): segments of rows 1..k-1 added one
 // by one, rows m == 1 or m % 15 == 0 and Jacobian sign changes written to
-// A.path.
+// A.path; PATH_WVROWS (18 equations) writes every row 0..k of every leg as
+// waveform-row records (run_wnl_wvfrm, main.infraga.sph.rngdep.cpp:1108).
 template <int NEQ, bool BLW, bool PATHS = false, bool CART = false, class V, class NextRay>
 GA_HD void rd_lane_loop(const RdFanArgs &A, int lane, NextRay next_ray, V ya, V yb, V ys)
 {
@@ -1201,6 +1234,9 @@ GA_HD void rd_lane_loop(const RdFanArgs &A, int lane, NextRay next_ray, V ya, V 
             rmax = (CART && M.strat) ? M.z0 : 0.0;
             k = 0; bnc = 0; nst = 0; narr = 0;
             fetch = false;
+            if constexpr (PATHS && NEQ == 18) {
+                if (A.path.flags & PATH_WVROWS) rd_wv_rows_put<CART>(A.path, pl, M, y0, ray, 0, 0, 0.0);
+            }
         }
         if (CART) s += rd3_rk4_step<NEQ>(M, L, y, y1, ys);
         else s += rd_rk4_step<NEQ, BLW>(M, L, y, y1, ys);
@@ -1221,10 +1257,16 @@ GA_HD void rd_lane_loop(const RdFanArgs &A, int lane, NextRay next_ray, V ya, V 
 #pragma unroll
         for (int i = 0; i < 6; ++i) o[i] = y[i];
         if (PATHS) {
-            if (!end) {
-                double dt, da;
+            const bool wv = NEQ == 18 && (A.path.flags & PATH_WVROWS);
+            double dt = 0.0, da = 0.0;
+            if (!end || wv) {
                 if (CART) rd3_seg_book(M, o, n, A.freq, dt, da);
                 else rd_seg_book(M, o, n, A.freq, dt, da);
+            }
+            if constexpr (NEQ == 18) {
+                if (wv) rd_wv_rows_put<CART>(A.path, pl, M, n, ray, bnc, k, dt);
+            }
+            if (!end) {
                 tt += dt;
                 att += da;
                 const bool caus = NEQ == 18 && (A.path.flags & GA_PATH_CAUSTICS);
@@ -1319,6 +1361,9 @@ GA_HD void rd_lane_loop(const RdFanArgs &A, int lane, NextRay next_ray, V ya, V 
                 ++bnc;
                 s = 0.0;
                 k = 0;
+                if constexpr (PATHS && NEQ == 18) {
+                    if (A.path.flags & PATH_WVROWS) rd_wv_rows_put<CART>(A.path, pl, M, yr, ray, bnc, 0, 0.0);
+                }
             }
         }
         if (done) {

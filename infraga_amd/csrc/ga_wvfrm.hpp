@@ -32,6 +32,7 @@
 #pragma once
 
 #include "ga_device.hpp"
+#include "ga_rdtrace.hpp"
 #include "../../include/infraga_gpu.h"
 
 namespace ga {
@@ -57,8 +58,14 @@ struct WvRay {
 constexpr int WV_PREP_T = 256;
 constexpr int WV_MAX_SLOTS = 256;    // legs per waveform (bounces + 1)
 
+// The medium the absorption is evaluated in: a stratified profile, or a
+// range-dependent one on the spherical or the Cartesian grid.
+constexpr int WV_STRAT = 0, WV_RD_SPH = 1, WV_RD_CART = 2;
+
 struct WvArgs {
-    Medium M;
+    Medium M;                    // WV_STRAT
+    RdMed RM;                    // WV_RD_SPH, WV_RD_CART
+    double ds_floor;             // smallest evolution step [km]
     int len, lg, n_wv;
     double ds_wv;                // wvfrm_ds
     const double *tw;            // [len/2][2]: cos, sin of -2 pi j / len
@@ -188,7 +195,7 @@ struct WvPoint {
     double cpt;        // c at the ray position
 };
 
-template <bool SPH>
+template <bool SPH, int MED>
 __global__ void __launch_bounds__(WV_T) wv_evolve_kernel(WvArgs A)
 {
     __shared__ double re[WV_LDS], im[WV_LDS];
@@ -272,7 +279,7 @@ __global__ void __launch_bounds__(WV_T) wv_evolve_kernel(WvArgs A)
                 const double bc = wv_beta(R, ks, s);
                 double ds = A.ds_wv / (PI * Uf_max * bc);
                 ds = fmin(ds, 1.0);
-                ds = fmax(ds, SPH ? 1.0e-7 : 1.0e-5);
+                ds = fmax(ds, A.ds_floor);
                 const double x_last = s_tot;
                 const double sn = fmin(s + ds, x_last);
                 const double bn = wv_beta(R, wv_seek(R, n, ks, sn), sn);
@@ -288,11 +295,25 @@ __global__ void __launch_bounds__(WV_T) wv_evolve_kernel(WvArgs A)
                     const double pa = wv_lin(R, ks, WV_F_POS, s), pb = wv_lin(R, ks, WV_F_POS + 1, s);
                     const double pc = wv_lin(R, ks, WV_F_POS + 2, s);
                     (void)pb;
-                    const double z = SPH ? pa - M.r0 : pc;
-                    const SegPt sp = seg_point(M, clampz(M, z), curz);
-                    const double cpt = herm_f(sp, 1), rho = herm_f(sp, 7);
+                    double z, cpt, rho;
+                    const double *sbc;
+                    if (MED == WV_STRAT) {
+                        z = SPH ? pa - M.r0 : pc;
+                        const SegPt sp = seg_point(M, clampz(M, z), curz);
+                        cpt = herm_f(sp, 1);
+                        rho = herm_f(sp, 7);
+                        sbc = M.sbc;
+                    } else {
+                        RdPt Q;
+                        if (MED == WV_RD_CART) rd_point(A.RM, pc, pa, pb, Q);
+                        else rd_point(A.RM, pa, pb, pc, Q);
+                        z = MED == WV_RD_CART ? pc : pa - A.RM.r0;
+                        cpt = Q.c;
+                        rho = Q.rho;
+                        sbc = A.RM.sbc;
+                    }
                     const SbGas g = sb_gas(cpt, rho, 0.0);
-                    const SbPoint P = sb_point<SPH>(z, g, M.sbc);
+                    const SbPoint P = sb_point<SPH>(z, g, sbc);
                     if (tid == WV_T - 64) {
                         pt.g = g;
                         pt.P = P;
@@ -343,7 +364,7 @@ __global__ void __launch_bounds__(WV_T) wv_evolve_kernel(WvArgs A)
                 double Pr[WV_BPT], Pi[WV_BPT], gf[WV_BPT];
                 {
                     const double cc = -pt.cpt / c0;
-                    const double tweak = M.tweak;
+                    const double tweak = MED == WV_STRAT ? M.tweak : A.RM.tweak;
 #pragma unroll
                     for (int j = 0; j < WV_BPT; ++j) {
                         const int b = tid + j * WV_T;

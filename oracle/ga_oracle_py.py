@@ -974,6 +974,8 @@ def _wv_bind(L):
     L.go_h_wnl_run.argtypes = [C.c_void_p, C.POINTER(Params), C.POINTER(WvParams), _dp, C.c_double, C.c_double,
                                C.c_int, C.c_double, C.c_double, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
                                C.POINTER(C.c_int)]
+    L.go_h_rd_wnl_run.restype = C.c_longlong
+    L.go_h_rd_wnl_run.argtypes = L.go_h_wnl_run.argtypes
     L._wv_bound = True
 
 
@@ -1018,7 +1020,8 @@ def wv_build(oracle, option="impulse", p0=10.0, t0=0.1, alpha=4.0, length=8192, 
 
 
 def wv_run(medium, params, wv, src, incl, az, bounces, t_src, p_src, freq=0.1, wvfrm_ref=1.0):
-    """run_wnl_wvfrm through a stratified Medium: dict with t_init, p_init,
+    """run_wnl_wvfrm through a stratified Medium or a range-dependent
+    RdMedium / Rd3Medium (src: lat, lon, alt or x, y, z): dict with t_init, p_init,
     t_out, p_out, arrival (pos0, pos1, time, cel, turn_ht, incl_arr, back_az,
     tl, absorption), ok (ray returned to the ground), steps."""
     L = medium.o.lib
@@ -1030,7 +1033,8 @@ def wv_run(medium, params, wv, src, incl, az, bounces, t_src, p_src, freq=0.1, w
     out = {k: np.zeros(n) for k in ("t_init", "p_init", "t_out", "p_out")}
     arr = np.zeros(9)
     ok = C.c_int(0)
-    steps = L.go_h_wnl_run(medium.h, C.byref(params), C.byref(wv), _ptr(s), float(incl), float(az), int(bounces),
+    run = L.go_h_rd_wnl_run if isinstance(medium, RdMedium) else L.go_h_wnl_run
+    steps = run(medium.h, C.byref(params), C.byref(wv), _ptr(s), float(incl), float(az), int(bounces),
                            float(freq), float(wvfrm_ref), _ptr(ts), _ptr(ps), _ptr(out["t_init"]), _ptr(out["p_init"]),
                            _ptr(out["t_out"]), _ptr(out["p_out"]), _ptr(arr), C.byref(ok))
     out.update(arrival=arr, ok=bool(ok.value), steps=int(steps))

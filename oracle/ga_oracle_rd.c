@@ -1816,3 +1816,15 @@ void *go_rd_new_xy(const double *x_km, int nx, const double *y_km, int ny, const
     if (go_rd_init_xy(a, x_km, nx, y_km, ny, z, nz, c, u, v, rho)) { free(a); return NULL; }
     return a;
 }
+
+/* Public forms of the Cartesian set's prop_rk4 and segment bookkeeping
+ * (the waveform driver, ga_wvfrm.c). */
+int go_rd3_prop_rk4(go_rd *a, const go_params *p, const go_src *s, double *rows, int length, int *check)
+{
+    return rd3_prop_rk4(a, p, s, rows, length, check);
+}
+
+void go_rd3_seg(go_rd *a, const double *y0, const double *y1, double freq, double *dt, double *datt)
+{
+    rd3_seg(a, y0, y1, freq, dt, datt);
+}

@@ -141,6 +141,8 @@ void   go_rd3_rhs(go_rd *a, const go_params *p, const double *y, double *f);
 void   go_rd3_set_initial(go_rd *a, const go_params *p, go_src *s, double *y);
 void   go_rd3_set_refl(go_rd *a, const go_params *p, const go_src *s, const double *rk2, const double *rk1, double *y0);
 double go_rd3_jacobian(go_rd *a, const double *y);
+int    go_rd3_prop_rk4(go_rd *a, const go_params *p, const go_src *s, double *rows, int length, int *check);
+void   go_rd3_seg(go_rd *a, const double *y0, const double *y1, double freq, double *dt, double *datt);
 /* infraga-3d-rngdep -eig_search / -eig_direct: the 3D search of
  * geoac.eigenray.3d.cpp over this medium; src = (x, y, z) [km], rcvr = (x, y). */
 int    go_rd3_eig_search(go_rd *a, const go_params *p, const double src_in[3], const double rcvr[2],

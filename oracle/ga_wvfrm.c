@@ -8,6 +8,7 @@
  * with FFTW's sign convention and no normalisation.
  */
 #include "ga_wvfrm.h"
+#include "ga_oracle_rd.h"
 
 #include <math.h>
 #include <stdio.h>
@@ -179,7 +180,86 @@ static double st_tz(void *m, double a, double b)
 go_wv_med go_wv_strat_med(go_atmo *a)
 {
     go_wv_med M = {a, a->geom == GA_GEOM_SPH, a->r0, st_init, st_prop, st_refl, st_tt, st_att, st_c, st_rho,
-                   st_winds, st_sb, st_jac, st_amp, st_tz};
+                   st_winds, st_sb, st_jac, st_amp, st_tz, a->geom == GA_GEOM_SPH ? 1.0e-7 : 1.0e-5};
+    return M;
+}
+
+/* ---- the range-dependent media (ga_oracle_rd.c): sph rows (r, lat, lon);
+ * Cartesian rows (x, y, z), whose point functions take (z, x, y) ---------- */
+static void rd_init(void *m, const go_params *p, go_src *s, double *y)
+{
+    go_rd *a = (go_rd *)m;
+    if (a->cart) go_rd3_set_initial(a, p, s, y);
+    else go_rd_set_initial(a, p, s, y);
+}
+static int rd_prop(void *m, const go_params *p, const go_src *s, double *rows, int length, int *check)
+{
+    go_rd *a = (go_rd *)m;
+    return a->cart ? go_rd3_prop_rk4(a, p, s, rows, length, check) : go_rd_prop_rk4(a, p, s, rows, length, check);
+}
+static void rd_refl(void *m, const go_params *p, const go_src *s, const double *k2, const double *k1, double *o)
+{
+    go_rd *a = (go_rd *)m;
+    if (a->cart) go_rd3_set_refl(a, p, s, k2, k1, o);
+    else go_rd_set_refl(a, p, s, k2, k1, o);
+}
+static double rd_tt(void *m, const double *y0, const double *y1)
+{
+    go_rd *a = (go_rd *)m;
+    if (!a->cart) return go_rd_travel_time_seg(a, y0, y1);
+    double dt, da;
+    go_rd3_seg(a, y0, y1, 0.1, &dt, &da);
+    return dt;
+}
+static double rd_att(void *m, const double *y0, const double *y1, double f)
+{
+    go_rd *a = (go_rd *)m;
+    if (!a->cart) return go_rd_atten_seg(a, y0, y1, f);
+    double dt, da;
+    go_rd3_seg(a, y0, y1, f, &dt, &da);
+    return da;
+}
+#define RD_PT(a, p0, p1, p2) ((a)->cart ? (p2) : (p0)), ((a)->cart ? (p0) : (p1)), ((a)->cart ? (p1) : (p2))
+static double rd_c(void *m, double p0, double p1, double p2) { go_rd *a = (go_rd *)m; return go_rd_c(a, RD_PT(a, p0, p1, p2)); }
+static double rd_rho(void *m, double p0, double p1, double p2) { go_rd *a = (go_rd *)m; return go_rd_rho(a, RD_PT(a, p0, p1, p2)); }
+static void rd_winds(void *m, double p0, double p1, double p2, double w[3])
+{
+    go_rd *a = (go_rd *)m;
+    w[0] = go_rd_u(a, RD_PT(a, p0, p1, p2));
+    w[1] = go_rd_v(a, RD_PT(a, p0, p1, p2));
+    w[2] = go_rd_w(a, RD_PT(a, p0, p1, p2));
+}
+static double rd_sb(void *m, double p0, double p1, double p2, double f)
+{
+    go_rd *a = (go_rd *)m;
+    if (!a->cart) return go_rd_sb_alpha(a, p0, p1, p2, f);
+    return go_sb_alpha_at(go_rd_c(a, p2, p0, p1), go_rd_rho(a, p2, p0, p1), p2, 0, a->tweak_abs, f);
+}
+#undef RD_PT
+static double rd_jac(void *m, const double *y)
+{
+    go_rd *a = (go_rd *)m;
+    return a->cart ? go_rd3_jacobian(a, y) : go_rd_jacobian(a, y);
+}
+static double rd_amp(void *m, const go_src *s, const double *y)
+{
+    go_rd *a = (go_rd *)m;
+    return a->cart ? go_rd3_amp(a, s, y) : go_rd_amp(a, s, y);
+}
+static double rd_tz(void *m, double p0, double p1)
+{
+    go_rd *a = (go_rd *)m;
+    return go_rd_topo_z(a, p0, p1) - a->r0;       /* altitude (r0 = 0 on a Cartesian grid) */
+}
+
+/* The range-dependent equation sets floor the evolution step at 1e-6 km
+ * (geoac.eqset.sph.rngdep.cpp and 3d.rngdep.cpp, wnl_wvfrm); a stratified
+ * profile over terrain runs the stratified sets' wnl_wvfrm. */
+go_wv_med go_wv_rd_med(void *h)
+{
+    go_rd *a = (go_rd *)h;
+    go_wv_med M = {a, !a->cart, a->r0, rd_init, rd_prop, rd_refl, rd_tt, rd_att, rd_c, rd_rho, rd_winds, rd_sb,
+                   rd_jac, rd_amp, rd_tz, a->strat ? (a->cart ? 1.0e-5 : 1.0e-7) : 1.0e-6};
     return M;
 }
 
@@ -260,7 +340,7 @@ double go_wnl_evolve(const go_wv_med *M, const go_wv_params *W, const double *ro
         else bc = lin_f(&L[5], s);
         ds = W->ds / (WV_PI * Uf_max * bc);
         ds = fmin(ds, 1.0);
-        ds = fmax(ds, sph ? 1.0e-7 : 1.0e-5);
+        ds = fmax(ds, M->ds_floor);
         const double sn = fmin(s + ds, x_last);
         if (fabs(lin_f(&L[3], sn)) >= 0.5) bn = lin_f(&L[4], sn);
         else bn = lin_f(&L[5], sn);
@@ -460,6 +540,17 @@ long long go_wnl_run(const go_wv_med *M, const go_params *p_in, const go_wv_para
     free(rows);
     free(u);
     return started ? steps : -1;
+}
+
+/* ctypes entry: go_wnl_run over a range-dependent handle. */
+long long go_h_rd_wnl_run(void *h, const go_params *p, const go_wv_params *W, const double src_in[3], double incl_deg,
+                          double az_deg, int bounces, double freq, double wvfrm_ref, const double *t_src,
+                          const double *p_src, double *t_init, double *p_init, double *t_out, double *p_out,
+                          double arrival[9], int *ok)
+{
+    go_wv_med M = go_wv_rd_med(h);
+    return go_wnl_run(&M, p, W, src_in, incl_deg, az_deg, bounces, freq, wvfrm_ref, t_src, p_src, t_init, p_init,
+                      t_out, p_out, arrival, ok);
 }
 
 /* ctypes entry: go_wnl_run over a stratified atmosphere handle. */

@@ -11,6 +11,8 @@
  *   src/geoac/geoac.eqset.3d.strat.cpp   wnl_wvfrm (3D)
  *   src/main.infraga.sph.cpp             run_wnl_wvfrm, :1142-1479
  *   src/main.infraga.3d.cpp              run_wnl_wvfrm
+ *   src/geoac/geoac.eqset.{sph,3d}.rngdep.cpp, src/main.infraga.{sph,3d}.rngdep.cpp
+ *                                        the range-dependent forms
  * The reference transforms with FFTW (r2c / c2r, unnormalised), which this
  * image lacks; go_fft is a radix-2 transform with FFTW's sign convention
  * and scaling, so the waveform lengths here are powers of two.
@@ -65,10 +67,14 @@ typedef struct {
     double (*sb)(void *m, double a, double b, double c, double freq);
     double (*jac)(void *m, const double *y);
     double (*amp)(void *m, const go_src *s, const double *y);
-    double (*topo_z)(void *m, double a, double b);      /* 3D: ground height under (x, y) */
+    double (*topo_z)(void *m, double a, double b);      /* ground height under the row position [km] */
+    double ds_floor;          /* smallest evolution step: 1e-7 (sph), 1e-5 (3D), 1e-6 (range-dependent sets) */
 } go_wv_med;
 
 go_wv_med go_wv_strat_med(go_atmo *a);
+/* A range-dependent medium (ga_oracle_rd.h handle; spherical or Cartesian,
+ * with the stratified semantics when go_rd_h_set_strat is on). */
+go_wv_med go_wv_rd_med(void *rd);
 
 /* wnl_wvfrm over rows k1..k2-1 (18 doubles each) of one leg: evolves the
  * scaled waveform u (time t[len], values in place) and returns the leg
@@ -97,6 +103,12 @@ long long go_h_wnl_run(void *h, const go_params *p, const go_wv_params *W, const
                        double az_deg, int bounces, double freq, double wvfrm_ref, const double *t_src,
                        const double *p_src, double *t_init, double *p_init, double *t_out, double *p_out,
                        double arrival[9], int *ok);
+
+/* go_wnl_run over a range-dependent handle (ctypes). */
+long long go_h_rd_wnl_run(void *rd, const go_params *p, const go_wv_params *W, const double src_in[3],
+                          double incl_deg, double az_deg, int bounces, double freq, double wvfrm_ref,
+                          const double *t_src, const double *p_src, double *t_init, double *p_init, double *t_out,
+                          double *p_out, double arrival[9], int *ok);
 
 #ifdef __cplusplus
 }
