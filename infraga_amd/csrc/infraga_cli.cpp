@@ -72,7 +72,7 @@ void usage()
               << "  -prop        ray paths for a fan of inclinations and azimuths\n"
               << "  -eig_search  eigenrays between the source and a receiver\n"
               << "  -eig_direct  one eigenray from a launch-angle estimate (incl_est, az_est)\n"
-              << "  -wnl_wvfrm   weakly non-linear waveform along one ray (infraga-sph, infraga-3d)\n"
+              << "  -wnl_wvfrm   weakly non-linear waveform along one ray\n"
               << "Parameters use the key=value names of the reference infraga drivers.\n";
 }
 
@@ -1490,19 +1490,31 @@ bool wv_load(const char *path, std::vector<double> &t, std::vector<double> &p)
     return len > 0;
 }
 
-// infraga-sph / infraga-3d -wnl_wvfrm (run_wnl_wvfrm, main.infraga.sph.cpp:
-// 1142-1479, main.infraga.3d.cpp:1087-1430): one ray, its waveform set at
+// -wnl_wvfrm (run_wnl_wvfrm: main.infraga.sph.cpp:1142-1479,
+// main.infraga.3d.cpp:1087-1430, main.infraga.sph.rngdep.cpp:1108-1410,
+// main.infraga.3d.rngdep.cpp:1051-1350): one ray, its waveform set at
 // wvfrm_ref km along the ray and evolved to the last ground return;
 // <id>.wvfrm_init.dat, <id>.wvfrm_out.dat and, with write_ray=true,
-// <id>.raypaths.dat.  Terrain (topo_file=) and intermediate waveform dumps
-// (wvfrm_out_step < 1e4) are refused; the waveform length must be a power
-// of two (the engine's transform).
+// <id>.raypaths.dat.  The stratified binaries take topo_file= (the source
+// then defaults to the terrain's centre), the range-dependent ones their
+// node files.  Intermediate waveform dumps (wvfrm_out_step < 1e4) are
+// refused; the waveform length must be a power of two (the engine's
+// transform).
 int run_wnl_wvfrm(int argc, char **argv)
 {
-    const bool sph = GA_CLI_GEOM == GA_GEOM_SPH;
+    constexpr bool rd = GA_CLI_GEOM == GA_GEOM_SPH_RNGDEP || GA_CLI_GEOM == GA_GEOM_3D_RNGDEP;
+    constexpr bool cart = GA_CLI_GEOM == GA_GEOM_3D || GA_CLI_GEOM == GA_GEOM_3D_RNGDEP;
+    const bool sph = !cart;
+    const int first = rd ? 5 : 3;
+    if (rd && argc < 5) {
+        std::cerr << GA_CLI_NAME << ": usage: -wnl_wvfrm <profile prefix> <node file> <node file> [parameters]\n";
+        return 2;
+    }
     double incl = 15.0, az = 90.0;
     int bounces = 0;
-    double s0 = sph ? 30.0 : 0.0, s1 = 0.0, z_src = 0.0;
+    double s0 = rd ? NAN : (sph ? 30.0 : 0.0), s1 = rd ? NAN : 0.0, z_src = 0.0;
+    bool src_set[2] = {false, false};
+    const char *topo_file = nullptr;
     bool write_rays = false, reverse_winds = false, custom_id = false;
     double freq = 0.1, wvfrm_ref = 1.0, wvfrm_out_step = 1.0e10;
     double wp0 = 10.0, wt0 = 0.1, walpha = 4.0;
@@ -1517,18 +1529,19 @@ int run_wnl_wvfrm(int argc, char **argv)
     ga_prop_params p;
     ga_prop_params_default(&p, GA_CLI_GEOM);
     double z_grnd = NAN;
-    for (int i = 3; i < argc; i++) {
+    for (int i = first; i < argc; i++) {
         const char *a = argv[i], *v = nullptr;
         if (key(a, "inclination=", v)) incl = std::atof(v);
         else if (key(a, "azimuth=", v)) az = std::atof(v);
         else if (key(a, "bounces=", v)) bounces = std::max(0, std::atoi(v));
         else if (key2(a, "src_alt=", "alt_src=", v)) z_src = std::atof(v);
-        else if (sph && key2(a, "src_lat=", "lat_src=", v)) s0 = std::atof(v);
-        else if (sph && key2(a, "src_lon=", "lon_src=", v)) s1 = std::atof(v);
-        else if (!sph && key2(a, "src_x=", "x_src=", v)) s0 = std::atof(v);
+        else if (sph && key2(a, "src_lat=", "lat_src=", v)) { s0 = std::atof(v); src_set[0] = true; }
+        else if (sph && key2(a, "src_lon=", "lon_src=", v)) { s1 = std::atof(v); src_set[1] = true; }
+        else if (!sph && key2(a, "src_x=", "x_src=", v)) { s0 = std::atof(v); src_set[0] = true; }
         // main.infraga.3d.cpp compares "y_src_y=" over 6 characters, so
         // only src_y= sets y there
-        else if (!sph && key(a, "src_y=", v)) s1 = std::atof(v);
+        else if (!sph && (rd ? key2(a, "src_y=", "y_src=", v) : key(a, "src_y=", v))) { s1 = std::atof(v); src_set[1] = true; }
+        else if (rd && key(a, "calc_amp=", v)) { /* the waveform needs the auxiliary equations */ }
         else if (key(a, "write_ray=", v)) write_rays = str2bool(v);
         else if (key(a, "wvfrm_file=", v)) wvfrm_file = v;
         else if (key(a, "wvfrm_opt=", v)) wvfrm_opt = v;
@@ -1569,11 +1582,9 @@ int run_wnl_wvfrm(int argc, char **argv)
         else if (key(a, "reverse_winds=", v)) reverse_winds = str2bool(v);
         else if (key(a, "output_id=", v)) { custom_id = true; output_id = v; }
         else if (key(a, "z_grnd=", v)) z_grnd = std::atof(v);
-        else if (key(a, "topo_use_BLw=", v)) { /* vertical wind needs terrain */ }
-        else if (key(a, "topo_file=", v)) {
-            std::cerr << GA_CLI_NAME << ": -wnl_wvfrm over terrain is not supported by this engine\n";
-            return 2;
-        } else {
+        else if (key(a, "topo_use_BLw=", v)) p.topo_use_blw = str2bool(v) ? 1 : 0;
+        else if (key(a, "topo_file=", v)) topo_file = v;
+        else {
             std::cerr << GA_CLI_NAME << ": unrecognized parameter entry: " << a << '\n';
             return 2;
         }
@@ -1603,18 +1614,55 @@ int run_wnl_wvfrm(int argc, char **argv)
     ga_ctx *ctx = nullptr;
     if (ga_ctx_create(&ctx, 0) != GA_OK) return die(nullptr, "no usable GPU");
     int aid = 0;
-    if (ga_atmo_load_file(ctx, argv[2], prof_format, reverse_winds ? 1 : 0, &aid) != GA_OK)
-        return die(ctx, "cannot load the atmosphere");
-    const double z0 = std::isnan(z_grnd) ? profile_bottom(argv[2]) : z_grnd;
-    p.z_grnd = z0;
+    double z0 = 0.0;
+    const bool is_topo = topo_file != nullptr;
+    if (rd || is_topo) {
+        // range-dependent node files, or the stratified profile over terrain
+        // as identical columns (set_region with topo_file); the source at the
+        // region's centre unless given (main.infraga.sph.cpp:1187-1191,
+        // main.infraga.3d.cpp:1131-1134, main.infraga.sph.rngdep.cpp:1140-1142)
+        int rc;
+        if (rd) rc = (cart ? ga_rngdep_load_files_xy : ga_rngdep_load_files)(ctx, argv[2], argv[3], argv[4], topo_file,
+                                                                          prof_format, reverse_winds ? 1 : 0, &aid);
+        else rc = (cart ? ga_atmo_load_file_topo_xy : ga_atmo_load_file_topo)(ctx, argv[2], topo_file, prof_format,
+                                                                            reverse_winds ? 1 : 0, &aid);
+        if (rc != GA_OK) return die(ctx, "cannot load the atmosphere");
+        ga_prop_params q;
+        ga_prop_params_default(&q, cart ? GA_GEOM_3D_RNGDEP : GA_GEOM_SPH_RNGDEP);
+        q.abs_coeff = p.abs_coeff;
+        q.alt_max = p.alt_max;
+        q.rng_max = p.rng_max;
+        q.ds_min = p.ds_min;
+        q.ds_max = p.ds_max;
+        q.s_max = p.s_max;
+        q.topo_use_blw = is_topo ? p.topo_use_blw : 0;
+        for (int k = 0; k < 4; ++k) q.bounds[k] = NAN;
+        double info[8];
+        if (ga_rngdep_info(ctx, aid, info) != GA_OK) return die(ctx, "region");
+        const double deg = cart ? 1.0 : Pi / 180.0;
+        if (!src_set[0]) s0 = (info[1] + info[0]) / 2.0 / deg;
+        if (!src_set[1]) s1 = (info[3] + info[2]) / 2.0 / deg;
+        z0 = (!is_topo && !std::isnan(z_grnd)) ? z_grnd : info[5];
+        if (!is_topo) q.z_grnd = z0;
+        double pt[3] = {s0 * deg, s1 * deg, 0.0}, atmo_out[40], topo_out[10];
+        if (ga_rngdep_eval(ctx, aid, pt, 1, atmo_out, topo_out) != GA_OK) return die(ctx, "terrain at the source");
+        const double zg = is_topo ? topo_out[0] : (cart ? 0.0 : z0);
+        z_src = std::max(z_src, zg);
+        p = q;
+    } else {
+        if (ga_atmo_load_file(ctx, argv[2], prof_format, reverse_winds ? 1 : 0, &aid) != GA_OK)
+            return die(ctx, "cannot load the atmosphere");
+        z0 = std::isnan(z_grnd) ? profile_bottom(argv[2]) : z_grnd;
+        p.z_grnd = z0;
+        z_src = std::max(z_src, z0);
+    }
     p.bounces = bounces;
     p.freq = freq;
     p.calc_amp = 1;
-    z_src = std::max(z_src, z0);
     p.src[0] = s0;
     p.src[1] = s1;
     p.src[2] = z_src;
-    const std::string id = custom_id ? std::string(output_id) : strip_ext(argv[2]);
+    const std::string id = custom_id ? std::string(output_id) : (rd ? std::string(argv[2]) : strip_ext(argv[2]));
     const double launch[2] = {incl, az};
     std::vector<double> ti((size_t)len), pi_((size_t)len), to((size_t)len), po((size_t)len);
     ga_arrival A{};
@@ -1685,7 +1733,12 @@ int run_wnl_wvfrm(int argc, char **argv)
     }
     std::ofstream f(id + ".wvfrm_out.dat");
     f << "# " << GA_CLI_NAME << " -wnl_wvfrm summary:" << '\n';
-    f << "#" << '\t' << "profile: " << argv[2] << '\n';
+    if (rd && cart) {
+        f << "#" << '\t' << "profile prefix: " << argv[2] << '\n';
+        f << "#" << '\t' << "profile grid files: " << argv[3] << ", " << argv[4] << '\n';
+    } else {
+        f << "#" << '\t' << "profile: " << argv[2] << '\n';
+    }
     f << "#" << '\t' << "inclination: " << (incl * (Pi / 180.0)) * (180.0 / Pi) << '\n';
     // the reference prints geoac::phi (90 - azimuth) here
     f << "#" << '\t' << "azimuth: " << (Pi / 2.0 - az * (Pi / 180.0)) * (180.0 / Pi) << '\n';
@@ -1694,7 +1747,11 @@ int run_wnl_wvfrm(int argc, char **argv)
         f << "#" << '\t' << "source (lat, lon, alt): " << (s0 * (Pi / 180.0)) * (180.0 / Pi) << ", "
           << (s1 * (Pi / 180.0)) * (180.0 / Pi) << ", " << z_src << '\n';
     else f << "#" << '\t' << "source location: " << s0 << ", " << s1 << ", " << z_src << '\n';
-    f << "#" << '\t' << "ground elevation: " << z0 << '\n';
+    if (is_topo) f << "#" << '\t' << "topo file:" << topo_file << '\n';
+    else f << "#" << '\t' << "ground elevation: " << z0 << '\n';
+    // the spherical range-dependent writer adds this line, spelling included
+    // (main.infraga.sph.rngdep.cpp:1388)
+    if (rd && !cart) f << "#" << '\t' << "ground evevation: " << z0 << '\n';
     f << "#" << '\t' << "S&B atten coeff: " << p.abs_coeff << '\n';
     if (from_file) f << "#" << '\t' << "waveform source file: " << wvfrm_file << '\n';
     else {
@@ -1728,6 +1785,8 @@ int main(int argc, char **argv)
         if (std::strncmp(argv[1], "-prop", 5) == 0) return run_prop_rd(argc, argv);
         if (std::strncmp(argv[1], "-eig_search", 11) == 0) return run_eig_rd(argc, argv, false);
         if (std::strncmp(argv[1], "-eig_direct", 11) == 0) return run_eig_rd(argc, argv, true);
+        if (std::strncmp(argv[1], "-wnl_wvfrm", 10) == 0 || std::strncmp(argv[1], "-w", 2) == 0)
+            return run_wnl_wvfrm(argc, argv);
         std::cerr << GA_CLI_NAME << ": option " << argv[1] << " is not supported by this engine\n";
         return 2;
     }
