@@ -357,8 +357,12 @@ typedef struct {
 } ga_wvfrm_stats;
 
 /* -wnl_wvfrm for n launches (incl, az) [deg] from p->src over a stratified
- * atmosphere (GA_GEOM_SPH or GA_GEOM_3D; p->calc_amp is implied, and, as in
- * run_wnl_wvfrm, no turning-height or time limit stops a ray).  The source
+ * atmosphere (GA_GEOM_SPH, GA_GEOM_3D) or a range-dependent one
+ * (GA_GEOM_SPH_RNGDEP, GA_GEOM_3D_RNGDEP, also a stratified profile over
+ * terrain; geoac.eqset.{sph,3d}.rngdep.cpp wnl_wvfrm,
+ * src/main.infraga.sph.rngdep.cpp:1108-1410,
+ * src/main.infraga.3d.rngdep.cpp:1051-1350); p->calc_amp is implied, and,
+ * as in run_wnl_wvfrm, no turning-height or time limit stops a ray.  The source
  * waveform t_src[len] [s], p_src[len] [Pa] (a built-in one or a loaded
  * file) is set at ray length w->ref and evolved along every leg to the last
  * ground return.  Per launch i, rows i*len .. i*len+len-1 of t_init / p_init
