@@ -2559,6 +2559,8 @@ int go2_fan(go_2d *g, const double *launches, int n_launch, int bounces, double 
     return (cnt > out_cap || np > pts_cap) ? -1 : 0;
 }
 
+double go2_topo_z(go_2d *g, double r) { return g2_topo_z(g, r); }
+
 void *go2_h_new(void *atmo_h) { go_2d *g = (go_2d *)malloc(sizeof(go_2d)); go2_init(g, (go_atmo *)atmo_h); return g; }
 void go2_h_free(void *h) { if (h) { go2_free((go_2d *)h); free(h); } }
 void go2_h_config(void *h, int calc_amp, double z_reflect, double alt_max, double rng_max, double s_max, double ds_min,

@@ -312,6 +312,7 @@ double go2_atten_seg(go_2d *g, const double *y0, const double *y1, double freq);
 double go2_jacobian(go_2d *g, const double *y);
 double go2_amp(go_2d *g, const double *y);
 double go2_est_dev(go_2d *g, const double *rows, int k);
+double go2_topo_z(go_2d *g, double r);   /* topo::z(r): terrain, or the ground z0 */
 int    go2_fan(go_2d *g, const double *launches, int n_launch, int bounces, double src_alt, double freq, int flags,
                go_arrival *out, int out_cap, int *n_out, go_path_pt *pts, long long pts_cap, long long *n_pts,
                long long *ray_steps);

@@ -976,6 +976,9 @@ def _wv_bind(L):
                                C.POINTER(C.c_int)]
     L.go_h_rd_wnl_run.restype = C.c_longlong
     L.go_h_rd_wnl_run.argtypes = L.go_h_wnl_run.argtypes
+    L.go2_h_wnl_run.restype = C.c_longlong
+    L.go2_h_wnl_run.argtypes = [C.c_void_p, C.POINTER(WvParams), C.c_double, C.c_double, C.c_double, C.c_int,
+                                C.c_double, C.c_double, _dp, _dp, _dp, _dp, _dp, _dp, _dp, C.POINTER(C.c_int)]
     L._wv_bound = True
 
 
@@ -1037,5 +1040,24 @@ def wv_run(medium, params, wv, src, incl, az, bounces, t_src, p_src, freq=0.1, w
     steps = run(medium.h, C.byref(params), C.byref(wv), _ptr(s), float(incl), float(az), int(bounces),
                            float(freq), float(wvfrm_ref), _ptr(ts), _ptr(ps), _ptr(out["t_init"]), _ptr(out["p_init"]),
                            _ptr(out["t_out"]), _ptr(out["p_out"]), _ptr(arr), C.byref(ok))
+    out.update(arrival=arr, ok=bool(ok.value), steps=int(steps))
+    return out
+
+
+def wv_run_2d(medium, wv, incl, az, bounces, t_src, p_src, src_alt=0.0, freq=0.1, wvfrm_ref=1.0):
+    """infraga-2d run_wnl_wvfrm through a Medium2D: dict with t_init,
+    p_init, t_out, p_out, arrival (range, -, time, cel, turn_ht, -, -, tl,
+    absorption), ok, steps."""
+    L = medium.o.lib
+    _wv_bind(L)
+    n = wv.len
+    ts = np.ascontiguousarray(t_src, dtype=np.float64)
+    ps = np.ascontiguousarray(p_src, dtype=np.float64)
+    out = {k: np.zeros(n) for k in ("t_init", "p_init", "t_out", "p_out")}
+    arr = np.zeros(9)
+    ok = C.c_int(0)
+    steps = L.go2_h_wnl_run(medium.h, C.byref(wv), float(src_alt), float(incl), float(az), int(bounces), float(freq),
+                            float(wvfrm_ref), _ptr(ts), _ptr(ps), _ptr(out["t_init"]), _ptr(out["p_init"]),
+                            _ptr(out["t_out"]), _ptr(out["p_out"]), _ptr(arr), C.byref(ok))
     out.update(arrival=arr, ok=bool(ok.value), steps=int(steps))
     return out

@@ -263,21 +263,120 @@ go_wv_med go_wv_rd_med(void *h)
     return M;
 }
 
-/* ---- wnl_wvfrm (geoac.eqset.sph.strat.cpp:853-1148; 3D: eqset.3d.strat.cpp) -- */
-double go_wnl_evolve(const go_wv_med *M, const go_wv_params *W, const double *rows, int k1, int k2, double c_src,
-                     double c0, double cg0, double nu0, double rho0, double D0, double p0, double *t, double *u,
-                     long long *n_steps)
+/* ---- the step loop of wnl_wvfrm, shared by the geometries ---------------
+ * Splines over one abscissa (lx, n rows): D, beta, beta_caustic.  absorb(ctx,
+ * s, freq) returns -c / c0 * alpha(freq) at arc length s (the geometry's c
+ * and point).  strict: the sph / 2D caustic test (s + ds < s_tot and a
+ * strict sign change) instead of the 3D inclusive one. */
+typedef double (*wv_absorb_fn)(void *ctx, double s, double freq);
+
+static long long wv_evolve_core(const go_wv_params *W, lin1 *LD, lin1 *Lb, lin1 *Lbc, double s_tot, double x_last,
+                                double ds_floor, int strict, wv_absorb_fn absorb, void *actx, const double *t,
+                                double *u)
 {
-    const int len = W->len, fft_len = len / 2 + 1, n = k2 - k1;
-    const int sph = M->sph;
-    if (n_steps) *n_steps = 0;
-    if (n < 2) return 0.0;
-    double *buf = (double *)malloc(sizeof(double) * ((size_t)len * 4 + (size_t)fft_len * 10 + (size_t)n * 18));
+    const int len = W->len, fft_len = len / 2 + 1;
+    double *buf = (double *)malloc(sizeof(double) * ((size_t)len * 4 + (size_t)fft_len * 10));
     double *fft_time = buf, *u_sqr = fft_time + len, *du_sqr = u_sqr + len, *tukey = du_sqr + len;
     double *filter1 = tukey + len, *filter2 = filter1 + fft_len;
     double *Ur = filter2 + fft_len, *Ui = Ur + fft_len, *Ar = Ui + fft_len, *Ai = Ar + fft_len;
     double *Br = Ai + fft_len, *Bi = Br + fft_len, *Cr = Bi + fft_len, *Ci = Cr + fft_len;
-    double *lx = Ci + fft_len;
+
+    const double dt = t[1] - t[0];
+    const double tp = 0.8;
+    for (int m = 0; m < len; m++) {
+        if (m < tp * (len - 1) / 2.0) tukey[m] = 1.0 / 2.0 * (1.0 + cos(WV_PI * (2.0 * m / (tp * (len - 1)) - 1.0)));
+        else if (m > (len - 1) * (1.0 - tp / 2.0))
+            tukey[m] = 1.0 / 2.0 * (1.0 + cos(WV_PI * (2.0 * m / (tp * (len - 1)) - 2.0 / tp + 1.0)));
+        else tukey[m] = 1.0;
+    }
+    for (int m = 0; m < fft_len; m++) {
+        filter1[m] = pow(1.0 + pow(m / (W->filt1_g * fft_len), W->filt1_n1), -W->filt1_n2);
+        filter2[m] = pow(1.0 + pow(m / (W->filt2_g * fft_len), W->filt2_n1), -W->filt2_n2);
+    }
+    go_rfft(u, len, Ur, Ui);
+    double Uf_max = 0.0;
+    for (int m = 0; m < fft_len; m++) Uf_max = fmax(Uf_max, (1.0 / dt * m / len) * sqrt(sq(Ur[m]) + sq(Ui[m])));
+
+    long long steps = 0;
+    double s = 0.0, ds = W->ds;
+    while (s < s_tot) {
+        double bc, bn;
+        if (fabs(lin_f(LD, s)) >= 0.5) bc = lin_f(Lb, s);
+        else bc = lin_f(Lbc, s);
+        ds = W->ds / (WV_PI * Uf_max * bc);
+        ds = fmin(ds, 1.0);
+        ds = fmax(ds, ds_floor);
+        const double sn = fmin(s + ds, x_last);
+        if (fabs(lin_f(LD, sn)) >= 0.5) bn = lin_f(Lb, sn);
+        else bn = lin_f(Lbc, sn);
+
+        for (int m = 0; m < len; m++) u_sqr[m] = sq(u[m]);
+        du_sqr[0] = (u_sqr[1] - u_sqr[0]) / (t[1] - t[0]);
+        for (int m = 1; m < len - 1; m++) du_sqr[m] = (u_sqr[m + 1] - u_sqr[m - 1]) / (t[m + 1] - t[m - 1]);
+        du_sqr[len - 1] = (u_sqr[len - 1] - u_sqr[len - 2]) / (t[len - 1] - t[len - 2]);
+        go_rfft(u_sqr, len, Ar, Ai);
+        for (int m = 0; m < len; m++) fft_time[m] = u[m] * du_sqr[m];
+        go_rfft(fft_time, len, Br, Bi);
+        for (int m = 0; m < len; m++) fft_time[m] = sq(du_sqr[m]);
+        go_rfft(fft_time, len, Cr, Ci);
+
+        /* caustic between s and s + ds: phase shift by pi / 2 */
+        if (strict ? (s + ds < s_tot) : (s + ds <= s_tot)) {
+            const double d0 = lin_f(LD, s), d1 = lin_f(LD, s + ds);
+            if (strict ? (d0 * d1 < 0.0) : (d0 * d1 <= 0.0))
+                for (int m = 0; m < fft_len; m++) {
+                    double tmp = Ur[m];
+                    Ur[m] = -Ui[m];
+                    Ui[m] = tmp;
+                }
+        }
+        for (int m = 0; m < fft_len; m++) {
+            const double freq = 1.0 / dt * m / len;
+            const double absorp = pow(10.0, absorb(actx, s, freq) * ds / 10.0);
+            Ur[m] -= WV_PI * freq * (ds * (bc + bn) / 2.0 * Ai[m] + pow(ds, 2) / 2.0 * bc * bn * Bi[m] +
+                                     pow(ds, 3) / 8.0 * pow(bc, 2) * bn * Ci[m]) * filter1[m];
+            Ui[m] += WV_PI * freq * (ds * (bc + bn) / 2.0 * Ar[m] + pow(ds, 2) / 2.0 * bc * bn * Br[m] +
+                                     pow(ds, 3) / 8.0 * pow(bc, 2) * bn * Cr[m]) * filter1[m];
+            Ur[m] *= absorp * filter2[m];
+            Ui[m] *= absorp * filter2[m];
+        }
+        go_irfft(Ur, Ui, len, fft_time);
+        for (int m = 0; m < len; m++) u[m] = fft_time[m] * tukey[m] / len;
+        Uf_max = 0.0;
+        for (int m = 0; m < fft_len; m++) Uf_max = fmax(Uf_max, (1.0 / dt * m / len) * sqrt(sq(Ur[m]) + sq(Ui[m])));
+        s += ds;
+        steps++;
+    }
+    free(buf);
+    return steps;
+}
+
+/* ---- wnl_wvfrm (geoac.eqset.sph.strat.cpp:853-1148; 3D: eqset.3d.strat.cpp) -- */
+typedef struct {
+    const go_wv_med *M;
+    lin1 *L;                 /* ray position splines */
+    double c0;
+} wv_pos_absorb;
+
+/* -c(pos) / c0 * alpha(pos, freq) at the ray position of s */
+static double absorb_at_pos(void *ctx, double s, double freq)
+{
+    wv_pos_absorb *q = (wv_pos_absorb *)ctx;
+    const go_wv_med *M = q->M;
+    const double pa = lin_f(&q->L[0], s), pb = lin_f(&q->L[1], s), pc = lin_f(&q->L[2], s);
+    const double cpt = M->c(M->m, pa, pb, pc);
+    return -cpt / q->c0 * M->sb(M->m, pa, pb, pc, freq);
+}
+
+double go_wnl_evolve(const go_wv_med *M, const go_wv_params *W, const double *rows, int k1, int k2, double c_src,
+                     double c0, double cg0, double nu0, double rho0, double D0, double p0, double *t, double *u,
+                     long long *n_steps)
+{
+    const int n = k2 - k1;
+    const int sph = M->sph;
+    if (n_steps) *n_steps = 0;
+    if (n < 2) return 0.0;
+    double *lx = (double *)malloc(sizeof(double) * (size_t)n * 13);
     lin1 L[6];   /* ray_r, ray_th, ray_ph, D, beta, beta_caustic */
     for (int i = 0; i < 6; i++) {
         L[i].n = n; L[i].accel = 0;
@@ -313,79 +412,10 @@ double go_wnl_evolve(const go_wv_med *M, const go_wv_params *W, const double *ro
     L[5].f[0] = L[5].f[1];
     const double s_tot = s;
     for (int i = 0; i < 6; i++) lin_set(&L[i]);
-
-    const double dt = t[1] - t[0];
-    const double tp = 0.8;
-    for (int m = 0; m < len; m++) {
-        if (m < tp * (len - 1) / 2.0) tukey[m] = 1.0 / 2.0 * (1.0 + cos(WV_PI * (2.0 * m / (tp * (len - 1)) - 1.0)));
-        else if (m > (len - 1) * (1.0 - tp / 2.0))
-            tukey[m] = 1.0 / 2.0 * (1.0 + cos(WV_PI * (2.0 * m / (tp * (len - 1)) - 2.0 / tp + 1.0)));
-        else tukey[m] = 1.0;
-    }
-    for (int m = 0; m < fft_len; m++) {
-        filter1[m] = pow(1.0 + pow(m / (W->filt1_g * fft_len), W->filt1_n1), -W->filt1_n2);
-        filter2[m] = pow(1.0 + pow(m / (W->filt2_g * fft_len), W->filt2_n1), -W->filt2_n2);
-    }
-    go_rfft(u, len, Ur, Ui);
-    double Uf_max = 0.0;
-    for (int m = 0; m < fft_len; m++) Uf_max = fmax(Uf_max, (1.0 / dt * m / len) * sqrt(sq(Ur[m]) + sq(Ui[m])));
-
-    long long steps = 0;
-    s = 0.0;
-    double ds = W->ds;
-    const double x_last = lx[n - 1];
-    while (s < s_tot) {
-        double bc, bn;
-        if (fabs(lin_f(&L[3], s)) >= 0.5) bc = lin_f(&L[4], s);
-        else bc = lin_f(&L[5], s);
-        ds = W->ds / (WV_PI * Uf_max * bc);
-        ds = fmin(ds, 1.0);
-        ds = fmax(ds, M->ds_floor);
-        const double sn = fmin(s + ds, x_last);
-        if (fabs(lin_f(&L[3], sn)) >= 0.5) bn = lin_f(&L[4], sn);
-        else bn = lin_f(&L[5], sn);
-
-        for (int m = 0; m < len; m++) u_sqr[m] = sq(u[m]);
-        du_sqr[0] = (u_sqr[1] - u_sqr[0]) / (t[1] - t[0]);
-        for (int m = 1; m < len - 1; m++) du_sqr[m] = (u_sqr[m + 1] - u_sqr[m - 1]) / (t[m + 1] - t[m - 1]);
-        du_sqr[len - 1] = (u_sqr[len - 1] - u_sqr[len - 2]) / (t[len - 1] - t[len - 2]);
-        go_rfft(u_sqr, len, Ar, Ai);
-        for (int m = 0; m < len; m++) fft_time[m] = u[m] * du_sqr[m];
-        go_rfft(fft_time, len, Br, Bi);
-        for (int m = 0; m < len; m++) fft_time[m] = sq(du_sqr[m]);
-        go_rfft(fft_time, len, Cr, Ci);
-
-        /* caustic between s and s + ds: phase shift by pi / 2 (sph: strict
-         * bounds, 3D: inclusive ones) */
-        if (sph ? (s + ds < s_tot) : (s + ds <= s_tot)) {
-            const double d0 = lin_f(&L[3], s), d1 = lin_f(&L[3], s + ds);
-            if (sph ? (d0 * d1 < 0.0) : (d0 * d1 <= 0.0))
-                for (int m = 0; m < fft_len; m++) {
-                    double tmp = Ur[m];
-                    Ur[m] = -Ui[m];
-                    Ui[m] = tmp;
-                }
-        }
-        for (int m = 0; m < fft_len; m++) {
-            const double freq = 1.0 / dt * m / len;
-            const double pa = lin_f(&L[0], s), pb = lin_f(&L[1], s), pc = lin_f(&L[2], s);
-            const double cpt = M->c(M->m, pa, pb, pc);
-            const double absorp = pow(10.0, -cpt / c0 * M->sb(M->m, pa, pb, pc, freq) * ds / 10.0);
-            Ur[m] -= WV_PI * freq * (ds * (bc + bn) / 2.0 * Ai[m] + pow(ds, 2) / 2.0 * bc * bn * Bi[m] +
-                                     pow(ds, 3) / 8.0 * pow(bc, 2) * bn * Ci[m]) * filter1[m];
-            Ui[m] += WV_PI * freq * (ds * (bc + bn) / 2.0 * Ar[m] + pow(ds, 2) / 2.0 * bc * bn * Br[m] +
-                                     pow(ds, 3) / 8.0 * pow(bc, 2) * bn * Cr[m]) * filter1[m];
-            Ur[m] *= absorp * filter2[m];
-            Ui[m] *= absorp * filter2[m];
-        }
-        go_irfft(Ur, Ui, len, fft_time);
-        for (int m = 0; m < len; m++) u[m] = fft_time[m] * tukey[m] / len;
-        Uf_max = 0.0;
-        for (int m = 0; m < fft_len; m++) Uf_max = fmax(Uf_max, (1.0 / dt * m / len) * sqrt(sq(Ur[m]) + sq(Ui[m])));
-        s += ds;
-        steps++;
-    }
-    free(buf);
+    wv_pos_absorb q = {M, L, c0};
+    const long long steps = wv_evolve_core(W, &L[3], &L[4], &L[5], s_tot, lx[n - 1], M->ds_floor, sph, absorb_at_pos,
+                                           &q, t, u);
+    free(lx);
     if (n_steps) *n_steps = steps;
     return s_tot;
 }
@@ -540,6 +570,155 @@ long long go_wnl_run(const go_wv_med *M, const go_params *p_in, const go_wv_para
     free(rows);
     free(u);
     return started ? steps : -1;
+}
+
+/* ---- the 2D geometry (geoac.eqset.2d.cpp wnl_wvfrm, main.infraga.2d.cpp
+ * run_wnl_wvfrm): rows (r, z, ...) of 8 doubles; beta from the effective
+ * sound speed c + u cos(phi) + v sin(phi) without the ray-direction
+ * factors; the absorption scaled by the effective sound speed of the leg's
+ * last evolved row (the set-up loop's last c, which the step loop reuses)
+ * at the altitude of s; every prop_rk4 leg, ground or elevated reflection,
+ * counts toward bounces. ------------------------------------------------ */
+static double g2w_ceff(go_2d *g, double z)
+{
+    go_atmo *a = g->a;
+    return go_c(a, 0.0, 0.0, z) + go_u(a, 0.0, 0.0, z) * cos(g->phi) + go_v(a, 0.0, 0.0, z) * sin(g->phi);
+}
+
+typedef struct {
+    go_2d *g;
+    lin1 *alt;
+    double c_last, c0;
+} wv2_absorb;
+
+static double absorb_2d(void *ctx, double s, double freq)
+{
+    wv2_absorb *q = (wv2_absorb *)ctx;
+    return -q->c_last / q->c0 * go_sb_alpha(q->g->a, 0.0, 0.0, lin_f(q->alt, s), freq);
+}
+
+double go2_wnl_evolve(go_2d *g, const go_wv_params *W, const double *rows, int k1, int k2, double c0, double rho0,
+                      double D0, double p0, double *t, double *u, long long *n_steps)
+{
+    const int n = k2 - k1;
+    if (n_steps) *n_steps = 0;
+    if (n < 2) return 0.0;
+    double *lx = (double *)malloc(sizeof(double) * (size_t)n * 9);
+    lin1 L[4];   /* alt, D, beta, beta_caustic */
+    for (int i = 0; i < 4; i++) {
+        L[i].n = n; L[i].accel = 0;
+        L[i].x = lx;
+        L[i].f = lx + n * (1 + 2 * i);
+        L[i].sl = lx + n * (2 + 2 * i);
+    }
+    const double beta0 = 1.2 * p0 / ((rho0 * 1.0e3) * pow(c0 * 1.0e3, 2));
+    double s = 0.0, c = 0.0;
+    for (int k = 0; k < n; k++) {
+        const double *y = rows + 8 * (size_t)(k1 + k);
+        lx[k] = s;
+        L[0].f[k] = y[1];
+        L[1].f[k] = go2_jacobian(g, y);
+        c = g2w_ceff(g, y[1]);
+        const double rho = go_rho(g->a, 0.0, 0.0, y[1]);
+        L[2].f[k] = beta0 / c0 * sqrt(fabs((D0 * rho0 * c0) / (L[1].f[k] * rho * c)));
+        if (k < 1) L[3].f[k] = 0.0;
+        else L[3].f[k] = beta0 / c0 * sqrt(fabs((D0 * rho0 * c0) / ((L[1].f[k] - L[1].f[k - 1]) / (lx[k] - lx[k - 1]) *
+                                                                    rho * c)));
+        if ((k + 1) < n) {
+            const double *z = rows + 8 * (size_t)(k1 + k + 1);
+            s += sqrt(pow(z[0] - y[0], 2) + pow(z[1] - y[1], 2));
+        }
+    }
+    L[3].f[0] = L[3].f[1];
+    const double s_tot = s;
+    for (int i = 0; i < 4; i++) lin_set(&L[i]);
+    wv2_absorb q = {g, &L[0], c, c0};
+    const long long steps = wv_evolve_core(W, &L[1], &L[2], &L[3], s_tot, lx[n - 1], 1.0e-7, 1, absorb_2d, &q, t, u);
+    free(lx);
+    if (n_steps) *n_steps = steps;
+    return s_tot;
+}
+
+long long go2_wnl_run(go_2d *g, const go_wv_params *W, double src_alt, double incl_deg, double az_deg, int bounces,
+                      double freq, double wvfrm_ref, const double *t_src, const double *p_src, double *t_init,
+                      double *p_init, double *t_out, double *p_out, double arrival[9], int *ok)
+{
+    const int len = W->len;
+    const int length = (int)(g->s_max / g->ds_min);
+    double *rows = (double *)malloc(sizeof(double) * 8 * (size_t)length);
+    double *u = (double *)malloc(sizeof(double) * 2 * (size_t)len), *t = u + len;
+    for (int i = 0; i < len; i++) { t[i] = t_src[i]; u[i] = p_src[i]; }
+    g->calc_amp = 1;
+    g->theta = incl_deg * (WV_PI / 180.0);
+    g->phi = WV_PI / 2.0 - az_deg * (WV_PI / 180.0);
+    const double zg0 = go2_topo_z(g, 0.0);
+    const double z_src = src_alt > zg0 ? src_alt : zg0;          /* z_src = max(z_src, topo::z(r_src)) */
+    go2_set_initial(g, rows, 0.0, z_src);
+    double tt = 0.0, att = 0.0, zmax = 0.0, ray_length = 0.0;
+    double c0 = 0, rho0 = 0, D0 = 0, p0 = 0;
+    int started = 0, brk = 0, k = 0;
+    long long steps = 0;
+    for (int bnc = 0; bnc <= bounces; bnc++) {
+        k = go2_prop_rk4(g, rows, length, &brk);
+        for (int m = 1; m < k; m++) {
+            tt += go2_travel_time_seg(g, rows + 8 * (m - 1), rows + 8 * m);
+            att += go2_atten_seg(g, rows + 8 * (m - 1), rows + 8 * m, freq);
+            zmax = zmax > rows[8 * m + 1] ? zmax : rows[8 * m + 1];
+        }
+        if (!started) {
+            for (int kr = 0; kr < k; kr++) {
+                const double *a = rows + 8 * kr, *b = rows + 8 * (kr + 1);
+                ray_length += sqrt(pow(b[0] - a[0], 2) + pow(b[1] - a[1], 2));
+                if (ray_length >= wvfrm_ref) {
+                    c0 = g2w_ceff(g, b[1]);
+                    rho0 = go_rho(g->a, 0.0, 0.0, b[1]);
+                    D0 = go2_jacobian(g, b);
+                    double tref = 0.0;
+                    for (int q = 0; q < kr + 1; q++) tref += go2_travel_time_seg(g, rows + 8 * q, rows + 8 * (q + 1));
+                    for (int i = 0; i < len; i++) { t_init[i] = tref + t[i]; p_init[i] = u[i]; }
+                    for (int i = 0; i < len; i++) p0 = fmax(p0, fabs(u[i]));
+                    for (int i = 0; i < len; i++) u[i] /= p0;
+                    started = 1;
+                    long long ns = 0;
+                    ray_length = go2_wnl_evolve(g, W, rows, kr + 1, k, c0, rho0, D0, p0, t, u, &ns);
+                    steps += ns;
+                    break;
+                }
+            }
+        } else {
+            long long ns = 0;
+            ray_length += go2_wnl_evolve(g, W, rows, 0, k, c0, rho0, D0, p0, t, u, &ns);
+            steps += ns;
+        }
+        if (brk) break;
+        go2_set_refl(g, rows + 8 * (k - 2), rows + 8 * (k - 1), rows);
+    }
+    const double *yk = rows + 8 * (size_t)k;
+    memset(arrival, 0, sizeof(double) * 9);
+    arrival[0] = yk[0];
+    arrival[2] = tt;
+    arrival[3] = yk[0] / tt;
+    arrival[4] = zmax;
+    arrival[7] = 20.0 * log10(go2_amp(g, yk));
+    arrival[8] = -2.0 * att;
+    *ok = !brk;
+    const double rho = go_rho(g->a, 0.0, 0.0, yk[1]), ce = g2w_ceff(g, yk[1]);
+    for (int i = 0; i < len; i++) {
+        t_out[i] = tt + t[i];
+        p_out[i] = u[i] * p0 * sqrt(fabs((rho * ce) / (rho0 * c0) * D0 / go2_jacobian(g, yk)));
+    }
+    free(rows);
+    free(u);
+    return started ? steps : -1;
+}
+
+/* ctypes entry: go2_wnl_run over a 2D handle (go2_h_new). */
+long long go2_h_wnl_run(void *h, const go_wv_params *W, double src_alt, double incl_deg, double az_deg, int bounces,
+                        double freq, double wvfrm_ref, const double *t_src, const double *p_src, double *t_init,
+                        double *p_init, double *t_out, double *p_out, double arrival[9], int *ok)
+{
+    return go2_wnl_run((go_2d *)h, W, src_alt, incl_deg, az_deg, bounces, freq, wvfrm_ref, t_src, p_src, t_init,
+                       p_init, t_out, p_out, arrival, ok);
 }
 
 /* ctypes entry: go_wnl_run over a range-dependent handle. */

@@ -104,6 +104,19 @@ long long go_h_wnl_run(void *h, const go_params *p, const go_wv_params *W, const
                        const double *p_src, double *t_init, double *p_init, double *t_out, double *p_out,
                        double arrival[9], int *ok);
 
+/* The 2D geometry (geoac.eqset.2d.cpp wnl_wvfrm, main.infraga.2d.cpp
+ * run_wnl_wvfrm) over a go_2d (ga_oracle.h): source at range 0, altitude
+ * src_alt; arrival[0] = range, [2] time, [3] celerity, [4] turning height,
+ * [7] TL, [8] absorption. */
+double go2_wnl_evolve(go_2d *g, const go_wv_params *W, const double *rows, int k1, int k2, double c0, double rho0,
+                      double D0, double p0, double *t, double *u, long long *n_steps);
+long long go2_wnl_run(go_2d *g, const go_wv_params *W, double src_alt, double incl_deg, double az_deg, int bounces,
+                      double freq, double wvfrm_ref, const double *t_src, const double *p_src, double *t_init,
+                      double *p_init, double *t_out, double *p_out, double arrival[9], int *ok);
+long long go2_h_wnl_run(void *h, const go_wv_params *W, double src_alt, double incl_deg, double az_deg, int bounces,
+                        double freq, double wvfrm_ref, const double *t_src, const double *p_src, double *t_init,
+                        double *p_init, double *t_out, double *p_out, double arrival[9], int *ok);
+
 /* go_wnl_run over a range-dependent handle (ctypes). */
 long long go_h_rd_wnl_run(void *rd, const go_params *p, const go_wv_params *W, const double src_in[3],
                           double incl_deg, double az_deg, int bounces, double freq, double wvfrm_ref,
