@@ -234,6 +234,34 @@ GA_HD void refl2d(const Fan2DArgs &A, const double *yk2, const double *yk1, doub
 }
 
 // Per-lane 2D fan driver (main.infraga.2d.cpp:318-395).
+// The two waveform-row records of row y (PATH_WVROWS; ga_trace.hpp) in
+// 2D: the state (r, z, nu_r, nu_z), then the Jacobian determinant, the
+// effective sound speed c + u cos(phi) + v sin(phi) (also in the group-speed
+// slot), rho and the travel time dt of the segment into the row.
+GA_HD void wv2d_rows_put(const Fan2DArgs &A, PathLane &pl, const double *y, double cph, double sph, double c0,
+                         int item, int leg, int m, double dt, int &cur)
+{
+    double c, u, v, rho;
+    pt2d(A.M, y[1], cur, c, u, v, rho);
+    const double ce = c + u * cph + v * sph;
+    ga_path_pt a, b;
+    a.launch_idx = b.launch_idx = item;
+    a.leg = b.leg = leg;
+    a.step = b.step = m;
+    a.kind = PT_WV_STATE;
+    a.pos0 = y[0]; a.pos1 = y[1]; a.z = 0.0;
+    a.tl = y[2]; a.absorption = y[3]; a.time = 0.0;
+    b.kind = PT_WV_MEDIUM;
+    b.pos0 = jac2d(A, y, c0, cur);
+    b.pos1 = ce;
+    b.z = rho;
+    b.tl = ce;
+    b.absorption = dt;
+    b.time = 0.0;
+    path_put(A.path, pl, item, a);
+    path_put(A.path, pl, item, b);
+}
+
 template <int NEQ, class NextRay>
 GA_HD void lane_loop_2d(const Fan2DArgs &A, const Medium &M, NextRay next_ray)
 {
@@ -246,6 +274,10 @@ GA_HD void lane_loop_2d(const Fan2DArgs &A, const Medium &M, NextRay next_ray)
     path_reset(pl);
     const bool caus = NEQ == 8 && (A.path.flags & GA_PATH_CAUSTICS);
     const bool rays = (A.path.flags & GA_PATH_RAYS) != 0;
+    // waveform rows: every row of every leg, and (run_wnl_wvfrm's loop,
+    // main.infraga.2d.cpp) every leg counts toward bounces, elevated
+    // reflections included
+    const bool wv = NEQ == 8 && (A.path.flags & PATH_WVROWS);
 #pragma unroll
     for (int i = 0; i < NEQ; ++i) { y[i] = 0.0; yp[i] = 0.0; }
 
@@ -269,6 +301,7 @@ GA_HD void lane_loop_2d(const Fan2DArgs &A, const Medium &M, NextRay next_ray)
             s = 0.0; tt = 0.0; att = 0.0; dev = 0.0; zmax = 0.0;
             k = 0; bnc = 0; nst = 0; narr = 0;
             fetch = false;
+            if (wv) wv2d_rows_put(A, pl, y, cph, sph, c0, ray, 0, 0, 0.0, cur);
         }
         double y1[NEQ];
         s += rk4_2d<NEQ>(A, y, y1, cph, sph, c0, cur, tcur);
@@ -291,11 +324,15 @@ GA_HD void lane_loop_2d(const Fan2DArgs &A, const Medium &M, NextRay next_ray)
             const double ce = c + u * cph + v * sph;
             const double vp = -u * sph + v * cph;
             dev += vp / ce * 1.0 / sqrt(1.0 + 2.0 * vp / ce + (u * u + v * v) / (ce * ce)) * ds;
-            if (!end) {
+            if (!end || wv) {
                 const double zm = y[1] + (y1[1] - y[1]) / 2.0;
                 pt2d(M, zm, cur, c, u, v, rho);
-                tt += ds / (c + u * cph + v * sph);
-                att += sb_alpha<false>(zm, c, rho, A.freq, M.tweak, M.sbc) * ds;
+                const double dt = ds / (c + u * cph + v * sph);
+                if (!end) {
+                    tt += dt;
+                    att += sb_alpha<false>(zm, c, rho, A.freq, M.tweak, M.sbc) * ds;
+                }
+                if (wv) wv2d_rows_put(A, pl, y1, cph, sph, c0, ray, bnc, k, dt, cur);
             }
         }
         zmax = fmax(zmax, y[1]);
@@ -352,6 +389,9 @@ GA_HD void lane_loop_2d(const Fan2DArgs &A, const Medium &M, NextRay next_ray)
                 ++narr;
                 if (bnc == A.bounces) done = true;
                 else ++bnc;
+            } else if (wv) {
+                if (bnc == A.bounces) done = true;
+                else ++bnc;
             }
             if (!done) {
                 double yr[NEQ];
@@ -361,6 +401,7 @@ GA_HD void lane_loop_2d(const Fan2DArgs &A, const Medium &M, NextRay next_ray)
                 s = 0.0;
                 k = 0;
                 dev = 0.0;
+                if (wv) wv2d_rows_put(A, pl, y, cph, sph, c0, ray, bnc, 0, 0.0, cur);
             }
         }
         if (done) {

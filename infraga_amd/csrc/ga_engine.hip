@@ -2651,8 +2651,10 @@ int ga_wnl_wvfrm(ga_ctx *ctx, int atmo_id, const ga_prop_params *p_in, const ga_
     if (n > 0 && (!launches || !t_src || !p_src || !t_init || !p_init || !t_out || !p_out || !arrivals || !status))
         return fail(ctx, GA_E_ARG, "null argument");
     const int geom = p_in->geometry;
-    if (geom != GA_GEOM_SPH && geom != GA_GEOM_3D && geom != GA_GEOM_SPH_RNGDEP && geom != GA_GEOM_3D_RNGDEP)
-        return fail(ctx, GA_E_ARG, "waveforms run in the spherical and 3D geometries (not GA_GEOM_2D)");
+    if (geom != GA_GEOM_SPH && geom != GA_GEOM_3D && geom != GA_GEOM_SPH_RNGDEP && geom != GA_GEOM_3D_RNGDEP &&
+        geom != GA_GEOM_2D)
+        return fail(ctx, GA_E_ARG, "bad geometry %d", geom);
+    const bool two_d = geom == GA_GEOM_2D;
     const bool rdm = geom == GA_GEOM_SPH_RNGDEP || geom == GA_GEOM_3D_RNGDEP;
     if (p_in->bounces < 0) return fail(ctx, GA_E_ARG, "bounces must be >= 0");
     const int len = w->len;
@@ -2688,6 +2690,9 @@ int ga_wnl_wvfrm(ga_ctx *ctx, int atmo_id, const ga_prop_params *p_in, const ga_
     if (!d_l.alloc(nl * 2) || !d_o.alloc(nl * slots) || !d_c.alloc(nl) || !d_s.alloc(nl))
         return fail(ctx, GA_E_NOMEM, "device buffers for %d rays", n);
     HIPCHK(ctx, hipMemcpyAsync(d_l.p, launches, nl * 2 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    // the 2D trace fills only ground-return slots: an empty last slot marks
+    // a ray whose last leg did not end on the ground
+    HIPCHK(ctx, hipMemsetAsync(d_o.p, 0, nl * slots * sizeof(ga_arrival), ctx->stream));
     Events ev;
     if (!ev.ok) return fail(ctx, GA_E_HIP, "hipEventCreate failed");
 
@@ -2765,10 +2770,13 @@ int ga_wnl_wvfrm(ga_ctx *ctx, int atmo_id, const ga_prop_params *p_in, const ga_
                            (const int32_t *)pd.item.p, (const int32_t *)pd.seq.p, (const int32_t *)pd.fill.p, used,
                            (const int64_t *)d_off.p, n, d_rec.p);
     if (sph)
-        hipLaunchKernelGGL(wv_prep_kernel<true>, dim3(n), dim3(WV_PREP_T), 0, st, (const ga_path_pt *)d_rec.p,
+        hipLaunchKernelGGL((wv_prep_kernel<true>), dim3(n), dim3(WV_PREP_T), 0, st, (const ga_path_pt *)d_rec.p,
+                           (const int64_t *)d_off.p, n, slots, w->ref, p0, d_rows.p, d_lf.p, d_ln.p, d_ray.p);
+    else if (two_d)
+        hipLaunchKernelGGL((wv_prep_kernel<false, true>), dim3(n), dim3(WV_PREP_T), 0, st, (const ga_path_pt *)d_rec.p,
                            (const int64_t *)d_off.p, n, slots, w->ref, p0, d_rows.p, d_lf.p, d_ln.p, d_ray.p);
     else
-        hipLaunchKernelGGL(wv_prep_kernel<false>, dim3(n), dim3(WV_PREP_T), 0, st, (const ga_path_pt *)d_rec.p,
+        hipLaunchKernelGGL((wv_prep_kernel<false>), dim3(n), dim3(WV_PREP_T), 0, st, (const ga_path_pt *)d_rec.p,
                            (const int64_t *)d_off.p, n, slots, w->ref, p0, d_rows.p, d_lf.p, d_ln.p, d_ray.p);
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipEventRecord(ev.e1, st));
@@ -2794,8 +2802,10 @@ int ga_wnl_wvfrm(ga_ctx *ctx, int atmo_id, const ga_prop_params *p_in, const ga_
         int rc = GA_OK;
         if (!build_medium(ctx, it->second, &p, A.M, L, rc)) return rc;
         A.M.abs = nullptr;                        // per-bin absorption is evaluated directly
-        A.ds_floor = sph ? 1.0e-7 : 1.0e-5;
+        A.ds_floor = (sph || two_d) ? 1.0e-7 : 1.0e-5;
+        if (two_d) med = WV_2D;
     }
+    A.strict = (sph || two_d) ? 1 : 0;
     A.len = len;
     A.lg = lg;
     A.n_wv = n;
@@ -2813,7 +2823,9 @@ int ga_wnl_wvfrm(ga_ctx *ctx, int atmo_id, const ga_prop_params *p_in, const ga_
     A.ray = d_ray.p;
     A.steps = d_st.p;
     HIPCHK(ctx, hipEventRecord(ev2.e0, st));
-    if (med == WV_STRAT) {
+    if (med == WV_2D) {
+        hipLaunchKernelGGL((wv_evolve_kernel<false, WV_2D>), dim3(n), dim3(WV_T), 0, st, A);
+    } else if (med == WV_STRAT) {
         if (sph) hipLaunchKernelGGL((wv_evolve_kernel<true, WV_STRAT>), dim3(n), dim3(WV_T), 0, st, A);
         else hipLaunchKernelGGL((wv_evolve_kernel<false, WV_STRAT>), dim3(n), dim3(WV_T), 0, st, A);
     } else if (med == WV_RD_SPH) {
@@ -2841,7 +2853,7 @@ int ga_wnl_wvfrm(ga_ctx *ctx, int atmo_id, const ga_prop_params *p_in, const ga_
     for (int i = 0; i < n; ++i) {
         ray_steps += stp[(size_t)i];
         const WvRay &o = rays[(size_t)i];
-        const bool ok = cnt[(size_t)i] == slots;
+        const bool ok = two_d ? all[(size_t)i * slots + (slots - 1)].time > 0.0 : cnt[(size_t)i] == slots;
         status[i] = !o.started ? -1 : (ok ? 1 : 0);
         if (ok) arrivals[i] = all[(size_t)i * slots + (slots - 1)];
         else {

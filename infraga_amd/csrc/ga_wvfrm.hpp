@@ -60,12 +60,13 @@ constexpr int WV_MAX_SLOTS = 256;    // legs per waveform (bounces + 1)
 
 // The medium the absorption is evaluated in: a stratified profile, or a
 // range-dependent one on the spherical or the Cartesian grid.
-constexpr int WV_STRAT = 0, WV_RD_SPH = 1, WV_RD_CART = 2;
+constexpr int WV_STRAT = 0, WV_RD_SPH = 1, WV_RD_CART = 2, WV_2D = 3;
 
 struct WvArgs {
     Medium M;                    // WV_STRAT
     RdMed RM;                    // WV_RD_SPH, WV_RD_CART
     double ds_floor;             // smallest evolution step [km]
+    int strict;                  // caustic test: sph and 2D strict, 3D inclusive
     int len, lg, n_wv;
     double ds_wv;                // wvfrm_ds
     const double *tw;            // [len/2][2]: cos, sin of -2 pi j / len
@@ -284,21 +285,20 @@ __global__ void __launch_bounds__(WV_T) wv_evolve_kernel(WvArgs A)
                 const double sn = fmin(s + ds, x_last);
                 const double bn = wv_beta(R, wv_seek(R, n, ks, sn), sn);
                 bool caustic = false;
-                if (SPH ? (s + ds < s_tot) : (s + ds <= s_tot)) {
+                if (A.strict ? (s + ds < s_tot) : (s + ds <= s_tot)) {
                     const double d0 = wv_lin(R, ks, WV_F_D, s);
                     const double d1 = wv_lin(R, wv_seek(R, n, ks, s + ds), WV_F_D, s + ds);
-                    caustic = SPH ? (d0 * d1 < 0.0) : (d0 * d1 <= 0.0);
+                    caustic = A.strict ? (d0 * d1 < 0.0) : (d0 * d1 <= 0.0);
                 }
                 // absorption point factors at the ray position of s (the last
                 // wave: wave 0 holds the extra Nyquist bin)
                 if (tid >= WV_T - 64) {
                     const double pa = wv_lin(R, ks, WV_F_POS, s), pb = wv_lin(R, ks, WV_F_POS + 1, s);
                     const double pc = wv_lin(R, ks, WV_F_POS + 2, s);
-                    (void)pb;
                     double z, cpt, rho;
                     const double *sbc;
-                    if (MED == WV_STRAT) {
-                        z = SPH ? pa - M.r0 : pc;
+                    if (MED == WV_STRAT || MED == WV_2D) {
+                        z = MED == WV_2D ? pb : (SPH ? pa - M.r0 : pc);
                         const SegPt sp = seg_point(M, clampz(M, z), curz);
                         cpt = herm_f(sp, 1);
                         rho = herm_f(sp, 7);
@@ -363,8 +363,11 @@ __global__ void __launch_bounds__(WV_T) wv_evolve_kernel(WvArgs A)
                 const double h3 = (ds * ds * ds) / 8.0 * (bc * bc) * bn;
                 double Pr[WV_BPT], Pi[WV_BPT], gf[WV_BPT];
                 {
-                    const double cc = -pt.cpt / c0;
-                    const double tweak = MED == WV_STRAT ? M.tweak : A.RM.tweak;
+                    // 2D: the effective sound speed of the leg's last evolved
+                    // row, which geoac.eqset.2d.cpp's step loop reuses from its
+                    // set-up (kept in the first row's pad)
+                    const double cc = -(MED == WV_2D ? R[WV_ROW - 1] : pt.cpt) / c0;
+                    const double tweak = (MED == WV_STRAT || MED == WV_2D) ? M.tweak : A.RM.tweak;
 #pragma unroll
                     for (int j = 0; j < WV_BPT; ++j) {
                         const int b = tid + j * WV_T;
@@ -464,7 +467,7 @@ __global__ void __launch_bounds__(WV_T) wv_evolve_kernel(WvArgs A)
 // on in its leg, whole legs after it, rows up to K-1), whose arc length,
 // position and D it writes; then all threads fill beta and the caustic beta
 // of those rows (wnl_wvfrm's set-up, geoac.eqset.sph.strat.cpp:903-941).
-template <bool SPH>
+template <bool SPH, bool TWO_D = false>
 __global__ void __launch_bounds__(WV_PREP_T) wv_prep_kernel(const ga_path_pt *rec, const int64_t *off, int n_wv,
                                                            int slots, double ref, double p0, double *rows,
                                                            int64_t *leg_first, int32_t *leg_n, WvRay *ray)
@@ -558,10 +561,18 @@ __global__ void __launch_bounds__(WV_PREP_T) wv_prep_kernel(const ga_path_pt *re
             const int64_t kl = nr - 1;                   // row K of the last leg
             const ga_path_pt &md = P[2 * kl + 1];
             const double c = md.pos1, rho = md.z, nuk = nu(kl);
-            R.scale = sqrt(fabs((rho * (c * c * c) * nuk) / (R.rho0 * (R.c0 * R.c0 * R.c0) * R.nu0) * (R.cg0 * R.D0) /
-                                (md.tl * md.pos0)));
             const double c_src = P[1].pos1;
-            s_beta0 = 1.2 * p0 / ((R.rho0 * 1.0e3) * ((R.c0 * 1.0e3) * (R.c0 * 1.0e3))) * (R.nu0 * R.c0) / (R.cg0 * c_src);
+            if (TWO_D) {
+                // main.infraga.2d.cpp run_wnl_wvfrm: the effective sound speed,
+                // no ray-direction factors
+                R.scale = sqrt(fabs((rho * c) / (R.rho0 * R.c0) * R.D0 / md.pos0));
+                s_beta0 = 1.2 * p0 / ((R.rho0 * 1.0e3) * ((R.c0 * 1.0e3) * (R.c0 * 1.0e3)));
+            } else {
+                R.scale = sqrt(fabs((rho * (c * c * c) * nuk) / (R.rho0 * (R.c0 * R.c0 * R.c0) * R.nu0) *
+                                    (R.cg0 * R.D0) / (md.tl * md.pos0)));
+                s_beta0 = 1.2 * p0 / ((R.rho0 * 1.0e3) * ((R.c0 * 1.0e3) * (R.c0 * 1.0e3))) * (R.nu0 * R.c0) /
+                          (R.cg0 * c_src);
+            }
         }
         s_ray = R;
         ray[w] = R;
@@ -585,13 +596,25 @@ __global__ void __launch_bounds__(WV_PREP_T) wv_prep_kernel(const ga_path_pt *re
             const double nuq = sqrt(st.tl * st.tl + st.absorption * st.absorption + st.time * st.time);
             double *o = rows + (size_t)m * WV_ROW;
             const double Dq = o[WV_F_D];
-            o[WV_F_BETA] = beta0 * sqrt(fabs(num / (Dq * rho * (c * c * c)) * (c * (nuq * nuq * nuq)) / den0));
-            if (q > 0) {
-                const double *op = o - WV_ROW;
-                o[WV_F_BETAC] = beta0 * sqrt(fabs(num / ((Dq - op[WV_F_D]) / (o[WV_F_S] - op[WV_F_S]) * rho * (c * c * c)) *
-                                                  (c * (nuq * nuq * nuq)) / den0));
+            if (TWO_D) {
+                // geoac.eqset.2d.cpp wnl_wvfrm: beta0 / c0 sqrt(D0 rho0 c0 / (D rho c))
+                const double n2 = R.D0 * R.rho0 * R.c0;
+                o[WV_F_BETA] = beta0 / R.c0 * sqrt(fabs(n2 / (Dq * rho * c)));
+                if (q > 0) {
+                    const double *op = o - WV_ROW;
+                    o[WV_F_BETAC] = beta0 / R.c0 * sqrt(fabs(n2 / ((Dq - op[WV_F_D]) / (o[WV_F_S] - op[WV_F_S]) * rho * c)));
+                }
+            } else {
+                o[WV_F_BETA] = beta0 * sqrt(fabs(num / (Dq * rho * (c * c * c)) * (c * (nuq * nuq * nuq)) / den0));
+                if (q > 0) {
+                    const double *op = o - WV_ROW;
+                    o[WV_F_BETAC] = beta0 * sqrt(fabs(num / ((Dq - op[WV_F_D]) / (o[WV_F_S] - op[WV_F_S]) * rho *
+                                                             (c * c * c)) * (c * (nuq * nuq * nuq)) / den0));
+                }
             }
-            o[WV_ROW - 1] = 0.0;
+            // the first row's pad keeps the effective sound speed of the
+            // leg's last evolved row (2D absorption scale)
+            if (q == 0) o[WV_ROW - 1] = rec[2 * (f + nn - 1) + 1].pos1;
         }
     }
     __syncthreads();

@@ -2560,6 +2560,8 @@ int go2_fan(go_2d *g, const double *launches, int n_launch, int bounces, double 
 }
 
 double go2_topo_z(go_2d *g, double r) { return g2_topo_z(g, r); }
+double go2_ceff(go_2d *g, double z) { return g2_ceff(g, z); }
+double go2_rho(go_2d *g, double z) { return g2_pt(g, &g->a->rho, z); }
 
 void *go2_h_new(void *atmo_h) { go_2d *g = (go_2d *)malloc(sizeof(go_2d)); go2_init(g, (go_atmo *)atmo_h); return g; }
 void go2_h_free(void *h) { if (h) { go2_free((go_2d *)h); free(h); } }

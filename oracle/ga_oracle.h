@@ -313,6 +313,8 @@ double go2_jacobian(go_2d *g, const double *y);
 double go2_amp(go_2d *g, const double *y);
 double go2_est_dev(go_2d *g, const double *rows, int k);
 double go2_topo_z(go_2d *g, double r);   /* topo::z(r): terrain, or the ground z0 */
+double go2_ceff(go_2d *g, double z);     /* c + u cos(phi) + v sin(phi), no boundary-layer scaling in 2D */
+double go2_rho(go_2d *g, double z);
 int    go2_fan(go_2d *g, const double *launches, int n_launch, int bounces, double src_alt, double freq, int flags,
                go_arrival *out, int out_cap, int *n_out, go_path_pt *pts, long long pts_cap, long long *n_pts,
                long long *ray_steps);

@@ -579,12 +579,6 @@ long long go_wnl_run(const go_wv_med *M, const go_params *p_in, const go_wv_para
  * last evolved row (the set-up loop's last c, which the step loop reuses)
  * at the altitude of s; every prop_rk4 leg, ground or elevated reflection,
  * counts toward bounces. ------------------------------------------------ */
-static double g2w_ceff(go_2d *g, double z)
-{
-    go_atmo *a = g->a;
-    return go_c(a, 0.0, 0.0, z) + go_u(a, 0.0, 0.0, z) * cos(g->phi) + go_v(a, 0.0, 0.0, z) * sin(g->phi);
-}
-
 typedef struct {
     go_2d *g;
     lin1 *alt;
@@ -618,8 +612,8 @@ double go2_wnl_evolve(go_2d *g, const go_wv_params *W, const double *rows, int k
         lx[k] = s;
         L[0].f[k] = y[1];
         L[1].f[k] = go2_jacobian(g, y);
-        c = g2w_ceff(g, y[1]);
-        const double rho = go_rho(g->a, 0.0, 0.0, y[1]);
+        c = go2_ceff(g, y[1]);
+        const double rho = go2_rho(g, y[1]);
         L[2].f[k] = beta0 / c0 * sqrt(fabs((D0 * rho0 * c0) / (L[1].f[k] * rho * c)));
         if (k < 1) L[3].f[k] = 0.0;
         else L[3].f[k] = beta0 / c0 * sqrt(fabs((D0 * rho0 * c0) / ((L[1].f[k] - L[1].f[k - 1]) / (lx[k] - lx[k - 1]) *
@@ -670,8 +664,8 @@ long long go2_wnl_run(go_2d *g, const go_wv_params *W, double src_alt, double in
                 const double *a = rows + 8 * kr, *b = rows + 8 * (kr + 1);
                 ray_length += sqrt(pow(b[0] - a[0], 2) + pow(b[1] - a[1], 2));
                 if (ray_length >= wvfrm_ref) {
-                    c0 = g2w_ceff(g, b[1]);
-                    rho0 = go_rho(g->a, 0.0, 0.0, b[1]);
+                    c0 = go2_ceff(g, b[1]);
+                    rho0 = go2_rho(g, b[1]);
                     D0 = go2_jacobian(g, b);
                     double tref = 0.0;
                     for (int q = 0; q < kr + 1; q++) tref += go2_travel_time_seg(g, rows + 8 * q, rows + 8 * (q + 1));
@@ -702,7 +696,7 @@ long long go2_wnl_run(go_2d *g, const go_wv_params *W, double src_alt, double in
     arrival[7] = 20.0 * log10(go2_amp(g, yk));
     arrival[8] = -2.0 * att;
     *ok = !brk;
-    const double rho = go_rho(g->a, 0.0, 0.0, yk[1]), ce = g2w_ceff(g, yk[1]);
+    const double rho = go2_rho(g, yk[1]), ce = go2_ceff(g, yk[1]);
     for (int i = 0; i < len; i++) {
         t_out[i] = tt + t[i];
         p_out[i] = u[i] * p0 * sqrt(fabs((rho * ce) / (rho0 * c0) * D0 / go2_jacobian(g, yk)));
