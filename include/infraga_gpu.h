@@ -357,7 +357,9 @@ typedef struct {
 } ga_wvfrm_stats;
 
 /* -wnl_wvfrm for n launches (incl, az) [deg] from p->src over a stratified
- * atmosphere (GA_GEOM_SPH, GA_GEOM_3D) or a range-dependent one
+ * atmosphere (GA_GEOM_SPH, GA_GEOM_3D; GA_GEOM_2D, geoac.eqset.2d.cpp and
+ * src/main.infraga.2d.cpp run_wnl_wvfrm, every leg counting toward
+ * bounces) or a range-dependent one
  * (GA_GEOM_SPH_RNGDEP, GA_GEOM_3D_RNGDEP, also a stratified profile over
  * terrain; geoac.eqset.{sph,3d}.rngdep.cpp wnl_wvfrm,
  * src/main.infraga.sph.rngdep.cpp:1108-1410,

@@ -1769,6 +1769,181 @@ int run_wnl_wvfrm(int argc, char **argv)
     return 0;
 }
 
+// infraga-2d -wnl_wvfrm (run_wnl_wvfrm, main.infraga.2d.cpp): the waveform
+// along one ray of the 2D geometry, every leg (ground or elevated
+// reflection) counted toward bounces; <id>.wvfrm_init.dat and
+// <id>.wvfrm_out.dat.  write_ray=true is refused (the path file would follow
+// the -prop leg count), as are intermediate dumps.
+int run_wnl_wvfrm_2d(int argc, char **argv)
+{
+    double incl = 15.0, az = 90.0, z_src = 0.0, freq = 0.1, wvfrm_ref = 1.0, wvfrm_out_step = 1.0e10;
+    double wp0 = 10.0, wt0 = 0.1, walpha = 4.0, z_grnd = NAN;
+    int bounces = 0, n_rise = 10, n_period = 40;
+    long len = 8192;
+    bool reverse_winds = false, custom_id = false;
+    const char *wvfrm_file = "none", *wvfrm_opt = "impulse", *prof_format = "zTuvdp", *output_id = nullptr;
+    const char *topo_file = nullptr;
+    ga_wvfrm_params w{};
+    w.ds = 0.5;
+    w.filt1_g = 0.5; w.filt1_n1 = 4.0; w.filt1_n2 = 0.5;
+    w.filt2_g = 0.75; w.filt2_n1 = 2.0; w.filt2_n2 = 1.0;
+    ga_prop_params p;
+    ga_prop_params_default(&p, GA_GEOM_2D);
+    for (int i = 3; i < argc; i++) {
+        const char *a = argv[i], *v = nullptr;
+        if (key(a, "inclination=", v)) incl = std::atof(v);
+        else if (key(a, "azimuth=", v)) az = std::atof(v);
+        else if (key(a, "bounces=", v)) bounces = std::max(0, std::atoi(v));
+        else if (key(a, "src_alt=", v)) z_src = std::atof(v);
+        else if (key(a, "write_ray=", v)) {
+            if (str2bool(v)) {
+                std::cerr << GA_CLI_NAME << ": write_ray with -wnl_wvfrm is not supported by this engine\n";
+                return 2;
+            }
+        }
+        else if (key(a, "wvfrm_file=", v)) wvfrm_file = v;
+        else if (key(a, "wvfrm_opt=", v)) wvfrm_opt = v;
+        else if (key(a, "wvfrm_p0=", v)) wp0 = std::atof(v);
+        else if (key(a, "wvfrm_t0=", v)) wt0 = std::atof(v);
+        else if (key(a, "wvfrm_alpha=", v)) walpha = std::atof(v);
+        else if (key(a, "wvfrm_ref=", v)) wvfrm_ref = std::atof(v);
+        else if (key(a, "wvfrm_out_step=", v)) wvfrm_out_step = std::atof(v);
+        else if (key(a, "wvfrm_ds=", v)) w.ds = std::atof(v);
+        else if (key(a, "wvfrm_N_rise=", v)) n_rise = (int)std::atof(std::strlen(a) >= 15 ? a + 15 : "");
+        else if (key(a, "wvfrm_N_period=", v)) n_period = (int)std::atof(v);
+        else if (key(a, "wvfrm_len=", v)) len = (long)std::atof(v);
+        else if (key(a, "wvfrm_filt1_g=", v)) w.filt1_g = std::atof(v);
+        else if (key(a, "wvfrm_filt1_n1=", v)) w.filt1_n1 = std::atof(v);
+        else if (key(a, "wvfrm_filt1_n2=", v)) w.filt1_n2 = std::atof(v);
+        else if (key(a, "wvfrm_filt2_g=", v)) w.filt2_g = std::atof(v);
+        else if (key(a, "wvfrm_filt2_n1=", v)) w.filt2_n1 = std::atof(v);
+        else if (key(a, "wvfrm_filt2_n2=", v)) w.filt2_n2 = std::atof(v);
+        else if (key(a, "freq=", v)) freq = std::atof(v);
+        else if (key(a, "abs_coeff=", v)) p.abs_coeff = std::max(0.0, std::atof(v));
+        else if (key2(a, "max_alt=", "alt_max=", v)) p.alt_max = std::atof(v);
+        else if (key2(a, "max_rng=", "rng_max=", v)) p.rng_max = std::atof(v);
+        else if (key2(a, "min_ds=", "ds_min=", v)) p.ds_min = std::atof(v);
+        else if (key2(a, "max_ds=", "ds_max=", v)) p.ds_max = std::atof(v);
+        else if (key2(a, "max_s=", "s_max=", v)) p.s_max = std::atof(v);
+        else if (key2(a, "refl_alt=", "alt_refl=", v)) p.refl_alt = std::atof(v);
+        else if (key(a, "write_atmo=", v)) { /* profile dump: not produced */ }
+        else if (key(a, "prof_format=", v)) prof_format = v;
+        else if (key(a, "reverse_winds=", v)) reverse_winds = str2bool(v);
+        else if (key(a, "output_id=", v)) { custom_id = true; output_id = v; }
+        else if (key(a, "z_grnd=", v)) z_grnd = std::atof(v);
+        else if (key(a, "topo_file=", v)) topo_file = v;
+        else {
+            std::cerr << GA_CLI_NAME << ": unrecognized parameter entry: " << a << '\n';
+            return 2;
+        }
+    }
+    if (wvfrm_out_step < 1.0e4) {
+        std::cerr << GA_CLI_NAME << ": wvfrm_out_step (intermediate waveform files) is not supported by this engine\n";
+        return 2;
+    }
+    std::vector<double> t, ps;
+    const bool from_file = std::strncmp(wvfrm_file, "none", 4) != 0;
+    if (from_file) {
+        if (!wv_load(wvfrm_file, t, ps)) {
+            std::cerr << GA_CLI_NAME << ": cannot read waveform file " << wvfrm_file << '\n';
+            return 1;
+        }
+        len = (long)t.size();
+    } else {
+        if (len < 1 || len > (1L << 24)) {
+            std::cerr << GA_CLI_NAME << ": bad wvfrm_len\n";
+            return 2;
+        }
+        wv_build(wvfrm_opt, wp0, wt0, walpha, (int)len, n_rise, n_period, t, ps);
+    }
+    w.len = (int32_t)len;
+    w.ref = wvfrm_ref;
+
+    ga_ctx *ctx = nullptr;
+    if (ga_ctx_create(&ctx, 0) != GA_OK) return die(nullptr, "no usable GPU");
+    int aid = 0;
+    if (ga_atmo_load_file(ctx, argv[2], prof_format, reverse_winds ? 1 : 0, &aid) != GA_OK)
+        return die(ctx, "cannot load the atmosphere");
+    if (topo_file) {
+        std::ifstream tf(topo_file);
+        if (!tf) return die(ctx, "cannot read the terrain file");
+        std::vector<double> r, zt;
+        std::string line;
+        while (std::getline(tf, line)) {
+            if (line.find('#') == 0) continue;
+            double a = 0.0, b = 0.0;
+            if (std::sscanf(line.c_str(), "%lf %lf", &a, &b) == 2) { r.push_back(a); zt.push_back(b); }
+        }
+        if (ga_topo_load_2d(ctx, aid, r.data(), zt.data(), (int)r.size()) != GA_OK) return die(ctx, "terrain");
+    }
+    const double zb = profile_bottom(argv[2]);
+    const double z0 = (!topo_file && !std::isnan(z_grnd)) ? z_grnd : zb;
+    p.z_grnd = topo_file ? NAN : z0;
+    double t0[3];
+    if (ga_topo_eval_2d(ctx, aid, 0.0, t0) != GA_OK) return die(ctx, "terrain");
+    z_src = std::max(z_src, topo_file ? t0[0] : z0);          // z_src = max(z_src, topo::z(r_src))
+    p.src[2] = z_src;
+    p.bounces = bounces;
+    p.freq = freq;
+    p.calc_amp = 1;
+    const std::string id = custom_id ? std::string(output_id) : strip_ext(argv[2]);
+    const double launch[2] = {incl, az};
+    std::vector<double> ti((size_t)len), pi_((size_t)len), to((size_t)len), po((size_t)len);
+    ga_arrival A{};
+    int32_t status = 0;
+    int64_t steps = 0;
+    ga_wvfrm_stats st{};
+    if (ga_wnl_wvfrm(ctx, aid, &p, &w, launch, 1, t.data(), ps.data(), ti.data(), pi_.data(), to.data(), po.data(),
+                     &A, &status, &steps, &st) != GA_OK)
+        return die(ctx, "waveform failed");
+    if (status < 0) {
+        std::cout << "\n\t\tRay path does not reach the waveform reference location (" << wvfrm_ref << " km).\n\n";
+        ga_ctx_destroy(ctx);
+        return 1;
+    }
+    {
+        std::ofstream f(id + ".wvfrm_init.dat");
+        f << "# t [sec]" << '\t' << "p(t) [Pa]" << '\n';
+        for (long n = 0; n < len; ++n) f << std::setprecision(8) << ti[(size_t)n] << '\t' << pi_[(size_t)n] << '\n';
+    }
+    if (status == 1) {
+        std::cout << '\n' << '\t' << "Arrival summary:" << '\n';
+        std::cout << "\t\trange [km] = " << A.pos0 << '\n';
+        std::cout << "\t\ttime [s] = " << A.time << '\n';
+        std::cout << "\t\tcelerity [km/s] = " << A.cel << '\n';
+        std::cout << "\t\tturning height [km] = " << A.turn_ht << '\n';
+        std::cout << "\t\tarrival inclination [deg] = " << A.incl_arr << '\n';
+        std::cout << "\t\ttrans. coeff. [dB] = " << A.tl << '\n';
+        std::cout << "\t\tabsorption [dB] = " << A.absorption << '\n' << '\n';
+    } else {
+        std::cout << "\n\t\tRay path does not return to the ground.\n\n";
+    }
+    std::ofstream f(id + ".wvfrm_out.dat");
+    f << "# " << GA_CLI_NAME << " -wnl_wvfrm summary:" << '\n';
+    f << "#" << '\t' << "profile: " << argv[2] << '\n';
+    f << "#" << '\t' << "inclination: " << (incl * (Pi / 180.0)) * (180.0 / Pi) << '\n';
+    f << "#" << '\t' << "azimuth: " << (Pi / 2.0 - az * (Pi / 180.0)) * (180.0 / Pi) << '\n';
+    f << "#" << '\t' << "bounces: " << bounces << '\n';
+    f << "#" << '\t' << "source location: " << 0.0 << ", " << z_src << '\n';
+    if (topo_file) f << "#" << '\t' << "topo file:" << topo_file << '\n';
+    else f << "#" << '\t' << "ground elevation: " << z0 << '\n';
+    f << "#" << '\t' << "S&B atten coeff: " << p.abs_coeff << '\n';
+    if (from_file) f << "#" << '\t' << "waveform source file: " << wvfrm_file << '\n';
+    else {
+        f << "#" << '\t' << "waveform option: " << wvfrm_opt << '\n';
+        f << "#" << '\t' << "waveform p0: " << wp0 << '\n';
+        f << "#" << '\t' << "waveform t0: " << wt0 << '\n';
+        f << "#" << '\t' << "waveform alpha: " << walpha << '\n';
+    }
+    f << "#" << '\t' << "waveform reference location: " << wvfrm_ref << '\n' << '\n';
+    f << "# t [sec]" << '\t' << "p(t) [Pa]" << '\n';
+    for (long n = 0; n < len; ++n) f << std::setprecision(8) << to[(size_t)n] << '\t' << po[(size_t)n] << '\n';
+    std::cout << GA_CLI_NAME << ": waveform of " << len << " samples, " << steps << " evolution steps, "
+              << st.ray_steps << " ray-steps, " << st.trace_ms + st.evolve_ms << " ms on the GPU\n";
+    ga_ctx_destroy(ctx);
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char **argv)
@@ -1778,6 +1953,8 @@ int main(int argc, char **argv)
     if (argc < 3) { usage(); return argc < 2 ? 0 : 2; }
     if (GA_CLI_GEOM == GA_GEOM_2D) {
         if (std::strncmp(argv[1], "-prop", 5) == 0) return run_prop_2d(argc, argv);
+        if (std::strncmp(argv[1], "-wnl_wvfrm", 10) == 0 || std::strncmp(argv[1], "-w", 2) == 0)
+            return run_wnl_wvfrm_2d(argc, argv);
         std::cerr << GA_CLI_NAME << ": option " << argv[1] << " is not supported by this engine\n";
         return 2;
     }

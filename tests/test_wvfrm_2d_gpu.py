@@ -50,3 +50,32 @@ def test_2d_waveforms_match_oracle(engine, oracle, pert_units, name, refl, topo)
     st = g["stats"]
     print(f"\n{name}: status {list(g['status'])}, {st.wv_steps} evolution steps, worst |dp|/peak {worst:.2e}")
     assert (g["status"] == 1).all()
+
+
+def test_cli_2d_wnl_wvfrm(engine, oracle, pert_units, tmp_path):
+    """infraga-2d -wnl_wvfrm: wvfrm_out.dat equal to the oracle's 2D waveform
+    at the file's 8 digits, the writer's header; write_ray refused."""
+    import os
+    import subprocess
+    from ga_oracle_py import Medium2D, WvParams, wv_build, wv_run_2d
+    z, c, u, v, rho = pert_units
+    met = os.path.join(str(tmp_path), "pert.met")
+    with open(met, "w") as f:
+        for row in zip(z, c * 1e3, u * 1e3, v * 1e3, rho):
+            f.write(" ".join("%.17g" % x for x in row) + "\n")
+    units = (z, (c * 1e3) / 1000.0, (u * 1e3) / 1000.0, (v * 1e3) / 1000.0, rho)
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "infraga_amd", "bin", "infraga-2d")
+    args = [exe, "-wnl_wvfrm", met, "prof_format=zcuvd", "inclination=14.0", "azimuth=-90.0", "bounces=1",
+            "wvfrm_len=1024", "wvfrm_p0=500.0"]
+    r = subprocess.run(args, capture_output=True, text=True, cwd=tmp_path, timeout=300)
+    assert r.returncode == 0, r.stderr
+    t, ps, _ = wv_build(oracle, "impulse", p0=500.0, t0=0.1, alpha=4.0, length=1024)
+    ref = wv_run_2d(Medium2D(oracle, *units), WvParams.default(len=1024), 14.0, -90.0, 1, t, ps)
+    text = open(os.path.join(str(tmp_path), "pert.wvfrm_out.dat")).read()
+    assert text.startswith("# infraga-2d -wnl_wvfrm summary:\n") and "#\tsource location: 0, 0\n" in text
+    out = np.array([[float(x) for x in l.split("\t")] for l in text.splitlines() if l.strip() and not l.startswith("#")])
+    np.testing.assert_allclose(out[:, 0], ref["t_out"], rtol=1e-7)
+    peak = np.abs(ref["p_out"]).max()
+    np.testing.assert_allclose(out[:, 1], ref["p_out"], rtol=1e-7, atol=1e-7 * peak)
+    r = subprocess.run(args + ["write_ray=true"], capture_output=True, text=True, cwd=tmp_path, timeout=300)
+    assert r.returncode == 2
