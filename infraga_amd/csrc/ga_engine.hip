@@ -924,8 +924,6 @@ int rd_setup(ga_ctx *ctx, FanBufs &fb, int rd_id, const ga_prop_params *p, RdFan
     if (cart != R.cart)
         return fail(ctx, GA_E_ARG, cart ? "GA_GEOM_3D_RNGDEP needs a ga_rngdep_load_xy atmosphere"
                                         : "a ga_rngdep_load_xy atmosphere needs GA_GEOM_3D_RNGDEP");
-    if (cart && p->topo_use_blw)
-        return fail(ctx, GA_E_ARG, "topo_use_BLw is not supported in the 3D range-dependent geometry");
     if (p->bounces < 0) return fail(ctx, GA_E_ARG, "bounces must be >= 0");
     if (!(p->ds_min > 0.0) || !(p->ds_max >= p->ds_min) || !(p->s_max > 0.0))
         return fail(ctx, GA_E_ARG, "invalid step parameters ds_min=%g ds_max=%g s_max=%g", p->ds_min, p->ds_max, p->s_max);
@@ -935,6 +933,10 @@ int rd_setup(ga_ctx *ctx, FanBufs &fb, int rd_id, const ga_prop_params *p, RdFan
     M.t = topo_grid_of(R);
     M.is_topo = R.d_ttab ? 1 : 0;
     M.use_blw = p->topo_use_blw ? 1 : 0;
+    M.cart = cart ? 1 : 0;
+    // topo::bndlyr_param / topo::vert_wind_grad (atmo_state.sph.rngdep.cpp:95-96,
+    // atmo_state.3d.rngdep.cpp:81-82, atmo_state.3d.strat.cpp:85-86)
+    M.bl_sg = cart ? 0.1 / 1.0 : 0.1 / 0.5;
     M.strat = R.strat ? 1 : 0;
     M.r0 = cart ? 0.0 : p->r_earth;
     M.tweak = std::max(0.0, p->abs_coeff);
@@ -1039,12 +1041,16 @@ int rd_fan_enqueue(ga_ctx *ctx, FanBufs &fb, int rd_id, const ga_prop_params *p,
     // [amp][blw], write mode
     const RdKern pkerns[2][2] = {{rd_fan_kernel<6, false, false, true>, rd_fan_kernel<6, false, true, true>},
                                  {rd_fan_kernel<18, false, false, true>, rd_fan_kernel<18, false, true, true>}};
-    // 3D Cartesian: [amp][one_shot], write mode [amp]
-    const RdKern ckerns[2][2] = {{rd_fan_kernel<6, false, false, false, true>, rd_fan_kernel<6, true, false, false, true>},
-                                 {rd_fan_kernel<18, false, false, false, true>, rd_fan_kernel<18, true, false, false, true>}};
-    const RdKern cpkerns[2] = {rd_fan_kernel<6, false, false, true, true>, rd_fan_kernel<18, false, false, true, true>};
+    // 3D Cartesian: [amp][one_shot][blw], write mode [amp][blw]
+    const RdKern ckerns[2][2][2] = {
+        {{rd_fan_kernel<6, false, false, false, true>, rd_fan_kernel<6, false, true, false, true>},
+         {rd_fan_kernel<6, true, false, false, true>, rd_fan_kernel<6, true, true, false, true>}},
+        {{rd_fan_kernel<18, false, false, false, true>, rd_fan_kernel<18, false, true, false, true>},
+         {rd_fan_kernel<18, true, false, false, true>, rd_fan_kernel<18, true, true, false, true>}}};
+    const RdKern cpkerns[2][2] = {{rd_fan_kernel<6, false, false, true, true>, rd_fan_kernel<6, false, true, true, true>},
+                                  {rd_fan_kernel<18, false, false, true, true>, rd_fan_kernel<18, false, true, true, true>}};
     const bool cart = p->geometry == GA_GEOM_3D_RNGDEP;
-    const RdKern kern = cart ? (path ? cpkerns[amp ? 1 : 0] : ckerns[amp ? 1 : 0][A.one_shot ? 1 : 0])
+    const RdKern kern = cart ? (path ? cpkerns[amp ? 1 : 0][blw ? 1 : 0] : ckerns[amp ? 1 : 0][A.one_shot ? 1 : 0][blw ? 1 : 0])
                              : (path ? pkerns[amp ? 1 : 0][blw ? 1 : 0] : kerns[amp ? 1 : 0][A.one_shot ? 1 : 0][blw ? 1 : 0]);
     int per_cu = 0;
     constexpr int W = GA_WG_RD;
@@ -1469,7 +1475,7 @@ struct EigGpu {
                   DevBuf<double> &yp, unsigned int *queue, hipStream_t s, const PathOut *path = nullptr)
     {
         const bool blw = RF.M.is_topo && RF.M.use_blw;
-        auto kern = cart ? rd_eig_kernel<NEQ, false, PATHS, true>
+        auto kern = cart ? (blw ? rd_eig_kernel<NEQ, true, PATHS, true> : rd_eig_kernel<NEQ, false, PATHS, true>)
                          : (blw ? rd_eig_kernel<NEQ, true, PATHS, false> : rd_eig_kernel<NEQ, false, PATHS, false>);
         int per_cu = 0;
         HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, GA_WG_RD, 0));

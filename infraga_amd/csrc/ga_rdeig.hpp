@@ -60,7 +60,7 @@ GA_HD void rd_eig_lane_loop(const RdEigArgs &E, int lane, NextJob next_job, V ya
             brk_leg = legs + 1;
             fetch = false;
         }
-        if (CART) s += rd3_rk4_step<NEQ>(M, L, y, y1, ys);
+        if (CART) s += rd3_rk4_step<NEQ, BLW>(M, L, y, y1, ys);
         else s += rd_rk4_step<NEQ, BLW>(M, L, y, y1, ys);
         ++k;
         ++nst;

@@ -31,7 +31,8 @@
 namespace ga {
 
 constexpr double RD_BL = 0.1;     // topo::bndlyr_param [km]
-constexpr double RD_VWG = 0.5;    // topo::vert_wind_grad
+// topo::vert_wind_grad is 0.5 in the spherical sources and 1.0 in the 3D
+// ones: RdMed::bl_sg holds the ratio.
 constexpr int F_C = 0, F_U = 1, F_V = 2, F_RHO = 3;
 
 struct RdMed {
@@ -45,6 +46,8 @@ struct RdMed {
     double sbc[3];
     int strat;          // a stratified profile over terrain (identical columns):
                         // the stratified driver's arrival inclination sign
+    int cart;           // the 3D Cartesian medium: (z, x, y) in the places of (r, lat, lon)
+    double bl_sg;       // topo::bndlyr_param / topo::vert_wind_grad: 0.1 / 0.5 (sph), 0.1 / 1.0 (3D)
 };
 
 struct RdFanArgs {
@@ -178,13 +181,19 @@ GA_HD void rd_point(const RdMed &M, double r, double lat, double lon, RdPt &P)
     rd_field_eval<0>(M.g, cl, F_V, v0, nullptr, nullptr);
     P.w = 0.0;
     if (bl && M.is_topo && M.use_blw) {
-        // atmo::w, :308-338
-        const double sg = RD_BL / RD_VWG;
+        // atmo::w, :308-338; Cartesian: atmo_state.3d.rngdep.cpp:243-267
+        // (u pairs with the x slope, v with the y slope, no metric factors)
+        const double sg = M.bl_sg;
         const double q = X / ((1.0 + X) * (1.0 + X));
         const double dS0 = -2.0 / RD_BL * q * td0;
         const double dS1 = -2.0 / RD_BL * q * td1;
-        double w0 = v0 * dS0 + u0 * dS1 / cos(lat);
-        w0 *= -2.0 * RD_BL / (M.r0 + zg);
+        double w0;
+        if (M.cart) {
+            w0 = -2.0 * RD_BL * (u0 * dS0 + v0 * dS1);
+        } else {
+            w0 = v0 * dS0 + u0 * dS1 / cos(lat);
+            w0 *= -2.0 * RD_BL / (M.r0 + zg);
+        }
         P.w = sc * X * w0 / (X + (sg / 2.0) * (1.0 - X * X));
     }
     P.u = u0 * sc;
@@ -221,11 +230,12 @@ GA_HD void rd_point_d(const RdMed &M, double r, double lat, double lon, RdPtD &P
         double dX[3] = {-X / RD_BL, X / RD_BL * T.d[0], X / RD_BL * T.d[1]};
         for (int n = 0; n < 3; ++n) dsc[n] = -2.0 * dX[n] / opx2;
         if (M.is_topo && M.use_blw) {
-            // atmo::w and atmo::dw, :308-403
-            const double sg = RD_BL / RD_VWG;
-            const double cl_ = cos(lat), sl = sin(lat);
-            double w0 = v0 * dsc[1] + u0 * dsc[2] / cl_;
-            w0 *= -2.0 * RD_BL / (M.r0 + T.zg);
+            // atmo::w and atmo::dw, :308-403; Cartesian: atmo_state.3d.rngdep.cpp:243-331
+            const double sg = M.bl_sg;
+            const double cl_ = M.cart ? 1.0 : cos(lat), sl = M.cart ? 0.0 : sin(lat);
+            const double gsc = M.cart ? -2.0 * RD_BL : -2.0 * RD_BL / (M.r0 + T.zg);
+            double w0 = M.cart ? u0 * dsc[1] + v0 * dsc[2] : v0 * dsc[1] + u0 * dsc[2] / cl_;
+            w0 *= gsc;
             const double N = sc * X * w0;
             const double D = X + (sg / 2.0) * (1.0 - X * X);
             P.w = N / D;
@@ -240,14 +250,20 @@ GA_HD void rd_point_d(const RdMed &M, double r, double lat, double lon, RdPtD &P
                 }
                 const double ddS0 = -2.0 * (ddX[0] / opx2 - 2.0 * dX[1] * dX[n] / (opx2 * (1.0 + X)));
                 const double ddS1 = -2.0 * (ddX[1] / opx2 - 2.0 * dX[2] * dX[n] / (opx2 * (1.0 + X)));
-                double dw0 = v0 * ddS0 + u0 * ddS1 / cl_;
-                if (n == 0) {
-                    dw0 += dv0[0] * dsc[1];
-                    dw0 += du0[0] * dsc[2] / cl_;
-                } else if (n == 1) {
-                    dw0 += u0 * dsc[2] * sl / (cl_ * cl_);
+                double dw0;
+                if (M.cart) {
+                    dw0 = u0 * ddS0 + v0 * ddS1;
+                    if (n == 0) dw0 += du0[0] * dsc[1] + dv0[0] * dsc[2];
+                } else {
+                    dw0 = v0 * ddS0 + u0 * ddS1 / cl_;
+                    if (n == 0) {
+                        dw0 += dv0[0] * dsc[1];
+                        dw0 += du0[0] * dsc[2] / cl_;
+                    } else if (n == 1) {
+                        dw0 += u0 * dsc[2] * sl / (cl_ * cl_);
+                    }
                 }
-                dw0 *= -2.0 * RD_BL / (M.r0 + T.zg);
+                dw0 *= gsc;
                 const double dN = (dsc[n] * X + sc * dX[n]) * w0 + sc * X * dw0;
                 const double dD = (1.0 - sg * X) * dX[n];
                 P.dw[n] = (dN * D - N * dD) / (D * D);
@@ -382,11 +398,14 @@ GA_HD void rd_medium_rhs(const RdMed &M, double r, double lat, double lon, RdAtm
     if (SECOND)
         for (int n = 0; n < 6; ++n) { ddu0[n] = W.ddu0[n]; ddv0[n] = W.ddv0[n]; }
 
-    // boundary-layer vertical wind (calc_uvw :658-690, :808-876)
-    const double sg = RD_BL / RD_VWG;
-    const double cl_ = cos(lat), sl = sin(lat), icl = 1.0 / cl_;
-    const double gsc = -2.0 * RD_BL / (M.r0 + T.zg);
-    double w0 = v0 * dsc0[1] + u0 * dsc0[2] * icl;
+    // boundary-layer vertical wind (calc_uvw :658-690, :808-876; Cartesian:
+    // atmo_state.3d.rngdep.cpp:543-563 and :667-720, u with the x slope, v
+    // with the y slope, no metric factors, no horizontal derivatives of u0
+    // and v0)
+    const double sg = M.bl_sg;
+    const double cl_ = CART ? 1.0 : cos(lat), sl = CART ? 0.0 : sin(lat), icl = 1.0 / cl_;
+    const double gsc = CART ? -2.0 * RD_BL : -2.0 * RD_BL / (M.r0 + T.zg);
+    double w0 = CART ? u0 * dsc0[1] + v0 * dsc0[2] : v0 * dsc0[1] + u0 * dsc0[2] * icl;
     w0 *= gsc;
     const double N = sc0 * X * w0;
     const double D = X + (sg / 2.0) * (1.0 - X * X);
@@ -394,7 +413,10 @@ GA_HD void rd_medium_rhs(const RdMed &M, double r, double lat, double lon, RdAtm
     if (!SECOND) {
         for (int n = 0; n < 3; ++n) {
             double dw0;
-            if (n == 0) {
+            if (CART) {
+                dw0 = u0 * ddsc0[1][n] + v0 * ddsc0[2][n];
+                if (n == 0) dw0 += du0[0] * dsc0[1] + dv0[0] * dsc0[2];
+            } else if (n == 0) {
                 dw0 = dv0[0] * dsc0[1] + v0 * ddsc0[1][0];
                 dw0 += (du0[0] * dsc0[2] + u0 * ddsc0[2][0]) * icl;
             } else if (n == 1) {
@@ -432,6 +454,33 @@ GA_HD void rd_medium_rhs(const RdMed &M, double r, double lat, double lon, RdAtm
             }
     }
     double dw0[3], dN[3], dD[3];
+    if constexpr (CART) {
+        for (int n = 0; n < 3; ++n) {
+            dw0[n] = u0 * ddsc0[1][n] + v0 * ddsc0[2][n];
+            if (n == 0) dw0[n] += du0[0] * dsc0[1] + dv0[0] * dsc0[2];
+            dw0[n] *= gsc;
+            dN[n] = (dsc0[n] * X + sc0 * dX[n]) * w0 + sc0 * X * dw0[n];
+            dD[n] = (1.0 - sg * X) * dX[n];
+            A.dw[n] = (dN[n] * D - N * dD[n]) / (D * D);
+        }
+        // every pair (p, q) of medium indices, packed as p6 (:667-720)
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const int p = k < 3 ? k : (k == 5 ? 1 : 0), q = k < 3 ? k : (k == 3 ? 1 : 2);
+            double ddw0 = u0 * dddsc0[0][p][q] + v0 * dddsc0[1][p][q];
+            if (p == 0) ddw0 += ddsc0[1][q] * du0[0] + ddsc0[2][q] * dv0[0];
+            if (q == 0) ddw0 += ddsc0[1][p] * du0[0] + ddsc0[2][p] * dv0[0];
+            if (p == 0 && q == 0) ddw0 += dsc0[1] * ddu0[0] + dsc0[2] * ddv0[0];
+            ddw0 *= gsc;
+            double ddN = ddsc0[p][q] * X * w0 + sc0 * ddX[p][q] * w0 + sc0 * X * ddw0;
+            ddN += dsc0[p] * dX[q] * w0 + dsc0[p] * X * dw0[q] + sc0 * dX[p] * dw0[q];
+            ddN += dsc0[q] * dX[p] * w0 + dsc0[q] * X * dw0[p] + sc0 * dX[q] * dw0[p];
+            const double ddD = -sg * dX[p] * dX[q] + (1.0 - sg * X) * ddX[p][q];
+            A.ddw[p6(p, q)] =
+                ((ddN * D - dN[p] * dD[q] - dN[q] * dD[p] - N * ddD) * (D * D) + 2.0 * N * D * dD[p] * dD[q]) / (D * D * D * D);
+        }
+        return;
+    }
     for (int n = 0; n < 3; ++n) {
         dw0[n] = v0 * ddsc0[1][n] + u0 * ddsc0[2][n] * icl;
         if (n == 0) dw0[n] += dv0[0] * dsc0[1] + du0[0] * dsc0[2] * icl;
@@ -853,31 +902,36 @@ GA_HD int c3(int n) { return n == 2 ? 0 : n + 1; }   // Cartesian index -> mediu
 // stratified profile over terrain (M.strat, infraga-3d with topo_file)
 // contracts the upper triangle only (geoac.eqset.3d.strat.cpp:305-310): the
 // lower terms enter with weight zero, which leaves every sum as the
-// reference forms it.
-template <int NEQ, class V, class SV>
+// reference forms it.  BLW: the boundary-layer vertical wind w and its
+// derivatives enter c_g[2], the nu equations and the auxiliary sums
+// (refs.w, refs.dw, refs.ddw: :242-317, :349-379); without it they are zero
+// and drop out at compile time.
+template <int NEQ, bool BLW = false, class V, class SV>
 GA_HD void rd3_stage(const RdMed &M, const Stage &S, const V &y, const SV &stg, const V &acc)
 {
     const double nu0 = stg[3], nu1 = stg[4], nu2 = stg[5];
     RdAtm A;
-    rd_medium_rhs<NEQ == 18, false, true>(M, stg[2], stg[0], stg[1], A);
+    rd_medium_rhs<NEQ == 18, BLW, true>(M, stg[2], stg[0], stg[1], A);
     value_fence(A);
-    double dc[3], du[3], dv[3];
+    double dc[3], du[3], dv[3], dw[3];
 #pragma unroll
     for (int n = 0; n < 3; ++n) {
         dc[n] = A.dc[c3(n)];
         du[n] = A.du[c3(n)];
         dv[n] = A.dv[c3(n)];
+        dw[n] = BLW ? A.dw[c3(n)] : 0.0;
     }
+    const double Aw = BLW ? A.w : 0.0;
     const double nu_mag = sqrt(nu0 * nu0 + nu1 * nu1 + nu2 * nu2);
     const double inu = 1.0 / nu_mag;
     const double cg0 = A.c * nu0 * inu + A.u;
     const double cg1 = A.c * nu1 * inu + A.v;
-    const double cg2 = A.c * nu2 * inu;
+    const double cg2 = A.c * nu2 * inu + Aw;
     const double cg_mag = sqrt(cg0 * cg0 + cg1 * cg1 + cg2 * cg2);
     const double icg = 1.0 / cg_mag;
     double Sg[3];
 #pragma unroll
-    for (int n = 0; n < 3; ++n) Sg[n] = nu_mag * dc[n] + nu0 * du[n] + nu1 * dv[n];
+    for (int n = 0; n < 3; ++n) Sg[n] = nu_mag * dc[n] + nu0 * du[n] + nu1 * dv[n] + nu2 * dw[n];
     const double cgv[3] = {cg0, cg1, cg2};
 
     if (NEQ == 18) {
@@ -892,13 +946,14 @@ GA_HD void rd3_stage(const RdMed &M, const Stage &S, const V &y, const SV &stg, 
             const int oX = j ? 12 : 6, om = j ? 15 : 9;
             const double Xj[3] = {stg[oX], stg[oX + 1], stg[oX + 2]};
             const double mj[3] = {stg[om], stg[om + 1], stg[om + 2]};
-            double dcj = 0.0, duj = 0.0;
-            double ddc[3], ddu[3], ddv[3];
+            double dcj = 0.0, duj = 0.0, dwj = 0.0;
+            double ddc[3], ddu[3], ddv[3], ddw[3];
 #pragma unroll
             for (int n = 0; n < 3; ++n) {
                 dcj += Xj[n] * dc[n];
                 duj += Xj[n] * du[n];
-                ddc[n] = ddu[n] = ddv[n] = 0.0;
+                if (BLW) dwj += Xj[n] * dw[n];
+                ddc[n] = ddu[n] = ddv[n] = ddw[n] = 0.0;
 #pragma unroll
                 for (int m = 0; m < 3; ++m) {
                     const int q = p6(c3(n), c3(m));
@@ -906,6 +961,7 @@ GA_HD void rd3_stage(const RdMed &M, const Stage &S, const V &y, const SV &stg, 
                     ddc[n] += xm * A.ddc[q];
                     ddu[n] += xm * A.ddu[q];
                     ddv[n] += xm * A.ddv[q];
+                    if (BLW) ddw[n] += xm * A.ddw[q];
                 }
             }
             const double dvj = j ? 0.0 : dv3;
@@ -913,14 +969,14 @@ GA_HD void rd3_stage(const RdMed &M, const Stage &S, const V &y, const SV &stg, 
             const double cnd = A.c * inu2 * dnu;
             const double dcg[3] = {nu0 * inu * dcj + A.c * mj[0] * inu - cnd * nu0 + duj,
                                    nu1 * inu * dcj + A.c * mj[1] * inu - cnd * nu1 + dvj,
-                                   nu2 * inu * dcj + A.c * mj[2] * inu - cnd * nu2};
+                                   nu2 * inu * dcj + A.c * mj[2] * inu - cnd * nu2 + dwj};
             const double dcgm = (cg0 * dcg[0] + cg1 * dcg[1] + cg2 * dcg[2]) * icg;
             const double cgm2 = dcgm * icg2;
 #pragma unroll
             for (int n = 0; n < 3; ++n) {
                 const double fX = dcg[n] * icg - cgv[n] * cgm2;
                 const double fm = cgm2 * Sg[n] - icg * (dnu * dc[n] + nu_mag * ddc[n] + mj[0] * du[n] + mj[1] * dv[n] +
-                                                        nu0 * ddu[n] + nu1 * ddv[n]);
+                                                        mj[2] * dw[n] + nu0 * ddu[n] + nu1 * ddv[n] + nu2 * ddw[n]);
                 rd_upd(S, y, stg, acc, oX + n, fX);
                 rd_upd(S, y, stg, acc, om + n, fm);
             }
@@ -933,7 +989,7 @@ GA_HD void rd3_stage(const RdMed &M, const Stage &S, const V &y, const SV &stg, 
     }
 }
 
-template <int NEQ, class V, class SV>
+template <int NEQ, bool BLW = false, class V, class SV>
 GA_HD double rd3_rk4_step(const RdMed &M, const Limits &L, const V &y, const V &y1, SV &stg)
 {
     const double ds = rd_set_ds(M, L, y[2], y[0], y[1]);
@@ -944,7 +1000,7 @@ GA_HD double rd3_rk4_step(const RdMed &M, const Limits &L, const V &y, const V &
     for (int st = 0; st < 4; ++st) {
         const double ha = (st == 0 || st == 3) ? h6 : h3;
         const double hn = (st == 2) ? ds : h2;
-        rd3_stage<NEQ>(M, Stage{ha, hn, false}, y, stg, y1);
+        rd3_stage<NEQ, BLW>(M, Stage{ha, hn, false}, y, stg, y1);
     }
     return ds;
 }
@@ -1238,7 +1294,7 @@ GA_HD void rd_lane_loop(const RdFanArgs &A, int lane, NextRay next_ray, V ya, V 
                 if (A.path.flags & PATH_WVROWS) rd_wv_rows_put<CART>(A.path, pl, M, y0, ray, 0, 0, 0.0);
             }
         }
-        if (CART) s += rd3_rk4_step<NEQ>(M, L, y, y1, ys);
+        if (CART) s += rd3_rk4_step<NEQ, BLW>(M, L, y, y1, ys);
         else s += rd_rk4_step<NEQ, BLW>(M, L, y, y1, ys);
         ++k;
         ++nst;

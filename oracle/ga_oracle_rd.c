@@ -271,10 +271,78 @@ double go_rd_dv(go_rd *a, double r, double lat, double lon, int n) { return dwin
 
 static int blw_on(go_rd *a, double r) { return a->is_topo && (r - a->r0 < a->z_bndlyr) && a->use_blw; }
 
+/* Cartesian atmo::w (atmo_state.3d.strat.cpp:235-256, the same expression as
+ * atmo_state.3d.rngdep.cpp:243-267): u pairs with the x slope, v with the y
+ * slope, no metric factors.  Arguments in the medium's (z, x, y) order. */
+static double rd3_w(go_rd *a, double z, double x, double y)
+{
+    double xe, ye, ze, X, S, dS[2], w0, sg_ratio, result = 0.0, bp = a->bl_param;
+    if (blw_on(a, z)) {
+        ev(a, &a->v, z, x, y, &xe, &ye, &ze);
+        sg_ratio = bp / a->vert_wind_grad;
+        X = exp(-(z - go_rd_topo_z(a, x, y)) / bp);
+        S = (1.0 - X) / (1.0 + X);
+        dS[0] = -2.0 / bp * (X / sq(1.0 + X)) * go_rd_topo_dz(a, x, y, 0);
+        dS[1] = -2.0 / bp * (X / sq(1.0 + X)) * go_rd_topo_dz(a, x, y, 1);
+        w0 = go_h3d_f(&a->u, xe, ye, ze) * dS[0];
+        w0 += go_h3d_f(&a->v, xe, ye, ze) * dS[1];
+        w0 *= -2.0 * bp;
+        result = S * X * w0;
+        result /= (X + (sg_ratio / 2.0) * (1.0 - sq(X)));
+    }
+    return result;
+}
+
+/* Cartesian atmo::dw (atmo_state.3d.strat.cpp:258-317; rngdep :269-331);
+ * n is the medium's derivative index (0 z, 1 x, 2 y). */
+static double rd3_dw(go_rd *a, double z, double x, double y, int n)
+{
+    double xe, ye, ze, X, dX[3], ddX[2], S, dS[3], ddS[2], N, D, sg_ratio, result = 0.0, bp = a->bl_param;
+    double dzg[2], u0, v0, w0, dw0, dN, dD;
+    const int c = n == 0 ? 2 : n - 1;   /* the reference's index: 0 x, 1 y, 2 z */
+    if (blw_on(a, z)) {
+        ev(a, &a->v, z, x, y, &xe, &ye, &ze);
+        sg_ratio = bp / a->vert_wind_grad;
+        double zg = go_rd_topo_z(a, x, y);
+        dzg[0] = go_rd_topo_dz(a, x, y, 0);
+        dzg[1] = go_rd_topo_dz(a, x, y, 1);
+        u0 = go_h3d_f(&a->u, xe, ye, ze);
+        v0 = go_h3d_f(&a->v, xe, ye, ze);
+        X = exp(-(z - zg) / bp);
+        for (int j = 0; j < 2; j++) dX[j] = X / bp * dzg[j];
+        dX[2] = -X / bp;
+        S = (1.0 - X) / (1.0 + X);
+        for (int j = 0; j < 3; j++) dS[j] = -2.0 * dX[j] / sq(1.0 + X);
+        if (c < 2) {
+            ddX[0] = X / bp * (go_rd_topo_ddz(a, x, y, 0, c) + 1.0 / bp * dzg[0] * dzg[c]);
+            ddX[1] = X / bp * (go_rd_topo_ddz(a, x, y, 1, c) + 1.0 / bp * dzg[1] * dzg[c]);
+        } else {
+            ddX[0] = -X / sq(bp) * dzg[0];
+            ddX[1] = -X / sq(bp) * dzg[1];
+        }
+        ddS[0] = -2.0 * (ddX[0] / sq(1.0 + X) - 2.0 * dX[0] * dX[c] / pow(1.0 + X, 3));
+        ddS[1] = -2.0 * (ddX[1] / sq(1.0 + X) - 2.0 * dX[1] * dX[c] / pow(1.0 + X, 3));
+        w0 = -2.0 * bp * (u0 * dS[0] + v0 * dS[1]);
+        N = S * X * w0;
+        D = X + (sg_ratio / 2.0) * (1.0 - sq(X));
+        dw0 = u0 * ddS[0] + v0 * ddS[1];
+        if (c == 2) {
+            dw0 += go_h3d_df(&a->u, xe, ye, ze, 2) * dS[0];
+            dw0 += go_h3d_df(&a->v, xe, ye, ze, 2) * dS[1];
+        }
+        dw0 *= -2.0 * bp;
+        dN = (dS[c] * X + S * dX[c]) * w0 + S * X * dw0;
+        dD = (1.0 - sg_ratio * X) * dX[c];
+        result = (dN * D - N * dD) / sq(D);
+    }
+    return result;
+}
+
 /* atmo::w, :308-338 */
 double go_rd_w(go_rd *a, double r, double lat, double lon)
 {
     double x, y, z, zg, dzg[2], X, S, dS[2], w0, sg_ratio, result = 0.0, bp = a->bl_param;
+    if (a->cart) return rd3_w(a, r, lat, lon);
     if (blw_on(a, r)) {
         ev(a, &a->v, r, lat, lon, &x, &y, &z);
         zg = go_rd_topo_z(a, lat, lon);
@@ -299,6 +367,7 @@ double go_rd_dw(go_rd *a, double r, double lat, double lon, int n)
 {
     double x, y, z, X, dX[3], ddX[2], S, dS[3], ddS[2], N, D, sg_ratio, result = 0.0, bp = a->bl_param;
     double zg, dzg[2], u0, v0, w0, dw0, dN, dD;
+    if (a->cart) return rd3_dw(a, r, lat, lon, n);
     if (blw_on(a, r)) {
         ev(a, &a->v, r, lat, lon, &x, &y, &z);
         sg_ratio = bp / a->vert_wind_grad;
@@ -401,8 +470,12 @@ void go_rd_calc_uvw(go_rd *a, double r, double lat, double lon, double *u, doubl
             if (n > 0) dX[n] = X / bp * dzg[n - 1];
             else dX[n] = -X / bp;
             for (int m = 0; m < 3; m++) {
-                if ((n > 0) && (m > 0)) ddX[n][m] = X / bp * (ddzg[n - 1][m - 1] + 1.0 / bp * dzg[n - 1] * dzg[m - 1]);
-                else if ((n == 0) && (m > 0)) ddX[n][m] = -X / sq(bp) * dzg[m - 1];
+                if ((n > 0) && (m > 0)) {
+                    /* the Cartesian calc_uvw fills x <= y and mirrors
+                     * (atmo_state.3d.rngdep.cpp:515-521) */
+                    const int i = (a->cart && m < n) ? m : n, j = (a->cart && m < n) ? n : m;
+                    ddX[n][m] = X / bp * (ddzg[i - 1][j - 1] + 1.0 / bp * dzg[i - 1] * dzg[j - 1]);
+                } else if ((n == 0) && (m > 0)) ddX[n][m] = -X / sq(bp) * dzg[m - 1];
                 else if ((n > 0) && (m == 0)) ddX[n][m] = -X / sq(bp) * dzg[n - 1];
                 else ddX[n][m] = X / sq(bp);
             }
@@ -484,7 +557,63 @@ void go_rd_calc_uvw(go_rd *a, double r, double lat, double lon, double *u, doubl
         ddv[5] = ddv0[5] * sc0 + dv0[1] * dsc0[2] + dv0[2] * dsc0[1] + v0 * ddsc0[1][2];
     }
 
-    if (a->is_topo && (r - a->r0 < a->z_bndlyr) && a->use_blw) {
+    if (a->cart && blw_on(a, r)) {
+        /* Cartesian calc_uvw vertical wind: atmo_state.3d.strat.cpp:531-553
+         * (first overload) and :650-710 (second), in their operation order.
+         * The 3D range-dependent calc_uvw (atmo_state.3d.rngdep.cpp:543-563,
+         * :667-720) and its atmo::ddw (:333-468, which its equation set
+         * contracts) are the same expressions without the horizontal
+         * derivatives of u0 and v0, which a stratified profile does not have:
+         * those terms enter for the stratified case only.  The reference's
+         * index i (0 x, 1 y, 2 z) sits at medium index I[i]. */
+        static const int I[3] = {1, 2, 0};
+        double dw0[3], dN[3], dD[3], ddw0, ddN, ddD;
+        w0 = -2.0 * bp * (u0 * dsc0[1] + v0 * dsc0[2]);
+        sg_ratio = bp / a->vert_wind_grad;
+        N = sc0 * X * w0;
+        D = X + sg_ratio / 2.0 * (1.0 - sq(X));
+        *w = N / D;
+        for (int i = 0; i < 3; i++) {
+            const int n = I[i];
+            dw0[i] = u0 * ddsc0[1][n] + v0 * ddsc0[2][n];
+            if (i == 2 || a->strat) dw0[i] += du0[n] * dsc0[1] + dv0[n] * dsc0[2];
+            dw0[i] *= -2.0 * bp;
+            dN[i] = (dsc0[n] * X + sc0 * dX[n]) * w0 + sc0 * X * dw0[i];
+            dD[i] = (1.0 - sg_ratio * X) * dX[n];
+            dw[n] = (dN[i] * D - N * dD[i]) / sq(D);
+            if (!second) continue;
+            if (i < 2) {
+                ddw0 = -2.0 * bp * (dddsc0[0][n][n] * u0 + dddsc0[1][n][n] * v0);
+            } else {
+                ddw0 = dddsc0[0][0][0] * u0 + 2.0 * ddsc0[1][0] * du0[0] + dsc0[1] * ddu0[0];
+                ddw0 += dddsc0[1][0][0] * v0 + 2.0 * ddsc0[2][0] * dv0[0] + dsc0[2] * ddv0[0];
+                ddw0 *= -2.0 * bp;
+            }
+            ddN = ddsc0[n][n] * X * w0 + sc0 * ddX[n][n] * w0 + sc0 * X * ddw0;
+            ddN += 2.0 * (dsc0[n] * dX[n] * w0 + dsc0[n] * X * dw0[i] + sc0 * dX[n] * dw0[i]);
+            ddD = (1.0 - sg_ratio * X) * ddX[n][n] - sg_ratio * sq(dX[n]);
+            ddw[n] = ((ddN * D - 2.0 * dN[i] * dD[i] - N * ddD) * sq(D) + 2.0 * N * D * sq(dD[i])) / pow(D, 4);
+        }
+        if (second) {
+            /* the reference's ddw[3..5] = xy, xz, yz: medium packed 5, 3, 4 */
+            static const int P[3][3] = {{0, 1, 5}, {0, 2, 3}, {1, 2, 4}};
+            for (int k = 0; k < 3; k++) {
+                const int i = P[k][0], j = P[k][1], p = I[i], q = I[j];
+                if (j < 2) {
+                    ddw0 = -2.0 * bp * (dddsc0[0][p][q] * u0 + dddsc0[1][p][q] * v0);
+                } else {
+                    ddw0 = dddsc0[0][p][q] * u0 + ddsc0[1][p] * du0[0];
+                    ddw0 += dddsc0[1][p][q] * v0 + ddsc0[2][p] * dv0[0];
+                    ddw0 *= -2.0 * bp;
+                }
+                ddN = ddsc0[p][q] * X * w0 + sc0 * ddX[p][q] * w0 + sc0 * X * ddw0;
+                ddN += dsc0[p] * dX[q] * w0 + dsc0[p] * X * dw0[j] + sc0 * dX[p] * dw0[j];
+                ddN += dsc0[q] * dX[p] * w0 + dsc0[q] * X * dw0[i] + sc0 * dX[q] * dw0[i];
+                ddD = (1.0 - sg_ratio * X) * ddX[p][q] - sg_ratio * dX[p] * dX[q];
+                ddw[P[k][2]] = ((ddN * D - dN[i] * dD[j] - dN[j] * dD[i] - N * ddD) * sq(D) + 2.0 * N * D * dD[i] * dD[j]) / pow(D, 4);
+            }
+        }
+    } else if (a->is_topo && (r - a->r0 < a->z_bndlyr) && a->use_blw) {
         w0 = v0 * dsc0[1] + u0 * dsc0[2] / cos(lat);
         w0 *= -2.0 * bp / (a->r0 + zg);
         sg_ratio = bp / a->vert_wind_grad;
@@ -1306,7 +1435,8 @@ void go_rd_h_limits(void *h, double out[8])
  * the medium above with r0 = 0, node coordinates in km, and (z, x, y) in
  * the places of (r, lat, lon) (atmo_state.3d.rngdep.cpp has the same
  * functions with those arguments; its boundary-layer vertical wind, whose
- * geometric factors differ, is not restated: use_blw stays 0).  State rows
+ * geometric factors differ, is rd3_w / rd3_dw and the Cartesian block of
+ * go_rd_calc_uvw).  State rows
  * are the reference's (x, y, z, nu, X_th, mu_th, X_ph, mu_ph).
  * ==================================================================== */
 static int mi3(int n) { return n == 2 ? 0 : n + 1; }   /* Cartesian -> medium derivative index */

@@ -183,3 +183,54 @@ def test_no_travel_time_limit(oracle):
     p.time_max = 1.0e-3          # hours
     a2, _, s2 = m.fan3(L, p, bounces=2, src=(0.0, 0.0, 0.0))
     assert s1 == s2 and len(a1) == len(a2) == 6 and (a2["time"] > 3.6).all()
+
+
+def test_boundary_layer_vertical_wind_derivatives(oracle, base_units):
+    """Cartesian boundary-layer vertical wind (atmo::w / dw and both calc_uvw
+    overloads) over a hill, on columns with horizontally uniform winds (where
+    the reference's derivative formulas, which leave out the horizontal
+    derivatives of u0 and v0, are the exact derivatives): dw is the central
+    difference of w, and the second overload's ddw the central difference of
+    the first overload's dw, in every direction.  The stratified-over-terrain
+    form of the same code is bit-identical to the reference's compiled 3D
+    medium (test_oracle_strat_topo3d.py); this pins the range-dependent
+    branch, its u-x / v-y pairing and the packed index order (zz, xx, yy, zx,
+    zy, xy)."""
+    z, c, u, v, rho = base_units
+    u = u + 0.012 * np.exp(-z / 5.0)
+    v = v - 0.007 * np.exp(-z / 5.0)
+    xs = (-600.0, -200.0, 200.0, 600.0)
+    m = Rd3Medium(oracle, *columns(z, c, u, v, rho, x=xs, y=xs))
+    tx = np.linspace(-600.0, 600.0, 49)
+    X, Y = np.meshgrid(tx, tx, indexing="ij")
+    # above the profile bottom (z = 0), where the spline clamps its height
+    zg = 1.2 * np.exp(-((X - 50.0) ** 2 + (Y + 30.0) ** 2) / 200.0 ** 2) + 0.3 * (1.0 + np.sin(X / 90.0) * np.cos(Y / 130.0))
+    m.set_topo(tx, tx, zg)
+    m.config(use_blw=True)
+    rng = np.random.default_rng(31)
+    hs = (1e-6, 1e-3, 1e-3)   # z, x, y steps [km]
+    checked, w_max = 0, 0.0
+    for _ in range(40):
+        x, y = rng.uniform(-400.0, 400.0, 2)
+        zz = m.topo(x, y)[0] + rng.uniform(0.02, 0.6)
+        P = m.point(zz, x, y)
+        c1 = m.calc_uvw(zz, x, y, False)
+        c2 = m.calc_uvw(zz, x, y, True)
+        w, dw, dw1, ddw = P[4], P[14:17], c1[9:12], c2[24:30]
+        w_max = max(w_max, abs(w))
+        assert np.allclose(c1[2], w, rtol=1e-13, atol=0) and np.allclose(c2[2], w, rtol=1e-13, atol=0)
+        assert np.allclose(dw1, dw, rtol=1e-12, atol=1e-18) and np.allclose(c2[9:12], dw, rtol=1e-12, atol=1e-18)
+        fd_w, fd_dw = np.zeros(3), np.zeros((3, 3))
+        for n in range(3):
+            e = np.zeros(3)
+            e[n] = hs[n]
+            pp, pm = (zz + e[0], x + e[1], y + e[2]), (zz - e[0], x - e[1], y - e[2])
+            fd_w[n] = (m.point(*pp)[4] - m.point(*pm)[4]) / (2.0 * hs[n])
+            fd_dw[n] = (m.calc_uvw(*pp, False)[9:12] - m.calc_uvw(*pm, False)[9:12]) / (2.0 * hs[n])
+        assert np.allclose(dw, fd_w, rtol=2e-6, atol=1e-6 * np.abs(dw).max()), (dw, fd_w)
+        full = lambda q: np.array([[q[0], q[3], q[4]], [q[3], q[1], q[5]], [q[4], q[5], q[2]]])
+        D = full(ddw)
+        scale = np.abs(D).max()
+        assert np.allclose(D, fd_dw, rtol=2e-6, atol=2e-6 * scale), (D, fd_dw)
+        checked += 1
+    assert checked == 40 and w_max > 1e-6

@@ -80,17 +80,21 @@ def as_reference_order(o, second):
 
 
 @pytest.mark.skipif(not HAVE_REF, reason="oracle/_ref/libref_3d.so not built (no reference tree)")
-def test_medium_matches_reference_3d_strat_over_terrain(oracle, tmp_path):
+@pytest.mark.parametrize("use_blw", [False, True])
+def test_medium_matches_reference_3d_strat_over_terrain(oracle, tmp_path, use_blw):
     prof, topo = scene()
     met, tp = write_files(str(tmp_path), prof, topo)
     ref = Ref(GEOM_3D)
     ref.load_topo(met, tp)
+    ref.lib.ref_use_blw(int(use_blw))
     m = strat_medium(oracle, prof, topo)
+    m.config(use_blw=use_blw)
     try:
         lim, rlim = m.limits(), ref.limits()
         assert np.array_equal(lim, rlim), (lim, rlim)
         assert abs(lim[6] - (topo[2].max() + 0.1)) < 1e-12 and abs(lim[7] - lim[6] - 2.0) < 1e-12
         rng = np.random.default_rng(23)
+        w_seen = 0
         for _ in range(400):
             x = rng.uniform(-300.0, 300.0)
             y = rng.uniform(-350.0, 350.0)
@@ -102,7 +106,12 @@ def test_medium_matches_reference_3d_strat_over_terrain(oracle, tmp_path):
             exp = np.concatenate([[ref.lib.ref_c(x, y, z_), ref.lib.ref_w(x, y, z_)],
                                   ref.calc_uvw(x, y, z_, False), ref.calc_uvw(x, y, z_, True), ref.topo(x, y)])
             assert np.array_equal(got, exp), (z_, np.flatnonzero(got != exp), got[got != exp], exp[got != exp])
+            if use_blw and z_ - zt < 2.0:
+                w_seen += abs(got[1]) > 1e-6
+        if use_blw:
+            assert w_seen > 10   # the vertical wind is on inside the boundary layer
     finally:
+        ref.lib.ref_use_blw(0)
         ref.lib.ref_clear()
 
 

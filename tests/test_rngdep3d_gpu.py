@@ -77,13 +77,11 @@ def test_fan_matches_oracle(engine, rd3, case, amp, flags):
 
 @pytest.mark.gpu
 def test_geometry_and_grid_must_agree(engine, rd3):
-    """A Cartesian grid with the spherical geometry (and the converse) is an error."""
+    """A Cartesian grid with the spherical geometry is an error."""
     from infraga_amd.engine import GA_GEOM_SPH_RNGDEP, EngineError
     rid, launches = rd3
     with pytest.raises(EngineError):
         engine.fan(rid, prop_params(GA_GEOM_SPH_RNGDEP), launches[:1])
-    with pytest.raises(EngineError):
-        engine.fan(rid, prop_params(GA_GEOM_3D_RNGDEP, topo_use_blw=1), launches[:1])
 
 
 @pytest.mark.gpu
