@@ -91,3 +91,34 @@ def test_fan_blw_matches_oracle(engine, media, medium, amp, flags):
     arr0, _ = engine.fan(aid, p0, launches)
     n = min(len(arr0), len(arr))
     assert len(arr0) != len(arr) or np.abs(arr0["time"][:n] - arr["time"][:n]).max() > 1e-9
+
+
+def test_cli_3d_rngdep_prop_blw(tmp_path, oracle):
+    """infraga-3d-rngdep -prop with topo_use_BLw=true over the reference's
+    node-file layout: the arrivals file equals the oracle's rows with the
+    vertical wind to the printed digits (source 3 km up, above the terrain,
+    so the inclinations start at incl_min; amplitudes on, the binary's
+    default, so TL is compared with the oracle's 18-equation fan)."""
+    import subprocess
+    from ga_oracle_py import Rd3Medium, write_rd_files
+    units, topo, _ = rngdep_case()
+    x, y, z, c, u, v, rho = units
+    pre, xs, ys, tp = write_rd_files(str(tmp_path), x, y, z, c * 1000.0, u * 1000.0, v * 1000.0, rho, topo=topo)
+    exe = os.path.join(os.path.dirname(HERE), "infraga_amd", "bin", "infraga-3d-rngdep")
+    r = subprocess.run([exe, "-prop", pre, xs, ys, f"topo_file={tp}", "prof_format=zcuvd", "write_rays=false",
+                        "incl_min=4", "incl_max=18", "incl_step=7", "azimuth=-90", "bounces=2",
+                        "src_alt=3.0", "topo_use_BLw=true"],
+                       capture_output=True, text=True, cwd=tmp_path, timeout=120)
+    assert r.returncode == 0, r.stderr
+    rows = [[float(t) for t in l.split("\t")] for l in open(pre + ".arrivals.dat") if l.strip() and not l.startswith("#")]
+    assert len(rows) > 0
+    m = Rd3Medium(oracle, *units)
+    m.set_topo(*topo)
+    m.config(use_blw=True)
+    L = sorted({(row[0], row[1]) for row in rows})
+    ref, _, _ = m.fan3(L, m.params(True), bounces=2, src=(float("nan"), float("nan"), 3.0))
+    assert len(ref) == len(rows)
+    for row, a in zip(rows, ref):
+        exp = [a["incl_launch"], a["az_launch"], a["n_bounce"], a["pos0"], a["pos1"], a["time"], a["cel"],
+               a["turn_ht"], a["incl_arr"], a["back_az"], a["tl"], a["absorption"]]
+        np.testing.assert_allclose(row, exp, rtol=1e-5, atol=1e-5)
