@@ -383,6 +383,26 @@ int ga_wnl_wvfrm(ga_ctx *ctx, int atmo_id, const ga_prop_params *p, const ga_wvf
                  double *t_init, double *p_init, double *t_out, double *p_out,
                  ga_arrival *arrivals, int32_t *status, int64_t *evolve_steps, ga_wvfrm_stats *stats);
 
+/* ga_wnl_wvfrm with the wvfrm_out_step dumps (the sc_spectrum-<L>.dat /
+ * sc_waveform-<L>.dat files of wnl_wvfrm, geoac.eqset.sph.strat.cpp:989-1002,
+ * and the same in every equation set; out_step from wvfrm_out_step=,
+ * main.infraga.sph.cpp:1210): at the top of each evolution step whose
+ * previous step size carries the total ray length s + s_prev across a
+ * multiple of out_step, the state before the step.  s_prev is ref on the leg
+ * from the reference point and the legs' accumulated length after it.
+ * Snapshot j of launch i sits at snaps + (i * snap_cap + j) * (1 + len +
+ * 2 (len / 2 + 1)): [label L = out_step ceil((s + s_prev) / out_step), the
+ * scaled pressure u[len] (sc_waveform's second column; its first is the
+ * source time axis), (Re U, Im U) for bins 0 .. len / 2 (sc_spectrum's
+ * columns 2-3 at f = bin / (len dt))].  snap_n[i] counts the dumps reached,
+ * which may exceed snap_cap (then only the first snap_cap are written).
+ * out_step >= 1e4 writes none, as the reference. */
+int ga_wnl_wvfrm_dump(ga_ctx *ctx, int atmo_id, const ga_prop_params *p, const ga_wvfrm_params *w,
+                      const double *launches, int n, const double *t_src, const double *p_src,
+                      double *t_init, double *p_init, double *t_out, double *p_out,
+                      ga_arrival *arrivals, int32_t *status, int64_t *evolve_steps, ga_wvfrm_stats *stats,
+                      double out_step, int snap_cap, double *snaps, int32_t *snap_n);
+
 /* Terrain profile z(r) [km] of the 2D geometry (infraga-2d topo_file=, read
  * by set_region, src/atmo/atmo_io.2d.cpp:138-156, into topo::spline): n >= 3
  * strictly increasing ranges.  Attached to a stratified atmosphere; used by

@@ -2645,10 +2645,11 @@ void wv_tables(const ga_wvfrm_params *w, std::vector<double> &tukey, std::vector
 
 }  // namespace
 
-int ga_wnl_wvfrm(ga_ctx *ctx, int atmo_id, const ga_prop_params *p_in, const ga_wvfrm_params *w,
-                 const double *launches, int n, const double *t_src, const double *p_src,
-                 double *t_init, double *p_init, double *t_out, double *p_out,
-                 ga_arrival *arrivals, int32_t *status, int64_t *evolve_steps, ga_wvfrm_stats *stats)
+static int wnl_wvfrm_impl(ga_ctx *ctx, int atmo_id, const ga_prop_params *p_in, const ga_wvfrm_params *w,
+                          const double *launches, int n, const double *t_src, const double *p_src,
+                          double *t_init, double *p_init, double *t_out, double *p_out,
+                          ga_arrival *arrivals, int32_t *status, int64_t *evolve_steps, ga_wvfrm_stats *stats,
+                          double out_step, int snap_cap, double *snaps, int32_t *snap_n)
 {
     auto t0 = std::chrono::steady_clock::now();
     if (!ctx || !p_in || !w) return fail(ctx, GA_E_ARG, "null argument");
@@ -2828,16 +2829,37 @@ int ga_wnl_wvfrm(ga_ctx *ctx, int atmo_id, const ga_prop_params *p_in, const ga_
     A.rows = d_rows.p;
     A.ray = d_ray.p;
     A.steps = d_st.p;
+    const bool dump = snaps && snap_n && snap_cap > 0 && out_step < 1.0e4;
+#define WV_EVOLVE(S, MD)                                                                              \
+    do {                                                                                              \
+        if (dump) hipLaunchKernelGGL((wv_evolve_kernel<S, MD, true>), dim3(n), dim3(WV_T), 0, st, A); \
+        else hipLaunchKernelGGL((wv_evolve_kernel<S, MD>), dim3(n), dim3(WV_T), 0, st, A);            \
+    } while (0)
+    const size_t snap_w = 1 + (size_t)len + 2 * ((size_t)len / 2 + 1);
+    DevBuf<double> d_snap;
+    DevBuf<int32_t> d_sn;
+    A.out_step = out_step;
+    A.ref = w->ref;
+    A.snap = nullptr;
+    A.snap_cap = 0;
+    A.snap_n = nullptr;
+    if (dump && n > 0) {
+        if (!d_snap.ensure((size_t)n * snap_cap * snap_w) || !d_sn.ensure((size_t)n))
+            return fail(ctx, GA_E_NOMEM, "waveform dump buffer (%d x %d snapshots)", n, snap_cap);
+        A.snap = d_snap.p;
+        A.snap_cap = snap_cap;
+        A.snap_n = d_sn.p;
+    }
     HIPCHK(ctx, hipEventRecord(ev2.e0, st));
     if (med == WV_2D) {
-        hipLaunchKernelGGL((wv_evolve_kernel<false, WV_2D>), dim3(n), dim3(WV_T), 0, st, A);
+        WV_EVOLVE(false, WV_2D);
     } else if (med == WV_STRAT) {
-        if (sph) hipLaunchKernelGGL((wv_evolve_kernel<true, WV_STRAT>), dim3(n), dim3(WV_T), 0, st, A);
-        else hipLaunchKernelGGL((wv_evolve_kernel<false, WV_STRAT>), dim3(n), dim3(WV_T), 0, st, A);
+        if (sph) WV_EVOLVE(true, WV_STRAT);
+        else WV_EVOLVE(false, WV_STRAT);
     } else if (med == WV_RD_SPH) {
-        hipLaunchKernelGGL((wv_evolve_kernel<true, WV_RD_SPH>), dim3(n), dim3(WV_T), 0, st, A);
+        WV_EVOLVE(true, WV_RD_SPH);
     } else {
-        hipLaunchKernelGGL((wv_evolve_kernel<false, WV_RD_CART>), dim3(n), dim3(WV_T), 0, st, A);
+        WV_EVOLVE(false, WV_RD_CART);
     }
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipEventRecord(ev2.e1, st));
@@ -2852,6 +2874,13 @@ int ga_wnl_wvfrm(ga_ctx *ctx, int atmo_id, const ga_prop_params *p_in, const ga_
     HIPCHK(ctx, hipMemcpy(rays.data(), d_ray.p, nl * sizeof(WvRay), hipMemcpyDeviceToHost));
     HIPCHK(ctx, hipMemcpy(uq.data(), d_u.p, uq.size() * sizeof(double), hipMemcpyDeviceToHost));
     HIPCHK(ctx, hipMemcpy(sq.data(), d_st.p, sq.size() * sizeof(long long), hipMemcpyDeviceToHost));
+#undef WV_EVOLVE
+    if (dump && n > 0) {
+        HIPCHK(ctx, hipMemcpy(snap_n, d_sn.p, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost));
+        HIPCHK(ctx, hipMemcpy(snaps, d_snap.p, (size_t)n * snap_cap * snap_w * sizeof(double), hipMemcpyDeviceToHost));
+    } else if (snap_n) {
+        for (int i = 0; i < n; ++i) snap_n[i] = 0;
+    }
 
     // 4. outputs: wvfrm_init.dat and wvfrm_out.dat rows (main.infraga.sph.cpp:
     // 1380-1386, 1440-1461), the arrival summary
@@ -3582,3 +3611,24 @@ void ga_server_destroy(ga_server *srv)
 }
 
 }  // extern "C"
+
+int ga_wnl_wvfrm(ga_ctx *ctx, int atmo_id, const ga_prop_params *p_in, const ga_wvfrm_params *w,
+                 const double *launches, int n, const double *t_src, const double *p_src,
+                 double *t_init, double *p_init, double *t_out, double *p_out,
+                 ga_arrival *arrivals, int32_t *status, int64_t *evolve_steps, ga_wvfrm_stats *stats)
+{
+    return wnl_wvfrm_impl(ctx, atmo_id, p_in, w, launches, n, t_src, p_src, t_init, p_init, t_out, p_out, arrivals,
+                          status, evolve_steps, stats, 1.0e10, 0, nullptr, nullptr);
+}
+
+int ga_wnl_wvfrm_dump(ga_ctx *ctx, int atmo_id, const ga_prop_params *p_in, const ga_wvfrm_params *w,
+                      const double *launches, int n, const double *t_src, const double *p_src,
+                      double *t_init, double *p_init, double *t_out, double *p_out,
+                      ga_arrival *arrivals, int32_t *status, int64_t *evolve_steps, ga_wvfrm_stats *stats,
+                      double out_step, int snap_cap, double *snaps, int32_t *snap_n)
+{
+    if (!(out_step > 0.0) || snap_cap < 1 || !snaps || !snap_n)
+        return fail(ctx, GA_E_ARG, "waveform dumps need out_step > 0, snap_cap >= 1 and output buffers");
+    return wnl_wvfrm_impl(ctx, atmo_id, p_in, w, launches, n, t_src, p_src, t_init, p_init, t_out, p_out, arrivals,
+                          status, evolve_steps, stats, out_step, snap_cap, snaps, snap_n);
+}

@@ -81,6 +81,14 @@ struct WvArgs {
     const double *rows;          // [n_rows][WV_ROW]
     const struct WvRay *ray;     // [n_wv]: reference-point quantities (c0)
     long long *steps;            // [n_wv]: evolution steps
+    // wvfrm_out_step dumps (the DUMP instantiation): the state at the top
+    // of a step whose previous step size carries the total ray length
+    // across a multiple of out_step (geoac.eqset.sph.strat.cpp:989-1002)
+    double out_step;
+    double ref;                  // wvfrm_ref: the total length before the first leg
+    double *snap;                // [n_wv][snap_cap][1 + len + 2 (len/2 + 1)]
+    int snap_cap;
+    int *snap_n;                 // [n_wv]: dumps reached (may exceed snap_cap)
 };
 
 // Segment of the leg's abscissa holding x at or after segment k (queries
@@ -196,7 +204,7 @@ struct WvPoint {
     double cpt;        // c at the ray position
 };
 
-template <bool SPH, int MED>
+template <bool SPH, int MED, bool DUMP = false>
 __global__ void __launch_bounds__(WV_T) wv_evolve_kernel(WvArgs A)
 {
     __shared__ double re[WV_LDS], im[WV_LDS];
@@ -221,12 +229,20 @@ __global__ void __launch_bounds__(WV_T) wv_evolve_kernel(WvArgs A)
     for (int j = 0; j < WV_BPT; ++j) { Ur[j] = 0.0; Ui[j] = 0.0; }
     long long nstep = 0;
     int curz = 0;                                 // spline cursor of the absorption point
+    double ray_len = 0.0;                         // the drivers' ray_length before each leg
+    bool begun = false;
+    int nsnap = 0;
 
     for (int leg = 0; leg < A.slots; ++leg) {
         const double *R = A.rows + (size_t)A.leg_first[(size_t)w * A.slots + leg] * WV_ROW;
         const int n = A.leg_n[(size_t)w * A.slots + leg];
         if (n < 2) continue;
         const double s_tot = R[(size_t)(n - 1) * WV_ROW];
+        // s_prev of the leg: wvfrm_ref from the reference point, then the
+        // legs' accumulated length (main.infraga.sph.cpp:1394-1399)
+        const double sp = begun ? ray_len : A.ref;
+        ray_len += s_tot;
+        begun = true;
 
         // U = FFT(u): u from LDS (natural order) to bit-reversed order
         {
@@ -269,6 +285,7 @@ __global__ void __launch_bounds__(WV_T) wv_evolve_kernel(WvArgs A)
             // step loop
             double s = 0.0;
             int ks = 0;
+            double ds_prev = A.ds_wv;
             while (s < s_tot) {
                 // the lane index passes through an empty asm each step, so the
                 // per-lane addresses and frequencies below are recomputed
@@ -276,6 +293,28 @@ __global__ void __launch_bounds__(WV_T) wv_evolve_kernel(WvArgs A)
                 // loop, which spills at 1024 threads
                 int tid = tid0;
                 asm volatile("" : "+v"(tid));
+                if (DUMP) {
+                    // u (LDS, natural order, published) and U (registers)
+                    // before the step; label out_step ceil((s + s_prev) / out_step)
+                    const double os = A.out_step;
+                    const double lab = os * ceil((s + sp) / os);
+                    if (fabs(os * ceil((s + sp + ds_prev) / os) - lab) > 0.0) {
+                        if (nsnap < A.snap_cap) {
+                            double *S = A.snap + ((size_t)w * A.snap_cap + nsnap) * (size_t)(1 + len + 2 * (half + 1));
+                            if (tid == 0) S[0] = lab;
+                            for (int m = tid; m < len; m += WV_T) S[1 + m] = re[wv_ix(m)];
+#pragma unroll
+                            for (int j = 0; j < WV_BPT; ++j) {
+                                const int b = tid + j * WV_T;
+                                if (b <= half) {
+                                    S[1 + len + 2 * b] = Ur[j];
+                                    S[2 + len + 2 * b] = Ui[j];
+                                }
+                            }
+                        }
+                        ++nsnap;
+                    }
+                }
                 ks = wv_seek(R, n, ks, s);
                 const double bc = wv_beta(R, ks, s);
                 double ds = A.ds_wv / (PI * Uf_max * bc);
@@ -449,12 +488,14 @@ __global__ void __launch_bounds__(WV_T) wv_evolve_kernel(WvArgs A)
                 }
                 Uf_max = wv_block_max(fm, red, tid);   // its barriers publish the new u
                 s += ds;
+                ds_prev = ds;
                 ++nstep;
             }
         }
     }
     for (int m = tid; m < len; m += WV_T) A.u[(size_t)w * len + m] = re[wv_ix(m)];
     if (tid == 0) A.steps[w] = nstep;
+    if (DUMP && tid == 0) A.snap_n[w] = nsnap;
 }
 
 

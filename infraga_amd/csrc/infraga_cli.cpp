@@ -1490,6 +1490,47 @@ bool wv_load(const char *path, std::vector<double> &t, std::vector<double> &p)
     return len > 0;
 }
 
+// One waveform; with wvfrm_out_step < 1e4 also the intermediate dumps
+// (geoac.eqset.sph.strat.cpp:989-1002, every equation set alike):
+// sc_spectrum-<L>.dat (f = n / (len dt), Re U, Im U) and sc_waveform-<L>.dat
+// (source time axis, scaled pressure) in the working directory, at the
+// stream's default precision, a later dump of the same label replacing the
+// file.
+static int wv_run_dumped(ga_ctx *ctx, int aid, const ga_prop_params &p, const ga_wvfrm_params &w,
+                         const double *launch, const std::vector<double> &t, const std::vector<double> &ps,
+                         double *ti, double *pi_, double *to, double *po, ga_arrival *A, int32_t *status,
+                         int64_t *steps, ga_wvfrm_stats *st, double out_step, int bounces)
+{
+    if (!(out_step < 1.0e4))
+        return ga_wnl_wvfrm(ctx, aid, &p, &w, launch, 1, t.data(), ps.data(), ti, pi_, to, po, A, status, steps, st);
+    const int len = w.len;
+    const size_t width = 1 + (size_t)len + 2 * ((size_t)len / 2 + 1);
+    int cap = (int)std::min(4096.0, std::ceil(p.s_max / out_step) + 2.0 * (bounces + 2));
+    std::vector<double> snaps;
+    int32_t cnt = 0;
+    for (;;) {
+        snaps.assign((size_t)cap * width, 0.0);
+        const int rc = ga_wnl_wvfrm_dump(ctx, aid, &p, &w, launch, 1, t.data(), ps.data(), ti, pi_, to, po, A, status,
+                                         steps, st, out_step, cap, snaps.data(), &cnt);
+        if (rc != GA_OK) return rc;
+        if (cnt <= cap) break;
+        cap = cnt;   // more crossings than the first guess: once more with room for all
+    }
+    const double dt = t[1] - t[0];
+    for (int j = 0; j < cnt; ++j) {
+        const double *S = snaps.data() + (size_t)j * width;
+        char name[64];
+        std::snprintf(name, sizeof name, "sc_spectrum-%04g.dat", S[0]);
+        std::ofstream fs(name);
+        for (int b = 0; b <= len / 2; ++b)
+            fs << 1.0 / dt * b / len << '\t' << S[1 + len + 2 * b] << '\t' << S[2 + len + 2 * b] << '\n';
+        std::snprintf(name, sizeof name, "sc_waveform-%04g.dat", S[0]);
+        std::ofstream fw(name);
+        for (int m = 0; m < len; ++m) fw << t[(size_t)m] << '\t' << S[1 + m] << '\n';
+    }
+    return GA_OK;
+}
+
 // -wnl_wvfrm (run_wnl_wvfrm: main.infraga.sph.cpp:1142-1479,
 // main.infraga.3d.cpp:1087-1430, main.infraga.sph.rngdep.cpp:1108-1410,
 // main.infraga.3d.rngdep.cpp:1051-1350): one ray, its waveform set at
@@ -1497,8 +1538,8 @@ bool wv_load(const char *path, std::vector<double> &t, std::vector<double> &p)
 // <id>.wvfrm_init.dat, <id>.wvfrm_out.dat and, with write_ray=true,
 // <id>.raypaths.dat.  The stratified binaries take topo_file= (the source
 // then defaults to the terrain's centre), the range-dependent ones their
-// node files.  Intermediate waveform dumps (wvfrm_out_step < 1e4) are
-// refused; the waveform length must be a power of two (the engine's
+// node files.  wvfrm_out_step < 1e4 writes the intermediate dumps
+// (wv_run_dumped); the waveform length must be a power of two (the engine's
 // transform).
 int run_wnl_wvfrm(int argc, char **argv)
 {
@@ -1589,10 +1630,6 @@ int run_wnl_wvfrm(int argc, char **argv)
             return 2;
         }
     }
-    if (wvfrm_out_step < 1.0e4) {
-        std::cerr << GA_CLI_NAME << ": wvfrm_out_step (intermediate waveform files) is not supported by this engine\n";
-        return 2;
-    }
     std::vector<double> t, ps;
     const bool from_file = std::strncmp(wvfrm_file, "none", 4) != 0;
     if (from_file) {
@@ -1669,8 +1706,8 @@ int run_wnl_wvfrm(int argc, char **argv)
     int32_t status = 0;
     int64_t steps = 0;
     ga_wvfrm_stats st{};
-    if (ga_wnl_wvfrm(ctx, aid, &p, &w, launch, 1, t.data(), ps.data(), ti.data(), pi_.data(), to.data(), po.data(),
-                     &A, &status, &steps, &st) != GA_OK)
+    if (wv_run_dumped(ctx, aid, p, w, launch, t, ps, ti.data(), pi_.data(), to.data(), po.data(), &A, &status, &steps,
+                      &st, wvfrm_out_step, bounces) != GA_OK)
         return die(ctx, "waveform failed");
     if (write_rays) {
         // the same ray's write_rays rows (m == 1 or m % 15 == 0 of every leg)
@@ -1837,10 +1874,6 @@ int run_wnl_wvfrm_2d(int argc, char **argv)
             return 2;
         }
     }
-    if (wvfrm_out_step < 1.0e4) {
-        std::cerr << GA_CLI_NAME << ": wvfrm_out_step (intermediate waveform files) is not supported by this engine\n";
-        return 2;
-    }
     std::vector<double> t, ps;
     const bool from_file = std::strncmp(wvfrm_file, "none", 4) != 0;
     if (from_file) {
@@ -1893,8 +1926,8 @@ int run_wnl_wvfrm_2d(int argc, char **argv)
     int32_t status = 0;
     int64_t steps = 0;
     ga_wvfrm_stats st{};
-    if (ga_wnl_wvfrm(ctx, aid, &p, &w, launch, 1, t.data(), ps.data(), ti.data(), pi_.data(), to.data(), po.data(),
-                     &A, &status, &steps, &st) != GA_OK)
+    if (wv_run_dumped(ctx, aid, p, w, launch, t, ps, ti.data(), pi_.data(), to.data(), po.data(), &A, &status, &steps,
+                      &st, wvfrm_out_step, bounces) != GA_OK)
         return die(ctx, "waveform failed");
     if (status < 0) {
         std::cout << "\n\t\tRay path does not reach the waveform reference location (" << wvfrm_ref << " km).\n\n";

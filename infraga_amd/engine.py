@@ -30,7 +30,8 @@ EXPORTED = ["ga_prop_params_default", "ga_eig_params_default", "ga_ctx_create", 
             "ga_rngdep_release", "ga_rngdep_load_files", "ga_rngdep_info", "ga_fan_run_paths",
             "ga_eig_search_paths", "ga_eig_direct", "ga_topo_load_2d", "ga_topo_eval_2d",
             "ga_fan_run_device_noalloc", "ga_fan_autotune", "ga_atmo_load_strat_topo", "ga_atmo_load_file_topo",
-            "ga_atmo_load_strat_topo_xy", "ga_atmo_load_file_topo_xy", "ga_wnl_wvfrm"]
+            "ga_atmo_load_strat_topo_xy", "ga_atmo_load_file_topo_xy", "ga_wnl_wvfrm",
+            "ga_wnl_wvfrm_dump"]
 
 GA_PATH_RAYS, GA_PATH_CAUSTICS = 1, 2
 GA_PT_RAY, GA_PT_CAUSTIC = 0, 1
@@ -176,6 +177,7 @@ def load_library(path: str | None = None):
     L.ga_wnl_wvfrm.argtypes = [C.c_void_p, C.c_int, C.POINTER(PropParams), C.POINTER(WvfrmParams), _dp, C.c_int,
                                _dp, _dp, _dp, _dp, _dp, _dp, C.POINTER(Arrival), C.POINTER(C.c_int32),
                                C.POINTER(C.c_int64), C.POINTER(WvfrmStats)]
+    L.ga_wnl_wvfrm_dump.argtypes = L.ga_wnl_wvfrm.argtypes + [C.c_double, C.c_int, _dp, C.POINTER(C.c_int32)]
     L.ga_server_create.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_void_p)]
     L.ga_server_submit.argtypes = [C.c_void_p, C.POINTER(Request), C.POINTER(C.c_uint64)]
     L.ga_server_poll.argtypes = [C.c_void_p, C.c_uint64, C.c_int, C.POINTER(Result), C.POINTER(Arrival), C.c_int]
@@ -306,10 +308,14 @@ class Engine:
         self._check(rc)
         return out[: n_out.value].copy(), pts[: n_pts.value].copy(), st
 
-    def wnl_wvfrm(self, aid, params: PropParams, w: WvfrmParams, launches, t_src, p_src):
+    def wnl_wvfrm(self, aid, params: PropParams, w: WvfrmParams, launches, t_src, p_src, out_step=None,
+                  dump_cap=64):
         """-wnl_wvfrm for each (incl, az) launch with the source waveform
         (t_src, p_src); returns a dict of t_init, p_init, t_out, p_out
-        ([n, len] each), arrivals, status, steps and the WvfrmStats."""
+        ([n, len] each), arrivals, status, steps and the WvfrmStats.  With
+        out_step (wvfrm_out_step=, km) it also returns "dumps": per launch a
+        list of (label, u[len], U[len/2 + 1] complex) for the first dump_cap
+        snapshots, and "dump_count" (ga_wnl_wvfrm_dump)."""
         Lc = np.ascontiguousarray(launches, dtype=np.float64).reshape(-1, 2)
         n = Lc.shape[0]
         ln = int(w.len)
@@ -323,11 +329,21 @@ class Engine:
         steps = np.zeros(max(n, 1), dtype=np.int64)
         st = WvfrmStats()
         dp = lambda a: a.ctypes.data_as(_dp)
-        self._check(self.L.ga_wnl_wvfrm(self.h, aid, C.byref(params), C.byref(w), dp(Lc), n, dp(t), dp(pz),
-                                        dp(out["t_init"]), dp(out["p_init"]), dp(out["t_out"]), dp(out["p_out"]),
-                                        arr.ctypes.data_as(C.POINTER(Arrival)),
-                                        status.ctypes.data_as(C.POINTER(C.c_int32)),
-                                        steps.ctypes.data_as(C.POINTER(C.c_int64)), C.byref(st)))
+        args = [self.h, aid, C.byref(params), C.byref(w), dp(Lc), n, dp(t), dp(pz),
+                dp(out["t_init"]), dp(out["p_init"]), dp(out["t_out"]), dp(out["p_out"]),
+                arr.ctypes.data_as(C.POINTER(Arrival)), status.ctypes.data_as(C.POINTER(C.c_int32)),
+                steps.ctypes.data_as(C.POINTER(C.c_int64)), C.byref(st)]
+        if out_step is None:
+            self._check(self.L.ga_wnl_wvfrm(*args))
+        else:
+            width = 1 + ln + 2 * (ln // 2 + 1)
+            snaps = np.zeros((max(n, 1), int(dump_cap), width))
+            cnt = np.zeros(max(n, 1), dtype=np.int32)
+            self._check(self.L.ga_wnl_wvfrm_dump(*args, float(out_step), int(dump_cap), dp(snaps),
+                                                 cnt.ctypes.data_as(C.POINTER(C.c_int32))))
+            out["dump_count"] = cnt[:n].copy()
+            out["dumps"] = [[(S[0], S[1:1 + ln].copy(), S[1 + ln::2] + 1j * S[2 + ln::2])
+                             for S in snaps[i, :min(int(cnt[i]), int(dump_cap))]] for i in range(n)]
         out.update(arrivals=arr[:n].copy(), status=status[:n].copy(), steps=steps[:n].copy(), stats=st)
         return out
 

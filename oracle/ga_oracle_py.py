@@ -1022,13 +1022,30 @@ def wv_build(oracle, option="impulse", p0=10.0, t0=0.1, alpha=4.0, length=8192, 
     return t, p, a.value
 
 
-def wv_run(medium, params, wv, src, incl, az, bounces, t_src, p_src, freq=0.1, wvfrm_ref=1.0):
+def wv_run(medium, params, wv, src, incl, az, bounces, t_src, p_src, freq=0.1, wvfrm_ref=1.0, dump=None):
     """run_wnl_wvfrm through a stratified Medium or a range-dependent
     RdMedium / Rd3Medium (src: lat, lon, alt or x, y, z): dict with t_init, p_init,
     t_out, p_out, arrival (pos0, pos1, time, cel, turn_ht, incl_arr, back_az,
-    tl, absorption), ok (ray returned to the ground), steps."""
+    tl, absorption), ok (ray returned to the ground), steps.  dump =
+    (out_step, cap): also "dumps", the wvfrm_out_step snapshots (label,
+    u[len], U[len/2 + 1] complex), and "dump_count"."""
     L = medium.o.lib
     _wv_bind(L)
+    if dump is not None:
+        L.go_wv_dump_set.argtypes = [C.c_double, _dp, C.c_int]
+        L.go_wv_dump_count.restype = C.c_int
+        width = 1 + wv.len + 2 * (wv.len // 2 + 1)
+        dbuf = np.zeros((int(dump[1]), width))
+        L.go_wv_dump_set(float(dump[0]), _ptr(dbuf), int(dump[1]))
+        try:
+            out = wv_run(medium, params, wv, src, incl, az, bounces, t_src, p_src, freq, wvfrm_ref)
+            cnt = L.go_wv_dump_count()
+        finally:
+            L.go_wv_dump_set(1.0e10, None, 0)
+        n = wv.len
+        out["dump_count"] = cnt
+        out["dumps"] = [(S[0], S[1:1 + n].copy(), S[1 + n::2] + 1j * S[2 + n::2]) for S in dbuf[:min(cnt, len(dbuf))]]
+        return out
     n = wv.len
     ts = np.ascontiguousarray(t_src, dtype=np.float64)
     ps = np.ascontiguousarray(p_src, dtype=np.float64)

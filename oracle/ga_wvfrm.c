@@ -270,6 +270,46 @@ go_wv_med go_wv_rd_med(void *h)
  * strict sign change) instead of the 3D inclusive one. */
 typedef double (*wv_absorb_fn)(void *ctx, double s, double freq);
 
+/* wvfrm_out_step dumps (geoac.eqset.sph.strat.cpp:989-1002, the same in every
+ * equation set): at the top of a step whose previous step size carries the
+ * total ray length s + s_prev across a multiple of out_step (the first step
+ * uses ds_wvfrm), the state u, U before the step, labelled
+ * out_step * ceil((s + s_prev) / out_step).  s_prev is wvfrm_ref on the leg
+ * from the reference point and the ray length accumulated before the later
+ * legs (main.infraga.sph.cpp:1394-1399).  A snapshot is [label, u[len],
+ * (Re U, Im U)[len/2 + 1]]; count keeps counting past cap. */
+static struct {
+    double out_step, s_prev;
+    double *snap;
+    int cap, n;
+} wv_dump = {1.0e10, 0.0, NULL, 0, 0};
+
+void go_wv_dump_set(double out_step, double *snap, int cap)
+{
+    wv_dump.out_step = out_step;
+    wv_dump.snap = snap;
+    wv_dump.cap = cap;
+    wv_dump.n = 0;
+    wv_dump.s_prev = 0.0;
+}
+int go_wv_dump_count(void) { return wv_dump.n; }
+
+static void wv_dump_step(int len, double s, double ds_prev, const double *u, const double *Ur, const double *Ui)
+{
+    const double os = wv_dump.out_step;
+    if (!(os < 1.0e4)) return;
+    const double lab = os * ceil((s + wv_dump.s_prev) / os);
+    if (!(fabs(os * ceil((s + wv_dump.s_prev + ds_prev) / os) - lab) > 0.0)) return;
+    if (wv_dump.snap && wv_dump.n < wv_dump.cap) {
+        const int fft_len = len / 2 + 1;
+        double *S = wv_dump.snap + (size_t)wv_dump.n * (1 + len + 2 * fft_len);
+        S[0] = lab;
+        for (int m = 0; m < len; m++) S[1 + m] = u[m];
+        for (int b = 0; b < fft_len; b++) { S[1 + len + 2 * b] = Ur[b]; S[2 + len + 2 * b] = Ui[b]; }
+    }
+    wv_dump.n++;
+}
+
 static long long wv_evolve_core(const go_wv_params *W, lin1 *LD, lin1 *Lb, lin1 *Lbc, double s_tot, double x_last,
                                 double ds_floor, int strict, wv_absorb_fn absorb, void *actx, const double *t,
                                 double *u)
@@ -301,6 +341,7 @@ static long long wv_evolve_core(const go_wv_params *W, lin1 *LD, lin1 *Lb, lin1 
     double s = 0.0, ds = W->ds;
     while (s < s_tot) {
         double bc, bn;
+        wv_dump_step(len, s, ds, u, Ur, Ui);
         if (fabs(lin_f(LD, s)) >= 0.5) bc = lin_f(Lb, s);
         else bc = lin_f(Lbc, s);
         ds = W->ds / (WV_PI * Uf_max * bc);
@@ -504,6 +545,7 @@ long long go_wnl_run(const go_wv_med *M, const go_params *p_in, const go_wv_para
                     for (int i = 0; i < len; i++) u[i] /= p0;
                     started = 1;
                     long long ns = 0;
+                    wv_dump.s_prev = wvfrm_ref;
                     ray_length = go_wnl_evolve(M, W, rows, kr + 1, k, c_src, c0, cg0, nu0, rho0, D0, p0, t, u, &ns);
                     steps += ns;
                     break;
@@ -511,6 +553,7 @@ long long go_wnl_run(const go_wv_med *M, const go_params *p_in, const go_wv_para
             }
         } else {
             long long ns = 0;
+            wv_dump.s_prev = ray_length;
             ray_length += go_wnl_evolve(M, W, rows, 0, k, c_src, c0, cg0, nu0, rho0, D0, p0, t, u, &ns);
             steps += ns;
         }
@@ -674,6 +717,7 @@ long long go2_wnl_run(go_2d *g, const go_wv_params *W, double src_alt, double in
                     for (int i = 0; i < len; i++) u[i] /= p0;
                     started = 1;
                     long long ns = 0;
+                    wv_dump.s_prev = wvfrm_ref;
                     ray_length = go2_wnl_evolve(g, W, rows, kr + 1, k, c0, rho0, D0, p0, t, u, &ns);
                     steps += ns;
                     break;
@@ -681,6 +725,7 @@ long long go2_wnl_run(go_2d *g, const go_wv_params *W, double src_alt, double in
             }
         } else {
             long long ns = 0;
+            wv_dump.s_prev = ray_length;
             ray_length += go2_wnl_evolve(g, W, rows, 0, k, c0, rho0, D0, p0, t, u, &ns);
             steps += ns;
         }

@@ -81,6 +81,10 @@ go_wv_med go_wv_rd_med(void *rd);
  * length s_tot.  c_src: c at the ray's source (the beta0 normalisation);
  * (c0, cg0, nu0, rho0, D0, p0): the reference point's quantities;
  * *n_steps receives the number of evolution steps (may be NULL). */
+/* wvfrm_out_step dumps of the runs that follow (out_step >= 1e4: none);
+ * snapshots of 1 + len + 2 (len / 2 + 1) doubles, at most cap written. */
+void go_wv_dump_set(double out_step, double *snap, int cap);
+int go_wv_dump_count(void);
 double go_wnl_evolve(const go_wv_med *M, const go_wv_params *W, const double *rows, int k1, int k2, double c_src,
                      double c0, double cg0, double nu0, double rho0, double D0, double p0, double *t, double *u,
                      long long *n_steps);

@@ -124,7 +124,6 @@ def test_accel_launch_list_is_the_mpi_drivers(toy_met, tmp_path):
     ("-eig_search", "prefix", "lats.dat"),
     ("-eig_search", "prefix", "lats.dat", "lons.dat", "no_such_key=1"),
     ("-eig_direct", "prefix", "lats.dat"),
-    ("-wnl_wvfrm", "prefix", "lats.dat", "lons.dat", "wvfrm_out_step=5"),
     ("-wnl_wvfrm", "prefix", "lats.dat"),
 ])
 def test_rngdep_rejected_arguments(args):
@@ -133,12 +132,10 @@ def test_rngdep_rejected_arguments(args):
 
 
 @pytest.mark.parametrize("args", [
-    ("-wnl_wvfrm", "x.met", "wvfrm_out_step=10"),
     ("-wnl_wvfrm", "x.met", "no_such_key=1"),
 ])
 def test_wnl_wvfrm_rejected_arguments(args):
-    """Intermediate waveform dumps are refused before any GPU work, as are
-    unknown keys."""
+    """Unknown keys are refused before any GPU work."""
     r = run("infraga-sph", *args)
     assert r.returncode == 2, (r.stdout, r.stderr)
 

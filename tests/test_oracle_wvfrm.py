@@ -210,3 +210,38 @@ def test_package_waveforms_match_oracle(oracle, option, alpha):
     to, po, _ = wv_build(oracle, option, p0=300.0, t0=0.2, alpha=alpha, length=2048, n_rise=8, n_period=30)
     np.testing.assert_allclose(t, to, rtol=1e-15, atol=1e-15)
     np.testing.assert_allclose(p, po, rtol=1e-13, atol=1e-12)
+
+
+def test_out_step_dumps(oracle, base_units):
+    """wvfrm_out_step snapshots (the sc_spectrum / sc_waveform dumps of
+    wnl_wvfrm): labels are multiples of out_step in increasing order, one per
+    crossing, the first one out_step ceil(wvfrm_ref / out_step) (the first
+    leg starts at s_prev = wvfrm_ref); each snapshot is the state at the top
+    of a step after the first, so its pressure is the windowed inverse
+    transform of its spectrum (u = IFFT(U) w_tukey, the end of the previous
+    step); and the dumps do not change the run."""
+    from ga_oracle_py import GEOM_SPH, Medium, WvParams, wv_run
+    from infraga_amd import waveforms
+    m = Medium(oracle, GEOM_SPH, *base_units)
+    p = m.params(True)
+    length = 1024
+    t, ps = waveforms.build("impulse", 500.0, 0.1, 4.0, length, 10, 40)
+    wv = WvParams.default(len=length)
+    plain = wv_run(m, p, wv, (30.0, 0.0, 0.0), 8.0, 90.0, 1, t, ps)
+    out_step = 50.0
+    r = wv_run(m, p, wv, (30.0, 0.0, 0.0), 8.0, 90.0, 1, t, ps, dump=(out_step, 200))
+    assert np.array_equal(r["p_out"], plain["p_out"]) and r["steps"] == plain["steps"]
+    d = r["dumps"]
+    assert r["dump_count"] == len(d) > 5
+    labels = np.array([x[0] for x in d])
+    assert np.all(np.diff(labels) > 0)
+    assert np.allclose(labels / out_step, np.round(labels / out_step), rtol=0, atol=1e-9)
+    lab0, u0, U0 = d[0]
+    assert lab0 == out_step * math.ceil(1.0 / out_step)        # s = 0, s_prev = wvfrm_ref = 1 km
+    tp = 0.8
+    mm = np.arange(length)
+    tk = np.where(mm < tp * (length - 1) / 2.0, 0.5 * (1.0 + np.cos(np.pi * (2.0 * mm / (tp * (length - 1)) - 1.0))),
+                  np.where(mm > (length - 1) * (1.0 - tp / 2.0),
+                           0.5 * (1.0 + np.cos(np.pi * (2.0 * mm / (tp * (length - 1)) - 2.0 / tp + 1.0))), 1.0))
+    for lab, u, U in d:
+        assert np.allclose(u, np.fft.irfft(U, length) * tk, rtol=0, atol=1e-9 * np.abs(u).max())

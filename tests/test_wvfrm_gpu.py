@@ -200,3 +200,47 @@ def test_cli_wnl_wvfrm_nwave_from_file(oracle, base_units, tmp_path):
     peak = np.abs(ref["p_out"]).max()
     np.testing.assert_allclose(out[:, 1], ref["p_out"], rtol=1e-7, atol=1e-7 * peak)
     assert "#\twaveform source file: " + wf + "\n" in open(os.path.join(str(tmp_path), "nw.wvfrm_out.dat")).read()
+
+
+@pytest.mark.gpu
+def test_out_step_dumps_match_oracle(engine, oracle, base, base_units, tmp_path):
+    """wvfrm_out_step snapshots (ga_wnl_wvfrm_dump) against the oracle's on
+    the same waveform: the same labels in the same order, pressure and
+    spectrum within 1e-6 of their peaks; the run's outputs equal those of
+    ga_wnl_wvfrm.  Then infraga-sph -wnl_wvfrm wvfrm_out_step= writes one
+    sc_waveform / sc_spectrum file per label with those values."""
+    import os
+    import subprocess
+    from ga_oracle_py import GEOM_SPH, Medium, WvParams, wv_run
+    length, out_step = 1024, 50.0
+    t, ps = source(oracle, "impulse", length)
+    p = prop_params(GA_GEOM_SPH, bounces=1, src=(30.0, 0.0, 0.0))
+    plain = engine.wnl_wvfrm(base, p, wvfrm_params(length), [(8.0, 90.0)], t, ps)
+    g = engine.wnl_wvfrm(base, p, wvfrm_params(length), [(8.0, 90.0)], t, ps, out_step=out_step, dump_cap=200)
+    for k in ("t_out", "p_out", "steps"):
+        assert np.array_equal(g[k], plain[k]), k
+    m = Medium(oracle, GEOM_SPH, *base_units)
+    r = wv_run(m, m.params(True), WvParams.default(len=length), (30.0, 0.0, 0.0), 8.0, 90.0, 1, t, ps,
+               freq=0.1, wvfrm_ref=1.0, dump=(out_step, 200))
+    print(f"\n{r['dump_count']} dumps, labels {[d[0] for d in r['dumps']][:6]}", flush=True)
+    assert g["dump_count"][0] == r["dump_count"] > 5
+    for (lg, ug, Ug), (lo, uo, Uo) in zip(g["dumps"][0], r["dumps"]):
+        assert lg == lo
+        assert np.abs(ug - uo).max() <= 1e-6 * np.abs(uo).max()
+        assert np.abs(Ug - Uo).max() <= 1e-6 * np.abs(Uo).max()
+    met, _ = _write_met(str(tmp_path), base_units)
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "infraga_amd", "bin", "infraga-sph")
+    cli = subprocess.run([exe, "-wnl_wvfrm", met, "prof_format=zcuvd", "inclination=8.0", "azimuth=90.0", "bounces=1",
+                          "src_lat=30.0", "src_lon=0.0", f"wvfrm_len={length}", "wvfrm_p0=500.0",
+                          f"wvfrm_out_step={out_step:g}"],
+                         capture_output=True, text=True, cwd=tmp_path, timeout=300)
+    assert cli.returncode == 0, cli.stderr
+    labels = sorted({lab for lab, _, _ in g["dumps"][0]})
+    assert len(list(tmp_path.glob("sc_waveform-*.dat"))) == len(labels)
+    assert len(list(tmp_path.glob("sc_spectrum-*.dat"))) == len(labels)
+    lab, ug, Ug = g["dumps"][0][-1]
+    w = np.loadtxt(tmp_path / ("sc_waveform-%04g.dat" % lab))
+    sp = np.loadtxt(tmp_path / ("sc_spectrum-%04g.dat" % lab))
+    np.testing.assert_allclose(w[:, 0], t, rtol=1e-5, atol=1e-8)
+    np.testing.assert_allclose(w[:, 1], ug, rtol=0, atol=1e-5 * np.abs(ug).max())
+    np.testing.assert_allclose(sp[:, 1] + 1j * sp[:, 2], Ug, rtol=0, atol=1e-5 * np.abs(Ug).max())
